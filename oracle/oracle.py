@@ -1,0 +1,314 @@
+"""ctypes binding of the CPU FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (sors_amd) never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+BE, CN = 0, 1
+NEWTON, NEWTON_LS, SQP = 0, 1, 2
+LDLT, PCG = 0, 1
+TEST_POLY = -4
+ERRORS = {
+    -1: "Inverted Elements",
+    -2: "Newton's method did not converge",
+    -3: "Newton's method diverged",
+    -4: "Line search failed",
+    -5: "bad argument",
+    -6: "linear solve failed",
+}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("linear_solver", C.c_int),
+        ("project", C.c_int),
+        ("max_iter", C.c_int),
+        ("dx_tol", C.c_double),
+        ("g_tol", C.c_double),
+        ("pcg_rtol", C.c_double),
+        ("pcg_max_iter", C.c_int),
+        ("max_backtrack", C.c_int),
+        ("nthreads", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("newton_iters", C.c_int),
+        ("pcg_iters", C.c_longlong),
+        ("backtracks", C.c_int),
+        ("gnorm", C.c_double),
+        ("dxnorm", C.c_double),
+        ("t_grad", C.c_double),
+        ("t_hess", C.c_double),
+        ("t_solve", C.c_double),
+        ("t_energy", C.c_double),
+    ]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp, _ip, _dp, _dp, C.c_double,
+                                 C.c_int, C.c_int]
+        L.orc_free.argtypes = [C.c_void_p]
+        L.orc_energy.restype = C.c_double
+        L.orc_energy.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.orc_gradient.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.orc_pattern.restype = C.c_longlong
+        L.orc_pattern.argtypes = [C.c_void_p, C.c_double]
+        L.orc_get_pattern.argtypes = [C.c_void_p, _ip, _ip]
+        L.orc_get_scatter_map.argtypes = [C.c_void_p, _ip]
+        L.orc_hessian.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int, _dp]
+        L.orc_step.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int, C.POINTER(Opts), C.POINTER(Stats)]
+        L.orc_minimize.argtypes = [C.c_void_p, _dp, C.c_double, C.POINTER(Opts), C.POINTER(Stats)]
+        L.orc_update_state.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.orc_get_mass.argtypes = [C.c_void_p, _dp]
+        L.orc_set_initial_velocity.argtypes = [C.c_void_p, _dp]
+        L.orc_set_initial_deformation.argtypes = [C.c_void_p, _dp]
+        L.orc_get_state.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_external_force.argtypes = [C.c_void_p, _dp]
+        L.orc_default_opts.argtypes = [C.POINTER(Opts)]
+        L.orc_elem_eval.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp]
+        L.orc_tet_precompute.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_hex_precompute.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_nh_psi.restype = C.c_double
+        L.orc_nh_psi.argtypes = [_dp, C.c_double, C.c_double]
+        L.orc_nh_dpsi.argtypes = [_dp, C.c_double, C.c_double, _dp]
+        L.orc_nh_d2psi.argtypes = [_dp, C.c_double, C.c_double, _dp]
+        L.orc_project_psd.argtypes = [C.c_int, _dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def opts(**kw) -> Opts:
+    o = Opts()
+    lib().orc_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+# ---- element level -------------------------------------------------------
+def tet_precompute(Xe):
+    Xe = f64(Xe).reshape(12)
+    Dminv, V0, B = np.zeros(9), np.zeros(1), np.zeros(108)
+    lib().orc_tet_precompute(_d(Xe), _d(Dminv), _d(V0), _d(B))
+    return Dminv.reshape(3, 3), float(V0[0]), B.reshape(9, 12)
+
+
+def hex_precompute(Xe):
+    Xe = f64(Xe).reshape(24)
+    G, B, V0 = np.zeros(192), np.zeros(1728), np.zeros(1)
+    lib().orc_hex_precompute(_d(Xe), _d(G), _d(B), _d(V0))
+    return G.reshape(8, 8, 3), B.reshape(8, 9, 24), float(V0[0])
+
+
+def nh_psi(F, mu, lam):
+    F = f64(F).reshape(9)
+    return lib().orc_nh_psi(_d(F), mu, lam)
+
+
+def nh_dpsi(F, mu, lam):
+    F = f64(F).reshape(9)
+    P = np.zeros(9)
+    lib().orc_nh_dpsi(_d(F), mu, lam, _d(P))
+    return P.reshape(3, 3)
+
+
+def nh_d2psi(F, mu, lam):
+    F = f64(F).reshape(9)
+    K = np.zeros(81)
+    st = lib().orc_nh_d2psi(_d(F), mu, lam, _d(K))
+    if st:
+        raise RuntimeError(ERRORS.get(st, st))
+    return K.reshape(9, 9)
+
+
+def project_psd(K):
+    K = f64(K).copy()
+    n = K.shape[0]
+    lib().orc_project_psd(n, _d(K.reshape(-1)))
+    return K
+
+
+def elem_eval(Xe, xe, E, nu, project=False, ed=None):
+    Xe = f64(Xe).reshape(-1)
+    xe = f64(xe).reshape(-1)
+    if ed is None:
+        ed = xe.size // 3
+    nd = 3 * (4 if ed == TEST_POLY else ed)
+    e, g, h = np.zeros(1), np.zeros(nd), np.zeros(nd * nd)
+    st = lib().orc_elem_eval(ed, _d(Xe), _d(xe), E, nu, int(project), _d(e), _d(g), _d(h))
+    if st:
+        raise RuntimeError(ERRORS.get(st, st))
+    return float(e[0]), g, h.reshape(nd, nd)
+
+
+# ---- system level --------------------------------------------------------
+class System:
+    """Energy + TimeIntegrator state of one rollout (energy.cpp:4-204)."""
+
+    def __init__(self, X, conn, youngs, poisson, density, bcmask=None, bcvalue=None, grav=(0, 0, 0), alpha=0.0,
+                 scheme=BE, method=NEWTON, ed=None, nthreads=1):
+        self.X = f64(X).reshape(-1, 3)
+        self.conn = i32(conn)
+        nel, edc = self.conn.shape
+        self.N = self.X.shape[0]
+        self.n = 3 * self.N
+        self.ed = edc if ed is None else ed
+        self._keep = [f64(youngs), f64(poisson), f64(density),
+                      i32(bcmask if bcmask is not None else np.zeros(self.n)),
+                      f64(bcvalue if bcvalue is not None else np.zeros(self.n)), f64(grav)]
+        Y, P, D, M, V, G = self._keep
+        self.h = lib().orc_create(self.N, nel, self.ed, _d(self.X), _i(self.conn), _d(Y), _d(P), _d(D), _i(M),
+                                  _d(V), _d(G), float(alpha), scheme, method)
+        if not self.h:
+            raise ValueError("orc_create failed")
+        self.nel = nel
+        lib().orc_set_threads(self.h, int(nthreads))
+
+    @classmethod
+    def from_scene(cls, sc, scheme=BE, method=NEWTON_LS, nthreads=1):
+        s = cls(sc.X, sc.conn, sc.youngs, sc.poisson, sc.density, sc.bcmask, sc.bcvalue, sc.grav, sc.alpha,
+                scheme=scheme, method=method, nthreads=nthreads)
+        s.set_initial_velocity(sc.v0)
+        return s
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib().orc_free(h)
+            self.h = None
+
+    def set_threads(self, n):
+        lib().orc_set_threads(self.h, int(n))
+
+    def mass(self):
+        m = np.zeros(self.n)
+        lib().orc_get_mass(self.h, _d(m))
+        return m
+
+    def set_initial_velocity(self, v):
+        v = f64(v).reshape(-1)
+        lib().orc_set_initial_velocity(self.h, _d(v))
+
+    def set_initial_deformation(self, q):
+        q = f64(q).reshape(-1)
+        lib().orc_set_initial_deformation(self.h, _d(q))
+
+    def set_external_force(self, f):
+        if f is None:
+            lib().orc_set_external_force(self.h, None)
+        else:
+            f = f64(f).reshape(-1)
+            lib().orc_set_external_force(self.h, _d(f))
+
+    def state(self):
+        q, v = np.zeros(self.n), np.zeros(self.n)
+        lib().orc_get_state(self.h, _d(q), _d(v))
+        return q, v
+
+    def energy(self, q, dt):
+        q = f64(q).reshape(-1)
+        return lib().orc_energy(self.h, _d(q), float(dt))
+
+    def gradient(self, q, dt):
+        q = f64(q).reshape(-1)
+        g = np.zeros(self.n)
+        lib().orc_gradient(self.h, _d(q), float(dt), _d(g))
+        return g
+
+    def pattern(self, dt):
+        nnz = lib().orc_pattern(self.h, float(dt))
+        rp = np.zeros(self.n + 1, dtype=np.int32)
+        cols = np.zeros(max(nnz, 1), dtype=np.int32)
+        lib().orc_get_pattern(self.h, _i(rp), _i(cols))
+        return rp, cols[:nnz]
+
+    def scatter_map(self, dt):
+        lib().orc_pattern(self.h, float(dt))
+        nd = 3 * (4 if self.ed == TEST_POLY else self.ed)
+        m = np.zeros(self.nel * nd * nd, dtype=np.int32)
+        lib().orc_get_scatter_map(self.h, _i(m))
+        return m.reshape(self.nel, nd, nd)
+
+    def hessian(self, q, dt, project=False):
+        q = f64(q).reshape(-1)
+        rp, cols = self.pattern(dt)
+        vals = np.zeros(max(len(cols), 1))
+        st = lib().orc_hessian(self.h, _d(q), float(dt), int(project), _d(vals))
+        if st:
+            raise RuntimeError(ERRORS.get(st, st))
+        return rp, cols, vals[: len(cols)]
+
+    def hessian_dense(self, q, dt, project=False):
+        rp, cols, vals = self.hessian(q, dt, project)
+        H = np.zeros((self.n, self.n))
+        for r in range(self.n):
+            H[r, cols[rp[r]:rp[r + 1]]] = vals[rp[r]:rp[r + 1]]
+        return H
+
+    def step(self, q, dt, substeps=1, **kw):
+        q = f64(q).reshape(-1).copy()
+        o = opts(**kw)
+        st = Stats()
+        code = lib().orc_step(self.h, _d(q), float(dt), int(substeps), C.byref(o), C.byref(st))
+        if code:
+            raise RuntimeError(ERRORS.get(code, code))
+        return q, st
+
+    def minimize(self, q, dt, **kw):
+        q = f64(q).reshape(-1).copy()
+        o = opts(**kw)
+        st = Stats()
+        code = lib().orc_minimize(self.h, _d(q), float(dt), C.byref(o), C.byref(st))
+        if code:
+            raise RuntimeError(ERRORS.get(code, code))
+        return q, st
+
+    def update_state(self, q, dt):
+        q = f64(q).reshape(-1)
+        lib().orc_update_state(self.h, _d(q), float(dt))

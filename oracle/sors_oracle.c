@@ -1,0 +1,1128 @@
+/*
+ * sors_oracle.c -- CPU FP64 restatement of the SORS implicit Newton step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the thing measured or
+ * shipped).  See sors_oracle.h for the conventions and the list of
+ * reference files restated here.
+ *
+ * Pinning (what checks each function; tests/test_oracle_*.py):
+ *   orc_tet_precompute      cpp/src/_test.cpp:217-235 known-answer dF/dx;
+ *                           closed-form volume; Dm * Dm^-1 = I
+ *   orc_hex_precompute      axis-aligned box: analytic shape gradients,
+ *                           F(x = X) = I at all Gauss points, V0 = box volume
+ *   orc_nh_psi/dpsi/d2psi   cpp/src/_test.cpp:238-286 (zero at rest);
+ *                           torch FP64 autograd golden vectors; central FD
+ *   orc_project_psd         numpy.linalg.eigh clamp
+ *   orc_elem_eval           torch autograd of Psi(F(x)) per element
+ *   orc_gradient/energy     cpp/src/_test.cpp:136-159 (Testahedron energy =
+ *                           3, gradient ground truth, via the polynomial
+ *                           test element), FD of the energy
+ *   orc_pattern/hessian     brute-force triplet sort+sum on small meshes;
+ *                           _test.cpp:162-176 diagonal ground truth
+ *   orc_step                _test.cpp:395-443 (free fall -10 m/s^2 +-1e-3),
+ *                           _test.cpp:180-199 (||g||^2 < 1e-9); LDL^T vs PCG
+ *                           agree on converged states
+ */
+#include "sors_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_TEST_POLY (-4) /* ed value selecting the polynomial test element */
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 3x3 helpers (row-major)                                                   */
+/* ------------------------------------------------------------------------ */
+static double det3(const double* A) {
+    return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+           A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+
+/* inverse by cofactors (Eigen's fixed-size 3x3 inverse is cofactor-based) */
+static double inv3(const double* A, double* Ai) {
+    double d = det3(A);
+    double id = 1.0 / d;
+    Ai[0] = (A[4] * A[8] - A[5] * A[7]) * id;
+    Ai[1] = (A[2] * A[7] - A[1] * A[8]) * id;
+    Ai[2] = (A[1] * A[5] - A[2] * A[4]) * id;
+    Ai[3] = (A[5] * A[6] - A[3] * A[8]) * id;
+    Ai[4] = (A[0] * A[8] - A[2] * A[6]) * id;
+    Ai[5] = (A[2] * A[3] - A[0] * A[5]) * id;
+    Ai[6] = (A[3] * A[7] - A[4] * A[6]) * id;
+    Ai[7] = (A[1] * A[6] - A[0] * A[7]) * id;
+    Ai[8] = (A[0] * A[4] - A[1] * A[3]) * id;
+    return d;
+}
+
+static void transpose3(const double* A, double* At) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) At[3 * j + i] = A[3 * i + j];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Element level                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Lame constants: neoHookean.cpp:10-11 */
+void orc_lame(double E, double nu, double* mu, double* lam) {
+    *lam = (E * nu) / ((1 + nu) * (1 - 2 * nu));
+    *mu = E / (2 * (1 + nu));
+}
+
+/* volume of a tet: tetrahedron.cpp:137-150 (|(b-a).((c-a)x(d-a))|/6) */
+static double tet_volume(const double* a, const double* b, const double* c, const double* d) {
+    double ab[3], ac[3], ad[3];
+    for (int k = 0; k < 3; ++k) {
+        ab[k] = b[k] - a[k];
+        ac[k] = c[k] - a[k];
+        ad[k] = d[k] - a[k];
+    }
+    double cx = ac[1] * ad[2] - ac[2] * ad[1];
+    double cy = ac[2] * ad[0] - ac[0] * ad[2];
+    double cz = ac[0] * ad[1] - ac[1] * ad[0];
+    return fabs(ab[0] * cx + ab[1] * cy + ab[2] * cz) / 6.0;
+}
+
+/* tet 9x12 dF/dx from Dm^-1: tetrahedron.cpp:17-37.  Column i<9 is
+ * vec(E_{i%3, i/3} Dm^-1); columns 9..11 are vec(R_m Dm^-1) with R_m a row
+ * of -1 at row m.  Row-major 9x12. */
+static void tet_dfdx(const double* Dminv, double* B) {
+    memset(B, 0, sizeof(double) * 108);
+    for (int i = 0; i < 9; ++i) {
+        int r = i % 3, node = i / 3;
+        for (int c = 0; c < 3; ++c) B[(3 * r + c) * 12 + i] = Dminv[3 * node + c];
+    }
+    for (int m = 0; m < 3; ++m)
+        for (int c = 0; c < 3; ++c)
+            B[(3 * m + c) * 12 + 9 + m] = -(Dminv[c] + Dminv[3 + c] + Dminv[6 + c]);
+}
+
+/* Tetrahedron constructor: tetrahedron.cpp:5-37 + compute_volume :137-150 */
+int orc_tet_precompute(const double Xe[12], double Dminv[9], double* V0, double B[108]) {
+    double Dm[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) Dm[3 * r + c] = Xe[3 * c + r] - Xe[9 + r];
+    double d = inv3(Dm, Dminv);
+    *V0 = tet_volume(Xe, Xe + 3, Xe + 6, Xe + 9);
+    if (B) tet_dfdx(Dminv, B);
+    return (d != 0.0 && *V0 > 0.0) ? ORC_OK : ORC_ERR_ARG;
+}
+
+/* reference cube nodes: hexahedron.cpp:171-179 */
+static const double HEX_XI[8][3] = {{1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, -1},
+                                    {1, -1, 1},  {1, 1, 1},  {-1, 1, 1},  {-1, -1, 1}};
+/* 5-tet split used for the hex volume: hexahedron.cpp:106-130 */
+static const int HEX_TETS[5][4] = {{0, 1, 3, 4}, {1, 2, 3, 6}, {1, 4, 5, 6}, {3, 4, 6, 7}, {1, 3, 4, 6}};
+
+/* Hexahedron constructor: compute_shape_functions_gradients (hexahedron.cpp:170-207),
+ * compute_deformation_hessians (:145-168), compute_volume (:106-130).
+ * G is [s][a][c] (8 x 8 x 3); B is [s] 9x24 row-major. */
+int orc_hex_precompute(const double Xe[24], double G[192], double B[1728], double* V0) {
+    const double g = 1.0 / sqrt(3.0);
+    int ok = ORC_OK;
+    for (int s = 0; s < 8; ++s) {
+        double qs[3] = {g * HEX_XI[s][0], g * HEX_XI[s][1], g * HEX_XI[s][2]};
+        double hatG[8][3];
+        for (int i = 0; i < 8; ++i) {
+            const double* ni = HEX_XI[i];
+            hatG[i][0] = 0.125 * (ni[0] * (1 + ni[1] * qs[1]) * (1 + ni[2] * qs[2]));
+            hatG[i][1] = 0.125 * (ni[1] * (1 + ni[0] * qs[0]) * (1 + ni[2] * qs[2]));
+            hatG[i][2] = 0.125 * (ni[2] * (1 + ni[0] * qs[0]) * (1 + ni[1] * qs[1]));
+        }
+        /* hatF = X^T hatG (3x3) */
+        double hatF[9] = {0}, hatFi[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0;
+                for (int i = 0; i < 8; ++i) acc += Xe[3 * i + r] * hatG[i][c];
+                hatF[3 * r + c] = acc;
+            }
+        if (inv3(hatF, hatFi) == 0.0) ok = ORC_ERR_ARG;
+        /* G_s = (hatF^-T hatG^T)^T = hatG hatF^-1 */
+        for (int a = 0; a < 8; ++a)
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0;
+                for (int k = 0; k < 3; ++k) acc += hatG[a][k] * hatFi[3 * k + c];
+                G[s * 24 + 3 * a + c] = acc;
+            }
+        if (B) {
+            double* Bs = B + s * 216;
+            memset(Bs, 0, sizeof(double) * 216);
+            /* column i = 3a + j: vec(E_{j,a} G_s) (3x8 unit times 8x3) */
+            for (int a = 0; a < 8; ++a)
+                for (int j = 0; j < 3; ++j)
+                    for (int c = 0; c < 3; ++c) Bs[(3 * j + c) * 24 + 3 * a + j] = G[s * 24 + 3 * a + c];
+        }
+    }
+    double vol = 0;
+    for (int t = 0; t < 5; ++t) {
+        double v = tet_volume(Xe + 3 * HEX_TETS[t][0], Xe + 3 * HEX_TETS[t][1], Xe + 3 * HEX_TETS[t][2],
+                              Xe + 3 * HEX_TETS[t][3]);
+        if (!(v > 0)) ok = ORC_ERR_ARG;
+        vol += v;
+    }
+    *V0 = vol;
+    return ok;
+}
+
+/* NeoHookeanEnergy::compute_energy per quadrature point: neoHookean.cpp:31-37 */
+double orc_nh_psi(const double F[9], double mu, double lam) {
+    double J = det3(F);
+    double logJ = log(J);
+    double trFtF = 0;
+    for (int i = 0; i < 9; ++i) trFtF += F[i] * F[i];
+    return 0.5 * mu * (trFtF - 3) - mu * logJ + 0.5 * lam * logJ * logJ;
+}
+
+/* dE/dF (row-major vec): neoHookean.cpp:61-68 */
+void orc_nh_dpsi(const double F[9], double mu, double lam, double P[9]) {
+    double Fi[9], FiT[9];
+    double J = inv3(F, Fi);
+    transpose3(Fi, FiT);
+    double logJ = log(J);
+    for (int i = 0; i < 9; ++i) P[i] = mu * F[i] + (lam * logJ - mu) * FiT[i];
+}
+
+/* d2E/dF2 (9x9): neoHookean.cpp:93-125.
+ *   K = mu I + lam f f^T + (lam logJ - mu) FinvTGrad,
+ *   FinvTGrad(:, 3p+q) = vec(-F^-T E_pq^T F^-T)  (unitMatrices_ (p,q), :46-53). */
+int orc_nh_d2psi(const double F[9], double mu, double lam, double K[81]) {
+    double Fi[9], FiT[9];
+    double J = inv3(F, Fi);
+    transpose3(Fi, FiT);
+    if (J <= 0) return ORC_ERR_INVERTED;
+    double logJ = log(J);
+    double c = lam * logJ - mu;
+    for (int r = 0; r < 9; ++r)
+        for (int col = 0; col < 9; ++col) {
+            int a = r / 3, b = r % 3, p = col / 3, q = col % 3;
+            double gradFinvT = -FiT[3 * a + q] * FiT[3 * p + b];
+            K[9 * r + col] = (r == col ? mu : 0.0) + lam * FiT[r] * FiT[col] + c * gradFinvT;
+        }
+    return ORC_OK;
+}
+
+/* Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix, negative
+ * eigenvalues clamped to zero, matrix rebuilt in place (textbook method,
+ * Golub & Van Loan 8.5).  Used to SPD-project the 9x9 d2Psi/dF2. */
+void orc_project_psd(int n, double* K) {
+    double A[81 * 9], V[81 * 9];
+    if (n > 27) return;
+    memcpy(A, K, sizeof(double) * n * n);
+    for (int i = 0; i < n * n; ++i) V[i] = 0;
+    for (int i = 0; i < n; ++i) V[i * n + i] = 1;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                tot += A[i * n + j] * A[i * n + j];
+                if (i != j) off += A[i * n + j] * A[i * n + j];
+            }
+        if (off <= 1e-30 * tot || off == 0) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = A[p * n + q];
+                if (apq == 0) continue;
+                double app = A[p * n + p], aqq = A[q * n + q];
+                double theta = (aqq - app) / (2 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                double cs = 1 / sqrt(t * t + 1), sn = t * cs;
+                for (int k = 0; k < n; ++k) {
+                    double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = cs * akp - sn * akq;
+                    A[k * n + q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = cs * apk - sn * aqk;
+                    A[q * n + k] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = cs * vkp - sn * vkq;
+                    V[k * n + q] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n * n; ++i) K[i] = 0;
+    for (int k = 0; k < n; ++k) {
+        double lk = A[k * n + k];
+        if (lk <= 0) continue;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) K[i * n + j] += lk * V[i * n + k] * V[j * n + k];
+    }
+}
+
+/* Polynomial test element (the reference's Testahedron, cpp/src/_test.cpp:8-66):
+ * psi = sum x^2 (x^2 - 0.5) over the element's 12 coordinates. */
+static void poly_eval(const double* xe, double* energy, double* grad, double* hess) {
+    if (energy) {
+        double e = 0;
+        for (int i = 0; i < 12; ++i) e += xe[i] * xe[i] * (xe[i] * xe[i] - 0.5);
+        *energy = e;
+    }
+    if (grad)
+        for (int i = 0; i < 12; ++i) grad[i] = (2 * xe[i] - 1) * (2 * xe[i] + 1) * xe[i];
+    if (hess) {
+        memset(hess, 0, sizeof(double) * 144);
+        for (int i = 0; i < 12; ++i) hess[13 * i] = 12 * xe[i] * xe[i] - 1;
+    }
+}
+
+/* Element::compute_energy (element.h:40-58) / Tetrahedron::compute_gradient,
+ * compute_hessian (tetrahedron.cpp:99-134) / Hexahedron (hexahedron.cpp:68-103)
+ * with the NeoHookeanEnergy quadrature (weights 1/nquad, neoHookean.cpp:26-27,
+ * 56-57, 85-86).  Literal dense form: grad = V0 sum_s w dEdF_s B_s,
+ * hess = V0 sum_s w B_s^T K_s B_s. */
+static int elem_eval_pre(int ed, const double* pre, double V0, const double* xe, double mu, double lam,
+                         int project, double* energy, double* grad, double* hess) {
+    int nq = (ed == 4) ? 1 : 8;
+    int nd = 3 * ed;
+    double w = 1.0 / nq;
+    double Bbuf[216];
+    double e = 0;
+    if (grad) memset(grad, 0, sizeof(double) * nd);
+    if (hess) memset(hess, 0, sizeof(double) * nd * nd);
+    int status = ORC_OK;
+    for (int s = 0; s < nq; ++s) {
+        double F[9];
+        const double* Gs; /* per-node gradient vectors: node a -> Gs[3a..3a+2] */
+        double gtet[12];
+        if (ed == 4) {
+            /* F = Ds Dm^-1 (tetrahedron.cpp:152-169) */
+            const double* Dminv = pre;
+            double Ds[9];
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 3; ++r) Ds[3 * r + c] = xe[3 * c + r] - xe[9 + r];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c)
+                    F[3 * r + c] = Ds[3 * r] * Dminv[c] + Ds[3 * r + 1] * Dminv[3 + c] + Ds[3 * r + 2] * Dminv[6 + c];
+            tet_dfdx(Dminv, Bbuf);
+            (void)gtet;
+            Gs = NULL;
+        } else {
+            /* F_s = x^T G_s (hexahedron.cpp:133-142) */
+            Gs = pre + 24 * s;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    double acc = 0;
+                    for (int a = 0; a < 8; ++a) acc += xe[3 * a + r] * Gs[3 * a + c];
+                    F[3 * r + c] = acc;
+                }
+            memset(Bbuf, 0, sizeof(double) * 216);
+            for (int a = 0; a < 8; ++a)
+                for (int j = 0; j < 3; ++j)
+                    for (int c = 0; c < 3; ++c) Bbuf[(3 * j + c) * 24 + 3 * a + j] = Gs[3 * a + c];
+        }
+        if (energy) e += orc_nh_psi(F, mu, lam) * w;
+        if (grad) {
+            double P[9];
+            orc_nh_dpsi(F, mu, lam, P);
+            for (int i = 0; i < nd; ++i) {
+                double acc = 0;
+                for (int r = 0; r < 9; ++r) acc += P[r] * Bbuf[r * nd + i];
+                grad[i] += acc * w;
+            }
+        }
+        if (hess) {
+            double K[81], KB[9 * 24];
+            int st = orc_nh_d2psi(F, mu, lam, K);
+            if (st != ORC_OK) status = st;
+            if (project) orc_project_psd(9, K);
+            for (int r = 0; r < 9; ++r)
+                for (int i = 0; i < nd; ++i) {
+                    double acc = 0;
+                    for (int k = 0; k < 9; ++k) acc += K[9 * r + k] * Bbuf[k * nd + i];
+                    KB[r * nd + i] = acc;
+                }
+            for (int i = 0; i < nd; ++i)
+                for (int j = 0; j < nd; ++j) {
+                    double acc = 0;
+                    for (int r = 0; r < 9; ++r) acc += Bbuf[r * nd + i] * KB[r * nd + j];
+                    hess[i * nd + j] += acc * w;
+                }
+        }
+    }
+    if (energy) *energy = e * V0;
+    if (grad)
+        for (int i = 0; i < nd; ++i) grad[i] *= V0;
+    if (hess)
+        for (int i = 0; i < nd * nd; ++i) hess[i] *= V0;
+    return status;
+}
+
+int orc_elem_eval(int ed, const double* Xe, const double* xe, double E, double nu, int project,
+                  double* energy, double* grad, double* hess) {
+    double mu, lam, V0;
+    if (ed == ORC_TEST_POLY) {
+        poly_eval(xe, energy, grad, hess);
+        return ORC_OK;
+    }
+    orc_lame(E, nu, &mu, &lam);
+    if (ed == 4) {
+        double Dminv[9];
+        int st = orc_tet_precompute(Xe, Dminv, &V0, NULL);
+        if (st) return st;
+        return elem_eval_pre(4, Dminv, V0, xe, mu, lam, project, energy, grad, hess);
+    }
+    if (ed == 8) {
+        double G[192];
+        int st = orc_hex_precompute(Xe, G, NULL, &V0);
+        if (st) return st;
+        return elem_eval_pre(8, G, V0, xe, mu, lam, project, energy, grad, hess);
+    }
+    return ORC_ERR_ARG;
+}
+
+/* ------------------------------------------------------------------------ */
+/* System level                                                              */
+/* ------------------------------------------------------------------------ */
+struct orc_sys {
+    int N, nel, ed, nd, n, method, scheme, poly;
+    int* conn;
+    double* pre;   /* per element: Dm^-1 (9) or G (192) */
+    double* V0;
+    double* mu;
+    double* lam;
+    double* mass;  /* per DOF, energy.cpp:176-189 */
+    int* mask;
+    double* bcval;
+    double grav[3];
+    double alpha;
+    double *qprev, *vprev, *gprev, *gprevtmp;
+    /* pattern */
+    double* fext;  /* external vertex force (VertexForce), NULL if none */
+    int pattern_dtnz; /* -1: none built */
+    int* rowptr;
+    int* cols;
+    long long nnz;
+    int nthreads;
+};
+
+void orc_default_opts(orc_opts* o) {
+    o->linear_solver = ORC_LIN_LDLT;
+    o->project = 0;
+    o->max_iter = 100;
+    o->dx_tol = 1e-9;
+    o->g_tol = 0;
+    o->pcg_rtol = 1e-10;
+    o->pcg_max_iter = 20000;
+    o->max_backtrack = 60;
+    o->nthreads = 1;
+}
+
+/* Energy::Energy (energy.cpp:4-204): element precompute, lumped mass
+ * (:176-189), Dirichlet mask (:61-68), TimeIntegrator(qPrev = X, vPrev = 0)
+ * (integrator.h:155-161). ed == -4 selects the polynomial test element. */
+orc_sys* orc_create(int N, int nel, int ed, const double* X, const int* conn, const double* youngs,
+                    const double* poisson, const double* density, const int* bcmask, const double* bcvalue,
+                    const double grav[3], double alpha, int scheme, int method) {
+    int poly = (ed == ORC_TEST_POLY);
+    if (poly) ed = 4;
+    if (ed != 4 && ed != 8) return NULL;
+    orc_sys* s = (orc_sys*)calloc(1, sizeof(orc_sys));
+    s->N = N;
+    s->nel = nel;
+    s->ed = ed;
+    s->nd = 3 * ed;
+    s->n = 3 * N;
+    s->method = method;
+    s->scheme = scheme;
+    s->poly = poly;
+    s->nthreads = 1;
+    s->conn = (int*)malloc(sizeof(int) * (size_t)nel * ed);
+    memcpy(s->conn, conn, sizeof(int) * (size_t)nel * ed);
+    int psz = (ed == 4) ? 9 : 192;
+    s->pre = (double*)malloc(sizeof(double) * (size_t)nel * psz);
+    s->V0 = (double*)malloc(sizeof(double) * nel);
+    s->mu = (double*)malloc(sizeof(double) * nel);
+    s->lam = (double*)malloc(sizeof(double) * nel);
+    s->mass = (double*)calloc(s->n, sizeof(double));
+    s->mask = (int*)calloc(s->n, sizeof(int));
+    s->bcval = (double*)calloc(s->n, sizeof(double));
+    s->qprev = (double*)malloc(sizeof(double) * s->n);
+    s->vprev = (double*)calloc(s->n, sizeof(double));
+    s->gprev = (double*)calloc(s->n, sizeof(double));
+    s->gprevtmp = (double*)calloc(s->n, sizeof(double));
+    s->rowptr = NULL;
+    s->cols = NULL;
+    s->pattern_dtnz = -1;
+    for (int i = 0; i < 3; ++i) s->grav[i] = grav ? grav[i] : 0.0;
+    s->alpha = alpha;
+    if (bcmask)
+        for (int i = 0; i < s->n; ++i) s->mask[i] = bcmask[i] != 0;
+    if (bcvalue) memcpy(s->bcval, bcvalue, sizeof(double) * s->n);
+    memcpy(s->qprev, X, sizeof(double) * s->n);
+    for (int e = 0; e < nel; ++e) {
+        double Xe[24];
+        for (int a = 0; a < ed; ++a)
+            for (int k = 0; k < 3; ++k) Xe[3 * a + k] = X[3 * (size_t)conn[(size_t)e * ed + a] + k];
+        if (ed == 4)
+            orc_tet_precompute(Xe, s->pre + (size_t)e * 9, &s->V0[e], NULL);
+        else
+            orc_hex_precompute(Xe, s->pre + (size_t)e * 192, NULL, &s->V0[e]);
+        orc_lame(youngs ? youngs[e] : 0.0, poisson ? poisson[e] : 0.0, &s->mu[e], &s->lam[e]);
+        double me = (density ? density[e] : 0.0) * s->V0[e]; /* element.h:31 */
+        for (int a = 0; a < ed; ++a)
+            for (int k = 0; k < 3; ++k) s->mass[3 * (size_t)conn[(size_t)e * ed + a] + k] += me / ed;
+    }
+    return s;
+}
+
+void orc_free(orc_sys* s) {
+    if (!s) return;
+    free(s->conn);
+    free(s->pre);
+    free(s->V0);
+    free(s->mu);
+    free(s->lam);
+    free(s->mass);
+    free(s->mask);
+    free(s->bcval);
+    free(s->qprev);
+    free(s->vprev);
+    free(s->gprev);
+    free(s->gprevtmp);
+    free(s->rowptr);
+    free(s->cols);
+    free(s->fext);
+    free(s);
+}
+
+/* VertexForce::compute_force returns the actuation vector (vertexForce.cpp:4-8);
+ * Energy adds f.(q - qPrev) to the energy (energy.cpp:228-244) and subtracts f
+ * from the gradient (:282-293).  f == NULL removes the force. */
+void orc_set_external_force(orc_sys* s, const double* f) {
+    if (!f) {
+        free(s->fext);
+        s->fext = NULL;
+        return;
+    }
+    if (!s->fext) s->fext = (double*)malloc(sizeof(double) * s->n);
+    memcpy(s->fext, f, sizeof(double) * s->n);
+}
+
+int orc_ndof(const orc_sys* s) { return s->n; }
+void orc_get_mass(const orc_sys* s, double* m) { memcpy(m, s->mass, sizeof(double) * s->n); }
+/* integrator.h:101-102 */
+void orc_set_initial_velocity(orc_sys* s, const double* v) { memcpy(s->vprev, v, sizeof(double) * s->n); }
+void orc_set_initial_deformation(orc_sys* s, const double* q) { memcpy(s->qprev, q, sizeof(double) * s->n); }
+void orc_get_state(const orc_sys* s, double* qprev, double* vprev) {
+    if (qprev) memcpy(qprev, s->qprev, sizeof(double) * s->n);
+    if (vprev) memcpy(vprev, s->vprev, sizeof(double) * s->n);
+}
+void orc_set_threads(orc_sys* s, int nthreads) { s->nthreads = nthreads > 0 ? nthreads : 1; }
+
+static void gather(const orc_sys* s, int e, const double* q, double* xe) {
+    for (int a = 0; a < s->ed; ++a)
+        for (int k = 0; k < 3; ++k) xe[3 * a + k] = q[3 * (size_t)s->conn[(size_t)e * s->ed + a] + k];
+}
+
+static int elem(const orc_sys* s, int e, const double* xe, int project, double* energy, double* grad,
+                double* hess) {
+    if (s->poly) {
+        poly_eval(xe, energy, grad, hess);
+        return ORC_OK;
+    }
+    int psz = (s->ed == 4) ? 9 : 192;
+    return elem_eval_pre(s->ed, s->pre + (size_t)e * psz, s->V0[e], xe, s->mu[e], s->lam[e], project, energy, grad,
+                         hess);
+}
+
+/* TimeIntegrator::integrate_energy (integrator.cpp:3-43) */
+static double integrate_energy(const orc_sys* s, const double* q, double dt, double eint) {
+    double grav = 0;
+    for (int i = 0; i < s->n; ++i) grav -= s->mass[i] * q[i] * s->grav[i % 3];
+    if (dt == 0) return grav + eint;
+    double kin = 0, damp = 0;
+    for (int i = 0; i < s->n; ++i) {
+        double v, a;
+        if (s->scheme == ORC_BE) {
+            v = (q[i] - s->qprev[i]) / dt;
+        } else {
+            v = 2 * (q[i] - s->qprev[i]) / dt - s->vprev[i];
+        }
+        a = (v - s->vprev[i]) / dt;
+        kin += s->mass[i] * a * a;
+        damp += s->mass[i] * v * v;
+    }
+    if (s->scheme == ORC_BE) {
+        kin = 0.5 * kin * dt * dt;
+        damp = 0.5 * s->alpha * dt * damp;
+    } else {
+        kin = 0.5 * kin * dt * dt / 2;
+        damp = 0.25 * s->alpha * dt * damp;
+    }
+    return kin + grav + damp + eint;
+}
+
+/* Energy::compute_energy (energy.cpp:208-248), external forces omitted */
+double orc_energy(orc_sys* s, const double* q, double dt) {
+    double energy = 0;
+    int bad = 0;
+#pragma omp parallel for reduction(+ : energy) reduction(| : bad) num_threads(s->nthreads) schedule(static)
+    for (int e = 0; e < s->nel; ++e) {
+        double xe[24], ee;
+        gather(s, e, q, xe);
+        elem(s, e, xe, 0, &ee, NULL, NULL);
+        energy += ee;
+    }
+    (void)bad;
+    if (s->fext)
+        for (int i = 0; i < s->n; ++i) energy += s->fext[i] * (q[i] - s->qprev[i]);
+    return integrate_energy(s, q, dt, energy);
+}
+
+/* Energy::compute_gradient (energy.cpp:252-308) + integrate_gradient
+ * (integrator.cpp:45-84).  Masked element terms are dropped unless SQP
+ * semantics (:275); the Dirichlet override g_i = q_i - qprev_i - dt*v_i is
+ * applied for the Newton methods (:299-306). */
+int orc_gradient(orc_sys* s, const double* q, double dt, double* g) {
+    int n = s->n, nt = s->nthreads;
+    double* acc = (double*)calloc((size_t)n * nt, sizeof(double));
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double* gl = acc + (size_t)tid * n;
+#pragma omp for schedule(static)
+        for (int e = 0; e < s->nel; ++e) {
+            double xe[24], ge[24];
+            gather(s, e, q, xe);
+            elem(s, e, xe, 0, NULL, ge, NULL);
+            for (int a = 0; a < s->ed; ++a)
+                for (int k = 0; k < 3; ++k) {
+                    int R = 3 * s->conn[(size_t)e * s->ed + a] + k;
+                    if (s->method == ORC_SQP || s->mask[R] == 0) gl[R] += ge[3 * a + k];
+                }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        double v = 0;
+        for (int t = 0; t < nt; ++t) v += acc[(size_t)t * n + i];
+        g[i] = v;
+    }
+    free(acc);
+    if (s->fext)
+        for (int i = 0; i < n; ++i) g[i] -= s->fext[i];
+    /* integrate_gradient */
+    if (dt != 0) {
+        memcpy(s->gprevtmp, g, sizeof(double) * n);
+        for (int i = 0; i < n; ++i) {
+            double gg = -s->mass[i] * s->grav[i % 3];
+            double v;
+            if (s->scheme == ORC_BE) {
+                v = (q[i] - s->qprev[i]) / dt;
+                g[i] = s->mass[i] * ((v - s->vprev[i]) / dt) + gg + s->alpha * s->mass[i] * v + g[i];
+            } else {
+                v = 2 * (q[i] - s->qprev[i]) / dt - s->vprev[i];
+                g[i] = s->mass[i] * ((v - s->vprev[i]) / dt) + gg + s->alpha * s->mass[i] * v +
+                       0.5 * (g[i] + s->gprev[i]);
+            }
+        }
+    } else {
+        for (int i = 0; i < n; ++i) g[i] += -s->mass[i] * s->grav[i % 3];
+    }
+    if (s->method != ORC_SQP)
+        for (int i = 0; i < n; ++i)
+            if (s->mask[i]) g[i] = -(s->qprev[i] + dt * s->bcval[i] - q[i]);
+    return ORC_OK;
+}
+
+static int cmp_int(const void* a, const void* b) {
+    int x = *(const int*)a, y = *(const int*)b;
+    return (x > y) - (x < y);
+}
+
+/* CSR pattern of Energy::compute_hessian (energy.cpp:312-390) after
+ * setFromTriplets (:377), integrate_hessian (integrator.cpp:87-124) and the
+ * unit diagonal of masked DOFs (:381-388).  Rule:
+ *   {(R,C): R,C DOFs of a common element, both unmasked (all under SQP)}
+ *   u {(0,0) if any element pair was skipped: default Triplet() slots}
+ *   u {(i,i) for all i if dt != 0} u {masked (i,i) for the Newton methods}.
+ * Built per row from node adjacency; brute-force triplet enumeration checks
+ * it in the tests. */
+long long orc_pattern(orc_sys* s, double dt) {
+    int dtnz = (dt != 0);
+    if (s->pattern_dtnz == dtnz) return s->nnz;
+    free(s->rowptr);
+    free(s->cols);
+    int N = s->N, n = s->n, ed = s->ed;
+    /* node -> element incidence */
+    int* cnt = (int*)calloc(N + 1, sizeof(int));
+    for (size_t i = 0; i < (size_t)s->nel * ed; ++i) cnt[s->conn[i] + 1]++;
+    for (int i = 0; i < N; ++i) cnt[i + 1] += cnt[i];
+    int* inc = (int*)malloc(sizeof(int) * (size_t)s->nel * ed);
+    int* fill = (int*)calloc(N, sizeof(int));
+    for (int e = 0; e < s->nel; ++e)
+        for (int a = 0; a < ed; ++a) {
+            int v = s->conn[(size_t)e * ed + a];
+            inc[cnt[v] + fill[v]++] = e;
+        }
+    int anyskip = 0;
+    if (s->method != ORC_SQP)
+        for (size_t i = 0; i < (size_t)s->nel * ed && !anyskip; ++i)
+            for (int k = 0; k < 3; ++k)
+                if (s->mask[3 * s->conn[i] + k]) anyskip = 1;
+    /* per node sorted unique neighbours */
+    int* nb_ptr = (int*)calloc(N + 1, sizeof(int));
+    int cap = 0;
+    for (int v = 0; v < N; ++v) cap = (cnt[v + 1] - cnt[v]) * ed > cap ? (cnt[v + 1] - cnt[v]) * ed : cap;
+    int* tmp = (int*)malloc(sizeof(int) * (cap + 1));
+    int* nbc = (int*)malloc(sizeof(int) * (size_t)(cnt[N]) * ed + 1);
+    for (int v = 0; v < N; ++v) {
+        int m = 0;
+        for (int t = cnt[v]; t < cnt[v + 1]; ++t)
+            for (int a = 0; a < ed; ++a) tmp[m++] = s->conn[(size_t)inc[t] * ed + a];
+        qsort(tmp, m, sizeof(int), cmp_int);
+        int u = 0;
+        for (int i = 0; i < m; ++i)
+            if (u == 0 || tmp[i] != tmp[u - 1]) tmp[u++] = tmp[i];
+        nb_ptr[v + 1] = nb_ptr[v] + u;
+        memcpy(nbc + nb_ptr[v], tmp, sizeof(int) * u);
+    }
+    /* rows: element-pair columns, then the forced diagonal inserted in order */
+    s->rowptr = (int*)calloc(n + 1, sizeof(int));
+    int sqp = (s->method == ORC_SQP);
+    int* row = (int*)malloc(sizeof(int) * (3 * (size_t)cap + 2));
+    for (int pass = 0; pass < 2; ++pass) {
+        long long pos = 0;
+        for (int R = 0; R < n; ++R) {
+            int v = R / 3, m = 0;
+            if (sqp || !s->mask[R])
+                for (int t = nb_ptr[v]; t < nb_ptr[v + 1]; ++t)
+                    for (int l = 0; l < 3; ++l) {
+                        int C = 3 * nbc[t] + l;
+                        if (sqp || !s->mask[C]) row[m++] = C;
+                    }
+            int diag_forced = dtnz || (!sqp && s->mask[R]) || (R == 0 && anyskip);
+            if (diag_forced) {
+                int at = 0;
+                while (at < m && row[at] < R) ++at;
+                if (at == m || row[at] != R) {
+                    memmove(row + at + 1, row + at, sizeof(int) * (m - at));
+                    row[at] = R;
+                    ++m;
+                }
+            }
+            if (pass)
+                memcpy(s->cols + pos, row, sizeof(int) * m);
+            else
+                s->rowptr[R + 1] = m;
+            pos += m;
+        }
+        if (!pass) {
+            for (int R = 0; R < n; ++R) s->rowptr[R + 1] += s->rowptr[R];
+            s->nnz = s->rowptr[n];
+            s->cols = (int*)malloc(sizeof(int) * (size_t)(s->nnz > 0 ? s->nnz : 1));
+        }
+    }
+    free(row);
+    free(cnt);
+    free(inc);
+    free(fill);
+    free(nb_ptr);
+    free(tmp);
+    free(nbc);
+    s->pattern_dtnz = dtnz;
+    return s->nnz;
+}
+
+void orc_get_pattern(const orc_sys* s, int* rowptr, int* cols) {
+    memcpy(rowptr, s->rowptr, sizeof(int) * (s->n + 1));
+    memcpy(cols, s->cols, sizeof(int) * (size_t)s->nnz);
+}
+
+static int csr_find(const orc_sys* s, int R, int C) {
+    int lo = s->rowptr[R], hi = s->rowptr[R + 1] - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        if (s->cols[mid] == C) return mid;
+        if (s->cols[mid] < C)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    return -1;
+}
+
+/* scatter map: for element e, local (r, c) -> CSR value index, -1 where the
+ * reference skips the triplet (energy.cpp:337). */
+void orc_get_scatter_map(const orc_sys* s, int* map) {
+    int nd = s->nd, ed = s->ed;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+    for (int e = 0; e < s->nel; ++e)
+        for (int r = 0; r < nd; ++r)
+            for (int c = 0; c < nd; ++c) {
+                int R = 3 * s->conn[(size_t)e * ed + r / 3] + r % 3;
+                int C = 3 * s->conn[(size_t)e * ed + c / 3] + c % 3;
+                int keep = (s->method == ORC_SQP) || (!s->mask[R] && !s->mask[C]);
+                map[((size_t)e * nd + r) * nd + c] = keep ? csr_find(s, R, C) : -1;
+            }
+}
+
+/* Hessian values on the pattern (energy.cpp:312-390, integrator.cpp:87-124). */
+int orc_hessian(orc_sys* s, const double* q, double dt, int project, double* vals) {
+    orc_pattern(s, dt);
+    int nd = s->nd, ed = s->ed;
+    memset(vals, 0, sizeof(double) * (size_t)s->nnz);
+    int status = ORC_OK;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+    for (int e = 0; e < s->nel; ++e) {
+        double xe[24], he[576];
+        gather(s, e, q, xe);
+        int st = elem(s, e, xe, project, NULL, NULL, he);
+        if (st != ORC_OK) {
+#pragma omp atomic write
+            status = st;
+        }
+        for (int r = 0; r < nd; ++r) {
+            int R = 3 * s->conn[(size_t)e * ed + r / 3] + r % 3;
+            for (int c = 0; c < nd; ++c) {
+                int C = 3 * s->conn[(size_t)e * ed + c / 3] + c % 3;
+                if (s->method != ORC_SQP && (s->mask[R] || s->mask[C])) continue;
+                int idx = csr_find(s, R, C);
+                if (s->nthreads > 1) {
+#pragma omp atomic
+                    vals[idx] += he[r * nd + c];
+                } else {
+                    vals[idx] += he[r * nd + c];
+                }
+            }
+        }
+    }
+    if (dt != 0) {
+        double fac = (s->scheme == ORC_BE) ? 1.0 : 0.5;
+        if (fac != 1.0)
+            for (long long i = 0; i < s->nnz; ++i) vals[i] *= fac;
+        for (int i = 0; i < s->n; ++i) {
+            int idx = csr_find(s, i, i);
+            double m = s->mass[i];
+            vals[idx] += (s->scheme == ORC_BE) ? (m / (dt * dt) + s->alpha * m / dt)
+                                               : (2 * m / (dt * dt) + 2 * s->alpha * m / dt);
+        }
+    }
+    if (s->method != ORC_SQP)
+        for (int i = 0; i < s->n; ++i)
+            if (s->mask[i]) vals[csr_find(s, i, i)] = 1.0;
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Linear solvers                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Envelope (skyline) LDL^T without pivoting -- stands in for Eigen's
+ * SimplicialLDLT (solver.cpp:65,82-84).  Returns ORC_ERR_SOLVE on a zero
+ * pivot. */
+static int ldlt_solve(int n, const int* rowptr, const int* cols, const double* vals, const double* b, double* x) {
+    int* first = (int*)malloc(sizeof(int) * n);
+    long long* off = (long long*)malloc(sizeof(long long) * (n + 1));
+    for (int i = 0; i < n; ++i) {
+        int f = i;
+        for (int p = rowptr[i]; p < rowptr[i + 1]; ++p)
+            if (cols[p] < f) f = cols[p];
+        first[i] = f;
+    }
+    off[0] = 0;
+    for (int i = 0; i < n; ++i) off[i + 1] = off[i] + (i - first[i]);
+    double* L = (double*)calloc((size_t)off[n] + 1, sizeof(double));
+    double* D = (double*)malloc(sizeof(double) * n);
+    double* u = (double*)malloc(sizeof(double) * n);
+    int status = ORC_OK;
+    for (int i = 0; i < n && status == ORC_OK; ++i) {
+        int fi = first[i];
+        double aii = 0;
+        for (int j = fi; j < i; ++j) u[j] = 0;
+        for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+            int c = cols[p];
+            if (c < i) u[c] = vals[p];
+            if (c == i) aii = vals[p];
+        }
+        double* Li = L + off[i] - fi; /* Li[j] for j in [fi, i) */
+        for (int j = fi; j < i; ++j) {
+            int fj = first[j];
+            int k0 = fi > fj ? fi : fj;
+            const double* Lj = L + off[j] - fj;
+            double sum = u[j];
+            for (int k = k0; k < j; ++k) sum -= u[k] * Lj[k];
+            u[j] = sum;
+        }
+        double di = aii;
+        for (int j = fi; j < i; ++j) {
+            Li[j] = u[j] / D[j];
+            di -= u[j] * Li[j];
+        }
+        if (di == 0 || di != di) status = ORC_ERR_SOLVE;
+        D[i] = di;
+    }
+    if (status == ORC_OK) {
+        for (int i = 0; i < n; ++i) x[i] = b[i];
+        for (int i = 0; i < n; ++i) {
+            const double* Li = L + off[i] - first[i];
+            double sum = x[i];
+            for (int j = first[i]; j < i; ++j) sum -= Li[j] * x[j];
+            x[i] = sum;
+        }
+        for (int i = 0; i < n; ++i) x[i] /= D[i];
+        for (int i = n - 1; i >= 0; --i) {
+            const double* Li = L + off[i] - first[i];
+            double xi = x[i];
+            for (int j = first[i]; j < i; ++j) x[j] -= Li[j] * xi;
+        }
+    }
+    free(first);
+    free(off);
+    free(L);
+    free(D);
+    free(u);
+    return status;
+}
+
+/* FP64 PCG with a 3x3 block-Jacobi preconditioner (the CPU form of the
+ * device's linear solver).  Returns the iteration count or a negative code. */
+static long long pcg_solve(int n, const int* rowptr, const int* cols, const double* vals, const double* b, double* x,
+                           double rtol, int maxit, int nt) {
+    int nb = n / 3;
+    double* Minv = (double*)calloc((size_t)nb * 9, sizeof(double));
+    for (int v = 0; v < nb; ++v) {
+        double Bk[9] = {0};
+        for (int k = 0; k < 3; ++k) {
+            int R = 3 * v + k;
+            for (int p = rowptr[R]; p < rowptr[R + 1]; ++p) {
+                int C = cols[p];
+                if (C >= 3 * v && C < 3 * v + 3) Bk[3 * k + (C - 3 * v)] = vals[p];
+            }
+        }
+        double d = det3(Bk);
+        if (d != 0 && d == d) {
+            inv3(Bk, Minv + 9 * (size_t)v);
+        } else {
+            for (int k = 0; k < 3; ++k) Minv[9 * (size_t)v + 4 * k] = Bk[4 * k] != 0 ? 1.0 / Bk[4 * k] : 1.0;
+        }
+    }
+    double *r = (double*)malloc(sizeof(double) * n), *z = (double*)malloc(sizeof(double) * n),
+           *p = (double*)malloc(sizeof(double) * n), *Ap = (double*)malloc(sizeof(double) * n);
+    for (int i = 0; i < n; ++i) {
+        x[i] = 0;
+        r[i] = b[i];
+    }
+    double bb = 0;
+#pragma omp parallel for reduction(+ : bb) num_threads(nt) schedule(static)
+    for (int i = 0; i < n; ++i) bb += b[i] * b[i];
+    long long it = 0;
+    double rz = 0;
+    if (bb == 0) goto done;
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int v = 0; v < nb; ++v)
+        for (int k = 0; k < 3; ++k) {
+            const double* M = Minv + 9 * (size_t)v + 3 * k;
+            z[3 * v + k] = M[0] * r[3 * v] + M[1] * r[3 * v + 1] + M[2] * r[3 * v + 2];
+        }
+#pragma omp parallel for reduction(+ : rz) num_threads(nt) schedule(static)
+    for (int i = 0; i < n; ++i) rz += r[i] * z[i];
+    memcpy(p, z, sizeof(double) * n);
+    for (it = 0; it < maxit;) {
+        double pAp = 0;
+#pragma omp parallel for reduction(+ : pAp) num_threads(nt) schedule(static)
+        for (int i = 0; i < n; ++i) {
+            double acc = 0;
+            for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) acc += vals[q] * p[cols[q]];
+            Ap[i] = acc;
+            pAp += acc * p[i];
+        }
+        if (!(pAp > 0)) {
+            it = -1;
+            goto done;
+        }
+        double alpha = rz / pAp;
+        double rr = 0;
+#pragma omp parallel for reduction(+ : rr) num_threads(nt) schedule(static)
+        for (int i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * Ap[i];
+            rr += r[i] * r[i];
+        }
+        ++it;
+        if (rr <= rtol * rtol * bb) break;
+        double rzn = 0;
+#pragma omp parallel for reduction(+ : rzn) num_threads(nt) schedule(static)
+        for (int v = 0; v < nb; ++v)
+            for (int k = 0; k < 3; ++k) {
+                const double* M = Minv + 9 * (size_t)v + 3 * k;
+                double zz = M[0] * r[3 * v] + M[1] * r[3 * v + 1] + M[2] * r[3 * v + 2];
+                z[3 * v + k] = zz;
+                rzn += r[3 * v + k] * zz;
+            }
+        double beta = rzn / rz;
+        rz = rzn;
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+done:
+    free(Minv);
+    free(r);
+    free(z);
+    free(p);
+    free(Ap);
+    return it;
+}
+
+static double norm2(int n, const double* v) {
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += v[i] * v[i];
+    return sqrt(s);
+}
+
+/* TimeIntegrator::update_state (integrator.h:79-97) via Energy::update_state
+ * (energy.cpp:539-557) */
+void orc_update_state(orc_sys* s, const double* q, double dt) {
+    for (int i = 0; i < s->n; ++i) {
+        double v = (s->scheme == ORC_BE) ? (q[i] - s->qprev[i]) / dt : 2 * (q[i] - s->qprev[i]) / dt - s->vprev[i];
+        s->vprev[i] = v;
+        s->qprev[i] = q[i];
+        s->gprev[i] = s->gprevtmp[i];
+    }
+}
+
+/* Newton minimisation: minimize_newton (solver.cpp:56-105) and
+ * minimize_newton_linesearch (:201-264, Armijo alpha0 = 1, beta = 0.5,
+ * c = 0.1) with the minimize_newton Dirichlet handling. */
+int orc_minimize(orc_sys* s, double* x, double dt, const orc_opts* o, orc_stats* st) {
+    int n = s->n;
+    orc_pattern(s, dt);
+    double* g = (double*)malloc(sizeof(double) * n);
+    double* dx = (double*)malloc(sizeof(double) * n);
+    double* rhs = (double*)malloc(sizeof(double) * n);
+    double* xn = (double*)malloc(sizeof(double) * n);
+    double* vals = (double*)malloc(sizeof(double) * (size_t)(s->nnz > 0 ? s->nnz : 1));
+    int status = ORC_ERR_NOCONV;
+    int converged = 0;
+    for (int it = 0; it < o->max_iter; ++it) {
+        double t0 = now_s();
+        orc_gradient(s, x, dt, g);
+        double t1 = now_s();
+        st->t_grad += t1 - t0;
+        st->gnorm = norm2(n, g);
+        if (o->g_tol > 0 && st->gnorm < o->g_tol) {
+            converged = 1;
+            break;
+        }
+        int hst = orc_hessian(s, x, dt, o->project, vals);
+        double t2 = now_s();
+        st->t_hess += t2 - t1;
+        if (hst != ORC_OK) {
+            status = hst;
+            break;
+        }
+        for (int i = 0; i < n; ++i) rhs[i] = -g[i];
+        if (o->linear_solver == ORC_LIN_LDLT) {
+            if (ldlt_solve(n, s->rowptr, s->cols, vals, rhs, dx) != ORC_OK) {
+                for (int i = 0; i < n; ++i) dx[i] = NAN;
+            }
+        } else {
+            long long k = pcg_solve(n, s->rowptr, s->cols, vals, rhs, dx, o->pcg_rtol, o->pcg_max_iter, s->nthreads);
+            if (k < 0) {
+                status = ORC_ERR_SOLVE;
+                break;
+            }
+            st->pcg_iters += k;
+        }
+        st->t_solve += now_s() - t2;
+        st->newton_iters++;
+        double dxn = norm2(n, dx);
+        if (dxn != dxn) {
+            status = ORC_ERR_NAN;
+            break;
+        }
+        if (s->method != ORC_NEWTON_LS || dxn < o->dx_tol) {
+            for (int i = 0; i < n; ++i) x[i] += dx[i];
+            st->dxnorm = dxn;
+            if (dxn < o->dx_tol) {
+                converged = 1;
+                break;
+            }
+            continue;
+        }
+        double te = now_s();
+        double E0 = orc_energy(s, x, dt);
+        double gdx = 0;
+        for (int i = 0; i < n; ++i) gdx += g[i] * dx[i];
+        double alpha = 1.0;
+        int accepted = 0;
+        for (int bt = 0; bt <= o->max_backtrack; ++bt) {
+            for (int i = 0; i < n; ++i) xn[i] = x[i] + alpha * dx[i];
+            double En = orc_energy(s, xn, dt);
+            if (En <= E0 + 0.1 * alpha * gdx) {
+                accepted = 1;
+                break;
+            }
+            alpha *= 0.5;
+            st->backtracks++;
+        }
+        st->t_energy += now_s() - te;
+        if (!accepted) {
+            if (alpha * 2 * dxn < o->dx_tol) {
+                accepted = 1;
+            } else {
+                status = ORC_ERR_LS;
+                break;
+            }
+        }
+        double stepn = 0;
+        for (int i = 0; i < n; ++i) {
+            double d = xn[i] - x[i];
+            stepn += d * d;
+            x[i] = xn[i];
+        }
+        st->dxnorm = sqrt(stepn);
+        if (st->dxnorm < o->dx_tol) {
+            converged = 1;
+            break;
+        }
+    }
+    if (converged) status = ORC_OK;
+    free(g);
+    free(dx);
+    free(rhs);
+    free(xn);
+    free(vals);
+    return status;
+}
+
+/* Solver::step (solver.cpp:5-44) without CFL substepping: dt == 0 uses the
+ * stored dt given by the caller, substeps >= 1, update_state after each. */
+int orc_step(orc_sys* s, double* q, double dt, int substeps, const orc_opts* o, orc_stats* st) {
+    orc_opts od;
+    orc_stats sd;
+    if (!o) {
+        orc_default_opts(&od);
+        o = &od;
+    }
+    if (!st) st = &sd;
+    memset(st, 0, sizeof(*st));
+    if (substeps < 1) substeps = 1;
+    int saved = s->nthreads;
+    if (o->nthreads > 0) s->nthreads = o->nthreads;
+    double h = dt / substeps;
+    int status = ORC_OK;
+    for (int j = 0; j < substeps && status == ORC_OK; ++j) {
+        status = orc_minimize(s, q, h, o, st);
+        if (status == ORC_OK) orc_update_state(s, q, h);
+    }
+    s->nthreads = saved;
+    return status;
+}

@@ -1,0 +1,121 @@
+/*
+ * sors_oracle.h -- CPU FP64 restatement of the SORS implicit Newton step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path in
+ * sors_amd/: only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py may load it.  Nothing in the product links or calls it.
+ *
+ * Every function restates reference code (paths relative to the reference
+ * repository root, srl-ethz/sors):
+ *   cpp/src/geometries/tetrahedron.cpp, cpp/src/geometries/hexahedron.cpp,
+ *   cpp/headers/geometries/element.h, cpp/src/energies/neoHookean.cpp,
+ *   cpp/src/energy.cpp, cpp/src/integrator.cpp, cpp/headers/integrator.h,
+ *   cpp/src/solver.cpp, cpp/headers/common.h.
+ * Citations are given per function in sors_oracle.c.
+ *
+ * Conventions (same as the reference):
+ *   - positions q are FP64 node-major [x0,y0,z0,x1,...] (energy.cpp:216-221);
+ *   - element-local DOF = 3*localNode + comp, global DOF = 3*node + comp
+ *     (energy.cpp:270-277);
+ *   - 3x3 matrices are row-major; "vec" flattens row-major
+ *     (neoHookean.cpp:63-64 reshape of the transpose).
+ *
+ * Linear algebra that the reference takes from Eigen (3x3 inverse, sparse
+ * triplet assembly, SimplicialLDLT) is restated here from the published
+ * algorithms: cofactor inverse, sort-and-sum triplet assembly, envelope
+ * (skyline) LDL^T without pivoting.  Iterate-by-iterate rounding of Eigen's
+ * factorisation is not reproduced (only the solution to FP64 accuracy).
+ */
+#ifndef SORS_ORACLE_H
+#define SORS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_BE = 0, ORC_CN = 1 };
+/* solverMethod: minimize_newton, minimize_newton_linesearch (with the
+ * minimize_newton Dirichlet handling), and the SQP assembly semantics (no
+ * masking of element terms, no Dirichlet override). */
+enum { ORC_NEWTON = 0, ORC_NEWTON_LS = 1, ORC_SQP = 2 };
+enum { ORC_LIN_LDLT = 0, ORC_LIN_PCG = 1 };
+enum {
+    ORC_OK = 0,
+    ORC_ERR_INVERTED = -1,   /* "Inverted Elements" (neoHookean.cpp:97-101) */
+    ORC_ERR_NOCONV = -2,     /* "Newton's method did not converge" (solver.cpp:100-103) */
+    ORC_ERR_NAN = -3,        /* dx has NaN (solver.cpp:90-93) */
+    ORC_ERR_LS = -4,         /* backtracking cap reached (an addition) */
+    ORC_ERR_ARG = -5,
+    ORC_ERR_SOLVE = -6       /* zero pivot / PCG breakdown */
+};
+
+/* ---- element level ---------------------------------------------------- */
+void   orc_lame(double E, double nu, double* mu, double* lam);
+int    orc_tet_precompute(const double Xe[12], double Dminv[9], double* V0, double B[108]);
+int    orc_hex_precompute(const double Xe[24], double G[192], double B[1728], double* V0);
+double orc_nh_psi(const double F[9], double mu, double lam);
+void   orc_nh_dpsi(const double F[9], double mu, double lam, double P[9]);
+int    orc_nh_d2psi(const double F[9], double mu, double lam, double K[81]);
+void   orc_project_psd(int n, double* K);
+/* Per-element energy (x V0), gradient (3*ed) and Hessian ((3*ed)^2, row-major).
+ * grad / hess may be NULL.  project != 0 clamps each Gauss point's 9x9
+ * d2Psi/dF2 to PSD before contraction. */
+int    orc_elem_eval(int ed, const double* Xe, const double* xe, double E, double nu,
+                     int project, double* energy, double* grad, double* hess);
+
+/* ---- system level ----------------------------------------------------- */
+typedef struct orc_sys orc_sys;
+
+typedef struct {
+    int    linear_solver;   /* ORC_LIN_LDLT | ORC_LIN_PCG */
+    int    project;         /* SPD-project element blocks (needed for PCG) */
+    int    max_iter;        /* Newton iterations (MAX_ITER = 100, common.h:21) */
+    double dx_tol;          /* DX_TOL = 1e-9 (common.h:25) */
+    double g_tol;           /* optional residual criterion (<=0: off) */
+    double pcg_rtol;        /* relative residual for PCG */
+    int    pcg_max_iter;
+    int    max_backtrack;   /* cap on Armijo halvings */
+    int    nthreads;
+} orc_opts;
+
+typedef struct {
+    int    newton_iters;
+    long long pcg_iters;
+    int    backtracks;
+    double gnorm;           /* ||g|| at the last evaluated iterate */
+    double dxnorm;          /* last accepted step norm */
+    double t_grad, t_hess, t_solve, t_energy;   /* seconds */
+} orc_stats;
+
+void     orc_default_opts(orc_opts* o);
+orc_sys* orc_create(int N, int nel, int ed, const double* X, const int* conn,
+                    const double* youngs, const double* poisson, const double* density,
+                    const int* bcmask, const double* bcvalue,
+                    const double grav[3], double alpha, int scheme, int method);
+void     orc_free(orc_sys* s);
+int      orc_ndof(const orc_sys* s);
+void     orc_get_mass(const orc_sys* s, double* m);
+void     orc_set_initial_velocity(orc_sys* s, const double* v);
+void     orc_set_initial_deformation(orc_sys* s, const double* q);
+void     orc_get_state(const orc_sys* s, double* qprev, double* vprev);
+void     orc_set_threads(orc_sys* s, int nthreads);
+void     orc_set_external_force(orc_sys* s, const double* f);
+
+double   orc_energy(orc_sys* s, const double* q, double dt);
+int      orc_gradient(orc_sys* s, const double* q, double dt, double* g);
+/* CSR pattern of the assembled Hessian for this dt (dt==0 vs dt!=0 differ). */
+long long orc_pattern(orc_sys* s, double dt);
+void     orc_get_pattern(const orc_sys* s, int* rowptr, int* cols);
+/* scatter map: nel*(3ed)^2 CSR value indices, -1 for masked pairs */
+void     orc_get_scatter_map(const orc_sys* s, int* map);
+int      orc_hessian(orc_sys* s, const double* q, double dt, int project, double* vals);
+/* Solver::step for one substep-loop call (solver.cpp:5-44, no CFL). */
+int      orc_step(orc_sys* s, double* q, double dt, int substeps, const orc_opts* o, orc_stats* st);
+void     orc_update_state(orc_sys* s, const double* q, double dt);
+/* one minimisation without the state update (minimize_newton[_linesearch]) */
+int      orc_minimize(orc_sys* s, double* x, double dt, const orc_opts* o, orc_stats* st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

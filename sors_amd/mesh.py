@@ -1,0 +1,207 @@
+"""Seeded structured meshes for the beam / cube / prism workloads.
+
+The reference builds structured meshes with ``IO::make_voxel_hex_grid``
+(cpp/src/io.cpp:931-987) and splits hexes with ``IO::hex_to_5_tets``
+(io.cpp:899-928).  That 5-tet split uses one pattern in every cell, so
+neighbouring cells pick different face diagonals and the tet mesh is not
+conforming.  This module keeps the reference's local hex corner order
+(io.cpp:963-976: 0=(0,0,0) 1=(1,0,0) 2=(1,1,0) 3=(0,1,0), top face +z) but
+splits each cell into the 6 Kuhn tets that share the cell's main diagonal;
+every face is then cut along the diagonal from its lowest to its highest
+corner, which is the same for both cells sharing it.
+
+Node ids run with x slowest: ``id = k + NZ * (j + NY * i)``, which keeps the
+matrix profile narrow (the CPU skyline LDL^T depends on it).
+
+Random material fields use SplitMix64 (Steele, Lea, Flood 2014) so that the
+C++ and Python sides can reproduce them bit for bit.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+_M64 = (1 << 64) - 1
+
+# Kuhn split of the unit cell in the reference's local corner order: one tet
+# per permutation of the axes, walking 000 -> ... -> 111.  Odd permutations
+# swap their middle corners so every tet has positive orientation.
+KUHN_TETS = np.array(
+    [
+        [0, 1, 2, 6],  # x y z
+        [0, 5, 1, 6],  # x z y (odd)
+        [0, 2, 3, 6],  # y x z (odd)
+        [0, 3, 7, 6],  # y z x
+        [0, 4, 5, 6],  # z x y
+        [0, 7, 4, 6],  # z y x (odd)
+    ],
+    dtype=np.int32,
+)
+
+
+def splitmix64(seed: int, count: int) -> np.ndarray:
+    """``count`` successive SplitMix64 outputs as uint64."""
+    out = np.empty(count, dtype=np.uint64)
+    x = seed & _M64
+    for i in range(count):
+        x = (x + 0x9E3779B97F4A7C15) & _M64
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        out[i] = z ^ (z >> 31)
+    return out
+
+
+def splitmix64_np(seed: int, count: int) -> np.ndarray:
+    """Vectorised SplitMix64 (identical to :func:`splitmix64`)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed & _M64) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed: int, count: int) -> np.ndarray:
+    """Doubles in [0, 1) from the top 53 bits of SplitMix64."""
+    return (splitmix64_np(seed, count) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def log_uniform(seed: int, count: int, lo: float, hi: float) -> np.ndarray:
+    u = uniform01(seed, count)
+    return np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))
+
+
+def box_grid(nx: int, ny: int, nz: int, lx: float, ly: float, lz: float):
+    """Vertices (N x 3) and hex connectivity (E x 8) of an nx*ny*nz box grid."""
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    i, j, k = np.meshgrid(np.arange(NX), np.arange(NY), np.arange(NZ), indexing="ij")
+    X = np.stack([i * (lx / nx), j * (ly / ny), k * (lz / nz)], axis=-1).reshape(-1, 3).astype(np.float64)
+
+    def nid(a, b, c):
+        return c + NZ * (b + NY * a)
+
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    hexes = np.stack(
+        [
+            nid(ci, cj, ck),
+            nid(ci + 1, cj, ck),
+            nid(ci + 1, cj + 1, ck),
+            nid(ci, cj + 1, ck),
+            nid(ci, cj, ck + 1),
+            nid(ci + 1, cj, ck + 1),
+            nid(ci + 1, cj + 1, ck + 1),
+            nid(ci, cj + 1, ck + 1),
+        ],
+        axis=1,
+    ).astype(np.int32)
+    return X, hexes
+
+
+def kuhn_split(hexes: np.ndarray) -> np.ndarray:
+    """6 conforming tets per hex (E*6 x 4), tets of one cell are contiguous."""
+    return hexes[:, KUHN_TETS].reshape(-1, 4).astype(np.int32)
+
+
+def hex_to_5_tets(hexes: np.ndarray) -> np.ndarray:
+    """The reference's 5-tet split (io.cpp:899-928), for callers that want it."""
+    pat = np.array([[0, 1, 3, 4], [1, 2, 3, 6], [1, 4, 5, 6], [3, 4, 6, 7], [1, 3, 4, 6]], dtype=np.int32)
+    return hexes[:, pat].reshape(-1, 4).astype(np.int32)
+
+
+@dataclasses.dataclass
+class Scene:
+    """Everything one rollout needs, as plain arrays (node-major FP64)."""
+
+    name: str
+    X: np.ndarray          # N x 3 rest positions
+    conn: np.ndarray       # E x ed
+    youngs: np.ndarray     # E
+    poisson: np.ndarray    # E
+    density: np.ndarray    # E
+    bcmask: np.ndarray     # 3N int32 (1 = Dirichlet DOF)
+    bcvalue: np.ndarray    # 3N prescribed velocity
+    v0: np.ndarray         # 3N initial velocity
+    grav: np.ndarray       # 3
+    dt: float
+    steps: int
+    alpha: float = 0.0
+    tip: np.ndarray | None = None  # node ids of the tip face
+
+    @property
+    def ed(self) -> int:
+        return int(self.conn.shape[1])
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.X.shape[0])
+
+    @property
+    def n_elems(self) -> int:
+        return int(self.conn.shape[0])
+
+    @property
+    def bbox_diag(self) -> float:
+        return float(np.linalg.norm(self.X.max(0) - self.X.min(0)))
+
+
+def beam(nx=20, ny=4, nz=4, lx=0.10, ly=0.02, lz=0.02, element="tet", E=1e6, nu=0.45, rho=1100.0,
+         e_range=None, seed=0, tip_speed=0.5, dt=1e-3, steps=200, name="beam") -> Scene:
+    """Cantilever beam clamped at x = 0, gravity -z, initial velocity
+    v_z = tip_speed * x / L (zero at the clamp, tip_speed at the tip).
+
+    ``e_range=(lo, hi)`` draws Young's modulus per hex cell, log-uniform from
+    SplitMix64(seed); the 6 tets of a cell inherit the cell's value so the
+    hex and tet meshes carry the same material field.
+    """
+    X, hexes = box_grid(nx, ny, nz, lx, ly, lz)
+    ncell = hexes.shape[0]
+    if e_range is None:
+        Ecell = np.full(ncell, float(E))
+    else:
+        Ecell = log_uniform(seed, ncell, float(e_range[0]), float(e_range[1]))
+    if element == "tet":
+        conn = kuhn_split(hexes)
+        Eel = np.repeat(Ecell, 6)
+    elif element == "hex":
+        conn = hexes
+        Eel = Ecell
+    else:
+        raise ValueError(f"unknown element kind {element!r}")
+    nel = conn.shape[0]
+    N = X.shape[0]
+    clamp = X[:, 0] < X[:, 0].min() + 1e-4 * lx
+    bcmask = np.repeat(clamp.astype(np.int32), 3)
+    v0 = np.zeros((N, 3))
+    v0[:, 2] = tip_speed * X[:, 0] / lx
+    tip = np.nonzero(X[:, 0] > X[:, 0].max() - 1e-4 * lx)[0].astype(np.int32)
+    return Scene(
+        name=name,
+        X=X,
+        conn=conn.astype(np.int32),
+        youngs=Eel.astype(np.float64),
+        poisson=np.full(nel, float(nu)),
+        density=np.full(nel, float(rho)),
+        bcmask=bcmask,
+        bcvalue=np.zeros(3 * N),
+        v0=v0.reshape(-1),
+        grav=np.array([0.0, 0.0, -9.81]),
+        dt=dt,
+        steps=steps,
+        tip=tip,
+    )
+
+
+def cantilever_small(**kw) -> Scene:
+    """Plain FP64 cantilever: 20x4x4 cells, 1,920 tets, 525 nodes, E=1e6,
+    nu=0.45, rho=1100, 200 steps of dt=1e-3."""
+    return beam(**kw)
+
+
+def cantilever_large(element="tet", seed=0, steps=20, **kw) -> Scene:
+    """Scaled beam: 200x40x40 cells (320,000 hex8 or 1,920,000 tet4, 337,881
+    nodes), E log-uniform in [5e5, 5e6] per cell, 20 steps of dt=1e-3."""
+    return beam(nx=200, ny=40, nz=40, element=element, e_range=(5e5, 5e6), seed=seed, steps=steps,
+                name=f"beam200x40x40_{element}", **kw)
