@@ -1,0 +1,151 @@
+/*
+ * sors_amd.h -- C ABI of the MI355X implicit Newton step for energy-based
+ * FEM (Neo-Hookean tet4 / hex8), a drop-in for the CPU/Eigen path of SORS.
+ *
+ * Plain C types only: host pointers and sizes in, status codes out.  Every
+ * entry point returns SORS_OK (0) or a negative SORS_ERR_* code; the message
+ * is available from sors_last_error(ctx) (or sors_last_error(NULL) for
+ * sors_create).  Arrays are node-major FP64 like the reference's state
+ * vector q = [x0,y0,z0,x1,...] (cpp/src/energy.cpp:216-221); element-local
+ * DOF = 3*localNode + comp.
+ *
+ * Threading: one context per caller thread at a time; calls on one context
+ * must be serialised.  Each context owns its device buffers and one HIP
+ * stream.  Host pointers are borrowed for the duration of a call.
+ */
+#ifndef SORS_AMD_H
+#define SORS_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SORS_OK 0
+#define SORS_ERR_INVERTED (-1)   /* "Inverted Elements" (cpp/src/energies/neoHookean.cpp:97-101) */
+#define SORS_ERR_NOCONV (-2)     /* "Newton's method did not converge" (cpp/src/solver.cpp:100-103) */
+#define SORS_ERR_NAN (-3)        /* step has NaN (cpp/src/solver.cpp:90-93) */
+#define SORS_ERR_LINESEARCH (-4) /* Armijo backtracking cap reached */
+#define SORS_ERR_ARG (-5)
+#define SORS_ERR_HIP (-6)
+#define SORS_ERR_UNSUPPORTED (-7)
+
+/* SimulationSettings::timeSteppingScheme (cpp/headers/simulationSettings.h:31) */
+enum { SORS_BACKWARD_EULER = 0, SORS_CRANK_NICOLSON = 1 };
+/* SimulationSettings::solverMethod (cpp/src/solver.cpp:28-40).  The line
+ * search variant keeps minimize_newton's Dirichlet handling; SQP selects the
+ * unmasked assembly semantics (energy.cpp:275,337) for inspection calls. */
+enum { SORS_MINIMIZE_NEWTON = 0, SORS_MINIMIZE_NEWTON_LINESEARCH = 1, SORS_MINIMIZE_SQP = 2 };
+/* ElementEnergy plugins (cpp/headers/energies/elementEnergy.h): compressible
+ * Neo-Hookean, and the polynomial test energy of the reference's unit tests
+ * (cpp/src/_test.cpp:8-66). */
+enum { SORS_NEOHOOKEAN = 0, SORS_TEST_POLY = 1 };
+enum { SORS_FP64 = 0, SORS_MIXED = 1 };
+
+/* Arguments of Energy<3,E>::Energy (cpp/headers/energy.h:73-84) as arrays:
+ * undeformedVertices, eleIdx, per-element Params "density" /
+ * "youngsModulus" / "poissonsRatio", the "neumannBC" constraint's
+ * neumannBCmask / neumannBCvalue (energy.cpp:61-68), gravAcceleration,
+ * dampingAlpha and the SimulationSettings fields that Energy reads. */
+typedef struct {
+    int n_nodes;
+    int n_elems;
+    int nodes_per_elem;     /* 4 (Tetrahedron) or 8 (Hexahedron) */
+    const double* rest;     /* [n_nodes][3] */
+    const int* conn;        /* [n_elems][nodes_per_elem] */
+    const double* youngs;   /* [n_elems] */
+    const double* poisson;  /* [n_elems] */
+    const double* density;  /* [n_elems] */
+    int material;           /* SORS_NEOHOOKEAN | SORS_TEST_POLY */
+    const int* bc_mask;     /* [3 n_nodes] or NULL */
+    const double* bc_value; /* [3 n_nodes] prescribed velocity or NULL */
+    double grav[3];
+    double damping_alpha;
+    int scheme;             /* SORS_BACKWARD_EULER | SORS_CRANK_NICOLSON */
+    int method;             /* SORS_MINIMIZE_NEWTON[_LINESEARCH] | SORS_MINIMIZE_SQP */
+} sors_system_desc;
+
+/* Solver controls.  Defaults (sors_default_options): mixed precision, SPD
+ * projection on, MAX_ITER = 100 and DX_TOL = 1e-9 (cpp/headers/common.h:21-25),
+ * Armijo alpha0 = 1, beta = 0.5, c = 0.1 (cpp/src/solver.cpp:208-210). */
+typedef struct {
+    int precision;        /* SORS_FP64: all-FP64; SORS_MIXED: FP32 element tangents,
+                             CSR values, SpMV and PCG vectors; FP64 state, residual,
+                             energies and every dot product */
+    int project_spd;      /* clamp each element tangent to PSD (needed by PCG) */
+    int max_newton;
+    double dx_tol;        /* converged when ||accepted step||_2 < dx_tol */
+    double g_tol;         /* also converged when ||g||_2 < g_tol (<= 0: off) */
+    double pcg_rtol;      /* relative residual of each linear solve */
+    int pcg_max_iter;
+    int max_backtrack;
+    int pcg_check_every;  /* PCG iterations between host convergence checks */
+    int use_graph;        /* replay PCG iteration chunks from a captured hipGraph */
+} sors_options;
+
+typedef struct {
+    int newton_iters;
+    long long pcg_iters;
+    int backtracks;
+    double gnorm;         /* ||g||_2 at the last evaluated iterate (FP64) */
+    double dxnorm;        /* ||last accepted step||_2 */
+    double ms_assemble;
+    double ms_solve;
+    double ms_linesearch;
+    double ms_total;
+} sors_stats;
+
+typedef struct sors_ctx sors_ctx;
+
+/* Energy constructor + TimeIntegrator(qPrev = rest, vPrev = 0)
+ * (cpp/src/energy.cpp:4-204, cpp/headers/integrator.h:155-161). */
+int sors_create(const sors_system_desc* desc, sors_ctx** out);
+void sors_destroy(sors_ctx* ctx);
+const char* sors_last_error(const sors_ctx* ctx);
+void sors_default_options(sors_options* opt);
+int sors_set_options(sors_ctx* ctx, const sors_options* opt);
+int sors_get_options(const sors_ctx* ctx, sors_options* opt);
+
+/* Energy::set_initial_velocity / set_initial_deformation (cpp/headers/energy.h:203-204) */
+int sors_set_initial_velocity(sors_ctx* ctx, const double* v);
+int sors_set_initial_deformation(sors_ctx* ctx, const double* q);
+/* "vertexForce" external force (cpp/src/forces/vertexForce.cpp:4-8): the
+ * actuation vector [3 n_nodes]; NULL removes it. */
+int sors_set_external_force(sors_ctx* ctx, const double* f);
+int sors_get_state(sors_ctx* ctx, double* q_prev, double* v_prev);
+
+/* Solver::step (cpp/src/solver.cpp:5-44): q is the initial guess on entry
+ * and the new positions on return; dt == 0 is rejected (the caller resolves
+ * settings.dt); substeps >= 1.  stats may be NULL. */
+int sors_step(sors_ctx* ctx, double* q, double dt, int substeps, sors_stats* stats);
+/* Same, starting from the device-resident state of the previous step (no
+ * host transfer); read positions with sors_get_positions. */
+int sors_step_resident(sors_ctx* ctx, double dt, int substeps, sors_stats* stats);
+int sors_get_positions(sors_ctx* ctx, double* q);
+
+/* ---- inspection entry points used by the parity tests ---------------- */
+/* Energy::compute_energy (cpp/src/energy.cpp:208-248) */
+int sors_energy(sors_ctx* ctx, const double* q, double dt, double* energy);
+/* Energy::compute_gradient (cpp/src/energy.cpp:252-308), FP64 */
+int sors_gradient(sors_ctx* ctx, const double* q, double dt, double* g);
+/* CSR pattern of Energy::compute_hessian for this dt (pattern rule in the
+ * library's pattern.h); rowptr[3 n_nodes + 1], cols[nnz] */
+long long sors_pattern_nnz(sors_ctx* ctx, double dt);
+int sors_get_pattern(sors_ctx* ctx, double dt, int* rowptr, int* cols);
+/* scatter map [n_elems][(3 ed)^2]: CSR value index or -1 (energy.cpp:337) */
+int sors_get_scatter_map(sors_ctx* ctx, double dt, int* map);
+/* assembled Hessian values on the pattern, returned as FP64 */
+int sors_hessian(sors_ctx* ctx, const double* q, double dt, int precision, int project, double* vals);
+/* per-element energy [E], gradient [E][3ed], Hessian [E][3ed][3ed] (any may be NULL);
+ * Element::compute_energy / Tetrahedron::compute_gradient,compute_hessian
+ * (cpp/headers/geometries/element.h:40-58, cpp/src/geometries/tetrahedron.cpp:99-134,
+ * cpp/src/geometries/hexahedron.cpp:68-103) */
+int sors_element_eval(sors_ctx* ctx, const double* q, int precision, int project, double* energy, double* grad,
+                      double* hess);
+/* solve H(q) x = b with the device block-Jacobi PCG (precision per options) */
+int sors_linear_solve(sors_ctx* ctx, const double* q, double dt, const double* b, double* x, double rtol,
+                      int max_iter, int* iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

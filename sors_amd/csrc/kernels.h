@@ -1,0 +1,101 @@
+// kernels.h -- gfx950 kernels of the implicit Newton step.
+//
+// Data layout in HBM (one context = one rollout):
+//   conn   int32 [ed][E]        element-major SoA: lane e reads conn[k*E + e]
+//   rest   fp64  [3N]           node-major rest positions
+//   q, qprev, vprev, g, gint    fp64 [3N] node-major state / residual
+//   mu, lam fp64 [E]            Lame constants
+//   rowptr int32 [n+1], cols int32 [nnz]     fixed CSR pattern
+//   vals   HT [nnz]             HT = float (mixed) or double
+//   rank   uint8 [ed*ed][E]     compact scatter map (pattern.h)
+//   Minv   HT [N][9]            block-Jacobi inverse
+//   PCG vectors HT [n]: r, u, p, s, w, x
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nh_math.h"
+#include "pattern.h"
+
+namespace sors {
+
+constexpr int kBlock = 256;
+constexpr int kMaxRedBlocks = 2048;   // partial-sum slots per reduction
+constexpr int kNumXcd = 8;            // MI355X: 8 XCDs, blocks dispatched round-robin
+
+enum Scalar {
+    S_GG = 0,       // ||g||^2
+    S_DD,           // ||dx||^2
+    S_GD,           // g . dx
+    S_EELEM,        // element energy (absolute or difference)
+    S_ENODE,        // node energy terms
+    S_AUX,
+    S_COUNT
+};
+
+struct PcgState {
+    double alpha, beta, gamma, gamma_new, delta, rr, rr0, tol2;
+    int done, iter, breakdown, pad;
+};
+
+struct Reduce {
+    double* partials;    // [kMaxRedBlocks * 4]
+    unsigned* counter;   // one per reduction site
+};
+
+struct ElemArgs {
+    int E, ed, N;
+    const int* conn;
+    const double* rest;
+    const double* q;
+    const double* dq;     // trial direction or null
+    double alpha;         // positions = q + alpha * dq
+    const double* mu;
+    const double* lam;
+    const int* mask;      // 3N or null
+    int sqp;
+    int material;
+    double hscale;        // element tangent multiplier (BE 1, CN 1/2)
+    double* gint;         // fp64 atomics target or null
+    void* vals;           // HT atomics target or null
+    const int* rowptr;
+    const uint8_t* rank;
+    const int* fullmap;
+    int* flag;            // bit 0: inverted element
+    double* ex_energy;    // export (test mode) or null
+    double* ex_grad;
+    double* ex_hess;
+};
+
+struct NodeArgs {
+    int n;
+    double dt;
+    int scheme;
+    int sqp;
+    double damping;
+    double grav[3];
+    const double* mass;
+    const double* q;
+    const double* qprev;
+    const double* vprev;
+    const double* gprev;
+    const double* gint;
+    const double* fext;
+    const int* mask;
+    const double* bcval;
+    double* g;            // out
+    double* gtmp;         // out: internal gradient (for Crank-Nicolson history)
+    void* vals;           // HT or null
+    const int* diag;
+};
+
+// host launchers (sors_amd.hip)
+template <typename HT>
+void launch_assemble(const ElemArgs& a, int project, hipStream_t st);
+void launch_element_energy(const ElemArgs& a, int diff, double* out, Reduce red, hipStream_t st);
+template <typename HT>
+void launch_node_terms(const NodeArgs& a, double* gg_out, Reduce red, hipStream_t st);
+
+}  // namespace sors

@@ -1,0 +1,69 @@
+// pattern.h -- fixed CSR pattern of the assembled Hessian and the scatter
+// map from element-local entries to CSR value slots.
+//
+// The pattern reproduces what the reference's Energy::compute_hessian
+// produces after Eigen's setFromTriplets and the time-integrator diagonal
+// (cpp/src/energy.cpp:312-390, cpp/src/integrator.cpp:87-124):
+//   {(R,C): DOFs R and C belong to a common element and neither is a
+//           Dirichlet DOF (every pair under the SQP semantics)}
+//   u {(0,0) if any element pair was skipped -- the unassigned default
+//      Triplet() slots of energy.cpp:315 sum into entry (0,0)}
+//   u {(i,i) for every DOF if dt != 0}
+//   u {(i,i) for every Dirichlet DOF under the Newton methods (:381-388)}.
+// Rows are sorted by column; duplicates summed, explicit zeros kept.
+//
+// Scatter map: entry (e, r, c) of element e's (3*ed)^2 matrix lands at CSR
+// value index idx, or nowhere (-1) when the reference skips the triplet
+// (energy.cpp:337).  When every node's three DOFs share one Dirichlet flag
+// ("node-granular" masks, the case of every reference demo) the map is
+// stored compactly as one byte per node pair: the rank of node b among the
+// kept neighbours of node a; see rank_index().
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#ifndef SORS_HD
+#if defined(__HIPCC__)
+#define SORS_HD __host__ __device__ __forceinline__
+#else
+#define SORS_HD inline
+#endif
+#endif
+
+namespace sors {
+
+constexpr uint8_t kRankSkip = 255;
+
+// CSR value index of entry (3*na + c, 3*nb + d) for a kept node pair whose
+// rank is `rank`: all three rows of node na have the same columns, and the
+// three columns of node nb are contiguous.
+SORS_HD long long rank_index(const int* rowptr, int na, int c, int rank, int d) {
+    long long base = rowptr[3 * na];
+    long long len = rowptr[3 * na + 1] - base;
+    return base + c * len + 3 * rank + d;
+}
+
+struct HostPattern {
+    int n = 0;                      // DOFs
+    long long nnz = 0;
+    bool dt_nonzero = true;
+    bool node_granular = true;      // compact rank map in use
+    std::vector<int> rowptr;        // n + 1
+    std::vector<int> cols;          // nnz
+    std::vector<int> diag;          // n: index of (i,i)
+    std::vector<uint8_t> rank;      // [ed*ed][E] when node_granular
+    std::vector<int> fullmap;       // [E][(3ed)^2] otherwise
+};
+
+// mask: 3N flags (may be null).  sqp: SQP assembly semantics (no masking).
+// Returns false (with msg) on invalid connectivity.
+bool build_pattern(int n_nodes, int n_elems, int ed, const int* conn, const int* mask, bool dt_nonzero, bool sqp,
+                   HostPattern& out, std::string* msg);
+
+// full (e, r, c) -> index map from a built pattern (for export / checks)
+void scatter_map_full(const HostPattern& p, int n_elems, int ed, const int* conn, const int* mask, bool sqp,
+                      int* map);
+
+}  // namespace sors

@@ -1,0 +1,1759 @@
+// sors_amd.hip -- MI355X implicit Newton step: element kernels, fixed-CSR
+// assembly, block-Jacobi PCG, device line search, and the C ABI.
+//
+// Reference path restated here (cpp/... paths of srl-ethz/sors):
+//   Solver::step / minimize_newton / minimize_newton_linesearch
+//       (cpp/src/solver.cpp:5-44, 56-105, 201-264)
+//   Energy::compute_energy / compute_gradient / compute_hessian
+//       (cpp/src/energy.cpp:208-248, 252-308, 312-390)
+//   TimeIntegrator::integrate_* / update_state
+//       (cpp/src/integrator.cpp:3-124, cpp/headers/integrator.h:79-97)
+//   Tetrahedron / Hexahedron / NeoHookeanEnergy element math
+//       (cpp/src/geometries/tetrahedron.cpp, hexahedron.cpp, cpp/src/energies/neoHookean.cpp)
+// The direct SimplicialLDLT solve (solver.cpp:82-84) is replaced by a
+// block-Jacobi preconditioned conjugate gradient on the fixed CSR pattern,
+// with SPD-projected element tangents.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "nh_math.h"
+#include "pattern.h"
+#include "sors_amd.h"
+
+namespace sors {
+
+// ===========================================================================
+// Reductions: deterministic two-level sums (block tree, then the last block
+// to finish sums the per-block partials in a fixed order).
+// ===========================================================================
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+    __shared__ double sh[NV][kBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double x = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        v[k] = x;
+    }
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[k][w];
+            v[k] = s;
+        }
+    __syncthreads();
+}
+
+__device__ __forceinline__ double load_coherent(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns true in the last block, where `v` then holds the grid totals
+// (valid in thread 0).
+template <int NV>
+__device__ bool grid_sum(double (&v)[NV], Reduce red) {
+    block_sum<NV>(v);
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red.partials[k * kMaxRedBlocks + blockIdx.x] = v[k];
+        __threadfence();
+        unsigned t = atomicAdd(red.counter, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double s[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) s[k] += load_coherent(red.partials + k * kMaxRedBlocks + i);
+    block_sum<NV>(s);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = s[k];
+        *red.counter = 0u;
+    }
+    return true;
+}
+
+static inline int red_grid(long long work, int per_block) {
+    long long b = (work + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    return (int)std::min<long long>(b, kMaxRedBlocks);
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of logical blocks so that neighbouring rows / elements
+// share that XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = (nb + kNumXcd - 1) / kNumXcd;
+    const int xcd = b % kNumXcd, local = b / kNumXcd;
+    const int full = nb % kNumXcd;  // when nb is not a multiple of 8 the tail XCDs get one block fewer
+    int logical;
+    if (full == 0 || xcd < full)
+        logical = xcd * per + local;
+    else
+        logical = full * per + (xcd - full) * (per - 1) + local;
+    return logical;
+}
+
+// ===========================================================================
+// Fill / vector kernels
+// ===========================================================================
+template <typename T>
+__global__ void k_fill(T* p, long long n, T v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+template <typename T>
+static void fill(T* p, long long n, T v, hipStream_t st) {
+    if (n <= 0) return;
+    int grid = (int)std::min<long long>((n + kBlock - 1) / kBlock, 8192);
+    k_fill<T><<<grid, kBlock, 0, st>>>(p, n, v);
+}
+
+// ===========================================================================
+// Element kernels
+// ===========================================================================
+template <typename HT>
+__device__ __forceinline__ void atomic_add_ht(HT* p, HT v) {
+    atomicAdd(p, v);
+}
+
+// gather node positions: x = q + alpha*dq, X = rest
+__device__ __forceinline__ void gather_node(const ElemArgs& a, int node, double X[3], double x[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        X[c] = a.rest[3 * node + c];
+        double v = a.q[3 * node + c];
+        if (a.dq) v += a.alpha * a.dq[3 * node + c];
+        x[c] = v;
+    }
+}
+
+// Polynomial test element (cpp/src/_test.cpp:8-66): energy sum x^2 (x^2 - 1/2)
+// over the element coordinates, not scaled by volume.
+__device__ __forceinline__ double poly_psi(double x) { return x * x * (x * x - 0.5); }
+__device__ __forceinline__ double poly_dpsi(double x) { return (2 * x - 1) * (2 * x + 1) * x; }
+__device__ __forceinline__ double poly_d2psi(double x) { return 12 * x * x - 1; }
+
+// scatter one 3x3 node block into the CSR values
+template <typename HT>
+__device__ __forceinline__ void scatter_block(const ElemArgs& a, int e, int la, int lb, int na, const HT blk[9]) {
+    HT* vals = (HT*)a.vals;
+    if (a.rank) {
+        const int rk = a.rank[(size_t)(la * a.ed + lb) * a.E + e];
+        if (rk == kRankSkip) return;
+        const long long base = a.rowptr[3 * na];
+        const long long len = a.rowptr[3 * na + 1] - base;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) atomic_add_ht(vals + base + c * len + 3 * rk + d, blk[3 * c + d]);
+    } else {
+        const int nd = 3 * a.ed;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int idx = a.fullmap[((size_t)e * nd + 3 * la + c) * nd + 3 * lb + d];
+                if (idx >= 0) atomic_add_ht(vals + idx, blk[3 * c + d]);
+            }
+    }
+}
+
+__device__ __forceinline__ bool dof_kept(const ElemArgs& a, int dof) { return a.sqp || !a.mask || a.mask[dof] == 0; }
+
+// ---- tet4: one element per lane ------------------------------------------
+// MODE 0: assemble gradient + Hessian; MODE 1: gradient only; MODE 2: export
+// per-element energy / gradient / Hessian (FP64 gradient & energy; Hessian in HT)
+template <typename HT, int PROJ, int MODE>
+__global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    int nd[4];
+    double X[4][3], x[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        nd[k] = a.conn[(size_t)k * a.E + e];
+        gather_node(a, nd[k], X[k], x[k]);
+    }
+    if (a.material == SORS_TEST_POLY) {
+        double en = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double v = x[k][c];
+                en += poly_psi(v);
+                const double gv = poly_dpsi(v);
+                if (MODE == 2) {
+                    if (a.ex_grad) a.ex_grad[(size_t)e * 12 + 3 * k + c] = gv;
+                } else if (a.gint && dof_kept(a, 3 * nd[k] + c)) {
+                    atomicAdd(a.gint + 3 * nd[k] + c, gv);
+                }
+            }
+        if (MODE == 2 && a.ex_energy) a.ex_energy[e] = en;
+        if (MODE == 2 && a.ex_hess) {
+            for (int i = 0; i < 144; ++i) a.ex_hess[(size_t)e * 144 + i] = 0;
+            for (int i = 0; i < 12; ++i) a.ex_hess[(size_t)e * 144 + 13 * i] = poly_d2psi(x[i / 3][i % 3]);
+        }
+        if (MODE == 0 && a.vals) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                HT blk[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) blk[i] = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) blk[4 * c] = (HT)(a.hscale * poly_d2psi(x[k][c]));
+                // the diagonal block (k,k) carries everything; zero blocks elsewhere keep the pattern
+                scatter_block<HT>(a, e, k, k, nd[k], blk);
+            }
+        }
+        return;
+    }
+    // Dm = [X0-X3, X1-X3, X2-X3] (tetrahedron.cpp:11-15), Dm^-1 by cofactors
+    M3<double> Dm;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Dm(r, c) = X[c][r] - X[3][r];
+    const M3<double> Cm = cofactor3(Dm);
+    const double detm = Dm.m[0] * Cm.m[0] + Dm.m[1] * Cm.m[1] + Dm.m[2] * Cm.m[2];
+    const double V0 = fabs(detm) / 6.0;
+    // g_a = row a of Dm^-1 = column a of cof(Dm) / det; g_3 = -(g_0+g_1+g_2)
+    double g[4][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[k][c] = Cm(c, k) / detm;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[3][c] = -(g[0][c] + g[1][c] + g[2][c]);
+    // displacement gradient D = sum_a (x_a - X_a) g_a^T, F = I + D
+    M3<double> D;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc += (x[k][r] - X[k][r]) * g[k][c];
+            D(r, c) = acc;
+        }
+    const double mu = a.mu[e], lam = a.lam[e];
+    const NHPoint<double> p = nh_point(D);
+    if (!(p.J > 0)) atomicOr(a.flag, 1);
+    // gradient (FP64): grad_a = V0 P g_a
+    {
+        const M3<double> P = nh_stress(p, mu, lam);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const double gv = V0 * (P(r, 0) * g[k][0] + P(r, 1) * g[k][1] + P(r, 2) * g[k][2]);
+                if (MODE == 2) {
+                    if (a.ex_grad) a.ex_grad[(size_t)e * 12 + 3 * k + r] = gv;
+                } else if (a.gint && dof_kept(a, 3 * nd[k] + r)) {
+                    atomicAdd(a.gint + 3 * nd[k] + r, gv);
+                }
+            }
+        if (MODE == 2 && a.ex_energy) {
+            bool inv = false;
+            a.ex_energy[e] = V0 * nh_psi_disp(D, mu, lam, &inv);
+        }
+    }
+    if (MODE == 1) return;
+    // Hessian blocks in HT
+    const HT w = (HT)(V0 * a.hscale);
+    HT gh[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gh[k][c] = (HT)g[k][c];
+    const HT muh = (HT)mu, lamh = (HT)lam;
+    if (PROJ) {
+        M3<HT> Dh;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Dh.m[i] = (HT)D.m[i];
+        const NHPoint<HT> ph = nh_point(Dh);
+        Sym9<HT> S;
+        nh_projected_tangent(Dh, ph, muh, lamh, S);
+#pragma unroll
+        for (int la = 0; la < 4; ++la)
+#pragma unroll
+            for (int lb = 0; lb < 4; ++lb) {
+                HT blk[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) blk[i] = 0;
+                block_from_tangent(S, gh[la], gh[lb], w, blk);
+                if (MODE == 2) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d)
+                            a.ex_hess[(size_t)e * 144 + (3 * la + c) * 12 + 3 * lb + d] = (double)blk[3 * c + d];
+                } else {
+                    scatter_block<HT>(a, e, la, lb, nd[la], blk);
+                }
+            }
+    } else {
+        // raw tangent: H_ab = V0 [mu (g_a.g_b) I + lam h_a h_b^T - c h_b h_a^T]
+        const HT c = (HT)(lam * p.lnJ - mu);
+        HT h[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                h[k][r] = (HT)(p.FiT(r, 0) * g[k][0] + p.FiT(r, 1) * g[k][1] + p.FiT(r, 2) * g[k][2]);
+#pragma unroll
+        for (int la = 0; la < 4; ++la)
+#pragma unroll
+            for (int lb = 0; lb < 4; ++lb) {
+                HT blk[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) blk[i] = 0;
+                block_raw(muh, lamh, c, gh[la], gh[lb], h[la], h[lb], w, blk);
+                if (MODE == 2) {
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d)
+                            a.ex_hess[(size_t)e * 144 + (3 * la + cc) * 12 + 3 * lb + d] = (double)blk[3 * cc + d];
+                } else {
+                    scatter_block<HT>(a, e, la, lb, nd[la], blk);
+                }
+            }
+    }
+}
+
+// ---- hex8: one element per lane --------------------------------------------
+// Reference cube corners (hexahedron.cpp:171-179) and 5-tet volume split (:106-130)
+__constant__ signed char kHexXi[8][3] = {{1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, -1},
+                                         {1, -1, 1},  {1, 1, 1},  {-1, 1, 1},  {-1, -1, 1}};
+__constant__ unsigned char kHexTets[5][4] = {{0, 1, 3, 4}, {1, 2, 3, 6}, {1, 4, 5, 6}, {3, 4, 6, 7}, {1, 3, 4, 6}};
+
+// G_s = hatG_s hatF_s^-1 with hatF_s = X^T hatG_s (hexahedron.cpp:190-205)
+template <typename T>
+__device__ __forceinline__ void hex_shape_grads(const double X[8][3], int s, T G[8][3]) {
+    const double gq = 0.57735026918962576451;  // 1/sqrt(3)
+    const double qs0 = gq * kHexXi[s][0], qs1 = gq * kHexXi[s][1], qs2 = gq * kHexXi[s][2];
+    double hg[8][3];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const double n0 = kHexXi[i][0], n1 = kHexXi[i][1], n2 = kHexXi[i][2];
+        hg[i][0] = 0.125 * (n0 * (1 + n1 * qs1) * (1 + n2 * qs2));
+        hg[i][1] = 0.125 * (n1 * (1 + n0 * qs0) * (1 + n2 * qs2));
+        hg[i][2] = 0.125 * (n2 * (1 + n0 * qs0) * (1 + n1 * qs1));
+    }
+    M3<double> hF;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) acc += X[i][r] * hg[i][c];
+            hF(r, c) = acc;
+        }
+    const M3<double> C = cofactor3(hF);
+    const double det = hF.m[0] * C.m[0] + hF.m[1] * C.m[1] + hF.m[2] * C.m[2];
+    // hF^-1 (k, c) = C(c, k) / det
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) G[i][c] = (T)((hg[i][0] * C(c, 0) + hg[i][1] * C(c, 1) + hg[i][2] * C(c, 2)) / det);
+}
+
+__device__ __forceinline__ double tet_vol(const double* a, const double* b, const double* c, const double* d) {
+    const double ab0 = b[0] - a[0], ab1 = b[1] - a[1], ab2 = b[2] - a[2];
+    const double ac0 = c[0] - a[0], ac1 = c[1] - a[1], ac2 = c[2] - a[2];
+    const double ad0 = d[0] - a[0], ad1 = d[1] - a[1], ad2 = d[2] - a[2];
+    return fabs(ab0 * (ac1 * ad2 - ac2 * ad1) + ab1 * (ac2 * ad0 - ac0 * ad2) + ab2 * (ac0 * ad1 - ac1 * ad0)) / 6.0;
+}
+
+template <typename HT, int PROJ, int MODE>
+__global__ __launch_bounds__(128) void k_hex(ElemArgs a) {
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    int nd[8];
+    double X[8][3], u[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        nd[k] = a.conn[(size_t)k * a.E + e];
+        double xk[3];
+        gather_node(a, nd[k], X[k], xk);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[k][c] = xk[c] - X[k][c];
+    }
+    double V0 = 0;
+#pragma unroll
+    for (int t = 0; t < 5; ++t)
+        V0 += tet_vol(X[kHexTets[t][0]], X[kHexTets[t][1]], X[kHexTets[t][2]], X[kHexTets[t][3]]);
+    const double mu = a.mu[e], lam = a.lam[e];
+    const double wq = V0 * 0.125;  // quadrature weight 1/8 times V0 (neoHookean.cpp:26-27)
+    // gradient and energy, FP64
+    {
+        double gr[8][3];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) gr[k][0] = gr[k][1] = gr[k][2] = 0;
+        double en = 0;
+        for (int s = 0; s < 8; ++s) {
+            double G[8][3];
+            hex_shape_grads<double>(X, s, G);
+            M3<double> D;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double acc = 0;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) acc += u[k][r] * G[k][c];
+                    D(r, c) = acc;
+                }
+            const NHPoint<double> p = nh_point(D);
+            if (!(p.J > 0)) atomicOr(a.flag, 1);
+            const M3<double> P = nh_stress(p, mu, lam);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) gr[k][r] += wq * (P(r, 0) * G[k][0] + P(r, 1) * G[k][1] + P(r, 2) * G[k][2]);
+            if (MODE == 2) {
+                bool inv = false;
+                en += wq * nh_psi_disp(D, mu, lam, &inv);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                if (MODE == 2) {
+                    if (a.ex_grad) a.ex_grad[(size_t)e * 24 + 3 * k + r] = gr[k][r];
+                } else if (a.gint && dof_kept(a, 3 * nd[k] + r)) {
+                    atomicAdd(a.gint + 3 * nd[k] + r, gr[k][r]);
+                }
+            }
+        if (MODE == 2 && a.ex_energy) a.ex_energy[e] = en;
+    }
+    if (MODE == 1) return;
+    // Hessian: row block la at a time, Gauss points recomputed per row block
+    const HT muh = (HT)mu, lamh = (HT)lam;
+    const HT w = (HT)(wq * a.hscale);
+    for (int la = 0; la < 8; ++la) {
+        HT acc[8][9];
+#pragma unroll
+        for (int lb = 0; lb < 8; ++lb)
+#pragma unroll
+            for (int i = 0; i < 9; ++i) acc[lb][i] = 0;
+        for (int s = 0; s < 8; ++s) {
+            HT G[8][3];
+            hex_shape_grads<HT>(X, s, G);
+            M3<HT> D;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    HT t = 0;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) t += (HT)u[k][r] * G[k][c];
+                    D(r, c) = t;
+                }
+            const NHPoint<HT> p = nh_point(D);
+            if (PROJ) {
+                Sym9<HT> S;
+                nh_projected_tangent(D, p, muh, lamh, S);
+#pragma unroll
+                for (int lb = 0; lb < 8; ++lb) block_from_tangent(S, G[la], G[lb], w, acc[lb]);
+            } else {
+                const HT c = lamh * p.lnJ - muh;
+                HT ha[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) ha[r] = p.FiT(r, 0) * G[la][0] + p.FiT(r, 1) * G[la][1] + p.FiT(r, 2) * G[la][2];
+#pragma unroll
+                for (int lb = 0; lb < 8; ++lb) {
+                    HT hb[3];
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+                        hb[r] = p.FiT(r, 0) * G[lb][0] + p.FiT(r, 1) * G[lb][1] + p.FiT(r, 2) * G[lb][2];
+                    block_raw(muh, lamh, c, G[la], G[lb], ha, hb, w, acc[lb]);
+                }
+            }
+        }
+#pragma unroll
+        for (int lb = 0; lb < 8; ++lb) {
+            if (MODE == 2) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        a.ex_hess[(size_t)e * 576 + (3 * la + c) * 24 + 3 * lb + d] = (double)acc[lb][3 * c + d];
+            } else {
+                scatter_block<HT>(a, e, la, lb, nd[la], acc[lb]);
+            }
+        }
+    }
+}
+
+// ---- element energy: absolute (diff = 0) or E(q + alpha dq) - E(q) -------
+__device__ double tet_energy_at(const ElemArgs& a, int e, const int nd[4], double alpha, bool* inv) {
+    double X[4][3], x[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            X[k][c] = a.rest[3 * nd[k] + c];
+            double v = a.q[3 * nd[k] + c];
+            if (alpha != 0.0) v += alpha * a.dq[3 * nd[k] + c];
+            x[k][c] = v;
+        }
+    if (a.material == SORS_TEST_POLY) {
+        double en = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) en += poly_psi(x[k][c]);
+        return en;
+    }
+    M3<double> Dm;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Dm(r, c) = X[c][r] - X[3][r];
+    const M3<double> Cm = cofactor3(Dm);
+    const double detm = Dm.m[0] * Cm.m[0] + Dm.m[1] * Cm.m[1] + Dm.m[2] * Cm.m[2];
+    double g[4][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[k][c] = Cm(c, k) / detm;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[3][c] = -(g[0][c] + g[1][c] + g[2][c]);
+    M3<double> D;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc += (x[k][r] - X[k][r]) * g[k][c];
+            D(r, c) = acc;
+        }
+    return fabs(detm) / 6.0 * nh_psi_disp(D, a.mu[e], a.lam[e], inv);
+}
+
+__device__ double hex_energy_at(const ElemArgs& a, int e, const int nd[8], double alpha, bool* inv) {
+    double X[8][3], u[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            X[k][c] = a.rest[3 * nd[k] + c];
+            double v = a.q[3 * nd[k] + c];
+            if (alpha != 0.0) v += alpha * a.dq[3 * nd[k] + c];
+            u[k][c] = v - X[k][c];
+        }
+    double V0 = 0;
+#pragma unroll
+    for (int t = 0; t < 5; ++t)
+        V0 += tet_vol(X[kHexTets[t][0]], X[kHexTets[t][1]], X[kHexTets[t][2]], X[kHexTets[t][3]]);
+    double en = 0;
+    for (int s = 0; s < 8; ++s) {
+        double G[8][3];
+        hex_shape_grads<double>(X, s, G);
+        M3<double> D;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) acc += u[k][r] * G[k][c];
+                D(r, c) = acc;
+            }
+        en += nh_psi_disp(D, a.mu[e], a.lam[e], inv);
+    }
+    return V0 * 0.125 * en;
+}
+
+template <int ED>
+__global__ __launch_bounds__(kBlock) void k_elem_energy(ElemArgs a, int diff, double* out, Reduce red) {
+    double v[1] = {0.0};
+    bool inv = false;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < a.E; e += gridDim.x * blockDim.x) {
+        int nd[ED];
+#pragma unroll
+        for (int k = 0; k < ED; ++k) nd[k] = a.conn[(size_t)k * a.E + e];
+        double en;
+        if (ED == 4) {
+            en = tet_energy_at(a, e, nd, a.alpha, &inv);
+            if (diff) en -= tet_energy_at(a, e, nd, 0.0, &inv);
+        } else {
+            en = hex_energy_at(a, e, nd, a.alpha, &inv);
+            if (diff) en -= hex_energy_at(a, e, nd, 0.0, &inv);
+        }
+        v[0] += en;
+    }
+    if (inv) atomicOr(a.flag, 2);
+    if (grid_sum<1>(v, red) && threadIdx.x == 0) *out = v[0];
+}
+
+// ===========================================================================
+// Node kernels: inertia, gravity, damping, external force, Dirichlet rows
+// (integrator.cpp:45-124, energy.cpp:282-306, 381-388)
+// ===========================================================================
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_node_terms(NodeArgs a, double* gg_out, Reduce red) {
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        const double m = a.mass[i], qi = a.q[i];
+        double gi = a.gint ? a.gint[i] : 0.0;
+        if (a.fext) gi -= a.fext[i];
+        const double gc = -m * a.grav[i % 3];
+        double g, hd = 0.0;
+        if (a.dt != 0.0) {
+            if (a.scheme == SORS_BACKWARD_EULER) {
+                const double vel = (qi - a.qprev[i]) / a.dt;
+                g = m * ((vel - a.vprev[i]) / a.dt) + gc + a.damping * m * vel + gi;
+                hd = m / (a.dt * a.dt) + a.damping * m / a.dt;
+            } else {
+                const double vel = 2 * (qi - a.qprev[i]) / a.dt - a.vprev[i];
+                g = m * ((vel - a.vprev[i]) / a.dt) + gc + a.damping * m * vel + 0.5 * (gi + a.gprev[i]);
+                hd = 2 * m / (a.dt * a.dt) + 2 * a.damping * m / a.dt;
+            }
+            if (a.gtmp) a.gtmp[i] = gi;
+        } else {
+            g = gc + gi;
+        }
+        const bool dir = !a.sqp && a.mask && a.mask[i];
+        if (dir) g = -(a.qprev[i] + a.dt * a.bcval[i] - qi);
+        a.g[i] = g;
+        v[0] += g * g;
+        if (a.vals) {
+            HT* vals = (HT*)a.vals;
+            const int di = a.diag[i];
+            if (di < 0) {
+                // row without a diagonal slot: only empty rows of isolated nodes at dt == 0
+            } else if (dir)
+                vals[di] = (HT)1;
+            else if (hd != 0.0)
+                vals[di] += (HT)hd;
+        }
+    }
+    if (grid_sum<1>(v, red) && threadIdx.x == 0) *gg_out = v[0];
+}
+
+// energy terms of the nodes (integrate_energy, integrator.cpp:3-43, and the
+// external-force work, energy.cpp:228-244).  diff != 0: E(q + alpha dq) - E(q)
+// in difference form (no cancellation of the absolute terms).
+__global__ __launch_bounds__(kBlock) void k_node_energy(NodeArgs a, const double* dq, double alpha, int diff,
+                                                         double* out, Reduce red) {
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        const double m = a.mass[i], gc = a.grav[i % 3];
+        const double qi = a.q[i];
+        double en;
+        if (diff) {
+            const double d = alpha * dq[i];
+            en = -m * d * gc;
+            if (a.fext) en += a.fext[i] * d;
+            if (a.dt != 0.0) {
+                if (a.scheme == SORS_BACKWARD_EULER) {
+                    const double vel = (qi - a.qprev[i]) / a.dt, acc = (vel - a.vprev[i]) / a.dt;
+                    const double dv = d / a.dt, da = dv / a.dt;
+                    en += 0.5 * m * (2 * acc * da + da * da) * a.dt * a.dt;
+                    en += 0.5 * a.damping * a.dt * m * (2 * vel * dv + dv * dv);
+                } else {
+                    const double vel = 2 * (qi - a.qprev[i]) / a.dt - a.vprev[i], acc = (vel - a.vprev[i]) / a.dt;
+                    const double dv = 2 * d / a.dt, da = dv / a.dt;
+                    en += 0.25 * m * (2 * acc * da + da * da) * a.dt * a.dt;
+                    en += 0.25 * a.damping * a.dt * m * (2 * vel * dv + dv * dv);
+                }
+            }
+        } else {
+            en = -m * qi * gc;
+            if (a.fext) en += a.fext[i] * (qi - a.qprev[i]);
+            if (a.dt != 0.0) {
+                if (a.scheme == SORS_BACKWARD_EULER) {
+                    const double vel = (qi - a.qprev[i]) / a.dt, acc = (vel - a.vprev[i]) / a.dt;
+                    en += 0.5 * m * acc * acc * a.dt * a.dt + 0.5 * a.damping * a.dt * m * vel * vel;
+                } else {
+                    const double vel = 2 * (qi - a.qprev[i]) / a.dt - a.vprev[i], acc = (vel - a.vprev[i]) / a.dt;
+                    en += 0.5 * m * acc * acc * a.dt * a.dt / 2 + 0.25 * a.damping * a.dt * m * vel * vel;
+                }
+            }
+        }
+        v[0] += en;
+    }
+    if (grid_sum<1>(v, red) && threadIdx.x == 0) *out = v[0];
+}
+
+// ===========================================================================
+// Block-Jacobi preconditioner: 3x3 diagonal node blocks inverted in FP64
+// ===========================================================================
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_block_jacobi(int N, const int* rowptr, const int* cols, const int* diag,
+                                                         const HT* vals, HT* Minv) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    double B[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int R = 3 * v + c;
+        const int di = diag[R];
+        const int lo = rowptr[R], hi = rowptr[R + 1];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int k = di + (d - c);
+            B[3 * c + d] = (k >= lo && k < hi && cols[k] == 3 * v + d) ? (double)vals[k] : 0.0;
+        }
+    }
+    M3<double> A;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) A.m[i] = B[i];
+    const M3<double> C = cofactor3(A);
+    const double det = A.m[0] * C.m[0] + A.m[1] * C.m[1] + A.m[2] * C.m[2];
+    double out[9];
+    if (det != 0.0 && isfinite(det)) {
+        // A^-1 (i, j) = C(j, i) / det
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) out[3 * i + j] = C(j, i) / det;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out[4 * i] = (B[4 * i] != 0.0) ? 1.0 / B[4 * i] : 1.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Minv[(size_t)9 * v + i] = (HT)out[i];
+}
+
+// ===========================================================================
+// PCG (Chronopoulos-Gear form: one SpMV and one fused vector pass per
+// iteration, FP64 dot products, device-side scalars and convergence flag)
+// ===========================================================================
+// init: x = 0, r = sign*b, p = s = 0, u = M r, gamma = r.u, rr0 = r.r
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(int N, const double* b, double sign, const HT* Minv, HT* x, HT* r,
+                                                     HT* u, HT* p, HT* s, PcgState* st, double rtol, Reduce red) {
+    double v[2] = {0.0, 0.0};
+    for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < N; nd += gridDim.x * blockDim.x) {
+        HT rr[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            rr[c] = (HT)(sign * b[3 * nd + c]);
+            r[3 * nd + c] = rr[c];
+            x[3 * nd + c] = 0;
+            p[3 * nd + c] = 0;
+            s[3 * nd + c] = 0;
+        }
+        const HT* M = Minv + (size_t)9 * nd;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const HT uu = M[3 * c] * rr[0] + M[3 * c + 1] * rr[1] + M[3 * c + 2] * rr[2];
+            u[3 * nd + c] = uu;
+            v[0] += (double)rr[c] * (double)uu;
+            v[1] += (double)rr[c] * (double)rr[c];
+        }
+    }
+    if (grid_sum<2>(v, red) && threadIdx.x == 0) {
+        st->gamma = v[0];
+        st->rr = v[1];
+        st->rr0 = v[1];
+        st->tol2 = rtol * rtol * v[1];
+        st->beta = 0.0;
+        st->alpha = 0.0;
+        st->iter = 0;
+        st->breakdown = 0;
+        st->done = (v[1] == 0.0 || !(v[0] > 0.0)) ? 1 : 0;
+    }
+}
+
+// w = A u (LPR lanes per row), delta = w.u; finalize alpha/beta.
+template <typename HT, int LPR>
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* rowptr, const int* cols, const HT* vals,
+                                                     const HT* u, HT* w, PcgState* st, int first, Reduce red) {
+    if (st->done) return;
+    double v[1] = {0.0};
+    const int sub = threadIdx.x % LPR;
+    const int rows_per_block = blockDim.x / LPR;
+    const int nblk_rows = (n + rows_per_block - 1) / rows_per_block;
+    // XCD-contiguous row ranges
+    const int per_xcd = (nblk_rows + kNumXcd - 1) / kNumXcd;
+    for (int t = blockIdx.x; t < kNumXcd * per_xcd; t += gridDim.x) {
+        const int xcd = t % kNumXcd, local = t / kNumXcd;
+        const int rb = xcd * per_xcd + local;
+        if (rb >= nblk_rows || local >= per_xcd) continue;
+        const int row = rb * rows_per_block + threadIdx.x / LPR;
+        HT acc = 0;
+        if (row < n) {
+            const int lo = rowptr[row], hi = rowptr[row + 1];
+            for (int k = lo + sub; k < hi; k += LPR) acc += vals[k] * u[cols[k]];
+        }
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+        if (sub == 0 && row < n) {
+            w[row] = acc;
+            v[0] += (double)acc * (double)u[row];
+        }
+    }
+    if (grid_sum<1>(v, red) && threadIdx.x == 0) {
+        const double delta = v[0];
+        st->delta = delta;
+        if (first) {
+            st->alpha = st->gamma / delta;
+            st->beta = 0.0;
+            if (!(delta > 0.0)) {
+                st->breakdown = 1;
+                st->done = 1;
+            }
+        } else {
+            const double beta = st->gamma_new / st->gamma;
+            const double den = delta - beta * st->gamma_new / st->alpha;
+            if (!(den > 0.0) || !(st->gamma_new > 0.0)) {
+                st->breakdown = 1;
+                st->done = 1;
+            } else {
+                st->alpha = st->gamma_new / den;
+                st->beta = beta;
+                st->gamma = st->gamma_new;
+            }
+        }
+    }
+}
+
+// p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = M r;
+// gamma_new = r.u; rr = r.r; converged when rr <= tol2
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int N, const HT* Minv, HT* x, HT* r, HT* u, HT* p, HT* s,
+                                                       const HT* w, PcgState* st, Reduce red) {
+    if (st->done) return;
+    const HT alpha = (HT)st->alpha, beta = (HT)st->beta;
+    double v[2] = {0.0, 0.0};
+    for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < N; nd += gridDim.x * blockDim.x) {
+        HT rr[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int i = 3 * nd + c;
+            const HT pp = u[i] + beta * p[i];
+            const HT ss = w[i] + beta * s[i];
+            p[i] = pp;
+            s[i] = ss;
+            x[i] += alpha * pp;
+            rr[c] = r[i] - alpha * ss;
+            r[i] = rr[c];
+        }
+        const HT* M = Minv + (size_t)9 * nd;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const HT uu = M[3 * c] * rr[0] + M[3 * c + 1] * rr[1] + M[3 * c + 2] * rr[2];
+            u[3 * nd + c] = uu;
+            v[0] += (double)rr[c] * (double)uu;
+            v[1] += (double)rr[c] * (double)rr[c];
+        }
+    }
+    if (grid_sum<2>(v, red) && threadIdx.x == 0) {
+        st->gamma_new = v[0];
+        st->rr = v[1];
+        st->iter += 1;
+        if (v[1] <= st->tol2 || !(v[1] == v[1])) st->done = 1;
+    }
+}
+
+// ===========================================================================
+// Newton vector kernels
+// ===========================================================================
+// ||dx||^2 and g.dx (FP64)
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_step_dots(int n, const HT* dx, const double* g, double* out_dd,
+                                                      double* out_gd, Reduce red) {
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double d = (double)dx[i];
+        v[0] += d * d;
+        v[1] += g[i] * d;
+    }
+    if (grid_sum<2>(v, red) && threadIdx.x == 0) {
+        *out_dd = v[0];
+        *out_gd = v[1];
+    }
+}
+
+template <typename HT>
+__global__ void k_to_f64(long long n, const HT* a, double* out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = (double)a[i];
+}
+
+__global__ void k_axpy(int n, double alpha, const double* dx, double* q) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) q[i] += alpha * dx[i];
+}
+
+// TimeIntegrator::update_state (integrator.h:79-97)
+__global__ void k_update_state(int n, int scheme, double dt, const double* q, double* qprev, double* vprev,
+                               double* gprev, const double* gtmp) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v = (scheme == SORS_BACKWARD_EULER) ? (q[i] - qprev[i]) / dt : 2 * (q[i] - qprev[i]) / dt - vprev[i];
+        vprev[i] = v;
+        qprev[i] = q[i];
+        gprev[i] = gtmp[i];
+    }
+}
+
+}  // namespace sors
+
+// ===========================================================================
+// Host side: context and C ABI
+// ===========================================================================
+using namespace sors;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    bool alloc(size_t b) {
+        if (p && bytes >= b) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (b == 0) b = 8;
+        if (hipMalloc(&p, b) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        bytes = b;
+        return true;
+    }
+    template <typename T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+struct HipError {
+    std::string msg;
+};
+
+#define HIPX(expr)                                                                                      \
+    do {                                                                                                \
+        hipError_t _e = (expr);                                                                         \
+        if (_e != hipSuccess) throw HipError{std::string(#expr) + ": " + hipGetErrorString(_e)};         \
+    } while (0)
+
+}  // namespace
+
+struct sors_ctx {
+    int N = 0, E = 0, ed = 0, n = 0;
+    int material = SORS_NEOHOOKEAN, scheme = SORS_BACKWARD_EULER, method = SORS_MINIMIZE_NEWTON;
+    double grav[3] = {0, 0, 0};
+    double damping = 0;
+    bool has_mask = false, has_fext = false;
+    sors_options opt;
+    std::string err;
+    hipStream_t stream = nullptr;
+    // host copies needed for patterns / exports
+    std::vector<int> h_conn;   // [E][ed]
+    std::vector<int> h_mask;   // [3N]
+    // device buffers
+    DevMem conn, rest, mu, lam, mass, mask, bcval;
+    DevMem q, qprev, vprev, gprev, gtmp, gint, g, fext, dxd, qtmp;
+    DevMem rowptr, cols, diag, rank, fullmap, vals, Minv;
+    DevMem pr, pu, pp, ps, pw, px;
+    DevMem partials, counters, scal, pcgst, flag;
+    double* h_pin = nullptr;   // pinned readback area
+    HostPattern pat;
+    int pat_dtnz = -1;
+    bool pat_on_device = false;
+    hipEvent_t ev[8] = {};
+
+    unsigned* counter(int k) { return counters.as<unsigned>() + k; }
+    Reduce red(int k) { return Reduce{partials.as<double>(), counter(k)}; }
+    bool sqp() const { return method == SORS_MINIMIZE_SQP; }
+};
+
+namespace {
+
+int fail(sors_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    return code;
+}
+
+template <typename T>
+void upload(DevMem& d, const T* h, size_t count, hipStream_t st) {
+    if (!d.alloc(sizeof(T) * count)) throw HipError{"hipMalloc failed (" + std::to_string(sizeof(T) * count) + " bytes)"};
+    if (count) HIPX(hipMemcpyAsync(d.p, h, sizeof(T) * count, hipMemcpyHostToDevice, st));
+}
+
+void ensure(DevMem& d, size_t bytes) {
+    if (!d.alloc(bytes)) throw HipError{"hipMalloc failed (" + std::to_string(bytes) + " bytes)"};
+}
+
+int grid_for(long long work, int block = kBlock) {
+    long long b = (work + block - 1) / block;
+    return (int)std::max<long long>(1, std::min<long long>(b, 1 << 30));
+}
+
+void sync(sors_ctx* c) { HIPX(hipStreamSynchronize(c->stream)); }
+
+// (re)build the pattern for dt == 0 / dt != 0 and upload it
+void ensure_pattern(sors_ctx* c, double dt) {
+    const int dtnz = dt != 0.0;
+    if (c->pat_dtnz == dtnz && c->pat_on_device) return;
+    std::string msg;
+    if (!build_pattern(c->N, c->E, c->ed, c->h_conn.data(), c->has_mask ? c->h_mask.data() : nullptr, dtnz, c->sqp(),
+                       c->pat, &msg))
+        throw HipError{"pattern: " + msg};
+    upload(c->rowptr, c->pat.rowptr.data(), c->pat.rowptr.size(), c->stream);
+    upload(c->cols, c->pat.cols.data(), c->pat.cols.size(), c->stream);
+    upload(c->diag, c->pat.diag.data(), c->pat.diag.size(), c->stream);
+    if (c->pat.node_granular) {
+        upload(c->rank, c->pat.rank.data(), c->pat.rank.size(), c->stream);
+    } else {
+        upload(c->fullmap, c->pat.fullmap.data(), c->pat.fullmap.size(), c->stream);
+    }
+    ensure(c->vals, sizeof(double) * std::max<long long>(c->pat.nnz, 1));
+    sync(c);  // host vectors of the pattern stay alive in c->pat
+    c->pat_dtnz = dtnz;
+    c->pat_on_device = true;
+}
+
+ElemArgs elem_args(sors_ctx* c, const double* q) {
+    ElemArgs a{};
+    a.E = c->E;
+    a.ed = c->ed;
+    a.N = c->N;
+    a.conn = c->conn.as<int>();
+    a.rest = c->rest.as<double>();
+    a.q = q;
+    a.dq = nullptr;
+    a.alpha = 0.0;
+    a.mu = c->mu.as<double>();
+    a.lam = c->lam.as<double>();
+    a.mask = c->has_mask ? c->mask.as<int>() : nullptr;
+    a.sqp = c->sqp();
+    a.material = c->material;
+    a.hscale = 1.0;
+    a.rowptr = c->rowptr.as<int>();
+    a.rank = c->pat.node_granular ? c->rank.as<uint8_t>() : nullptr;
+    a.fullmap = c->pat.node_granular ? nullptr : c->fullmap.as<int>();
+    a.flag = c->flag.as<int>();
+    return a;
+}
+
+NodeArgs node_args(sors_ctx* c, const double* q, double dt) {
+    NodeArgs a{};
+    a.n = c->n;
+    a.dt = dt;
+    a.scheme = c->scheme;
+    a.sqp = c->sqp();
+    a.damping = c->damping;
+    for (int k = 0; k < 3; ++k) a.grav[k] = c->grav[k];
+    a.mass = c->mass.as<double>();
+    a.q = q;
+    a.qprev = c->qprev.as<double>();
+    a.vprev = c->vprev.as<double>();
+    a.gprev = c->gprev.as<double>();
+    a.gint = c->gint.as<double>();
+    a.fext = c->has_fext ? c->fext.as<double>() : nullptr;
+    a.mask = c->has_mask ? c->mask.as<int>() : nullptr;
+    a.bcval = c->bcval.as<double>();
+    a.g = c->g.as<double>();
+    a.gtmp = c->gtmp.as<double>();
+    a.diag = c->diag.as<int>();
+    return a;
+}
+
+template <typename HT, int MODE>
+void launch_elem(sors_ctx* c, const ElemArgs& a, int project) {
+    if (c->E == 0) return;
+    if (c->ed == 4) {
+        const int grid = grid_for(c->E, kBlock);
+        if (project)
+            k_tet<HT, 1, MODE><<<grid, kBlock, 0, c->stream>>>(a);
+        else
+            k_tet<HT, 0, MODE><<<grid, kBlock, 0, c->stream>>>(a);
+    } else {
+        const int grid = grid_for(c->E, 128);
+        if (project)
+            k_hex<HT, 1, MODE><<<grid, 128, 0, c->stream>>>(a);
+        else
+            k_hex<HT, 0, MODE><<<grid, 128, 0, c->stream>>>(a);
+    }
+    HIPX(hipGetLastError());
+}
+
+int read_flag(sors_ctx* c) {
+    int f = 0;
+    HIPX(hipMemcpyAsync(&f, c->flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return f;
+}
+
+void clear_flag(sors_ctx* c) { fill<int>(c->flag.as<int>(), 1, 0, c->stream); }
+
+// gradient (FP64) and, when HT != void, the assembled Hessian; returns ||g||^2 in scal[S_GG]
+template <typename HT>
+void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project) {
+    ElemArgs a = elem_args(c, q);
+    fill<double>(c->gint.as<double>(), c->n, 0.0, c->stream);
+    if (hessian) fill<HT>((HT*)c->vals.p, c->pat.nnz, (HT)0, c->stream);
+    a.gint = c->gint.as<double>();
+    a.vals = hessian ? c->vals.p : nullptr;
+    a.hscale = (c->scheme == SORS_CRANK_NICOLSON && dt != 0.0) ? 0.5 : 1.0;
+    if (hessian)
+        launch_elem<HT, 0>(c, a, project);
+    else
+        launch_elem<HT, 1>(c, a, 0);
+    NodeArgs na = node_args(c, q, dt);
+    na.vals = hessian ? c->vals.p : nullptr;
+    const int grid = red_grid(c->n, kBlock);
+    k_node_terms<HT><<<grid, kBlock, 0, c->stream>>>(na, c->scal.as<double>() + S_GG, c->red(0));
+    HIPX(hipGetLastError());
+}
+
+template <typename HT>
+void build_preconditioner(sors_ctx* c) {
+    k_block_jacobi<HT><<<grid_for(c->N), kBlock, 0, c->stream>>>(c->N, c->rowptr.as<int>(), c->cols.as<int>(),
+                                                                  c->diag.as<int>(), (const HT*)c->vals.p,
+                                                                  (HT*)c->Minv.p);
+    HIPX(hipGetLastError());
+}
+
+// PCG on the assembled matrix: solves A x = sign*b; x in c->px (HT)
+template <typename HT>
+long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_iter, bool* breakdown) {
+    const int N = c->N, n = c->n;
+    HT *x = (HT*)c->px.p, *r = (HT*)c->pr.p, *u = (HT*)c->pu.p, *p = (HT*)c->pp.p, *s = (HT*)c->ps.p,
+       *w = (HT*)c->pw.p;
+    PcgState* st = c->pcgst.as<PcgState>();
+    const HT* Minv = (const HT*)c->Minv.p;
+    const int gn = red_grid(N, kBlock);
+    constexpr int LPR = 8;
+    const int rows_per_block = kBlock / LPR;
+    // multiple of 8 so that blockIdx % 8 is the XCD the row range belongs to
+    const int gs = (red_grid((long long)(n + rows_per_block - 1) / rows_per_block * kBlock, kBlock) + kNumXcd - 1) /
+                   kNumXcd * kNumXcd;
+    const int* rp = c->rowptr.as<int>();
+    const int* cl = c->cols.as<int>();
+    const HT* vals = (const HT*)c->vals.p;
+    k_pcg_init<HT><<<gn, kBlock, 0, c->stream>>>(N, b, sign, Minv, x, r, u, p, s, st, rtol, c->red(1));
+    k_pcg_spmv<HT, LPR><<<gs, kBlock, 0, c->stream>>>(n, rp, cl, vals, u, w, st, 1, c->red(2));
+    HIPX(hipGetLastError());
+    const int chunk = std::max(1, c->opt.pcg_check_every);
+    PcgState* hst = (PcgState*)c->h_pin;
+    int it = 0;
+    while (it < max_iter) {
+        const int k = std::min(chunk, max_iter - it);
+        for (int j = 0; j < k; ++j) {
+            k_pcg_update<HT><<<gn, kBlock, 0, c->stream>>>(N, Minv, x, r, u, p, s, w, st, c->red(1));
+            k_pcg_spmv<HT, LPR><<<gs, kBlock, 0, c->stream>>>(n, rp, cl, vals, u, w, st, 0, c->red(2));
+        }
+        HIPX(hipGetLastError());
+        it += k;
+        HIPX(hipMemcpyAsync(hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        if (hst->done) break;
+    }
+    HIPX(hipMemcpyAsync(hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    if (breakdown) *breakdown = hst->breakdown != 0;
+    return hst->iter;
+}
+
+double read_scalar(sors_ctx* c, int slot) {
+    HIPX(hipMemcpyAsync(c->h_pin, c->scal.as<double>() + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return c->h_pin[0];
+}
+
+void read_scalars(sors_ctx* c, double* out, int first, int count) {
+    HIPX(hipMemcpyAsync(c->h_pin, c->scal.as<double>() + first, sizeof(double) * count, hipMemcpyDeviceToHost,
+                        c->stream));
+    sync(c);
+    for (int i = 0; i < count; ++i) out[i] = c->h_pin[i];
+}
+
+// line-search energy difference E(q + alpha dx) - E(q) (FP64, device)
+double energy_delta(sors_ctx* c, const double* q, const double* dx, double alpha, double dt) {
+    ElemArgs a = elem_args(c, q);
+    a.dq = dx;
+    a.alpha = alpha;
+    const int ge = red_grid(c->E, kBlock);
+    if (c->ed == 4)
+        k_elem_energy<4><<<ge, kBlock, 0, c->stream>>>(a, 1, c->scal.as<double>() + S_EELEM, c->red(3));
+    else
+        k_elem_energy<8><<<ge, kBlock, 0, c->stream>>>(a, 1, c->scal.as<double>() + S_EELEM, c->red(3));
+    NodeArgs na = node_args(c, q, dt);
+    k_node_energy<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(na, dx, alpha, 1,
+                                                                     c->scal.as<double>() + S_ENODE, c->red(4));
+    HIPX(hipGetLastError());
+    double v[2];
+    read_scalars(c, v, S_EELEM, 2);
+    return v[0] + v[1];
+}
+
+double energy_abs(sors_ctx* c, const double* q, double dt) {
+    ElemArgs a = elem_args(c, q);
+    const int ge = red_grid(c->E, kBlock);
+    if (c->ed == 4)
+        k_elem_energy<4><<<ge, kBlock, 0, c->stream>>>(a, 0, c->scal.as<double>() + S_EELEM, c->red(3));
+    else
+        k_elem_energy<8><<<ge, kBlock, 0, c->stream>>>(a, 0, c->scal.as<double>() + S_EELEM, c->red(3));
+    NodeArgs na = node_args(c, q, dt);
+    k_node_energy<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(na, nullptr, 0.0, 0,
+                                                                     c->scal.as<double>() + S_ENODE, c->red(4));
+    HIPX(hipGetLastError());
+    double v[2];
+    read_scalars(c, v, S_EELEM, 2);
+    return v[0] + v[1];
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// minimize_newton / minimize_newton_linesearch on the device state c->q
+template <typename HT>
+int minimize(sors_ctx* c, double dt, sors_stats* stt) {
+    const sors_options& o = c->opt;
+    double* q = c->q.as<double>();
+    double* g = c->g.as<double>();
+    double* dx = c->dxd.as<double>();
+    bool converged = false;
+    for (int it = 0; it < o.max_newton; ++it) {
+        auto t0 = std::chrono::steady_clock::now();
+        clear_flag(c);
+        assemble<HT>(c, q, dt, true, o.project_spd);
+        build_preconditioner<HT>(c);
+        const double gg = read_scalar(c, S_GG);
+        if (read_flag(c) & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
+        stt->gnorm = std::sqrt(gg);
+        stt->ms_assemble += ms_since(t0);
+        if (!(gg == gg)) return fail(c, SORS_ERR_NAN, "Newton's method diverged (NaN residual)");
+        if (o.g_tol > 0 && stt->gnorm < o.g_tol) {
+            converged = true;
+            break;
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        bool brk = false;
+        stt->pcg_iters += pcg<HT>(c, g, -1.0, o.pcg_rtol, o.pcg_max_iter, &brk);
+        k_to_f64<HT><<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, (const HT*)c->px.p, dx);
+        k_step_dots<HT><<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
+            c->n, (const HT*)c->px.p, g, c->scal.as<double>() + S_DD, c->scal.as<double>() + S_GD, c->red(5));
+        HIPX(hipGetLastError());
+        double dots[2];
+        read_scalars(c, dots, S_DD, 2);
+        stt->ms_solve += ms_since(t1);
+        stt->newton_iters++;
+        const double dxn = std::sqrt(dots[0]), gdx = dots[1];
+        if (!(dxn == dxn)) return fail(c, SORS_ERR_NAN, "Newton's method diverged");
+        if (brk && dxn == 0.0) return fail(c, SORS_ERR_NAN, "PCG breakdown (indefinite tangent)");
+        if (c->method != SORS_MINIMIZE_NEWTON_LINESEARCH || dxn < o.dx_tol) {
+            k_axpy<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, 1.0, dx, q);
+            stt->dxnorm = dxn;
+            if (dxn < o.dx_tol) {
+                converged = true;
+                break;
+            }
+            continue;
+        }
+        // Armijo backtracking (solver.cpp:225-240), energy decrease in FP64
+        auto t2 = std::chrono::steady_clock::now();
+        double alpha = 1.0;
+        bool accepted = false;
+        for (int bt = 0; bt <= o.max_backtrack; ++bt) {
+            const double dE = energy_delta(c, q, dx, alpha, dt);
+            if (dE <= 0.1 * alpha * gdx) {
+                accepted = true;
+                break;
+            }
+            alpha *= 0.5;
+            stt->backtracks++;
+        }
+        stt->ms_linesearch += ms_since(t2);
+        if (!accepted) {
+            if (alpha * 2 * dxn < o.dx_tol)
+                alpha *= 2;
+            else
+                return fail(c, SORS_ERR_LINESEARCH, "line search failed to decrease the energy");
+        }
+        k_axpy<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, alpha, dx, q);
+        HIPX(hipGetLastError());
+        stt->dxnorm = alpha * dxn;
+        if (stt->dxnorm < o.dx_tol) {
+            converged = true;
+            break;
+        }
+    }
+    if (!converged) return fail(c, SORS_ERR_NOCONV, "Newton's method did not converge");
+    return SORS_OK;
+}
+
+int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
+    if (!(dt > 0.0)) return fail(c, SORS_ERR_ARG, "dt must be > 0 (resolve settings.dt before the call)");
+    if (substeps < 1) substeps = 1;
+    if (c->method == SORS_MINIMIZE_SQP)
+        return fail(c, SORS_ERR_UNSUPPORTED, "minimize_SQP (OSQP contact) is not implemented on the device");
+    sors_stats local{};
+    if (!st) st = &local;
+    *st = sors_stats{};
+    auto t0 = std::chrono::steady_clock::now();
+    const double h = dt / substeps;
+    ensure_pattern(c, h);
+    for (int j = 0; j < substeps; ++j) {
+        int rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, h, st) : minimize<double>(c, h, st);
+        if (rc != SORS_OK) return rc;
+        k_update_state<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
+            c->n, c->scheme, h, c->q.as<double>(), c->qprev.as<double>(), c->vprev.as<double>(),
+            c->gprev.as<double>(), c->gtmp.as<double>());
+        HIPX(hipGetLastError());
+    }
+    sync(c);
+    st->ms_total = ms_since(t0);
+    return SORS_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* sors_last_error(const sors_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+void sors_default_options(sors_options* o) {
+    o->precision = SORS_MIXED;
+    o->project_spd = 1;
+    o->max_newton = 100;
+    o->dx_tol = 1e-9;
+    o->g_tol = 0.0;
+    o->pcg_rtol = 1e-6;
+    o->pcg_max_iter = 20000;
+    o->max_backtrack = 60;
+    o->pcg_check_every = 32;
+    o->use_graph = 0;
+}
+
+int sors_create(const sors_system_desc* d, sors_ctx** out) {
+    g_create_error.clear();
+    if (!d || !out) {
+        g_create_error = "null argument";
+        return SORS_ERR_ARG;
+    }
+    *out = nullptr;
+    if (d->nodes_per_elem != 4 && d->nodes_per_elem != 8) {
+        g_create_error = "nodes_per_elem must be 4 (Tetrahedron) or 8 (Hexahedron)";
+        return SORS_ERR_ARG;
+    }
+    if (d->material == SORS_TEST_POLY && d->nodes_per_elem != 4) {
+        g_create_error = "the polynomial test energy is defined for tetrahedra";
+        return SORS_ERR_ARG;
+    }
+    if (d->n_nodes <= 0 || d->n_elems < 0 || !d->rest || (d->n_elems > 0 && !d->conn)) {
+        g_create_error = "empty or missing mesh arrays";
+        return SORS_ERR_ARG;
+    }
+    if (d->scheme != SORS_BACKWARD_EULER && d->scheme != SORS_CRANK_NICOLSON) {
+        g_create_error = "Unknown time stepping scheme";
+        return SORS_ERR_ARG;
+    }
+    if (d->method < SORS_MINIMIZE_NEWTON || d->method > SORS_MINIMIZE_SQP) {
+        g_create_error = "Solver method not recognized";
+        return SORS_ERR_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_create_error = "no HIP device available";
+        return SORS_ERR_HIP;
+    }
+    std::unique_ptr<sors_ctx> c(new sors_ctx());
+    try {
+        c->N = d->n_nodes;
+        c->E = d->n_elems;
+        c->ed = d->nodes_per_elem;
+        c->n = 3 * c->N;
+        c->material = d->material;
+        c->scheme = d->scheme;
+        c->method = d->method;
+        for (int k = 0; k < 3; ++k) c->grav[k] = d->grav[k];
+        c->damping = d->damping_alpha;
+        sors_default_options(&c->opt);
+        HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        const int N = c->N, E = c->E, ed = c->ed, n = c->n;
+        c->h_conn.assign(d->conn, d->conn + (size_t)E * ed);
+        for (size_t i = 0; i < c->h_conn.size(); ++i)
+            if (c->h_conn[i] < 0 || c->h_conn[i] >= N) {
+                g_create_error = "element index out of range";
+                return SORS_ERR_ARG;
+            }
+        c->has_mask = false;
+        c->h_mask.assign(n, 0);
+        if (d->bc_mask)
+            for (int i = 0; i < n; ++i) {
+                c->h_mask[i] = d->bc_mask[i] != 0;
+                c->has_mask |= c->h_mask[i] != 0;
+            }
+        // element SoA connectivity, Lame constants (neoHookean.cpp:10-11),
+        // lumped mass rho V0 / ed per DOF (energy.cpp:176-189)
+        std::vector<int> connT((size_t)E * ed);
+        std::vector<double> mu(E), lam(E), mass(n, 0.0);
+        for (int e = 0; e < E; ++e) {
+            for (int k = 0; k < ed; ++k) connT[(size_t)k * E + e] = c->h_conn[(size_t)e * ed + k];
+            const double Ey = d->youngs ? d->youngs[e] : 0.0, nu = d->poisson ? d->poisson[e] : 0.0;
+            lam[e] = (Ey * nu) / ((1 + nu) * (1 - 2 * nu));
+            mu[e] = Ey / (2 * (1 + nu));
+            const int* cn = &c->h_conn[(size_t)e * ed];
+            auto P = [&](int k) { return d->rest + 3 * (size_t)cn[k]; };
+            auto tv = [&](int a, int b, int cc, int dd) {
+                const double *A = P(a), *B = P(b), *C = P(cc), *D = P(dd);
+                double ab[3], ac[3], ad[3];
+                for (int t = 0; t < 3; ++t) {
+                    ab[t] = B[t] - A[t];
+                    ac[t] = C[t] - A[t];
+                    ad[t] = D[t] - A[t];
+                }
+                return std::fabs(ab[0] * (ac[1] * ad[2] - ac[2] * ad[1]) + ab[1] * (ac[2] * ad[0] - ac[0] * ad[2]) +
+                                 ab[2] * (ac[0] * ad[1] - ac[1] * ad[0])) /
+                       6.0;
+            };
+            double V0;
+            if (ed == 4) {
+                V0 = tv(0, 1, 2, 3);
+            } else {
+                static const int T5[5][4] = {{0, 1, 3, 4}, {1, 2, 3, 6}, {1, 4, 5, 6}, {3, 4, 6, 7}, {1, 3, 4, 6}};
+                V0 = 0;
+                for (int t = 0; t < 5; ++t) V0 += tv(T5[t][0], T5[t][1], T5[t][2], T5[t][3]);
+            }
+            if (!(V0 > 0.0) && c->material == SORS_NEOHOOKEAN) {
+                g_create_error = "Volume of element must be positive (element " + std::to_string(e) + ")";
+                return SORS_ERR_ARG;
+            }
+            const double me = (d->density ? d->density[e] : 0.0) * V0;
+            for (int k = 0; k < ed; ++k)
+                for (int t = 0; t < 3; ++t) mass[3 * (size_t)cn[k] + t] += me / ed;
+        }
+        hipStream_t st = c->stream;
+        upload(c->conn, connT.data(), connT.size(), st);
+        upload(c->rest, d->rest, (size_t)n, st);
+        upload(c->mu, mu.data(), mu.size(), st);
+        upload(c->lam, lam.data(), lam.size(), st);
+        upload(c->mass, mass.data(), mass.size(), st);
+        upload(c->mask, c->h_mask.data(), c->h_mask.size(), st);
+        std::vector<double> bcv(n, 0.0);
+        if (d->bc_value) std::copy(d->bc_value, d->bc_value + n, bcv.begin());
+        upload(c->bcval, bcv.data(), bcv.size(), st);
+        upload(c->qprev, d->rest, (size_t)n, st);
+        upload(c->q, d->rest, (size_t)n, st);
+        std::vector<double> zeros(n, 0.0);
+        upload(c->vprev, zeros.data(), n, st);
+        upload(c->gprev, zeros.data(), n, st);
+        upload(c->gtmp, zeros.data(), n, st);
+        ensure(c->gint, sizeof(double) * n);
+        ensure(c->g, sizeof(double) * n);
+        ensure(c->dxd, sizeof(double) * n);
+        ensure(c->qtmp, sizeof(double) * n);
+        for (DevMem* v : {&c->pr, &c->pu, &c->pp, &c->ps, &c->pw, &c->px}) ensure(*v, sizeof(double) * n);
+        ensure(c->Minv, sizeof(double) * 9 * (size_t)N);
+        ensure(c->partials, sizeof(double) * kMaxRedBlocks * 4);
+        ensure(c->counters, sizeof(unsigned) * 16);
+        ensure(c->scal, sizeof(double) * S_COUNT);
+        ensure(c->pcgst, sizeof(PcgState));
+        ensure(c->flag, sizeof(int));
+        fill<unsigned>(c->counters.as<unsigned>(), 16, 0u, st);
+        fill<double>(c->scal.as<double>(), S_COUNT, 0.0, st);
+        fill<int>(c->flag.as<int>(), 1, 0, st);
+        HIPX(hipHostMalloc((void**)&c->h_pin, 4096, hipHostMallocDefault));
+        HIPX(hipStreamSynchronize(st));
+    } catch (const HipError& e) {
+        g_create_error = e.msg;
+        sors_destroy(c.release());
+        return SORS_ERR_HIP;
+    }
+    *out = c.release();
+    return SORS_OK;
+}
+
+void sors_destroy(sors_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+    delete c;
+}
+
+int sors_set_options(sors_ctx* c, const sors_options* o) {
+    if (!c || !o) return SORS_ERR_ARG;
+    if (o->precision != SORS_FP64 && o->precision != SORS_MIXED) return fail(c, SORS_ERR_ARG, "bad precision");
+    if (o->max_newton < 1 || o->pcg_max_iter < 1 || o->max_backtrack < 0)
+        return fail(c, SORS_ERR_ARG, "iteration limits must be positive");
+    c->opt = *o;
+    return SORS_OK;
+}
+
+int sors_get_options(const sors_ctx* c, sors_options* o) {
+    if (!c || !o) return SORS_ERR_ARG;
+    *o = c->opt;
+    return SORS_OK;
+}
+
+#define SORS_GUARD(c)                                                   \
+    if (!(c)) return SORS_ERR_ARG;                                      \
+    try {
+#define SORS_GUARD_END(c)                                               \
+    }                                                                   \
+    catch (const HipError& e) {                                         \
+        return fail(c, SORS_ERR_HIP, e.msg);                            \
+    }
+
+int sors_set_initial_velocity(sors_ctx* c, const double* v) {
+    SORS_GUARD(c)
+    if (!v) return fail(c, SORS_ERR_ARG, "null velocity");
+    upload(c->vprev, v, (size_t)c->n, c->stream);
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_set_initial_deformation(sors_ctx* c, const double* q) {
+    SORS_GUARD(c)
+    if (!q) return fail(c, SORS_ERR_ARG, "null positions");
+    upload(c->qprev, q, (size_t)c->n, c->stream);
+    upload(c->q, q, (size_t)c->n, c->stream);
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_set_external_force(sors_ctx* c, const double* f) {
+    SORS_GUARD(c)
+    if (!f) {
+        c->has_fext = false;
+        return SORS_OK;
+    }
+    upload(c->fext, f, (size_t)c->n, c->stream);
+    sync(c);
+    c->has_fext = true;
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_get_state(sors_ctx* c, double* qprev, double* vprev) {
+    SORS_GUARD(c)
+    if (qprev) HIPX(hipMemcpyAsync(qprev, c->qprev.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    if (vprev) HIPX(hipMemcpyAsync(vprev, c->vprev.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_get_positions(sors_ctx* c, double* q) {
+    SORS_GUARD(c)
+    if (!q) return fail(c, SORS_ERR_ARG, "null output");
+    HIPX(hipMemcpyAsync(q, c->q.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_step(sors_ctx* c, double* q, double dt, int substeps, sors_stats* st) {
+    SORS_GUARD(c)
+    if (!q) return fail(c, SORS_ERR_ARG, "null positions");
+    upload(c->q, q, (size_t)c->n, c->stream);
+    int rc = do_steps(c, dt, substeps, st);
+    if (rc != SORS_OK) return rc;
+    HIPX(hipMemcpyAsync(q, c->q.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_step_resident(sors_ctx* c, double dt, int substeps, sors_stats* st) {
+    SORS_GUARD(c)
+    return do_steps(c, dt, substeps, st);
+    SORS_GUARD_END(c)
+}
+
+int sors_energy(sors_ctx* c, const double* q, double dt, double* out) {
+    SORS_GUARD(c)
+    if (!q || !out) return fail(c, SORS_ERR_ARG, "null argument");
+    upload(c->qtmp, q, (size_t)c->n, c->stream);
+    clear_flag(c);
+    *out = energy_abs(c, c->qtmp.as<double>(), dt);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_gradient(sors_ctx* c, const double* q, double dt, double* g) {
+    SORS_GUARD(c)
+    if (!q || !g) return fail(c, SORS_ERR_ARG, "null argument");
+    upload(c->qtmp, q, (size_t)c->n, c->stream);
+    ensure_pattern(c, dt);
+    clear_flag(c);
+    assemble<double>(c, c->qtmp.as<double>(), dt, false, 0);
+    HIPX(hipMemcpyAsync(g, c->g.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+long long sors_pattern_nnz(sors_ctx* c, double dt) {
+    if (!c) return SORS_ERR_ARG;
+    try {
+        ensure_pattern(c, dt);
+        return c->pat.nnz;
+    } catch (const HipError& e) {
+        return fail(c, SORS_ERR_HIP, e.msg);
+    }
+}
+
+int sors_get_pattern(sors_ctx* c, double dt, int* rowptr, int* cols) {
+    SORS_GUARD(c)
+    ensure_pattern(c, dt);
+    // read back what the kernels use
+    if (rowptr) HIPX(hipMemcpyAsync(rowptr, c->rowptr.p, sizeof(int) * (c->n + 1), hipMemcpyDeviceToHost, c->stream));
+    if (cols) HIPX(hipMemcpyAsync(cols, c->cols.p, sizeof(int) * c->pat.nnz, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_get_scatter_map(sors_ctx* c, double dt, int* map) {
+    SORS_GUARD(c)
+    if (!map) return fail(c, SORS_ERR_ARG, "null output");
+    ensure_pattern(c, dt);
+    // expand the device-resident compact map with the kernels' index rule
+    HostPattern p;
+    p.node_granular = c->pat.node_granular;
+    p.rowptr.resize(c->n + 1);
+    HIPX(hipMemcpyAsync(p.rowptr.data(), c->rowptr.p, sizeof(int) * (c->n + 1), hipMemcpyDeviceToHost, c->stream));
+    if (p.node_granular) {
+        p.rank.resize(c->pat.rank.size());
+        HIPX(hipMemcpyAsync(p.rank.data(), c->rank.p, p.rank.size(), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        scatter_map_full(p, c->E, c->ed, c->h_conn.data(), c->has_mask ? c->h_mask.data() : nullptr, c->sqp(), map);
+    } else {
+        HIPX(hipMemcpyAsync(map, c->fullmap.p, sizeof(int) * c->pat.fullmap.size(), hipMemcpyDeviceToHost,
+                            c->stream));
+        sync(c);
+    }
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_hessian(sors_ctx* c, const double* q, double dt, int precision, int project, double* vals) {
+    SORS_GUARD(c)
+    if (!q || !vals) return fail(c, SORS_ERR_ARG, "null argument");
+    upload(c->qtmp, q, (size_t)c->n, c->stream);
+    ensure_pattern(c, dt);
+    clear_flag(c);
+    const long long nnz = c->pat.nnz;
+    if (precision == SORS_MIXED) {
+        assemble<float>(c, c->qtmp.as<double>(), dt, true, project);
+    } else {
+        assemble<double>(c, c->qtmp.as<double>(), dt, true, project);
+    }
+    sync(c);
+    if (precision == SORS_MIXED) {
+        std::vector<float> tmp(nnz);
+        HIPX(hipMemcpyAsync(tmp.data(), c->vals.p, sizeof(float) * nnz, hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        for (long long i = 0; i < nnz; ++i) vals[i] = tmp[i];
+    } else {
+        HIPX(hipMemcpyAsync(vals, c->vals.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+    }
+    if (read_flag(c) & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_element_eval(sors_ctx* c, const double* q, int precision, int project, double* energy, double* grad,
+                      double* hess) {
+    SORS_GUARD(c)
+    if (!q) return fail(c, SORS_ERR_ARG, "null positions");
+    upload(c->qtmp, q, (size_t)c->n, c->stream);
+    const int nd = 3 * c->ed;
+    DevMem de, dg, dh;
+    ensure(de, sizeof(double) * std::max(1, c->E));
+    ensure(dg, sizeof(double) * std::max<size_t>(1, (size_t)c->E * nd));
+    ensure(dh, sizeof(double) * std::max<size_t>(1, (size_t)c->E * nd * nd));
+    ElemArgs a = elem_args(c, c->qtmp.as<double>());
+    a.ex_energy = de.as<double>();
+    a.ex_grad = dg.as<double>();
+    a.ex_hess = dh.as<double>();
+    clear_flag(c);
+    if (precision == SORS_MIXED)
+        launch_elem<float, 2>(c, a, project);
+    else
+        launch_elem<double, 2>(c, a, project);
+    if (energy) HIPX(hipMemcpyAsync(energy, de.p, sizeof(double) * c->E, hipMemcpyDeviceToHost, c->stream));
+    if (grad) HIPX(hipMemcpyAsync(grad, dg.p, sizeof(double) * (size_t)c->E * nd, hipMemcpyDeviceToHost, c->stream));
+    if (hess)
+        HIPX(hipMemcpyAsync(hess, dh.p, sizeof(double) * (size_t)c->E * nd * nd, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_linear_solve(sors_ctx* c, const double* q, double dt, const double* b, double* x, double rtol, int max_iter,
+                      int* iters) {
+    SORS_GUARD(c)
+    if (!q || !b || !x) return fail(c, SORS_ERR_ARG, "null argument");
+    upload(c->qtmp, q, (size_t)c->n, c->stream);
+    upload(c->dxd, b, (size_t)c->n, c->stream);
+    ensure_pattern(c, dt);
+    clear_flag(c);
+    bool brk = false;
+    long long it;
+    if (c->opt.precision == SORS_MIXED) {
+        assemble<float>(c, c->qtmp.as<double>(), dt, true, c->opt.project_spd);
+        build_preconditioner<float>(c);
+        it = pcg<float>(c, c->dxd.as<double>(), 1.0, rtol, max_iter, &brk);
+        k_to_f64<float><<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, c->px.as<float>(), c->dxd.as<double>());
+    } else {
+        assemble<double>(c, c->qtmp.as<double>(), dt, true, c->opt.project_spd);
+        build_preconditioner<double>(c);
+        it = pcg<double>(c, c->dxd.as<double>(), 1.0, rtol, max_iter, &brk);
+        k_to_f64<double><<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, c->px.as<double>(), c->dxd.as<double>());
+    }
+    HIPX(hipMemcpyAsync(x, c->dxd.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    if (iters) *iters = (int)it;
+    if (brk) return fail(c, SORS_ERR_NAN, "PCG breakdown (indefinite tangent)");
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+}  // extern "C"

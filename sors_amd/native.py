@@ -1,0 +1,297 @@
+"""ctypes binding of the C ABI in include/sors_amd.h (libsors_amd.so).
+
+This is the only route from Python to the device path; there is no CPU
+fallback.  Loading fails loudly when the HIP library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libsors_amd.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "sors_amd.h")
+
+OK = 0
+ERR_INVERTED, ERR_NOCONV, ERR_NAN, ERR_LINESEARCH, ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6, -7
+BACKWARD_EULER, CRANK_NICOLSON = 0, 1
+MINIMIZE_NEWTON, MINIMIZE_NEWTON_LINESEARCH, MINIMIZE_SQP = 0, 1, 2
+NEOHOOKEAN, TEST_POLY = 0, 1
+FP64, MIXED = 0, 1
+
+SCHEMES = {"backward_euler": BACKWARD_EULER, "crank_nicolson": CRANK_NICOLSON}
+METHODS = {"minimize_newton": MINIMIZE_NEWTON, "minimize_newton_linesearch": MINIMIZE_NEWTON_LINESEARCH,
+           "minimize_SQP": MINIMIZE_SQP}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int), ("n_elems", C.c_int), ("nodes_per_elem", C.c_int),
+        ("rest", _dp), ("conn", _ip), ("youngs", _dp), ("poisson", _dp), ("density", _dp),
+        ("material", C.c_int), ("bc_mask", _ip), ("bc_value", _dp),
+        ("grav", C.c_double * 3), ("damping_alpha", C.c_double), ("scheme", C.c_int), ("method", C.c_int),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int), ("project_spd", C.c_int), ("max_newton", C.c_int), ("dx_tol", C.c_double),
+        ("g_tol", C.c_double), ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int), ("max_backtrack", C.c_int),
+        ("pcg_check_every", C.c_int), ("use_graph", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("newton_iters", C.c_int), ("pcg_iters", C.c_longlong), ("backtracks", C.c_int), ("gnorm", C.c_double),
+        ("dxnorm", C.c_double), ("ms_assemble", C.c_double), ("ms_solve", C.c_double),
+        ("ms_linesearch", C.c_double), ("ms_total", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SorsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libsors_amd.so; raises if it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run `python -m sors_amd.build` "
+                              "(the HIP library is required; there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.sors_create.argtypes = [C.POINTER(SystemDesc), C.POINTER(vp)]
+        L.sors_destroy.argtypes = [vp]
+        L.sors_destroy.restype = None
+        L.sors_last_error.argtypes = [vp]
+        L.sors_last_error.restype = C.c_char_p
+        L.sors_default_options.argtypes = [C.POINTER(Options)]
+        L.sors_default_options.restype = None
+        L.sors_set_options.argtypes = [vp, C.POINTER(Options)]
+        L.sors_get_options.argtypes = [vp, C.POINTER(Options)]
+        for f in ("sors_set_initial_velocity", "sors_set_initial_deformation", "sors_set_external_force",
+                  "sors_get_positions"):
+            getattr(L, f).argtypes = [vp, _dp]
+        L.sors_get_state.argtypes = [vp, _dp, _dp]
+        L.sors_step.argtypes = [vp, _dp, C.c_double, C.c_int, C.POINTER(Stats)]
+        L.sors_step_resident.argtypes = [vp, C.c_double, C.c_int, C.POINTER(Stats)]
+        L.sors_energy.argtypes = [vp, _dp, C.c_double, _dp]
+        L.sors_gradient.argtypes = [vp, _dp, C.c_double, _dp]
+        L.sors_pattern_nnz.argtypes = [vp, C.c_double]
+        L.sors_pattern_nnz.restype = C.c_longlong
+        L.sors_get_pattern.argtypes = [vp, C.c_double, _ip, _ip]
+        L.sors_get_scatter_map.argtypes = [vp, C.c_double, _ip]
+        L.sors_hessian.argtypes = [vp, _dp, C.c_double, C.c_int, C.c_int, _dp]
+        L.sors_element_eval.argtypes = [vp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.sors_linear_solve.argtypes = [vp, _dp, C.c_double, _dp, _dp, C.c_double, C.c_int, _ip]
+        _lib = L
+    return _lib
+
+
+def declared_symbols(header: str = HEADER) -> list[str]:
+    """Function names declared in include/sors_amd.h."""
+    text = open(header).read()
+    return sorted(set(re.findall(r"^\s*[\w\s\*]+?\b(sors_\w+)\s*\(", text, flags=re.M)))
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().sors_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Rollout:
+    """One device-resident rollout: an Energy + TimeIntegrator state on the GPU."""
+
+    def __init__(self, X, conn, youngs, poisson, density, bcmask=None, bcvalue=None, grav=(0.0, 0.0, 0.0),
+                 alpha=0.0, scheme=BACKWARD_EULER, method=MINIMIZE_NEWTON_LINESEARCH, material=NEOHOOKEAN,
+                 **options):
+        L = lib()
+        self.X = f64(X).reshape(-1, 3)
+        self.conn = i32(conn)
+        self.N = self.X.shape[0]
+        self.n = 3 * self.N
+        self.E, self.ed = self.conn.shape
+        n = self.n
+        self._keep = dict(Y=f64(youngs), P=f64(poisson), D=f64(density),
+                          M=i32(bcmask) if bcmask is not None else None,
+                          V=f64(bcvalue) if bcvalue is not None else None)
+        d = SystemDesc()
+        d.n_nodes, d.n_elems, d.nodes_per_elem = self.N, self.E, self.ed
+        d.rest, d.conn = _d(self.X), _i(self.conn)
+        d.youngs, d.poisson, d.density = _d(self._keep["Y"]), _d(self._keep["P"]), _d(self._keep["D"])
+        d.material = material
+        d.bc_mask = _i(self._keep["M"])
+        d.bc_value = _d(self._keep["V"])
+        for k in range(3):
+            d.grav[k] = float(grav[k])
+        d.damping_alpha = float(alpha)
+        d.scheme, d.method = scheme, method
+        h = C.c_void_p()
+        rc = L.sors_create(C.byref(d), C.byref(h))
+        if rc != OK:
+            raise SorsError(rc, L.sors_last_error(None).decode())
+        self.h = h
+        self._n = n
+        if options:
+            self.set_options(**options)
+
+    @classmethod
+    def from_scene(cls, sc, method=MINIMIZE_NEWTON_LINESEARCH, scheme=BACKWARD_EULER, **options):
+        r = cls(sc.X, sc.conn, sc.youngs, sc.poisson, sc.density, sc.bcmask, sc.bcvalue, sc.grav, sc.alpha,
+                scheme=scheme, method=method, **options)
+        r.set_initial_velocity(sc.v0)
+        return r
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sors_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != OK:
+            raise SorsError(rc, lib().sors_last_error(self.h).decode())
+
+    # ---- options / state -------------------------------------------------
+    def options(self) -> Options:
+        o = Options()
+        self._check(lib().sors_get_options(self.h, C.byref(o)))
+        return o
+
+    def set_options(self, **kw):
+        o = self.options()
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self._check(lib().sors_set_options(self.h, C.byref(o)))
+
+    def set_initial_velocity(self, v):
+        v = f64(v).reshape(-1)
+        self._check(lib().sors_set_initial_velocity(self.h, _d(v)))
+
+    def set_initial_deformation(self, q):
+        q = f64(q).reshape(-1)
+        self._check(lib().sors_set_initial_deformation(self.h, _d(q)))
+
+    def set_external_force(self, f):
+        if f is None:
+            self._check(lib().sors_set_external_force(self.h, None))
+        else:
+            f = f64(f).reshape(-1)
+            self._check(lib().sors_set_external_force(self.h, _d(f)))
+
+    def state(self):
+        q, v = np.zeros(self.n), np.zeros(self.n)
+        self._check(lib().sors_get_state(self.h, _d(q), _d(v)))
+        return q, v
+
+    def positions(self):
+        q = np.zeros(self.n)
+        self._check(lib().sors_get_positions(self.h, _d(q)))
+        return q
+
+    # ---- stepping --------------------------------------------------------
+    def step(self, q, dt, substeps=1):
+        q = f64(q).reshape(-1).copy()
+        st = Stats()
+        self._check(lib().sors_step(self.h, _d(q), float(dt), int(substeps), C.byref(st)))
+        return q, st
+
+    def step_resident(self, dt, substeps=1):
+        st = Stats()
+        self._check(lib().sors_step_resident(self.h, float(dt), int(substeps), C.byref(st)))
+        return st
+
+    # ---- inspection ------------------------------------------------------
+    def energy(self, q, dt):
+        q = f64(q).reshape(-1)
+        out = np.zeros(1)
+        self._check(lib().sors_energy(self.h, _d(q), float(dt), _d(out)))
+        return float(out[0])
+
+    def gradient(self, q, dt):
+        q = f64(q).reshape(-1)
+        g = np.zeros(self.n)
+        self._check(lib().sors_gradient(self.h, _d(q), float(dt), _d(g)))
+        return g
+
+    def pattern(self, dt):
+        nnz = lib().sors_pattern_nnz(self.h, float(dt))
+        if nnz < 0:
+            self._check(int(nnz))
+        rp = np.zeros(self.n + 1, np.int32)
+        cols = np.zeros(max(nnz, 1), np.int32)
+        self._check(lib().sors_get_pattern(self.h, float(dt), _i(rp), _i(cols)))
+        return rp, cols[:nnz]
+
+    def scatter_map(self, dt):
+        nd = 3 * self.ed
+        m = np.zeros(self.E * nd * nd, np.int32)
+        self._check(lib().sors_get_scatter_map(self.h, float(dt), _i(m)))
+        return m.reshape(self.E, nd, nd)
+
+    def hessian(self, q, dt, precision=FP64, project=False):
+        q = f64(q).reshape(-1)
+        rp, cols = self.pattern(dt)
+        vals = np.zeros(max(len(cols), 1))
+        self._check(lib().sors_hessian(self.h, _d(q), float(dt), int(precision), int(project), _d(vals)))
+        return rp, cols, vals[: len(cols)]
+
+    def hessian_dense(self, q, dt, precision=FP64, project=False):
+        rp, cols, vals = self.hessian(q, dt, precision, project)
+        H = np.zeros((self.n, self.n))
+        for r in range(self.n):
+            H[r, cols[rp[r]:rp[r + 1]]] = vals[rp[r]:rp[r + 1]]
+        return H
+
+    def element_eval(self, q, precision=FP64, project=False):
+        q = f64(q).reshape(-1)
+        nd = 3 * self.ed
+        e, g, h = np.zeros(self.E), np.zeros(self.E * nd), np.zeros(self.E * nd * nd)
+        self._check(lib().sors_element_eval(self.h, _d(q), int(precision), int(project), _d(e), _d(g), _d(h)))
+        return e, g.reshape(self.E, nd), h.reshape(self.E, nd, nd)
+
+    def linear_solve(self, q, dt, b, rtol=1e-10, max_iter=20000):
+        q = f64(q).reshape(-1)
+        b = f64(b).reshape(-1)
+        x = np.zeros(self.n)
+        it = C.c_int(0)
+        self._check(lib().sors_linear_solve(self.h, _d(q), float(dt), _d(b), _d(x), float(rtol), int(max_iter),
+                                            C.byref(it)))
+        return x, it.value

@@ -1,0 +1,212 @@
+"""Parity of the HIP path (through the C ABI) with the CPU FP64 oracle.
+
+Bars: CSR pattern and scatter map bit-exact; per-element FP64 energy /
+gradient / Hessian to rel 1e-10; FP32 element tangents to rel 1e-4 of the
+block scale; converged positions within 1e-6 of the bounding-box diagonal.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from sors_amd import mesh, native
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "element_nh.npz")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rollout(sc, **kw):
+    return native.Rollout.from_scene(sc, **kw)
+
+
+def perturbed(sc, scale, seed=0):
+    rng = np.random.default_rng(seed)
+    q = sc.X.reshape(-1) + rng.normal(scale=scale, size=3 * sc.n_nodes) * (1 - sc.bcmask)
+    return q
+
+
+def rel_err(a, b, scale=None):
+    s = np.abs(b).max() if scale is None else scale
+    return np.abs(a - b).max() / max(s, 1e-300)
+
+
+@pytest.mark.parametrize("element", ["tet", "hex"])
+def test_element_fp64_matches_oracle(orc, element):
+    sc = mesh.beam(nx=4, ny=2, nz=3, element=element, e_range=(5e5, 5e6), seed=3)
+    say("element_eval", element, sc.n_elems)
+    r = rollout(sc)
+    q = perturbed(sc, 3e-4)
+    e, g, H = r.element_eval(q, native.FP64, project=False)
+    ed = sc.ed
+    for k in range(sc.n_elems):
+        Xe = sc.X[sc.conn[k]]
+        xe = q.reshape(-1, 3)[sc.conn[k]]
+        oe, og, oh = orc.elem_eval(Xe, xe, sc.youngs[k], sc.poisson[k])
+        hs = np.abs(oh).max()
+        size = np.ptp(Xe, axis=0).max()
+        assert abs(e[k] - oe) <= 1e-10 * max(abs(oe), 1e-3 * hs * size**2)
+        assert np.abs(g[k] - og).max() <= 1e-10 * max(np.abs(og).max(), 1e-3 * hs * size)
+        assert np.abs(H[k] - oh).max() <= 1e-10 * hs
+    assert g.shape == (sc.n_elems, 3 * ed)
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_element_golden_vectors_fp64(kind):
+    d = np.load(GOLD)
+    Xs, xs, Es, nus = d[f"{kind}_X"], d[f"{kind}_x"], d[f"{kind}_E"], d[f"{kind}_nu"]
+    ed = Xs.shape[1]
+    for X, x, E, nu, e, g, H in zip(Xs, xs, Es, nus, d[f"{kind}_energy"], d[f"{kind}_grad"], d[f"{kind}_hess"]):
+        conn = np.arange(ed, dtype=np.int32)[None]
+        r = native.Rollout(X, conn, [E], [nu], [1000.0])
+        de, dg, dh = r.element_eval(x.reshape(-1), native.FP64)
+        hs = np.abs(H).max()
+        size = np.ptp(X, axis=0).max()
+        assert abs(de[0] - e) <= 1e-10 * max(abs(e), 1e-3 * hs * size**2)
+        assert np.abs(dg[0] - g).max() <= 1e-10 * max(np.abs(g).max(), 1e-3 * hs * size)
+        assert np.abs(dh[0] - H).max() <= 1e-10 * hs
+        r.close()
+
+
+@pytest.mark.parametrize("element", ["tet", "hex"])
+def test_projected_tangent_matches_eigen_clamp(orc, element):
+    sc = mesh.beam(nx=3, ny=2, nz=2, element=element, E=3e5, nu=0.45)
+    r = rollout(sc)
+    q = perturbed(sc, 8e-4, seed=1)
+    _, _, H64 = r.element_eval(q, native.FP64, project=True)
+    _, _, H32 = r.element_eval(q, native.MIXED, project=True)
+    worst32 = 0.0
+    for k in range(sc.n_elems):
+        Xe = sc.X[sc.conn[k]]
+        xe = q.reshape(-1, 3)[sc.conn[k]]
+        _, _, oh = orc.elem_eval(Xe, xe, sc.youngs[k], sc.poisson[k], project=True)
+        hs = np.abs(oh).max()
+        assert np.abs(H64[k] - oh).max() <= 1e-9 * hs
+        worst32 = max(worst32, np.abs(H32[k] - oh).max() / hs)
+        assert np.linalg.eigvalsh(0.5 * (H64[k] + H64[k].T)).min() >= -1e-9 * hs
+    say("projected FP32 tangent worst rel err", worst32)
+    assert worst32 < 1e-4
+
+
+@pytest.mark.parametrize("case", ["tet_clamp", "hex_clamp", "tet_partial", "tet_sqp", "tet_dt0"])
+def test_pattern_and_scatter_map_bit_exact(orc, case):
+    element = "hex" if case.startswith("hex") else "tet"
+    sc = mesh.beam(nx=4, ny=2, nz=3, element=element)
+    mask = sc.bcmask.copy()
+    if case == "tet_partial":
+        mask[:] = 0
+        mask[[0, 7, 8, 20, 33]] = 1  # per-DOF mask: the general (full) map path
+    method = native.MINIMIZE_SQP if case == "tet_sqp" else native.MINIMIZE_NEWTON
+    omethod = orc.SQP if case == "tet_sqp" else orc.NEWTON
+    dt = 0.0 if case == "tet_dt0" else 1e-3
+    say("pattern", case)
+    r = native.Rollout(sc.X, sc.conn, sc.youngs, sc.poisson, sc.density, mask, sc.bcvalue, sc.grav, method=method)
+    o = orc.System(sc.X, sc.conn, sc.youngs, sc.poisson, sc.density, mask, sc.bcvalue, sc.grav, method=omethod)
+    rp, cols = r.pattern(dt)
+    orp, ocols = o.pattern(dt)
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+    assert np.array_equal(r.scatter_map(dt), o.scatter_map(dt))
+
+
+def test_pattern_and_map_cantilever_small(orc):
+    sc = mesh.cantilever_small()
+    r = rollout(sc)
+    o = orc.System.from_scene(sc)
+    rp, cols = r.pattern(sc.dt)
+    orp, ocols = o.pattern(sc.dt)
+    assert len(cols) == 52437
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+    assert np.array_equal(r.scatter_map(sc.dt), o.scatter_map(sc.dt))
+
+
+@pytest.mark.parametrize("element", ["tet", "hex"])
+@pytest.mark.parametrize("scheme", ["be", "cn"])
+def test_gradient_energy_hessian_fp64(orc, element, scheme):
+    sc = mesh.beam(nx=4, ny=2, nz=2, element=element, e_range=(5e5, 5e6), seed=5)
+    osch = orc.BE if scheme == "be" else orc.CN
+    dsch = native.BACKWARD_EULER if scheme == "be" else native.CRANK_NICOLSON
+    r = rollout(sc, scheme=dsch, method=native.MINIMIZE_NEWTON)
+    o = orc.System.from_scene(sc, scheme=osch, method=orc.NEWTON)
+    q = perturbed(sc, 2e-4, seed=2)
+    dt = 1e-3
+    gd, go = r.gradient(q, dt), o.gradient(q, dt)
+    assert rel_err(gd, go) < 1e-10
+    ed_, eo = r.energy(q, dt), o.energy(q, dt)
+    assert abs(ed_ - eo) <= 1e-12 * max(abs(eo), 1.0)
+    _, _, hd = r.hessian(q, dt, native.FP64, project=False)
+    _, _, ho = o.hessian(q, dt)
+    assert rel_err(hd, ho) < 1e-10
+    _, _, hm = r.hessian(q, dt, native.MIXED, project=False)
+    say("FP32 Hessian rel err", rel_err(hm, ho))
+    assert rel_err(hm, ho) < 1e-5
+
+
+def test_reference_dummy_energy_tests_on_device():
+    # cpp/src/_test.cpp:136-176 with the polynomial test element on the device
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, -1, 0]], float)
+    conn = np.array([[0, 1, 2, 3], [0, 1, 4, 3]], np.int32)
+    r = native.Rollout(X, conn, [0.0, 0.0], [0.0, 0.0], [1.0, 1.0], scheme=native.CRANK_NICOLSON,
+                       method=native.MINIMIZE_NEWTON, material=native.TEST_POLY)
+    q = X.reshape(-1)
+    assert r.energy(q, 0.0) == 3.0
+    gradGT = np.array([[0, 0, 0], [6, 0, 0], [0, 3, 0], [0, 0, 6], [0, -3, 0]], float).reshape(-1)
+    assert np.allclose(r.gradient(q, 0.0), gradGT)
+    hess = np.array([[-2, -2, -2], [22, -2, -2], [-1, 11, -1], [-2, -2, 22], [-1, 11, -1]], float).reshape(-1)
+    assert np.allclose(r.hessian_dense(q, 0.0), np.diag(hess))
+
+
+def test_cube_free_fall_on_device():
+    # cpp/src/_test.cpp:395-443
+    X = np.array([[0, 0, 0], [0, 1, 0], [1, 1, 0], [1, 0, 0], [0, 0, 1], [0, 1, 1], [1, 1, 1], [1, 0, 1]], float)
+    conn = np.array([[0, 1, 3, 4], [1, 4, 5, 6], [1, 2, 3, 6], [3, 4, 6, 7], [1, 3, 4, 6]], np.int32)
+    for prec in (native.FP64, native.MIXED):
+        r = native.Rollout(X, conn, np.full(5, 5e5), np.full(5, 0.4), np.ones(5), grav=(0, 0, -10),
+                           method=native.MINIMIZE_NEWTON, precision=prec)
+        q = X.reshape(-1).copy()
+        com = [q.reshape(-1, 3).mean(0)[2]]
+        for _ in range(9):
+            q, st = r.step(q, 0.01)
+            com.append(q.reshape(-1, 3).mean(0)[2])
+        com = np.array(com)
+        acc = (-2 * com[1:-1] + com[2:] + com[:-2]) / 0.01**2
+        assert acc[1:6].mean() == pytest.approx(-10.0, abs=1e-3)
+
+
+@pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
+def test_linear_solve_matches_direct(orc, precision):
+    sc = mesh.beam(nx=6, ny=2, nz=2, E=1e6)
+    r = rollout(sc, method=native.MINIMIZE_NEWTON, precision=precision)
+    o = orc.System.from_scene(sc, method=orc.NEWTON)
+    q = perturbed(sc, 1e-4, seed=4)
+    dt = 1e-3
+    H = o.hessian_dense(q, dt, project=True)
+    rng = np.random.default_rng(9)
+    b = rng.normal(size=3 * sc.n_nodes)
+    x, it = r.linear_solve(q, dt, b, rtol=1e-10 if precision == native.FP64 else 1e-6)
+    xr = np.linalg.solve(H, b)
+    err = np.abs(x - xr).max() / np.abs(xr).max()
+    say("pcg iters", it, "rel err", err)
+    assert err < (1e-8 if precision == native.FP64 else 1e-4)
+
+
+@pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
+def test_cantilever_small_trajectory(orc, precision):
+    """20x4x4 Kuhn beam, 200 implicit steps, Newton + Armijo: every state
+    within 1e-6 of the bounding-box diagonal of the FP64 oracle (LDL^T on the
+    raw Hessian, the reference's algorithm)."""
+    sc = mesh.cantilever_small()
+    say("cantilever_small", "fp64" if precision == native.FP64 else "mixed")
+    o = orc.System.from_scene(sc, method=orc.NEWTON_LS)
+    r = rollout(sc, precision=precision, pcg_rtol=1e-6)
+    q = qo = sc.X.reshape(-1).copy()
+    worst = 0.0
+    for k in range(sc.steps):
+        qo, _ = o.step(qo, sc.dt)
+        q, st = r.step(q, sc.dt)
+        worst = max(worst, np.abs(q - qo).max())
+    say("worst |dq| / bbox diag", worst / sc.bbox_diag)
+    assert worst <= 1e-6 * sc.bbox_diag
