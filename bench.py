@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Implicit Newton steps per second on one MI355X.
+
+Workload: the scaled beam -- 200x40x40 cells split into 1,920,000 Kuhn tet4
+(337,881 nodes, 1,013,643 DOFs), compressible Neo-Hookean with E drawn
+log-uniform in [5e5, 5e6] Pa per cell (SplitMix64 seed 0), nu = 0.45,
+rho = 1100, gravity -z, x = 0 clamped, initial velocity 0.5 m/s * x/L in +z,
+backward Euler dt = 1e-3, Newton + Armijo line search, block-Jacobi PCG.
+A step is one call of Solver::step on device-resident state.
+
+The timed run uses mixed precision (FP32 element tangents / CSR / SpMV / PCG
+vectors, FP64 state, residual, energies and dot products).  The same steps
+are then repeated in all-FP64 and the final states compared; the CPU
+baseline is the FP64 oracle timed on a bounded sample on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --profile-kernel spmv|assemble|copy|update [--reps R]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "implicit steps/sec @1M tet (≤1e-6 err vs FP64); served solves/sec + p50/p99"
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_scene(args):
+    from sors_amd import mesh
+
+    if args.small:
+        return mesh.beam(nx=40, ny=8, nz=8, element=args.element, e_range=(5e5, 5e6), seed=0, steps=20,
+                         name="beam40x8x8")
+    return mesh.cantilever_large(element=args.element, seed=0)
+
+
+def run_gpu(sc, precision, steps, warmup, profile):
+    from sors_amd import native
+
+    r = native.Rollout.from_scene(sc, precision=precision, pcg_rtol=1e-4 if precision == native.MIXED else 1e-4,
+                                  profile_kernels=int(profile))
+    for _ in range(warmup):
+        r.step_resident(sc.dt)
+    agg = {"newton_iters": 0, "pcg_iters": 0, "backtracks": 0, "spmv_ms": 0.0, "spmv_launches": 0,
+           "assemble_kernel_ms": 0.0, "assemble_launches": 0, "ms_assemble": 0.0, "ms_solve": 0.0,
+           "ms_linesearch": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = r.step_resident(sc.dt)
+        for k in agg:
+            agg[k] += getattr(st, k)
+    elapsed = time.perf_counter() - t0
+    q = r.positions()
+    return r, q, elapsed, agg
+
+
+def cpu_baseline(sc, agg, steps, cores):
+    """FP64 oracle timed on a bounded sample of the same workload."""
+    from oracle import oracle as O
+
+    o = O.System.from_scene(sc, method=O.NEWTON_LS, nthreads=cores)
+    q = sc.X.reshape(-1).copy()
+    rng = np.random.default_rng(0)
+    q = q + 1e-6 * rng.normal(size=q.size) * (1 - sc.bcmask)
+    o.pattern(sc.dt)
+    t0 = time.perf_counter()
+    o.gradient(q, sc.dt)
+    t_grad = time.perf_counter() - t0
+    pcg_iters = 20
+    t0 = time.perf_counter()
+    t_iter = o.bench_pcg(q, sc.dt, pcg_iters, project=True)  # includes one projected Hessian assembly
+    t_hess_and_pcg = time.perf_counter() - t0
+    t_hess = t_hess_and_pcg - t_iter * pcg_iters
+    t0 = time.perf_counter()
+    o.energy(q, sc.dt)
+    t_energy = time.perf_counter() - t0
+    nn = agg["newton_iters"] / steps
+    npcg = agg["pcg_iters"] / steps
+    nen = (agg["newton_iters"] + agg["backtracks"]) / steps  # energy passes per step (2 per trial: new + old)
+    sec_per_step = nn * (t_grad + t_hess) + npcg * t_iter + 2 * nen * t_energy
+    sample = (f"FP64 oracle, {cores} OpenMP threads: 1 gradient ({t_grad:.2f}s) + 1 SPD-projected Hessian "
+              f"({t_hess:.2f}s) + {pcg_iters} block-Jacobi PCG iterations ({t_iter * 1e3:.1f} ms/iter) + 1 energy "
+              f"({t_energy:.2f}s) on the full mesh, scaled by the GPU run's per-step counts "
+              f"({nn:.2f} Newton, {npcg:.0f} PCG, {nen:.2f} line-search trials)")
+    return {"value": 1.0 / sec_per_step, "unit": "steps/s", "cores": cores, "kind": "port", "sample": sample}
+
+
+def load_traffic(key):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def profile_kernel(args):
+    """Launch one kernel `reps` times on an assembled system (for rocprofv3 runs)."""
+    from sors_amd import native
+
+    sc = make_scene(args)
+    prec = native.FP64 if args.fp64 else native.MIXED
+    r = native.Rollout.from_scene(sc, precision=prec)
+    rng = np.random.default_rng(1)
+    q = sc.X.reshape(-1) + 1e-5 * rng.normal(size=3 * sc.n_nodes) * (1 - sc.bcmask)
+    r.set_initial_deformation(q)
+    r.linear_solve(q, sc.dt, np.ones(3 * sc.n_nodes), rtol=0.0, max_iter=2)
+    kind = {"spmv": native.KERNEL_SPMV, "assemble": native.KERNEL_ASSEMBLE, "copy": native.KERNEL_COPY,
+            "update": native.KERNEL_PCG_UPDATE}[args.profile_kernel]
+    ms, by = r.bench_kernel(kind, args.reps)
+    print(json.dumps({"kernel": args.profile_kernel, "avg_ms": ms, "algo_bytes": by,
+                      "GBps": by / (ms * 1e-3) / 1e9, "reps": args.reps}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--element", choices=["tet", "hex"], default="tet")
+    ap.add_argument("--small", action="store_true", help="40x8x8 cells instead of 200x40x40 (debug)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--profile-kernel", choices=["spmv", "assemble", "copy", "update"])
+    ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--fp64", action="store_true", help="profile-kernel in FP64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+    if args.profile_kernel:
+        profile_kernel(args)
+        return
+    from sors_amd import native
+
+    sc = make_scene(args)
+    log(f"workload {sc.name}: {sc.n_elems} elements, {sc.n_nodes} nodes; warmup {args.warmup}, steps {args.steps}")
+    r, q_mixed, elapsed, agg = run_gpu(sc, native.MIXED, args.steps, args.warmup, profile=True)
+    nnz = int(len(r.pattern(sc.dt)[1]))
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"mixed: {ms_per_step:.1f} ms/step, {agg}")
+    spmv_ms = agg["spmv_ms"] / max(1, agg["spmv_launches"])
+    n = 3 * sc.n_nodes
+    # compulsory bytes of one SpMV launch for the layout in use (values once,
+    # one column index per nonzero 3x3 node block, x and y once)
+    iso_ms, spmv_bytes = r.bench_kernel(native.KERNEL_SPMV, reps=20)
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    r.close()
+    out = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{sc.name}: {sc.n_elems} {'tet4' if sc.ed == 4 else 'hex8'}, {sc.n_nodes} nodes, "
+                        f"{n} DOFs, Hessian nnz {nnz}; E log-uniform [5e5,5e6] seed 0, nu 0.45, BE dt 1e-3, "
+                        "Newton + Armijo, block-Jacobi PCG rtol 1e-4, mixed FP32/FP64",
+            "elements": sc.n_elems, "nodes": sc.n_nodes, "nnz": nnz, "precision": "mixed",
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "k_pcg_spmv_blk (sliced 3x3 node-block SpMV + w.u)", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic("spmv_tet_mixed"), "algo_bytes_per_launch": spmv_bytes,
+            "avg_launch_ms": spmv_ms, "launches": agg["spmv_launches"], "isolated_avg_launch_ms": iso_ms,
+        },
+        "solver": {k: (v / args.steps) for k, v in agg.items()},
+    }
+    if agg["assemble_launches"]:
+        asm_ms = agg["assemble_kernel_ms"] / agg["assemble_launches"]
+        asm_bytes = 4.0 * sc.ed * sc.n_elems + 16.0 * sc.n_elems + 48.0 * sc.n_nodes + 4.0 * nnz + 8.0 * n
+        out["assembly"] = {"avg_launch_ms": asm_ms, "algo_bytes_per_launch": asm_bytes,
+                           "achieved_GBps": asm_bytes / (asm_ms * 1e-3) / 1e9}
+    if not args.no_fp64:
+        r64, q64, el64, agg64 = run_gpu(sc, native.FP64, args.steps, args.warmup, profile=False)
+        r64.close()
+        err = float(np.abs(q_mixed - q64).max() / sc.bbox_diag)
+        out["fp64_gpu"] = {"value": args.steps / el64, "unit": "steps/s", "ms_per_step": 1e3 * el64 / args.steps,
+                           "max_pos_diff_over_bbox_vs_mixed": err,
+                           "newton_per_step": agg64["newton_iters"] / args.steps,
+                           "pcg_per_step": agg64["pcg_iters"] / args.steps}
+        log(f"fp64: {1e3 * el64 / args.steps:.1f} ms/step; mixed vs fp64 max diff / bbox = {err:.3e}")
+    if not args.no_cpu:
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(sc, agg, args.steps, cores)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

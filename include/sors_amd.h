@@ -80,6 +80,7 @@ typedef struct {
     int max_backtrack;
     int pcg_check_every;  /* PCG iterations between host convergence checks */
     int use_graph;        /* replay PCG iteration chunks from a captured hipGraph */
+    int profile_kernels;  /* time every SpMV and assembly launch with HIP events */
 } sors_options;
 
 typedef struct {
@@ -92,6 +93,10 @@ typedef struct {
     double ms_solve;
     double ms_linesearch;
     double ms_total;
+    double spmv_ms;            /* summed SpMV kernel time (profile_kernels) */
+    long long spmv_launches;
+    double assemble_kernel_ms; /* summed element-assembly kernel time (profile_kernels) */
+    long long assemble_launches;
 } sors_stats;
 
 typedef struct sors_ctx sors_ctx;
@@ -144,6 +149,16 @@ int sors_element_eval(sors_ctx* ctx, const double* q, int precision, int project
 /* solve H(q) x = b with the device block-Jacobi PCG (precision per options) */
 int sors_linear_solve(sors_ctx* ctx, const double* q, double dt, const double* b, double* x, double rtol,
                       int max_iter, int* iters);
+
+/* ---- kernel micro-benchmarks (profiling runs) ------------------------ */
+enum { SORS_KERNEL_SPMV = 0, SORS_KERNEL_ASSEMBLE = 1, SORS_KERNEL_COPY = 2, SORS_KERNEL_PCG_UPDATE = 3 };
+/* Launch one kernel `reps` times back to back on the context's stream, on the
+ * matrix / state left by the last step, timed with HIP events.  avg_ms is
+ * the mean duration of one launch; algo_bytes the compulsory HBM bytes of
+ * one launch (SpMV: vals + cols + rowptr + x + y; COPY: n values read and
+ * written; ASSEMBLE: connectivity, Lame constants, node positions, CSR
+ * values and gradient written once). */
+int sors_bench_kernel(sors_ctx* ctx, int kernel, int reps, double* avg_ms, double* algo_bytes);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,8 @@ def lib():
         L.orc_nh_dpsi.argtypes = [_dp, C.c_double, C.c_double, _dp]
         L.orc_nh_d2psi.argtypes = [_dp, C.c_double, C.c_double, _dp]
         L.orc_project_psd.argtypes = [C.c_int, _dp]
+        L.orc_bench_pcg.restype = C.c_double
+        L.orc_bench_pcg.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -308,6 +310,10 @@ class System:
         if code:
             raise RuntimeError(ERRORS.get(code, code))
         return q, st
+
+    def bench_pcg(self, q, dt, iters, project=True):
+        q = f64(q).reshape(-1)
+        return lib().orc_bench_pcg(self.h, _d(q), float(dt), int(iters), int(project))
 
     def update_state(self, q, dt):
         q = f64(q).reshape(-1)

@@ -1126,3 +1126,22 @@ int orc_step(orc_sys* s, double* q, double dt, int substeps, const orc_opts* o, 
     s->nthreads = saved;
     return status;
 }
+
+/* CPU-baseline sample: assemble the (projected) Hessian at q and run exactly
+ * `iters` FP64 PCG iterations on it; returns seconds per PCG iteration. */
+double orc_bench_pcg(orc_sys* s, const double* q, double dt, int iters, int project) {
+    orc_pattern(s, dt);
+    int n = s->n;
+    double* vals = (double*)malloc(sizeof(double) * (size_t)(s->nnz > 0 ? s->nnz : 1));
+    double* b = (double*)malloc(sizeof(double) * n);
+    double* x = (double*)malloc(sizeof(double) * n);
+    orc_hessian(s, q, dt, project, vals);
+    for (int i = 0; i < n; ++i) b[i] = 1.0 + 1e-3 * (i % 7);
+    double t0 = now_s();
+    long long k = pcg_solve(n, s->rowptr, s->cols, vals, b, x, 0.0, iters, s->nthreads);
+    double t = now_s() - t0;
+    free(vals);
+    free(b);
+    free(x);
+    return k > 0 ? t / (double)k : -1.0;
+}

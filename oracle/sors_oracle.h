@@ -114,6 +114,9 @@ int      orc_step(orc_sys* s, double* q, double dt, int substeps, const orc_opts
 void     orc_update_state(orc_sys* s, const double* q, double dt);
 /* one minimisation without the state update (minimize_newton[_linesearch]) */
 int      orc_minimize(orc_sys* s, double* x, double dt, const orc_opts* o, orc_stats* st);
+/* CPU-baseline sample: seconds per FP64 PCG iteration on H(q) (includes the
+ * block-Jacobi apply and both dot products; assembly not timed). */
+double   orc_bench_pcg(orc_sys* s, const double* q, double dt, int iters, int project);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,263 @@
+// py_sors.cpp -- pybind11 module with the reference's Python surface
+// (cpp/py_sors.cpp:25-261) over the device path: Energy(...), EnergyTet/Hex,
+// Solver(settings), SolverTet/Hex.step, Params, SimulationSettings and the
+// structured-mesh IO statics.  numpy arrays are copied in and out like the
+// reference's Eigen conversions; step() releases the GIL (an addition).
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "sors_facade.h"
+
+namespace py = pybind11;
+using namespace sors_host;
+
+namespace {
+
+Mat to_mat(const py::handle& obj) {
+    if (py::isinstance<py::float_>(obj) || py::isinstance<py::int_>(obj)) return Mat(1, 1, obj.cast<double>());
+    auto a = py::array_t<double, py::array::forcecast>::ensure(obj);
+    if (!a) throw std::invalid_argument("expected a number or an array of numbers");
+    if (a.ndim() == 0) return Mat(1, 1, *a.data());
+    if (a.ndim() == 1) {
+        Mat m((long)a.shape(0), 1);
+        auto r = a.unchecked<1>();
+        for (long i = 0; i < m.rows; ++i) m(i, 0) = r(i);
+        return m;
+    }
+    if (a.ndim() != 2) throw std::invalid_argument("expected a 1-D or 2-D array");
+    Mat m((long)a.shape(0), (long)a.shape(1));
+    auto r = a.unchecked<2>();
+    for (long i = 0; i < m.rows; ++i)
+        for (long j = 0; j < m.cols; ++j) m(i, j) = r(i, j);
+    return m;
+}
+
+py::array_t<double> from_mat(const Mat& m) {
+    py::array_t<double> out({(py::ssize_t)m.rows, (py::ssize_t)m.cols});
+    auto w = out.mutable_unchecked<2>();
+    for (long i = 0; i < m.rows; ++i)
+        for (long j = 0; j < m.cols; ++j) w(i, j) = m(i, j);
+    return out;
+}
+
+std::vector<double> to_vec(const py::handle& obj) {
+    auto a = py::array_t<double, py::array::forcecast>::ensure(obj);
+    if (!a) throw std::invalid_argument("expected an array of numbers");
+    std::vector<double> v((size_t)a.size());
+    const double* p = a.data();
+    // element order of a C-contiguous copy (row-major flattening)
+    auto c = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(obj);
+    p = c.data();
+    for (size_t i = 0; i < v.size(); ++i) v[i] = p[i];
+    return v;
+}
+
+py::array_t<double> from_vec(const std::vector<double>& v) {
+    py::array_t<double> out((py::ssize_t)v.size());
+    std::copy(v.begin(), v.end(), out.mutable_data());
+    return out;
+}
+
+std::vector<int> conn_of(const py::handle& obj, int E) {
+    auto a = py::array_t<int, py::array::c_style | py::array::forcecast>::ensure(obj);
+    if (!a || a.ndim() != 2 || a.shape(1) != E)
+        throw std::invalid_argument("eleIdx must be a 2-D integer array with " + std::to_string(E) + " columns");
+    return std::vector<int>(a.data(), a.data() + a.size());
+}
+
+template <int E>
+Energy<E>* make_energy(const py::handle& V, const py::handle& eleIdx, const SimulationSettings& settings,
+                       const py::handle& grav, double alpha, const std::vector<std::set<std::string>>& ee,
+                       const std::vector<Params>& ep, const std::vector<std::string>& ct, const Params& cp,
+                       const std::vector<std::string>& ft, const Params& fp) {
+    Mat X = to_mat(V);
+    auto g = to_vec(grav);
+    if (g.size() != 3) throw std::invalid_argument("gravAcceleration must have 3 entries");
+    double gv[3] = {g[0], g[1], g[2]};
+    return new Energy<E>(X, conn_of(eleIdx, E), settings, gv, alpha, ee, ep, ct, cp, ft, fp);
+}
+
+template <int E>
+void bind_energy(py::module_& m, const char* name) {
+    py::class_<Energy<E>>(m, name)
+        .def(py::init(&make_energy<E>), py::arg("undeformedVertices"), py::arg("eleIdx"), py::arg("settings"),
+             py::arg("gravAcceleration"), py::arg("dampingAlpha") = 0.0, py::arg("elementEnergiesList"),
+             py::arg("elementParameterList"), py::arg("constraintTypesList"), py::arg("constraintParameterList"),
+             py::arg("forceTypesList"), py::arg("forceParameterList"))
+        .def("set_initial_velocity", [](Energy<E>& e, const py::handle& v) { e.set_initial_velocity(to_mat(v)); })
+        .def("set_initial_deformation",
+             [](Energy<E>& e, const py::handle& v) { e.set_initial_deformation(to_mat(v)); })
+        .def("get_vPrev", [](const Energy<E>& e) { return from_vec(e.get_vPrev()); })
+        .def("compute_energy",
+             [](Energy<E>& e, const py::handle& q, double dt, const Params& act) {
+                 return e.compute_energy(to_vec(q), dt, act);
+             },
+             py::arg("x"), py::arg("dt"), py::arg("actuation") = Params())
+        .def("compute_gradient",
+             [](Energy<E>& e, const py::handle& q, double dt, const Params& act) {
+                 return from_vec(e.compute_gradient(to_vec(q), dt, act));
+             },
+             py::arg("x"), py::arg("dt"), py::arg("actuation") = Params());
+}
+
+template <int E>
+void bind_solver(py::module_& m, const char* name) {
+    py::class_<Solver<E>>(m, name)
+        .def(
+            "step",
+            [](const Solver<E>& s, const py::handle& sol, Energy<E>& en, const Params& act, double dt, int sub) {
+                auto q = to_vec(sol);
+                std::vector<double> out;
+                {
+                    py::gil_scoped_release nogil;
+                    out = s.step(q, en, act, dt, sub);
+                }
+                return from_vec(out);
+            },
+            "Step solution forward in time.", py::arg("solution"), py::arg("systemEnergy"), py::arg("act"),
+            py::arg("dt") = 0.0, py::arg("substeps") = 0)
+        .def(
+            "step",
+            [](const Solver<E>& s, const py::handle& sol, Energy<E>& en, double dt, int sub) {
+                auto q = to_vec(sol);
+                std::vector<double> out;
+                {
+                    py::gil_scoped_release nogil;
+                    out = s.step(q, en, dt, sub);
+                }
+                return from_vec(out);
+            },
+            "Step solution forward in time with default actuation.", py::arg("solution"), py::arg("systemEnergy"),
+            py::arg("dt") = 0.0, py::arg("substeps") = 0)
+        .def_property_readonly("last_stats", [](const Solver<E>& s) {
+            const sors_stats& t = s.last_stats;
+            py::dict d;
+            d["newton_iters"] = t.newton_iters;
+            d["pcg_iters"] = t.pcg_iters;
+            d["backtracks"] = t.backtracks;
+            d["gnorm"] = t.gnorm;
+            d["dxnorm"] = t.dxnorm;
+            d["ms_total"] = t.ms_total;
+            return d;
+        });
+}
+
+// IO::make_voxel_hex_grid (cpp/src/io.cpp:931-987): vid = i + NX*(j + NY*k)
+py::tuple make_voxel_hex_grid(int nx, int ny, int nz, double dx, double dy, double dz) {
+    if (nx < 1 || ny < 1 || nz < 1) throw std::runtime_error("nx, ny, nz must be >= 1");
+    if (dx <= 0.0 || dy <= 0.0 || dz <= 0.0) throw std::runtime_error("dx, dy, dz must be > 0");
+    const int NX = nx + 1, NY = ny + 1, NZ = nz + 1;
+    py::array_t<double> V({(py::ssize_t)NX * NY * NZ, (py::ssize_t)3});
+    auto v = V.mutable_unchecked<2>();
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const int id = i + NX * (j + NY * k);
+                v(id, 0) = i * dx;
+                v(id, 1) = j * dy;
+                v(id, 2) = k * dz;
+            }
+    py::array_t<int> H({(py::ssize_t)nx * ny * nz, (py::ssize_t)8});
+    auto h = H.mutable_unchecked<2>();
+    int e = 0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i, ++e) {
+                auto id = [&](int a, int b, int c) { return a + NX * (b + NY * c); };
+                const int c8[8] = {id(i, j, k),         id(i + 1, j, k),         id(i + 1, j + 1, k),
+                                   id(i, j + 1, k),     id(i, j, k + 1),         id(i + 1, j, k + 1),
+                                   id(i + 1, j + 1, k + 1), id(i, j + 1, k + 1)};
+                for (int t = 0; t < 8; ++t) h(e, t) = c8[t];
+            }
+    return py::make_tuple(V, H);
+}
+
+// IO::hex_to_5_tets (cpp/src/io.cpp:899-928)
+py::array_t<int> hex_to_5_tets(const py::handle& hexes) {
+    auto a = py::array_t<int, py::array::c_style | py::array::forcecast>::ensure(hexes);
+    if (!a || a.ndim() != 2 || a.shape(1) != 8) throw std::invalid_argument("hexahedra must have 8 vertices");
+    static const int pat[5][4] = {{0, 1, 3, 4}, {1, 2, 3, 6}, {1, 4, 5, 6}, {3, 4, 6, 7}, {1, 3, 4, 6}};
+    const py::ssize_t nh = a.shape(0);
+    py::array_t<int> T({nh * 5, (py::ssize_t)4});
+    auto t = T.mutable_unchecked<2>();
+    auto r = a.unchecked<2>();
+    for (py::ssize_t j = 0; j < nh; ++j)
+        for (int s = 0; s < 5; ++s)
+            for (int c = 0; c < 4; ++c) t(5 * j + s, c) = r(j, pat[s][c]);
+    return T;
+}
+
+struct IO {};
+
+}  // namespace
+
+PYBIND11_MODULE(py_sors, m) {
+    m.doc() = "SORS implicit Newton step on MI355X (HIP device path behind the reference's interface)";
+
+    py::class_<Params>(m, "Params")
+        .def(py::init<>())
+        .def(py::init([](const std::vector<std::string>& keys, const py::list& values) {
+            if (keys.size() != values.size()) throw std::invalid_argument("keys and values differ in length");
+            Params p;
+            for (size_t i = 0; i < keys.size(); ++i) p.set_param(keys[i], to_mat(values[i]));
+            return p;
+        }))
+        .def("get_value", [](const Params& p, const std::string& k) { return from_mat(p.get_value(k)); })
+        .def("set_param", [](Params& p, const std::string& k, const py::object& v) { p.set_param(k, to_mat(v)); })
+        .def("set_param", [](Params& p, const std::string& filename) { p.set_param(filename); });
+
+    py::class_<SimulationSettings>(m, "SimulationSettings")
+        .def(py::init<>())
+        .def_readwrite("description", &SimulationSettings::description)
+        .def_readwrite("verbose", &SimulationSettings::verbose)
+        .def_readwrite("meshFilePath", &SimulationSettings::meshFilePath)
+        .def_readwrite("outputFolder", &SimulationSettings::outputFolder)
+        .def_readwrite("meshFileName", &SimulationSettings::meshFileName)
+        .def_readwrite("meshType", &SimulationSettings::meshType)
+        .def_readwrite("dt", &SimulationSettings::dt)
+        .def_readwrite("numSteps", &SimulationSettings::numSteps)
+        .def_readwrite("substeps", &SimulationSettings::substeps)
+        .def_readwrite("CFLtimeSteppingFlag", &SimulationSettings::CFLtimeSteppingFlag)
+        .def_readwrite("timeSteppingScheme", &SimulationSettings::timeSteppingScheme)
+        .def_readwrite("solverMethod", &SimulationSettings::solverMethod)
+        .def_readwrite("numThreads", &SimulationSettings::numThreads)
+        .def_readwrite("precision", &SimulationSettings::precision)
+        .def_readwrite("pcgTolerance", &SimulationSettings::pcgTolerance)
+        .def("verbose_print", &SimulationSettings::verbose_print);
+
+    bind_energy<4>(m, "EnergyTet");
+    bind_energy<8>(m, "EnergyHex");
+    bind_solver<4>(m, "SolverTet");
+    bind_solver<8>(m, "SolverHex");
+
+    // py_sors.cpp:30-62: dispatch on settings.meshType
+    m.def(
+        "Energy",
+        [](const py::handle& V, const py::handle& eleIdx, const SimulationSettings& settings, const py::handle& grav,
+           double alpha, const std::vector<std::set<std::string>>& ee, const std::vector<Params>& ep,
+           const std::vector<std::string>& ct, const Params& cp, const std::vector<std::string>& ft,
+           const Params& fp) -> py::object {
+            if (settings.meshType == "Tetrahedron")
+                return py::cast(make_energy<4>(V, eleIdx, settings, grav, alpha, ee, ep, ct, cp, ft, fp),
+                                py::return_value_policy::take_ownership);
+            if (settings.meshType == "Hexahedron")
+                return py::cast(make_energy<8>(V, eleIdx, settings, grav, alpha, ee, ep, ct, cp, ft, fp),
+                                py::return_value_policy::take_ownership);
+            throw std::runtime_error("Unknown mesh type: " + settings.meshType);
+        });
+    // py_sors.cpp:99-111
+    m.def("Solver", [](const SimulationSettings& settings) -> py::object {
+        if (settings.meshType == "Tetrahedron")
+            return py::cast(new Solver<4>(settings), py::return_value_policy::take_ownership);
+        if (settings.meshType == "Hexahedron")
+            return py::cast(new Solver<8>(settings), py::return_value_policy::take_ownership);
+        throw std::runtime_error("Unknown mesh type: " + settings.meshType);
+    });
+
+    py::class_<IO>(m, "IO")
+        .def_static("make_voxel_hex_grid", &make_voxel_hex_grid, py::arg("nx"), py::arg("ny"), py::arg("nz"),
+                    py::arg("dx"), py::arg("dy"), py::arg("dz"))
+        .def_static("hex_to_5_tets", &hex_to_5_tets, "Converts hexahedral elements to 5 tetrahedra.",
+                    py::arg("hexElements"));
+}

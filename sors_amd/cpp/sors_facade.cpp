@@ -1,0 +1,286 @@
+// sors_facade.cpp -- host interfaces of the reference over the device C ABI.
+#include "sors_facade.h"
+
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+
+namespace sors_host {
+
+// ---------------------------------------------------------------- Params
+Params::Params(const std::vector<std::string>& keys, const std::vector<Mat>& values) {
+    if (keys.size() != values.size()) throw std::invalid_argument("Params: keys and values differ in length");
+    for (size_t i = 0; i < keys.size(); ++i) set_param(keys[i], values[i]);
+}
+
+Params::Params(const std::vector<std::string>& keys, const std::vector<double>& values) {
+    if (keys.size() != values.size()) throw std::invalid_argument("Params: keys and values differ in length");
+    for (size_t i = 0; i < keys.size(); ++i) set_param(keys[i], values[i]);
+}
+
+Mat Params::get_value(const std::string& key) const {
+    auto it = dict_.find(key);
+    if (it == dict_.end()) throw std::out_of_range("Key not found in the parameter list: " + key);
+    return it->second;
+}
+
+// params.h:43-49: an existing key keeps its size
+void Params::set_param(const std::string& key, const Mat& value) {
+    auto it = dict_.find(key);
+    if (it != dict_.end() && it->second.size() != value.size())
+        throw std::invalid_argument("Params: new value for '" + key + "' changes its size");
+    dict_[key] = value;
+}
+
+void Params::set_param(const std::string& key, const std::vector<double>& values) {
+    Mat m((long)values.size(), 1);
+    for (size_t i = 0; i < values.size(); ++i) m.data[i] = values[i];
+    set_param(key, m);
+}
+
+void Params::set_param(const std::string& key, double value) {
+    Mat m(1, 1, value);
+    set_param(key, m);
+}
+
+// params.h:118-174: CSV lines "key,v1[,v2...]"
+void Params::set_param(const std::string& filename) {
+    if (filename.size() < 4 || filename.substr(filename.find_last_of('.') + 1) != "csv")
+        throw std::invalid_argument("Could not load this type of file: " + filename);
+    std::ifstream f(filename);
+    if (!f.is_open()) throw std::invalid_argument("Could not open the file: " + filename);
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.empty()) continue;
+        std::vector<std::string> cells;
+        std::stringstream ss(line);
+        std::string cell;
+        while (std::getline(ss, cell, ',')) cells.push_back(cell);
+        if (cells.size() < 2) throw std::invalid_argument("Params CSV line needs a key and a value: " + line);
+        if (cells.size() > 2) {
+            std::vector<double> v;
+            for (size_t i = 1; i < cells.size(); ++i) v.push_back(std::stod(cells[i]));
+            set_param(cells[0], v);
+        } else {
+            set_param(cells[0], std::stod(cells[1]));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- settings
+void SimulationSettings::verbose_print() const {
+    auto na = [](const std::string& s) { return s.empty() ? std::string("not indicated") : s; };
+    std::cout << "\n============================ Simulation Description ============================\n"
+              << na(description) << "\n"
+              << "\n============================== Simulation Settings =============================\n";
+    std::cout << std::left << std::setw(25) << "Mesh File Path:" << na(meshFilePath) << "\n"
+              << std::setw(25) << "Output Folder:" << na(outputFolder) << "\n"
+              << std::setw(25) << "Mesh File Name:" << na(meshFileName) << "\n"
+              << std::setw(25) << "Mesh Type" << na(meshType) << "\n"
+              << std::setw(25) << "Time Step (dt):";
+    if (std::isnan(dt))
+        std::cout << "not indicated\n";
+    else
+        std::cout << dt << "\n";
+    std::cout << std::setw(25) << "Number of Steps:" << (numSteps < 0 ? "not indicated" : std::to_string(numSteps))
+              << "\n"
+              << std::setw(25) << "Number of Substeps:" << substeps << "\n"
+              << std::setw(25) << "CFL Timestepping:" << (CFLtimeSteppingFlag ? "On" : "Off") << "\n"
+              << std::setw(25) << "Time Scheme:" << na(timeSteppingScheme) << "\n"
+              << std::setw(25) << "Solver Method:" << na(solverMethod) << "\n"
+              << std::setw(25) << "Precision:" << precision << "\n"
+              << "================================================================================\n";
+}
+
+// ---------------------------------------------------------------- errors
+void check_status(int code, const sors_ctx* ctx) {
+    if (code == SORS_OK) return;
+    std::string detail = sors_last_error(ctx);
+    switch (code) {
+        case SORS_ERR_INVERTED: throw std::runtime_error("Inverted Elements");
+        case SORS_ERR_NOCONV: throw std::runtime_error("Newton's method did not converge");
+        case SORS_ERR_NAN: throw std::runtime_error("Newton's method did not converge (" + detail + ")");
+        case SORS_ERR_LINESEARCH: throw std::runtime_error("Newton's method did not converge (" + detail + ")");
+        case SORS_ERR_UNSUPPORTED: throw std::runtime_error("Solver method not recognized (" + detail + ")");
+        default: throw std::runtime_error(detail.empty() ? "sors device error" : detail);
+    }
+}
+
+// ---------------------------------------------------------------- Energy
+template <int E>
+Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSettings& settings,
+                  const double grav[3], double dampingAlpha,
+                  const std::vector<std::set<std::string>>& elementEnergiesList,
+                  const std::vector<Params>& elementParameterList, const std::vector<std::string>& constraintTypesList,
+                  const Params& constraintParameterList, const std::vector<std::string>& forceTypesList,
+                  const Params& /*forceParameterList*/)
+    : settings_(settings), forceTypesList_(forceTypesList) {
+    if (X.cols != 3) throw std::invalid_argument("undeformedVertices must be N x 3");
+    numVertices_ = (int)X.rows;
+    const int nel = (int)(conn.size() / E);
+    if ((long)conn.size() != (long)nel * E) throw std::invalid_argument("eleIdx size is not a multiple of E");
+    if ((int)elementEnergiesList.size() != nel || (int)elementParameterList.size() != nel)
+        throw std::invalid_argument("one energy set and one Params per element are required");
+    std::vector<double> rest = X.row_major();
+    std::vector<double> Y(nel), P(nel), D(nel);
+    int material = SORS_NEOHOOKEAN;
+    for (int e = 0; e < nel; ++e) {
+        const auto& set = elementEnergiesList[e];
+        for (const auto& name : set)
+            if (name != "neohookean")
+                throw std::runtime_error("Unknown energy type: " + name +
+                                         " (the device path implements \"neohookean\")");
+        if (set.empty()) throw std::runtime_error("element " + std::to_string(e) + " has no energy");
+        const Params& p = elementParameterList[e];
+        D[e] = p.get_value("density")(0, 0);
+        Y[e] = p.get_value("youngsModulus")(0, 0);
+        P[e] = p.get_value("poissonsRatio")(0, 0);
+    }
+    std::vector<int> mask;
+    std::vector<double> bcv;
+    for (const auto& c : constraintTypesList) {
+        if (c == "neumannBC") {
+            // energy.cpp:61-68
+            Mat m = constraintParameterList.get_value("neumannBCmask");
+            Mat v = constraintParameterList.get_value("neumannBCvalue");
+            auto mr = m.row_major(), vr = v.row_major();
+            if ((int)mr.size() != 3 * numVertices_ || (int)vr.size() != 3 * numVertices_)
+                throw std::invalid_argument("neumannBCmask / neumannBCvalue must have 3N entries");
+            mask.resize(mr.size());
+            for (size_t i = 0; i < mr.size(); ++i) mask[i] = (int)mr[i];
+            bcv = vr;
+        } else {
+            throw std::runtime_error("Unknown constraint type: " + c + " (the device path implements neumannBC)");
+        }
+    }
+    for (const auto& f : forceTypesList)
+        if (f != "vertexForce")
+            throw std::runtime_error("Unknown force type: " + f + " (the device path implements vertexForce)");
+    sors_system_desc d{};
+    d.n_nodes = numVertices_;
+    d.n_elems = nel;
+    d.nodes_per_elem = E;
+    d.rest = rest.data();
+    d.conn = conn.data();
+    d.youngs = Y.data();
+    d.poisson = P.data();
+    d.density = D.data();
+    d.material = material;
+    d.bc_mask = mask.empty() ? nullptr : mask.data();
+    d.bc_value = bcv.empty() ? nullptr : bcv.data();
+    for (int k = 0; k < 3; ++k) d.grav[k] = grav[k];
+    d.damping_alpha = dampingAlpha;
+    if (settings.timeSteppingScheme == "backward_euler")
+        d.scheme = SORS_BACKWARD_EULER;
+    else if (settings.timeSteppingScheme == "crank_nicolson")
+        d.scheme = SORS_CRANK_NICOLSON;
+    else
+        throw std::runtime_error("Unknown time stepping scheme: " + settings.timeSteppingScheme);
+    if (settings.solverMethod == "minimize_newton")
+        d.method = SORS_MINIMIZE_NEWTON;
+    else if (settings.solverMethod == "minimize_newton_linesearch")
+        d.method = SORS_MINIMIZE_NEWTON_LINESEARCH;
+    else if (settings.solverMethod == "minimize_SQP")
+        d.method = SORS_MINIMIZE_SQP;
+    else
+        throw std::runtime_error("Solver method not recognized");
+    check_status(sors_create(&d, &ctx_), nullptr);
+    sors_options o;
+    sors_default_options(&o);
+    o.precision = settings.precision == "fp64" ? SORS_FP64 : SORS_MIXED;
+    o.pcg_rtol = settings.pcgTolerance;
+    check_status(sors_set_options(ctx_, &o), ctx_);
+}
+
+template <int E>
+Energy<E>::~Energy() {
+    sors_destroy(ctx_);
+}
+
+template <int E>
+void Energy<E>::set_initial_velocity(const Mat& v) {
+    if (v.rows != numVertices_ || v.cols != 3) throw std::invalid_argument("velocities must be N x 3");
+    auto r = v.row_major();
+    check_status(sors_set_initial_velocity(ctx_, r.data()), ctx_);
+}
+
+template <int E>
+void Energy<E>::set_initial_deformation(const Mat& x) {
+    if (x.rows != numVertices_ || x.cols != 3) throw std::invalid_argument("vertices must be N x 3");
+    auto r = x.row_major();
+    check_status(sors_set_initial_deformation(ctx_, r.data()), ctx_);
+}
+
+template <int E>
+std::vector<double> Energy<E>::get_vPrev() const {
+    std::vector<double> v(3 * (size_t)numVertices_);
+    check_status(sors_get_state(ctx_, nullptr, v.data()), ctx_);
+    return v;
+}
+
+// VertexForce actuation: Params "vertexForce" (N x 3) -> 3N node-major
+// (energy.cpp:282-293, vertexForce.cpp:4-8)
+template <int E>
+void Energy<E>::apply_actuation(const Params& act) {
+    bool vf = false;
+    for (const auto& f : forceTypesList_) vf |= (f == "vertexForce");
+    if (!vf) return;
+    if (!act.has("vertexForce")) {
+        check_status(sors_set_external_force(ctx_, nullptr), ctx_);
+        return;
+    }
+    auto f = act.get_value("vertexForce").row_major();
+    if ((int)f.size() != 3 * numVertices_) throw std::invalid_argument("vertexForce actuation must be N x 3");
+    check_status(sors_set_external_force(ctx_, f.data()), ctx_);
+}
+
+template <int E>
+double Energy<E>::compute_energy(const std::vector<double>& q, double dt, const Params& actuation) {
+    apply_actuation(actuation);
+    double e = 0;
+    check_status(sors_energy(ctx_, q.data(), dt, &e), ctx_);
+    return e;
+}
+
+template <int E>
+std::vector<double> Energy<E>::compute_gradient(const std::vector<double>& q, double dt, const Params& actuation) {
+    apply_actuation(actuation);
+    std::vector<double> g(q.size());
+    check_status(sors_gradient(ctx_, q.data(), dt, g.data()), ctx_);
+    return g;
+}
+
+// ---------------------------------------------------------------- Solver
+// Solver::step (solver.cpp:5-44)
+template <int E>
+std::vector<double> Solver<E>::step(const std::vector<double>& solution, Energy<E>& en, const Params& act, double dt,
+                                    int substeps) const {
+    if (dt == 0.0) dt = en.settings_.dt;
+    if (substeps == 0) substeps = en.settings_.substeps;
+    if (en.settings_.CFLtimeSteppingFlag)
+        throw std::runtime_error("CFL substepping is not implemented on the device path");
+    if (!(dt > 0.0)) throw std::invalid_argument("dt must be positive (settings.dt is not set)");
+    if (en.settings_.solverMethod != "minimize_newton" && en.settings_.solverMethod != "minimize_newton_linesearch")
+        throw std::runtime_error("Solver method not recognized");
+    if ((int)solution.size() != 3 * en.num_vertices()) throw std::invalid_argument("solution must have 3N entries");
+    en.apply_actuation(act);
+    std::vector<double> q = solution;
+    sors_stats st{};
+    check_status(sors_step(en.ctx(), q.data(), dt, substeps, &st), en.ctx());
+    last_stats = st;
+    return q;
+}
+
+template <int E>
+std::vector<double> Solver<E>::step(const std::vector<double>& solution, Energy<E>& en, double dt,
+                                    int substeps) const {
+    Params act;
+    return step(solution, en, act, dt, substeps);
+}
+
+template class Energy<4>;
+template class Energy<8>;
+template class Solver<4>;
+template class Solver<8>;
+
+}  // namespace sors_host

@@ -1,0 +1,130 @@
+// sors_facade.h -- the reference's host interfaces (Params, SimulationSettings,
+// Energy<3,E>, Solver<3,E>) over the device C ABI in include/sors_amd.h.
+//
+// Same names, argument meaning and error messages as
+//   cpp/headers/params.h:30-175, cpp/headers/simulationSettings.h:18-38,
+//   cpp/headers/energy.h:73-205, cpp/headers/solver.h:27-95,
+// with a small dense column-major matrix type standing in for Eigen::MatrixXd
+// (Eigen is not a dependency).  Energies, constraints and forces outside the
+// device path raise std::runtime_error instead of the reference's exit(1).
+#pragma once
+
+#include <cmath>
+#include <iostream>
+#include <limits>
+#include <map>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sors_amd.h"
+
+namespace sors_host {
+
+// column-major dense matrix (the role of Eigen::MatrixXd in the reference)
+struct Mat {
+    long rows = 0, cols = 0;
+    std::vector<double> data;
+    Mat() = default;
+    Mat(long r, long c, double v = 0.0) : rows(r), cols(c), data((size_t)(r * c), v) {}
+    double& operator()(long r, long c) { return data[(size_t)(c * rows + r)]; }
+    double operator()(long r, long c) const { return data[(size_t)(c * rows + r)]; }
+    long size() const { return rows * cols; }
+    // row-major flattening, the reference's .reshaped<RowMajor>() / .transpose().reshaped()
+    std::vector<double> row_major() const {
+        std::vector<double> out((size_t)size());
+        for (long r = 0; r < rows; ++r)
+            for (long c = 0; c < cols; ++c) out[(size_t)(r * cols + c)] = (*this)(r, c);
+        return out;
+    }
+};
+
+// Key-value parameter store (cpp/headers/params.h:30-175)
+class Params {
+public:
+    Params() = default;
+    Params(const std::vector<std::string>& keys, const std::vector<Mat>& values);
+    Params(const std::vector<std::string>& keys, const std::vector<double>& values);
+    Mat get_value(const std::string& key) const;
+    bool has(const std::string& key) const { return dict_.count(key) != 0; }
+    void set_param(const std::string& key, const Mat& value);
+    void set_param(const std::string& key, const std::vector<double>& values);
+    void set_param(const std::string& key, double value);
+    void set_param(const std::string& filename);  // CSV: key,value[,value...]
+    const std::map<std::string, Mat>& dict() const { return dict_; }
+
+private:
+    std::map<std::string, Mat> dict_;
+};
+
+// Run settings (cpp/headers/simulationSettings.h:18-38), same defaults
+struct SimulationSettings {
+    std::string description;
+    int verbose = 1;
+    std::string meshFilePath;
+    std::string outputFolder;
+    std::string meshFileName;
+    std::string meshType;
+    double dt = std::numeric_limits<double>::quiet_NaN();
+    int numSteps = -1;
+    int substeps = 1;
+    bool CFLtimeSteppingFlag = false;
+    std::string timeSteppingScheme = "crank_nicolson";
+    std::string solverMethod = "minimize_newton";
+    int numThreads = 1;
+    // device-path additions (not in the reference)
+    std::string precision = "mixed";  // "mixed" | "fp64"
+    double pcgTolerance = 1e-6;
+    void verbose_print() const;
+};
+
+// Energy<3,E> over a device context (cpp/headers/energy.h:73-205)
+template <int E>
+class Energy {
+public:
+    Energy(const Mat& undeformedVertices, const std::vector<int>& eleIdxRowMajor, const SimulationSettings& settings,
+           const double gravAcceleration[3], double dampingAlpha,
+           const std::vector<std::set<std::string>>& elementEnergiesList,
+           const std::vector<Params>& elementParameterList, const std::vector<std::string>& constraintTypesList,
+           const Params& constraintParameterList, const std::vector<std::string>& forceTypesList,
+           const Params& forceParameterList);
+    ~Energy();
+    Energy(const Energy&) = delete;
+    Energy& operator=(const Energy&) = delete;
+
+    void set_initial_velocity(const Mat& velocities);      // N x 3
+    void set_initial_deformation(const Mat& vertices);     // N x 3
+    std::vector<double> get_vPrev() const;
+    double compute_energy(const std::vector<double>& q, double dt, const Params& actuation);
+    std::vector<double> compute_gradient(const std::vector<double>& q, double dt, const Params& actuation);
+    void apply_actuation(const Params& actuation);
+
+    sors_ctx* ctx() const { return ctx_; }
+    int num_vertices() const { return numVertices_; }
+    SimulationSettings settings_;
+    std::vector<std::string> forceTypesList_;
+
+private:
+    sors_ctx* ctx_ = nullptr;
+    int numVertices_ = 0;
+};
+
+// Solver<3,E> (cpp/headers/solver.h:27-95)
+template <int E>
+class Solver {
+public:
+    Solver() = default;
+    explicit Solver(const SimulationSettings& settings) : settings_(settings) {}
+    SimulationSettings settings_;
+    std::vector<double> step(const std::vector<double>& solution, Energy<E>& systemEnergy, const Params& act,
+                             double dt = 0.0, int substeps = 0) const;
+    std::vector<double> step(const std::vector<double>& solution, Energy<E>& systemEnergy, double dt = 0.0,
+                             int substeps = 0) const;
+    mutable sors_stats last_stats{};
+};
+
+// throws std::runtime_error carrying the reference's message for a status code
+void check_status(int code, const sors_ctx* ctx);
+
+}  // namespace sors_host

@@ -63,6 +63,7 @@ struct ElemArgs {
     const int* rowptr;
     const uint8_t* rank;
     const int* fullmap;
+    const int* sbase;     // sliced node-block layout of the values (pattern.h) or null for CSR
     int* flag;            // bit 0: inverted element
     double* ex_energy;    // export (test mode) or null
     double* ex_grad;

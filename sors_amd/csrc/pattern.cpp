@@ -160,6 +160,64 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
                     }
                 }
         });
+        // sliced node-block layout
+        const int ns = (N + kSlice - 1) / kSlice;
+        P.nslices = ns;
+        P.sbase.assign(ns + 1, 0);
+        P.self_slot.assign(N, 0);
+        std::vector<int> deg(N, 1);
+        long long nblocks = 0;
+        for (int v = 0; v < N; ++v) {
+            if (masked(3 * v)) {
+                deg[v] = 1;
+                ++nblocks;
+                continue;
+            }
+            int k = 0, self = -1;
+            for (int u : nbr[v]) {
+                if (masked(3 * u)) continue;
+                if (u == v) self = k;
+                ++k;
+            }
+            if (k == 0) {
+                // isolated node: only its (forced) diagonal can exist
+                deg[v] = 1;
+                self = 0;
+                nblocks += forced(3 * v) ? 1 : 0;
+            } else {
+                deg[v] = k;
+                nblocks += k;
+            }
+            P.self_slot[v] = self < 0 ? 0 : self;
+        }
+        P.nblocks = nblocks;
+        for (int sl = 0; sl < ns; ++sl) {
+            int w = 1;
+            for (int v = sl * kSlice; v < std::min(N, (sl + 1) * kSlice); ++v) w = std::max(w, deg[v]);
+            P.sbase[sl + 1] = P.sbase[sl] + w * kSlice;
+        }
+        if (9LL * P.sbase[ns] > 0x7fffffffLL) {
+            if (msg) *msg = "block layout exceeds 2^31 values";
+            return false;
+        }
+        P.bcol.assign(P.sbase[ns], 0);
+        parallel_for(ns, [&](long long lo, long long hi) {
+            for (long long sl = lo; sl < hi; ++sl) {
+                const int w = (P.sbase[sl + 1] - P.sbase[sl]) / kSlice;
+                for (int lane = 0; lane < kSlice; ++lane) {
+                    const int v = (int)sl * kSlice + lane;
+                    int* col = P.bcol.data() + P.sbase[sl] + lane;
+                    int k = 0;
+                    if (v < N && !masked(3 * v))
+                        for (int u : nbr[v])
+                            if (!masked(3 * u)) col[kSlice * k++] = u;
+                    for (; k < w; ++k) col[kSlice * k] = (v < N) ? v : 0;  // padding: own node, zero values
+                }
+            }
+        });
+        P.bdiag.assign(n, -1);
+        for (int v = 0; v < N; ++v)
+            for (int c = 0; c < 3; ++c) P.bdiag[3 * v + c] = (int)block_index(P.sbase.data(), v, P.self_slot[v], c, c);
     } else {
         int nd = 3 * ed;
         P.fullmap.resize((size_t)E * nd * nd);

@@ -45,6 +45,22 @@ SORS_HD long long rank_index(const int* rowptr, int na, int c, int rank, int d) 
     return base + c * len + 3 * rank + d;
 }
 
+// Sliced node-block layout of the same matrix (used for the device values
+// whenever the map is node-granular).  Nodes are grouped in slices of 64
+// (one wavefront, one lane per node = three matrix rows).  Node v lists its
+// kept neighbour nodes in ascending order -- slot k of v is the node of rank
+// k, so the rank map addresses slots directly -- and a Dirichlet node lists
+// only itself (its rows are the identity).  Slice s holds width_s slots per
+// lane, padded with the lane's own node and zero values:
+//   column of (v, slot k):      bcol[sbase[s] + 64 k + lane]
+//   value (v, slot k, c, d):    vals[9 sbase[s] + 64 (9 k + 3 c + d) + lane]
+// with s = v / 64, lane = v % 64.
+constexpr int kSlice = 64;
+
+SORS_HD long long block_index(const int* sbase, int node, int slot, int c, int d) {
+    return 9LL * sbase[node >> 6] + (long long)kSlice * (9 * slot + 3 * c + d) + (node & (kSlice - 1));
+}
+
 struct HostPattern {
     int n = 0;                      // DOFs
     long long nnz = 0;
@@ -52,9 +68,16 @@ struct HostPattern {
     bool node_granular = true;      // compact rank map in use
     std::vector<int> rowptr;        // n + 1
     std::vector<int> cols;          // nnz
-    std::vector<int> diag;          // n: index of (i,i)
+    std::vector<int> diag;          // n: index of (i,i) in the CSR values
     std::vector<uint8_t> rank;      // [ed*ed][E] when node_granular
     std::vector<int> fullmap;       // [E][(3ed)^2] otherwise
+    // sliced node-block layout (node_granular only)
+    int nslices = 0;
+    std::vector<int> sbase;         // nslices + 1, prefix sums of 64 * width
+    std::vector<int> bcol;          // sbase[nslices]
+    std::vector<int> self_slot;     // N
+    std::vector<int> bdiag;         // n: block-layout index of (i,i)
+    long long nblocks = 0;          // 3x3 node blocks holding at least one CSR entry
 };
 
 // mask: 3N flags (may be null).  sqp: SQP assembly semantics (no masking).

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.h"
@@ -164,7 +165,13 @@ __device__ __forceinline__ double poly_d2psi(double x) { return 12 * x * x - 1; 
 template <typename HT>
 __device__ __forceinline__ void scatter_block(const ElemArgs& a, int e, int la, int lb, int na, const HT blk[9]) {
     HT* vals = (HT*)a.vals;
-    if (a.rank) {
+    if (a.sbase) {
+        const int rk = a.rank[(size_t)(la * a.ed + lb) * a.E + e];
+        if (rk == kRankSkip) return;
+        const long long base = block_index(a.sbase, na, rk, 0, 0);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) atomic_add_ht(vals + base + (long long)kSlice * k, blk[k]);
+    } else if (a.rank) {
         const int rk = a.rank[(size_t)(la * a.ed + lb) * a.E + e];
         if (rk == kRankSkip) return;
         const long long base = a.rowptr[3 * na];
@@ -751,6 +758,55 @@ __global__ __launch_bounds__(kBlock) void k_block_jacobi(int N, const int* rowpt
     for (int i = 0; i < 9; ++i) Minv[(size_t)9 * v + i] = (HT)out[i];
 }
 
+// block-Jacobi from the sliced node-block layout: the node's self block
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_block_jacobi_blk(int N, const int* sbase, const int* self_slot,
+                                                             const HT* vals, HT* Minv) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const long long base = block_index(sbase, v, self_slot[v], 0, 0);
+    M3<double> A;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) A.m[k] = (double)vals[base + (long long)kSlice * k];
+    const M3<double> C = cofactor3(A);
+    const double det = A.m[0] * C.m[0] + A.m[1] * C.m[1] + A.m[2] * C.m[2];
+    double out[9];
+    if (det != 0.0 && isfinite(det)) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) out[3 * i + j] = C(j, i) / det;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out[4 * i] = (A.m[4 * i] != 0.0) ? 1.0 / A.m[4 * i] : 1.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Minv[(size_t)9 * v + i] = (HT)out[i];
+}
+
+// CSR values from the sliced node-block layout (export for inspection).
+// Row R = 3v + c of a kept node lists 3 columns per kept neighbour in slot
+// order; a Dirichlet node's row holds only (R, R).
+template <typename HT>
+__global__ void k_block_to_csr(int n, const int* rowptr, const int* cols, const int* sbase, const int* self_slot,
+                               const int* mask, const HT* bvals, double* out) {
+    const int R = blockIdx.x * blockDim.x + threadIdx.x;
+    if (R >= n) return;
+    const int v = R / 3, c = R % 3;
+    const bool dir = mask && mask[R];
+    for (int p = rowptr[R]; p < rowptr[R + 1]; ++p) {
+        const int C = cols[p];
+        long long idx;
+        if (dir)
+            idx = block_index(sbase, v, self_slot[v], c, c);
+        else
+            idx = block_index(sbase, v, (p - rowptr[R]) / 3, c, C % 3);
+        out[p] = (double)bvals[idx];
+    }
+}
+
 // ===========================================================================
 // PCG (Chronopoulos-Gear form: one SpMV and one fused vector pass per
 // iteration, FP64 dot products, device-side scalars and convergence flag)
@@ -883,6 +939,113 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int N, const HT* Minv, HT
     }
 }
 
+// w = A u on the sliced node-block layout: one lane per node (three rows),
+// one wavefront per slice of 64 nodes; values and column indices are read
+// with unit stride across the wavefront.
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv_blk(int N, int nslices, const int* sbase, const int* bcol,
+                                                         const HT* vals, const HT* u, HT* w, PcgState* st,
+                                                         int first, Reduce red) {
+    if (st->done) return;
+    double v[1] = {0.0};
+    const int lane = threadIdx.x & (kSlice - 1), wv = threadIdx.x / kSlice;
+    constexpr int kPer = kBlock / kSlice;  // slices per block
+    const int ngroups = (nslices + kPer - 1) / kPer;
+    const int per_xcd = (ngroups + kNumXcd - 1) / kNumXcd;
+    for (int t = blockIdx.x; t < kNumXcd * per_xcd; t += gridDim.x) {
+        const int xcd = t % kNumXcd, local = t / kNumXcd;
+        const int g = xcd * per_xcd + local;
+        const int s = g * kPer + wv;
+        if (local >= per_xcd || g >= ngroups || s >= nslices) continue;  // uniform per wavefront
+        const int base = sbase[s];
+        const int width = (sbase[s + 1] - base) / kSlice;
+        const int* cp = bcol + base + lane;
+        const HT* vp = vals + 9LL * base + lane;
+        HT a0 = 0, a1 = 0, a2 = 0;
+        for (int j = 0; j < width; ++j) {
+            const int nb = cp[kSlice * j];
+            const HT x0 = u[3 * nb], x1 = u[3 * nb + 1], x2 = u[3 * nb + 2];
+            const HT* b = vp + (long long)kSlice * 9 * j;
+            a0 += b[0] * x0 + b[kSlice] * x1 + b[2 * kSlice] * x2;
+            a1 += b[3 * kSlice] * x0 + b[4 * kSlice] * x1 + b[5 * kSlice] * x2;
+            a2 += b[6 * kSlice] * x0 + b[7 * kSlice] * x1 + b[8 * kSlice] * x2;
+        }
+        const int node = s * kSlice + lane;
+        if (node < N) {
+            w[3 * node] = a0;
+            w[3 * node + 1] = a1;
+            w[3 * node + 2] = a2;
+            v[0] += (double)a0 * (double)u[3 * node] + (double)a1 * (double)u[3 * node + 1] +
+                    (double)a2 * (double)u[3 * node + 2];
+        }
+    }
+    if (grid_sum<1>(v, red) && threadIdx.x == 0) {
+        const double delta = v[0];
+        st->delta = delta;
+        if (first) {
+            st->alpha = st->gamma / delta;
+            st->beta = 0.0;
+            if (!(delta > 0.0)) {
+                st->breakdown = 1;
+                st->done = 1;
+            }
+        } else {
+            const double beta = st->gamma_new / st->gamma;
+            const double den = delta - beta * st->gamma_new / st->alpha;
+            if (!(den > 0.0) || !(st->gamma_new > 0.0)) {
+                st->breakdown = 1;
+                st->done = 1;
+            } else {
+                st->alpha = st->gamma_new / den;
+                st->beta = beta;
+                st->gamma = st->gamma_new;
+            }
+        }
+    }
+}
+
+// Same update as k_pcg_update with one thread per DOF (unit-stride vector
+// traffic); the three residual components of a node meet in LDS for u = M r.
+constexpr int kUpdBlock = 192;  // 64 nodes
+template <typename HT>
+__global__ __launch_bounds__(kUpdBlock) void k_pcg_update_dof(int n, const HT* Minv, HT* x, HT* r, HT* u, HT* p,
+                                                              HT* s, const HT* w, PcgState* st, Reduce red) {
+    if (st->done) return;
+    const HT alpha = (HT)st->alpha, beta = (HT)st->beta;
+    __shared__ HT rs[kUpdBlock];
+    double v[2] = {0.0, 0.0};
+    const int c = threadIdx.x % 3, first = threadIdx.x - c;
+    for (int base = blockIdx.x * kUpdBlock; base < n; base += gridDim.x * kUpdBlock) {
+        const int i = base + threadIdx.x;
+        HT rr = 0;
+        if (i < n) {
+            const HT pp = u[i] + beta * p[i];
+            const HT ss = w[i] + beta * s[i];
+            p[i] = pp;
+            s[i] = ss;
+            x[i] += alpha * pp;
+            rr = r[i] - alpha * ss;
+            r[i] = rr;
+        }
+        rs[threadIdx.x] = rr;
+        __syncthreads();
+        if (i < n) {
+            const HT* M = Minv + 3 * (size_t)i;  // row c of node i/3: Minv[9 (i/3) + 3 c] = Minv[3 i]
+            const HT uu = M[0] * rs[first] + M[1] * rs[first + 1] + M[2] * rs[first + 2];
+            u[i] = uu;
+            v[0] += (double)rr * (double)uu;
+            v[1] += (double)rr * (double)rr;
+        }
+        __syncthreads();
+    }
+    if (grid_sum<2>(v, red) && threadIdx.x == 0) {
+        st->gamma_new = v[0];
+        st->rr = v[1];
+        st->iter += 1;
+        if (v[1] <= st->tol2 || !(v[1] == v[1])) st->done = 1;
+    }
+}
+
 // ===========================================================================
 // Newton vector kernels
 // ===========================================================================
@@ -921,6 +1084,12 @@ __global__ void k_update_state(int n, int scheme, double dt, const double* q, do
         qprev[i] = q[i];
         gprev[i] = gtmp[i];
     }
+}
+
+// plain copy (calibration of the HBM byte counters)
+__global__ void k_copy(long long n4, const int4* __restrict__ src, int4* __restrict__ dst) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 }  // namespace sors
@@ -996,7 +1165,25 @@ struct sors_ctx {
     HostPattern pat;
     int pat_dtnz = -1;
     bool pat_on_device = false;
-    hipEvent_t ev[8] = {};
+    DevMem pcgst_bench, copybuf;
+    DevMem sbase, bcol, selfslot;   // sliced node-block layout (node-granular masks)
+    long long nvals = 0;            // value slots of the active layout
+    // captured PCG iteration chunk (one per precision)
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    const void* gkey[2] = {nullptr, nullptr};
+    int gchunk[2] = {0, 0};
+    bool block() const { return pat.node_granular; }
+    // per-launch kernel timing (options.profile_kernels)
+    sors_stats* prof = nullptr;
+    std::vector<hipEvent_t> pev;
+    hipEvent_t* events(int count) {
+        while ((int)pev.size() < count) {
+            hipEvent_t e;
+            HIPX(hipEventCreate(&e));
+            pev.push_back(e);
+        }
+        return pev.data();
+    }
 
     unsigned* counter(int k) { return counters.as<unsigned>() + k; }
     Reduce red(int k) { return Reduce{partials.as<double>(), counter(k)}; }
@@ -1037,13 +1224,20 @@ void ensure_pattern(sors_ctx* c, double dt) {
         throw HipError{"pattern: " + msg};
     upload(c->rowptr, c->pat.rowptr.data(), c->pat.rowptr.size(), c->stream);
     upload(c->cols, c->pat.cols.data(), c->pat.cols.size(), c->stream);
-    upload(c->diag, c->pat.diag.data(), c->pat.diag.size(), c->stream);
     if (c->pat.node_granular) {
         upload(c->rank, c->pat.rank.data(), c->pat.rank.size(), c->stream);
+        upload(c->sbase, c->pat.sbase.data(), c->pat.sbase.size(), c->stream);
+        upload(c->bcol, c->pat.bcol.data(), c->pat.bcol.size(), c->stream);
+        upload(c->selfslot, c->pat.self_slot.data(), c->pat.self_slot.size(), c->stream);
+        upload(c->diag, c->pat.bdiag.data(), c->pat.bdiag.size(), c->stream);
+        c->nvals = 9LL * c->pat.sbase[c->pat.nslices];
     } else {
         upload(c->fullmap, c->pat.fullmap.data(), c->pat.fullmap.size(), c->stream);
+        upload(c->diag, c->pat.diag.data(), c->pat.diag.size(), c->stream);
+        c->nvals = c->pat.nnz;
     }
-    ensure(c->vals, sizeof(double) * std::max<long long>(c->pat.nnz, 1));
+    ensure(c->vals, sizeof(double) * std::max<long long>(c->nvals, 1));
+    for (int k = 0; k < 2; ++k) c->gkey[k] = nullptr;  // captured graphs hold the old buffers
     sync(c);  // host vectors of the pattern stay alive in c->pat
     c->pat_dtnz = dtnz;
     c->pat_on_device = true;
@@ -1068,6 +1262,7 @@ ElemArgs elem_args(sors_ctx* c, const double* q) {
     a.rowptr = c->rowptr.as<int>();
     a.rank = c->pat.node_granular ? c->rank.as<uint8_t>() : nullptr;
     a.fullmap = c->pat.node_granular ? nullptr : c->fullmap.as<int>();
+    a.sbase = c->block() ? c->sbase.as<int>() : nullptr;
     a.flag = c->flag.as<int>();
     return a;
 }
@@ -1128,14 +1323,24 @@ template <typename HT>
 void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project) {
     ElemArgs a = elem_args(c, q);
     fill<double>(c->gint.as<double>(), c->n, 0.0, c->stream);
-    if (hessian) fill<HT>((HT*)c->vals.p, c->pat.nnz, (HT)0, c->stream);
+    if (hessian) fill<HT>((HT*)c->vals.p, c->nvals, (HT)0, c->stream);
     a.gint = c->gint.as<double>();
     a.vals = hessian ? c->vals.p : nullptr;
     a.hscale = (c->scheme == SORS_CRANK_NICOLSON && dt != 0.0) ? 0.5 : 1.0;
+    hipEvent_t* ev = c->prof ? c->events(2) : nullptr;
+    if (ev) HIPX(hipEventRecord(ev[0], c->stream));
     if (hessian)
         launch_elem<HT, 0>(c, a, project);
     else
         launch_elem<HT, 1>(c, a, 0);
+    if (ev) {
+        HIPX(hipEventRecord(ev[1], c->stream));
+        HIPX(hipEventSynchronize(ev[1]));
+        float ms = 0;
+        HIPX(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        c->prof->assemble_kernel_ms += ms;
+        c->prof->assemble_launches += 1;
+    }
     NodeArgs na = node_args(c, q, dt);
     na.vals = hessian ? c->vals.p : nullptr;
     const int grid = red_grid(c->n, kBlock);
@@ -1145,49 +1350,132 @@ void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project
 
 template <typename HT>
 void build_preconditioner(sors_ctx* c) {
-    k_block_jacobi<HT><<<grid_for(c->N), kBlock, 0, c->stream>>>(c->N, c->rowptr.as<int>(), c->cols.as<int>(),
-                                                                  c->diag.as<int>(), (const HT*)c->vals.p,
-                                                                  (HT*)c->Minv.p);
+    if (c->block())
+        k_block_jacobi_blk<HT><<<grid_for(c->N), kBlock, 0, c->stream>>>(
+            c->N, c->sbase.as<int>(), c->selfslot.as<int>(), (const HT*)c->vals.p, (HT*)c->Minv.p);
+    else
+        k_block_jacobi<HT><<<grid_for(c->N), kBlock, 0, c->stream>>>(c->N, c->rowptr.as<int>(), c->cols.as<int>(),
+                                                                      c->diag.as<int>(), (const HT*)c->vals.p,
+                                                                      (HT*)c->Minv.p);
     HIPX(hipGetLastError());
 }
 
-// PCG on the assembled matrix: solves A x = sign*b; x in c->px (HT)
+// launch geometry of the PCG kernels
+struct PcgGrid {
+    int gn;     // per-node reductions (init)
+    int gu;     // DOF-parallel update
+    int gs;     // SpMV
+};
+
+PcgGrid pcg_grid(sors_ctx* c) {
+    PcgGrid g;
+    g.gn = red_grid(c->N, kBlock);
+    g.gu = red_grid(c->n, kUpdBlock);
+    if (c->block()) {
+        const int groups = (c->pat.nslices + kBlock / kSlice - 1) / (kBlock / kSlice);
+        g.gs = (std::min(groups, kMaxRedBlocks) + kNumXcd - 1) / kNumXcd * kNumXcd;
+    } else {
+        const int rows_per_block = kBlock / 8;
+        g.gs = (red_grid((long long)(c->n + rows_per_block - 1) / rows_per_block * kBlock, kBlock) + kNumXcd - 1) /
+               kNumXcd * kNumXcd;
+    }
+    g.gs = std::min(g.gs, kMaxRedBlocks);
+    return g;
+}
+
+template <typename HT>
+void launch_spmv(sors_ctx* c, const PcgGrid& g, const HT* u, HT* w, PcgState* st, int first) {
+    if (c->block())
+        k_pcg_spmv_blk<HT><<<g.gs, kBlock, 0, c->stream>>>(c->N, c->pat.nslices, c->sbase.as<int>(),
+                                                           c->bcol.as<int>(), (const HT*)c->vals.p, u, w, st, first,
+                                                           c->red(2));
+    else
+        k_pcg_spmv<HT, 8><<<g.gs, kBlock, 0, c->stream>>>(c->n, c->rowptr.as<int>(), c->cols.as<int>(),
+                                                          (const HT*)c->vals.p, u, w, st, first, c->red(2));
+}
+
+template <typename HT>
+void launch_update(sors_ctx* c, const PcgGrid& g, PcgState* st) {
+    k_pcg_update_dof<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(c->n, (const HT*)c->Minv.p, (HT*)c->px.p, (HT*)c->pr.p,
+                                                            (HT*)c->pu.p, (HT*)c->pp.p, (HT*)c->ps.p,
+                                                            (const HT*)c->pw.p, st, c->red(1));
+}
+
+// PCG on the assembled matrix: solves A x = sign*b; x in c->px (HT).
+// Chunks of pcg_check_every iterations run back to back (from a captured
+// hipGraph when use_graph); the host reads the device convergence flag
+// between chunks.  With profile_kernels the first chunk of every solve is
+// launched directly with a HIP event pair around each SpMV.
 template <typename HT>
 long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_iter, bool* breakdown) {
-    const int N = c->N, n = c->n;
+    const int N = c->N;
     HT *x = (HT*)c->px.p, *r = (HT*)c->pr.p, *u = (HT*)c->pu.p, *p = (HT*)c->pp.p, *s = (HT*)c->ps.p,
        *w = (HT*)c->pw.p;
     PcgState* st = c->pcgst.as<PcgState>();
     const HT* Minv = (const HT*)c->Minv.p;
-    const int gn = red_grid(N, kBlock);
-    constexpr int LPR = 8;
-    const int rows_per_block = kBlock / LPR;
-    // multiple of 8 so that blockIdx % 8 is the XCD the row range belongs to
-    const int gs = (red_grid((long long)(n + rows_per_block - 1) / rows_per_block * kBlock, kBlock) + kNumXcd - 1) /
-                   kNumXcd * kNumXcd;
-    const int* rp = c->rowptr.as<int>();
-    const int* cl = c->cols.as<int>();
-    const HT* vals = (const HT*)c->vals.p;
-    k_pcg_init<HT><<<gn, kBlock, 0, c->stream>>>(N, b, sign, Minv, x, r, u, p, s, st, rtol, c->red(1));
-    k_pcg_spmv<HT, LPR><<<gs, kBlock, 0, c->stream>>>(n, rp, cl, vals, u, w, st, 1, c->red(2));
-    HIPX(hipGetLastError());
+    const PcgGrid g = pcg_grid(c);
     const int chunk = std::max(1, c->opt.pcg_check_every);
+    const int gi = std::is_same<HT, float>::value ? 1 : 0;
+    hipEvent_t* ev = c->prof ? c->events(2 * (chunk + 1)) : nullptr;
+    auto spmv = [&](int j, int first, bool timed) {
+        if (timed) HIPX(hipEventRecord(ev[2 * j], c->stream));
+        launch_spmv<HT>(c, g, u, w, st, first);
+        if (timed) HIPX(hipEventRecord(ev[2 * j + 1], c->stream));
+    };
+    auto account = [&](int count) {
+        for (int j = 0; j < count; ++j) {
+            float ms = 0;
+            HIPX(hipEventElapsedTime(&ms, ev[2 * j], ev[2 * j + 1]));
+            c->prof->spmv_ms += ms;
+            c->prof->spmv_launches += 1;
+        }
+    };
     PcgState* hst = (PcgState*)c->h_pin;
-    int it = 0;
+    k_pcg_init<HT><<<g.gn, kBlock, 0, c->stream>>>(N, b, sign, Minv, x, r, u, p, s, st, rtol, c->red(1));
+    HIPX(hipGetLastError());
+    spmv(0, 1, false);
+    HIPX(hipGetLastError());
+    int it = 0, iter_before = 0;
+    bool first_chunk = true;
     while (it < max_iter) {
         const int k = std::min(chunk, max_iter - it);
-        for (int j = 0; j < k; ++j) {
-            k_pcg_update<HT><<<gn, kBlock, 0, c->stream>>>(N, Minv, x, r, u, p, s, w, st, c->red(1));
-            k_pcg_spmv<HT, LPR><<<gs, kBlock, 0, c->stream>>>(n, rp, cl, vals, u, w, st, 0, c->red(2));
+        const bool timed = ev && first_chunk;
+        if (!timed && c->opt.use_graph && k == chunk) {
+            if (!c->gexec[gi] || c->gkey[gi] != c->vals.p || c->gchunk[gi] != chunk) {
+                if (c->gexec[gi]) HIPX(hipGraphExecDestroy(c->gexec[gi]));
+                c->gexec[gi] = nullptr;
+                hipGraph_t graph = nullptr;
+                HIPX(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                for (int j = 0; j < k; ++j) {
+                    launch_update<HT>(c, g, st);
+                    launch_spmv<HT>(c, g, u, w, st, 0);
+                }
+                HIPX(hipStreamEndCapture(c->stream, &graph));
+                HIPX(hipGraphInstantiate(&c->gexec[gi], graph, nullptr, nullptr, 0));
+                HIPX(hipGraphDestroy(graph));
+                c->gkey[gi] = c->vals.p;
+                c->gchunk[gi] = chunk;
+            }
+            HIPX(hipGraphLaunch(c->gexec[gi], c->stream));
+        } else {
+            for (int j = 0; j < k; ++j) {
+                launch_update<HT>(c, g, st);
+                spmv(j, 0, timed);
+            }
         }
         HIPX(hipGetLastError());
         it += k;
         HIPX(hipMemcpyAsync(hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, c->stream));
         sync(c);
+        if (timed) {
+            // launches after the update that set `done` return at once: not counted
+            const int worked = hst->done ? std::max(0, hst->iter - iter_before - 1) : k;
+            account(std::min(worked, k));
+        }
+        first_chunk = false;
+        iter_before = hst->iter;
         if (hst->done) break;
     }
-    HIPX(hipMemcpyAsync(hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, c->stream));
-    sync(c);
     if (breakdown) *breakdown = hst->breakdown != 0;
     return hst->iter;
 }
@@ -1329,6 +1617,11 @@ int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
     sors_stats local{};
     if (!st) st = &local;
     *st = sors_stats{};
+    c->prof = c->opt.profile_kernels ? st : nullptr;
+    struct ProfReset {
+        sors_ctx* c;
+        ~ProfReset() { c->prof = nullptr; }
+    } prof_reset{c};
     auto t0 = std::chrono::steady_clock::now();
     const double h = dt / substeps;
     ensure_pattern(c, h);
@@ -1362,7 +1655,8 @@ void sors_default_options(sors_options* o) {
     o->pcg_max_iter = 20000;
     o->max_backtrack = 60;
     o->pcg_check_every = 32;
-    o->use_graph = 0;
+    o->use_graph = 1;
+    o->profile_kernels = 0;
 }
 
 int sors_create(const sors_system_desc* d, sors_ctx** out) {
@@ -1508,6 +1802,9 @@ void sors_destroy(sors_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
+    for (hipEvent_t e : c->pev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; ++k)
+        if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
     delete c;
@@ -1685,7 +1982,22 @@ int sors_hessian(sors_ctx* c, const double* q, double dt, int precision, int pro
         assemble<double>(c, c->qtmp.as<double>(), dt, true, project);
     }
     sync(c);
-    if (precision == SORS_MIXED) {
+    if (c->block()) {
+        DevMem out;
+        ensure(out, sizeof(double) * std::max<long long>(nnz, 1));
+        const int* mk = c->has_mask && !c->sqp() ? c->mask.as<int>() : nullptr;
+        if (precision == SORS_MIXED)
+            k_block_to_csr<float><<<grid_for(c->n), kBlock, 0, c->stream>>>(
+                c->n, c->rowptr.as<int>(), c->cols.as<int>(), c->sbase.as<int>(), c->selfslot.as<int>(), mk,
+                c->vals.as<float>(), out.as<double>());
+        else
+            k_block_to_csr<double><<<grid_for(c->n), kBlock, 0, c->stream>>>(
+                c->n, c->rowptr.as<int>(), c->cols.as<int>(), c->sbase.as<int>(), c->selfslot.as<int>(), mk,
+                c->vals.as<double>(), out.as<double>());
+        HIPX(hipGetLastError());
+        HIPX(hipMemcpyAsync(vals, out.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+    } else if (precision == SORS_MIXED) {
         std::vector<float> tmp(nnz);
         HIPX(hipMemcpyAsync(tmp.data(), c->vals.p, sizeof(float) * nnz, hipMemcpyDeviceToHost, c->stream));
         sync(c);
@@ -1752,6 +2064,82 @@ int sors_linear_solve(sors_ctx* c, const double* q, double dt, const double* b, 
     sync(c);
     if (iters) *iters = (int)it;
     if (brk) return fail(c, SORS_ERR_NAN, "PCG breakdown (indefinite tangent)");
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_bench_kernel(sors_ctx* c, int kernel, int reps, double* avg_ms, double* algo_bytes) {
+    SORS_GUARD(c)
+    if (reps < 1 || !avg_ms) return fail(c, SORS_ERR_ARG, "reps must be >= 1");
+    if (!c->pat_on_device) return fail(c, SORS_ERR_ARG, "no assembled system yet: run a step first");
+    const bool mixed = c->opt.precision == SORS_MIXED;
+    const double hb = mixed ? 4.0 : 8.0;
+    const int N = c->N, n = c->n;
+    const long long nnz = c->pat.nnz;
+    ensure(c->pcgst_bench, sizeof(PcgState));
+    PcgState* bst = c->pcgst_bench.as<PcgState>();
+    PcgState init{};
+    init.gamma = 1.0;
+    init.gamma_new = 1.0;
+    init.alpha = 0.0;
+    init.beta = 0.0;
+    init.tol2 = -1.0;
+    hipEvent_t* ev = c->events(2);
+    double bytes = 0;
+    auto run = [&](auto launch) {
+        launch();  // warm
+        HIPX(hipMemcpyAsync(bst, &init, sizeof(PcgState), hipMemcpyHostToDevice, c->stream));
+        sync(c);
+        HIPX(hipEventRecord(ev[0], c->stream));
+        for (int i = 0; i < reps; ++i) launch();
+        HIPX(hipEventRecord(ev[1], c->stream));
+        HIPX(hipEventSynchronize(ev[1]));
+        HIPX(hipGetLastError());
+        float ms = 0;
+        HIPX(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        *avg_ms = ms / reps;
+    };
+    const PcgGrid g = pcg_grid(c);
+    if (kernel == SORS_KERNEL_SPMV) {
+        // compulsory bytes: every stored nonzero once, one column index per
+        // row (CSR) or per nonzero 3x3 node block (sliced layout), x and y once
+        bytes = c->block() ? hb * nnz + 4.0 * c->pat.nblocks + 2.0 * hb * n
+                           : nnz * (hb + 4.0) + 4.0 * (n + 1) + 2.0 * hb * n;
+        if (mixed) {
+            fill<float>(c->pu.as<float>(), n, 1.0f, c->stream);
+            run([&] { launch_spmv<float>(c, g, c->pu.as<float>(), c->pw.as<float>(), bst, 1); });
+        } else {
+            fill<double>(c->pu.as<double>(), n, 1.0, c->stream);
+            run([&] { launch_spmv<double>(c, g, c->pu.as<double>(), c->pw.as<double>(), bst, 1); });
+        }
+    } else if (kernel == SORS_KERNEL_PCG_UPDATE) {
+        bytes = hb * (11.0 * n + 9.0 * N);
+        if (mixed)
+            run([&] { launch_update<float>(c, g, bst); });
+        else
+            run([&] { launch_update<double>(c, g, bst); });
+    } else if (kernel == SORS_KERNEL_COPY) {
+        const long long n4 = nnz / 4;  // int4 elements of the column array
+        ensure(c->copybuf, sizeof(int4) * std::max<long long>(n4, 1));
+        bytes = 2.0 * 16.0 * n4;
+        const int grid = (int)std::min<long long>((n4 + kBlock - 1) / kBlock, 8192);
+        run([&] {
+            k_copy<<<std::max(grid, 1), kBlock, 0, c->stream>>>(n4, c->cols.as<int4>(), c->copybuf.as<int4>());
+        });
+    } else if (kernel == SORS_KERNEL_ASSEMBLE) {
+        bytes = 4.0 * c->ed * c->E + 16.0 * c->E + 48.0 * N + hb * nnz + 8.0 * n;  // values written once
+        ElemArgs a = elem_args(c, c->q.as<double>());
+        a.gint = c->gint.as<double>();
+        a.vals = c->vals.p;
+        a.hscale = 1.0;
+        if (mixed)
+            run([&] { launch_elem<float, 0>(c, a, c->opt.project_spd); });
+        else
+            run([&] { launch_elem<double, 0>(c, a, c->opt.project_spd); });
+    } else {
+        return fail(c, SORS_ERR_ARG, "unknown kernel id");
+    }
+    if (algo_bytes) *algo_bytes = bytes;
     return SORS_OK;
     SORS_GUARD_END(c)
 }

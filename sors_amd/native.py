@@ -21,6 +21,7 @@ BACKWARD_EULER, CRANK_NICOLSON = 0, 1
 MINIMIZE_NEWTON, MINIMIZE_NEWTON_LINESEARCH, MINIMIZE_SQP = 0, 1, 2
 NEOHOOKEAN, TEST_POLY = 0, 1
 FP64, MIXED = 0, 1
+KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_COPY, KERNEL_PCG_UPDATE = 0, 1, 2, 3
 
 SCHEMES = {"backward_euler": BACKWARD_EULER, "crank_nicolson": CRANK_NICOLSON}
 METHODS = {"minimize_newton": MINIMIZE_NEWTON, "minimize_newton_linesearch": MINIMIZE_NEWTON_LINESEARCH,
@@ -43,7 +44,7 @@ class Options(C.Structure):
     _fields_ = [
         ("precision", C.c_int), ("project_spd", C.c_int), ("max_newton", C.c_int), ("dx_tol", C.c_double),
         ("g_tol", C.c_double), ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int), ("max_backtrack", C.c_int),
-        ("pcg_check_every", C.c_int), ("use_graph", C.c_int),
+        ("pcg_check_every", C.c_int), ("use_graph", C.c_int), ("profile_kernels", C.c_int),
     ]
 
 
@@ -51,7 +52,8 @@ class Stats(C.Structure):
     _fields_ = [
         ("newton_iters", C.c_int), ("pcg_iters", C.c_longlong), ("backtracks", C.c_int), ("gnorm", C.c_double),
         ("dxnorm", C.c_double), ("ms_assemble", C.c_double), ("ms_solve", C.c_double),
-        ("ms_linesearch", C.c_double), ("ms_total", C.c_double),
+        ("ms_linesearch", C.c_double), ("ms_total", C.c_double), ("spmv_ms", C.c_double),
+        ("spmv_launches", C.c_longlong), ("assemble_kernel_ms", C.c_double), ("assemble_launches", C.c_longlong),
     ]
 
     def as_dict(self):
@@ -100,6 +102,7 @@ def lib():
         L.sors_hessian.argtypes = [vp, _dp, C.c_double, C.c_int, C.c_int, _dp]
         L.sors_element_eval.argtypes = [vp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]
         L.sors_linear_solve.argtypes = [vp, _dp, C.c_double, _dp, _dp, C.c_double, C.c_int, _ip]
+        L.sors_bench_kernel.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -286,6 +289,11 @@ class Rollout:
         e, g, h = np.zeros(self.E), np.zeros(self.E * nd), np.zeros(self.E * nd * nd)
         self._check(lib().sors_element_eval(self.h, _d(q), int(precision), int(project), _d(e), _d(g), _d(h)))
         return e, g.reshape(self.E, nd), h.reshape(self.E, nd, nd)
+
+    def bench_kernel(self, kernel, reps=20):
+        ms, by = np.zeros(1), np.zeros(1)
+        self._check(lib().sors_bench_kernel(self.h, int(kernel), int(reps), _d(ms), _d(by)))
+        return float(ms[0]), float(by[0])
 
     def linear_solve(self, q, dt, b, rtol=1e-10, max_iter=20000):
         q = f64(q).reshape(-1)
