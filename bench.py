@@ -45,11 +45,11 @@ def make_scene(args):
     return mesh.cantilever_large(element=args.element, seed=0)
 
 
-def run_gpu(sc, precision, steps, warmup, profile):
+def run_gpu(sc, precision, steps, warmup, profile, use_graph=True):
     from sors_amd import native
 
-    r = native.Rollout.from_scene(sc, precision=precision, pcg_rtol=1e-4 if precision == native.MIXED else 1e-4,
-                                  profile_kernels=int(profile))
+    r = native.Rollout.from_scene(sc, precision=precision, pcg_rtol=1e-4, profile_kernels=int(profile),
+                                  use_graph=int(use_graph))
     for _ in range(warmup):
         r.step_resident(sc.dt)
     agg = {"newton_iters": 0, "pcg_iters": 0, "backtracks": 0, "spmv_ms": 0.0, "spmv_launches": 0,
@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--profile-kernel", choices=["spmv", "assemble", "copy", "update"])
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--fp64", action="store_true", help="profile-kernel in FP64")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch PCG kernels directly instead of replaying captured hipGraphs (profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -146,7 +148,8 @@ def main():
 
     sc = make_scene(args)
     log(f"workload {sc.name}: {sc.n_elems} elements, {sc.n_nodes} nodes; warmup {args.warmup}, steps {args.steps}")
-    r, q_mixed, elapsed, agg = run_gpu(sc, native.MIXED, args.steps, args.warmup, profile=True)
+    r, q_mixed, elapsed, agg = run_gpu(sc, native.MIXED, args.steps, args.warmup, profile=True,
+                                       use_graph=not args.no_graph)
     nnz = int(len(r.pattern(sc.dt)[1]))
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"mixed: {ms_per_step:.1f} ms/step, {agg}")
@@ -190,7 +193,8 @@ def main():
         out["assembly"] = {"avg_launch_ms": asm_ms, "algo_bytes_per_launch": asm_bytes,
                            "achieved_GBps": asm_bytes / (asm_ms * 1e-3) / 1e9}
     if not args.no_fp64:
-        r64, q64, el64, agg64 = run_gpu(sc, native.FP64, args.steps, args.warmup, profile=False)
+        r64, q64, el64, agg64 = run_gpu(sc, native.FP64, args.steps, args.warmup, profile=False,
+                                        use_graph=not args.no_graph)
         r64.close()
         err = float(np.abs(q_mixed - q64).max() / sc.bbox_diag)
         out["fp64_gpu"] = {"value": args.steps / el64, "unit": "steps/s", "ms_per_step": 1e3 * el64 / args.steps,

@@ -35,14 +35,20 @@ enum Scalar {
     S_COUNT
 };
 
+// PCG scalars carried between iterations (two copies, used alternately)
 struct PcgState {
-    double alpha, beta, gamma, gamma_new, delta, rr, rr0, tol2;
-    int done, iter, breakdown, pad;
+    double alpha, gamma, rr0, tol2;
+    int iter, pad;
+};
+
+// PCG outcome, written once by the kernel that ends the solve
+struct PcgCtl {
+    int done, breakdown, iter, pad;
+    double rr;
 };
 
 struct Reduce {
-    double* partials;    // [kMaxRedBlocks * 4]
-    unsigned* counter;   // one per reduction site
+    double* partials;    // [NV][kMaxRedBlocks] per-block sums
 };
 
 struct ElemArgs {

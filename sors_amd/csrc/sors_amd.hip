@@ -62,39 +62,47 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     __syncthreads();
 }
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Each block writes its NV sums to partials[k * kMaxRedBlocks + blockIdx.x]
+// with plain stores; the partials become visible to the next kernel on the
+// stream (kernel boundaries publish them across XCDs).  No fences: an
+// agent-scope release on gfx950 writes back the XCD's whole L2.
+template <int NV>
+__device__ __forceinline__ void block_partial(double (&v)[NV], double* partials) {
+    block_sum<NV>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) partials[k * kMaxRedBlocks + blockIdx.x] = v[k];
 }
 
-// Returns true in the last block, where `v` then holds the grid totals
-// (valid in thread 0).
+// Sum of the first nb partial slots, identical (same order) in every block
+// and valid in every thread.
 template <int NV>
-__device__ bool grid_sum(double (&v)[NV], Reduce red) {
-    block_sum<NV>(v);
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) red.partials[k * kMaxRedBlocks + blockIdx.x] = v[k];
-        __threadfence();
-        unsigned t = atomicAdd(red.counter, 1u);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
+__device__ __forceinline__ void sum_partials(const double* partials, int nb, double (&out)[NV]) {
+    __shared__ double bc[NV];
     double s[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) s[k] = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
+    for (int k = 0; k < NV; ++k) s[k] = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
 #pragma unroll
-        for (int k = 0; k < NV; ++k) s[k] += load_coherent(red.partials + k * kMaxRedBlocks + i);
+        for (int k = 0; k < NV; ++k) s[k] += partials[k * kMaxRedBlocks + i];
     block_sum<NV>(s);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] = s[k];
-        *red.counter = 0u;
-    }
-    return true;
+        for (int k = 0; k < NV; ++k) bc[k] = s[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = bc[k];
+    __syncthreads();
+}
+
+// one block: out[k] = sum of the nb partials of slot k
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_finalize(const double* partials, int nb, double* out) {
+    double t[NV];
+    sum_partials<NV>(partials, nb, t);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) out[k] = t[k];
 }
 
 static inline int red_grid(long long work, int per_block) {
@@ -623,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_elem_energy(ElemArgs a, int diff, do
         v[0] += en;
     }
     if (inv) atomicOr(a.flag, 2);
-    if (grid_sum<1>(v, red) && threadIdx.x == 0) *out = v[0];
+    block_partial<1>(v, red.partials);
 }
 
 // ===========================================================================
@@ -668,7 +676,7 @@ __global__ __launch_bounds__(kBlock) void k_node_terms(NodeArgs a, double* gg_ou
                 vals[di] += (HT)hd;
         }
     }
-    if (grid_sum<1>(v, red) && threadIdx.x == 0) *gg_out = v[0];
+    block_partial<1>(v, red.partials);
 }
 
 // energy terms of the nodes (integrate_energy, integrator.cpp:3-43, and the
@@ -713,7 +721,7 @@ __global__ __launch_bounds__(kBlock) void k_node_energy(NodeArgs a, const double
         }
         v[0] += en;
     }
-    if (grid_sum<1>(v, red) && threadIdx.x == 0) *out = v[0];
+    block_partial<1>(v, red.partials);
 }
 
 // ===========================================================================
@@ -808,13 +816,25 @@ __global__ void k_block_to_csr(int n, const int* rowptr, const int* cols, const 
 }
 
 // ===========================================================================
-// PCG (Chronopoulos-Gear form: one SpMV and one fused vector pass per
-// iteration, FP64 dot products, device-side scalars and convergence flag)
+// PCG, Chronopoulos-Gear form: per iteration k one fused vector pass U_k and
+// one SpMV A_k.  FP64 dot products.  Scalars live on the device: U_k and A_k
+// start by summing the previous kernel's per-block partials (identically in
+// every block), so no kernel needs a grid-wide fence.  Ping-pong state:
+//   U_k reads  st[(k-1)%2] = {gamma_{k-2}, alpha_{k-2}}, partU[(k-1)%2] =
+//              gamma_{k-1}, partA = delta_{k-1}; writes st[k%2] and
+//              partU[k%2] = (gamma_k, rr_k)
+//   A_k reads  partU[k%2] (convergence), writes partA = delta_k
+// A_0 / U_1 are the first iteration (alpha_0 = gamma_0/delta_0, beta_0 = 0).
 // ===========================================================================
-// init: x = 0, r = sign*b, p = s = 0, u = M r, gamma = r.u, rr0 = r.r
+struct PcgPart {
+    double* u[2];   // 2 slots each: gamma, rr
+    double* a;      // 1 slot: delta
+};
+
+// init: x = 0, r = sign*b, p = s = 0, u = M r; partU[0] = (r.u, r.r)
 template <typename HT>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int N, const double* b, double sign, const HT* Minv, HT* x, HT* r,
-                                                     HT* u, HT* p, HT* s, PcgState* st, double rtol, Reduce red) {
+                                                     HT* u, HT* p, HT* s, PcgPart part) {
     double v[2] = {0.0, 0.0};
     for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < N; nd += gridDim.x * blockDim.x) {
         HT rr[3];
@@ -835,29 +855,57 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int N, const double* b, dou
             v[1] += (double)rr[c] * (double)rr[c];
         }
     }
-    if (grid_sum<2>(v, red) && threadIdx.x == 0) {
-        st->gamma = v[0];
-        st->rr = v[1];
-        st->rr0 = v[1];
-        st->tol2 = rtol * rtol * v[1];
-        st->beta = 0.0;
-        st->alpha = 0.0;
-        st->iter = 0;
-        st->breakdown = 0;
-        st->done = (v[1] == 0.0 || !(v[0] > 0.0)) ? 1 : 0;
-    }
+    block_partial<2>(v, part.u[0]);
 }
 
-// w = A u (LPR lanes per row), delta = w.u; finalize alpha/beta.
+// SpMV prologue: returns true when the solve is over (converged, breakdown,
+// or a zero right-hand side).  Block 0 records the outcome in ctl.
+__device__ __forceinline__ bool spmv_prologue(PcgCtl* ctl, PcgState* st, const double* partU, int nbU, int first,
+                                              double rtol) {
+    if (ctl->done) return true;
+    double t[2];
+    sum_partials<2>(partU, nbU, t);  // gamma_k, rr_k
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (first) {
+        const bool zero = !(t[1] > 0.0) || !(t[0] > 0.0);
+        if (lead) {
+            st->rr0 = t[1];
+            st->tol2 = rtol * rtol * t[1];
+            st->gamma = t[0];
+            st->alpha = 0.0;
+            st->iter = 0;
+            if (zero) {
+                ctl->done = 1;
+                ctl->iter = 0;
+                ctl->rr = t[1];
+                ctl->breakdown = (t[1] > 0.0) ? 1 : 0;
+            }
+        }
+        return zero;
+    }
+    const bool nan = !(t[1] == t[1]);
+    if (t[1] <= st->tol2 || nan) {
+        if (lead) {
+            ctl->done = 1;
+            ctl->iter = st->iter;
+            ctl->rr = t[1];
+            ctl->breakdown = nan ? 1 : 0;
+        }
+        return true;
+    }
+    return false;
+}
+
+// w = A u on scalar CSR rows (general masks), LPR lanes per row
 template <typename HT, int LPR>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* rowptr, const int* cols, const HT* vals,
-                                                     const HT* u, HT* w, PcgState* st, int first, Reduce red) {
-    if (st->done) return;
+                                                     const HT* u, HT* w, PcgCtl* ctl, PcgState* st, PcgPart part,
+                                                     int par, int nbU, int first, double rtol) {
+    if (spmv_prologue(ctl, st, part.u[par], nbU, first, rtol)) return;
     double v[1] = {0.0};
     const int sub = threadIdx.x % LPR;
     const int rows_per_block = blockDim.x / LPR;
     const int nblk_rows = (n + rows_per_block - 1) / rows_per_block;
-    // XCD-contiguous row ranges
     const int per_xcd = (nblk_rows + kNumXcd - 1) / kNumXcd;
     for (int t = blockIdx.x; t < kNumXcd * per_xcd; t += gridDim.x) {
         const int xcd = t % kNumXcd, local = t / kNumXcd;
@@ -876,67 +924,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* rowptr, c
             v[0] += (double)acc * (double)u[row];
         }
     }
-    if (grid_sum<1>(v, red) && threadIdx.x == 0) {
-        const double delta = v[0];
-        st->delta = delta;
-        if (first) {
-            st->alpha = st->gamma / delta;
-            st->beta = 0.0;
-            if (!(delta > 0.0)) {
-                st->breakdown = 1;
-                st->done = 1;
-            }
-        } else {
-            const double beta = st->gamma_new / st->gamma;
-            const double den = delta - beta * st->gamma_new / st->alpha;
-            if (!(den > 0.0) || !(st->gamma_new > 0.0)) {
-                st->breakdown = 1;
-                st->done = 1;
-            } else {
-                st->alpha = st->gamma_new / den;
-                st->beta = beta;
-                st->gamma = st->gamma_new;
-            }
-        }
-    }
-}
-
-// p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = M r;
-// gamma_new = r.u; rr = r.r; converged when rr <= tol2
-template <typename HT>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int N, const HT* Minv, HT* x, HT* r, HT* u, HT* p, HT* s,
-                                                       const HT* w, PcgState* st, Reduce red) {
-    if (st->done) return;
-    const HT alpha = (HT)st->alpha, beta = (HT)st->beta;
-    double v[2] = {0.0, 0.0};
-    for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < N; nd += gridDim.x * blockDim.x) {
-        HT rr[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int i = 3 * nd + c;
-            const HT pp = u[i] + beta * p[i];
-            const HT ss = w[i] + beta * s[i];
-            p[i] = pp;
-            s[i] = ss;
-            x[i] += alpha * pp;
-            rr[c] = r[i] - alpha * ss;
-            r[i] = rr[c];
-        }
-        const HT* M = Minv + (size_t)9 * nd;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const HT uu = M[3 * c] * rr[0] + M[3 * c + 1] * rr[1] + M[3 * c + 2] * rr[2];
-            u[3 * nd + c] = uu;
-            v[0] += (double)rr[c] * (double)uu;
-            v[1] += (double)rr[c] * (double)rr[c];
-        }
-    }
-    if (grid_sum<2>(v, red) && threadIdx.x == 0) {
-        st->gamma_new = v[0];
-        st->rr = v[1];
-        st->iter += 1;
-        if (v[1] <= st->tol2 || !(v[1] == v[1])) st->done = 1;
-    }
+    block_partial<1>(v, part.a);
 }
 
 // w = A u on the sliced node-block layout: one lane per node (three rows),
@@ -944,9 +932,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int N, const HT* Minv, HT
 // with unit stride across the wavefront.
 template <typename HT>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv_blk(int N, int nslices, const int* sbase, const int* bcol,
-                                                         const HT* vals, const HT* u, HT* w, PcgState* st,
-                                                         int first, Reduce red) {
-    if (st->done) return;
+                                                         const HT* vals, const HT* u, HT* w, PcgCtl* ctl,
+                                                         PcgState* st, PcgPart part, int par, int nbU, int first,
+                                                         double rtol) {
+    if (spmv_prologue(ctl, st, part.u[par], nbU, first, rtol)) return;
     double v[1] = {0.0};
     const int lane = threadIdx.x & (kSlice - 1), wv = threadIdx.x / kSlice;
     constexpr int kPer = kBlock / kSlice;  // slices per block
@@ -955,22 +944,22 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv_blk(int N, int nslices, con
     for (int t = blockIdx.x; t < kNumXcd * per_xcd; t += gridDim.x) {
         const int xcd = t % kNumXcd, local = t / kNumXcd;
         const int g = xcd * per_xcd + local;
-        const int s = g * kPer + wv;
-        if (local >= per_xcd || g >= ngroups || s >= nslices) continue;  // uniform per wavefront
-        const int base = sbase[s];
-        const int width = (sbase[s + 1] - base) / kSlice;
+        const int sl = g * kPer + wv;
+        if (local >= per_xcd || g >= ngroups || sl >= nslices) continue;  // uniform per wavefront
+        const int base = sbase[sl];
+        const int width = (sbase[sl + 1] - base) / kSlice;
         const int* cp = bcol + base + lane;
         const HT* vp = vals + 9LL * base + lane;
         HT a0 = 0, a1 = 0, a2 = 0;
         for (int j = 0; j < width; ++j) {
             const int nb = cp[kSlice * j];
             const HT x0 = u[3 * nb], x1 = u[3 * nb + 1], x2 = u[3 * nb + 2];
-            const HT* b = vp + (long long)kSlice * 9 * j;
-            a0 += b[0] * x0 + b[kSlice] * x1 + b[2 * kSlice] * x2;
-            a1 += b[3 * kSlice] * x0 + b[4 * kSlice] * x1 + b[5 * kSlice] * x2;
-            a2 += b[6 * kSlice] * x0 + b[7 * kSlice] * x1 + b[8 * kSlice] * x2;
+            const HT* bb = vp + (long long)kSlice * 9 * j;
+            a0 += bb[0] * x0 + bb[kSlice] * x1 + bb[2 * kSlice] * x2;
+            a1 += bb[3 * kSlice] * x0 + bb[4 * kSlice] * x1 + bb[5 * kSlice] * x2;
+            a2 += bb[6 * kSlice] * x0 + bb[7 * kSlice] * x1 + bb[8 * kSlice] * x2;
         }
-        const int node = s * kSlice + lane;
+        const int node = sl * kSlice + lane;
         if (node < N) {
             w[3 * node] = a0;
             w[3 * node + 1] = a1;
@@ -979,71 +968,79 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv_blk(int N, int nslices, con
                     (double)a2 * (double)u[3 * node + 2];
         }
     }
-    if (grid_sum<1>(v, red) && threadIdx.x == 0) {
-        const double delta = v[0];
-        st->delta = delta;
-        if (first) {
-            st->alpha = st->gamma / delta;
-            st->beta = 0.0;
-            if (!(delta > 0.0)) {
-                st->breakdown = 1;
-                st->done = 1;
-            }
-        } else {
-            const double beta = st->gamma_new / st->gamma;
-            const double den = delta - beta * st->gamma_new / st->alpha;
-            if (!(den > 0.0) || !(st->gamma_new > 0.0)) {
-                st->breakdown = 1;
-                st->done = 1;
-            } else {
-                st->alpha = st->gamma_new / den;
-                st->beta = beta;
-                st->gamma = st->gamma_new;
-            }
-        }
-    }
+    block_partial<1>(v, part.a);
 }
 
-// Same update as k_pcg_update with one thread per DOF (unit-stride vector
-// traffic); the three residual components of a node meet in LDS for u = M r.
+// U_k: p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = M r;
+// partU[k%2] = (r.u, r.r).  One thread per DOF (unit-stride vector traffic);
+// the three residual components of a node meet in LDS for u = M r.
 constexpr int kUpdBlock = 192;  // 64 nodes
 template <typename HT>
-__global__ __launch_bounds__(kUpdBlock) void k_pcg_update_dof(int n, const HT* Minv, HT* x, HT* r, HT* u, HT* p,
-                                                              HT* s, const HT* w, PcgState* st, Reduce red) {
-    if (st->done) return;
-    const HT alpha = (HT)st->alpha, beta = (HT)st->beta;
+__global__ __launch_bounds__(kUpdBlock) void k_pcg_update(int n, const HT* Minv, HT* x, HT* r, HT* u, HT* p, HT* s,
+                                                          const HT* w, PcgCtl* ctl, PcgState* st_prev,
+                                                          PcgState* st_next, PcgPart part, int par, int nbU, int nbA,
+                                                          int first) {
+    if (ctl->done) return;
+    double g[2], d[1];
+    sum_partials<2>(part.u[par ^ 1], nbU, g);  // gamma_{k-1}, rr_{k-1}
+    sum_partials<1>(part.a, nbA, d);           // delta_{k-1}
+    double alpha, beta;
+    bool bad;
+    if (first) {
+        beta = 0.0;
+        alpha = g[0] / d[0];
+        bad = !(d[0] > 0.0);
+    } else {
+        beta = g[0] / st_prev->gamma;
+        const double den = d[0] - beta * g[0] / st_prev->alpha;
+        alpha = g[0] / den;
+        bad = !(den > 0.0) || !(g[0] > 0.0);
+    }
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (bad) {
+        if (lead) {
+            ctl->done = 1;
+            ctl->breakdown = 1;
+            ctl->iter = st_prev->iter;
+            ctl->rr = g[1];
+        }
+        return;
+    }
+    if (lead) {
+        st_next->gamma = g[0];
+        st_next->alpha = alpha;
+        st_next->rr0 = st_prev->rr0;
+        st_next->tol2 = st_prev->tol2;
+        st_next->iter = st_prev->iter + 1;
+    }
+    const HT a = (HT)alpha, bt = (HT)beta;
     __shared__ HT rs[kUpdBlock];
     double v[2] = {0.0, 0.0};
-    const int c = threadIdx.x % 3, first = threadIdx.x - c;
+    const int c = threadIdx.x % 3, first3 = threadIdx.x - c;
     for (int base = blockIdx.x * kUpdBlock; base < n; base += gridDim.x * kUpdBlock) {
         const int i = base + threadIdx.x;
         HT rr = 0;
         if (i < n) {
-            const HT pp = u[i] + beta * p[i];
-            const HT ss = w[i] + beta * s[i];
+            const HT pp = u[i] + bt * p[i];
+            const HT ss = w[i] + bt * s[i];
             p[i] = pp;
             s[i] = ss;
-            x[i] += alpha * pp;
-            rr = r[i] - alpha * ss;
+            x[i] += a * pp;
+            rr = r[i] - a * ss;
             r[i] = rr;
         }
         rs[threadIdx.x] = rr;
         __syncthreads();
         if (i < n) {
             const HT* M = Minv + 3 * (size_t)i;  // row c of node i/3: Minv[9 (i/3) + 3 c] = Minv[3 i]
-            const HT uu = M[0] * rs[first] + M[1] * rs[first + 1] + M[2] * rs[first + 2];
+            const HT uu = M[0] * rs[first3] + M[1] * rs[first3 + 1] + M[2] * rs[first3 + 2];
             u[i] = uu;
             v[0] += (double)rr * (double)uu;
             v[1] += (double)rr * (double)rr;
         }
         __syncthreads();
     }
-    if (grid_sum<2>(v, red) && threadIdx.x == 0) {
-        st->gamma_new = v[0];
-        st->rr = v[1];
-        st->iter += 1;
-        if (v[1] <= st->tol2 || !(v[1] == v[1])) st->done = 1;
-    }
+    block_partial<2>(v, part.u[par]);
 }
 
 // ===========================================================================
@@ -1059,10 +1056,7 @@ __global__ __launch_bounds__(kBlock) void k_step_dots(int n, const HT* dx, const
         v[0] += d * d;
         v[1] += g[i] * d;
     }
-    if (grid_sum<2>(v, red) && threadIdx.x == 0) {
-        *out_dd = v[0];
-        *out_gd = v[1];
-    }
+    block_partial<2>(v, red.partials);
 }
 
 template <typename HT>
@@ -1160,7 +1154,7 @@ struct sors_ctx {
     DevMem q, qprev, vprev, gprev, gtmp, gint, g, fext, dxd, qtmp;
     DevMem rowptr, cols, diag, rank, fullmap, vals, Minv;
     DevMem pr, pu, pp, ps, pw, px;
-    DevMem partials, counters, scal, pcgst, flag;
+    DevMem partials, scal, pcgst, pcgctl, pcgpart, flag;
     double* h_pin = nullptr;   // pinned readback area
     HostPattern pat;
     int pat_dtnz = -1;
@@ -1185,8 +1179,11 @@ struct sors_ctx {
         return pev.data();
     }
 
-    unsigned* counter(int k) { return counters.as<unsigned>() + k; }
-    Reduce red(int k) { return Reduce{partials.as<double>(), counter(k)}; }
+    Reduce red() { return Reduce{partials.as<double>()}; }
+    PcgPart pcg_part() {
+        double* b = pcgpart.as<double>();
+        return PcgPart{{b, b + 2 * kMaxRedBlocks}, b + 4 * kMaxRedBlocks};
+    }
     bool sqp() const { return method == SORS_MINIMIZE_SQP; }
 };
 
@@ -1344,7 +1341,8 @@ void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project
     NodeArgs na = node_args(c, q, dt);
     na.vals = hessian ? c->vals.p : nullptr;
     const int grid = red_grid(c->n, kBlock);
-    k_node_terms<HT><<<grid, kBlock, 0, c->stream>>>(na, c->scal.as<double>() + S_GG, c->red(0));
+    k_node_terms<HT><<<grid, kBlock, 0, c->stream>>>(na, c->scal.as<double>() + S_GG, c->red());
+    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), grid, c->scal.as<double>() + S_GG);
     HIPX(hipGetLastError());
 }
 
@@ -1361,67 +1359,68 @@ void build_preconditioner(sors_ctx* c) {
 }
 
 // launch geometry of the PCG kernels
+constexpr int kSpmvBlocks = 1024;   // multiple of 8 (XCD-contiguous row ranges)
+constexpr int kUpdBlocks = 512;
+
 struct PcgGrid {
-    int gn;     // per-node reductions (init)
+    int gn;     // init (per node)
     int gu;     // DOF-parallel update
     int gs;     // SpMV
 };
 
 PcgGrid pcg_grid(sors_ctx* c) {
     PcgGrid g;
-    g.gn = red_grid(c->N, kBlock);
-    g.gu = red_grid(c->n, kUpdBlock);
-    if (c->block()) {
-        const int groups = (c->pat.nslices + kBlock / kSlice - 1) / (kBlock / kSlice);
-        g.gs = (std::min(groups, kMaxRedBlocks) + kNumXcd - 1) / kNumXcd * kNumXcd;
-    } else {
-        const int rows_per_block = kBlock / 8;
-        g.gs = (red_grid((long long)(c->n + rows_per_block - 1) / rows_per_block * kBlock, kBlock) + kNumXcd - 1) /
-               kNumXcd * kNumXcd;
-    }
-    g.gs = std::min(g.gs, kMaxRedBlocks);
+    g.gn = std::min(red_grid(c->N, kBlock), kUpdBlocks);
+    g.gu = std::min(red_grid(c->n, kUpdBlock), kUpdBlocks);
+    long long work = c->block() ? (c->pat.nslices + kBlock / kSlice - 1) / (kBlock / kSlice)
+                                : (c->n + kBlock / 8 - 1) / (kBlock / 8);
+    g.gs = (int)std::min<long long>(std::max<long long>(work, 1), kSpmvBlocks);
+    g.gs = (g.gs + kNumXcd - 1) / kNumXcd * kNumXcd;
     return g;
 }
 
+// A_k (first: A_0) reading partU[par]; nbU = blocks of the producer of partU
 template <typename HT>
-void launch_spmv(sors_ctx* c, const PcgGrid& g, const HT* u, HT* w, PcgState* st, int first) {
+void launch_spmv(sors_ctx* c, const PcgGrid& g, const HT* u, HT* w, PcgCtl* ctl, PcgState* st, int par, int nbU,
+                 int first, double rtol) {
+    const PcgPart part = c->pcg_part();
     if (c->block())
         k_pcg_spmv_blk<HT><<<g.gs, kBlock, 0, c->stream>>>(c->N, c->pat.nslices, c->sbase.as<int>(),
-                                                           c->bcol.as<int>(), (const HT*)c->vals.p, u, w, st, first,
-                                                           c->red(2));
+                                                           c->bcol.as<int>(), (const HT*)c->vals.p, u, w, ctl, st,
+                                                           part, par, nbU, first, rtol);
     else
         k_pcg_spmv<HT, 8><<<g.gs, kBlock, 0, c->stream>>>(c->n, c->rowptr.as<int>(), c->cols.as<int>(),
-                                                          (const HT*)c->vals.p, u, w, st, first, c->red(2));
+                                                          (const HT*)c->vals.p, u, w, ctl, st, part, par, nbU, first,
+                                                          rtol);
 }
 
+// U_k reading partU[par ^ 1] (produced with nbU blocks) and partA
 template <typename HT>
-void launch_update(sors_ctx* c, const PcgGrid& g, PcgState* st) {
-    k_pcg_update_dof<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(c->n, (const HT*)c->Minv.p, (HT*)c->px.p, (HT*)c->pr.p,
-                                                            (HT*)c->pu.p, (HT*)c->pp.p, (HT*)c->ps.p,
-                                                            (const HT*)c->pw.p, st, c->red(1));
+void launch_update(sors_ctx* c, const PcgGrid& g, PcgCtl* ctl, PcgState* st_prev, PcgState* st_next, int par,
+                   int nbU, int first) {
+    k_pcg_update<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(c->n, (const HT*)c->Minv.p, (HT*)c->px.p, (HT*)c->pr.p,
+                                                        (HT*)c->pu.p, (HT*)c->pp.p, (HT*)c->ps.p, (const HT*)c->pw.p,
+                                                        ctl, st_prev, st_next, c->pcg_part(), par, nbU, g.gs, first);
 }
 
 // PCG on the assembled matrix: solves A x = sign*b; x in c->px (HT).
-// Chunks of pcg_check_every iterations run back to back (from a captured
-// hipGraph when use_graph); the host reads the device convergence flag
-// between chunks.  With profile_kernels the first chunk of every solve is
-// launched directly with a HIP event pair around each SpMV.
+// Sequence: init, A_0, U_1, A_1, then chunks of an even number of
+// iterations (U_k, A_k), replayed from a captured hipGraph when use_graph.
+// The host reads the outcome record between chunks.  With profile_kernels
+// the first chunk of every solve is launched directly with a HIP event pair
+// around each SpMV.
 template <typename HT>
 long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_iter, bool* breakdown) {
     const int N = c->N;
     HT *x = (HT*)c->px.p, *r = (HT*)c->pr.p, *u = (HT*)c->pu.p, *p = (HT*)c->pp.p, *s = (HT*)c->ps.p,
        *w = (HT*)c->pw.p;
     PcgState* st = c->pcgst.as<PcgState>();
+    PcgCtl* ctl = c->pcgctl.as<PcgCtl>();
     const HT* Minv = (const HT*)c->Minv.p;
     const PcgGrid g = pcg_grid(c);
-    const int chunk = std::max(1, c->opt.pcg_check_every);
+    const int chunk = std::max(2, (c->opt.pcg_check_every + 1) / 2 * 2);
     const int gi = std::is_same<HT, float>::value ? 1 : 0;
     hipEvent_t* ev = c->prof ? c->events(2 * (chunk + 1)) : nullptr;
-    auto spmv = [&](int j, int first, bool timed) {
-        if (timed) HIPX(hipEventRecord(ev[2 * j], c->stream));
-        launch_spmv<HT>(c, g, u, w, st, first);
-        if (timed) HIPX(hipEventRecord(ev[2 * j + 1], c->stream));
-    };
     auto account = [&](int count) {
         for (int j = 0; j < count; ++j) {
             float ms = 0;
@@ -1430,15 +1429,25 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
             c->prof->spmv_launches += 1;
         }
     };
-    PcgState* hst = (PcgState*)c->h_pin;
-    k_pcg_init<HT><<<g.gn, kBlock, 0, c->stream>>>(N, b, sign, Minv, x, r, u, p, s, st, rtol, c->red(1));
+    // iteration k >= 2 reads from st[(k-1)%2] and writes st[k%2]
+    auto iteration = [&](int k, int j, bool timed) {
+        const int par = k & 1;
+        launch_update<HT>(c, g, ctl, st + (par ^ 1), st + par, par, g.gu, 0);
+        if (timed) HIPX(hipEventRecord(ev[2 * j], c->stream));
+        launch_spmv<HT>(c, g, u, w, ctl, st + par, par, g.gu, 0, rtol);
+        if (timed) HIPX(hipEventRecord(ev[2 * j + 1], c->stream));
+    };
+    PcgCtl* hctl = (PcgCtl*)c->h_pin;
+    fill<int>((int*)ctl, sizeof(PcgCtl) / sizeof(int), 0, c->stream);
+    k_pcg_init<HT><<<g.gn, kBlock, 0, c->stream>>>(N, b, sign, Minv, x, r, u, p, s, c->pcg_part());
+    launch_spmv<HT>(c, g, u, w, ctl, st, 0, g.gn, 1, rtol);            // A_0
+    launch_update<HT>(c, g, ctl, st, st + 1, 1, g.gn, 1);              // U_1
+    launch_spmv<HT>(c, g, u, w, ctl, st + 1, 1, g.gu, 0, rtol);        // A_1
     HIPX(hipGetLastError());
-    spmv(0, 1, false);
-    HIPX(hipGetLastError());
-    int it = 0, iter_before = 0;
-    bool first_chunk = true;
-    while (it < max_iter) {
-        const int k = std::min(chunk, max_iter - it);
+    int launched = 1;
+    bool first_chunk = true, done = false;
+    while (launched < max_iter) {
+        const int k = std::min(chunk, max_iter - launched);
         const bool timed = ev && first_chunk;
         if (!timed && c->opt.use_graph && k == chunk) {
             if (!c->gexec[gi] || c->gkey[gi] != c->vals.p || c->gchunk[gi] != chunk) {
@@ -1446,10 +1455,7 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
                 c->gexec[gi] = nullptr;
                 hipGraph_t graph = nullptr;
                 HIPX(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                for (int j = 0; j < k; ++j) {
-                    launch_update<HT>(c, g, st);
-                    launch_spmv<HT>(c, g, u, w, st, 0);
-                }
+                for (int j = 0; j < k; ++j) iteration(launched + 1 + j, j, false);  // parity pattern repeats
                 HIPX(hipStreamEndCapture(c->stream, &graph));
                 HIPX(hipGraphInstantiate(&c->gexec[gi], graph, nullptr, nullptr, 0));
                 HIPX(hipGraphDestroy(graph));
@@ -1458,26 +1464,30 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
             }
             HIPX(hipGraphLaunch(c->gexec[gi], c->stream));
         } else {
-            for (int j = 0; j < k; ++j) {
-                launch_update<HT>(c, g, st);
-                spmv(j, 0, timed);
-            }
+            for (int j = 0; j < k; ++j) iteration(launched + 1 + j, j, timed);
         }
         HIPX(hipGetLastError());
-        it += k;
-        HIPX(hipMemcpyAsync(hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, c->stream));
+        launched += k;
+        HIPX(hipMemcpyAsync(hctl, ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, c->stream));
         sync(c);
         if (timed) {
-            // launches after the update that set `done` return at once: not counted
-            const int worked = hst->done ? std::max(0, hst->iter - iter_before - 1) : k;
+            // SpMV launches after the one that ended the solve return at once: not counted
+            const int worked =
+                hctl->done ? std::max(0, hctl->iter - (launched - k) - (hctl->breakdown ? 0 : 1)) : k;
             account(std::min(worked, k));
         }
         first_chunk = false;
-        iter_before = hst->iter;
-        if (hst->done) break;
+        if (hctl->done) {
+            done = true;
+            break;
+        }
     }
-    if (breakdown) *breakdown = hst->breakdown != 0;
-    return hst->iter;
+    if (!done) {
+        HIPX(hipMemcpyAsync(hctl, ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+    }
+    if (breakdown) *breakdown = hctl->breakdown != 0;
+    return hctl->done ? hctl->iter : launched;
 }
 
 double read_scalar(sors_ctx* c, int slot) {
@@ -1500,12 +1510,14 @@ double energy_delta(sors_ctx* c, const double* q, const double* dx, double alpha
     a.alpha = alpha;
     const int ge = red_grid(c->E, kBlock);
     if (c->ed == 4)
-        k_elem_energy<4><<<ge, kBlock, 0, c->stream>>>(a, 1, c->scal.as<double>() + S_EELEM, c->red(3));
+        k_elem_energy<4><<<ge, kBlock, 0, c->stream>>>(a, 1, c->scal.as<double>() + S_EELEM, c->red());
     else
-        k_elem_energy<8><<<ge, kBlock, 0, c->stream>>>(a, 1, c->scal.as<double>() + S_EELEM, c->red(3));
+        k_elem_energy<8><<<ge, kBlock, 0, c->stream>>>(a, 1, c->scal.as<double>() + S_EELEM, c->red());
+    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), ge, c->scal.as<double>() + S_EELEM);
     NodeArgs na = node_args(c, q, dt);
-    k_node_energy<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(na, dx, alpha, 1,
-                                                                     c->scal.as<double>() + S_ENODE, c->red(4));
+    const int gnn = red_grid(c->n, kBlock);
+    k_node_energy<<<gnn, kBlock, 0, c->stream>>>(na, dx, alpha, 1, c->scal.as<double>() + S_ENODE, c->red());
+    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gnn, c->scal.as<double>() + S_ENODE);
     HIPX(hipGetLastError());
     double v[2];
     read_scalars(c, v, S_EELEM, 2);
@@ -1516,12 +1528,14 @@ double energy_abs(sors_ctx* c, const double* q, double dt) {
     ElemArgs a = elem_args(c, q);
     const int ge = red_grid(c->E, kBlock);
     if (c->ed == 4)
-        k_elem_energy<4><<<ge, kBlock, 0, c->stream>>>(a, 0, c->scal.as<double>() + S_EELEM, c->red(3));
+        k_elem_energy<4><<<ge, kBlock, 0, c->stream>>>(a, 0, c->scal.as<double>() + S_EELEM, c->red());
     else
-        k_elem_energy<8><<<ge, kBlock, 0, c->stream>>>(a, 0, c->scal.as<double>() + S_EELEM, c->red(3));
+        k_elem_energy<8><<<ge, kBlock, 0, c->stream>>>(a, 0, c->scal.as<double>() + S_EELEM, c->red());
+    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), ge, c->scal.as<double>() + S_EELEM);
     NodeArgs na = node_args(c, q, dt);
-    k_node_energy<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(na, nullptr, 0.0, 0,
-                                                                     c->scal.as<double>() + S_ENODE, c->red(4));
+    const int gnn = red_grid(c->n, kBlock);
+    k_node_energy<<<gnn, kBlock, 0, c->stream>>>(na, nullptr, 0.0, 0, c->scal.as<double>() + S_ENODE, c->red());
+    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gnn, c->scal.as<double>() + S_ENODE);
     HIPX(hipGetLastError());
     double v[2];
     read_scalars(c, v, S_EELEM, 2);
@@ -1558,8 +1572,11 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
         bool brk = false;
         stt->pcg_iters += pcg<HT>(c, g, -1.0, o.pcg_rtol, o.pcg_max_iter, &brk);
         k_to_f64<HT><<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, (const HT*)c->px.p, dx);
-        k_step_dots<HT><<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
-            c->n, (const HT*)c->px.p, g, c->scal.as<double>() + S_DD, c->scal.as<double>() + S_GD, c->red(5));
+        const int gd = red_grid(c->n, kBlock);
+        k_step_dots<HT><<<gd, kBlock, 0, c->stream>>>(c->n, (const HT*)c->px.p, g, c->scal.as<double>() + S_DD,
+                                                       c->scal.as<double>() + S_GD, c->red());
+        static_assert(S_GD == S_DD + 1, "step dots are finalized as one pair");
+        k_finalize<2><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gd, c->scal.as<double>() + S_DD);
         HIPX(hipGetLastError());
         double dots[2];
         read_scalars(c, dots, S_DD, 2);
@@ -1780,11 +1797,11 @@ int sors_create(const sors_system_desc* d, sors_ctx** out) {
         for (DevMem* v : {&c->pr, &c->pu, &c->pp, &c->ps, &c->pw, &c->px}) ensure(*v, sizeof(double) * n);
         ensure(c->Minv, sizeof(double) * 9 * (size_t)N);
         ensure(c->partials, sizeof(double) * kMaxRedBlocks * 4);
-        ensure(c->counters, sizeof(unsigned) * 16);
+        ensure(c->pcgpart, sizeof(double) * 5 * kMaxRedBlocks);
+        ensure(c->pcgctl, sizeof(PcgCtl));
         ensure(c->scal, sizeof(double) * S_COUNT);
-        ensure(c->pcgst, sizeof(PcgState));
+        ensure(c->pcgst, 2 * sizeof(PcgState));
         ensure(c->flag, sizeof(int));
-        fill<unsigned>(c->counters.as<unsigned>(), 16, 0u, st);
         fill<double>(c->scal.as<double>(), S_COUNT, 0.0, st);
         fill<int>(c->flag.as<int>(), 1, 0, st);
         HIPX(hipHostMalloc((void**)&c->h_pin, 4096, hipHostMallocDefault));
@@ -2076,19 +2093,26 @@ int sors_bench_kernel(sors_ctx* c, int kernel, int reps, double* avg_ms, double*
     const double hb = mixed ? 4.0 : 8.0;
     const int N = c->N, n = c->n;
     const long long nnz = c->pat.nnz;
-    ensure(c->pcgst_bench, sizeof(PcgState));
+    // bench scalars: a solve that never converges and never breaks down
+    ensure(c->pcgst_bench, 2 * sizeof(PcgState) + sizeof(PcgCtl));
     PcgState* bst = c->pcgst_bench.as<PcgState>();
-    PcgState init{};
-    init.gamma = 1.0;
-    init.gamma_new = 1.0;
-    init.alpha = 0.0;
-    init.beta = 0.0;
-    init.tol2 = -1.0;
+    PcgCtl* bctl = (PcgCtl*)(bst + 2);
+    const PcgGrid g = pcg_grid(c);
+    PcgState init[2] = {};
+    for (auto& t : init) {
+        t.gamma = (double)g.gu;
+        t.alpha = 1e30;
+        t.tol2 = -1.0;
+        t.rr0 = 1.0;
+    }
+    PcgCtl cinit{};
     hipEvent_t* ev = c->events(2);
     double bytes = 0;
     auto run = [&](auto launch) {
+        fill<double>(c->pcgpart.as<double>(), 5LL * kMaxRedBlocks, 1.0, c->stream);
+        HIPX(hipMemcpyAsync(bst, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+        HIPX(hipMemcpyAsync(bctl, &cinit, sizeof(cinit), hipMemcpyHostToDevice, c->stream));
         launch();  // warm
-        HIPX(hipMemcpyAsync(bst, &init, sizeof(PcgState), hipMemcpyHostToDevice, c->stream));
         sync(c);
         HIPX(hipEventRecord(ev[0], c->stream));
         for (int i = 0; i < reps; ++i) launch();
@@ -2099,7 +2123,6 @@ int sors_bench_kernel(sors_ctx* c, int kernel, int reps, double* avg_ms, double*
         HIPX(hipEventElapsedTime(&ms, ev[0], ev[1]));
         *avg_ms = ms / reps;
     };
-    const PcgGrid g = pcg_grid(c);
     if (kernel == SORS_KERNEL_SPMV) {
         // compulsory bytes: every stored nonzero once, one column index per
         // row (CSR) or per nonzero 3x3 node block (sliced layout), x and y once
@@ -2107,17 +2130,17 @@ int sors_bench_kernel(sors_ctx* c, int kernel, int reps, double* avg_ms, double*
                            : nnz * (hb + 4.0) + 4.0 * (n + 1) + 2.0 * hb * n;
         if (mixed) {
             fill<float>(c->pu.as<float>(), n, 1.0f, c->stream);
-            run([&] { launch_spmv<float>(c, g, c->pu.as<float>(), c->pw.as<float>(), bst, 1); });
+            run([&] { launch_spmv<float>(c, g, c->pu.as<float>(), c->pw.as<float>(), bctl, bst, 0, g.gu, 0, 0.0); });
         } else {
             fill<double>(c->pu.as<double>(), n, 1.0, c->stream);
-            run([&] { launch_spmv<double>(c, g, c->pu.as<double>(), c->pw.as<double>(), bst, 1); });
+            run([&] { launch_spmv<double>(c, g, c->pu.as<double>(), c->pw.as<double>(), bctl, bst, 0, g.gu, 0, 0.0); });
         }
     } else if (kernel == SORS_KERNEL_PCG_UPDATE) {
         bytes = hb * (11.0 * n + 9.0 * N);
         if (mixed)
-            run([&] { launch_update<float>(c, g, bst); });
+            run([&] { launch_update<float>(c, g, bctl, bst, bst + 1, 1, g.gu, 0); });
         else
-            run([&] { launch_update<double>(c, g, bst); });
+            run([&] { launch_update<double>(c, g, bctl, bst, bst + 1, 1, g.gu, 0); });
     } else if (kernel == SORS_KERNEL_COPY) {
         const long long n4 = nnz / 4;  // int4 elements of the column array
         ensure(c->copybuf, sizeof(int4) * std::max<long long>(n4, 1));
