@@ -80,7 +80,10 @@ typedef struct {
     int max_backtrack;
     int pcg_check_every;  /* PCG iterations between host convergence checks */
     int use_graph;        /* replay PCG iteration chunks from a captured hipGraph */
-    int profile_kernels;  /* time every SpMV and assembly launch with HIP events */
+    int profile_kernels;  /* time SpMV and assembly launches with HIP events */
+    int predict;          /* start Newton from qPrev + dt*vPrev (Dirichlet DOFs:
+                             qPrev + dt*value) instead of the caller's guess; the
+                             converged state is the same minimiser */
 } sors_options;
 
 typedef struct {

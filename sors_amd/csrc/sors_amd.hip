@@ -13,6 +13,7 @@
 // The direct SimplicialLDLT solve (solver.cpp:82-84) is replaced by a
 // block-Jacobi preconditioned conjugate gradient on the fixed CSR pattern,
 // with SPD-projected element tangents.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1065,6 +1066,13 @@ __global__ void k_to_f64(long long n, const HT* a, double* out) {
         out[i] = (double)a[i];
 }
 
+// Newton initial guess q = qPrev + h vPrev (Dirichlet DOFs: qPrev + h value)
+__global__ void k_predict(int n, double h, const double* qprev, const double* vprev, const int* mask,
+                          const double* bcval, double* q) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        q[i] = qprev[i] + h * ((mask && mask[i]) ? bcval[i] : vprev[i]);
+}
+
 __global__ void k_axpy(int n, double alpha, const double* dx, double* q) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) q[i] += alpha * dx[i];
 }
@@ -1287,22 +1295,22 @@ NodeArgs node_args(sors_ctx* c, const double* q, double dt) {
     return a;
 }
 
+// element kernel launch; with e0/e1 the dispatch records the two events
 template <typename HT, int MODE>
-void launch_elem(sors_ctx* c, const ElemArgs& a, int project) {
+void launch_elem(sors_ctx* c, const ElemArgs& a, int project, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     if (c->E == 0) return;
-    if (c->ed == 4) {
-        const int grid = grid_for(c->E, kBlock);
-        if (project)
-            k_tet<HT, 1, MODE><<<grid, kBlock, 0, c->stream>>>(a);
-        else
-            k_tet<HT, 0, MODE><<<grid, kBlock, 0, c->stream>>>(a);
-    } else {
-        const int grid = grid_for(c->E, 128);
-        if (project)
-            k_hex<HT, 1, MODE><<<grid, 128, 0, c->stream>>>(a);
-        else
-            k_hex<HT, 0, MODE><<<grid, 128, 0, c->stream>>>(a);
-    }
+    const bool tet = c->ed == 4;
+    const int block = tet ? kBlock : 128;
+    const dim3 grid(grid_for(c->E, block));
+    void (*k)(ElemArgs);
+    if (tet)
+        k = project ? k_tet<HT, 1, MODE> : k_tet<HT, 0, MODE>;
+    else
+        k = project ? k_hex<HT, 1, MODE> : k_hex<HT, 0, MODE>;
+    if (e0)
+        hipExtLaunchKernelGGL(k, grid, dim3(block), 0, c->stream, e0, e1, 0, a);
+    else
+        k<<<grid, block, 0, c->stream>>>(a);
     HIPX(hipGetLastError());
 }
 
@@ -1325,13 +1333,11 @@ void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project
     a.vals = hessian ? c->vals.p : nullptr;
     a.hscale = (c->scheme == SORS_CRANK_NICOLSON && dt != 0.0) ? 0.5 : 1.0;
     hipEvent_t* ev = c->prof ? c->events(2) : nullptr;
-    if (ev) HIPX(hipEventRecord(ev[0], c->stream));
     if (hessian)
-        launch_elem<HT, 0>(c, a, project);
+        launch_elem<HT, 0>(c, a, project, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     else
-        launch_elem<HT, 1>(c, a, 0);
+        launch_elem<HT, 1>(c, a, 0, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     if (ev) {
-        HIPX(hipEventRecord(ev[1], c->stream));
         HIPX(hipEventSynchronize(ev[1]));
         float ms = 0;
         HIPX(hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -1380,14 +1386,24 @@ PcgGrid pcg_grid(sors_ctx* c) {
 }
 
 // A_k (first: A_0) reading partU[par]; nbU = blocks of the producer of partU
+// (with e0/e1: launched through hipExtLaunchKernelGGL, which records the two
+// events around this dispatch itself)
 template <typename HT>
 void launch_spmv(sors_ctx* c, const PcgGrid& g, const HT* u, HT* w, PcgCtl* ctl, PcgState* st, int par, int nbU,
-                 int first, double rtol) {
+                 int first, double rtol, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const PcgPart part = c->pcg_part();
-    if (c->block())
+    if (c->block() && e0)
+        hipExtLaunchKernelGGL(k_pcg_spmv_blk<HT>, dim3(g.gs), dim3(kBlock), 0, c->stream, e0, e1, 0, c->N,
+                              c->pat.nslices, (const int*)c->sbase.as<int>(), (const int*)c->bcol.as<int>(),
+                              (const HT*)c->vals.p, u, w, ctl, st, part, par, nbU, first, rtol);
+    else if (c->block())
         k_pcg_spmv_blk<HT><<<g.gs, kBlock, 0, c->stream>>>(c->N, c->pat.nslices, c->sbase.as<int>(),
                                                            c->bcol.as<int>(), (const HT*)c->vals.p, u, w, ctl, st,
                                                            part, par, nbU, first, rtol);
+    else if (e0)
+        hipExtLaunchKernelGGL(k_pcg_spmv<HT, 8>, dim3(g.gs), dim3(kBlock), 0, c->stream, e0, e1, 0, c->n,
+                              (const int*)c->rowptr.as<int>(), (const int*)c->cols.as<int>(), (const HT*)c->vals.p, u,
+                              w, ctl, st, part, par, nbU, first, rtol);
     else
         k_pcg_spmv<HT, 8><<<g.gs, kBlock, 0, c->stream>>>(c->n, c->rowptr.as<int>(), c->cols.as<int>(),
                                                           (const HT*)c->vals.p, u, w, ctl, st, part, par, nbU, first,
@@ -1433,9 +1449,10 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
     auto iteration = [&](int k, int j, bool timed) {
         const int par = k & 1;
         launch_update<HT>(c, g, ctl, st + (par ^ 1), st + par, par, g.gu, 0);
-        if (timed) HIPX(hipEventRecord(ev[2 * j], c->stream));
-        launch_spmv<HT>(c, g, u, w, ctl, st + par, par, g.gu, 0, rtol);
-        if (timed) HIPX(hipEventRecord(ev[2 * j + 1], c->stream));
+        if (timed)
+            launch_spmv<HT>(c, g, u, w, ctl, st + par, par, g.gu, 0, rtol, ev[2 * j], ev[2 * j + 1]);
+        else
+            launch_spmv<HT>(c, g, u, w, ctl, st + par, par, g.gu, 0, rtol);
     };
     PcgCtl* hctl = (PcgCtl*)c->h_pin;
     fill<int>((int*)ctl, sizeof(PcgCtl) / sizeof(int), 0, c->stream);
@@ -1570,7 +1587,12 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
         }
         auto t1 = std::chrono::steady_clock::now();
         bool brk = false;
-        stt->pcg_iters += pcg<HT>(c, g, -1.0, o.pcg_rtol, o.pcg_max_iter, &brk);
+        // forcing term: pcg_rtol, loosened when the residual criterion is in
+        // reach (the linear residual of the step only has to drop below
+        // g_tol: eta ||g|| < g_tol / 2), never above 0.1
+        double eta = o.pcg_rtol;
+        if (o.g_tol > 0 && stt->gnorm > 0) eta = std::min(0.1, std::max(eta, 0.5 * o.g_tol / stt->gnorm));
+        stt->pcg_iters += pcg<HT>(c, g, -1.0, eta, o.pcg_max_iter, &brk);
         k_to_f64<HT><<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, (const HT*)c->px.p, dx);
         const int gd = red_grid(c->n, kBlock);
         k_step_dots<HT><<<gd, kBlock, 0, c->stream>>>(c->n, (const HT*)c->px.p, g, c->scal.as<double>() + S_DD,
@@ -1643,6 +1665,10 @@ int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
     const double h = dt / substeps;
     ensure_pattern(c, h);
     for (int j = 0; j < substeps; ++j) {
+        if (c->opt.predict)
+            k_predict<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
+                c->n, h, c->qprev.as<double>(), c->vprev.as<double>(), c->has_mask ? c->mask.as<int>() : nullptr,
+                c->bcval.as<double>(), c->q.as<double>());
         int rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, h, st) : minimize<double>(c, h, st);
         if (rc != SORS_OK) return rc;
         k_update_state<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
@@ -1674,6 +1700,7 @@ void sors_default_options(sors_options* o) {
     o->pcg_check_every = 32;
     o->use_graph = 1;
     o->profile_kernels = 0;
+    o->predict = 1;
 }
 
 int sors_create(const sors_system_desc* d, sors_ctx** out) {
