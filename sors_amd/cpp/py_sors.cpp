@@ -224,6 +224,7 @@ PYBIND11_MODULE(py_sors, m) {
         .def_readwrite("numThreads", &SimulationSettings::numThreads)
         .def_readwrite("precision", &SimulationSettings::precision)
         .def_readwrite("pcgTolerance", &SimulationSettings::pcgTolerance)
+        .def_readwrite("predictor", &SimulationSettings::predictor)
         .def("verbose_print", &SimulationSettings::verbose_print);
 
     bind_energy<4>(m, "EnergyTet");
