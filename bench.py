@@ -138,6 +138,7 @@ def main():
     ap.add_argument("--pcg-rtol", type=float, default=3e-3)
     ap.add_argument("--g-tol", type=float, default=1e-8, help="Newton residual criterion ||g||_2 (N)")
     ap.add_argument("--no-predict", action="store_true", help="start Newton from the previous positions")
+    ap.add_argument("--check-every", type=int, default=32, help="PCG iterations between host convergence checks")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -151,7 +152,7 @@ def main():
     sc = make_scene(args)
     log(f"workload {sc.name}: {sc.n_elems} elements, {sc.n_nodes} nodes; warmup {args.warmup}, steps {args.steps}")
     opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=int(not args.no_predict),
-                use_graph=int(not args.no_graph))
+                use_graph=int(not args.no_graph), pcg_check_every=args.check_every)
     r, q_mixed, elapsed, agg = run_gpu(sc, native.MIXED, args.steps, args.warmup, profile=True, **opts)
     nnz = int(len(r.pattern(sc.dt)[1]))
     ms_per_step = 1e3 * elapsed / args.steps

@@ -1015,29 +1015,54 @@ __global__ __launch_bounds__(kUpdBlock) void k_pcg_update(int n, const HT* Minv,
         st_next->iter = st_prev->iter + 1;
     }
     const HT a = (HT)alpha, bt = (HT)beta;
-    __shared__ HT rs[kUpdBlock];
+    // kUnroll DOF groups per thread and trip: all their loads are issued
+    // before any result is needed (memory-level parallelism)
+    constexpr int kUnroll = 4;
+    __shared__ HT rs[kUnroll][kUpdBlock];
     double v[2] = {0.0, 0.0};
     const int c = threadIdx.x % 3, first3 = threadIdx.x - c;
-    for (int base = blockIdx.x * kUpdBlock; base < n; base += gridDim.x * kUpdBlock) {
-        const int i = base + threadIdx.x;
-        HT rr = 0;
-        if (i < n) {
-            const HT pp = u[i] + bt * p[i];
-            const HT ss = w[i] + bt * s[i];
-            p[i] = pp;
-            s[i] = ss;
-            x[i] += a * pp;
-            rr = r[i] - a * ss;
-            r[i] = rr;
+    const int stride = gridDim.x * kUpdBlock;
+    for (int base = blockIdx.x * kUpdBlock; base < n; base += kUnroll * stride) {
+        HT pp[kUnroll], ss[kUnroll], uu0[kUnroll], ww[kUnroll], rr0[kUnroll], xx[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            const int i = base + q * stride + threadIdx.x;
+            if (i < n) {
+                uu0[q] = u[i];
+                pp[q] = p[i];
+                ww[q] = w[i];
+                ss[q] = s[i];
+                rr0[q] = r[i];
+                xx[q] = x[i];
+            }
         }
-        rs[threadIdx.x] = rr;
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            const int i = base + q * stride + threadIdx.x;
+            HT rr = 0;
+            if (i < n) {
+                const HT pn = uu0[q] + bt * pp[q];
+                const HT sn = ww[q] + bt * ss[q];
+                p[i] = pn;
+                s[i] = sn;
+                x[i] = xx[q] + a * pn;
+                rr = rr0[q] - a * sn;
+                r[i] = rr;
+            }
+            rs[q][threadIdx.x] = rr;
+        }
         __syncthreads();
-        if (i < n) {
-            const HT* M = Minv + 3 * (size_t)i;  // row c of node i/3: Minv[9 (i/3) + 3 c] = Minv[3 i]
-            const HT uu = M[0] * rs[first3] + M[1] * rs[first3 + 1] + M[2] * rs[first3 + 2];
-            u[i] = uu;
-            v[0] += (double)rr * (double)uu;
-            v[1] += (double)rr * (double)rr;
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            const int i = base + q * stride + threadIdx.x;
+            if (i < n) {
+                const HT* M = Minv + 3 * (size_t)i;  // row c of node i/3: Minv[9 (i/3) + 3 c] = Minv[3 i]
+                const HT rr = rs[q][threadIdx.x];
+                const HT un = M[0] * rs[q][first3] + M[1] * rs[q][first3 + 1] + M[2] * rs[q][first3 + 2];
+                u[i] = un;
+                v[0] += (double)rr * (double)un;
+                v[1] += (double)rr * (double)rr;
+            }
         }
         __syncthreads();
     }
