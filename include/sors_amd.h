@@ -28,6 +28,8 @@ extern "C" {
 #define SORS_ERR_ARG (-5)
 #define SORS_ERR_HIP (-6)
 #define SORS_ERR_UNSUPPORTED (-7)
+#define SORS_ERR_TIMEOUT (-8)     /* sors_server_wait: request not finished yet */
+#define SORS_ERR_BUDGET (-9)      /* request can never fit the server's HBM budget */
 
 /* SimulationSettings::timeSteppingScheme (cpp/headers/simulationSettings.h:31) */
 enum { SORS_BACKWARD_EULER = 0, SORS_CRANK_NICOLSON = 1 };
@@ -162,6 +164,60 @@ enum { SORS_KERNEL_SPMV = 0, SORS_KERNEL_ASSEMBLE = 1, SORS_KERNEL_COPY = 2, SOR
  * written; ASSEMBLE: connectivity, Lame constants, node positions, CSR
  * values and gradient written once). */
 int sors_bench_kernel(sors_ctx* ctx, int kernel, int reps, double* avg_ms, double* algo_bytes);
+
+/* ---- Serving loop --------------------------------------------------------
+ * No counterpart in the reference: its analogue is a sequential Python loop
+ * over full rollouts (python/d05a_hopping_grid.py:103-106).  A server keeps
+ * many rollouts of uneven mesh size resident on the device, admits queued
+ * requests in arrival order whenever a slot and HBM budget are free, and
+ * advances every resident rollout in lock-step ticks: per-rollout assembly
+ * and line-search kernels run on the rollout's own stream while the PCG
+ * iterations of all rollouts in their linear solve run as one batched
+ * launch per kernel.  Each rollout follows exactly the Newton logic of
+ * sors_step with sors_step's predictor (options.predict), so its result is
+ * that of sors_step alone up to the rounding order of float atomics. */
+typedef struct sors_server sors_server;
+
+typedef struct {
+    double hbm_budget_bytes;   /* device memory resident rollouts may hold */
+    int max_active;            /* rollouts resident at once (<= 256) */
+    int prep_threads;          /* host threads building patterns / uploading */
+    int pcg_chunk;             /* batched PCG iterations between host checks (even) */
+    sors_options solver;       /* options of every rollout */
+} sors_server_options;
+
+typedef struct {
+    int status;                /* SORS_OK or the rollout's error code */
+    int steps;                 /* implicit steps completed */
+    long long newton_iters, pcg_iters, backtracks;
+    double latency_ms;         /* submit -> result ready */
+    double queue_ms;           /* submit -> admitted */
+    double prep_ms;            /* admitted -> resident (pattern, upload) */
+    double device_bytes;       /* device memory held while resident */
+} sors_request_stats;
+
+typedef struct {
+    long long submitted, completed, failed;
+    int active, peak_active;
+    double resident_bytes, peak_resident_bytes;
+    long long ticks, batched_launches;
+} sors_server_stats;
+
+void sors_server_default_options(sors_server_options* o);
+int sors_server_create(const sors_server_options* o, sors_server** out);
+/* Submit one rollout: the mesh and material of desc (copied), initial
+ * velocity v0 [3 n_nodes] or NULL, `steps` implicit steps of size dt.
+ * Arrival time is the time of the call. */
+int sors_server_submit(sors_server* s, const sors_system_desc* desc, const double* v0, double dt, int steps,
+                       long long* id);
+/* Wait up to timeout_s (< 0: forever) for request `id`; on completion copy
+ * its final positions q [3 n_nodes] (may be NULL) and stats, and release
+ * it.  Returns the request's status, or SORS_ERR_TIMEOUT. */
+int sors_server_wait(sors_server* s, long long id, double* q, sors_request_stats* st, double timeout_s);
+int sors_server_get_stats(sors_server* s, sors_server_stats* st);
+const char* sors_server_last_error(const sors_server* s);
+/* Finishes resident work, drops queued requests, frees everything. */
+void sors_server_destroy(sors_server* s);
 
 #ifdef __cplusplus
 }

@@ -19,11 +19,17 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "kernels.h"
@@ -832,41 +838,92 @@ struct PcgPart {
     double* a;      // 1 slot: delta
 };
 
-// init: x = 0, r = sign*b, p = s = 0, u = M r; partU[0] = (r.u, r.r)
+// One rollout's PCG data (sliced node-block matrix).
 template <typename HT>
-__global__ __launch_bounds__(kBlock) void k_pcg_init(int N, const double* b, double sign, const HT* Minv, HT* x, HT* r,
-                                                     HT* u, HT* p, HT* s, PcgPart part) {
-    double v[2] = {0.0, 0.0};
-    for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < N; nd += gridDim.x * blockDim.x) {
+struct PcgView {
+    int N, n, nslices;
+    const int* sbase;
+    const int* bcol;
+    const HT* vals;
+    const HT* Minv;
+    HT *x, *r, *u, *p, *s, *w;
+    PcgCtl* ctl;
+    PcgState* st;  // st[0], st[1]
+    double rtol;
+};
+
+// Where one kernel's blocks write their partial sums (slot k of block lb at
+// base[k * stride + lb]) and where the sums it consumes live.
+struct PartIO {
+    double* out;        // this kernel's partials
+    int stride;
+    const double* inU;  // consumed gamma/rr partials, with their producer's block count
+    int nbU;
+    const double* inA;  // consumed delta partials
+    int nbA;
+};
+
+template <int NV>
+__device__ __forceinline__ void block_partial_at(double (&v)[NV], double* base, int stride, int lb) {
+    block_sum<NV>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) base[k * stride + lb] = v[k];
+}
+
+template <int NV>
+__device__ __forceinline__ void sum_partials_at(const double* base, int stride, int nb, double (&out)[NV]) {
+    __shared__ double bc[NV];
+    double s[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) s[k] += base[k * stride + i];
+    block_sum<NV>(s);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) bc[k] = s[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = bc[k];
+    __syncthreads();
+}
+
+// ---- bodies (lb / nb: this block's index and the block count of its rollout)
+// init: x = 0, r = sign*b, p = s = 0, u = M r; partials (r.u, r.r)
+template <typename HT>
+__device__ void pcg_init_body(const PcgView<HT>& v, const double* b, double sign, PartIO io, int lb, int nb) {
+    double acc[2] = {0.0, 0.0};
+    for (int nd = lb * blockDim.x + threadIdx.x; nd < v.N; nd += nb * blockDim.x) {
         HT rr[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             rr[c] = (HT)(sign * b[3 * nd + c]);
-            r[3 * nd + c] = rr[c];
-            x[3 * nd + c] = 0;
-            p[3 * nd + c] = 0;
-            s[3 * nd + c] = 0;
+            v.r[3 * nd + c] = rr[c];
+            v.x[3 * nd + c] = 0;
+            v.p[3 * nd + c] = 0;
+            v.s[3 * nd + c] = 0;
         }
-        const HT* M = Minv + (size_t)9 * nd;
+        const HT* M = v.Minv + (size_t)9 * nd;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const HT uu = M[3 * c] * rr[0] + M[3 * c + 1] * rr[1] + M[3 * c + 2] * rr[2];
-            u[3 * nd + c] = uu;
-            v[0] += (double)rr[c] * (double)uu;
-            v[1] += (double)rr[c] * (double)rr[c];
+            v.u[3 * nd + c] = uu;
+            acc[0] += (double)rr[c] * (double)uu;
+            acc[1] += (double)rr[c] * (double)rr[c];
         }
     }
-    block_partial<2>(v, part.u[0]);
+    block_partial_at<2>(acc, io.out, io.stride, lb);
 }
 
 // SpMV prologue: returns true when the solve is over (converged, breakdown,
-// or a zero right-hand side).  Block 0 records the outcome in ctl.
-__device__ __forceinline__ bool spmv_prologue(PcgCtl* ctl, PcgState* st, const double* partU, int nbU, int first,
-                                              double rtol) {
+// or a zero right-hand side).  Block 0 of the rollout records the outcome.
+__device__ __forceinline__ bool spmv_prologue(PcgCtl* ctl, PcgState* st, const double* partU, int stride, int nbU,
+                                              int first, double rtol, bool lead) {
     if (ctl->done) return true;
     double t[2];
-    sum_partials<2>(partU, nbU, t);  // gamma_k, rr_k
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    sum_partials_at<2>(partU, stride, nbU, t);  // gamma_k, rr_k
     if (first) {
         const bool zero = !(t[1] > 0.0) || !(t[0] > 0.0);
         if (lead) {
@@ -897,12 +954,165 @@ __device__ __forceinline__ bool spmv_prologue(PcgCtl* ctl, PcgState* st, const d
     return false;
 }
 
+// A_k: w = A u on the sliced node-block layout (one lane per node, one
+// wavefront per slice of 64 nodes, unit-stride value and index loads)
+template <typename HT>
+__device__ void pcg_spmv_body(const PcgView<HT>& v, PcgState* st, PartIO io, int first, int lb, int nb) {
+    const bool lead = lb == 0 && threadIdx.x == 0;
+    if (spmv_prologue(v.ctl, st, io.inU, io.stride, io.nbU, first, v.rtol, lead)) return;
+    double acc[1] = {0.0};
+    const int lane = threadIdx.x & (kSlice - 1), wv = threadIdx.x / kSlice;
+    constexpr int kPer = kBlock / kSlice;  // slices per block
+    const int ngroups = (v.nslices + kPer - 1) / kPer;
+    const int per_xcd = (ngroups + kNumXcd - 1) / kNumXcd;
+    for (int t = lb; t < kNumXcd * per_xcd; t += nb) {
+        const int xcd = t % kNumXcd, local = t / kNumXcd;
+        const int g = xcd * per_xcd + local;
+        const int sl = g * kPer + wv;
+        if (local >= per_xcd || g >= ngroups || sl >= v.nslices) continue;  // uniform per wavefront
+        const int base = v.sbase[sl];
+        const int width = (v.sbase[sl + 1] - base) / kSlice;
+        const int* cp = v.bcol + base + lane;
+        const HT* vp = v.vals + 9LL * base + lane;
+        HT a0 = 0, a1 = 0, a2 = 0;
+        for (int j = 0; j < width; ++j) {
+            const int nbr = cp[kSlice * j];
+            const HT x0 = v.u[3 * nbr], x1 = v.u[3 * nbr + 1], x2 = v.u[3 * nbr + 2];
+            const HT* bb = vp + (long long)kSlice * 9 * j;
+            a0 += bb[0] * x0 + bb[kSlice] * x1 + bb[2 * kSlice] * x2;
+            a1 += bb[3 * kSlice] * x0 + bb[4 * kSlice] * x1 + bb[5 * kSlice] * x2;
+            a2 += bb[6 * kSlice] * x0 + bb[7 * kSlice] * x1 + bb[8 * kSlice] * x2;
+        }
+        const int node = sl * kSlice + lane;
+        if (node < v.N) {
+            v.w[3 * node] = a0;
+            v.w[3 * node + 1] = a1;
+            v.w[3 * node + 2] = a2;
+            acc[0] += (double)a0 * (double)v.u[3 * node] + (double)a1 * (double)v.u[3 * node + 1] +
+                      (double)a2 * (double)v.u[3 * node + 2];
+        }
+    }
+    block_partial_at<1>(acc, io.out, io.stride, lb);
+}
+
+// U_k: p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = M r;
+// partials (r.u, r.r).  One thread per DOF (unit-stride vector traffic), the
+// three residual components of a node meet in LDS for u = M r; kUnroll DOF
+// groups per trip keep their loads in flight together.
+constexpr int kUpdBlock = 192;  // 64 nodes
+template <typename HT>
+__device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgState* st_next, PartIO io, int first,
+                                int lb, int nb) {
+    if (v.ctl->done) return;
+    double g[2], d[1];
+    sum_partials_at<2>(io.inU, io.stride, io.nbU, g);  // gamma_{k-1}, rr_{k-1}
+    sum_partials_at<1>(io.inA, io.stride, io.nbA, d);  // delta_{k-1}
+    double alpha, beta;
+    bool bad;
+    if (first) {
+        beta = 0.0;
+        alpha = g[0] / d[0];
+        bad = !(d[0] > 0.0);
+    } else {
+        beta = g[0] / st_prev->gamma;
+        const double den = d[0] - beta * g[0] / st_prev->alpha;
+        alpha = g[0] / den;
+        bad = !(den > 0.0) || !(g[0] > 0.0);
+    }
+    const bool lead = lb == 0 && threadIdx.x == 0;
+    if (bad) {
+        if (lead) {
+            v.ctl->done = 1;
+            v.ctl->breakdown = 1;
+            v.ctl->iter = st_prev->iter;
+            v.ctl->rr = g[1];
+        }
+        return;
+    }
+    if (lead) {
+        st_next->gamma = g[0];
+        st_next->alpha = alpha;
+        st_next->rr0 = st_prev->rr0;
+        st_next->tol2 = st_prev->tol2;
+        st_next->iter = st_prev->iter + 1;
+    }
+    const HT a = (HT)alpha, bt = (HT)beta;
+    constexpr int kUnroll = 4;
+    __shared__ HT rs[kUnroll][kUpdBlock];
+    double acc[2] = {0.0, 0.0};
+    const int c = threadIdx.x % 3, first3 = threadIdx.x - c;
+    const int n = v.n;
+    const int stride = nb * kUpdBlock;
+    for (int base = lb * kUpdBlock; base < n; base += kUnroll * stride) {
+        HT pp[kUnroll], ss[kUnroll], uu0[kUnroll], ww[kUnroll], rr0[kUnroll], xx[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            const int i = base + q * stride + threadIdx.x;
+            if (i < n) {
+                uu0[q] = v.u[i];
+                pp[q] = v.p[i];
+                ww[q] = v.w[i];
+                ss[q] = v.s[i];
+                rr0[q] = v.r[i];
+                xx[q] = v.x[i];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            const int i = base + q * stride + threadIdx.x;
+            HT rr = 0;
+            if (i < n) {
+                const HT pn = uu0[q] + bt * pp[q];
+                const HT sn = ww[q] + bt * ss[q];
+                v.p[i] = pn;
+                v.s[i] = sn;
+                v.x[i] = xx[q] + a * pn;
+                rr = rr0[q] - a * sn;
+                v.r[i] = rr;
+            }
+            rs[q][threadIdx.x] = rr;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            const int i = base + q * stride + threadIdx.x;
+            if (i < n) {
+                const HT* M = v.Minv + 3 * (size_t)i;  // row c of node i/3: Minv[9 (i/3) + 3 c] = Minv[3 i]
+                const HT rr = rs[q][threadIdx.x];
+                const HT un = M[0] * rs[q][first3] + M[1] * rs[q][first3 + 1] + M[2] * rs[q][first3 + 2];
+                v.u[i] = un;
+                acc[0] += (double)rr * (double)un;
+                acc[1] += (double)rr * (double)rr;
+            }
+        }
+        __syncthreads();
+    }
+    block_partial_at<2>(acc, io.out, io.stride, lb);
+}
+
+// ---- single-rollout kernels ----------------------------------------------
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(PcgView<HT> v, const double* b, double sign, PartIO io) {
+    pcg_init_body(v, b, sign, io, blockIdx.x, gridDim.x);
+}
+
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv_blk(PcgView<HT> v, PcgState* st, PartIO io, int first) {
+    pcg_spmv_body(v, st, io, first, blockIdx.x, gridDim.x);
+}
+
+template <typename HT>
+__global__ __launch_bounds__(kUpdBlock) void k_pcg_update(PcgView<HT> v, PcgState* st_prev, PcgState* st_next,
+                                                          PartIO io, int first) {
+    pcg_update_body(v, st_prev, st_next, io, first, blockIdx.x, gridDim.x);
+}
+
 // w = A u on scalar CSR rows (general masks), LPR lanes per row
 template <typename HT, int LPR>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* rowptr, const int* cols, const HT* vals,
-                                                     const HT* u, HT* w, PcgCtl* ctl, PcgState* st, PcgPart part,
-                                                     int par, int nbU, int first, double rtol) {
-    if (spmv_prologue(ctl, st, part.u[par], nbU, first, rtol)) return;
+                                                     const HT* u, HT* w, PcgCtl* ctl, PcgState* st, PartIO io,
+                                                     int first, double rtol) {
+    if (spmv_prologue(ctl, st, io.inU, io.stride, io.nbU, first, rtol, blockIdx.x == 0 && threadIdx.x == 0)) return;
     double v[1] = {0.0};
     const int sub = threadIdx.x % LPR;
     const int rows_per_block = blockDim.x / LPR;
@@ -925,148 +1135,72 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* rowptr, c
             v[0] += (double)acc * (double)u[row];
         }
     }
-    block_partial<1>(v, part.a);
+    block_partial_at<1>(v, io.out, io.stride, blockIdx.x);
 }
 
-// w = A u on the sliced node-block layout: one lane per node (three rows),
-// one wavefront per slice of 64 nodes; values and column indices are read
-// with unit stride across the wavefront.
+// ---- batched kernels: one launch over many rollouts ---------------------
+// A rollout keeps its own partial buffer (gamma/rr pair for each parity,
+// then delta; stride kMaxRedBlocks) and its own block count per kernel, so
+// its sums are formed in the same order alone and inside a batch.
 template <typename HT>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv_blk(int N, int nslices, const int* sbase, const int* bcol,
-                                                         const HT* vals, const HT* u, HT* w, PcgCtl* ctl,
-                                                         PcgState* st, PcgPart part, int par, int nbU, int first,
-                                                         double rtol) {
-    if (spmv_prologue(ctl, st, part.u[par], nbU, first, rtol)) return;
-    double v[1] = {0.0};
-    const int lane = threadIdx.x & (kSlice - 1), wv = threadIdx.x / kSlice;
-    constexpr int kPer = kBlock / kSlice;  // slices per block
-    const int ngroups = (nslices + kPer - 1) / kPer;
-    const int per_xcd = (ngroups + kNumXcd - 1) / kNumXcd;
-    for (int t = blockIdx.x; t < kNumXcd * per_xcd; t += gridDim.x) {
-        const int xcd = t % kNumXcd, local = t / kNumXcd;
-        const int g = xcd * per_xcd + local;
-        const int sl = g * kPer + wv;
-        if (local >= per_xcd || g >= ngroups || sl >= nslices) continue;  // uniform per wavefront
-        const int base = sbase[sl];
-        const int width = (sbase[sl + 1] - base) / kSlice;
-        const int* cp = bcol + base + lane;
-        const HT* vp = vals + 9LL * base + lane;
-        HT a0 = 0, a1 = 0, a2 = 0;
-        for (int j = 0; j < width; ++j) {
-            const int nb = cp[kSlice * j];
-            const HT x0 = u[3 * nb], x1 = u[3 * nb + 1], x2 = u[3 * nb + 2];
-            const HT* bb = vp + (long long)kSlice * 9 * j;
-            a0 += bb[0] * x0 + bb[kSlice] * x1 + bb[2 * kSlice] * x2;
-            a1 += bb[3 * kSlice] * x0 + bb[4 * kSlice] * x1 + bb[5 * kSlice] * x2;
-            a2 += bb[6 * kSlice] * x0 + bb[7 * kSlice] * x1 + bb[8 * kSlice] * x2;
-        }
-        const int node = sl * kSlice + lane;
-        if (node < N) {
-            w[3 * node] = a0;
-            w[3 * node + 1] = a1;
-            w[3 * node + 2] = a2;
-            v[0] += (double)a0 * (double)u[3 * node] + (double)a1 * (double)u[3 * node + 1] +
-                    (double)a2 * (double)u[3 * node + 2];
-        }
+struct BMember {
+    PcgView<HT> v;
+    const double* rhs;  // b of A x = -b
+    double* part;
+    int nbI, nbA, nbU, pad;
+};
+
+// j-th rollout of this launch owns launch blocks [tab[j], tab[j+1])
+__device__ __forceinline__ int find_member(const int* tab, int cnt) {
+    int lo = 0, hi = cnt - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
     }
-    block_partial<1>(v, part.a);
+    return lo;
 }
 
-// U_k: p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = M r;
-// partU[k%2] = (r.u, r.r).  One thread per DOF (unit-stride vector traffic);
-// the three residual components of a node meet in LDS for u = M r.
-constexpr int kUpdBlock = 192;  // 64 nodes
 template <typename HT>
-__global__ __launch_bounds__(kUpdBlock) void k_pcg_update(int n, const HT* Minv, HT* x, HT* r, HT* u, HT* p, HT* s,
-                                                          const HT* w, PcgCtl* ctl, PcgState* st_prev,
-                                                          PcgState* st_next, PcgPart part, int par, int nbU, int nbA,
-                                                          int first) {
-    if (ctl->done) return;
-    double g[2], d[1];
-    sum_partials<2>(part.u[par ^ 1], nbU, g);  // gamma_{k-1}, rr_{k-1}
-    sum_partials<1>(part.a, nbA, d);           // delta_{k-1}
-    double alpha, beta;
-    bool bad;
-    if (first) {
-        beta = 0.0;
-        alpha = g[0] / d[0];
-        bad = !(d[0] > 0.0);
-    } else {
-        beta = g[0] / st_prev->gamma;
-        const double den = d[0] - beta * g[0] / st_prev->alpha;
-        alpha = g[0] / den;
-        bad = !(den > 0.0) || !(g[0] > 0.0);
+__global__ __launch_bounds__(kBlock) void k_b_pcg_init(const BMember<HT>* mem, const int* idx, const int* tab,
+                                                       int cnt) {
+    const int j = find_member(tab, cnt);
+    const BMember<HT>& m = mem[idx[j]];
+    const int lb = blockIdx.x - tab[j], nb = tab[j + 1] - tab[j];
+    if (lb == 0 && threadIdx.x == 0) {
+        m.v.ctl->done = 0;
+        m.v.ctl->breakdown = 0;
+        m.v.ctl->iter = 0;
+        m.v.ctl->rr = 0.0;
     }
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (bad) {
-        if (lead) {
-            ctl->done = 1;
-            ctl->breakdown = 1;
-            ctl->iter = st_prev->iter;
-            ctl->rr = g[1];
-        }
-        return;
-    }
-    if (lead) {
-        st_next->gamma = g[0];
-        st_next->alpha = alpha;
-        st_next->rr0 = st_prev->rr0;
-        st_next->tol2 = st_prev->tol2;
-        st_next->iter = st_prev->iter + 1;
-    }
-    const HT a = (HT)alpha, bt = (HT)beta;
-    // kUnroll DOF groups per thread and trip: all their loads are issued
-    // before any result is needed (memory-level parallelism)
-    constexpr int kUnroll = 4;
-    __shared__ HT rs[kUnroll][kUpdBlock];
-    double v[2] = {0.0, 0.0};
-    const int c = threadIdx.x % 3, first3 = threadIdx.x - c;
-    const int stride = gridDim.x * kUpdBlock;
-    for (int base = blockIdx.x * kUpdBlock; base < n; base += kUnroll * stride) {
-        HT pp[kUnroll], ss[kUnroll], uu0[kUnroll], ww[kUnroll], rr0[kUnroll], xx[kUnroll];
-#pragma unroll
-        for (int q = 0; q < kUnroll; ++q) {
-            const int i = base + q * stride + threadIdx.x;
-            if (i < n) {
-                uu0[q] = u[i];
-                pp[q] = p[i];
-                ww[q] = w[i];
-                ss[q] = s[i];
-                rr0[q] = r[i];
-                xx[q] = x[i];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kUnroll; ++q) {
-            const int i = base + q * stride + threadIdx.x;
-            HT rr = 0;
-            if (i < n) {
-                const HT pn = uu0[q] + bt * pp[q];
-                const HT sn = ww[q] + bt * ss[q];
-                p[i] = pn;
-                s[i] = sn;
-                x[i] = xx[q] + a * pn;
-                rr = rr0[q] - a * sn;
-                r[i] = rr;
-            }
-            rs[q][threadIdx.x] = rr;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < kUnroll; ++q) {
-            const int i = base + q * stride + threadIdx.x;
-            if (i < n) {
-                const HT* M = Minv + 3 * (size_t)i;  // row c of node i/3: Minv[9 (i/3) + 3 c] = Minv[3 i]
-                const HT rr = rs[q][threadIdx.x];
-                const HT un = M[0] * rs[q][first3] + M[1] * rs[q][first3 + 1] + M[2] * rs[q][first3 + 2];
-                u[i] = un;
-                v[0] += (double)rr * (double)un;
-                v[1] += (double)rr * (double)rr;
-            }
-        }
-        __syncthreads();
-    }
-    block_partial<2>(v, part.u[par]);
+    const PartIO io{m.part, kMaxRedBlocks, nullptr, 0, nullptr, 0};
+    pcg_init_body(m.v, m.rhs, -1.0, io, lb, nb);
+}
+
+// par: parity of the iteration; first: A_0 (consumes the init partials)
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_b_pcg_spmv(const BMember<HT>* mem, const int* idx, const int* tab, int cnt,
+                                                       int par, int first) {
+    const int j = find_member(tab, cnt);
+    const BMember<HT>& m = mem[idx[j]];
+    const int lb = blockIdx.x - tab[j], nb = tab[j + 1] - tab[j];
+    const PartIO io{m.part + 4 * kMaxRedBlocks, kMaxRedBlocks, m.part + 2 * par * kMaxRedBlocks,
+                    first ? m.nbI : m.nbU, nullptr, 0};
+    pcg_spmv_body(m.v, m.v.st + par, io, first, lb, nb);
+}
+
+template <typename HT>
+__global__ __launch_bounds__(kUpdBlock) void k_b_pcg_update(const BMember<HT>* mem, const int* idx, const int* tab,
+                                                            int cnt, int par, int first) {
+    const int j = find_member(tab, cnt);
+    const BMember<HT>& m = mem[idx[j]];
+    const int lb = blockIdx.x - tab[j], nb = tab[j + 1] - tab[j];
+    const PartIO io{m.part + 2 * par * kMaxRedBlocks, kMaxRedBlocks, m.part + 2 * (par ^ 1) * kMaxRedBlocks,
+                    first ? m.nbI : m.nbU, m.part + 4 * kMaxRedBlocks, m.nbA};
+    pcg_update_body(m.v, m.v.st + (par ^ 1), m.v.st + par, io, first, lb, nb);
 }
 
 // ===========================================================================
@@ -1188,6 +1322,12 @@ struct sors_ctx {
     DevMem rowptr, cols, diag, rank, fullmap, vals, Minv;
     DevMem pr, pu, pp, ps, pw, px;
     DevMem partials, scal, pcgst, pcgctl, pcgpart, flag;
+    // a serving loop points the result scalars, the element flag and the PCG
+    // outcome record of its rollouts into one slot array it reads at once
+    double* scal_ext = nullptr;
+    int* flag_ext = nullptr;
+    double* S() const { return scal_ext ? scal_ext : scal.as<double>(); }
+    int* F() const { return flag_ext ? flag_ext : flag.as<int>(); }
     double* h_pin = nullptr;   // pinned readback area
     HostPattern pat;
     int pat_dtnz = -1;
@@ -1293,7 +1433,7 @@ ElemArgs elem_args(sors_ctx* c, const double* q) {
     a.rank = c->pat.node_granular ? c->rank.as<uint8_t>() : nullptr;
     a.fullmap = c->pat.node_granular ? nullptr : c->fullmap.as<int>();
     a.sbase = c->block() ? c->sbase.as<int>() : nullptr;
-    a.flag = c->flag.as<int>();
+    a.flag = c->F();
     return a;
 }
 
@@ -1346,7 +1486,7 @@ int read_flag(sors_ctx* c) {
     return f;
 }
 
-void clear_flag(sors_ctx* c) { fill<int>(c->flag.as<int>(), 1, 0, c->stream); }
+void clear_flag(sors_ctx* c) { fill<int>(c->F(), 1, 0, c->stream); }
 
 // gradient (FP64) and, when HT != void, the assembled Hessian; returns ||g||^2 in scal[S_GG]
 template <typename HT>
@@ -1372,8 +1512,8 @@ void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project
     NodeArgs na = node_args(c, q, dt);
     na.vals = hessian ? c->vals.p : nullptr;
     const int grid = red_grid(c->n, kBlock);
-    k_node_terms<HT><<<grid, kBlock, 0, c->stream>>>(na, c->scal.as<double>() + S_GG, c->red());
-    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), grid, c->scal.as<double>() + S_GG);
+    k_node_terms<HT><<<grid, kBlock, 0, c->stream>>>(na, c->S() + S_GG, c->red());
+    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), grid, c->S() + S_GG);
     HIPX(hipGetLastError());
 }
 
@@ -1410,38 +1550,59 @@ PcgGrid pcg_grid(sors_ctx* c) {
     return g;
 }
 
+template <typename HT>
+PcgView<HT> pcg_view(sors_ctx* c, double rtol) {
+    PcgView<HT> v;
+    v.N = c->N;
+    v.n = c->n;
+    v.nslices = c->pat.nslices;
+    v.sbase = c->sbase.as<int>();
+    v.bcol = c->bcol.as<int>();
+    v.vals = (const HT*)c->vals.p;
+    v.Minv = (const HT*)c->Minv.p;
+    v.x = (HT*)c->px.p;
+    v.r = (HT*)c->pr.p;
+    v.u = (HT*)c->pu.p;
+    v.p = (HT*)c->pp.p;
+    v.s = (HT*)c->ps.p;
+    v.w = (HT*)c->pw.p;
+    v.ctl = c->pcgctl.as<PcgCtl>();
+    v.st = c->pcgst.as<PcgState>();
+    v.rtol = rtol;
+    return v;
+}
+
 // A_k (first: A_0) reading partU[par]; nbU = blocks of the producer of partU
 // (with e0/e1: launched through hipExtLaunchKernelGGL, which records the two
 // events around this dispatch itself)
 template <typename HT>
-void launch_spmv(sors_ctx* c, const PcgGrid& g, const HT* u, HT* w, PcgCtl* ctl, PcgState* st, int par, int nbU,
-                 int first, double rtol, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+void launch_spmv(sors_ctx* c, const PcgGrid& g, const PcgView<HT>& v, int par, int nbU, int first,
+                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const PcgPart part = c->pcg_part();
+    const PartIO io{part.a, kMaxRedBlocks, part.u[par], nbU, nullptr, 0};
+    PcgState* st = v.st + par;
     if (c->block() && e0)
-        hipExtLaunchKernelGGL(k_pcg_spmv_blk<HT>, dim3(g.gs), dim3(kBlock), 0, c->stream, e0, e1, 0, c->N,
-                              c->pat.nslices, (const int*)c->sbase.as<int>(), (const int*)c->bcol.as<int>(),
-                              (const HT*)c->vals.p, u, w, ctl, st, part, par, nbU, first, rtol);
+        hipExtLaunchKernelGGL(k_pcg_spmv_blk<HT>, dim3(g.gs), dim3(kBlock), 0, c->stream, e0, e1, 0, v, st, io,
+                              first);
     else if (c->block())
-        k_pcg_spmv_blk<HT><<<g.gs, kBlock, 0, c->stream>>>(c->N, c->pat.nslices, c->sbase.as<int>(),
-                                                           c->bcol.as<int>(), (const HT*)c->vals.p, u, w, ctl, st,
-                                                           part, par, nbU, first, rtol);
+        k_pcg_spmv_blk<HT><<<g.gs, kBlock, 0, c->stream>>>(v, st, io, first);
     else if (e0)
         hipExtLaunchKernelGGL(k_pcg_spmv<HT, 8>, dim3(g.gs), dim3(kBlock), 0, c->stream, e0, e1, 0, c->n,
-                              (const int*)c->rowptr.as<int>(), (const int*)c->cols.as<int>(), (const HT*)c->vals.p, u,
-                              w, ctl, st, part, par, nbU, first, rtol);
+                              (const int*)c->rowptr.as<int>(), (const int*)c->cols.as<int>(), (const HT*)c->vals.p,
+                              (const HT*)v.u, v.w, v.ctl, st, io, first, v.rtol);
     else
         k_pcg_spmv<HT, 8><<<g.gs, kBlock, 0, c->stream>>>(c->n, c->rowptr.as<int>(), c->cols.as<int>(),
-                                                          (const HT*)c->vals.p, u, w, ctl, st, part, par, nbU, first,
-                                                          rtol);
+                                                          (const HT*)c->vals.p, v.u, v.w, v.ctl, st, io, first,
+                                                          v.rtol);
 }
 
-// U_k reading partU[par ^ 1] (produced with nbU blocks) and partA
+// U_k reading partU[par ^ 1] (produced with nbU blocks) and partA; writes
+// partU[par] and st[par]
 template <typename HT>
-void launch_update(sors_ctx* c, const PcgGrid& g, PcgCtl* ctl, PcgState* st_prev, PcgState* st_next, int par,
-                   int nbU, int first) {
-    k_pcg_update<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(c->n, (const HT*)c->Minv.p, (HT*)c->px.p, (HT*)c->pr.p,
-                                                        (HT*)c->pu.p, (HT*)c->pp.p, (HT*)c->ps.p, (const HT*)c->pw.p,
-                                                        ctl, st_prev, st_next, c->pcg_part(), par, nbU, g.gs, first);
+void launch_update(sors_ctx* c, const PcgGrid& g, const PcgView<HT>& v, int par, int nbU, int first) {
+    const PcgPart part = c->pcg_part();
+    const PartIO io{part.u[par], kMaxRedBlocks, part.u[par ^ 1], nbU, part.a, g.gs};
+    k_pcg_update<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(v, v.st + (par ^ 1), v.st + par, io, first);
 }
 
 // PCG on the assembled matrix: solves A x = sign*b; x in c->px (HT).
@@ -1452,12 +1613,8 @@ void launch_update(sors_ctx* c, const PcgGrid& g, PcgCtl* ctl, PcgState* st_prev
 // around each SpMV.
 template <typename HT>
 long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_iter, bool* breakdown) {
-    const int N = c->N;
-    HT *x = (HT*)c->px.p, *r = (HT*)c->pr.p, *u = (HT*)c->pu.p, *p = (HT*)c->pp.p, *s = (HT*)c->ps.p,
-       *w = (HT*)c->pw.p;
-    PcgState* st = c->pcgst.as<PcgState>();
-    PcgCtl* ctl = c->pcgctl.as<PcgCtl>();
-    const HT* Minv = (const HT*)c->Minv.p;
+    const PcgView<HT> v = pcg_view<HT>(c, rtol);
+    PcgCtl* ctl = v.ctl;
     const PcgGrid g = pcg_grid(c);
     const int chunk = std::max(2, (c->opt.pcg_check_every + 1) / 2 * 2);
     const int gi = std::is_same<HT, float>::value ? 1 : 0;
@@ -1473,18 +1630,22 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
     // iteration k >= 2 reads from st[(k-1)%2] and writes st[k%2]
     auto iteration = [&](int k, int j, bool timed) {
         const int par = k & 1;
-        launch_update<HT>(c, g, ctl, st + (par ^ 1), st + par, par, g.gu, 0);
+        launch_update<HT>(c, g, v, par, g.gu, 0);
         if (timed)
-            launch_spmv<HT>(c, g, u, w, ctl, st + par, par, g.gu, 0, rtol, ev[2 * j], ev[2 * j + 1]);
+            launch_spmv<HT>(c, g, v, par, g.gu, 0, ev[2 * j], ev[2 * j + 1]);
         else
-            launch_spmv<HT>(c, g, u, w, ctl, st + par, par, g.gu, 0, rtol);
+            launch_spmv<HT>(c, g, v, par, g.gu, 0);
     };
     PcgCtl* hctl = (PcgCtl*)c->h_pin;
     fill<int>((int*)ctl, sizeof(PcgCtl) / sizeof(int), 0, c->stream);
-    k_pcg_init<HT><<<g.gn, kBlock, 0, c->stream>>>(N, b, sign, Minv, x, r, u, p, s, c->pcg_part());
-    launch_spmv<HT>(c, g, u, w, ctl, st, 0, g.gn, 1, rtol);            // A_0
-    launch_update<HT>(c, g, ctl, st, st + 1, 1, g.gn, 1);              // U_1
-    launch_spmv<HT>(c, g, u, w, ctl, st + 1, 1, g.gu, 0, rtol);        // A_1
+    {
+        const PcgPart part = c->pcg_part();
+        const PartIO io{part.u[0], kMaxRedBlocks, nullptr, 0, nullptr, 0};
+        k_pcg_init<HT><<<g.gn, kBlock, 0, c->stream>>>(v, b, sign, io);
+    }
+    launch_spmv<HT>(c, g, v, 0, g.gn, 1);      // A_0
+    launch_update<HT>(c, g, v, 1, g.gn, 1);    // U_1
+    launch_spmv<HT>(c, g, v, 1, g.gu, 0);      // A_1
     HIPX(hipGetLastError());
     int launched = 1;
     bool first_chunk = true, done = false;
@@ -1533,55 +1694,62 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
 }
 
 double read_scalar(sors_ctx* c, int slot) {
-    HIPX(hipMemcpyAsync(c->h_pin, c->scal.as<double>() + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipMemcpyAsync(c->h_pin, c->S() + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     sync(c);
     return c->h_pin[0];
 }
 
 void read_scalars(sors_ctx* c, double* out, int first, int count) {
-    HIPX(hipMemcpyAsync(c->h_pin, c->scal.as<double>() + first, sizeof(double) * count, hipMemcpyDeviceToHost,
+    HIPX(hipMemcpyAsync(c->h_pin, c->S() + first, sizeof(double) * count, hipMemcpyDeviceToHost,
                         c->stream));
     sync(c);
     for (int i = 0; i < count; ++i) out[i] = c->h_pin[i];
 }
 
-// line-search energy difference E(q + alpha dx) - E(q) (FP64, device)
-double energy_delta(sors_ctx* c, const double* q, const double* dx, double alpha, double dt) {
+// energy kernels into S[S_EELEM], S[S_ENODE]: with dx, the line-search
+// difference E(q + alpha dx) - E(q); without, E(q) (FP64, device)
+void enqueue_energy(sors_ctx* c, const double* q, const double* dx, double alpha, double dt) {
+    const int diff = dx != nullptr;
     ElemArgs a = elem_args(c, q);
     a.dq = dx;
     a.alpha = alpha;
     const int ge = red_grid(c->E, kBlock);
     if (c->ed == 4)
-        k_elem_energy<4><<<ge, kBlock, 0, c->stream>>>(a, 1, c->scal.as<double>() + S_EELEM, c->red());
+        k_elem_energy<4><<<ge, kBlock, 0, c->stream>>>(a, diff, c->S() + S_EELEM, c->red());
     else
-        k_elem_energy<8><<<ge, kBlock, 0, c->stream>>>(a, 1, c->scal.as<double>() + S_EELEM, c->red());
-    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), ge, c->scal.as<double>() + S_EELEM);
+        k_elem_energy<8><<<ge, kBlock, 0, c->stream>>>(a, diff, c->S() + S_EELEM, c->red());
+    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), ge, c->S() + S_EELEM);
     NodeArgs na = node_args(c, q, dt);
     const int gnn = red_grid(c->n, kBlock);
-    k_node_energy<<<gnn, kBlock, 0, c->stream>>>(na, dx, alpha, 1, c->scal.as<double>() + S_ENODE, c->red());
-    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gnn, c->scal.as<double>() + S_ENODE);
+    k_node_energy<<<gnn, kBlock, 0, c->stream>>>(na, dx, alpha, diff, c->S() + S_ENODE, c->red());
+    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gnn, c->S() + S_ENODE);
     HIPX(hipGetLastError());
+}
+
+double energy_delta(sors_ctx* c, const double* q, const double* dx, double alpha, double dt) {
+    enqueue_energy(c, q, dx, alpha, dt);
     double v[2];
     read_scalars(c, v, S_EELEM, 2);
     return v[0] + v[1];
 }
 
 double energy_abs(sors_ctx* c, const double* q, double dt) {
-    ElemArgs a = elem_args(c, q);
-    const int ge = red_grid(c->E, kBlock);
-    if (c->ed == 4)
-        k_elem_energy<4><<<ge, kBlock, 0, c->stream>>>(a, 0, c->scal.as<double>() + S_EELEM, c->red());
-    else
-        k_elem_energy<8><<<ge, kBlock, 0, c->stream>>>(a, 0, c->scal.as<double>() + S_EELEM, c->red());
-    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), ge, c->scal.as<double>() + S_EELEM);
-    NodeArgs na = node_args(c, q, dt);
-    const int gnn = red_grid(c->n, kBlock);
-    k_node_energy<<<gnn, kBlock, 0, c->stream>>>(na, nullptr, 0.0, 0, c->scal.as<double>() + S_ENODE, c->red());
-    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gnn, c->scal.as<double>() + S_ENODE);
-    HIPX(hipGetLastError());
+    enqueue_energy(c, q, nullptr, 0.0, dt);
     double v[2];
     read_scalars(c, v, S_EELEM, 2);
     return v[0] + v[1];
+}
+
+// step dots into S[S_DD], S[S_GD] and the step dx = x (FP64)
+template <typename HT>
+void enqueue_step_dots(sors_ctx* c, const double* g, double* dx) {
+    k_to_f64<HT><<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, (const HT*)c->px.p, dx);
+    const int gd = red_grid(c->n, kBlock);
+    k_step_dots<HT><<<gd, kBlock, 0, c->stream>>>(c->n, (const HT*)c->px.p, g, c->S() + S_DD, c->S() + S_GD,
+                                                   c->red());
+    static_assert(S_GD == S_DD + 1, "step dots are finalized as one pair");
+    k_finalize<2><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gd, c->S() + S_DD);
+    HIPX(hipGetLastError());
 }
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -1618,13 +1786,7 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
         double eta = o.pcg_rtol;
         if (o.g_tol > 0 && stt->gnorm > 0) eta = std::min(0.1, std::max(eta, 0.5 * o.g_tol / stt->gnorm));
         stt->pcg_iters += pcg<HT>(c, g, -1.0, eta, o.pcg_max_iter, &brk);
-        k_to_f64<HT><<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, (const HT*)c->px.p, dx);
-        const int gd = red_grid(c->n, kBlock);
-        k_step_dots<HT><<<gd, kBlock, 0, c->stream>>>(c->n, (const HT*)c->px.p, g, c->scal.as<double>() + S_DD,
-                                                       c->scal.as<double>() + S_GD, c->red());
-        static_assert(S_GD == S_DD + 1, "step dots are finalized as one pair");
-        k_finalize<2><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gd, c->scal.as<double>() + S_DD);
-        HIPX(hipGetLastError());
+        enqueue_step_dots<HT>(c, g, dx);
         double dots[2];
         read_scalars(c, dots, S_DD, 2);
         stt->ms_solve += ms_since(t1);
@@ -1704,6 +1866,527 @@ int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
     sync(c);
     st->ms_total = ms_since(t0);
     return SORS_OK;
+}
+
+// ===========================================================================
+// Serving loop (no counterpart in the reference): continuous batching of
+// independent rollouts.  Requests wait in arrival order; the head is
+// admitted when a slot and HBM budget are free and handed to a prep thread
+// (context, pattern, upload); resident rollouts advance in ticks.  In one
+// tick every rollout outside its linear solve enqueues one phase of Newton
+// work on its own stream, and all rollouts inside it take pcg_chunk
+// iterations of one batched launch sequence on the server stream; then the
+// host reads every rollout's scalars with one copy and moves each to its
+// next phase, following minimize() exactly.
+
+struct SlotRec {
+    double scal[8];
+    PcgCtl ctl;
+    int flag, pad;
+};
+static_assert(S_COUNT <= 8, "slot scalars");
+
+struct Request {
+    long long id = 0;
+    int N = 0, E = 0, ed = 0, material = 0, scheme = 0, method = 0;
+    double grav[3] = {0, 0, 0}, damping = 0;
+    std::vector<double> rest, youngs, poisson, density, bc_value, v0;
+    std::vector<int> conn, bc_mask;
+    double dt = 0;
+    int steps = 0;
+    using clock = std::chrono::steady_clock;
+    clock::time_point t_submit, t_admit, t_resident;
+    double est_bytes = 0, dev_bytes = 0;
+    sors_ctx* ctx = nullptr;
+    int status = SORS_OK;
+    std::string err;
+    std::vector<double> q;
+    bool done = false;
+    sors_request_stats stats{};
+};
+
+struct Member {
+    Request* r = nullptr;
+    sors_ctx* c = nullptr;
+    int slot = -1;
+    enum Phase { ASSEMBLE, PCG_START, PCG_RUN, POST, ENERGY } phase = ASSEMBLE;
+    bool new_step = true, busy = false, brk = false;
+    int step = 0, it = 0, bt = 0;
+    long long launched = 0;
+    double eta = 0, dxn = 0, gdx = 0, alpha = 1;
+    sors_stats st{};
+    PcgGrid g{};
+};
+
+double ctx_bytes(const sors_ctx* c) {
+    const DevMem* all[] = {&c->conn,   &c->rest,  &c->mu,      &c->lam,   &c->mass,    &c->mask,    &c->bcval,
+                           &c->q,      &c->qprev, &c->vprev,   &c->gprev, &c->gtmp,    &c->gint,    &c->g,
+                           &c->fext,   &c->dxd,   &c->qtmp,    &c->rowptr, &c->cols,   &c->diag,    &c->rank,
+                           &c->fullmap, &c->vals, &c->Minv,    &c->pr,    &c->pu,      &c->pp,      &c->ps,
+                           &c->pw,     &c->px,    &c->partials, &c->scal, &c->pcgst,   &c->pcgctl,  &c->pcgpart,
+                           &c->flag,   &c->sbase, &c->bcol,    &c->selfslot};
+    double b = 0;
+    for (const DevMem* d : all) b += (double)d->bytes;
+    return b;
+}
+
+// device bytes of a rollout before its pattern exists (an upper estimate:
+// node-block slices padded to 24 (tet) / 32 (hex) blocks per node)
+double estimate_bytes(int N, int E, int ed) {
+    const double n = 3.0 * N, nblk = (double)N * (ed == 4 ? 24 : 32);
+    return n * (19 * 8 + 4) + 72.0 * N + nblk * (9 * 8 + 4 + 9 * 4) + (double)E * (ed * 4 + 16 + ed * ed) +
+           8.0 * (5 * kMaxRedBlocks + 4 * kMaxRedBlocks) + (1 << 20);
+}
+
+}  // namespace
+
+struct sors_server {
+    sors_server_options opt{};
+    int device = 0;
+    std::string err;
+    // shared state (mu)
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done, cv_prep;
+    std::deque<Request*> pending;
+    std::deque<Request*> ready;           // resident, not yet in a tick
+    std::deque<std::function<void()>> tasks;    // prep / cleanup
+    std::unordered_map<long long, std::unique_ptr<Request>> reqs;
+    long long next_id = 1;
+    int admitted = 0;                           // in prep, ready, active or cleanup
+    double reserved = 0;
+    bool stop = false, draining = false;
+    sors_server_stats stats{};
+    // worker-owned
+    std::vector<Member> active;
+    std::vector<int> free_slots;
+    DevMem slots, bmem;
+    void* h_stage = nullptr;                    // pinned: member table + launch tables
+    SlotRec* h_slots = nullptr;           // pinned readback
+    hipStream_t stream = nullptr;
+    long long ticks = 0, launches = 0;         // published to stats under mu
+    std::thread worker;
+    std::vector<std::thread> prep;
+};
+
+namespace {
+
+void srv_try_admit(sors_server* s) {  // mu held
+    while (!s->pending.empty() && s->admitted < s->opt.max_active && !s->stop) {
+        Request* r = s->pending.front();
+        if (s->admitted > 0 && s->reserved + r->est_bytes > s->opt.hbm_budget_bytes) break;  // arrival order
+        s->pending.pop_front();
+        s->admitted++;
+        s->reserved += r->est_bytes;
+        r->t_admit = Request::clock::now();
+        s->tasks.push_back([s, r] {
+            sors_system_desc d{};
+            d.n_nodes = r->N;
+            d.n_elems = r->E;
+            d.nodes_per_elem = r->ed;
+            d.rest = r->rest.data();
+            d.conn = r->conn.data();
+            d.youngs = r->youngs.data();
+            d.poisson = r->poisson.data();
+            d.density = r->density.data();
+            d.material = r->material;
+            d.bc_mask = r->bc_mask.empty() ? nullptr : r->bc_mask.data();
+            d.bc_value = r->bc_value.empty() ? nullptr : r->bc_value.data();
+            for (int k = 0; k < 3; ++k) d.grav[k] = r->grav[k];
+            d.damping_alpha = r->damping;
+            d.scheme = r->scheme;
+            d.method = r->method;
+            sors_ctx* c = nullptr;
+            int rc = sors_create(&d, &c);
+            std::string msg = rc ? sors_last_error(nullptr) : "";
+            if (rc == SORS_OK) {
+                c->opt = s->opt.solver;
+                try {
+                    if (!r->v0.empty()) upload(c->vprev, r->v0.data(), (size_t)c->n, c->stream);
+                    ensure_pattern(c, r->dt);
+                    sync(c);
+                } catch (const HipError& e) {
+                    rc = SORS_ERR_HIP;
+                    msg = e.msg;
+                }
+            }
+            std::lock_guard<std::mutex> lk(s->mu);
+            r->t_resident = Request::clock::now();
+            if (rc != SORS_OK) {
+                if (c) sors_destroy(c);
+                r->status = rc;
+                r->err = msg;
+                s->reserved -= r->est_bytes;
+                s->admitted--;
+                s->stats.failed++;
+                r->done = true;
+                s->cv_done.notify_all();
+                s->cv_work.notify_all();
+                s->cv_prep.notify_all();
+                srv_try_admit(s);
+                return;
+            }
+            r->ctx = c;
+            r->dev_bytes = ctx_bytes(c);
+            s->reserved += r->dev_bytes - r->est_bytes;  // actual size from here on
+            s->stats.resident_bytes = s->reserved;
+            s->stats.peak_resident_bytes = std::max(s->stats.peak_resident_bytes, s->reserved);
+            s->ready.push_back(r);
+            s->cv_work.notify_all();
+        });
+        s->cv_prep.notify_one();
+    }
+}
+
+// hand a finished (or failed) rollout to a prep thread: read q, free it
+void srv_retire(sors_server* s, Member& m, int status, const std::string& msg) {
+    Request* r = m.r;
+    sors_ctx* c = m.c;
+    c->scal_ext = nullptr;
+    c->flag_ext = nullptr;
+    s->free_slots.push_back(m.slot);
+    r->stats.steps = m.step;
+    r->stats.newton_iters = m.st.newton_iters;
+    r->stats.pcg_iters = m.st.pcg_iters;
+    r->stats.backtracks = m.st.backtracks;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->tasks.push_front([s, r, c, status, msg] {
+        int rc = status;
+        std::string e = msg;
+        if (rc == SORS_OK) {
+            r->q.resize((size_t)c->n);
+            if (sors_get_positions(c, r->q.data()) != SORS_OK) {
+                rc = SORS_ERR_HIP;
+                e = c->err;
+            }
+        }
+        sors_destroy(c);
+        std::lock_guard<std::mutex> lk2(s->mu);
+        r->ctx = nullptr;
+        r->status = rc;
+        r->err = e;
+        const auto now = Request::clock::now();
+        r->stats.status = rc;
+        r->stats.latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
+        r->stats.queue_ms = std::chrono::duration<double, std::milli>(r->t_admit - r->t_submit).count();
+        r->stats.prep_ms = std::chrono::duration<double, std::milli>(r->t_resident - r->t_admit).count();
+        r->stats.device_bytes = r->dev_bytes;
+        s->reserved -= r->dev_bytes;
+        s->stats.resident_bytes = s->reserved;
+        s->admitted--;
+        if (rc == SORS_OK)
+            s->stats.completed++;
+        else
+            s->stats.failed++;
+        r->done = true;
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+        s->cv_prep.notify_all();
+        srv_try_admit(s);
+    });
+    s->cv_prep.notify_one();
+}
+
+void srv_prep_loop(sors_server* s) {
+    (void)hipSetDevice(s->device);
+    for (;;) {
+        std::function<void()> t;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_prep.wait(lk, [&] { return !s->tasks.empty() || (s->stop && s->admitted == 0); });
+            if (s->tasks.empty()) return;
+            t = std::move(s->tasks.front());
+            s->tasks.pop_front();
+        }
+        t();
+    }
+}
+
+// one tick over the resident rollouts
+template <typename HT>
+void srv_tick(sors_server* s) {
+    const sors_options& o = s->opt.solver;
+    std::vector<int> starters, runners;
+    // 1. per-rollout phase work on the rollouts' streams
+    for (int i = 0; i < (int)s->active.size(); ++i) {
+        Member& m = s->active[i];
+        sors_ctx* c = m.c;
+        const double dt = m.r->dt;
+        double* q = c->q.as<double>();
+        m.busy = false;
+        switch (m.phase) {
+            case Member::ASSEMBLE:
+                if (m.new_step) {
+                    if (o.predict)
+                        k_predict<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
+                            c->n, dt, c->qprev.as<double>(), c->vprev.as<double>(),
+                            c->has_mask ? c->mask.as<int>() : nullptr, c->bcval.as<double>(), q);
+                    m.new_step = false;
+                }
+                clear_flag(c);
+                assemble<HT>(c, q, dt, true, o.project_spd);
+                build_preconditioner<HT>(c);
+                m.busy = true;
+                break;
+            case Member::POST:
+                enqueue_step_dots<HT>(c, c->g.as<double>(), c->dxd.as<double>());
+                if (c->method == SORS_MINIMIZE_NEWTON_LINESEARCH) enqueue_energy(c, q, c->dxd.as<double>(), 1.0, dt);
+                m.busy = true;
+                break;
+            case Member::ENERGY:
+                enqueue_energy(c, q, c->dxd.as<double>(), m.alpha, dt);
+                m.busy = true;
+                break;
+            case Member::PCG_START:
+                starters.push_back(i);
+                runners.push_back(i);
+                break;
+            case Member::PCG_RUN:
+                runners.push_back(i);
+                break;
+        }
+    }
+    // 2. batched PCG on the server stream
+    const int chunk = std::max(2, (s->opt.pcg_chunk + 1) / 2 * 2);
+    if (!runners.empty()) {
+        const int M = s->opt.max_active;
+        BMember<HT>* hm = (BMember<HT>*)s->h_stage;
+        int* ht = (int*)(hm + M);
+        int *idxS = ht, *tIS = ht + (M + 1), *tAS = ht + 2 * (M + 1), *tUS = ht + 3 * (M + 1);
+        int *idxR = ht + 4 * (M + 1), *tAR = ht + 5 * (M + 1), *tUR = ht + 6 * (M + 1);
+        for (int i : starters) {
+            Member& m = s->active[i];
+            BMember<HT>& b = hm[m.slot];
+            b.v = pcg_view<HT>(m.c, m.eta);
+            b.v.ctl = &((SlotRec*)s->slots.p)[m.slot].ctl;
+            b.rhs = m.c->g.as<double>();
+            b.part = m.c->pcgpart.as<double>();
+            b.nbI = m.g.gn;
+            b.nbA = m.g.gs;
+            b.nbU = m.g.gu;
+        }
+        tIS[0] = tAS[0] = tUS[0] = 0;
+        for (size_t j = 0; j < starters.size(); ++j) {
+            const Member& m = s->active[starters[j]];
+            idxS[j] = m.slot;
+            tIS[j + 1] = tIS[j] + m.g.gn;
+            tAS[j + 1] = tAS[j] + m.g.gs;
+            tUS[j + 1] = tUS[j] + m.g.gu;
+        }
+        tAR[0] = tUR[0] = 0;
+        for (size_t j = 0; j < runners.size(); ++j) {
+            const Member& m = s->active[runners[j]];
+            idxR[j] = m.slot;
+            tAR[j + 1] = tAR[j] + m.g.gs;
+            tUR[j + 1] = tUR[j] + m.g.gu;
+        }
+        const size_t stage = sizeof(BMember<HT>) * M + sizeof(int) * 7 * (M + 1);
+        HIPX(hipMemcpyAsync(s->bmem.p, s->h_stage, stage, hipMemcpyHostToDevice, s->stream));
+        const BMember<HT>* dm = (const BMember<HT>*)s->bmem.p;
+        const int* dt_ = (const int*)((const char*)s->bmem.p + sizeof(BMember<HT>) * M);
+        const int *dIdxS = dt_, *dIS = dt_ + (M + 1), *dAS = dt_ + 2 * (M + 1), *dUS = dt_ + 3 * (M + 1);
+        const int *dIdxR = dt_ + 4 * (M + 1), *dAR = dt_ + 5 * (M + 1), *dUR = dt_ + 6 * (M + 1);
+        const int ns = (int)starters.size(), nr = (int)runners.size();
+        if (ns) {
+            k_b_pcg_init<HT><<<tIS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dIS, ns);
+            k_b_pcg_spmv<HT><<<tAS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dAS, ns, 0, 1);     // A_0
+            k_b_pcg_update<HT><<<tUS[ns], kUpdBlock, 0, s->stream>>>(dm, dIdxS, dUS, ns, 1, 1);  // U_1
+            k_b_pcg_spmv<HT><<<tAS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dAS, ns, 1, 0);     // A_1
+            s->launches += 4;
+        }
+        // iterations k >= 2 of every rollout in its solve: every rollout has
+        // completed an odd number of iterations, so parity follows j
+        for (int j = 0; j < chunk; ++j) {
+            const int par = j & 1;
+            k_b_pcg_update<HT><<<tUR[nr], kUpdBlock, 0, s->stream>>>(dm, dIdxR, dUR, nr, par, 0);
+            k_b_pcg_spmv<HT><<<tAR[nr], kBlock, 0, s->stream>>>(dm, dIdxR, dAR, nr, par, 0);
+        }
+        s->launches += 2 * chunk;
+        HIPX(hipGetLastError());
+        for (int i : starters) {
+            s->active[i].phase = Member::PCG_RUN;
+            s->active[i].launched = 1;
+        }
+        for (int i : runners) s->active[i].launched += chunk;
+    }
+    // 3. one readback of every slot
+    for (Member& m : s->active)
+        if (m.busy) HIPX(hipStreamSynchronize(m.c->stream));
+    HIPX(hipMemcpyAsync(s->h_slots, s->slots.p, sizeof(SlotRec) * s->opt.max_active, hipMemcpyDeviceToHost,
+                        s->stream));
+    HIPX(hipStreamSynchronize(s->stream));
+    s->ticks++;
+    // 4. advance every rollout (minimize() / do_steps() restated as phases)
+    std::vector<Member> keep;
+    keep.reserve(s->active.size());
+    for (Member& m : s->active) {
+        sors_ctx* c = m.c;
+        const SlotRec& sr = s->h_slots[m.slot];
+        const double dt = m.r->dt;
+        double* q = c->q.as<double>();
+        int status = 1;  // 1: continue, 0: finished, < 0: error
+        std::string msg;
+        auto end_step = [&] {
+            k_update_state<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
+                c->n, c->scheme, dt, q, c->qprev.as<double>(), c->vprev.as<double>(), c->gprev.as<double>(),
+                c->gtmp.as<double>());
+            m.step++;
+            if (m.step >= m.r->steps) {
+                status = 0;
+                return;
+            }
+            m.new_step = true;
+            m.it = 0;
+            m.phase = Member::ASSEMBLE;
+        };
+        auto next_newton = [&] {
+            if (++m.it >= o.max_newton) {
+                status = SORS_ERR_NOCONV;
+                msg = "Newton's method did not converge";
+                return;
+            }
+            m.phase = Member::ASSEMBLE;
+        };
+        auto accept = [&](double alpha) {
+            k_axpy<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, alpha, c->dxd.as<double>(), q);
+            m.st.dxnorm = alpha * m.dxn;
+            if (m.st.dxnorm < o.dx_tol)
+                end_step();
+            else
+                next_newton();
+        };
+        auto try_alpha = [&](double dE) {
+            if (dE <= 0.1 * m.alpha * m.gdx) {
+                accept(m.alpha);
+                return;
+            }
+            m.alpha *= 0.5;
+            m.st.backtracks++;
+            if (++m.bt <= o.max_backtrack) {
+                m.phase = Member::ENERGY;
+                return;
+            }
+            if (m.alpha * 2 * m.dxn < o.dx_tol) {
+                accept(m.alpha * 2);
+            } else {
+                status = SORS_ERR_LINESEARCH;
+                msg = "line search failed to decrease the energy";
+            }
+        };
+        switch (m.phase) {
+            case Member::ASSEMBLE: {
+                const double gg = sr.scal[S_GG];
+                if (sr.flag & 1) {
+                    status = SORS_ERR_INVERTED;
+                    msg = "Inverted Elements";
+                    break;
+                }
+                m.st.gnorm = std::sqrt(gg);
+                if (!(gg == gg)) {
+                    status = SORS_ERR_NAN;
+                    msg = "Newton's method diverged (NaN residual)";
+                    break;
+                }
+                if (o.g_tol > 0 && m.st.gnorm < o.g_tol) {
+                    end_step();
+                    break;
+                }
+                double eta = o.pcg_rtol;
+                if (o.g_tol > 0 && m.st.gnorm > 0) eta = std::min(0.1, std::max(eta, 0.5 * o.g_tol / m.st.gnorm));
+                m.eta = eta;
+                m.phase = Member::PCG_START;
+                break;
+            }
+            case Member::PCG_RUN:
+                if (sr.ctl.done || m.launched >= o.pcg_max_iter) {
+                    m.st.pcg_iters += sr.ctl.done ? sr.ctl.iter : m.launched;
+                    m.brk = sr.ctl.done && sr.ctl.breakdown;
+                    m.phase = Member::POST;
+                }
+                break;
+            case Member::POST: {
+                m.st.newton_iters++;
+                m.dxn = std::sqrt(sr.scal[S_DD]);
+                m.gdx = sr.scal[S_GD];
+                if (!(m.dxn == m.dxn)) {
+                    status = SORS_ERR_NAN;
+                    msg = "Newton's method diverged";
+                    break;
+                }
+                if (m.brk && m.dxn == 0.0) {
+                    status = SORS_ERR_NAN;
+                    msg = "PCG breakdown (indefinite tangent)";
+                    break;
+                }
+                if (c->method != SORS_MINIMIZE_NEWTON_LINESEARCH || m.dxn < o.dx_tol) {
+                    k_axpy<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, 1.0, c->dxd.as<double>(), q);
+                    m.st.dxnorm = m.dxn;
+                    if (m.dxn < o.dx_tol)
+                        end_step();
+                    else
+                        next_newton();
+                    break;
+                }
+                m.alpha = 1.0;
+                m.bt = 0;
+                try_alpha(sr.scal[S_EELEM] + sr.scal[S_ENODE]);
+                break;
+            }
+            case Member::ENERGY:
+                try_alpha(sr.scal[S_EELEM] + sr.scal[S_ENODE]);
+                break;
+            default:
+                break;
+        }
+        if (status == 1)
+            keep.push_back(m);
+        else
+            srv_retire(s, m, status == 0 ? SORS_OK : status, msg);
+    }
+    s->active.swap(keep);
+}
+
+void srv_worker(sors_server* s) {
+    (void)hipSetDevice(s->device);
+    const bool mixed = s->opt.solver.precision == SORS_MIXED;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->active.empty())
+                s->cv_work.wait(lk, [&] { return !s->ready.empty() || (s->stop && s->admitted == 0); });
+            if (s->active.empty() && s->ready.empty() && s->stop && s->admitted == 0) return;
+            while (!s->ready.empty() && !s->free_slots.empty()) {
+                Request* r = s->ready.front();
+                s->ready.pop_front();
+                Member m;
+                m.r = r;
+                m.c = r->ctx;
+                m.slot = s->free_slots.back();
+                s->free_slots.pop_back();
+                SlotRec* sl = (SlotRec*)s->slots.p + m.slot;
+                m.c->scal_ext = sl->scal;
+                m.c->flag_ext = &sl->flag;
+                m.g = pcg_grid(m.c);
+                s->active.push_back(m);
+            }
+            s->stats.ticks = s->ticks;
+            s->stats.batched_launches = s->launches;
+            s->stats.active = (int)s->active.size();
+            s->stats.peak_active = std::max(s->stats.peak_active, s->stats.active);
+        }
+        if (s->active.empty()) continue;
+        try {
+            if (mixed)
+                srv_tick<float>(s);
+            else
+                srv_tick<double>(s);
+        } catch (const HipError& e) {
+            std::vector<Member> all;
+            all.swap(s->active);
+            for (Member& m : all) srv_retire(s, m, SORS_ERR_HIP, e.msg);
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->err = e.msg;
+        }
+    }
 }
 
 }  // namespace
@@ -1854,8 +2537,8 @@ int sors_create(const sors_system_desc* d, sors_ctx** out) {
         ensure(c->scal, sizeof(double) * S_COUNT);
         ensure(c->pcgst, 2 * sizeof(PcgState));
         ensure(c->flag, sizeof(int));
-        fill<double>(c->scal.as<double>(), S_COUNT, 0.0, st);
-        fill<int>(c->flag.as<int>(), 1, 0, st);
+        fill<double>(c->S(), S_COUNT, 0.0, st);
+        fill<int>(c->F(), 1, 0, st);
         HIPX(hipHostMalloc((void**)&c->h_pin, 4096, hipHostMallocDefault));
         HIPX(hipStreamSynchronize(st));
     } catch (const HipError& e) {
@@ -2182,17 +2865,30 @@ int sors_bench_kernel(sors_ctx* c, int kernel, int reps, double* avg_ms, double*
                            : nnz * (hb + 4.0) + 4.0 * (n + 1) + 2.0 * hb * n;
         if (mixed) {
             fill<float>(c->pu.as<float>(), n, 1.0f, c->stream);
-            run([&] { launch_spmv<float>(c, g, c->pu.as<float>(), c->pw.as<float>(), bctl, bst, 0, g.gu, 0, 0.0); });
+            PcgView<float> v = pcg_view<float>(c, 0.0);
+            v.st = bst;
+            v.ctl = bctl;
+            run([&] { launch_spmv<float>(c, g, v, 0, g.gu, 0); });
         } else {
             fill<double>(c->pu.as<double>(), n, 1.0, c->stream);
-            run([&] { launch_spmv<double>(c, g, c->pu.as<double>(), c->pw.as<double>(), bctl, bst, 0, g.gu, 0, 0.0); });
+            PcgView<double> v = pcg_view<double>(c, 0.0);
+            v.st = bst;
+            v.ctl = bctl;
+            run([&] { launch_spmv<double>(c, g, v, 0, g.gu, 0); });
         }
     } else if (kernel == SORS_KERNEL_PCG_UPDATE) {
         bytes = hb * (11.0 * n + 9.0 * N);
-        if (mixed)
-            run([&] { launch_update<float>(c, g, bctl, bst, bst + 1, 1, g.gu, 0); });
-        else
-            run([&] { launch_update<double>(c, g, bctl, bst, bst + 1, 1, g.gu, 0); });
+        if (mixed) {
+            PcgView<float> v = pcg_view<float>(c, 0.0);
+            v.st = bst;
+            v.ctl = bctl;
+            run([&] { launch_update<float>(c, g, v, 1, g.gu, 0); });
+        } else {
+            PcgView<double> v = pcg_view<double>(c, 0.0);
+            v.st = bst;
+            v.ctl = bctl;
+            run([&] { launch_update<double>(c, g, v, 1, g.gu, 0); });
+        }
     } else if (kernel == SORS_KERNEL_COPY) {
         const long long n4 = nnz / 4;  // int4 elements of the column array
         ensure(c->copybuf, sizeof(int4) * std::max<long long>(n4, 1));
@@ -2217,6 +2913,182 @@ int sors_bench_kernel(sors_ctx* c, int kernel, int reps, double* avg_ms, double*
     if (algo_bytes) *algo_bytes = bytes;
     return SORS_OK;
     SORS_GUARD_END(c)
+}
+
+// ---- serving loop --------------------------------------------------------
+void sors_server_default_options(sors_server_options* o) {
+    if (!o) return;
+    o->hbm_budget_bytes = 200e9;
+    o->max_active = 64;
+    o->prep_threads = 4;
+    o->pcg_chunk = 16;
+    sors_default_options(&o->solver);
+}
+
+int sors_server_create(const sors_server_options* o, sors_server** out) {
+    g_create_error.clear();
+    if (!o || !out) {
+        g_create_error = "null argument";
+        return SORS_ERR_ARG;
+    }
+    *out = nullptr;
+    if (o->max_active < 1 || o->max_active > 256 || o->prep_threads < 1 || o->prep_threads > 64 ||
+        o->pcg_chunk < 2 || !(o->hbm_budget_bytes > 0)) {
+        g_create_error = "server options out of range (max_active 1..256, prep_threads 1..64, pcg_chunk >= 2)";
+        return SORS_ERR_ARG;
+    }
+    const sors_options& so = o->solver;
+    if ((so.precision != SORS_FP64 && so.precision != SORS_MIXED) || so.max_newton < 1 || so.pcg_max_iter < 1 ||
+        so.max_backtrack < 0) {
+        g_create_error = "bad solver options";
+        return SORS_ERR_ARG;
+    }
+    std::unique_ptr<sors_server> s(new sors_server());
+    s->opt = *o;
+    try {
+        HIPX(hipGetDevice(&s->device));
+        HIPX(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        const int M = o->max_active;
+        ensure(s->slots, sizeof(SlotRec) * M);
+        HIPX(hipMemset(s->slots.p, 0, sizeof(SlotRec) * M));
+        const size_t stage = sizeof(BMember<double>) * M + sizeof(int) * 7 * (M + 1);
+        static_assert(sizeof(BMember<double>) >= sizeof(BMember<float>), "stage size");
+        ensure(s->bmem, stage);
+        HIPX(hipHostMalloc(&s->h_stage, stage, hipHostMallocDefault));
+        HIPX(hipHostMalloc((void**)&s->h_slots, sizeof(SlotRec) * M, hipHostMallocDefault));
+        std::memset(s->h_stage, 0, stage);
+        for (int i = M - 1; i >= 0; --i) s->free_slots.push_back(i);
+    } catch (const HipError& e) {
+        g_create_error = e.msg;
+        if (s->h_stage) (void)hipHostFree(s->h_stage);
+        if (s->h_slots) (void)hipHostFree(s->h_slots);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        return SORS_ERR_HIP;
+    }
+    sors_server* sp = s.release();
+    for (int i = 0; i < o->prep_threads; ++i) sp->prep.emplace_back(srv_prep_loop, sp);
+    sp->worker = std::thread(srv_worker, sp);
+    *out = sp;
+    return SORS_OK;
+}
+
+int sors_server_submit(sors_server* s, const sors_system_desc* d, const double* v0, double dt, int steps,
+                       long long* id) {
+    if (!s) return SORS_ERR_ARG;
+    auto bad = [&](const char* m) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = m;
+        return SORS_ERR_ARG;
+    };
+    if (!d || !id) return bad("null argument");
+    if (d->nodes_per_elem != 4 && d->nodes_per_elem != 8) return bad("nodes_per_elem must be 4 or 8");
+    if (d->n_nodes <= 0 || d->n_elems <= 0 || !d->rest || !d->conn) return bad("empty or missing mesh arrays");
+    if (!(dt > 0.0) || steps < 1) return bad("dt must be > 0 and steps >= 1");
+    if (d->method == SORS_MINIMIZE_SQP) return bad("minimize_SQP (OSQP contact) is not implemented on the device");
+    std::unique_ptr<Request> r(new Request());
+    r->N = d->n_nodes;
+    r->E = d->n_elems;
+    r->ed = d->nodes_per_elem;
+    r->material = d->material;
+    r->scheme = d->scheme;
+    r->method = d->method;
+    for (int k = 0; k < 3; ++k) r->grav[k] = d->grav[k];
+    r->damping = d->damping_alpha;
+    const size_t n = 3 * (size_t)r->N, E = (size_t)r->E;
+    r->rest.assign(d->rest, d->rest + n);
+    r->conn.assign(d->conn, d->conn + E * r->ed);
+    r->youngs.assign(E, 0.0);
+    r->poisson.assign(E, 0.0);
+    r->density.assign(E, 0.0);
+    if (d->youngs) r->youngs.assign(d->youngs, d->youngs + E);
+    if (d->poisson) r->poisson.assign(d->poisson, d->poisson + E);
+    if (d->density) r->density.assign(d->density, d->density + E);
+    if (d->bc_mask) r->bc_mask.assign(d->bc_mask, d->bc_mask + n);
+    if (d->bc_value) r->bc_value.assign(d->bc_value, d->bc_value + n);
+    if (v0) r->v0.assign(v0, v0 + n);
+    r->dt = dt;
+    r->steps = steps;
+    r->est_bytes = estimate_bytes(r->N, r->E, r->ed);
+    if (r->est_bytes > s->opt.hbm_budget_bytes) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = "request needs more device memory than the server's budget";
+        return SORS_ERR_BUDGET;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) {
+        s->err = "server is shutting down";
+        return SORS_ERR_ARG;
+    }
+    r->id = s->next_id++;
+    r->t_submit = Request::clock::now();
+    *id = r->id;
+    Request* rp = r.get();
+    s->reqs[rp->id] = std::move(r);
+    s->pending.push_back(rp);
+    s->stats.submitted++;
+    srv_try_admit(s);
+    return SORS_OK;
+}
+
+int sors_server_wait(sors_server* s, long long id, double* q, sors_request_stats* st, double timeout_s) {
+    if (!s) return SORS_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) {
+        s->err = "unknown request id";
+        return SORS_ERR_ARG;
+    }
+    Request* r = it->second.get();
+    auto pred = [&] { return r->done; };
+    if (timeout_s < 0)
+        s->cv_done.wait(lk, pred);
+    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
+        return SORS_ERR_TIMEOUT;
+    const int rc = r->status;
+    if (q && rc == SORS_OK) std::copy(r->q.begin(), r->q.end(), q);
+    if (st) *st = r->stats;
+    if (rc != SORS_OK) s->err = r->err;
+    s->reqs.erase(it);
+    return rc;
+}
+
+int sors_server_get_stats(sors_server* s, sors_server_stats* st) {
+    if (!s || !st) return SORS_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *st = s->stats;
+    return SORS_OK;
+}
+
+const char* sors_server_last_error(const sors_server* s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+void sors_server_destroy(sors_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        for (Request* r : s->pending) {
+            r->status = SORS_ERR_ARG;
+            r->err = "server shut down before the request was admitted";
+            r->done = true;
+            s->stats.failed++;
+        }
+        s->pending.clear();
+        s->cv_work.notify_all();
+        s->cv_prep.notify_all();
+        s->cv_done.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->cv_prep.notify_all();
+    }
+    for (std::thread& t : s->prep)
+        if (t.joinable()) t.join();
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->h_stage) (void)hipHostFree(s->h_stage);
+    if (s->h_slots) (void)hipHostFree(s->h_slots);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
 }
 
 }  // extern "C"

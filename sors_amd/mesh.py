@@ -19,6 +19,7 @@ C++ and Python sides can reproduce them bit for bit.
 from __future__ import annotations
 
 import dataclasses
+import math
 
 import numpy as np
 
@@ -205,3 +206,42 @@ def cantilever_large(element="tet", seed=0, steps=20, **kw) -> Scene:
     nodes), E log-uniform in [5e5, 5e6] per cell, 20 steps of dt=1e-3."""
     return beam(nx=200, ny=40, nz=40, element=element, e_range=(5e5, 5e6), seed=seed, steps=steps,
                 name=f"beam200x40x40_{element}", **kw)
+
+
+def serving_request(index: int, seed: int = 0, n_lo: int = 2_000, n_hi: int = 200_000, e_lo: float = 1e5,
+                    e_hi: float = 1e7, nu_lo: float = 0.30, nu_hi: float = 0.48, steps_lo: int = 10,
+                    steps_hi: int = 50, dt: float = 1e-3) -> Scene:
+    """Request `index` of the seeded serving stream: a beam (0.10 x 0.02 x
+    0.02 m, clamped at x = 0) or a cube (0.05 m, clamped at z = 0) tet mesh
+    whose node count is log-uniform in [n_lo, n_hi], one Young's modulus
+    log-uniform in [e_lo, e_hi], Poisson ratio uniform in [nu_lo, nu_hi],
+    steps uniform in [steps_lo, steps_hi], gravity -z.  The beam starts with
+    v_z = 0.5 x / L, the cube with v_x = 0.5 z / L."""
+    u = uniform01((seed + 1) * 0x9E3779B97F4A7C15 + index, 5)
+    kind = "beam" if u[0] < 0.5 else "cube"
+    nodes = math.exp(math.log(n_lo) + u[1] * (math.log(n_hi) - math.log(n_lo)))
+    E = math.exp(math.log(e_lo) + u[2] * (math.log(e_hi) - math.log(e_lo)))
+    nu = nu_lo + u[3] * (nu_hi - nu_lo)
+    steps = steps_lo + min(int(u[4] * (steps_hi - steps_lo + 1)), steps_hi - steps_lo)
+    if kind == "beam":
+        # (5k + 1)(k + 1)^2 nodes
+        k = max(1, int(round((nodes / 5.0) ** (1.0 / 3.0))))
+        sc = beam(nx=5 * k, ny=k, nz=k, E=E, nu=nu, dt=dt, steps=steps, name=f"req{index}_beam")
+        return sc
+    k = max(1, int(round(nodes ** (1.0 / 3.0))) - 1)
+    L = 0.05
+    X, hexes = box_grid(k, k, k, L, L, L)
+    conn = kuhn_split(hexes).astype(np.int32)
+    N, nel = X.shape[0], conn.shape[0]
+    clamp = X[:, 2] < 1e-4 * L
+    v0 = np.zeros((N, 3))
+    v0[:, 0] = 0.5 * X[:, 2] / L
+    return Scene(name=f"req{index}_cube", X=X, conn=conn, youngs=np.full(nel, E), poisson=np.full(nel, nu),
+                 density=np.full(nel, 1100.0), bcmask=np.repeat(clamp.astype(np.int32), 3), bcvalue=np.zeros(3 * N),
+                 v0=v0.reshape(-1), grav=np.array([0.0, 0.0, -9.81]), dt=dt, steps=steps)
+
+
+def serving_arrivals(count: int, rate: float = 200.0, seed: int = 0) -> np.ndarray:
+    """Poisson arrival times (s) of `count` requests at `rate` per second."""
+    u = uniform01(seed * 0x2545F4914F6CDD1D + 7, count)
+    return np.cumsum(-np.log1p(-u) / rate)

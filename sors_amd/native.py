@@ -17,6 +17,7 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "sors_amd.h")
 
 OK = 0
 ERR_INVERTED, ERR_NOCONV, ERR_NAN, ERR_LINESEARCH, ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6, -7
+ERR_TIMEOUT, ERR_BUDGET = -8, -9
 BACKWARD_EULER, CRANK_NICOLSON = 0, 1
 MINIMIZE_NEWTON, MINIMIZE_NEWTON_LINESEARCH, MINIMIZE_SQP = 0, 1, 2
 NEOHOOKEAN, TEST_POLY = 0, 1
@@ -54,6 +55,35 @@ class Stats(C.Structure):
         ("dxnorm", C.c_double), ("ms_assemble", C.c_double), ("ms_solve", C.c_double),
         ("ms_linesearch", C.c_double), ("ms_total", C.c_double), ("spmv_ms", C.c_double),
         ("spmv_launches", C.c_longlong), ("assemble_kernel_ms", C.c_double), ("assemble_launches", C.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_double), ("max_active", C.c_int), ("prep_threads", C.c_int),
+        ("pcg_chunk", C.c_int), ("solver", Options),
+    ]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [
+        ("status", C.c_int), ("steps", C.c_int), ("newton_iters", C.c_longlong), ("pcg_iters", C.c_longlong),
+        ("backtracks", C.c_longlong), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
+        ("prep_ms", C.c_double), ("device_bytes", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_longlong), ("completed", C.c_longlong), ("failed", C.c_longlong),
+        ("active", C.c_int), ("peak_active", C.c_int), ("resident_bytes", C.c_double),
+        ("peak_resident_bytes", C.c_double), ("ticks", C.c_longlong), ("batched_launches", C.c_longlong),
     ]
 
     def as_dict(self):
@@ -103,6 +133,17 @@ def lib():
         L.sors_element_eval.argtypes = [vp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]
         L.sors_linear_solve.argtypes = [vp, _dp, C.c_double, _dp, _dp, C.c_double, C.c_int, _ip]
         L.sors_bench_kernel.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
+        L.sors_server_default_options.argtypes = [C.POINTER(ServerOptions)]
+        L.sors_server_default_options.restype = None
+        L.sors_server_create.argtypes = [C.POINTER(ServerOptions), C.POINTER(vp)]
+        L.sors_server_submit.argtypes = [vp, C.POINTER(SystemDesc), _dp, C.c_double, C.c_int,
+                                         C.POINTER(C.c_longlong)]
+        L.sors_server_wait.argtypes = [vp, C.c_longlong, _dp, C.POINTER(RequestStats), C.c_double]
+        L.sors_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+        L.sors_server_last_error.argtypes = [vp]
+        L.sors_server_last_error.restype = C.c_char_p
+        L.sors_server_destroy.argtypes = [vp]
+        L.sors_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -303,3 +344,83 @@ class Rollout:
         self._check(lib().sors_linear_solve(self.h, _d(q), float(dt), _d(b), _d(x), float(rtol), int(max_iter),
                                             C.byref(it)))
         return x, it.value
+
+
+def _scene_desc(sc, method, scheme):
+    """SystemDesc over a Scene's arrays; returns (desc, keep-alive tuple)."""
+    X, conn = f64(sc.X).reshape(-1, 3), i32(sc.conn)
+    keep = (X, conn, f64(sc.youngs), f64(sc.poisson), f64(sc.density),
+            i32(sc.bcmask) if sc.bcmask is not None else None,
+            f64(sc.bcvalue) if sc.bcvalue is not None else None)
+    d = SystemDesc()
+    d.n_nodes, d.n_elems, d.nodes_per_elem = X.shape[0], conn.shape[0], conn.shape[1]
+    d.rest, d.conn = _d(keep[0]), _i(keep[1])
+    d.youngs, d.poisson, d.density = _d(keep[2]), _d(keep[3]), _d(keep[4])
+    d.material = NEOHOOKEAN
+    d.bc_mask, d.bc_value = _i(keep[5]), _d(keep[6])
+    for k in range(3):
+        d.grav[k] = float(sc.grav[k])
+    d.damping_alpha = float(sc.alpha)
+    d.scheme, d.method = scheme, method
+    return d, keep
+
+
+class Server:
+    """Serving loop: continuous batching of independent rollouts on one GPU.
+
+    submit() queues a rollout (mesh, material, initial velocity, steps) and
+    returns its id at once; wait(id) returns its final positions and stats.
+    """
+
+    def __init__(self, hbm_budget_bytes=200e9, max_active=64, prep_threads=4, pcg_chunk=16, **solver):
+        L = lib()
+        o = ServerOptions()
+        L.sors_server_default_options(C.byref(o))
+        o.hbm_budget_bytes = float(hbm_budget_bytes)
+        o.max_active, o.prep_threads, o.pcg_chunk = max_active, prep_threads, pcg_chunk
+        for k, v in solver.items():
+            setattr(o.solver, k, v)
+        h = C.c_void_p()
+        rc = L.sors_server_create(C.byref(o), C.byref(h))
+        if rc != OK:
+            raise SorsError(rc, L.sors_server_last_error(None).decode())
+        self.h = h
+        self._n = {}
+
+    def _check(self, rc):
+        if rc != OK:
+            raise SorsError(rc, lib().sors_server_last_error(self.h).decode())
+
+    def submit(self, sc, steps, dt, method=MINIMIZE_NEWTON_LINESEARCH, scheme=BACKWARD_EULER) -> int:
+        d, keep = _scene_desc(sc, method, scheme)
+        v0 = f64(sc.v0).reshape(-1) if sc.v0 is not None else None
+        rid = C.c_longlong()
+        self._check(lib().sors_server_submit(self.h, C.byref(d), _d(v0), float(dt), int(steps), C.byref(rid)))
+        self._n[rid.value] = 3 * d.n_nodes
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        """(q [N,3], stats dict); raises SorsError with the request's status on failure,
+        ERR_TIMEOUT if it is not finished within `timeout` seconds."""
+        q = np.empty(self._n[rid], dtype=np.float64)
+        st = RequestStats()
+        rc = lib().sors_server_wait(self.h, rid, _d(q), C.byref(st), float(timeout))
+        if rc == ERR_TIMEOUT:
+            raise SorsError(rc, "request not finished")
+        self._n.pop(rid, None)
+        if rc != OK:
+            raise SorsError(rc, lib().sors_server_last_error(self.h).decode())
+        return q.reshape(-1, 3), st.as_dict()
+
+    def stats(self):
+        st = ServerStats()
+        self._check(lib().sors_server_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sors_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

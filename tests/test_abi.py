@@ -25,3 +25,29 @@ def test_default_options_match_reference_constants():
     # MAX_ITER = 100, DX_TOL = 1e-9 (cpp/headers/common.h:21-25)
     assert o.max_newton == 100 and o.dx_tol == 1e-9
     assert o.project_spd == 1 and o.precision == native.MIXED
+
+
+def test_server_declarations_and_option_checks():
+    syms = native.declared_symbols()
+    for s in ("sors_server_create", "sors_server_submit", "sors_server_wait", "sors_server_get_stats",
+              "sors_server_destroy", "sors_server_default_options", "sors_server_last_error"):
+        assert s in syms
+    L = native.lib()
+    o = native.ServerOptions()
+    L.sors_server_default_options(ctypes.byref(o))
+    assert o.max_active == 64 and o.hbm_budget_bytes == 200e9 and o.pcg_chunk % 2 == 0
+    assert o.solver.max_newton == 100
+    # option validation happens before any device call
+    h = ctypes.c_void_p()
+    o.max_active = 0
+    assert L.sors_server_create(ctypes.byref(o), ctypes.byref(h)) == native.ERR_ARG
+    assert b"max_active" in L.sors_server_last_error(None)
+    o.max_active, o.pcg_chunk = 4, 1
+    assert L.sors_server_create(ctypes.byref(o), ctypes.byref(h)) == native.ERR_ARG
+
+
+def test_struct_sizes_match_header():
+    # layouts the ctypes mirror relies on (x86-64: 8-byte aligned doubles)
+    assert ctypes.sizeof(native.RequestStats) == 4 + 4 + 3 * 8 + 4 * 8
+    assert ctypes.sizeof(native.ServerStats) == 3 * 8 + 4 + 4 + 2 * 8 + 2 * 8
+    assert ctypes.sizeof(native.ServerOptions) == 8 + 3 * 4 + 4 + ctypes.sizeof(native.Options)

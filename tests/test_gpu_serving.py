@@ -1,0 +1,106 @@
+"""Serving loop (continuous batching) through the C ABI.
+
+Each rollout's result must be the same alone and in the mix: within 1e-6 of
+the bounding-box diagonal (the bar of converged positions), against a plain
+device-resident rollout stepped on its own.  The two differ only by the
+rounding order of float atomics in assembly.
+"""
+import numpy as np
+import pytest
+
+from sors_amd import mesh, native
+
+pytestmark = pytest.mark.gpu
+
+SOLVER = dict(g_tol=1e-8, pcg_rtol=3e-3)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def small_request(i, seed=11):
+    return mesh.serving_request(i, seed=seed, n_lo=300, n_hi=3000, steps_lo=3, steps_hi=6)
+
+
+def alone(sc, precision=native.MIXED):
+    r = native.Rollout.from_scene(sc, precision=precision, **SOLVER)
+    for _ in range(sc.steps):
+        r.step_resident(sc.dt)
+    q = r.positions().reshape(-1, 3)
+    r.close()
+    return q
+
+
+def test_serving_mix_matches_alone():
+    reqs = [small_request(i) for i in range(8)]
+    say("serving mix", [(sc.name, sc.n_nodes, sc.steps) for sc in reqs])
+    srv = native.Server(max_active=5, prep_threads=3, pcg_chunk=8, **SOLVER)
+    ids = [srv.submit(sc, sc.steps, sc.dt) for sc in reqs]
+    out = [srv.wait(i, timeout=120.0) for i in ids]
+    st = srv.stats()
+    srv.close()
+    say("server stats", st)
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert 1 <= st["peak_active"] <= 5
+    assert st["batched_launches"] > 0
+    for sc, (q, rs) in zip(reqs, out):
+        ref = alone(sc)
+        err = np.abs(q - ref).max() / sc.bbox_diag
+        say(sc.name, "mix vs alone", err, rs["newton_iters"], rs["pcg_iters"], rs["latency_ms"])
+        assert rs["steps"] == sc.steps and rs["status"] == native.OK
+        assert err <= 1e-6
+        assert np.abs(q - sc.X).max() > 1e-6  # the rollout moved
+
+
+def test_serving_single_request_matches_plain_fp64():
+    sc = small_request(3)
+    srv = native.Server(max_active=2, prep_threads=1, precision=native.FP64, **SOLVER)
+    q, rs = srv.wait(srv.submit(sc, sc.steps, sc.dt), timeout=120.0)
+    srv.close()
+    ref = alone(sc, precision=native.FP64)
+    err = np.abs(q - ref).max() / sc.bbox_diag
+    say("fp64 server vs alone", err)
+    assert err <= 1e-9
+
+
+def test_serving_budget_limits_residency():
+    reqs = [small_request(i, seed=5) for i in range(6)]
+    # budget for about two of the largest requests at once: not all fit
+    est = [native_estimate(sc) for sc in reqs]
+    per = max(est)
+    assert sum(est) > 2.5 * per
+    srv = native.Server(hbm_budget_bytes=2.5 * per, max_active=8, prep_threads=2, **SOLVER)
+    ids = [srv.submit(sc, sc.steps, sc.dt) for sc in reqs]
+    for i in ids:
+        srv.wait(i, timeout=120.0)
+    st = srv.stats()
+    srv.close()
+    say("budget stats", st, "budget", 2.5 * per)
+    assert st["completed"] == len(reqs)
+    assert 0 < st["peak_resident_bytes"] <= 2.5 * per
+    assert st["peak_active"] < len(reqs)
+    assert st["resident_bytes"] == 0.0  # everything released
+
+
+def test_serving_reports_failures():
+    sc = small_request(0)
+    srv = native.Server(max_active=2, prep_threads=1, max_newton=1, g_tol=1e-14, pcg_rtol=1e-6)
+    rid = srv.submit(sc, sc.steps, sc.dt)
+    with pytest.raises(native.SorsError) as ei:
+        srv.wait(rid, timeout=120.0)
+    assert ei.value.code == native.ERR_NOCONV
+    assert "did not converge" in str(ei.value)
+    # the server keeps serving after a failed request
+    srv2 = native.Server(max_active=2, prep_threads=1, **SOLVER)
+    q, rs = srv2.wait(srv2.submit(sc, 2, sc.dt), timeout=120.0)
+    assert rs["steps"] == 2
+    srv.close()
+    srv2.close()
+
+
+def native_estimate(sc):
+    """Device bytes the server reserves for a request before its pattern exists."""
+    n, N, E, ed = 3 * sc.n_nodes, sc.n_nodes, sc.n_elems, sc.ed
+    nblk = N * (24 if ed == 4 else 32)
+    return n * (19 * 8 + 4) + 72 * N + nblk * (9 * 8 + 4 + 9 * 4) + E * (ed * 4 + 16 + ed * ed) + 8 * 9 * 2048 + (1 << 20)
