@@ -95,6 +95,88 @@ def cpu_baseline(sc, agg, steps, cores):
     return {"value": 1.0 / sec_per_step, "unit": "steps/s", "cores": cores, "kind": "port", "sample": sample}
 
 
+def serve_leg(args, opts):
+    """Serving: seeded requests (beam / cube tet meshes, 2k-200k nodes,
+    E log-uniform [1e5, 1e7], nu uniform [0.3, 0.48], 10-50 steps of
+    dt = 1e-3) arriving as a Poisson stream, served mixed-precision with
+    continuous batching; then the same requests run one after another in
+    plain FP64 on the GPU, queued behind each other at the same arrival
+    times (first come, first served)."""
+    from sors_amd import mesh, native
+
+    count, rate = args.serve, args.serve_rate
+    t0 = time.perf_counter()
+    reqs = [mesh.serving_request(i, seed=args.serve_seed) for i in range(count)]
+    arrivals = mesh.serving_arrivals(count, rate=rate, seed=args.serve_seed)
+    log(f"serving: {count} requests generated in {time.perf_counter() - t0:.1f}s, "
+        f"{sum(sc.n_nodes for sc in reqs)} nodes, {sum(sc.steps for sc in reqs)} steps in total")
+    srv = native.Server(hbm_budget_bytes=args.serve_budget, max_active=args.serve_max_active,
+                        prep_threads=args.serve_prep_threads, pcg_chunk=args.serve_chunk,
+                        precision=native.MIXED, **opts)
+    ids = []
+    start = time.perf_counter()
+    for t, sc in zip(arrivals, reqs):
+        dt_wait = t - (time.perf_counter() - start)
+        if dt_wait > 0:
+            time.sleep(dt_wait)
+        ids.append(srv.submit(sc, sc.steps, sc.dt))
+    results = []
+    for k, i in enumerate(ids):
+        results.append(srv.wait(i))
+        if (k + 1) % max(1, count // 8) == 0:
+            log(f"  served {k + 1}/{count} after {time.perf_counter() - start:.1f}s")
+    wall = time.perf_counter() - start
+    sst = srv.stats()
+    srv.close()
+    lat = np.array([st["latency_ms"] for _, st in results])
+    steps_total = sum(sc.steps for sc in reqs)
+    out = {
+        "value": count / wall, "unit": "solves/s", "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)), "steps_per_s": steps_total / wall, "wall_s": wall,
+        "requests": count, "arrival_rate": rate,
+        "sample": f"first {count} requests of the seeded stream (seed {args.serve_seed}); the full "
+                  f"stream is 2,048 requests",
+        "queue_ms_p50": float(np.percentile([st["queue_ms"] for _, st in results], 50)),
+        "prep_ms_p50": float(np.percentile([st["prep_ms"] for _, st in results], 50)),
+        "newton_iters": int(sum(st["newton_iters"] for _, st in results)),
+        "pcg_iters": int(sum(st["pcg_iters"] for _, st in results)),
+        "server": {k: sst[k] for k in ("peak_active", "peak_resident_bytes", "ticks", "batched_launches")},
+        "options": {"max_active": args.serve_max_active, "prep_threads": args.serve_prep_threads,
+                    "pcg_chunk": args.serve_chunk, "hbm_budget_bytes": args.serve_budget},
+    }
+    log(f"served: {out['value']:.2f} solves/s, p50 {out['p50_ms']:.0f} ms, p99 {out['p99_ms']:.0f} ms, "
+        f"wall {wall:.1f}s, {sst}")
+    if args.no_serve_seq:
+        return out
+    # sequential plain FP64-GPU, same requests, same arrival times
+    service, errs = [], []
+    for k, sc in enumerate(reqs):
+        ts = time.perf_counter()
+        r = native.Rollout.from_scene(sc, precision=native.FP64, **opts)
+        for _ in range(sc.steps):
+            r.step_resident(sc.dt)
+        q64 = r.positions().reshape(-1, 3)
+        r.close()
+        service.append(time.perf_counter() - ts)
+        errs.append(float(np.abs(results[k][0] - q64).max() / sc.bbox_diag))
+        if (k + 1) % max(1, count // 8) == 0:
+            log(f"  sequential fp64 {k + 1}/{count}: {sum(service):.1f}s of service")
+    finish, done = [], 0.0
+    for t, sv in zip(arrivals, service):
+        done = max(done, t) + sv
+        finish.append(done)
+    lat64 = 1e3 * (np.array(finish) - arrivals)
+    out["sequential_fp64_gpu"] = {
+        "value": count / (finish[-1] - 0.0), "unit": "solves/s", "p50_ms": float(np.percentile(lat64, 50)),
+        "p99_ms": float(np.percentile(lat64, 99)), "service_s_total": float(sum(service)),
+        "how": "each request alone on the GPU in plain FP64 (create, steps, read back), one after another; "
+               "latency = FCFS queue of those measured service times at the same arrival times"}
+    out["vs_sequential_fp64"] = out["value"] / out["sequential_fp64_gpu"]["value"]
+    out["max_pos_err_over_bbox_vs_fp64"] = max(errs)
+    log(f"sequential fp64: {out['sequential_fp64_gpu']}; max err/bbox {max(errs):.3e}")
+    return out
+
+
 def load_traffic(key):
     try:
         with open(TRAFFIC_FILE) as f:
@@ -139,6 +221,15 @@ def main():
     ap.add_argument("--g-tol", type=float, default=1e-8, help="Newton residual criterion ||g||_2 (N)")
     ap.add_argument("--no-predict", action="store_true", help="start Newton from the previous positions")
     ap.add_argument("--check-every", type=int, default=32, help="PCG iterations between host convergence checks")
+    ap.add_argument("--serve", type=int, default=48, help="serving requests to run (0: skip the serving leg)")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-seed", type=int, default=0)
+    ap.add_argument("--serve-budget", type=float, default=200e9, help="HBM budget of resident rollouts (bytes)")
+    ap.add_argument("--serve-max-active", type=int, default=64)
+    ap.add_argument("--serve-prep-threads", type=int, default=8)
+    ap.add_argument("--serve-chunk", type=int, default=16)
+    ap.add_argument("--no-serve-seq", action="store_true", help="skip the sequential FP64-GPU serving baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -149,6 +240,10 @@ def main():
         return
     from sors_amd import native
 
+    if args.serve_only:
+        opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=int(not args.no_predict))
+        print(json.dumps({"metric": METRIC, "serving": serve_leg(args, opts)}), flush=True)
+        return
     sc = make_scene(args)
     log(f"workload {sc.name}: {sc.n_elems} elements, {sc.n_nodes} nodes; warmup {args.warmup}, steps {args.steps}")
     opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=int(not args.no_predict),
@@ -219,6 +314,9 @@ def main():
         cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sc, agg, args.steps, cores)
         log(f"cpu baseline: {out['cpu_baseline']}")
+    if args.serve > 0:
+        out["serving"] = serve_leg(args, dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol,
+                                              predict=int(not args.no_predict)))
     print(json.dumps(out), flush=True)
 
 
