@@ -165,6 +165,48 @@ enum { SORS_KERNEL_SPMV = 0, SORS_KERNEL_ASSEMBLE = 1, SORS_KERNEL_COPY = 2, SOR
  * values and gradient written once). */
 int sors_bench_kernel(sors_ctx* ctx, int kernel, int reps, double* avg_ms, double* algo_bytes);
 
+/* ---- Contact (penalty + lagged Coulomb friction) ---------------------------
+ * The reference resolves contact as hard constraints n.(x - p) >= 0 by SQP
+ * over OSQP (cpp/src/solver.cpp:109-197) with its "planeContact" and
+ * "diskContact" constraints (cpp/src/energy.cpp:69-105).  This device path
+ * keeps their geometry -- planes (point, normal; Plane3D, plane3D.h:30-33)
+ * and disks (centre, normal, radius; Disk3D, disk3D.h:33-45) acting on the
+ * surface vertices -- and adds to the step objective, per candidate node:
+ *   penalty   1/2 k w d^2 for d = n.(x - p) < 0 (w = 1 for a plane; for a
+ *             disk w = 1 inside the radius, a C1 cubic falloff over 1 mm
+ *             outside it, 0 beyond);
+ *   friction  mu N phi(|uT|): N, n and the primitive's displacement D over
+ *             the step are taken at the start of each step (largest penalty
+ *             force at qprev); u = x - qprev - D, uT its tangential part,
+ *             phi(y) = y^2/(2 eps) below eps = eps_v dt, y - eps/2 above.
+ * Parameters keep the reference's names: positionVectorListPlanes /
+ * normalVectorListPlanes, diskCenterVectorList / diskNormalVectorList /
+ * diskRadiusList, surfaceVerticesIdx.  Normals are normalised.  Applies to
+ * minimize_newton(_linesearch); masked DOFs take no contact terms. */
+typedef struct {
+    int n_planes;                   /* <= 4 */
+    const double* plane_point;      /* [n_planes][3] */
+    const double* plane_normal;     /* [n_planes][3] */
+    int n_disks;                    /* <= 4 */
+    const double* disk_center;      /* [n_disks][3] */
+    const double* disk_normal;      /* [n_disks][3] */
+    const double* disk_radius;      /* [n_disks] */
+    int n_surface;                  /* candidate nodes; 0 with NULL: every node */
+    const int* surface_nodes;
+    double stiffness;               /* k (N/m per node) */
+    double friction;                /* Coulomb mu (0: frictionless) */
+    double eps_v;                   /* stick/slip velocity threshold (m/s) */
+} sors_contact_desc;
+
+/* Set (or with NULL remove) contact; recomputes the friction lag from the
+ * current state.  Replaces the planeContact / diskContact entries of
+ * Energy::Energy's constraint list (energy.cpp:69-105). */
+int sors_set_contact(sors_ctx* ctx, const sors_contact_desc* desc);
+/* DiskContact::set_disk (cpp/headers/constraints/diskContact.h:72-80): move
+ * disk idx; the displacement since the last completed step drives the
+ * friction of the next step. */
+int sors_set_disk(sors_ctx* ctx, int idx, const double* center, const double* normal, double radius);
+
 /* ---- Serving loop --------------------------------------------------------
  * No counterpart in the reference: its analogue is a sequential Python loop
  * over full rollouts (python/d05a_hopping_grid.py:103-106).  A server keeps

@@ -105,16 +105,23 @@ def lib():
         L.orc_project_psd.argtypes = [C.c_int, _dp]
         L.orc_bench_pcg.restype = C.c_double
         L.orc_bench_pcg.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int, C.c_int]
+        L.orc_set_contact.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_int, _ip,
+                                      C.c_double, C.c_double, C.c_double]
+        L.orc_set_disk.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_double]
+        L.orc_contact_energy.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.orc_contact_energy.restype = C.c_double
+        L.orc_contact_lag.argtypes = [C.c_void_p, C.c_double]
+        L.orc_contact_lag.restype = None
         _lib = L
     return _lib
 
 
 def _d(a):
-    return a.ctypes.data_as(_dp)
+    return a.ctypes.data_as(_dp) if a is not None else None
 
 
 def _i(a):
-    return a.ctypes.data_as(_ip)
+    return a.ctypes.data_as(_ip) if a is not None else None
 
 
 def f64(a):
@@ -314,6 +321,33 @@ class System:
     def bench_pcg(self, q, dt, iters, project=True):
         q = f64(q).reshape(-1)
         return lib().orc_bench_pcg(self.h, _d(q), float(dt), int(iters), int(project))
+
+    def set_contact(self, planes=(), disks=(), surface=None, stiffness=1e5, friction=0.0, eps_v=1e-3):
+        """planes: [(point, normal)], disks: [(center, normal, radius)]; surface:
+        candidate node ids (None: all nodes)."""
+        pp = f64([p for p, _ in planes]).reshape(-1) if planes else None
+        pn = f64([n for _, n in planes]).reshape(-1) if planes else None
+        dc = f64([c for c, _, _ in disks]).reshape(-1) if disks else None
+        dn = f64([n for _, n, _ in disks]).reshape(-1) if disks else None
+        dr = f64([r for _, _, r in disks]) if disks else None
+        self._contact_keep = (pp, pn, dc, dn, dr, i32(surface) if surface is not None else None)
+        s = self._contact_keep[5]
+        rc = lib().orc_set_contact(self.h, len(planes), _d(pp), _d(pn), len(disks), _d(dc), _d(dn), _d(dr),
+                                   0 if s is None else len(s), _i(s), float(stiffness), float(friction),
+                                   float(eps_v))
+        if rc != 0:
+            raise ValueError(f"orc_set_contact: {rc}")
+
+    def set_disk(self, idx, center, normal, radius):
+        rc = lib().orc_set_disk(self.h, int(idx), _d(f64(center)), _d(f64(normal)), float(radius))
+        if rc != 0:
+            raise ValueError(f"orc_set_disk: {rc}")
+
+    def contact_energy(self, q, dt):
+        return lib().orc_contact_energy(self.h, _d(f64(q)), float(dt))
+
+    def contact_lag(self, dt):
+        lib().orc_contact_lag(self.h, float(dt))
 
     def update_state(self, q, dt):
         q = f64(q).reshape(-1)

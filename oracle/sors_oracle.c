@@ -388,6 +388,8 @@ int orc_elem_eval(int ed, const double* Xe, const double* xe, double E, double n
 /* ------------------------------------------------------------------------ */
 /* System level                                                              */
 /* ------------------------------------------------------------------------ */
+#define ORC_MAX_PRIM 4
+#define ORC_DISK_RIM 1e-3   /* rim falloff width: the reference's radial threshold */
 struct orc_sys {
     int N, nel, ed, nd, n, method, scheme, poly;
     int* conn;
@@ -408,6 +410,15 @@ struct orc_sys {
     int* cols;
     long long nnz;
     int nthreads;
+    /* contact (orc_set_contact) */
+    int nplanes, ndisks, nsurf, has_contact;
+    double plane_p[ORC_MAX_PRIM][3], plane_n[ORC_MAX_PRIM][3];
+    double disk_c[ORC_MAX_PRIM][3], disk_n[ORC_MAX_PRIM][3], disk_r[ORC_MAX_PRIM], disk_c_last[ORC_MAX_PRIM][3];
+    int* surf;
+    double kc, muf, eps_v, lag_dt;
+    double* lagN;   /* [nsurf] lagged normal force magnitude */
+    double* lagn;   /* [nsurf][3] its normal */
+    double* lagD;   /* [nsurf][3] displacement of its primitive over the step */
 };
 
 void orc_default_opts(orc_opts* o) {
@@ -497,6 +508,10 @@ void orc_free(orc_sys* s) {
     free(s->rowptr);
     free(s->cols);
     free(s->fext);
+    free(s->surf);
+    free(s->lagN);
+    free(s->lagn);
+    free(s->lagD);
     free(s);
 }
 
@@ -567,6 +582,209 @@ static double integrate_energy(const orc_sys* s, const double* q, double dt, dou
     return kin + grav + damp + eint;
 }
 
+
+/* ---- contact -------------------------------------------------------------
+ * Penalty, per candidate node x and primitive j (plane or disk):
+ *   d = n.(x - p)  (plane point p / disk centre c), active when d < 0;
+ *   E = 1/2 k w d^2,  w = 1 for a plane; for a disk w = s(rho) with
+ *   rho = |(x - c) - d n|, s = 1 for rho <= r, 1 - 3t^2 + 2t^3 with
+ *   t = (rho - r)/delta on the rim (delta = 1e-3 m), 0 beyond;
+ *   gradient  k w d n + 1/2 k d^2 s'(rho) (x - c - d n)/rho;
+ *   Hessian   k w n n^T (rim derivative terms left out: the block stays PSD).
+ * Friction (lagged, incremental-potential form): at the start of a step the
+ * primitive with the largest penalty force k w |d| at qprev gives the node
+ * its normal force N, normal n and the primitive's displacement D over the
+ * step (disk: centre now minus centre at the end of the last step).  With
+ * u = (x - qprev) - D, uT = (I - n n^T) u, y = |uT|, eps = eps_v dt:
+ *   F = mu N phi(y),  phi = y^2 / (2 eps) (y < eps), y - eps/2 (y >= eps);
+ *   gradient mu N min(1/eps, 1/y) uT;
+ *   Hessian  mu N / eps (I - n n^T)             (y < eps)
+ *            mu N / y (I - n n^T - t t^T), t = uT/y   (y >= eps).
+ * Masked DOFs take no contact terms (their rows are the Dirichlet rows). */
+static void normalize3(double* v) {
+    double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (l > 0)
+        for (int k = 0; k < 3; ++k) v[k] /= l;
+}
+
+int orc_set_contact(orc_sys* s, int nplanes, const double* pp, const double* pn, int ndisks, const double* dc,
+                    const double* dn, const double* dr, int nsurf, const int* surf, double stiffness,
+                    double friction, double eps_v) {
+    if (nplanes < 0 || nplanes > ORC_MAX_PRIM || ndisks < 0 || ndisks > ORC_MAX_PRIM || stiffness < 0 ||
+        friction < 0 || (friction > 0 && !(eps_v > 0)))
+        return ORC_ERR_ARG;
+    s->nplanes = nplanes;
+    s->ndisks = ndisks;
+    for (int j = 0; j < nplanes; ++j)
+        for (int k = 0; k < 3; ++k) {
+            s->plane_p[j][k] = pp[3 * j + k];
+            s->plane_n[j][k] = pn[3 * j + k];
+        }
+    for (int j = 0; j < nplanes; ++j) normalize3(s->plane_n[j]); /* Plane3D(point, normal.normalized()) */
+    for (int j = 0; j < ndisks; ++j) {
+        for (int k = 0; k < 3; ++k) {
+            s->disk_c[j][k] = s->disk_c_last[j][k] = dc[3 * j + k];
+            s->disk_n[j][k] = dn[3 * j + k];
+        }
+        normalize3(s->disk_n[j]); /* Disk3D(center, normal.normalized(), radius) */
+        s->disk_r[j] = dr[j];
+    }
+    free(s->surf);
+    free(s->lagN);
+    free(s->lagn);
+    free(s->lagD);
+    s->nsurf = surf ? nsurf : s->N;
+    s->surf = (int*)malloc(sizeof(int) * (s->nsurf > 0 ? s->nsurf : 1));
+    for (int i = 0; i < s->nsurf; ++i) s->surf[i] = surf ? surf[i] : i;
+    s->lagN = (double*)calloc(s->nsurf > 0 ? s->nsurf : 1, sizeof(double));
+    s->lagn = (double*)calloc(3 * (size_t)(s->nsurf > 0 ? s->nsurf : 1), sizeof(double));
+    s->lagD = (double*)calloc(3 * (size_t)(s->nsurf > 0 ? s->nsurf : 1), sizeof(double));
+    s->kc = stiffness;
+    s->muf = friction;
+    s->eps_v = eps_v;
+    s->lag_dt = 0;
+    s->has_contact = (nplanes + ndisks) > 0;
+    orc_contact_lag(s, 0.0);
+    return ORC_OK;
+}
+
+int orc_set_disk(orc_sys* s, int idx, const double c[3], const double n[3], double r) {
+    if (idx < 0 || idx >= s->ndisks) return ORC_ERR_ARG;
+    for (int k = 0; k < 3; ++k) {
+        s->disk_c[idx][k] = c[k];
+        s->disk_n[idx][k] = n[k];
+    }
+    normalize3(s->disk_n[idx]);
+    s->disk_r[idx] = r;
+    return ORC_OK;
+}
+
+/* rim weight s(rho) and ds/drho */
+static double rim_weight(double rho, double r, double* ds) {
+    if (rho <= r) {
+        *ds = 0;
+        return 1.0;
+    }
+    double t = (rho - r) / ORC_DISK_RIM;
+    if (t >= 1.0) {
+        *ds = 0;
+        return 0.0;
+    }
+    *ds = (-6 * t + 6 * t * t) / ORC_DISK_RIM;
+    return 1 - 3 * t * t + 2 * t * t * t;
+}
+
+/* penalty of one node against primitive j (j < nplanes: plane, else disk
+ * j - nplanes) with disk centre cc: returns energy, adds gradient/Hessian */
+static double penalty_term(const orc_sys* s, int j, const double* x, const double* cc, double* grad, double* H) {
+    const double *p, *nn;
+    if (j < s->nplanes) {
+        p = s->plane_p[j];
+        nn = s->plane_n[j];
+    } else {
+        p = cc;
+        nn = s->disk_n[j - s->nplanes];
+    }
+    double v[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
+    double d = nn[0] * v[0] + nn[1] * v[1] + nn[2] * v[2];
+    if (!(d < 0)) return 0.0;
+    double w = 1.0, ds = 0.0, tv[3] = {0, 0, 0}, rho = 0;
+    if (j >= s->nplanes) {
+        for (int k = 0; k < 3; ++k) tv[k] = v[k] - d * nn[k];
+        rho = sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
+        w = rim_weight(rho, s->disk_r[j - s->nplanes], &ds);
+        if (w == 0.0) return 0.0;
+    }
+    if (grad)
+        for (int k = 0; k < 3; ++k) {
+            grad[k] += s->kc * w * d * nn[k];
+            if (ds != 0.0 && rho > 0) grad[k] += 0.5 * s->kc * d * d * ds * tv[k] / rho;
+        }
+    if (H)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) H[3 * a + b] += s->kc * w * nn[a] * nn[b];
+    return 0.5 * s->kc * w * d * d;
+}
+
+static double friction_term(const orc_sys* s, int i, const double* x, const double* xprev, double dt, double* grad,
+                            double* H) {
+    double N = s->lagN[i];
+    if (!(N > 0) || !(s->muf > 0)) return 0.0;
+    const double* nn = s->lagn + 3 * (size_t)i;
+    const double* D = s->lagD + 3 * (size_t)i;
+    double u[3], uT[3];
+    for (int k = 0; k < 3; ++k) u[k] = x[k] - xprev[k] - D[k];
+    double un = nn[0] * u[0] + nn[1] * u[1] + nn[2] * u[2];
+    for (int k = 0; k < 3; ++k) uT[k] = u[k] - un * nn[k];
+    double y = sqrt(uT[0] * uT[0] + uT[1] * uT[1] + uT[2] * uT[2]);
+    double eps = s->eps_v * dt, muN = s->muf * N;
+    if (y < eps) {
+        if (grad)
+            for (int k = 0; k < 3; ++k) grad[k] += muN / eps * uT[k];
+        if (H)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) H[3 * a + b] += muN / eps * ((a == b) - nn[a] * nn[b]);
+        return muN * y * y / (2 * eps);
+    }
+    if (grad)
+        for (int k = 0; k < 3; ++k) grad[k] += muN / y * uT[k];
+    if (H)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                H[3 * a + b] += muN / y * ((a == b) - nn[a] * nn[b] - uT[a] * uT[b] / (y * y));
+    return muN * (y - 0.5 * eps);
+}
+
+/* energy, and optionally gradient (added into g) / node blocks (into Hn,
+ * [nsurf][9]) of all contact terms at q */
+static double contact_eval(const orc_sys* s, const double* q, double dt, double* g, double* Hn) {
+    if (!s->has_contact) return 0.0;
+    double e = 0;
+    for (int i = 0; i < s->nsurf; ++i) {
+        int node = s->surf[i];
+        const double* x = q + 3 * (size_t)node;
+        double gr[3] = {0, 0, 0}, H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < s->nplanes + s->ndisks; ++j)
+            e += penalty_term(s, j, x, j < s->nplanes ? NULL : s->disk_c[j - s->nplanes], gr, H);
+        if (dt != 0) e += friction_term(s, i, x, s->qprev + 3 * (size_t)node, dt, gr, H);
+        if (g)
+            for (int k = 0; k < 3; ++k)
+                if (!s->mask[3 * (size_t)node + k]) g[3 * (size_t)node + k] += gr[k];
+        if (Hn) memcpy(Hn + 9 * (size_t)i, H, sizeof(H));
+    }
+    return e;
+}
+
+double orc_contact_energy(orc_sys* s, const double* q, double dt) { return contact_eval(s, q, dt, NULL, NULL); }
+
+void orc_contact_lag(orc_sys* s, double dt) {
+    s->lag_dt = dt;
+    if (!s->has_contact) return;
+    for (int i = 0; i < s->nsurf; ++i) {
+        const double* x = s->qprev + 3 * (size_t)s->surf[i];
+        double best = 0, bn[3] = {0, 0, 0}, bD[3] = {0, 0, 0};
+        for (int j = 0; j < s->nplanes + s->ndisks; ++j) {
+            double gr[3] = {0, 0, 0};
+            const double* cc = j < s->nplanes ? NULL : s->disk_c_last[j - s->nplanes];
+            penalty_term(s, j, x, cc, gr, NULL);
+            const double* nn = j < s->nplanes ? s->plane_n[j] : s->disk_n[j - s->nplanes];
+            double f = fabs(gr[0] * nn[0] + gr[1] * nn[1] + gr[2] * nn[2]); /* k w |d| */
+            if (f > best) {
+                best = f;
+                for (int k = 0; k < 3; ++k) {
+                    bn[k] = nn[k];
+                    bD[k] = j < s->nplanes ? 0.0 : s->disk_c[j - s->nplanes][k] - s->disk_c_last[j - s->nplanes][k];
+                }
+            }
+        }
+        s->lagN[i] = best;
+        for (int k = 0; k < 3; ++k) {
+            s->lagn[3 * (size_t)i + k] = bn[k];
+            s->lagD[3 * (size_t)i + k] = bD[k];
+        }
+    }
+}
+
 /* Energy::compute_energy (energy.cpp:208-248), external forces omitted */
 double orc_energy(orc_sys* s, const double* q, double dt) {
     double energy = 0;
@@ -581,7 +799,7 @@ double orc_energy(orc_sys* s, const double* q, double dt) {
     (void)bad;
     if (s->fext)
         for (int i = 0; i < s->n; ++i) energy += s->fext[i] * (q[i] - s->qprev[i]);
-    return integrate_energy(s, q, dt, energy);
+    return integrate_energy(s, q, dt, energy) + contact_eval(s, q, dt, NULL, NULL);
 }
 
 /* Energy::compute_gradient (energy.cpp:252-308) + integrate_gradient
@@ -636,6 +854,7 @@ int orc_gradient(orc_sys* s, const double* q, double dt, double* g) {
     } else {
         for (int i = 0; i < n; ++i) g[i] += -s->mass[i] * s->grav[i % 3];
     }
+    contact_eval(s, q, dt, g, NULL);
     if (s->method != ORC_SQP)
         for (int i = 0; i < n; ++i)
             if (s->mask[i]) g[i] = -(s->qprev[i] + dt * s->bcval[i] - q[i]);
@@ -814,6 +1033,20 @@ int orc_hessian(orc_sys* s, const double* q, double dt, int project, double* val
             vals[idx] += (s->scheme == ORC_BE) ? (m / (dt * dt) + s->alpha * m / dt)
                                                : (2 * m / (dt * dt) + 2 * s->alpha * m / dt);
         }
+    }
+    if (s->has_contact) {
+        double* Hn = (double*)malloc(sizeof(double) * 9 * (size_t)(s->nsurf > 0 ? s->nsurf : 1));
+        contact_eval(s, q, dt, NULL, Hn);
+        for (int i = 0; i < s->nsurf; ++i)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    int R = 3 * s->surf[i] + a, C = 3 * s->surf[i] + b;
+                    if (s->mask[R] || s->mask[C]) continue;
+                    double h = Hn[9 * (size_t)i + 3 * a + b];
+                    int idx = csr_find(s, R, C);
+                    if (h != 0.0 && idx >= 0) vals[idx] += h;
+                }
+        free(Hn);
     }
     if (s->method != ORC_SQP)
         for (int i = 0; i < s->n; ++i)
@@ -1120,8 +1353,13 @@ int orc_step(orc_sys* s, double* q, double dt, int substeps, const orc_opts* o, 
     double h = dt / substeps;
     int status = ORC_OK;
     for (int j = 0; j < substeps && status == ORC_OK; ++j) {
+        orc_contact_lag(s, h);
         status = orc_minimize(s, q, h, o, st);
-        if (status == ORC_OK) orc_update_state(s, q, h);
+        if (status == ORC_OK) {
+            orc_update_state(s, q, h);
+            for (int d = 0; d < s->ndisks; ++d)
+                for (int k = 0; k < 3; ++k) s->disk_c_last[d][k] = s->disk_c[d][k];
+        }
     }
     s->nthreads = saved;
     return status;

@@ -118,6 +118,26 @@ int      orc_minimize(orc_sys* s, double* x, double dt, const orc_opts* o, orc_s
  * block-Jacobi apply and both dot products; assembly not timed). */
 double   orc_bench_pcg(orc_sys* s, const double* q, double dt, int iters, int project);
 
+/* ---- contact (penalty + lagged Coulomb friction) ------------------------
+ * A formulation of this build, not of the reference: the reference solves
+ * contact as hard constraints n.(x - p) >= 0 by SQP/OSQP.  Only the
+ * geometry is the reference's: plane distance n.(x - p)
+ * (planeContact.cpp:4-42, plane3D.h:30-33) and the disk's signed plane
+ * distance n.(x - c) and radial distance |(x - c) - (n.(x - c)) n|
+ * (disk3D.h:33-45, diskContact.cpp:4-42).  See sors_oracle.c for the
+ * energy.  surf == NULL: every node is a contact candidate.  Computes the
+ * friction lag from the current qprev. */
+int  orc_set_contact(orc_sys* s, int nplanes, const double* plane_point, const double* plane_normal, int ndisks,
+                     const double* disk_center, const double* disk_normal, const double* disk_radius, int nsurf,
+                     const int* surf, double stiffness, double friction, double eps_v);
+/* DiskContact::set_disk (diskContact.h:72-80): move disk idx (takes effect at
+ * the next step; the displacement since the last step drives friction). */
+int  orc_set_disk(orc_sys* s, int idx, const double c[3], const double n[3], double r);
+/* contact-only energy / gradient (added into g) at q for the current lag */
+double orc_contact_energy(orc_sys* s, const double* q, double dt);
+/* recompute the friction lag from qprev (done by orc_step before each step) */
+void orc_contact_lag(orc_sys* s, double dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,6 +96,24 @@ struct NodeArgs {
     double* gtmp;         // out: internal gradient (for Crank-Nicolson history)
     void* vals;           // HT or null
     const int* diag;
+    const double* gcon;   // contact gradient per DOF (masked DOFs 0) or null
+};
+
+// Contact primitives and per-candidate friction lag (penalty + lagged
+// Coulomb friction).  Geometry: plane n.(x - p) (planeContact.cpp:4-42),
+// disk n.(x - c) and radial |(x - c) - (n.(x - c)) n| (disk3D.h:33-45).
+constexpr int kMaxPrim = 4;
+constexpr double kDiskRim = 1e-3;   // rim falloff width (m)
+struct ContactArgs {
+    int nplanes, ndisks, nsurf, pad;
+    double k, mu, eps_v;
+    double pp[kMaxPrim][3], pn[kMaxPrim][3];
+    double dc[kMaxPrim][3], dn[kMaxPrim][3], dr[kMaxPrim], dc_last[kMaxPrim][3];
+    const int* surf;      // candidate nodes
+    const int* mask;      // per DOF or null
+    double* lagN;         // [nsurf]
+    double* lagn;         // [nsurf][3]
+    double* lagD;         // [nsurf][3]
 };
 
 // host launchers (sors_amd.hip)

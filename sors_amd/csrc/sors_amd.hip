@@ -668,6 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_node_terms(NodeArgs a, double* gg_ou
         } else {
             g = gc + gi;
         }
+        if (a.gcon) g += a.gcon[i];
         const bool dir = !a.sqp && a.mask && a.mask[i];
         if (dir) g = -(a.qprev[i] + a.dt * a.bcval[i] - qi);
         a.g[i] = g;
@@ -729,6 +730,188 @@ __global__ __launch_bounds__(kBlock) void k_node_energy(NodeArgs a, const double
         v[0] += en;
     }
     block_partial<1>(v, red.partials);
+}
+
+// ===========================================================================
+// Contact: penalty 1/2 k w d^2 against planes and disks (d < 0 penetrating;
+// w = 1 for a plane, a C1 cubic rim falloff over kDiskRim outside a disk's
+// radius) plus lagged Coulomb friction mu N phi(|uT|) with phi the
+// eps-smoothed norm (eps = eps_v dt), N / normal / primitive displacement
+// fixed at the start of the step.  Hessian blocks: k w n n^T (rim falloff
+// derivatives left out) and the exact friction blocks, all PSD.  One lane
+// per candidate node; the node's 3x3 self block takes the Hessian.
+// ===========================================================================
+__device__ __forceinline__ double rim_weight_dev(double rho, double r, double* ds) {
+    *ds = 0.0;
+    if (rho <= r) return 1.0;
+    const double t = (rho - r) / kDiskRim;
+    if (t >= 1.0) return 0.0;
+    *ds = (-6.0 * t + 6.0 * t * t) / kDiskRim;
+    return 1.0 - 3.0 * t * t + 2.0 * t * t * t;
+}
+
+// penalty of x against primitive j (planes first, then disks centred at cc)
+__device__ double penalty_dev(const ContactArgs& c, int j, const double* x, const double* cc, double* gr,
+                              double* H) {
+    const bool plane = j < c.nplanes;
+    const double* p = plane ? c.pp[j] : cc;
+    const double* nn = plane ? c.pn[j] : c.dn[j - c.nplanes];
+    const double v0 = x[0] - p[0], v1 = x[1] - p[1], v2 = x[2] - p[2];
+    const double d = nn[0] * v0 + nn[1] * v1 + nn[2] * v2;
+    if (!(d < 0.0)) return 0.0;
+    double w = 1.0, ds = 0.0, rho = 0.0, t[3] = {0.0, 0.0, 0.0};
+    if (!plane) {
+        t[0] = v0 - d * nn[0];
+        t[1] = v1 - d * nn[1];
+        t[2] = v2 - d * nn[2];
+        rho = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        w = rim_weight_dev(rho, c.dr[j - c.nplanes], &ds);
+        if (w == 0.0) return 0.0;
+    }
+    if (gr) {
+        const double kd = c.k * w * d;
+        const double rimf = (ds != 0.0 && rho > 0.0) ? 0.5 * c.k * d * d * ds / rho : 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) gr[a] += kd * nn[a] + rimf * t[a];
+    }
+    if (H)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) H[3 * a + b] += c.k * w * nn[a] * nn[b];
+    return 0.5 * c.k * w * d * d;
+}
+
+__device__ double friction_dev(const ContactArgs& c, int i, const double* x, const double* xprev, double dt,
+                               double* gr, double* H) {
+    const double N = c.lagN[i];
+    if (!(N > 0.0) || !(c.mu > 0.0)) return 0.0;
+    const double* nn = c.lagn + 3 * (size_t)i;
+    const double* D = c.lagD + 3 * (size_t)i;
+    double u[3], uT[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) u[a] = x[a] - xprev[a] - D[a];
+    const double un = nn[0] * u[0] + nn[1] * u[1] + nn[2] * u[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) uT[a] = u[a] - un * nn[a];
+    const double y = sqrt(uT[0] * uT[0] + uT[1] * uT[1] + uT[2] * uT[2]);
+    const double eps = c.eps_v * dt, muN = c.mu * N;
+    const bool stick = y < eps;
+    const double f = stick ? muN / eps : muN / y;
+    if (gr)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) gr[a] += f * uT[a];
+    if (H)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                H[3 * a + b] += f * ((a == b ? 1.0 : 0.0) - nn[a] * nn[b] - (stick ? 0.0 : uT[a] * uT[b] / (y * y)));
+    return stick ? muN * y * y / (2.0 * eps) : muN * (y - 0.5 * eps);
+}
+
+// all contact terms of candidate i at position x (xprev: its qprev)
+__device__ double contact_node(const ContactArgs& c, int i, const double* x, const double* xprev, double dt,
+                               double* gr, double* H) {
+    double e = 0.0;
+    for (int j = 0; j < c.nplanes + c.ndisks; ++j)
+        e += penalty_dev(c, j, x, j < c.nplanes ? nullptr : c.dc[j - c.nplanes], gr, H);
+    if (dt != 0.0) e += friction_dev(c, i, x, xprev, dt, gr, H);
+    return e;
+}
+
+// gradient into gcon (surface DOFs; masked DOFs 0) and, with vals, the
+// Hessian into each node's self block (sliced layout: sbase/self_slot;
+// scalar CSR: rowptr/cols)
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_contact_terms(ContactArgs c, const double* q, const double* qprev,
+                                                          double dt, double* gcon, HT* vals, const int* sbase,
+                                                          const int* self_slot, const int* rowptr, const int* cols) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.nsurf) return;
+    double gr[3] = {0.0, 0.0, 0.0}, H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int node = c.surf[i];
+    contact_node(c, i, q + 3 * (size_t)node, qprev + 3 * (size_t)node, dt, gr, vals ? H : nullptr);
+    bool m[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        m[a] = c.mask && c.mask[3 * node + a];
+        gcon[3 * node + a] = m[a] ? 0.0 : gr[a];
+    }
+    if (!vals) return;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            if (m[a] || m[b] || H[3 * a + b] == 0.0) continue;
+            long long idx = -1;
+            if (sbase) {
+                idx = block_index(sbase, node, self_slot[node], a, b);
+            } else {
+                const int R = 3 * node + a, C = 3 * node + b;
+                int lo = rowptr[R], hi = rowptr[R + 1] - 1;
+                while (lo <= hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (cols[mid] == C) {
+                        idx = mid;
+                        break;
+                    }
+                    if (cols[mid] < C)
+                        lo = mid + 1;
+                    else
+                        hi = mid - 1;
+                }
+            }
+            if (idx >= 0) vals[idx] += (HT)H[3 * a + b];
+        }
+}
+
+// contact energy (diff: E(q + alpha dq) - E(q) per node, then summed)
+__global__ __launch_bounds__(kBlock) void k_contact_energy(ContactArgs c, const double* q, const double* dq,
+                                                           double alpha, int diff, const double* qprev, double dt,
+                                                           Reduce red) {
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.nsurf; i += gridDim.x * blockDim.x) {
+        const int node = c.surf[i];
+        const double* x = q + 3 * (size_t)node;
+        const double* xp = qprev + 3 * (size_t)node;
+        if (diff) {
+            double xt[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) xt[a] = x[a] + alpha * dq[3 * (size_t)node + a];
+            v[0] += contact_node(c, i, xt, xp, dt, nullptr, nullptr) - contact_node(c, i, x, xp, dt, nullptr, nullptr);
+        } else {
+            v[0] += contact_node(c, i, x, xp, dt, nullptr, nullptr);
+        }
+    }
+    block_partial<1>(v, red.partials);
+}
+
+// friction lag from qprev: the primitive with the largest penalty force
+__global__ __launch_bounds__(kBlock) void k_contact_lag(ContactArgs c, const double* qprev) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.nsurf) return;
+    const double* x = qprev + 3 * (size_t)c.surf[i];
+    double best = 0.0, bn[3] = {0.0, 0.0, 0.0}, bD[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < c.nplanes + c.ndisks; ++j) {
+        double gr[3] = {0.0, 0.0, 0.0};
+        const bool plane = j < c.nplanes;
+        penalty_dev(c, j, x, plane ? nullptr : c.dc_last[j - c.nplanes], gr, nullptr);
+        const double* nn = plane ? c.pn[j] : c.dn[j - c.nplanes];
+        const double f = fabs(gr[0] * nn[0] + gr[1] * nn[1] + gr[2] * nn[2]);
+        if (f > best) {
+            best = f;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                bn[a] = nn[a];
+                bD[a] = plane ? 0.0 : c.dc[j - c.nplanes][a] - c.dc_last[j - c.nplanes][a];
+            }
+        }
+    }
+    c.lagN[i] = best;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c.lagn[3 * (size_t)i + a] = bn[a];
+        c.lagD[3 * (size_t)i + a] = bD[a];
+    }
 }
 
 // ===========================================================================
@@ -1334,6 +1517,10 @@ struct sors_ctx {
     bool pat_on_device = false;
     DevMem pcgst_bench, copybuf;
     DevMem sbase, bcol, selfslot;   // sliced node-block layout (node-granular masks)
+    // contact (sors_set_contact): primitives by value, candidates and lag on the device
+    bool has_contact = false;
+    ContactArgs cargs{};
+    DevMem csurf, lagN, lagn, lagD, gcon;
     long long nvals = 0;            // value slots of the active layout
     // captured PCG iteration chunk (one per precision)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -1457,7 +1644,24 @@ NodeArgs node_args(sors_ctx* c, const double* q, double dt) {
     a.g = c->g.as<double>();
     a.gtmp = c->gtmp.as<double>();
     a.diag = c->diag.as<int>();
+    a.gcon = c->has_contact ? c->gcon.as<double>() : nullptr;
     return a;
+}
+
+ContactArgs contact_args(sors_ctx* c) {
+    ContactArgs a = c->cargs;
+    a.surf = c->csurf.as<int>();
+    a.mask = c->has_mask ? c->mask.as<int>() : nullptr;
+    a.lagN = c->lagN.as<double>();
+    a.lagn = c->lagn.as<double>();
+    a.lagD = c->lagD.as<double>();
+    return a;
+}
+
+void contact_lag(sors_ctx* c) {
+    if (!c->has_contact || c->cargs.nsurf == 0) return;
+    k_contact_lag<<<grid_for(c->cargs.nsurf), kBlock, 0, c->stream>>>(contact_args(c), c->qprev.as<double>());
+    HIPX(hipGetLastError());
 }
 
 // element kernel launch; with e0/e1 the dispatch records the two events
@@ -1508,6 +1712,13 @@ void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project
         HIPX(hipEventElapsedTime(&ms, ev[0], ev[1]));
         c->prof->assemble_kernel_ms += ms;
         c->prof->assemble_launches += 1;
+    }
+    if (c->has_contact && c->cargs.nsurf > 0) {
+        const bool blk = c->block();
+        k_contact_terms<HT><<<grid_for(c->cargs.nsurf), kBlock, 0, c->stream>>>(
+            contact_args(c), q, c->qprev.as<double>(), dt, c->gcon.as<double>(), hessian ? (HT*)c->vals.p : nullptr,
+            blk ? c->sbase.as<int>() : nullptr, blk ? c->selfslot.as<int>() : nullptr, c->rowptr.as<int>(),
+            c->cols.as<int>());
     }
     NodeArgs na = node_args(c, q, dt);
     na.vals = hessian ? c->vals.p : nullptr;
@@ -1723,21 +1934,34 @@ void enqueue_energy(sors_ctx* c, const double* q, const double* dx, double alpha
     const int gnn = red_grid(c->n, kBlock);
     k_node_energy<<<gnn, kBlock, 0, c->stream>>>(na, dx, alpha, diff, c->S() + S_ENODE, c->red());
     k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gnn, c->S() + S_ENODE);
+    if (c->has_contact && c->cargs.nsurf > 0) {
+        const int gcn = red_grid(c->cargs.nsurf, kBlock);
+        k_contact_energy<<<gcn, kBlock, 0, c->stream>>>(contact_args(c), q, dx, alpha, diff, c->qprev.as<double>(),
+                                                        dt, c->red());
+        k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gcn, c->S() + S_AUX);
+    }
     HIPX(hipGetLastError());
 }
 
+// the energy scalars summed on the host: element + node (+ contact)
+double energy_sum(const sors_ctx* c, const double* s3) {
+    return s3[0] + s3[1] + ((c->has_contact && c->cargs.nsurf > 0) ? s3[2] : 0.0);
+}
+
+static_assert(S_ENODE == S_EELEM + 1 && S_AUX == S_EELEM + 2, "energy scalars are read as one triple");
+
 double energy_delta(sors_ctx* c, const double* q, const double* dx, double alpha, double dt) {
     enqueue_energy(c, q, dx, alpha, dt);
-    double v[2];
-    read_scalars(c, v, S_EELEM, 2);
-    return v[0] + v[1];
+    double v[3];
+    read_scalars(c, v, S_EELEM, 3);
+    return energy_sum(c, v);
 }
 
 double energy_abs(sors_ctx* c, const double* q, double dt) {
     enqueue_energy(c, q, nullptr, 0.0, dt);
-    double v[2];
-    read_scalars(c, v, S_EELEM, 2);
-    return v[0] + v[1];
+    double v[3];
+    read_scalars(c, v, S_EELEM, 3);
+    return energy_sum(c, v);
 }
 
 // step dots into S[S_DD], S[S_GD] and the step dx = x (FP64)
@@ -1856,12 +2080,14 @@ int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
             k_predict<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
                 c->n, h, c->qprev.as<double>(), c->vprev.as<double>(), c->has_mask ? c->mask.as<int>() : nullptr,
                 c->bcval.as<double>(), c->q.as<double>());
+        contact_lag(c);
         int rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, h, st) : minimize<double>(c, h, st);
         if (rc != SORS_OK) return rc;
         k_update_state<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
             c->n, c->scheme, h, c->q.as<double>(), c->qprev.as<double>(), c->vprev.as<double>(),
             c->gprev.as<double>(), c->gtmp.as<double>());
         HIPX(hipGetLastError());
+        std::memcpy(c->cargs.dc_last, c->cargs.dc, sizeof(c->cargs.dc));
     }
     sync(c);
     st->ms_total = ms_since(t0);
@@ -1924,7 +2150,8 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->fext,   &c->dxd,   &c->qtmp,    &c->rowptr, &c->cols,   &c->diag,    &c->rank,
                            &c->fullmap, &c->vals, &c->Minv,    &c->pr,    &c->pu,      &c->pp,      &c->ps,
                            &c->pw,     &c->px,    &c->partials, &c->scal, &c->pcgst,   &c->pcgctl,  &c->pcgpart,
-                           &c->flag,   &c->sbase, &c->bcol,    &c->selfslot};
+                           &c->flag,   &c->sbase, &c->bcol,    &c->selfslot, &c->csurf, &c->lagN,
+                           &c->lagn,   &c->lagD,  &c->gcon};
     double b = 0;
     for (const DevMem* d : all) b += (double)d->bytes;
     return b;
@@ -2328,11 +2555,11 @@ void srv_tick(sors_server* s) {
                 }
                 m.alpha = 1.0;
                 m.bt = 0;
-                try_alpha(sr.scal[S_EELEM] + sr.scal[S_ENODE]);
+                try_alpha(energy_sum(c, sr.scal + S_EELEM));
                 break;
             }
             case Member::ENERGY:
-                try_alpha(sr.scal[S_EELEM] + sr.scal[S_ENODE]);
+                try_alpha(energy_sum(c, sr.scal + S_EELEM));
                 break;
             default:
                 break;
@@ -2911,6 +3138,90 @@ int sors_bench_kernel(sors_ctx* c, int kernel, int reps, double* avg_ms, double*
         return fail(c, SORS_ERR_ARG, "unknown kernel id");
     }
     if (algo_bytes) *algo_bytes = bytes;
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+// ---- contact ---------------------------------------------------------------
+static bool normalized3(const double* v, double* out) {
+    const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (!(l > 0.0) || !std::isfinite(l)) return false;
+    for (int k = 0; k < 3; ++k) out[k] = v[k] / l;
+    return true;
+}
+
+int sors_set_contact(sors_ctx* c, const sors_contact_desc* d) {
+    SORS_GUARD(c)
+    if (!d) {
+        c->has_contact = false;
+        return SORS_OK;
+    }
+    if (d->n_planes < 0 || d->n_planes > kMaxPrim || d->n_disks < 0 || d->n_disks > kMaxPrim)
+        return fail(c, SORS_ERR_ARG, "at most 4 planes and 4 disks");
+    if ((d->n_planes && (!d->plane_point || !d->plane_normal)) ||
+        (d->n_disks && (!d->disk_center || !d->disk_normal || !d->disk_radius)))
+        return fail(c, SORS_ERR_ARG, "missing contact primitive arrays");
+    if (!(d->stiffness >= 0.0) || !(d->friction >= 0.0) || (d->friction > 0.0 && !(d->eps_v > 0.0)))
+        return fail(c, SORS_ERR_ARG, "contact needs stiffness >= 0, friction >= 0 and eps_v > 0 with friction");
+    if (c->method == SORS_MINIMIZE_SQP)
+        return fail(c, SORS_ERR_UNSUPPORTED, "contact terms apply to minimize_newton(_linesearch)");
+    ContactArgs a{};
+    a.nplanes = d->n_planes;
+    a.ndisks = d->n_disks;
+    a.k = d->stiffness;
+    a.mu = d->friction;
+    a.eps_v = d->eps_v;
+    for (int j = 0; j < a.nplanes; ++j) {
+        for (int k = 0; k < 3; ++k) a.pp[j][k] = d->plane_point[3 * j + k];
+        if (!normalized3(d->plane_normal + 3 * j, a.pn[j])) return fail(c, SORS_ERR_ARG, "zero plane normal");
+    }
+    for (int j = 0; j < a.ndisks; ++j) {
+        for (int k = 0; k < 3; ++k) a.dc[j][k] = a.dc_last[j][k] = d->disk_center[3 * j + k];
+        if (!normalized3(d->disk_normal + 3 * j, a.dn[j])) return fail(c, SORS_ERR_ARG, "zero disk normal");
+        if (!(d->disk_radius[j] >= 0.0)) return fail(c, SORS_ERR_ARG, "negative disk radius");
+        a.dr[j] = d->disk_radius[j];
+    }
+    std::vector<int> surf;
+    if (d->surface_nodes) {
+        if (d->n_surface < 0) return fail(c, SORS_ERR_ARG, "negative surface count");
+        surf.assign(d->surface_nodes, d->surface_nodes + d->n_surface);
+        std::vector<char> seen((size_t)c->N, 0);
+        for (int v : surf) {
+            if (v < 0 || v >= c->N) return fail(c, SORS_ERR_ARG, "surface vertex index out of range");
+            if (seen[(size_t)v]) return fail(c, SORS_ERR_ARG, "surface vertex listed twice");
+            seen[(size_t)v] = 1;
+        }
+    } else {
+        surf.resize((size_t)c->N);
+        for (int v = 0; v < c->N; ++v) surf[(size_t)v] = v;
+    }
+    a.nsurf = (int)surf.size();
+    upload(c->csurf, surf.data(), std::max<size_t>(surf.size(), 1), c->stream);
+    ensure(c->lagN, sizeof(double) * std::max(a.nsurf, 1));
+    ensure(c->lagn, sizeof(double) * 3 * std::max(a.nsurf, 1));
+    ensure(c->lagD, sizeof(double) * 3 * std::max(a.nsurf, 1));
+    ensure(c->gcon, sizeof(double) * c->n);
+    fill<double>(c->gcon.as<double>(), c->n, 0.0, c->stream);
+    fill<double>(c->lagN.as<double>(), std::max(a.nsurf, 1), 0.0, c->stream);
+    c->cargs = a;
+    c->has_contact = (a.nplanes + a.ndisks) > 0;
+    contact_lag(c);
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_set_disk(sors_ctx* c, int idx, const double* center, const double* normal, double radius) {
+    SORS_GUARD(c)
+    if (!c->has_contact || idx < 0 || idx >= c->cargs.ndisks) return fail(c, SORS_ERR_ARG, "diskIdx out of range in set_disk()");
+    if (!center || !normal || !(radius >= 0.0)) return fail(c, SORS_ERR_ARG, "bad disk");
+    double nn[3];
+    if (!normalized3(normal, nn)) return fail(c, SORS_ERR_ARG, "zero disk normal");
+    for (int k = 0; k < 3; ++k) {
+        c->cargs.dc[idx][k] = center[k];
+        c->cargs.dn[idx][k] = nn[k];
+    }
+    c->cargs.dr[idx] = radius;
     return SORS_OK;
     SORS_GUARD_END(c)
 }

@@ -245,3 +245,58 @@ def serving_arrivals(count: int, rate: float = 200.0, seed: int = 0) -> np.ndarr
     """Poisson arrival times (s) of `count` requests at `rate` per second."""
     u = uniform01(seed * 0x2545F4914F6CDD1D + 7, count)
     return np.cumsum(-np.log1p(-u) / rate)
+
+
+def box_surface_nodes(X: np.ndarray, tol: float = 1e-9) -> np.ndarray:
+    """Nodes on the boundary of an axis-aligned box grid (the surface vertex
+    set the reference extracts for contact, io.cpp:570-681)."""
+    lo, hi = X.min(0), X.max(0)
+    on = np.any((np.abs(X - lo) < tol) | (np.abs(X - hi) < tol), axis=1)
+    return np.nonzero(on)[0].astype(np.int32)
+
+
+@dataclasses.dataclass
+class PokeScene:
+    """Contact poke: a cube resting on the ground plane z = 0, indented from
+    above by a rigid disk (normal -z) moving down with a seeded lateral
+    jitter.  disk_center(t) gives the disk centre at the end of step t."""
+
+    scene: Scene
+    surface: np.ndarray
+    stiffness: float
+    friction: float
+    eps_v: float
+    radius: float
+    centers: np.ndarray   # [steps + 1][3]: initial, then after each step
+
+
+def contact_poke(n: int = 40, side: float = 0.05, E: float = 2e5, nu: float = 0.4, rho: float = 1100.0,
+                 stiffness: float = 1e5, friction: float = 0.5, radius: float = 0.01, speed: float = 0.02,
+                 jitter: float = 1e-3, jitter_every: int = 50, gap: float = 5e-4, steps: int = 500,
+                 dt: float = 5e-4, seed: int = 0, eps_v: float = 1e-3) -> PokeScene:
+    """n^3-cell Kuhn tet cube (6 n^3 tets).  The disk starts `gap` above the
+    top face's centre and moves -z at `speed`; its lateral offset is a seeded
+    Gaussian (std `jitter`) drawn every `jitter_every` steps and linearly
+    interpolated in between."""
+    X, hexes = box_grid(n, n, n, side, side, side)
+    conn = kuhn_split(hexes).astype(np.int32)
+    N, nel = X.shape[0], conn.shape[0]
+    sc = Scene(name=f"poke{n}", X=X, conn=conn, youngs=np.full(nel, E), poisson=np.full(nel, nu),
+               density=np.full(nel, rho), bcmask=np.zeros(3 * N, np.int32), bcvalue=np.zeros(3 * N),
+               v0=np.zeros(3 * N), grav=np.array([0.0, 0.0, -9.81]), dt=dt, steps=steps)
+    knots = steps // jitter_every + 2
+    u = uniform01(seed * 0x632BE59BD9B4E019 + 11, 4 * knots).reshape(knots, 4)
+    # Box-Muller from pairs of uniforms
+    r = np.sqrt(-2.0 * np.log1p(-u[:, 0]))
+    g = np.stack([r * np.cos(2 * np.pi * u[:, 1]), r * np.sin(2 * np.pi * u[:, 1])], axis=1) * jitter
+    g[0] = 0.0
+    t = np.arange(steps + 1) / jitter_every
+    k0 = np.minimum(np.floor(t).astype(int), knots - 2)
+    f = (t - k0)[:, None]
+    lateral = (1 - f) * g[k0] + f * g[k0 + 1]
+    c = np.zeros((steps + 1, 3))
+    c[:, 0] = 0.5 * side + lateral[:, 0]
+    c[:, 1] = 0.5 * side + lateral[:, 1]
+    c[:, 2] = side + gap - speed * dt * np.arange(steps + 1)
+    return PokeScene(scene=sc, surface=box_surface_nodes(X), stiffness=stiffness, friction=friction, eps_v=eps_v,
+                     radius=radius, centers=c)

@@ -61,6 +61,15 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ContactDesc(C.Structure):
+    _fields_ = [
+        ("n_planes", C.c_int), ("plane_point", _dp), ("plane_normal", _dp),
+        ("n_disks", C.c_int), ("disk_center", _dp), ("disk_normal", _dp), ("disk_radius", _dp),
+        ("n_surface", C.c_int), ("surface_nodes", _ip),
+        ("stiffness", C.c_double), ("friction", C.c_double), ("eps_v", C.c_double),
+    ]
+
+
 class ServerOptions(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_double), ("max_active", C.c_int), ("prep_threads", C.c_int),
@@ -133,6 +142,8 @@ def lib():
         L.sors_element_eval.argtypes = [vp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]
         L.sors_linear_solve.argtypes = [vp, _dp, C.c_double, _dp, _dp, C.c_double, C.c_int, _ip]
         L.sors_bench_kernel.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
+        L.sors_set_contact.argtypes = [vp, C.POINTER(ContactDesc)]
+        L.sors_set_disk.argtypes = [vp, C.c_int, _dp, _dp, C.c_double]
         L.sors_server_default_options.argtypes = [C.POINTER(ServerOptions)]
         L.sors_server_default_options.restype = None
         L.sors_server_create.argtypes = [C.POINTER(ServerOptions), C.POINTER(vp)]
@@ -232,6 +243,30 @@ class Rollout:
     def _check(self, rc):
         if rc != OK:
             raise SorsError(rc, lib().sors_last_error(self.h).decode())
+
+    # ---- contact -------------------------------------------------------------
+    def set_contact(self, planes=(), disks=(), surface=None, stiffness=1e5, friction=0.0, eps_v=1e-3):
+        """planes: [(point, normal)], disks: [(center, normal, radius)], surface:
+        candidate node ids (None: every node).  None for all three removes contact."""
+        if not planes and not disks and surface is None:
+            self._check(lib().sors_set_contact(self.h, None))
+            return
+        keep = [f64([p for p, _ in planes]).reshape(-1) if planes else None,
+                f64([n for _, n in planes]).reshape(-1) if planes else None,
+                f64([c for c, _, _ in disks]).reshape(-1) if disks else None,
+                f64([n for _, n, _ in disks]).reshape(-1) if disks else None,
+                f64([r for _, _, r in disks]) if disks else None,
+                i32(surface) if surface is not None else None]
+        d = ContactDesc()
+        d.n_planes, d.plane_point, d.plane_normal = len(planes), _d(keep[0]), _d(keep[1])
+        d.n_disks, d.disk_center, d.disk_normal, d.disk_radius = len(disks), _d(keep[2]), _d(keep[3]), _d(keep[4])
+        d.n_surface = 0 if keep[5] is None else len(keep[5])
+        d.surface_nodes = _i(keep[5])
+        d.stiffness, d.friction, d.eps_v = float(stiffness), float(friction), float(eps_v)
+        self._check(lib().sors_set_contact(self.h, C.byref(d)))
+
+    def set_disk(self, idx, center, normal, radius):
+        self._check(lib().sors_set_disk(self.h, int(idx), _d(f64(center)), _d(f64(normal)), float(radius)))
 
     # ---- options / state -------------------------------------------------
     def options(self) -> Options:

@@ -51,3 +51,11 @@ def test_struct_sizes_match_header():
     assert ctypes.sizeof(native.RequestStats) == 4 + 4 + 3 * 8 + 4 * 8
     assert ctypes.sizeof(native.ServerStats) == 3 * 8 + 4 + 4 + 2 * 8 + 2 * 8
     assert ctypes.sizeof(native.ServerOptions) == 8 + 3 * 4 + 4 + ctypes.sizeof(native.Options)
+
+
+def test_contact_entry_points_reject_null_context():
+    L = native.lib()
+    assert L.sors_set_contact(None, None) == native.ERR_ARG
+    assert L.sors_set_disk(None, 0, None, None, 0.0) == native.ERR_ARG
+    # ContactDesc mirrors sors_contact_desc: 3 counts, 6 array pointers, 3 doubles
+    assert ctypes.sizeof(native.ContactDesc) == 3 * 8 + 6 * 8 + 3 * 8
