@@ -177,6 +177,55 @@ def serve_leg(args, opts):
     return out
 
 
+def contact_leg(args, opts):
+    """Contact poke: 40^3-cell Kuhn tet cube (384,000 tets), E = 2e5, nu = 0.4,
+    resting on the ground plane (penalty 1e5 N/m, Coulomb mu = 0.5), a rigid
+    disk (r = 0.01 m) moving -z at 0.02 m/s with a seeded 1 mm lateral
+    jitter; dt = 5e-4, mixed precision.  The first `--contact-check` steps
+    are repeated in plain FP64 for the error."""
+    from sors_amd import mesh, native
+
+    p = mesh.contact_poke(n=40, steps=args.contact_steps, seed=0)
+    sc = p.scene
+    cfg = dict(planes=[((0.0, 0.0, 0.0), (0.0, 0.0, 1.0))], disks=[(p.centers[0], (0.0, 0.0, -1.0), p.radius)],
+               surface=p.surface, stiffness=p.stiffness, friction=p.friction, eps_v=p.eps_v)
+
+    def run(precision, steps):
+        r = native.Rollout.from_scene(sc, precision=precision, **opts)
+        r.set_contact(**cfg)
+        agg = {"newton_iters": 0, "pcg_iters": 0, "backtracks": 0}
+        t0 = time.perf_counter()
+        for k in range(steps):
+            r.set_disk(0, p.centers[k + 1], (0.0, 0.0, -1.0), p.radius)
+            st = r.step_resident(sc.dt)
+            for key in agg:
+                agg[key] += getattr(st, key)
+        el = time.perf_counter() - t0
+        q = r.positions().reshape(-1, 3)
+        r.close()
+        return q, el, agg
+
+    q, el, agg = run(native.MIXED, sc.steps)
+    c = p.centers[sc.steps]
+    top = sc.X[:, 2] > sc.X[:, 2].max() - 1e-9
+    under = top & (np.linalg.norm(q[:, :2] - c[:2], axis=1) < p.radius)
+    out = {"value": sc.steps / el, "unit": "steps/s", "steps": sc.steps, "ms_per_step": 1e3 * el / sc.steps,
+           "workload": f"{sc.name}: {sc.n_elems} tet4, {sc.n_nodes} nodes, {len(p.surface)} surface nodes; "
+                       f"ground penalty 1e5 N/m, mu 0.5, disk r 0.01 m at 0.02 m/s, dt 5e-4, mixed",
+           "newton_per_step": agg["newton_iters"] / sc.steps, "pcg_per_step": agg["pcg_iters"] / sc.steps,
+           "max_ground_penetration_m": float(max(0.0, -q[:, 2].min())),
+           "indentation_m": float(sc.X[:, 2].max() - q[under, 2].min()) if under.any() else 0.0}
+    log(f"contact: {out}")
+    if args.contact_check > 0:
+        k = min(args.contact_check, sc.steps)
+        qm, elm, _ = run(native.MIXED, k)
+        q64, el64, agg64 = run(native.FP64, k)
+        out["check_steps"] = k
+        out["max_pos_err_over_bbox_mixed_vs_fp64"] = float(np.abs(qm - q64).max() / sc.bbox_diag)
+        out["check_steps_per_s"] = {"mixed": k / elm, "fp64": k / el64}
+    return out
+
+
 def load_traffic(key):
     try:
         with open(TRAFFIC_FILE) as f:
@@ -230,6 +279,9 @@ def main():
     ap.add_argument("--serve-prep-threads", type=int, default=8)
     ap.add_argument("--serve-chunk", type=int, default=16)
     ap.add_argument("--no-serve-seq", action="store_true", help="skip the sequential FP64-GPU serving baseline")
+    ap.add_argument("--contact-steps", type=int, default=500, help="contact poke steps (0: skip the contact leg)")
+    ap.add_argument("--contact-check", type=int, default=50, help="contact steps repeated in FP64 for the error")
+    ap.add_argument("--contact-only", action="store_true", help="run only the contact leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -240,6 +292,10 @@ def main():
         return
     from sors_amd import native
 
+    if args.contact_only:
+        opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=int(not args.no_predict))
+        print(json.dumps({"metric": METRIC, "contact": contact_leg(args, opts)}), flush=True)
+        return
     if args.serve_only:
         opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=int(not args.no_predict))
         print(json.dumps({"metric": METRIC, "serving": serve_leg(args, opts)}), flush=True)
@@ -314,6 +370,9 @@ def main():
         cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sc, agg, args.steps, cores)
         log(f"cpu baseline: {out['cpu_baseline']}")
+    if args.contact_steps > 0:
+        out["contact"] = contact_leg(args, dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol,
+                                                predict=int(not args.no_predict)))
     if args.serve > 0:
         out["serving"] = serve_leg(args, dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol,
                                               predict=int(not args.no_predict)))

@@ -587,20 +587,25 @@ static double integrate_energy(const orc_sys* s, const double* q, double dt, dou
  * Penalty, per candidate node x and primitive j (plane or disk):
  *   d = n.(x - p)  (plane point p / disk centre c), active when d < 0;
  *   E = 1/2 k w d^2,  w = 1 for a plane; for a disk w = s(rho) with
- *   rho = |(x - c) - d n|, s = 1 for rho <= r, 1 - 3t^2 + 2t^3 with
- *   t = (rho - r)/delta on the rim (delta = 1e-3 m), 0 beyond;
- *   gradient  k w d n + 1/2 k d^2 s'(rho) (x - c - d n)/rho;
- *   Hessian   k w n n^T (rim derivative terms left out: the block stays PSD).
- * Friction (lagged, incremental-potential form): at the start of a step the
- * primitive with the largest penalty force k w |d| at qprev gives the node
- * its normal force N, normal n and the primitive's displacement D over the
- * step (disk: centre now minus centre at the end of the last step).  With
- * u = (x - qprev) - D, uT = (I - n n^T) u, y = |uT|, eps = eps_v dt:
- *   F = mu N phi(y),  phi = y^2 / (2 eps) (y < eps), y - eps/2 (y >= eps);
- *   gradient mu N min(1/eps, 1/y) uT;
- *   Hessian  mu N / eps (I - n n^T)             (y < eps)
- *            mu N / y (I - n n^T - t t^T), t = uT/y   (y >= eps).
- * Masked DOFs take no contact terms (their rows are the Dirichlet rows). */
+ *   t = (x - c) - d n, rho = |t|, s = 1 for rho <= r, 1 - 3z^2 + 2z^3 with
+ *   z = (rho - r)/delta on the rim (delta = 1e-3 m), 0 beyond;
+ *   gradient  k w d n + 1/2 k d^2 s' t/rho;
+ *   Hessian   k w n n^T + k d s' (n th^T + th n^T)
+ *             + 1/2 k d^2 (s'' th th^T + s' (I - n n^T - th th^T)/rho), th = t/rho.
+ * Friction (lagged, incremental-potential form; the C2 smoothing f0 of Li et
+ * al., "Incremental Potential Contact", SIGGRAPH 2020, Sec. 5): at the start
+ * of a step the primitive with the largest penalty force k w |d| at qprev
+ * gives the node its normal force N, normal n and the primitive's
+ * displacement D over the step (disk: centre now minus centre at the end of
+ * the last step).  With u = (x - qprev) - D, uT = (I - n n^T) u, y = |uT|,
+ * eps = eps_v dt:
+ *   F = mu N f0(y),  f0 = -y^3/(3 eps^2) + y^2/eps + eps/3 (y < eps), y (y >= eps);
+ *   gradient mu N (f1(y)/y) uT, f1 = f0' = -y^2/eps^2 + 2y/eps (y < eps), 1;
+ *   Hessian  mu N [(f1/y) P - uT uT^T/(y eps^2)]   (y < eps)
+ *            mu N / y (P - uT uT^T/y^2)            (y >= eps),  P = I - n n^T.
+ * The node's 3x3 contact block (all primitives + friction) is projected to
+ * PSD (eigenvalues clamped at 0); it only differs from the exact block on a
+ * disk rim.  Masked DOFs take no contact terms. */
 static void normalize3(double* v) {
     double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     if (l > 0)
@@ -659,19 +664,16 @@ int orc_set_disk(orc_sys* s, int idx, const double c[3], const double n[3], doub
     return ORC_OK;
 }
 
-/* rim weight s(rho) and ds/drho */
-static double rim_weight(double rho, double r, double* ds) {
-    if (rho <= r) {
-        *ds = 0;
-        return 1.0;
-    }
-    double t = (rho - r) / ORC_DISK_RIM;
-    if (t >= 1.0) {
-        *ds = 0;
-        return 0.0;
-    }
-    *ds = (-6 * t + 6 * t * t) / ORC_DISK_RIM;
-    return 1 - 3 * t * t + 2 * t * t * t;
+/* rim weight s(rho), ds/drho and d2s/drho2 */
+static double rim_weight(double rho, double r, double* ds, double* d2s) {
+    *ds = 0;
+    *d2s = 0;
+    if (rho <= r) return 1.0;
+    double z = (rho - r) / ORC_DISK_RIM;
+    if (z >= 1.0) return 0.0;
+    *ds = (-6 * z + 6 * z * z) / ORC_DISK_RIM;
+    *d2s = (-6 + 12 * z) / (ORC_DISK_RIM * ORC_DISK_RIM);
+    return 1 - 3 * z * z + 2 * z * z * z;
 }
 
 /* penalty of one node against primitive j (j < nplanes: plane, else disk
@@ -688,22 +690,33 @@ static double penalty_term(const orc_sys* s, int j, const double* x, const doubl
     double v[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
     double d = nn[0] * v[0] + nn[1] * v[1] + nn[2] * v[2];
     if (!(d < 0)) return 0.0;
-    double w = 1.0, ds = 0.0, tv[3] = {0, 0, 0}, rho = 0;
+    double w = 1.0, ds = 0.0, d2s = 0.0, th[3] = {0, 0, 0}, rho = 0;
     if (j >= s->nplanes) {
+        double tv[3];
         for (int k = 0; k < 3; ++k) tv[k] = v[k] - d * nn[k];
         rho = sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
-        w = rim_weight(rho, s->disk_r[j - s->nplanes], &ds);
+        w = rim_weight(rho, s->disk_r[j - s->nplanes], &ds, &d2s);
         if (w == 0.0) return 0.0;
+        if (rho > 0)
+            for (int k = 0; k < 3; ++k) th[k] = tv[k] / rho;
+        else
+            ds = d2s = 0.0; /* on the axis: inside the radius */
     }
+    double kc = s->kc;
     if (grad)
-        for (int k = 0; k < 3; ++k) {
-            grad[k] += s->kc * w * d * nn[k];
-            if (ds != 0.0 && rho > 0) grad[k] += 0.5 * s->kc * d * d * ds * tv[k] / rho;
-        }
+        for (int k = 0; k < 3; ++k) grad[k] += kc * w * d * nn[k] + 0.5 * kc * d * d * ds * th[k];
     if (H)
         for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) H[3 * a + b] += s->kc * w * nn[a] * nn[b];
-    return 0.5 * s->kc * w * d * d;
+            for (int b = 0; b < 3; ++b) {
+                double h = kc * w * nn[a] * nn[b];
+                if (ds != 0.0 || d2s != 0.0) {
+                    double P = (a == b) - nn[a] * nn[b] - th[a] * th[b];
+                    h += kc * d * ds * (nn[a] * th[b] + th[a] * nn[b]) +
+                         0.5 * kc * d * d * (d2s * th[a] * th[b] + ds * P / rho);
+                }
+                H[3 * a + b] += h;
+            }
+    return 0.5 * kc * w * d * d;
 }
 
 static double friction_term(const orc_sys* s, int i, const double* x, const double* xprev, double dt, double* grad,
@@ -718,21 +731,28 @@ static double friction_term(const orc_sys* s, int i, const double* x, const doub
     for (int k = 0; k < 3; ++k) uT[k] = u[k] - un * nn[k];
     double y = sqrt(uT[0] * uT[0] + uT[1] * uT[1] + uT[2] * uT[2]);
     double eps = s->eps_v * dt, muN = s->muf * N;
+    double f1y, c2, f0; /* f1(y)/y and the coefficient of uT uT^T */
     if (y < eps) {
-        if (grad)
-            for (int k = 0; k < 3; ++k) grad[k] += muN / eps * uT[k];
-        if (H)
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b) H[3 * a + b] += muN / eps * ((a == b) - nn[a] * nn[b]);
-        return muN * y * y / (2 * eps);
+        f1y = -y / (eps * eps) + 2.0 / eps;
+        c2 = y > 0 ? -1.0 / (y * eps * eps) : 0.0;
+        f0 = -y * y * y / (3 * eps * eps) + y * y / eps + eps / 3;
+    } else {
+        f1y = 1.0 / y;
+        c2 = -1.0 / (y * y * y);
+        f0 = y;
     }
     if (grad)
-        for (int k = 0; k < 3; ++k) grad[k] += muN / y * uT[k];
+        for (int k = 0; k < 3; ++k) grad[k] += muN * f1y * uT[k];
     if (H)
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b)
-                H[3 * a + b] += muN / y * ((a == b) - nn[a] * nn[b] - uT[a] * uT[b] / (y * y));
-    return muN * (y - 0.5 * eps);
+                H[3 * a + b] += muN * (f1y * ((a == b) - nn[a] * nn[b]) + c2 * uT[a] * uT[b]);
+    return muN * f0;
+}
+
+/* symmetric 3x3: clamp negative eigenvalues to 0 (cyclic Jacobi) */
+static void project_psd3(double* H) {
+    orc_project_psd(3, H);
 }
 
 /* energy, and optionally gradient (added into g) / node blocks (into Hn,
@@ -747,6 +767,7 @@ static double contact_eval(const orc_sys* s, const double* q, double dt, double*
         for (int j = 0; j < s->nplanes + s->ndisks; ++j)
             e += penalty_term(s, j, x, j < s->nplanes ? NULL : s->disk_c[j - s->nplanes], gr, H);
         if (dt != 0) e += friction_term(s, i, x, s->qprev + 3 * (size_t)node, dt, gr, H);
+        if (Hn) project_psd3(H);
         if (g)
             for (int k = 0; k < 3; ++k)
                 if (!s->mask[3 * (size_t)node + k]) g[3 * (size_t)node + k] += gr[k];

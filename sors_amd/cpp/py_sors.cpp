@@ -1,11 +1,14 @@
 // py_sors.cpp -- pybind11 module with the reference's Python surface
 // (cpp/py_sors.cpp:25-261) over the device path: Energy(...), EnergyTet/Hex,
 // Solver(settings), SolverTet/Hex.step, Params, SimulationSettings and the
-// structured-mesh IO statics.  numpy arrays are copied in and out like the
+// structured-mesh IO statics, Plane3D and the DiskContact handle.  numpy arrays are copied in and out like the
 // reference's Eigen conversions; step() releases the GIL (an addition).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
+
+#include <array>
+#include <cmath>
 
 #include "sors_facade.h"
 
@@ -190,6 +193,51 @@ py::array_t<int> hex_to_5_tets(const py::handle& hexes) {
 
 struct IO {};
 
+// Plane3D (cpp/headers/geometries/plane3D.h:15-89): a point and a unit normal
+struct Plane3D {
+    double p[3] = {0.0, 0.0, 0.0};
+    double n[3] = {0.0, 0.0, 1.0};
+    static std::array<double, 3> vec3(const py::handle& v) {
+        auto a = to_vec(v);
+        if (a.size() != 3) throw std::invalid_argument("expected a 3-vector");
+        return {a[0], a[1], a[2]};
+    }
+    void set_point(const py::handle& v) {
+        auto a = vec3(v);
+        std::copy(a.begin(), a.end(), p);
+    }
+    void set_normal(const py::handle& v) {  // plane3D.h:73-76 stores the normal as given
+        auto a = vec3(v);
+        std::copy(a.begin(), a.end(), n);
+    }
+    double distance(const py::handle& x) const {
+        auto a = vec3(x);
+        return n[0] * (a[0] - p[0]) + n[1] * (a[1] - p[1]) + n[2] * (a[2] - p[2]);
+    }
+};
+
+template <int E>
+DiskContactHandle<E> disk_handle(Energy<E>& en, int idx) {
+    if (idx < 0 || idx >= (int)en.constraintTypesList_.size()) throw py::index_error("constraint index out of range");
+    if (en.constraintTypesList_[idx] != "diskContact") throw py::value_error("constraint at index is not DiskContact");
+    return DiskContactHandle<E>{&en};
+}
+
+template <int E>
+void bind_disk(py::module_& m, const char* name) {
+    py::class_<DiskContactHandle<E>>(m, name)
+        .def(
+            "set_disk",
+            [](DiskContactHandle<E>& h, const py::handle& c, const py::handle& nrm, double r, int idx) {
+                auto cc = Plane3D::vec3(c), nn = Plane3D::vec3(nrm);
+                h.set_disk(cc.data(), nn.data(), r, idx);
+            },
+            py::arg("center"), py::arg("normal"), py::arg("radius"), py::arg("diskIdx") = 0);
+    m.def(
+        "get_disk_contact_constraint", [](Energy<E>& en, int idx) { return disk_handle<E>(en, idx); },
+        py::arg("energy"), py::arg("idx"), py::keep_alive<0, 1>());
+}
+
 }  // namespace
 
 PYBIND11_MODULE(py_sors, m) {
@@ -255,6 +303,36 @@ PYBIND11_MODULE(py_sors, m) {
             return py::cast(new Solver<8>(settings), py::return_value_policy::take_ownership);
         throw std::runtime_error("Unknown mesh type: " + settings.meshType);
     });
+
+    // py_sors.cpp:216-227
+    py::class_<Plane3D>(m, "Plane3D")
+        .def(py::init<>())
+        .def(py::init([](const py::handle& point, const py::handle& normal) {
+                 Plane3D pl;
+                 pl.set_point(point);
+                 auto a = Plane3D::vec3(normal);
+                 const double l = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+                 if (!(l > 0.0)) throw std::invalid_argument("plane normal must be nonzero");
+                 for (int k = 0; k < 3; ++k) pl.n[k] = a[k] / l;  // normal.normalized() (plane3D.h:21-22)
+                 return pl;
+             }),
+             py::arg("point"), py::arg("normal"))
+        .def("distance_to_point", [](const Plane3D& pl, const py::handle& x) { return pl.distance(x); })
+        .def("is_point_on_plane",
+             [](const Plane3D& pl, const py::handle& x) { return std::fabs(pl.distance(x)) < 1e-12; })
+        .def("project_point_on_plane",
+             [](const Plane3D& pl, const py::handle& x) {
+                 auto a = Plane3D::vec3(x);
+                 const double d = pl.distance(x);
+                 return from_vec({a[0] - d * pl.n[0], a[1] - d * pl.n[1], a[2] - d * pl.n[2]});
+             })
+        .def("get_point", [](const Plane3D& pl) { return from_vec({pl.p[0], pl.p[1], pl.p[2]}); })
+        .def("set_point", &Plane3D::set_point)
+        .def("get_normal", [](const Plane3D& pl) { return from_vec({pl.n[0], pl.n[1], pl.n[2]}); })
+        .def("set_normal", &Plane3D::set_normal);
+
+    bind_disk<4>(m, "DiskContact");
+    bind_disk<8>(m, "DiskContactHex");
 
     py::class_<IO>(m, "IO")
         .def_static("make_voxel_hex_grid", &make_voxel_hex_grid, py::arg("nx"), py::arg("ny"), py::arg("nz"),

@@ -138,7 +138,36 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
     }
     std::vector<int> mask;
     std::vector<double> bcv;
+    // planeContact / diskContact (energy.cpp:69-105): the device resolves them
+    // with a penalty + lagged Coulomb friction (sors_set_contact)
+    std::vector<double> pp, pn, dc, dn, dr;
+    std::vector<int> surf;
+    bool haveSurf = false;
+    constraintTypesList_ = constraintTypesList;
+    auto flat = [&](const char* key) { return constraintParameterList.get_value(key).row_major(); };
     for (const auto& c : constraintTypesList) {
+        if (c == "planeContact" || c == "diskContact") {
+            if (c == "planeContact") {
+                pp = flat("positionVectorListPlanes");
+                pn = flat("normalVectorListPlanes");
+                if (pp.size() != pn.size() || pp.size() % 3 != 0)
+                    throw std::invalid_argument("planeContact: positions / normals must be lists of 3-vectors");
+            } else {
+                dc = flat("diskCenterVectorList");
+                dn = flat("diskNormalVectorList");
+                dr = flat("diskRadiusList");
+                if (dc.size() != dn.size() || dc.size() % 3 != 0 || dr.size() * 3 != dc.size())
+                    throw std::invalid_argument("DiskContact: centers/normals must be same length and multiples of 3.");
+            }
+            auto sv = flat("surfaceVerticesIdx");
+            if (sv.empty()) throw std::invalid_argument("Surface vertices indices cannot be empty for contact.");
+            if (!haveSurf) {
+                for (double v : sv) surf.push_back((int)v);
+                haveSurf = true;
+            }
+            hasContact_ = true;
+            continue;
+        }
         if (c == "neumannBC") {
             // energy.cpp:61-68
             Mat m = constraintParameterList.get_value("neumannBCmask");
@@ -150,7 +179,8 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
             for (size_t i = 0; i < mr.size(); ++i) mask[i] = (int)mr[i];
             bcv = vr;
         } else {
-            throw std::runtime_error("Unknown constraint type: " + c + " (the device path implements neumannBC)");
+            throw std::runtime_error("Unknown constraint type: " + c +
+                                     " (the device path implements neumannBC, planeContact, diskContact)");
         }
     }
     for (const auto& f : forceTypesList)
@@ -181,16 +211,42 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
     else if (settings.solverMethod == "minimize_newton_linesearch")
         d.method = SORS_MINIMIZE_NEWTON_LINESEARCH;
     else if (settings.solverMethod == "minimize_SQP")
-        d.method = SORS_MINIMIZE_SQP;
+        // the reference's contact solver; with contact the device runs its
+        // penalty formulation under Newton + line search
+        d.method = hasContact_ ? SORS_MINIMIZE_NEWTON_LINESEARCH : SORS_MINIMIZE_SQP;
     else
         throw std::runtime_error("Solver method not recognized");
     check_status(sors_create(&d, &ctx_), nullptr);
+    if (hasContact_) {
+        sors_contact_desc cd{};
+        cd.n_planes = (int)(pp.size() / 3);
+        cd.plane_point = pp.data();
+        cd.plane_normal = pn.data();
+        cd.n_disks = (int)dr.size();
+        cd.disk_center = dc.data();
+        cd.disk_normal = dn.data();
+        cd.disk_radius = dr.data();
+        cd.n_surface = (int)surf.size();
+        cd.surface_nodes = surf.data();
+        auto opt = [&](const char* key, double def) {
+            return constraintParameterList.has(key) ? constraintParameterList.get_value(key)(0, 0) : def;
+        };
+        cd.stiffness = opt("contactStiffness", 1e5);
+        cd.friction = opt("frictionCoefficient", 0.0);
+        cd.eps_v = opt("frictionVelocity", 1e-3);
+        check_status(sors_set_contact(ctx_, &cd), ctx_);
+    }
     sors_options o;
     sors_default_options(&o);
     o.precision = settings.precision == "fp64" ? SORS_FP64 : SORS_MIXED;
     o.pcg_rtol = settings.pcgTolerance;
     o.predict = settings.predictor ? 1 : 0;
     check_status(sors_set_options(ctx_, &o), ctx_);
+}
+
+template <int E>
+void Energy<E>::set_disk(const double center[3], const double normal[3], double radius, int diskIdx) {
+    check_status(sors_set_disk(ctx_, diskIdx, center, normal, radius), ctx_);
 }
 
 template <int E>
@@ -261,7 +317,9 @@ std::vector<double> Solver<E>::step(const std::vector<double>& solution, Energy<
     if (en.settings_.CFLtimeSteppingFlag)
         throw std::runtime_error("CFL substepping is not implemented on the device path");
     if (!(dt > 0.0)) throw std::invalid_argument("dt must be positive (settings.dt is not set)");
-    if (en.settings_.solverMethod != "minimize_newton" && en.settings_.solverMethod != "minimize_newton_linesearch")
+    const bool sqpContact = en.settings_.solverMethod == "minimize_SQP" && en.has_contact();
+    if (en.settings_.solverMethod != "minimize_newton" && en.settings_.solverMethod != "minimize_newton_linesearch" &&
+        !sqpContact)
         throw std::runtime_error("Solver method not recognized");
     if ((int)solution.size() != 3 * en.num_vertices()) throw std::invalid_argument("solution must have 3N entries");
     en.apply_actuation(act);

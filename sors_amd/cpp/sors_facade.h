@@ -101,14 +101,30 @@ public:
     std::vector<double> compute_gradient(const std::vector<double>& q, double dt, const Params& actuation);
     void apply_actuation(const Params& actuation);
 
+    // DiskContact::set_disk (cpp/headers/constraints/diskContact.h:72-80)
+    void set_disk(const double center[3], const double normal[3], double radius, int diskIdx);
+    bool has_contact() const { return hasContact_; }
+
     sors_ctx* ctx() const { return ctx_; }
     int num_vertices() const { return numVertices_; }
     SimulationSettings settings_;
     std::vector<std::string> forceTypesList_;
+    std::vector<std::string> constraintTypesList_;
 
 private:
     sors_ctx* ctx_ = nullptr;
     int numVertices_ = 0;
+    bool hasContact_ = false;
+};
+
+// Non-owning handle on the diskContact entry of an Energy's constraint list
+// (py_sors.cpp:231-261 get_disk_contact_constraint / DiskContact.set_disk)
+template <int E>
+struct DiskContactHandle {
+    Energy<E>* energy;
+    void set_disk(const double center[3], const double normal[3], double radius, int diskIdx = 0) {
+        energy->set_disk(center, normal, radius, diskIdx);
+    }
 };
 
 // Solver<3,E> (cpp/headers/solver.h:27-95)

@@ -735,19 +735,23 @@ __global__ __launch_bounds__(kBlock) void k_node_energy(NodeArgs a, const double
 // ===========================================================================
 // Contact: penalty 1/2 k w d^2 against planes and disks (d < 0 penetrating;
 // w = 1 for a plane, a C1 cubic rim falloff over kDiskRim outside a disk's
-// radius) plus lagged Coulomb friction mu N phi(|uT|) with phi the
-// eps-smoothed norm (eps = eps_v dt), N / normal / primitive displacement
-// fixed at the start of the step.  Hessian blocks: k w n n^T (rim falloff
-// derivatives left out) and the exact friction blocks, all PSD.  One lane
-// per candidate node; the node's 3x3 self block takes the Hessian.
+// radius) plus lagged Coulomb friction mu N f0(|uT|) with the C2 smoothing
+// f0 of Li et al., "Incremental Potential Contact" (SIGGRAPH 2020, Sec. 5),
+// eps = eps_v dt; N, the normal and the primitive's displacement are fixed
+// at the start of the step.  Exact gradients and Hessian blocks; each
+// node's 3x3 contact block is projected to PSD (it differs from the exact
+// block only on a disk rim).  One lane per candidate node; the node's 3x3
+// self block takes the Hessian.
 // ===========================================================================
-__device__ __forceinline__ double rim_weight_dev(double rho, double r, double* ds) {
+__device__ __forceinline__ double rim_weight_dev(double rho, double r, double* ds, double* d2s) {
     *ds = 0.0;
+    *d2s = 0.0;
     if (rho <= r) return 1.0;
-    const double t = (rho - r) / kDiskRim;
-    if (t >= 1.0) return 0.0;
-    *ds = (-6.0 * t + 6.0 * t * t) / kDiskRim;
-    return 1.0 - 3.0 * t * t + 2.0 * t * t * t;
+    const double z = (rho - r) / kDiskRim;
+    if (z >= 1.0) return 0.0;
+    *ds = (-6.0 * z + 6.0 * z * z) / kDiskRim;
+    *d2s = (-6.0 + 12.0 * z) / (kDiskRim * kDiskRim);
+    return 1.0 - 3.0 * z * z + 2.0 * z * z * z;
 }
 
 // penalty of x against primitive j (planes first, then disks centred at cc)
@@ -759,26 +763,38 @@ __device__ double penalty_dev(const ContactArgs& c, int j, const double* x, cons
     const double v0 = x[0] - p[0], v1 = x[1] - p[1], v2 = x[2] - p[2];
     const double d = nn[0] * v0 + nn[1] * v1 + nn[2] * v2;
     if (!(d < 0.0)) return 0.0;
-    double w = 1.0, ds = 0.0, rho = 0.0, t[3] = {0.0, 0.0, 0.0};
+    double w = 1.0, ds = 0.0, d2s = 0.0, rho = 0.0, th[3] = {0.0, 0.0, 0.0};
     if (!plane) {
-        t[0] = v0 - d * nn[0];
-        t[1] = v1 - d * nn[1];
-        t[2] = v2 - d * nn[2];
-        rho = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-        w = rim_weight_dev(rho, c.dr[j - c.nplanes], &ds);
+        const double t0 = v0 - d * nn[0], t1 = v1 - d * nn[1], t2 = v2 - d * nn[2];
+        rho = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+        w = rim_weight_dev(rho, c.dr[j - c.nplanes], &ds, &d2s);
         if (w == 0.0) return 0.0;
+        if (rho > 0.0) {
+            th[0] = t0 / rho;
+            th[1] = t1 / rho;
+            th[2] = t2 / rho;
+        } else {
+            ds = d2s = 0.0;
+        }
     }
-    if (gr) {
-        const double kd = c.k * w * d;
-        const double rimf = (ds != 0.0 && rho > 0.0) ? 0.5 * c.k * d * d * ds / rho : 0.0;
+    if (gr)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) gr[a] += kd * nn[a] + rimf * t[a];
-    }
-    if (H)
+        for (int a = 0; a < 3; ++a) gr[a] += c.k * w * d * nn[a] + 0.5 * c.k * d * d * ds * th[a];
+    if (H) {
+        const bool rim = ds != 0.0 || d2s != 0.0;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 3; ++b) H[3 * a + b] += c.k * w * nn[a] * nn[b];
+            for (int b = 0; b < 3; ++b) {
+                double h = c.k * w * nn[a] * nn[b];
+                if (rim) {
+                    const double P = (a == b ? 1.0 : 0.0) - nn[a] * nn[b] - th[a] * th[b];
+                    h += c.k * d * ds * (nn[a] * th[b] + th[a] * nn[b]) +
+                         0.5 * c.k * d * d * (d2s * th[a] * th[b] + ds * P / rho);
+                }
+                H[3 * a + b] += h;
+            }
+    }
     return 0.5 * c.k * w * d * d;
 }
 
@@ -796,18 +812,41 @@ __device__ double friction_dev(const ContactArgs& c, int i, const double* x, con
     for (int a = 0; a < 3; ++a) uT[a] = u[a] - un * nn[a];
     const double y = sqrt(uT[0] * uT[0] + uT[1] * uT[1] + uT[2] * uT[2]);
     const double eps = c.eps_v * dt, muN = c.mu * N;
-    const bool stick = y < eps;
-    const double f = stick ? muN / eps : muN / y;
+    double f1y, c2, f0;  // f1(y)/y, coefficient of uT uT^T, f0(y)
+    if (y < eps) {
+        f1y = -y / (eps * eps) + 2.0 / eps;
+        c2 = y > 0.0 ? -1.0 / (y * eps * eps) : 0.0;
+        f0 = -y * y * y / (3.0 * eps * eps) + y * y / eps + eps / 3.0;
+    } else {
+        f1y = 1.0 / y;
+        c2 = -1.0 / (y * y * y);
+        f0 = y;
+    }
     if (gr)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) gr[a] += f * uT[a];
+        for (int a = 0; a < 3; ++a) gr[a] += muN * f1y * uT[a];
     if (H)
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = 0; b < 3; ++b)
-                H[3 * a + b] += f * ((a == b ? 1.0 : 0.0) - nn[a] * nn[b] - (stick ? 0.0 : uT[a] * uT[b] / (y * y)));
-    return stick ? muN * y * y / (2.0 * eps) : muN * (y - 0.5 * eps);
+                H[3 * a + b] += muN * (f1y * ((a == b ? 1.0 : 0.0) - nn[a] * nn[b]) + c2 * uT[a] * uT[b]);
+    return muN * f0;
+}
+
+// symmetric 3x3 with negative eigenvalues clamped to 0
+__device__ void project_psd3_dev(double* H) {
+    double A[3][3], V[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) A[a][b] = 0.5 * (H[3 * a + b] + H[3 * b + a]);
+    jacobi3<double>(A, V);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double h = 0.0;
+            for (int k = 0; k < 3; ++k)
+                if (A[k][k] > 0.0) h += A[k][k] * V[a][k] * V[b][k];
+            H[3 * a + b] = h;
+        }
 }
 
 // all contact terms of candidate i at position x (xprev: its qprev)
@@ -832,6 +871,7 @@ __global__ __launch_bounds__(kBlock) void k_contact_terms(ContactArgs c, const d
     double gr[3] = {0.0, 0.0, 0.0}, H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int node = c.surf[i];
     contact_node(c, i, q + 3 * (size_t)node, qprev + 3 * (size_t)node, dt, gr, vals ? H : nullptr);
+    if (vals) project_psd3_dev(H);
     bool m[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {

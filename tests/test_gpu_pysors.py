@@ -65,3 +65,49 @@ def test_inverted_element_message(orc):
     q[3] = [0, 0, -0.01]  # node 3 pushed through the opposite face
     with pytest.raises(RuntimeError, match="Inverted Elements"):
         solver.step(q.reshape(-1), energy, 1e-3)
+
+
+def test_pokeflex_style_disk_contact_driver(orc):
+    """The structure of python/d03_pokeflex.py: neumannBC + diskContact, the
+    SQP solver setting, the disk moved through get_disk_contact_constraint()
+    before every step.  The device runs its penalty formulation; compared
+    with the oracle's restatement of the same formulation."""
+    ps = load()
+    p = mesh.contact_poke(n=5, steps=20, gap=-5e-4, speed=0.05)
+    sc = p.scene
+    clamp = sc.X[:, 2] < 1e-9                       # fixed bottom (d03_pokeflex.py:113-121)
+    mask = np.repeat(clamp, 3).astype(float)
+    settings = ps.SimulationSettings()
+    settings.meshType = "Tetrahedron"
+    settings.timeSteppingScheme = "backward_euler"
+    settings.solverMethod = "minimize_SQP"
+    settings.dt = sc.dt
+    energies = [set(["neohookean"]) for _ in range(sc.n_elems)]
+    params = [ps.Params(["density", "youngsModulus", "poissonsRatio"], [1100.0, 2e5, 0.4])
+              for _ in range(sc.n_elems)]
+    normal = np.array([0.0, 0.0, -1.0])
+    cparams = ps.Params(["neumannBCmask", "neumannBCvalue", "diskCenterVectorList", "diskNormalVectorList",
+                         "diskRadiusList", "surfaceVerticesIdx", "contactStiffness", "frictionCoefficient"],
+                        [mask, np.zeros(3 * sc.n_nodes), p.centers[0], normal, [p.radius], p.surface.astype(float),
+                         [p.stiffness], [p.friction]])
+    energy = ps.Energy(sc.X, sc.conn, settings, sc.grav, 0.0, energies, params, ["neumannBC", "diskContact"],
+                       cparams, [], ps.Params())
+    disk = ps.get_disk_contact_constraint(energy, 1)
+    with pytest.raises(ValueError):
+        ps.get_disk_contact_constraint(energy, 0)
+    solver = ps.Solver(settings)
+    o = orc.System(sc.X, sc.conn, sc.youngs, sc.poisson, sc.density, np.repeat(clamp, 3).astype(np.int32),
+                   np.zeros(3 * sc.n_nodes), sc.grav, method=orc.NEWTON_LS)
+    o.set_contact(disks=[(p.centers[0], normal, p.radius)], surface=p.surface, stiffness=p.stiffness,
+                  friction=p.friction, eps_v=p.eps_v)
+    q = qo = sc.X.reshape(-1).copy()
+    for k in range(sc.steps):
+        disk.set_disk(p.centers[k + 1], normal, p.radius, diskIdx=0)
+        o.set_disk(0, p.centers[k + 1], normal, p.radius)
+        q = solver.step(q, energy)
+        qo, _ = o.step(qo, sc.dt)
+    err = np.abs(q - qo).max() / sc.bbox_diag
+    print("pokeflex-style driver vs oracle", err, flush=True)
+    assert err <= 1e-6
+    top = q.reshape(-1, 3)[:, 2].max()
+    assert top < sc.X[:, 2].max()  # indented

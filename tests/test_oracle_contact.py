@@ -9,7 +9,9 @@ model is this build's formulation.  What pins it:
     Plane3D / Disk3D (plane3D.h:30-33, disk3D.h:33-45);
   - central finite differences of energy -> gradient and gradient -> Hessian
     for each branch (plane, disk interior, disk rim, friction stick/slip);
-  - Coulomb's bound: the friction force never exceeds mu * N.
+    the node block equals the PSD projection of the finite-difference block;
+  - the friction smoothing's closed form (f0 of Li et al. 2020): full
+    Coulomb force mu * N when sliding, never more than mu * N.
 """
 import numpy as np
 import pytest
@@ -122,22 +124,29 @@ def test_penalty_fd(orc, case):
     qq[:3] = x0
     H = s.hessian_dense(qq, 0.0)[:3, :3] - s0.hessian_dense(qq, 0.0)[:3, :3]
     assert np.all(np.linalg.eigvalsh(0.5 * (H + H.T)) >= -1e-9 * k)
-    if case != "disk_rim":  # the rim's Hessian leaves out the falloff derivatives
-        Hfd = np.stack([fd_grad(lambda x: grad(x)[i], x0, 1e-8) for i in range(3)])
-        assert np.allclose(H, Hfd, rtol=1e-6, atol=1e-6 * k)
+    Hfd = np.stack([fd_grad(lambda x: grad(x)[i], x0, 1e-8) for i in range(3)])
+    Hfd = 0.5 * (Hfd + Hfd.T)
+    lam, V = np.linalg.eigh(Hfd)
+    Hpsd = (V * np.maximum(lam, 0.0)) @ V.T
+    if case == "disk_rim":
+        say_rim = np.linalg.eigvalsh(Hfd)
+        assert say_rim.min() < 0  # this point exercises the projection
+    assert np.allclose(H, Hpsd, rtol=1e-6, atol=1e-6 * np.abs(Hfd).max())
 
 
-@pytest.mark.parametrize("slide", [2e-9, 5e-5])  # stick (|uT| < eps) and slip
+@pytest.mark.parametrize("slide", [2e-9, 5e-7, 5e-5])  # stick (|uT| < eps = 1e-6), transition, slip
 def test_friction_fd_and_coulomb_bound(orc, slide):
-    """Node 0 rests 1 mm inside the plane z = 0.5 + 0.5 (normal +z); the lag
-    gives N = k * 1e-3.  Move it tangentially by `slide` over one step."""
+    """Node 0 rests 1 um inside the plane z = 0.5 + 1e-6 (normal +z); the lag
+    gives N = k * 1e-6.  Move it tangentially by `slide` over one step."""
     k, mu, eps_v, dt = 1e5, 0.5, 1e-3, 1e-3
+    depth = 1e-6
     X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) + np.array([0.3, 0.2, 0.5])
     s = orc.System(X, TET_CONN, [0.0], [0.0], [0.0], method=orc.NEWTON)
     s0 = orc.System(X, TET_CONN, [0.0], [0.0], [0.0], method=orc.NEWTON)
-    s.set_contact(planes=[((0, 0, 0.501), (0, 0, 1))], surface=[0], stiffness=k, friction=mu, eps_v=eps_v)
-    s.contact_lag(dt)  # from qprev = X: node 0 at depth 1e-3
-    N = k * 1e-3
+    s.set_contact(planes=[((0, 0, 0.5 + depth), (0, 0, 1))], surface=[0], stiffness=k, friction=mu, eps_v=eps_v)
+    s.contact_lag(dt)  # from qprev = X
+    N = k * depth
+    eps = eps_v * dt
     q = X.reshape(-1).copy()
     x0 = X[0] + np.array([slide * 0.6, slide * 0.8, 0.0])
 
@@ -152,13 +161,15 @@ def test_friction_fd_and_coulomb_bound(orc, slide):
         return (s.gradient(qq, dt) - s0.gradient(qq, dt))[:3]
 
     g = grad(x0)
-    penalty = np.array([0, 0, -k * 1e-3])
-    ft = g - penalty
-    assert np.linalg.norm(ft) <= mu * N * (1 + 1e-12)
-    if slide > eps_v * dt:
-        assert np.linalg.norm(ft) == pytest.approx(mu * N, rel=1e-12)  # sliding: full Coulomb force
-        assert np.allclose(ft / np.linalg.norm(ft), [0.6, 0.8, 0.0])
-    h = 1e-7  # stays inside one branch: stick |uT| + h < eps = 1e-6, slip |uT| - h > eps
+    assert g[2] == pytest.approx(-k * depth, rel=1e-8)  # the penalty; friction is tangential (n = +z)
+    ft = np.array([g[0], g[1], 0.0])
+    assert np.linalg.norm(ft) <= mu * N * (1 + 1e-9)  # N from a depth recovered to 1e-10 relative
+    y = slide
+    f1 = 1.0 if y >= eps else -y * y / eps**2 + 2 * y / eps  # f0' (Li et al. 2020)
+    # y is recovered from coordinates of size 0.3: rounding 1e-16 / 2e-9
+    assert np.linalg.norm(ft) == pytest.approx(mu * N * f1, rel=1e-6)
+    assert np.allclose(ft / np.linalg.norm(ft), [0.6, 0.8, 0.0], atol=1e-6)
+    h = min(1e-10, 1e-2 * slide)  # inside one branch, small against eps
     assert np.allclose(g, fd_grad(energy, x0, h), rtol=1e-6, atol=1e-6 * np.abs(g).max())
     qq = q.copy()
     qq[:3] = x0
