@@ -207,6 +207,48 @@ int sors_set_contact(sors_ctx* ctx, const sors_contact_desc* desc);
  * friction of the next step. */
 int sors_set_disk(sors_ctx* ctx, int idx, const double* center, const double* normal, double radius);
 
+/* ---- Actuation and the adjoint -------------------------------------------
+ * Active fibre stress (tet4): Psi_act = 1/2 sigma_max a (|F f|^2 - 1) per
+ * unit rest volume, with the element's unit fibre direction f and the
+ * activation a of its muscle group (first Piola stress sigma_max a (F f) f^T;
+ * at rest the active stress sigma_max a f (x) f).  It plays the role of the
+ * reference's actuated element energies, whose activations arrive per
+ * muscle group through the "actuation" Params of Solver::step (SoftCon
+ * muscle, cpp/src/energies/softconMuscle.cpp:12-134; element.h:52).
+ * elem_group[e] = -1 leaves element e passive. */
+typedef struct {
+    int n_groups;
+    const int* elem_group;     /* [n_elems] group in [0, n_groups) or -1 */
+    const double* fibre;       /* [n_elems][3] (normalised) */
+    double sigma_max;          /* Pa */
+} sors_active_desc;
+
+int sors_set_active(sors_ctx* ctx, const sors_active_desc* desc);   /* NULL removes it */
+/* activations [n_groups] used by the following steps (Solver::step's actuation) */
+int sors_set_activation(sors_ctx* ctx, const double* act);
+/* per-element Young's modulus (Lame constants recomputed with each element's
+ * Poisson ratio, neoHookean.cpp:10-11) */
+int sors_set_youngs(sors_ctx* ctx, const double* youngs);
+/* TimeIntegrator state (integrator.h:101-102): qPrev = q (and the positions),
+ * vPrev = v; either may be NULL to keep it */
+int sors_set_state(sors_ctx* ctx, const double* q, const double* v);
+
+/* Adjoint of one backward Euler step (no reference counterpart: the
+ * reference has no gradients).  Given the state before the step (q_prev,
+ * v_prev), the converged positions after it (q_next, v_next = (q_next -
+ * q_prev)/dt), the current activations and Young's moduli, and the
+ * adjoints qbar_next = dL/dq_next, vbar_next = dL/dv_next, it solves
+ *   H lambda = qbar_next + vbar_next/dt     (H: the unprojected FP64 Hessian
+ *                                            of the step at q_next, PCG)
+ * and returns qbar_prev = dL/dq_prev, vbar_prev = dL/dv_prev (overwritten)
+ * and ADDS dL/dE per element into youngs_bar and dL/da per group into
+ * act_bar (either may be NULL).  Backward Euler, minimize_newton(_linesearch),
+ * tet4; frictionless plane contact is included, friction and disks are not
+ * (SORS_ERR_UNSUPPORTED).  Overwrites the context's state. */
+int sors_adjoint_step(sors_ctx* ctx, const double* q_prev, const double* v_prev, const double* q_next, double dt,
+                      const double* qbar_next, const double* vbar_next, double* qbar_prev, double* vbar_prev,
+                      double* youngs_bar, double* act_bar, double rtol, int max_iter, long long* pcg_iters);
+
 /* ---- Serving loop --------------------------------------------------------
  * No counterpart in the reference: its analogue is a sequential Python loop
  * over full rollouts (python/d05a_hopping_grid.py:103-106).  A server keeps

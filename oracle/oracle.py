@@ -112,6 +112,11 @@ def lib():
         L.orc_contact_energy.restype = C.c_double
         L.orc_contact_lag.argtypes = [C.c_void_p, C.c_double]
         L.orc_contact_lag.restype = None
+        L.orc_set_active.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_double]
+        L.orc_set_activation.argtypes = [C.c_void_p, _dp]
+        L.orc_set_activation.restype = None
+        L.orc_set_youngs.argtypes = [C.c_void_p, _dp]
+        L.orc_set_youngs.restype = None
         _lib = L
     return _lib
 
@@ -348,6 +353,22 @@ class System:
 
     def contact_lag(self, dt):
         lib().orc_contact_lag(self.h, float(dt))
+
+    def set_active(self, group, fibre, sigma):
+        self._active_keep = (i32(group), f64(fibre).reshape(-1))
+        rc = lib().orc_set_active(self.h, int(max(group) + 1), _i(self._active_keep[0]), _d(self._active_keep[1]),
+                                  float(sigma))
+        if rc != 0:
+            raise ValueError(f"orc_set_active: {rc}")
+        self.ngroups = int(max(group) + 1)
+
+    def set_activation(self, act):
+        a = f64(act)
+        assert a.size == self.ngroups
+        lib().orc_set_activation(self.h, _d(a))
+
+    def set_youngs(self, E):
+        lib().orc_set_youngs(self.h, _d(f64(E)))
 
     def update_state(self, q, dt):
         q = f64(q).reshape(-1)

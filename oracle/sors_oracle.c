@@ -285,8 +285,12 @@ static void poly_eval(const double* xe, double* energy, double* grad, double* he
  * with the NeoHookeanEnergy quadrature (weights 1/nquad, neoHookean.cpp:26-27,
  * 56-57, 85-86).  Literal dense form: grad = V0 sum_s w dEdF_s B_s,
  * hess = V0 sum_s w B_s^T K_s B_s. */
+/* sa, f: active fibre stress sigma_max * a and unit fibre f (f == NULL or
+ * sa == 0: passive).  Psi_act = 1/2 sa (|F f|^2 - 1), P_act = sa (F f) f^T,
+ * dP_act/dF = sa I (x) f f^T (PSD, added after any projection). */
 static int elem_eval_pre(int ed, const double* pre, double V0, const double* xe, double mu, double lam,
-                         int project, double* energy, double* grad, double* hess) {
+                         int project, double* energy, double* grad, double* hess, double sa, const double* f) {
+    const int active = f != NULL && sa != 0.0;
     int nq = (ed == 4) ? 1 : 8;
     int nd = 3 * ed;
     double w = 1.0 / nq;
@@ -325,10 +329,19 @@ static int elem_eval_pre(int ed, const double* pre, double V0, const double* xe,
                 for (int j = 0; j < 3; ++j)
                     for (int c = 0; c < 3; ++c) Bbuf[(3 * j + c) * 24 + 3 * a + j] = Gs[3 * a + c];
         }
-        if (energy) e += orc_nh_psi(F, mu, lam) * w;
+        double Ff[3] = {0, 0, 0};
+        if (active)
+            for (int r = 0; r < 3; ++r) Ff[r] = F[3 * r] * f[0] + F[3 * r + 1] * f[1] + F[3 * r + 2] * f[2];
+        if (energy) {
+            e += orc_nh_psi(F, mu, lam) * w;
+            if (active) e += 0.5 * sa * (Ff[0] * Ff[0] + Ff[1] * Ff[1] + Ff[2] * Ff[2] - 1.0) * w;
+        }
         if (grad) {
             double P[9];
             orc_nh_dpsi(F, mu, lam, P);
+            if (active)
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) P[3 * r + c] += sa * Ff[r] * f[c];
             for (int i = 0; i < nd; ++i) {
                 double acc = 0;
                 for (int r = 0; r < 9; ++r) acc += P[r] * Bbuf[r * nd + i];
@@ -340,6 +353,10 @@ static int elem_eval_pre(int ed, const double* pre, double V0, const double* xe,
             int st = orc_nh_d2psi(F, mu, lam, K);
             if (st != ORC_OK) status = st;
             if (project) orc_project_psd(9, K);
+            if (active)
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        for (int l = 0; l < 3; ++l) K[9 * (3 * i + j) + 3 * i + l] += sa * f[j] * f[l];
             for (int r = 0; r < 9; ++r)
                 for (int i = 0; i < nd; ++i) {
                     double acc = 0;
@@ -374,13 +391,13 @@ int orc_elem_eval(int ed, const double* Xe, const double* xe, double E, double n
         double Dminv[9];
         int st = orc_tet_precompute(Xe, Dminv, &V0, NULL);
         if (st) return st;
-        return elem_eval_pre(4, Dminv, V0, xe, mu, lam, project, energy, grad, hess);
+        return elem_eval_pre(4, Dminv, V0, xe, mu, lam, project, energy, grad, hess, 0.0, NULL);
     }
     if (ed == 8) {
         double G[192];
         int st = orc_hex_precompute(Xe, G, NULL, &V0);
         if (st) return st;
-        return elem_eval_pre(8, G, V0, xe, mu, lam, project, energy, grad, hess);
+        return elem_eval_pre(8, G, V0, xe, mu, lam, project, energy, grad, hess, 0.0, NULL);
     }
     return ORC_ERR_ARG;
 }
@@ -416,6 +433,13 @@ struct orc_sys {
     double disk_c[ORC_MAX_PRIM][3], disk_n[ORC_MAX_PRIM][3], disk_r[ORC_MAX_PRIM], disk_c_last[ORC_MAX_PRIM][3];
     int* surf;
     double kc, muf, eps_v, lag_dt;
+    /* active fibre stress (orc_set_active) */
+    int ngroups;
+    int* agroup;
+    double* afib;   /* [nel][3] */
+    double* act;    /* [ngroups] */
+    double sigma;
+    double* nu;     /* Poisson ratio per element (orc_set_youngs) */
     double* lagN;   /* [nsurf] lagged normal force magnitude */
     double* lagn;   /* [nsurf][3] its normal */
     double* lagD;   /* [nsurf][3] displacement of its primitive over the step */
@@ -459,6 +483,7 @@ orc_sys* orc_create(int N, int nel, int ed, const double* X, const int* conn, co
     s->V0 = (double*)malloc(sizeof(double) * nel);
     s->mu = (double*)malloc(sizeof(double) * nel);
     s->lam = (double*)malloc(sizeof(double) * nel);
+    s->nu = (double*)malloc(sizeof(double) * (nel > 0 ? nel : 1));
     s->mass = (double*)calloc(s->n, sizeof(double));
     s->mask = (int*)calloc(s->n, sizeof(int));
     s->bcval = (double*)calloc(s->n, sizeof(double));
@@ -484,6 +509,7 @@ orc_sys* orc_create(int N, int nel, int ed, const double* X, const int* conn, co
         else
             orc_hex_precompute(Xe, s->pre + (size_t)e * 192, NULL, &s->V0[e]);
         orc_lame(youngs ? youngs[e] : 0.0, poisson ? poisson[e] : 0.0, &s->mu[e], &s->lam[e]);
+        s->nu[e] = poisson ? poisson[e] : 0.0;
         double me = (density ? density[e] : 0.0) * s->V0[e]; /* element.h:31 */
         for (int a = 0; a < ed; ++a)
             for (int k = 0; k < 3; ++k) s->mass[3 * (size_t)conn[(size_t)e * ed + a] + k] += me / ed;
@@ -508,6 +534,10 @@ void orc_free(orc_sys* s) {
     free(s->rowptr);
     free(s->cols);
     free(s->fext);
+    free(s->nu);
+    free(s->agroup);
+    free(s->afib);
+    free(s->act);
     free(s->surf);
     free(s->lagN);
     free(s->lagn);
@@ -551,8 +581,14 @@ static int elem(const orc_sys* s, int e, const double* xe, int project, double* 
         return ORC_OK;
     }
     int psz = (s->ed == 4) ? 9 : 192;
+    double sa = 0.0;
+    const double* f = NULL;
+    if (s->agroup && s->agroup[e] >= 0) {
+        sa = s->sigma * s->act[s->agroup[e]];
+        f = s->afib + 3 * (size_t)e;
+    }
     return elem_eval_pre(s->ed, s->pre + (size_t)e * psz, s->V0[e], xe, s->mu[e], s->lam[e], project, energy, grad,
-                         hess);
+                         hess, sa, f);
 }
 
 /* TimeIntegrator::integrate_energy (integrator.cpp:3-43) */
@@ -1403,4 +1439,35 @@ double orc_bench_pcg(orc_sys* s, const double* q, double dt, int iters, int proj
     free(b);
     free(x);
     return k > 0 ? t / (double)k : -1.0;
+}
+
+/* ---- active fibre stress (a formulation of this build; the reference's
+ * actuated energies receive per-group activations the same way,
+ * cpp/src/energies/softconMuscle.cpp:12-134, element.h:52) ---------------- */
+int orc_set_active(orc_sys* s, int ngroups, const int* group, const double* fibre, double sigma) {
+    if (s->ed != 4 || s->poly || ngroups < 1) return ORC_ERR_ARG;
+    free(s->agroup);
+    free(s->afib);
+    free(s->act);
+    s->agroup = (int*)malloc(sizeof(int) * (size_t)(s->nel > 0 ? s->nel : 1));
+    s->afib = (double*)malloc(sizeof(double) * 3 * (size_t)(s->nel > 0 ? s->nel : 1));
+    s->act = (double*)calloc((size_t)ngroups, sizeof(double));
+    for (int e = 0; e < s->nel; ++e) {
+        s->agroup[e] = group[e];
+        double f[3] = {fibre[3 * (size_t)e], fibre[3 * (size_t)e + 1], fibre[3 * (size_t)e + 2]};
+        double l = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        for (int k = 0; k < 3; ++k) s->afib[3 * (size_t)e + k] = l > 0 ? f[k] / l : 0.0;
+    }
+    s->ngroups = ngroups;
+    s->sigma = sigma;
+    return ORC_OK;
+}
+
+void orc_set_activation(orc_sys* s, const double* act) {
+    if (s->act) memcpy(s->act, act, sizeof(double) * (size_t)s->ngroups);
+}
+
+/* neoHookean.cpp:10-11 with the stored Poisson ratios */
+void orc_set_youngs(orc_sys* s, const double* youngs) {
+    for (int e = 0; e < s->nel; ++e) orc_lame(youngs[e], s->nu[e], &s->mu[e], &s->lam[e]);
 }

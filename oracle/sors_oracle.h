@@ -138,6 +138,13 @@ double orc_contact_energy(orc_sys* s, const double* q, double dt);
 /* recompute the friction lag from qprev (done by orc_step before each step) */
 void orc_contact_lag(orc_sys* s, double dt);
 
+/* ---- active fibre stress (tet4): Psi_act = 1/2 sigma a (|F f|^2 - 1) -------
+ * A formulation of this build (the reference's actuated element energies
+ * take per-group activations the same way: softconMuscle.cpp:12-134). */
+int  orc_set_active(orc_sys* s, int ngroups, const int* group, const double* fibre, double sigma);
+void orc_set_activation(orc_sys* s, const double* act);
+void orc_set_youngs(orc_sys* s, const double* youngs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,11 @@ struct ElemArgs {
     double* ex_energy;    // export (test mode) or null
     double* ex_grad;
     double* ex_hess;
+    // active fibre stress (tet4): Psi_act = 1/2 sigma_max a (|F f|^2 - 1)
+    const int* agroup;    // [E] muscle group or -1; null: no active stress
+    const double* afib;   // [3][E] unit fibre direction in the rest state (SoA)
+    const double* act;    // [n_groups] activation
+    double sigma;         // sigma_max (Pa)
 };
 
 struct NodeArgs {

@@ -209,6 +209,17 @@ __device__ __forceinline__ void scatter_block(const ElemArgs& a, int e, int la, 
 
 __device__ __forceinline__ bool dof_kept(const ElemArgs& a, int dof) { return a.sqp || !a.mask || a.mask[dof] == 0; }
 
+// active fibre stress of element e: returns sigma_max * a (0 if passive),
+// the fibre direction in f
+__device__ __forceinline__ double active_level(const ElemArgs& a, int e, double f[3]) {
+    if (!a.agroup) return 0.0;
+    const int gi = a.agroup[e];
+    if (gi < 0) return 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] = a.afib[(size_t)c * a.E + e];
+    return a.sigma * a.act[gi];
+}
+
 // ---- tet4: one element per lane ------------------------------------------
 // MODE 0: assemble gradient + Hessian; MODE 1: gradient only; MODE 2: export
 // per-element energy / gradient / Hessian (FP64 gradient & energy; Hessian in HT)
@@ -288,9 +299,23 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
     const double mu = a.mu[e], lam = a.lam[e];
     const NHPoint<double> p = nh_point(D);
     if (!(p.J > 0)) atomicOr(a.flag, 1);
+    // active fibre stress: P += sigma a (F f) f^T, H_ab += V0 sigma a (f.g_a)(f.g_b) I
+    double fa[3] = {0.0, 0.0, 0.0}, Ff[3] = {0.0, 0.0, 0.0}, fg[4] = {0.0, 0.0, 0.0, 0.0};
+    const double sa = active_level(a, e, fa);
+    if (sa != 0.0) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) Ff[r] = fa[r] + D(r, 0) * fa[0] + D(r, 1) * fa[1] + D(r, 2) * fa[2];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fg[k] = fa[0] * g[k][0] + fa[1] * g[k][1] + fa[2] * g[k][2];
+    }
     // gradient (FP64): grad_a = V0 P g_a
     {
-        const M3<double> P = nh_stress(p, mu, lam);
+        M3<double> P = nh_stress(p, mu, lam);
+        if (sa != 0.0)
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) P(r, c) += sa * Ff[r] * fa[c];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -304,7 +329,8 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
             }
         if (MODE == 2 && a.ex_energy) {
             bool inv = false;
-            a.ex_energy[e] = V0 * nh_psi_disp(D, mu, lam, &inv);
+            a.ex_energy[e] = V0 * (nh_psi_disp(D, mu, lam, &inv) +
+                                   0.5 * sa * (Ff[0] * Ff[0] + Ff[1] * Ff[1] + Ff[2] * Ff[2] - 1.0));
         }
     }
     if (MODE == 1) return;
@@ -331,6 +357,12 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
 #pragma unroll
                 for (int i = 0; i < 9; ++i) blk[i] = 0;
                 block_from_tangent(S, gh[la], gh[lb], w, blk);
+                if (sa != 0.0) {
+                    const HT s_ab = w * (HT)(sa * fg[la] * fg[lb]);
+                    blk[0] += s_ab;
+                    blk[4] += s_ab;
+                    blk[8] += s_ab;
+                }
                 if (MODE == 2) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
@@ -358,6 +390,12 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
 #pragma unroll
                 for (int i = 0; i < 9; ++i) blk[i] = 0;
                 block_raw(muh, lamh, c, gh[la], gh[lb], h[la], h[lb], w, blk);
+                if (sa != 0.0) {
+                    const HT s_ab = w * (HT)(sa * fg[la] * fg[lb]);
+                    blk[0] += s_ab;
+                    blk[4] += s_ab;
+                    blk[8] += s_ab;
+                }
                 if (MODE == 2) {
 #pragma unroll
                     for (int cc = 0; cc < 3; ++cc)
@@ -582,7 +620,19 @@ __device__ double tet_energy_at(const ElemArgs& a, int e, const int nd[4], doubl
             for (int k = 0; k < 4; ++k) acc += (x[k][r] - X[k][r]) * g[k][c];
             D(r, c) = acc;
         }
-    return fabs(detm) / 6.0 * nh_psi_disp(D, a.mu[e], a.lam[e], inv);
+    double en = nh_psi_disp(D, a.mu[e], a.lam[e], inv);
+    double fa[3];
+    const double sa = active_level(a, e, fa);
+    if (sa != 0.0) {
+        double ff = 0.0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const double v = fa[r] + D(r, 0) * fa[0] + D(r, 1) * fa[1] + D(r, 2) * fa[2];
+            ff += v * v;
+        }
+        en += 0.5 * sa * (ff - 1.0);
+    }
+    return fabs(detm) / 6.0 * en;
 }
 
 __device__ double hex_energy_at(const ElemArgs& a, int e, const int nd[8], double alpha, bool* inv) {
@@ -1056,6 +1106,117 @@ __global__ void k_block_to_csr(int n, const int* rowptr, const int* cols, const 
 //   A_k reads  partU[k%2] (convergence), writes partA = delta_k
 // A_0 / U_1 are the first iteration (alpha_0 = gamma_0/delta_0, beta_0 = 0).
 // ===========================================================================
+
+// ===========================================================================
+// Adjoint of one backward Euler step (no reference counterpart).  With the
+// step residual r(q1; q0, v0, theta) = 0 and lambda = H^-1 (qbar1 + vbar1/dt)
+// (H = dr/dq1, symmetric):
+//   qbar0 = -(dr/dq0)^T lambda - vbar1/dt,  vbar0 = -(dr/dv0)^T lambda,
+//   thetabar = -(dr/dtheta)^T lambda.
+// dr/dq0 = -(m/dt^2 + alpha m/dt), dr/dv0 = -m/dt on free DOFs; a Dirichlet
+// row r_i = q1_i - q0_i - dt bc_i gives dr/dq0 = -1, dr/dv0 = 0.
+// ===========================================================================
+__global__ __launch_bounds__(kBlock) void k_adj_state(int n, double dt, double damping, const double* mass,
+                                                      const int* mask, const double* lam, const double* vbar,
+                                                      double* qbar0, double* vbar0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = mass[i], l = lam[i];
+    const bool dir = mask && mask[i];
+    qbar0[i] = (dir ? l : (m / (dt * dt) + damping * m / dt) * l) - vbar[i] / dt;
+    vbar0[i] = dir ? 0.0 : m / dt * l;
+}
+
+// per-element parameter adjoints of the tet4 residual at q1:
+//   Ebar_e = -lambda_e . dg_e/dE (Neo-Hookean gradient with the Lame
+//   constants' derivatives dmu/dE = 1/(2(1+nu)), dlam/dE = nu/((1+nu)(1-2nu)));
+//   abar_g += -lambda_e . dg_e/da (active term with a = 1: V0 sigma (F f)(f.g_k)).
+// Dirichlet DOFs take no element terms (their rows are the Dirichlet rows).
+constexpr int kAdjGroupsLds = 64;
+__global__ __launch_bounds__(kBlock) void k_adj_params_tet(ElemArgs a, const double* nu, const double* lam,
+                                                           double* Ebar, double* abar, int ngroups) {
+    __shared__ double gacc[kAdjGroupsLds];
+    const bool lds = ngroups <= kAdjGroupsLds;
+    if (lds)
+        for (int i = threadIdx.x; i < ngroups; i += blockDim.x) gacc[i] = 0.0;
+    __syncthreads();
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < a.E) {
+        int nd[4];
+        double X[4][3], x[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            nd[k] = a.conn[(size_t)k * a.E + e];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                X[k][c] = a.rest[3 * nd[k] + c];
+                x[k][c] = a.q[3 * nd[k] + c];
+            }
+        }
+        M3<double> Dm;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Dm(r, c) = X[c][r] - X[3][r];
+        const M3<double> Cm = cofactor3(Dm);
+        const double detm = Dm.m[0] * Cm.m[0] + Dm.m[1] * Cm.m[1] + Dm.m[2] * Cm.m[2];
+        const double V0 = fabs(detm) / 6.0;
+        double g[4][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g[k][c] = Cm(c, k) / detm;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[3][c] = -(g[0][c] + g[1][c] + g[2][c]);
+        M3<double> D;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc += (x[k][r] - X[k][r]) * g[k][c];
+                D(r, c) = acc;
+            }
+        const double v = nu[e];
+        const double mu1 = 1.0 / (2.0 * (1.0 + v)), lam1 = v / ((1.0 + v) * (1.0 - 2.0 * v));
+        const NHPoint<double> p = nh_point(D);
+        const M3<double> P1 = nh_stress(p, mu1, lam1);
+        double fa[3] = {0.0, 0.0, 0.0}, Ff[3] = {0.0, 0.0, 0.0};
+        const int gi = a.agroup ? a.agroup[e] : -1;
+        if (gi >= 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) fa[c] = a.afib[(size_t)c * a.E + e];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) Ff[r] = fa[r] + D(r, 0) * fa[0] + D(r, 1) * fa[1] + D(r, 2) * fa[2];
+        }
+        double sE = 0.0, sA = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double fg = fa[0] * g[k][0] + fa[1] * g[k][1] + fa[2] * g[k][2];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const int dof = 3 * nd[k] + r;
+                if (!dof_kept(a, dof)) continue;
+                const double l = lam[dof];
+                sE += l * V0 * (P1(r, 0) * g[k][0] + P1(r, 1) * g[k][1] + P1(r, 2) * g[k][2]);
+                if (gi >= 0) sA += l * V0 * a.sigma * Ff[r] * fg;
+            }
+        }
+        Ebar[e] = -sE;
+        if (gi >= 0) {
+            if (lds)
+                atomicAdd(&gacc[gi], -sA);
+            else
+                atomicAdd(&abar[gi], -sA);
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (int i = threadIdx.x; i < ngroups; i += blockDim.x)
+            if (gacc[i] != 0.0) atomicAdd(&abar[i], gacc[i]);
+}
+
 struct PcgPart {
     double* u[2];   // 2 slots each: gamma, rr
     double* a;      // 1 slot: delta
@@ -1557,6 +1718,12 @@ struct sors_ctx {
     bool pat_on_device = false;
     DevMem pcgst_bench, copybuf;
     DevMem sbase, bcol, selfslot;   // sliced node-block layout (node-granular masks)
+    // active fibre stress (sors_set_active) and parameter adjoints
+    bool has_active = false;
+    int ngroups = 0;
+    double sigma = 0.0;
+    std::vector<double> h_nu;       // Poisson ratio per element (Lame updates, dE adjoint)
+    DevMem agroup, afib, act, nu, adjb, adjq, adjv, adjE, adjA;
     // contact (sors_set_contact): primitives by value, candidates and lag on the device
     bool has_contact = false;
     ContactArgs cargs{};
@@ -1661,6 +1828,12 @@ ElemArgs elem_args(sors_ctx* c, const double* q) {
     a.fullmap = c->pat.node_granular ? nullptr : c->fullmap.as<int>();
     a.sbase = c->block() ? c->sbase.as<int>() : nullptr;
     a.flag = c->F();
+    if (c->has_active) {
+        a.agroup = c->agroup.as<int>();
+        a.afib = c->afib.as<double>();
+        a.act = c->act.as<double>();
+        a.sigma = c->sigma;
+    }
     return a;
 }
 
@@ -2191,7 +2364,8 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->fullmap, &c->vals, &c->Minv,    &c->pr,    &c->pu,      &c->pp,      &c->ps,
                            &c->pw,     &c->px,    &c->partials, &c->scal, &c->pcgst,   &c->pcgctl,  &c->pcgpart,
                            &c->flag,   &c->sbase, &c->bcol,    &c->selfslot, &c->csurf, &c->lagN,
-                           &c->lagn,   &c->lagD,  &c->gcon};
+                           &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
+                           &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA};
     double b = 0;
     for (const DevMem* d : all) b += (double)d->bytes;
     return b;
@@ -2741,6 +2915,8 @@ int sors_create(const sors_system_desc* d, sors_ctx** out) {
         // lumped mass rho V0 / ed per DOF (energy.cpp:176-189)
         std::vector<int> connT((size_t)E * ed);
         std::vector<double> mu(E), lam(E), mass(n, 0.0);
+        c->h_nu.assign(E, 0.0);
+        if (d->poisson) std::copy(d->poisson, d->poisson + E, c->h_nu.begin());
         for (int e = 0; e < E; ++e) {
             for (int k = 0; k < ed; ++k) connT[(size_t)k * E + e] = c->h_conn[(size_t)e * ed + k];
             const double Ey = d->youngs ? d->youngs[e] : 0.0, nu = d->poisson ? d->poisson[e] : 0.0;
@@ -3262,6 +3438,138 @@ int sors_set_disk(sors_ctx* c, int idx, const double* center, const double* norm
         c->cargs.dn[idx][k] = nn[k];
     }
     c->cargs.dr[idx] = radius;
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+// ---- actuation and adjoint --------------------------------------------------
+int sors_set_active(sors_ctx* c, const sors_active_desc* d) {
+    SORS_GUARD(c)
+    if (!d) {
+        c->has_active = false;
+        return SORS_OK;
+    }
+    if (c->ed != 4 || c->material != SORS_NEOHOOKEAN)
+        return fail(c, SORS_ERR_UNSUPPORTED, "active fibre stress is implemented for Neo-Hookean tetrahedra");
+    if (d->n_groups < 1 || !d->elem_group || !d->fibre || !(d->sigma_max >= 0.0))
+        return fail(c, SORS_ERR_ARG, "active stress needs n_groups >= 1, groups, fibres and sigma_max >= 0");
+    const int E = c->E;
+    std::vector<double> fib((size_t)3 * E);
+    for (int e = 0; e < E; ++e) {
+        const int gi = d->elem_group[e];
+        if (gi < -1 || gi >= d->n_groups) return fail(c, SORS_ERR_ARG, "element muscle group out of range");
+        double f[3] = {d->fibre[3 * (size_t)e], d->fibre[3 * (size_t)e + 1], d->fibre[3 * (size_t)e + 2]};
+        const double l = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        if (gi >= 0 && !(l > 0.0)) return fail(c, SORS_ERR_ARG, "zero fibre direction");
+        for (int k = 0; k < 3; ++k) fib[(size_t)k * E + e] = l > 0.0 ? f[k] / l : 0.0;
+    }
+    upload(c->agroup, d->elem_group, (size_t)E, c->stream);
+    upload(c->afib, fib.data(), fib.size(), c->stream);
+    ensure(c->act, sizeof(double) * d->n_groups);
+    fill<double>(c->act.as<double>(), d->n_groups, 0.0, c->stream);
+    c->ngroups = d->n_groups;
+    c->sigma = d->sigma_max;
+    c->has_active = true;
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_set_activation(sors_ctx* c, const double* act) {
+    SORS_GUARD(c)
+    if (!c->has_active) return fail(c, SORS_ERR_ARG, "no active stress set (sors_set_active)");
+    if (!act) return fail(c, SORS_ERR_ARG, "null activation");
+    upload(c->act, act, (size_t)c->ngroups, c->stream);
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_set_youngs(sors_ctx* c, const double* youngs) {
+    SORS_GUARD(c)
+    if (!youngs) return fail(c, SORS_ERR_ARG, "null Young's moduli");
+    std::vector<double> mu(c->E), lam(c->E);
+    for (int e = 0; e < c->E; ++e) {
+        const double Ey = youngs[e], nu = c->h_nu[e];
+        lam[e] = (Ey * nu) / ((1 + nu) * (1 - 2 * nu));
+        mu[e] = Ey / (2 * (1 + nu));
+    }
+    upload(c->mu, mu.data(), mu.size(), c->stream);
+    upload(c->lam, lam.data(), lam.size(), c->stream);
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_set_state(sors_ctx* c, const double* q, const double* v) {
+    SORS_GUARD(c)
+    if (q) {
+        upload(c->qprev, q, (size_t)c->n, c->stream);
+        upload(c->q, q, (size_t)c->n, c->stream);
+    }
+    if (v) upload(c->vprev, v, (size_t)c->n, c->stream);
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_adjoint_step(sors_ctx* c, const double* q_prev, const double* v_prev, const double* q_next, double dt,
+                      const double* qbar_next, const double* vbar_next, double* qbar_prev, double* vbar_prev,
+                      double* youngs_bar, double* act_bar, double rtol, int max_iter, long long* pcg_iters) {
+    SORS_GUARD(c)
+    if (!q_prev || !v_prev || !q_next || !qbar_next || !vbar_next || !qbar_prev || !vbar_prev)
+        return fail(c, SORS_ERR_ARG, "null state or adjoint array");
+    if (!(dt > 0.0) || max_iter < 1) return fail(c, SORS_ERR_ARG, "dt must be > 0 and max_iter >= 1");
+    if (c->scheme != SORS_BACKWARD_EULER || c->method == SORS_MINIMIZE_SQP || c->ed != 4 ||
+        c->material != SORS_NEOHOOKEAN)
+        return fail(c, SORS_ERR_UNSUPPORTED,
+                    "the adjoint is implemented for backward Euler, Newton methods, Neo-Hookean tetrahedra");
+    if (c->has_contact && (c->cargs.mu > 0.0 || c->cargs.ndisks > 0))
+        return fail(c, SORS_ERR_UNSUPPORTED, "the adjoint takes frictionless plane contact only");
+    const int n = c->n, E = c->E;
+    ensure_pattern(c, dt);
+    upload(c->qprev, q_prev, (size_t)n, c->stream);
+    upload(c->vprev, v_prev, (size_t)n, c->stream);
+    upload(c->q, q_next, (size_t)n, c->stream);
+    std::vector<double> b((size_t)n);
+    for (int i = 0; i < n; ++i) b[(size_t)i] = qbar_next[i] + vbar_next[i] / dt;
+    upload(c->adjb, b.data(), b.size(), c->stream);
+    ensure(c->adjq, sizeof(double) * n);
+    ensure(c->adjv, sizeof(double) * n);
+    ensure(c->adjE, sizeof(double) * std::max(E, 1));
+    ensure(c->adjA, sizeof(double) * std::max(c->ngroups, 1));
+    clear_flag(c);
+    assemble<double>(c, c->q.as<double>(), dt, true, 0);
+    build_preconditioner<double>(c);
+    if (read_flag(c) & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
+    bool brk = false;
+    const long long it = pcg<double>(c, c->adjb.as<double>(), 1.0, rtol, max_iter, &brk);
+    if (pcg_iters) *pcg_iters = it;
+    if (brk) return fail(c, SORS_ERR_NAN, "adjoint: PCG breakdown (Hessian not positive definite)");
+    const double* lam = (const double*)c->px.p;
+    upload(c->dxd, vbar_next, (size_t)n, c->stream);
+    k_adj_state<<<grid_for(n), kBlock, 0, c->stream>>>(n, dt, c->damping, c->mass.as<double>(),
+                                                       c->has_mask ? c->mask.as<int>() : nullptr, lam,
+                                                       c->dxd.as<double>(), c->adjq.as<double>(),
+                                                       c->adjv.as<double>());
+    fill<double>(c->adjA.as<double>(), std::max(c->ngroups, 1), 0.0, c->stream);
+    if (E > 0) {
+        if (c->nu.bytes < sizeof(double) * (size_t)E) upload(c->nu, c->h_nu.data(), (size_t)E, c->stream);
+        ElemArgs a = elem_args(c, c->q.as<double>());
+        k_adj_params_tet<<<grid_for(E), kBlock, 0, c->stream>>>(a, c->nu.as<double>(), lam, c->adjE.as<double>(),
+                                                                 c->adjA.as<double>(), c->ngroups);
+    }
+    HIPX(hipGetLastError());
+    HIPX(hipMemcpyAsync(qbar_prev, c->adjq.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipMemcpyAsync(vbar_prev, c->adjv.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> eb(youngs_bar ? (size_t)E : 0), ab(act_bar ? (size_t)c->ngroups : 0);
+    if (youngs_bar && E > 0)
+        HIPX(hipMemcpyAsync(eb.data(), c->adjE.p, sizeof(double) * E, hipMemcpyDeviceToHost, c->stream));
+    if (act_bar && c->ngroups > 0)
+        HIPX(hipMemcpyAsync(ab.data(), c->adjA.p, sizeof(double) * c->ngroups, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    for (size_t e = 0; e < eb.size(); ++e) youngs_bar[e] += eb[e];
+    for (size_t g = 0; g < ab.size(); ++g) act_bar[g] += ab[g];
     return SORS_OK;
     SORS_GUARD_END(c)
 }

@@ -300,3 +300,58 @@ def contact_poke(n: int = 40, side: float = 0.05, E: float = 2e5, nu: float = 0.
     c[:, 2] = side + gap - speed * dt * np.arange(steps + 1)
     return PokeScene(scene=sc, surface=box_surface_nodes(X), stiffness=stiffness, friction=friction, eps_v=eps_v,
                      radius=radius, centers=c)
+
+
+@dataclasses.dataclass
+class LegScene:
+    """Fibre-actuated leg: a prism standing on z, its top face clamped, its
+    foot just above the ground plane z = 0 (frictionless penalty contact),
+    with per-element muscle groups, one seeded fibre direction, a seeded
+    activation schedule act[t, g] and a seeded target for the foot centroid."""
+
+    scene: Scene
+    group: np.ndarray      # [E] muscle group
+    fibre: np.ndarray      # [E, 3]
+    sigma_max: float
+    act: np.ndarray        # [steps, n_groups]
+    foot: np.ndarray       # node ids of the foot face
+    target: np.ndarray     # [3]
+    surface: np.ndarray    # contact candidates
+    stiffness: float
+
+
+def leg_prism(nx: int = 14, ny: int = 14, nz: int = 100, lx: float = 0.03, ly: float = 0.03, lz: float = 0.2,
+              E_lo: float = 3e5, E_hi: float = 1e6, nu: float = 0.4, rho: float = 1100.0, sigma_max: float = 2e5,
+              seg: int = 8, steps: int = 100, dt: float = 2e-3, gap: float = 1e-3, stiffness: float = 1e5,
+              act_max: float = 0.5, seed: int = 0) -> LegScene:
+    """nx*ny*nz Kuhn-split cells (6 tets each; 117,600 tets and 22,725 nodes at
+    14 x 14 x 100).  Muscle groups: `seg` segments along z times 4 quadrants in
+    xy.  E per element log-uniform in [E_lo, E_hi]."""
+    X, hexes = box_grid(nx, ny, nz, lx, ly, lz)
+    X = X.copy()
+    X[:, 2] += gap
+    conn = kuhn_split(hexes).astype(np.int32)
+    N, nel = X.shape[0], conn.shape[0]
+    Eel = log_uniform(seed * 0x5851F42D4C957F2D + 3, nel, E_lo, E_hi)
+    top = X[:, 2] > X[:, 2].max() - 1e-9
+    c = X[conn].mean(axis=1)
+    zs = np.minimum(((c[:, 2] - gap) / lz * seg).astype(int), seg - 1)
+    quad = (c[:, 0] > 0.5 * lx).astype(int) + 2 * (c[:, 1] > 0.5 * ly).astype(int)
+    group = (4 * zs + quad).astype(np.int32)
+    ng = 4 * seg
+    u = uniform01(seed * 0x2545F4914F6CDD1D + 5, 2 + 2 * ng + 3)
+    f = np.array([0.2 * (u[0] - 0.5), 0.2 * (u[1] - 0.5), 1.0])
+    f /= np.linalg.norm(f)
+    freq = 0.5 + u[2:2 + ng]
+    phase = 2 * np.pi * u[2 + ng:2 + 2 * ng]
+    t = np.arange(steps)[:, None] / max(steps, 1)
+    act = act_max * 0.5 * (1.0 + np.sin(2 * np.pi * freq[None, :] * t + phase[None, :]))
+    foot = np.nonzero(X[:, 2] < gap + 1e-9)[0].astype(np.int32)
+    foot_c = X[foot].mean(axis=0)
+    off = u[-3:] - 0.5
+    target = foot_c + np.array([0.02 * off[0], 0.02 * off[1], 0.005 + 0.01 * abs(off[2])])
+    sc = Scene(name=f"leg{nx}x{ny}x{nz}", X=X, conn=conn, youngs=Eel, poisson=np.full(nel, nu),
+               density=np.full(nel, rho), bcmask=np.repeat(top.astype(np.int32), 3), bcvalue=np.zeros(3 * N),
+               v0=np.zeros(3 * N), grav=np.array([0.0, 0.0, -9.81]), dt=dt, steps=steps)
+    return LegScene(scene=sc, group=group, fibre=np.tile(f, (nel, 1)), sigma_max=sigma_max, act=act, foot=foot,
+                    target=target, surface=box_surface_nodes(X), stiffness=stiffness)

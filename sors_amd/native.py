@@ -70,6 +70,10 @@ class ContactDesc(C.Structure):
     ]
 
 
+class ActiveDesc(C.Structure):
+    _fields_ = [("n_groups", C.c_int), ("elem_group", _ip), ("fibre", _dp), ("sigma_max", C.c_double)]
+
+
 class ServerOptions(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_double), ("max_active", C.c_int), ("prep_threads", C.c_int),
@@ -144,6 +148,12 @@ def lib():
         L.sors_bench_kernel.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
         L.sors_set_contact.argtypes = [vp, C.POINTER(ContactDesc)]
         L.sors_set_disk.argtypes = [vp, C.c_int, _dp, _dp, C.c_double]
+        L.sors_set_active.argtypes = [vp, C.POINTER(ActiveDesc)]
+        L.sors_set_activation.argtypes = [vp, _dp]
+        L.sors_set_youngs.argtypes = [vp, _dp]
+        L.sors_set_state.argtypes = [vp, _dp, _dp]
+        L.sors_adjoint_step.argtypes = [vp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                        C.c_int, C.POINTER(C.c_longlong)]
         L.sors_server_default_options.argtypes = [C.POINTER(ServerOptions)]
         L.sors_server_default_options.restype = None
         L.sors_server_create.argtypes = [C.POINTER(ServerOptions), C.POINTER(vp)]
@@ -267,6 +277,46 @@ class Rollout:
 
     def set_disk(self, idx, center, normal, radius):
         self._check(lib().sors_set_disk(self.h, int(idx), _d(f64(center)), _d(f64(normal)), float(radius)))
+
+    # ---- actuation / parameters / adjoint ------------------------------------
+    def set_active(self, group, fibre, sigma_max):
+        """Active fibre stress: group[e] in [0, n_groups) or -1, fibre [E, 3]."""
+        g = i32(group)
+        f = f64(fibre).reshape(-1)
+        self._active_keep = (g, f)
+        self.n_groups = int(g.max()) + 1
+        d = ActiveDesc(self.n_groups, _i(g), _d(f), float(sigma_max))
+        self._check(lib().sors_set_active(self.h, C.byref(d)))
+
+    def set_activation(self, act):
+        a = f64(act)
+        assert a.size == self.n_groups
+        self._check(lib().sors_set_activation(self.h, _d(a)))
+
+    def set_youngs(self, E):
+        e = f64(E)
+        assert e.size == self.E
+        self._check(lib().sors_set_youngs(self.h, _d(e)))
+
+    def set_state(self, q=None, v=None):
+        qq = f64(q).reshape(-1) if q is not None else None
+        vv = f64(v).reshape(-1) if v is not None else None
+        self._check(lib().sors_set_state(self.h, _d(qq), _d(vv)))
+
+    def adjoint_step(self, q_prev, v_prev, q_next, dt, qbar_next, vbar_next, youngs_bar=None, act_bar=None,
+                     rtol=1e-10, max_iter=20000):
+        """One backward-Euler step's adjoint: returns (qbar_prev, vbar_prev, pcg_iters);
+        youngs_bar / act_bar (float64 arrays) are accumulated in place."""
+        qb, vb = np.empty(self.n), np.empty(self.n)
+        it = C.c_longlong()
+        args = [f64(q_prev).reshape(-1), f64(v_prev).reshape(-1), f64(q_next).reshape(-1),
+                f64(qbar_next).reshape(-1), f64(vbar_next).reshape(-1)]
+        for a in (youngs_bar, act_bar):
+            assert a is None or (a.dtype == np.float64 and a.flags.c_contiguous)
+        self._check(lib().sors_adjoint_step(self.h, _d(args[0]), _d(args[1]), _d(args[2]), float(dt), _d(args[3]),
+                                            _d(args[4]), _d(qb), _d(vb), _d(youngs_bar), _d(act_bar), float(rtol),
+                                            int(max_iter), C.byref(it)))
+        return qb, vb, it.value
 
     # ---- options / state -------------------------------------------------
     def options(self) -> Options:

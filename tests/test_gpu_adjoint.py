@@ -1,0 +1,123 @@
+"""Active fibre stress and the adjoint on the device.
+
+- active stress: device gradient / energy / Hessian against the FP64 oracle
+  (rel 1e-10) and a short actuated trajectory (1e-6 of the bounding box);
+- adjoint: directional derivatives of the rollout loss |foot(q_T) - target|^2
+  against central finite differences (h = 1e-6) along seeded directions in
+  (E, activation) space, relative error <= 1e-4 (the reference has no
+  gradients to compare with);
+- checkpointed recomputation reproduces forward states (to the rounding of
+  the float atomics in assembly), and the torch.autograd.Function returns
+  the adjoint gradient.
+"""
+import numpy as np
+import pytest
+
+from sors_amd import adjoint, mesh, native
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def small_leg(steps=8, **kw):
+    return mesh.leg_prism(nx=3, ny=3, nz=12, seg=2, steps=steps, **kw)
+
+
+def test_active_stress_matches_oracle(orc):
+    leg = small_leg()
+    sc = leg.scene
+    r = native.Rollout.from_scene(sc, method=native.MINIMIZE_NEWTON)
+    o = orc.System.from_scene(sc, method=orc.NEWTON)
+    r.set_active(leg.group, leg.fibre, leg.sigma_max)
+    o.set_active(leg.group, leg.fibre, leg.sigma_max)
+    r.set_activation(leg.act[3])
+    o.set_activation(leg.act[3])
+    rng = np.random.default_rng(1)
+    q = sc.X.reshape(-1) + rng.normal(scale=2e-4, size=3 * sc.n_nodes) * (1 - sc.bcmask)
+    dt = sc.dt
+    gd, go = r.gradient(q, dt), o.gradient(q, dt)
+    assert np.abs(gd - go).max() <= 1e-10 * np.abs(go).max()
+    assert abs(r.energy(q, dt) - o.energy(q, dt)) <= 1e-12 * max(abs(o.energy(q, dt)), 1.0)
+    for proj in (False, True):
+        _, _, hd = r.hessian(q, dt, native.FP64, project=proj)
+        _, _, ho = o.hessian(q, dt, project=proj)
+        assert np.abs(hd - ho).max() <= 1e-10 * np.abs(ho).max()
+
+
+@pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
+def test_actuated_trajectory_matches_oracle(orc, precision):
+    leg = small_leg(steps=8)
+    sc = leg.scene
+    r = native.Rollout.from_scene(sc, precision=precision, pcg_rtol=1e-8)
+    o = orc.System.from_scene(sc, method=orc.NEWTON_LS)
+    for s in (r, o):
+        s.set_active(leg.group, leg.fibre, leg.sigma_max)
+        s.set_contact(planes=[((0, 0, 0), (0, 0, 1))], surface=leg.surface, stiffness=leg.stiffness)
+    q = qo = sc.X.reshape(-1).copy()
+    worst = 0.0
+    for t in range(sc.steps):
+        r.set_activation(leg.act[t])
+        o.set_activation(leg.act[t])
+        q, _ = r.step(q, sc.dt)
+        qo, _ = o.step(qo, sc.dt)
+        worst = max(worst, np.abs(q - qo).max())
+    say("actuated trajectory worst/bbox", worst / sc.bbox_diag)
+    assert worst <= 1e-6 * sc.bbox_diag
+    assert np.abs(q - sc.X.reshape(-1)).max() > 1e-4  # the muscles moved the leg
+
+
+def test_single_step_adjoint_matches_fd():
+    leg = small_leg(steps=1)
+    ro = adjoint.LegRollout(leg)
+    E = leg.scene.youngs.copy()
+    L, out = adjoint.fd_check(ro, E, leg.act, directions=3, h=1e-6, seed=1)
+    say("1-step loss", L, out)
+    for fd, ad in out:
+        assert abs(fd - ad) <= 1e-4 * max(abs(fd), abs(ad)) + 1e-14
+
+
+def test_rollout_adjoint_with_checkpoints_matches_fd():
+    leg = small_leg(steps=9)
+    ro = adjoint.LegRollout(leg, checkpoint_every=3)
+    E = leg.scene.youngs.copy()
+    L, out = adjoint.fd_check(ro, E, leg.act, directions=3, h=1e-6, seed=2)
+    say("9-step loss", L, out, ro.stats)
+    for fd, ad in out:
+        assert abs(fd - ad) <= 1e-4 * max(abs(fd), abs(ad)) + 1e-14
+
+
+def test_checkpoint_recompute_reproduces_forward():
+    leg = small_leg(steps=6)
+    ro = adjoint.LegRollout(leg, checkpoint_every=2)
+    E = leg.scene.youngs.copy()
+    L, qT, ck = ro.forward(E, leg.act)
+    keep = {}
+    q, v, _, _ = ro._run(E, leg.act, 2, 6, ck[2][0], ck[2][1], keep=keep)
+    say("recompute |dq|", np.abs(q - qT).max())
+    assert np.abs(q - qT).max() <= 1e-12 * leg.scene.bbox_diag
+    assert np.abs(keep[4][0] - ck[4][0]).max() <= 1e-12 * leg.scene.bbox_diag
+
+
+def test_torch_autograd_function():
+    torch = pytest.importorskip("torch")
+    leg = small_leg(steps=4)
+    ro = adjoint.LegRollout(leg, checkpoint_every=2)
+    E = torch.tensor(leg.scene.youngs, dtype=torch.float64, requires_grad=True)
+    A = torch.tensor(leg.act, dtype=torch.float64, requires_grad=True)
+    loss = adjoint.rollout_loss(E, A, ro)
+    (3.0 * loss).backward()
+    L, gE, gA = ro.gradient(leg.scene.youngs.copy(), leg.act)
+    assert float(loss) == pytest.approx(L, rel=1e-12)
+    assert np.allclose(E.grad.numpy(), 3.0 * gE, rtol=1e-9, atol=1e-9 * np.abs(gE).max())
+    assert np.allclose(A.grad.numpy(), 3.0 * gA, rtol=1e-9, atol=1e-9 * np.abs(gA).max())
+
+
+def test_adjoint_rejects_unsupported():
+    sc = mesh.beam(nx=2, ny=1, nz=1, element="hex", steps=1)
+    r = native.Rollout.from_scene(sc)
+    z = np.zeros(3 * sc.n_nodes)
+    with pytest.raises(native.SorsError, match="tetrahedra"):
+        r.adjoint_step(sc.X, z, sc.X, 1e-3, z, z)
