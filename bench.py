@@ -226,6 +226,56 @@ def contact_leg(args, opts):
     return out
 
 
+def adjoint_leg(args):
+    """Differentiable rollout: fibre-actuated leg prism 0.2 x 0.03 x 0.03 m
+    (14 x 14 x 100 Kuhn cells: 117,600 tets, 22,725 nodes), top clamped, foot
+    over a frictionless ground plane (penalty 1e5 N/m), 32 muscle groups,
+    sigma_max 2e5 Pa, 100 backward Euler steps of dt = 2e-3; loss = squared
+    distance of the foot centroid to a seeded target.  Gradient w.r.t. the
+    117,600 Young's moduli and the 100 x 32 activations by the adjoint with
+    sqrt(T) = 10 checkpoints, FP64; checked against central differences
+    (h = 1e-6) along `--adjoint-dirs` seeded directions."""
+    from sors_amd import adjoint, mesh
+
+    leg = mesh.leg_prism(steps=args.adjoint_steps, seed=0)
+    ro = adjoint.LegRollout(leg)
+    E = leg.scene.youngs.copy()
+    t0 = time.perf_counter()
+    fwd = ro.forward(E, leg.act)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    L, gE, gA = ro.gradient(E, leg.act, fwd=fwd)
+    t_bwd = time.perf_counter() - t0
+    out = {"workload": f"{leg.scene.name}: {leg.scene.n_elems} tet4, {leg.scene.n_nodes} nodes, "
+                       f"{args.adjoint_steps} steps dt 2e-3, 32 muscle groups, FP64, minimize_newton",
+           "loss": L, "forward_s": t_fwd, "gradient_s": t_bwd,
+           "gradient_over_forward": t_bwd / t_fwd, "params": int(gE.size + gA.size),
+           "forward_newton": ro.stats["forward_newton"], "forward_pcg": ro.stats["forward_pcg"],
+           "adjoint_pcg": ro.stats["adjoint_pcg"], "checkpoint_every": ro.K}
+    log(f"adjoint: {out}")
+    if args.adjoint_dirs > 0:
+        rng = np.random.default_rng(7)
+        checks = []
+        h = 1e-6
+        for k in range(args.adjoint_dirs):
+            dE = rng.normal(size=E.shape)
+            dA = rng.normal(size=leg.act.shape)
+            nrm = np.sqrt((dE * dE).sum() + (dA * dA).sum())
+            dE /= nrm
+            dA /= nrm
+            Lp = ro.loss(E * (1 + h * dE), leg.act + h * dA)
+            Lm = ro.loss(E * (1 - h * dE), leg.act - h * dA)
+            fd = (Lp - Lm) / (2 * h)
+            ad = float((gE * E) @ dE) + float((gA * dA).sum())
+            checks.append({"fd": fd, "adjoint": ad, "rel_err": abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)})
+            log(f"  direction {k}: fd {fd:.9e} adjoint {ad:.9e}")
+        out["fd_check"] = {"h": h, "directions": args.adjoint_dirs, "max_rel_err": max(c["rel_err"] for c in checks),
+                           "checks": checks,
+                           "sample": f"{args.adjoint_dirs} of the 20 seeded directions (each costs two rollouts)"}
+    ro.close()
+    return out
+
+
 def load_traffic(key):
     try:
         with open(TRAFFIC_FILE) as f:
@@ -282,6 +332,9 @@ def main():
     ap.add_argument("--contact-steps", type=int, default=500, help="contact poke steps (0: skip the contact leg)")
     ap.add_argument("--contact-check", type=int, default=50, help="contact steps repeated in FP64 for the error")
     ap.add_argument("--contact-only", action="store_true", help="run only the contact leg")
+    ap.add_argument("--adjoint-steps", type=int, default=100, help="leg rollout steps (0: skip the adjoint leg)")
+    ap.add_argument("--adjoint-dirs", type=int, default=3, help="finite-difference directions checked")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -292,6 +345,9 @@ def main():
         return
     from sors_amd import native
 
+    if args.adjoint_only:
+        print(json.dumps({"metric": METRIC, "adjoint": adjoint_leg(args)}), flush=True)
+        return
     if args.contact_only:
         opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=int(not args.no_predict))
         print(json.dumps({"metric": METRIC, "contact": contact_leg(args, opts)}), flush=True)
@@ -373,6 +429,8 @@ def main():
     if args.contact_steps > 0:
         out["contact"] = contact_leg(args, dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol,
                                                 predict=int(not args.no_predict)))
+    if args.adjoint_steps > 0:
+        out["adjoint"] = adjoint_leg(args)
     if args.serve > 0:
         out["serving"] = serve_leg(args, dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol,
                                               predict=int(not args.no_predict)))

@@ -29,17 +29,19 @@ class LegRollout:
 
     def __init__(self, leg, precision=native.FP64, checkpoint_every=None, adjoint_rtol=1e-11,
                  method=native.MINIMIZE_NEWTON, **solver):
-        """Solves use minimize_newton (solver.cpp:56-105) by default: its steps
-        converge quadratically to rounding, whereas an Armijo test stops
-        resolving energy decreases near 1e-16 of the energy, which would leave
-        forward states (and finite differences of the loss) noisy at that level."""
+        """Solves use minimize_newton (solver.cpp:56-105) on the unprojected
+        Hessian by default: its steps converge quadratically to rounding
+        (about 5 per step here), whereas an Armijo test stops resolving energy
+        decreases near 1e-16 of the energy and the SPD-projected tangent makes
+        Newton converge only linearly; either would leave forward states (and
+        finite differences of the loss) noisy at the solver tolerance."""
         self.leg = leg
         sc = leg.scene
         self.sc = sc
         self.T = sc.steps
         self.K = checkpoint_every or max(1, int(round(math.sqrt(self.T))))
         self.adjoint_rtol = adjoint_rtol
-        opts = dict(g_tol=0.0, dx_tol=1e-12, pcg_rtol=1e-12, max_newton=100)
+        opts = dict(g_tol=0.0, dx_tol=1e-12, pcg_rtol=1e-12, max_newton=100, project_spd=0)
         opts.update(solver)
         self.r = native.Rollout.from_scene(sc, method=method, precision=precision, **opts)
         self.r.set_contact(planes=[((0.0, 0.0, 0.0), (0.0, 0.0, 1.0))], surface=leg.surface,
