@@ -1387,6 +1387,36 @@ constexpr int kUpdBlock = 192;  // 64 nodes
 template <typename HT>
 __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgState* st_next, PartIO io, int first,
                                 int lb, int nb) {
+#ifndef SORS_UPD_UNROLL
+#define SORS_UPD_UNROLL 8
+#endif
+    constexpr int kUnroll = SORS_UPD_UNROLL;
+    const int n = v.n;
+    const int stride = nb * kUpdBlock;
+    const int c = threadIdx.x % 3, first3 = threadIdx.x - c;
+    // software pipeline: the first trip's vectors and preconditioner rows are
+    // loaded before the prologue's partial sums, whose latency they overlap
+    HT pp[kUnroll], ss[kUnroll], uu0[kUnroll], ww[kUnroll], rr0[kUnroll], xx[kUnroll], mm[kUnroll][3];
+    auto load = [&](int base) {
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            const int i = base + q * stride + threadIdx.x;
+            if (i < n) {
+                uu0[q] = v.u[i];
+                pp[q] = v.p[i];
+                ww[q] = v.w[i];
+                ss[q] = v.s[i];
+                rr0[q] = v.r[i];
+                xx[q] = v.x[i];
+                const HT* M = v.Minv + 3 * (size_t)i;  // row c of node i/3: Minv[9 (i/3) + 3 c] = Minv[3 i]
+                mm[q][0] = M[0];
+                mm[q][1] = M[1];
+                mm[q][2] = M[2];
+            }
+        }
+    };
+    int base = lb * kUpdBlock;
+    if (base < n) load(base);
     if (v.ctl->done) return;
     double g[2], d[1];
     sum_partials_at<2>(io.inU, io.stride, io.nbU, g);  // gamma_{k-1}, rr_{k-1}
@@ -1421,26 +1451,9 @@ __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgStat
         st_next->iter = st_prev->iter + 1;
     }
     const HT a = (HT)alpha, bt = (HT)beta;
-    constexpr int kUnroll = 4;
     __shared__ HT rs[kUnroll][kUpdBlock];
     double acc[2] = {0.0, 0.0};
-    const int c = threadIdx.x % 3, first3 = threadIdx.x - c;
-    const int n = v.n;
-    const int stride = nb * kUpdBlock;
-    for (int base = lb * kUpdBlock; base < n; base += kUnroll * stride) {
-        HT pp[kUnroll], ss[kUnroll], uu0[kUnroll], ww[kUnroll], rr0[kUnroll], xx[kUnroll];
-#pragma unroll
-        for (int q = 0; q < kUnroll; ++q) {
-            const int i = base + q * stride + threadIdx.x;
-            if (i < n) {
-                uu0[q] = v.u[i];
-                pp[q] = v.p[i];
-                ww[q] = v.w[i];
-                ss[q] = v.s[i];
-                rr0[q] = v.r[i];
-                xx[q] = v.x[i];
-            }
-        }
+    for (; base < n; base += kUnroll * stride) {
 #pragma unroll
         for (int q = 0; q < kUnroll; ++q) {
             const int i = base + q * stride + threadIdx.x;
@@ -1461,15 +1474,16 @@ __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgStat
         for (int q = 0; q < kUnroll; ++q) {
             const int i = base + q * stride + threadIdx.x;
             if (i < n) {
-                const HT* M = v.Minv + 3 * (size_t)i;  // row c of node i/3: Minv[9 (i/3) + 3 c] = Minv[3 i]
                 const HT rr = rs[q][threadIdx.x];
-                const HT un = M[0] * rs[q][first3] + M[1] * rs[q][first3 + 1] + M[2] * rs[q][first3 + 2];
+                const HT un = mm[q][0] * rs[q][first3] + mm[q][1] * rs[q][first3 + 1] + mm[q][2] * rs[q][first3 + 2];
                 v.u[i] = un;
                 acc[0] += (double)rr * (double)un;
                 acc[1] += (double)rr * (double)rr;
             }
         }
         __syncthreads();
+        const int next = base + kUnroll * stride;
+        if (next < n) load(next);
     }
     block_partial_at<2>(acc, io.out, io.stride, lb);
 }
@@ -1954,8 +1968,14 @@ void build_preconditioner(sors_ctx* c) {
 }
 
 // launch geometry of the PCG kernels
-constexpr int kSpmvBlocks = 1024;   // multiple of 8 (XCD-contiguous row ranges)
-constexpr int kUpdBlocks = 512;
+#ifndef SORS_SPMV_BLOCKS
+#define SORS_SPMV_BLOCKS 2048
+#endif
+#ifndef SORS_UPD_BLOCKS
+#define SORS_UPD_BLOCKS 512
+#endif
+constexpr int kSpmvBlocks = SORS_SPMV_BLOCKS;   // multiple of 8 (XCD-contiguous row ranges)
+constexpr int kUpdBlocks = SORS_UPD_BLOCKS;
 
 struct PcgGrid {
     int gn;     // init (per node)
