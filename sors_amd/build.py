@@ -32,6 +32,10 @@ HIP_FLAGS = [
     "-munsafe-fp-atomics",
     # no packed FP32 instructions in generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-add fusion decided per source expression, so a device function
+    # rounds the same way in every kernel that inlines it (a batched launch
+    # and a single-rollout launch give identical results)
+    "-ffp-contract=on",
     "-Wno-unused-command-line-argument",
 ]
 

@@ -40,6 +40,10 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
     P.n = n;
     P.dt_nonzero = dtnz;
     auto masked = [&](int dof) { return !sqp && mask && mask[dof] != 0; };
+    if ((long long)E * ed > 0x7fffffffLL) {
+        if (msg) *msg = "mesh exceeds 2^31 element-node incidences";
+        return false;
+    }
     for (long long i = 0; i < (long long)E * ed; ++i)
         if (conn[i] < 0 || conn[i] >= N) {
             if (msg) *msg = "element index out of range";
@@ -65,7 +69,7 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
     {
         std::vector<int> fill(iptr.begin(), iptr.end() - 1);
         for (int e = 0; e < E; ++e)
-            for (int a = 0; a < ed; ++a) inc[fill[conn[(size_t)e * ed + a]]++] = e;
+            for (int a = 0; a < ed; ++a) inc[fill[conn[(size_t)e * ed + a]]++] = e * ed + a;
     }
     // node -> sorted unique neighbour nodes (self included)
     std::vector<int> nptr(N + 1, 0);
@@ -75,7 +79,7 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
         for (long long v = lo; v < hi; ++v) {
             tmp.clear();
             for (int t = iptr[v]; t < iptr[v + 1]; ++t)
-                for (int a = 0; a < ed; ++a) tmp.push_back(conn[(size_t)inc[t] * ed + a]);
+                for (int a = 0; a < ed; ++a) tmp.push_back(conn[(size_t)(inc[t] / ed) * ed + a]);
             std::sort(tmp.begin(), tmp.end());
             tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
             nbr[v] = tmp;
@@ -218,6 +222,9 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
         P.bdiag.assign(n, -1);
         for (int v = 0; v < N; ++v)
             for (int c = 0; c < 3; ++c) P.bdiag[3 * v + c] = (int)block_index(P.sbase.data(), v, P.self_slot[v], c, c);
+        // node -> (element, local index) incidence for the node-centric assembly
+        P.incptr = std::move(iptr);
+        P.inc = std::move(inc);
     } else {
         int nd = 3 * ed;
         P.fullmap.resize((size_t)E * nd * nd);

@@ -78,6 +78,8 @@ struct HostPattern {
     std::vector<int> self_slot;     // N
     std::vector<int> bdiag;         // n: block-layout index of (i,i)
     long long nblocks = 0;          // 3x3 node blocks holding at least one CSR entry
+    std::vector<int> incptr;        // N + 1: node -> incident (element, local node) list
+    std::vector<int> inc;           // e * ed + local index, element order per node
 };
 
 // mask: 3N flags (may be null).  sqp: SQP assembly semantics (no masking).

@@ -409,6 +409,148 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
     }
 }
 
+
+// ---- tet4, node-centric assembly on the sliced node-block layout ---------
+// One lane per node (lanes of a wavefront = the 64 nodes of a slice).  The
+// lane walks its incident elements in a fixed order, recomputes each
+// element's row of 3x3 blocks (its own local node la against lb = 0..3) and
+// its gradient row, and accumulates them into its own row of the layout:
+// no atomics, every value written by exactly one lane, deterministic sums.
+// The lane first zeroes its row (padding included), so no fill is needed.
+// MODE 0: gradient + Hessian; MODE 1: gradient only.
+template <typename HT, int PROJ, int MODE>
+__global__ __launch_bounds__(kBlock) void k_tet_node(ElemArgs a, int nslices, const int* incptr, const int* inc) {
+    const int t = xcd_block() * blockDim.x + threadIdx.x;  // lane slot in the sliced layout
+    const int sl = t / kSlice, lane = t % kSlice;
+    if (sl >= nslices) return;
+    const int v = t;  // node id (>= N: padding lane)
+    HT* vals = (HT*)a.vals;
+    const int width = (a.sbase[sl + 1] - a.sbase[sl]) / kSlice;
+    const long long rowbase = 9LL * a.sbase[sl] + lane;
+    if (MODE == 0)
+        for (int k = 0; k < 9 * width; ++k) vals[rowbase + (long long)kSlice * k] = (HT)0;
+    if (v >= a.N) return;
+    double gacc[3] = {0.0, 0.0, 0.0};
+    bool inv = false;
+    const int j0 = incptr[v], j1 = incptr[v + 1];
+    for (int j = j0; j < j1; ++j) {
+        const int code = inc[j];
+        const int e = code >> 2, la = code & 3;
+        int nd[4];
+        double X[4][3], x[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            nd[k] = a.conn[(size_t)k * a.E + e];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                X[k][c] = a.rest[3 * nd[k] + c];
+                x[k][c] = a.q[3 * nd[k] + c];
+            }
+        }
+        M3<double> Dm;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Dm(r, c) = X[c][r] - X[3][r];
+        const M3<double> Cm = cofactor3(Dm);
+        const double detm = Dm.m[0] * Cm.m[0] + Dm.m[1] * Cm.m[1] + Dm.m[2] * Cm.m[2];
+        const double V0 = fabs(detm) / 6.0;
+        double g[4][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g[k][c] = Cm(c, k) / detm;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[3][c] = -(g[0][c] + g[1][c] + g[2][c]);
+        M3<double> D;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc += (x[k][r] - X[k][r]) * g[k][c];
+                D(r, c) = acc;
+            }
+        const double mu = a.mu[e], lam = a.lam[e];
+        const NHPoint<double> p = nh_point(D);
+        if (!(p.J > 0)) inv = true;
+        double fa[3] = {0.0, 0.0, 0.0}, Ff[3] = {0.0, 0.0, 0.0}, fg[4] = {0.0, 0.0, 0.0, 0.0};
+        const double sa = active_level(a, e, fa);
+        if (sa != 0.0) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) Ff[r] = fa[r] + D(r, 0) * fa[0] + D(r, 1) * fa[1] + D(r, 2) * fa[2];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) fg[k] = fa[0] * g[k][0] + fa[1] * g[k][1] + fa[2] * g[k][2];
+        }
+        // gradient row of node la: V0 P g_la (the element scatter's sum order is per element, so the
+        // same addends in another order)
+        {
+            M3<double> P = nh_stress(p, mu, lam);
+            if (sa != 0.0)
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) P(r, c) += sa * Ff[r] * fa[c];
+            double ga[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ga[c] = g[la][c];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                if (dof_kept(a, 3 * v + r)) gacc[r] += V0 * (P(r, 0) * ga[0] + P(r, 1) * ga[1] + P(r, 2) * ga[2]);
+        }
+        if (MODE == 1) continue;
+        const HT w = (HT)(V0 * a.hscale);
+        HT gh[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gh[k][c] = (HT)g[k][c];
+        const HT muh = (HT)mu, lamh = (HT)lam;
+        Sym9<HT> S;
+        HT cc = 0;
+        HT h[4][3];
+        if (PROJ) {
+            M3<HT> Dh;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Dh.m[i] = (HT)D.m[i];
+            const NHPoint<HT> ph = nh_point(Dh);
+            nh_projected_tangent(Dh, ph, muh, lamh, S);
+        } else {
+            cc = (HT)(lam * p.lnJ - mu);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                    h[k][r] = (HT)(p.FiT(r, 0) * g[k][0] + p.FiT(r, 1) * g[k][1] + p.FiT(r, 2) * g[k][2]);
+        }
+#pragma unroll
+        for (int lb = 0; lb < 4; ++lb) {
+            const int rk = a.rank[(size_t)(la * 4 + lb) * a.E + e];
+            if (rk == kRankSkip) continue;
+            HT blk[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) blk[i] = 0;
+            if (PROJ)
+                block_from_tangent(S, gh[la], gh[lb], w, blk);
+            else
+                block_raw(muh, lamh, cc, gh[la], gh[lb], h[la], h[lb], w, blk);
+            if (sa != 0.0) {
+                const HT s_ab = w * (HT)(sa * fg[la] * fg[lb]);
+                blk[0] += s_ab;
+                blk[4] += s_ab;
+                blk[8] += s_ab;
+            }
+            const long long base = rowbase + (long long)kSlice * 9 * rk;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) vals[base + (long long)kSlice * k] += blk[k];
+        }
+    }
+    if (inv) atomicOr(a.flag, 1);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) a.gint[3 * v + r] = gacc[r];
+}
+
 // ---- hex8: one element per lane --------------------------------------------
 // Reference cube corners (hexahedron.cpp:171-179) and 5-tet volume split (:106-130)
 __constant__ signed char kHexXi[8][3] = {{1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, -1},
@@ -1130,91 +1272,87 @@ __global__ __launch_bounds__(kBlock) void k_adj_state(int n, double dt, double d
 // per-element parameter adjoints of the tet4 residual at q1:
 //   Ebar_e = -lambda_e . dg_e/dE (Neo-Hookean gradient with the Lame
 //   constants' derivatives dmu/dE = 1/(2(1+nu)), dlam/dE = nu/((1+nu)(1-2nu)));
-//   abar_g += -lambda_e . dg_e/da (active term with a = 1: V0 sigma (F f)(f.g_k)).
+//   abar_e = -lambda_e . dg_e/da (active term with a = 1: V0 sigma (F f)(f.g_k)),
+//   summed per muscle group by k_group_sum in a fixed order.
 // Dirichlet DOFs take no element terms (their rows are the Dirichlet rows).
-constexpr int kAdjGroupsLds = 64;
 __global__ __launch_bounds__(kBlock) void k_adj_params_tet(ElemArgs a, const double* nu, const double* lam,
-                                                           double* Ebar, double* abar, int ngroups) {
-    __shared__ double gacc[kAdjGroupsLds];
-    const bool lds = ngroups <= kAdjGroupsLds;
-    if (lds)
-        for (int i = threadIdx.x; i < ngroups; i += blockDim.x) gacc[i] = 0.0;
-    __syncthreads();
+                                                           double* Ebar, double* abar_e) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < a.E) {
-        int nd[4];
-        double X[4][3], x[4][3];
+    if (e >= a.E) return;
+    int nd[4];
+    double X[4][3], x[4][3];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            nd[k] = a.conn[(size_t)k * a.E + e];
+    for (int k = 0; k < 4; ++k) {
+        nd[k] = a.conn[(size_t)k * a.E + e];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                X[k][c] = a.rest[3 * nd[k] + c];
-                x[k][c] = a.q[3 * nd[k] + c];
-            }
-        }
-        M3<double> Dm;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) Dm(r, c) = X[c][r] - X[3][r];
-        const M3<double> Cm = cofactor3(Dm);
-        const double detm = Dm.m[0] * Cm.m[0] + Dm.m[1] * Cm.m[1] + Dm.m[2] * Cm.m[2];
-        const double V0 = fabs(detm) / 6.0;
-        double g[4][3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) g[k][c] = Cm(c, k) / detm;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) g[3][c] = -(g[0][c] + g[1][c] + g[2][c]);
-        M3<double> D;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                double acc = 0;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) acc += (x[k][r] - X[k][r]) * g[k][c];
-                D(r, c) = acc;
-            }
-        const double v = nu[e];
-        const double mu1 = 1.0 / (2.0 * (1.0 + v)), lam1 = v / ((1.0 + v) * (1.0 - 2.0 * v));
-        const NHPoint<double> p = nh_point(D);
-        const M3<double> P1 = nh_stress(p, mu1, lam1);
-        double fa[3] = {0.0, 0.0, 0.0}, Ff[3] = {0.0, 0.0, 0.0};
-        const int gi = a.agroup ? a.agroup[e] : -1;
-        if (gi >= 0) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) fa[c] = a.afib[(size_t)c * a.E + e];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) Ff[r] = fa[r] + D(r, 0) * fa[0] + D(r, 1) * fa[1] + D(r, 2) * fa[2];
-        }
-        double sE = 0.0, sA = 0.0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double fg = fa[0] * g[k][0] + fa[1] * g[k][1] + fa[2] * g[k][2];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                const int dof = 3 * nd[k] + r;
-                if (!dof_kept(a, dof)) continue;
-                const double l = lam[dof];
-                sE += l * V0 * (P1(r, 0) * g[k][0] + P1(r, 1) * g[k][1] + P1(r, 2) * g[k][2]);
-                if (gi >= 0) sA += l * V0 * a.sigma * Ff[r] * fg;
-            }
-        }
-        Ebar[e] = -sE;
-        if (gi >= 0) {
-            if (lds)
-                atomicAdd(&gacc[gi], -sA);
-            else
-                atomicAdd(&abar[gi], -sA);
+        for (int c = 0; c < 3; ++c) {
+            X[k][c] = a.rest[3 * nd[k] + c];
+            x[k][c] = a.q[3 * nd[k] + c];
         }
     }
-    __syncthreads();
-    if (lds)
-        for (int i = threadIdx.x; i < ngroups; i += blockDim.x)
-            if (gacc[i] != 0.0) atomicAdd(&abar[i], gacc[i]);
+    M3<double> Dm;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Dm(r, c) = X[c][r] - X[3][r];
+    const M3<double> Cm = cofactor3(Dm);
+    const double detm = Dm.m[0] * Cm.m[0] + Dm.m[1] * Cm.m[1] + Dm.m[2] * Cm.m[2];
+    const double V0 = fabs(detm) / 6.0;
+    double g[4][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[k][c] = Cm(c, k) / detm;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[3][c] = -(g[0][c] + g[1][c] + g[2][c]);
+    M3<double> D;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc += (x[k][r] - X[k][r]) * g[k][c];
+            D(r, c) = acc;
+        }
+    const double v = nu[e];
+    const double mu1 = 1.0 / (2.0 * (1.0 + v)), lam1 = v / ((1.0 + v) * (1.0 - 2.0 * v));
+    const NHPoint<double> p = nh_point(D);
+    const M3<double> P1 = nh_stress(p, mu1, lam1);
+    double fa[3] = {0.0, 0.0, 0.0}, Ff[3] = {0.0, 0.0, 0.0};
+    const int gi = a.agroup ? a.agroup[e] : -1;
+    if (gi >= 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fa[c] = a.afib[(size_t)c * a.E + e];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) Ff[r] = fa[r] + D(r, 0) * fa[0] + D(r, 1) * fa[1] + D(r, 2) * fa[2];
+    }
+    double sE = 0.0, sA = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double fg = fa[0] * g[k][0] + fa[1] * g[k][1] + fa[2] * g[k][2];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int dof = 3 * nd[k] + r;
+            if (!dof_kept(a, dof)) continue;
+            const double l = lam[dof];
+            sE += l * V0 * (P1(r, 0) * g[k][0] + P1(r, 1) * g[k][1] + P1(r, 2) * g[k][2]);
+            if (gi >= 0) sA += l * V0 * a.sigma * Ff[r] * fg;
+        }
+    }
+    Ebar[e] = -sE;
+    if (abar_e) abar_e[e] = -sA;
+}
+
+// out[g] = sum of val[e] over elements of group g, in element order per
+// thread and a fixed tree across the block (one block per group)
+__global__ __launch_bounds__(kBlock) void k_group_sum(int E, const int* group, const double* val, double* out) {
+    const int gsel = blockIdx.x;
+    double s[1] = {0.0};
+    for (int e = threadIdx.x; e < E; e += blockDim.x)
+        if (group[e] == gsel) s[0] += val[e];
+    block_sum<1>(s);
+    if (threadIdx.x == 0) out[gsel] = s[0];
 }
 
 struct PcgPart {
@@ -1732,6 +1870,7 @@ struct sors_ctx {
     bool pat_on_device = false;
     DevMem pcgst_bench, copybuf;
     DevMem sbase, bcol, selfslot;   // sliced node-block layout (node-granular masks)
+    DevMem incptr, inc;             // node -> (element, local node) incidence (node-centric assembly)
     // active fibre stress (sors_set_active) and parameter adjoints
     bool has_active = false;
     int ngroups = 0;
@@ -1809,6 +1948,10 @@ void ensure_pattern(sors_ctx* c, double dt) {
         upload(c->selfslot, c->pat.self_slot.data(), c->pat.self_slot.size(), c->stream);
         upload(c->diag, c->pat.bdiag.data(), c->pat.bdiag.size(), c->stream);
         c->nvals = 9LL * c->pat.sbase[c->pat.nslices];
+        if (c->ed == 4) {
+            upload(c->incptr, c->pat.incptr.data(), c->pat.incptr.size(), c->stream);
+            upload(c->inc, c->pat.inc.data(), c->pat.inc.size(), c->stream);
+        }
     } else {
         upload(c->fullmap, c->pat.fullmap.data(), c->pat.fullmap.size(), c->stream);
         upload(c->diag, c->pat.diag.data(), c->pat.diag.size(), c->stream);
@@ -1892,8 +2035,25 @@ void contact_lag(sors_ctx* c) {
 }
 
 // element kernel launch; with e0/e1 the dispatch records the two events
+// the node-centric tet kernel writes every gradient entry and every value of
+// the sliced layout itself (no fills, no atomics)
+bool node_assembly(const sors_ctx* c) { return c->block() && c->ed == 4 && c->material == SORS_NEOHOOKEAN; }
+
 template <typename HT, int MODE>
 void launch_elem(sors_ctx* c, const ElemArgs& a, int project, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    if (MODE != 2 && node_assembly(c)) {
+        const int lanes = c->pat.nslices * kSlice;
+        const dim3 grid(grid_for(lanes, kBlock));
+        void (*k)(ElemArgs, int, const int*, const int*) =
+            project ? k_tet_node<HT, 1, MODE> : k_tet_node<HT, 0, MODE>;
+        if (e0)
+            hipExtLaunchKernelGGL(k, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a, c->pat.nslices,
+                                  (const int*)c->incptr.as<int>(), (const int*)c->inc.as<int>());
+        else
+            k<<<grid, kBlock, 0, c->stream>>>(a, c->pat.nslices, c->incptr.as<int>(), c->inc.as<int>());
+        HIPX(hipGetLastError());
+        return;
+    }
     if (c->E == 0) return;
     const bool tet = c->ed == 4;
     const int block = tet ? kBlock : 128;
@@ -1923,8 +2083,10 @@ void clear_flag(sors_ctx* c) { fill<int>(c->F(), 1, 0, c->stream); }
 template <typename HT>
 void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project) {
     ElemArgs a = elem_args(c, q);
-    fill<double>(c->gint.as<double>(), c->n, 0.0, c->stream);
-    if (hessian) fill<HT>((HT*)c->vals.p, c->nvals, (HT)0, c->stream);
+    if (!node_assembly(c)) {
+        fill<double>(c->gint.as<double>(), c->n, 0.0, c->stream);
+        if (hessian) fill<HT>((HT*)c->vals.p, c->nvals, (HT)0, c->stream);
+    }
     a.gint = c->gint.as<double>();
     a.vals = hessian ? c->vals.p : nullptr;
     a.hscale = (c->scheme == SORS_CRANK_NICOLSON && dt != 0.0) ? 0.5 : 1.0;
@@ -2385,6 +2547,7 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->pw,     &c->px,    &c->partials, &c->scal, &c->pcgst,   &c->pcgctl,  &c->pcgpart,
                            &c->flag,   &c->sbase, &c->bcol,    &c->selfslot, &c->csurf, &c->lagN,
                            &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
+                           &c->incptr, &c->inc,
                            &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA};
     double b = 0;
     for (const DevMem* d : all) b += (double)d->bytes;
@@ -3572,12 +3735,19 @@ int sors_adjoint_step(sors_ctx* c, const double* q_prev, const double* v_prev, c
                                                        c->has_mask ? c->mask.as<int>() : nullptr, lam,
                                                        c->dxd.as<double>(), c->adjq.as<double>(),
                                                        c->adjv.as<double>());
-    fill<double>(c->adjA.as<double>(), std::max(c->ngroups, 1), 0.0, c->stream);
     if (E > 0) {
         if (c->nu.bytes < sizeof(double) * (size_t)E) upload(c->nu, c->h_nu.data(), (size_t)E, c->stream);
         ElemArgs a = elem_args(c, c->q.as<double>());
-        k_adj_params_tet<<<grid_for(E), kBlock, 0, c->stream>>>(a, c->nu.as<double>(), lam, c->adjE.as<double>(),
-                                                                 c->adjA.as<double>(), c->ngroups);
+        double* ae = nullptr;
+        if (c->has_active) {
+            ensure(c->qtmp, sizeof(double) * std::max(E, c->n));
+            ae = c->qtmp.as<double>();
+        }
+        k_adj_params_tet<<<grid_for(E), kBlock, 0, c->stream>>>(a, c->nu.as<double>(), lam, c->adjE.as<double>(), ae);
+        if (c->has_active)
+            k_group_sum<<<c->ngroups, kBlock, 0, c->stream>>>(E, c->agroup.as<int>(), ae, c->adjA.as<double>());
+        else
+            fill<double>(c->adjA.as<double>(), 1, 0.0, c->stream);
     }
     HIPX(hipGetLastError());
     HIPX(hipMemcpyAsync(qbar_prev, c->adjq.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));

@@ -1,9 +1,9 @@
 """Serving loop (continuous batching) through the C ABI.
 
-Each rollout's result must be the same alone and in the mix: within 1e-6 of
-the bounding-box diagonal (the bar of converged positions), against a plain
-device-resident rollout stepped on its own.  The two differ only by the
-rounding order of float atomics in assembly.
+Each rollout's result must be the same alone and in the mix, against a plain
+device-resident rollout stepped on its own.  Tet assembly on the node-block
+layout is atomic-free and a batched PCG launch keeps each rollout's own
+block counts and reduction order, so the two agree bit for bit.
 """
 import numpy as np
 import pytest
@@ -49,7 +49,7 @@ def test_serving_mix_matches_alone():
         err = np.abs(q - ref).max() / sc.bbox_diag
         say(sc.name, "mix vs alone", err, rs["newton_iters"], rs["pcg_iters"], rs["latency_ms"])
         assert rs["steps"] == sc.steps and rs["status"] == native.OK
-        assert err <= 1e-6
+        assert np.array_equal(q, ref)
         assert np.abs(q - sc.X).max() > 1e-6  # the rollout moved
 
 
@@ -61,7 +61,7 @@ def test_serving_single_request_matches_plain_fp64():
     ref = alone(sc, precision=native.FP64)
     err = np.abs(q - ref).max() / sc.bbox_diag
     say("fp64 server vs alone", err)
-    assert err <= 1e-9
+    assert np.array_equal(q, ref)
 
 
 def test_serving_budget_limits_residency():
