@@ -276,6 +276,28 @@ def adjoint_leg(args):
     return out
 
 
+def hex_leg(args, opts):
+    """The same beam physics on the 200 x 40 x 40 hex8 grid (320,000
+    elements, 2 x 2 x 2 Gauss points, 27-point node stencil), mixed
+    precision, against plain FP64 on the GPU for the same steps."""
+    from sors_amd import mesh, native
+
+    sc = mesh.cantilever_large(element="hex", seed=0)
+    out = {"workload": f"{sc.name}: {sc.n_elems} hex8, {sc.n_nodes} nodes, E log-uniform [5e5,5e6] seed 0, "
+                       f"nu 0.45, BE dt 1e-3", "steps": args.hex_steps}
+    res = {}
+    for name, prec in (("mixed", native.MIXED), ("fp64", native.FP64)):
+        r, q, el, agg = run_gpu(sc, prec, args.hex_steps, 1, profile=False, **opts)
+        r.close()
+        res[name] = q
+        out[name] = {"steps_per_s": args.hex_steps / el, "ms_per_step": 1e3 * el / args.hex_steps,
+                     "newton_per_step": agg["newton_iters"] / args.hex_steps,
+                     "pcg_per_step": agg["pcg_iters"] / args.hex_steps}
+        log(f"hex {name}: {out[name]}")
+    out["max_pos_err_over_bbox_mixed_vs_fp64"] = float(np.abs(res["mixed"] - res["fp64"]).max() / sc.bbox_diag)
+    return out
+
+
 def load_traffic(key):
     try:
         with open(TRAFFIC_FILE) as f:
@@ -319,6 +341,8 @@ def main():
     ap.add_argument("--pcg-rtol", type=float, default=3e-3)
     ap.add_argument("--g-tol", type=float, default=1e-8, help="Newton residual criterion ||g||_2 (N)")
     ap.add_argument("--no-predict", action="store_true", help="start Newton from the previous positions")
+    ap.add_argument("--predictor", type=int, default=2, choices=[0, 1, 2],
+                    help="Newton start: 0 previous positions, 1 constant velocity, 2 constant acceleration")
     ap.add_argument("--check-every", type=int, default=32, help="PCG iterations between host convergence checks")
     ap.add_argument("--serve", type=int, default=48, help="serving requests to run (0: skip the serving leg)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
@@ -335,7 +359,11 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=100, help="leg rollout steps (0: skip the adjoint leg)")
     ap.add_argument("--adjoint-dirs", type=int, default=3, help="finite-difference directions checked")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg")
+    ap.add_argument("--hex-steps", type=int, default=2, help="hex8 beam steps (0: skip the hex leg)")
+    ap.add_argument("--hex-only", action="store_true", help="run only the hex8 leg")
     args = ap.parse_args()
+    if args.no_predict:
+        args.predictor = 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
@@ -345,20 +373,24 @@ def main():
         return
     from sors_amd import native
 
+    if args.hex_only:
+        opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=args.predictor)
+        print(json.dumps({"metric": METRIC, "hex": hex_leg(args, opts)}), flush=True)
+        return
     if args.adjoint_only:
         print(json.dumps({"metric": METRIC, "adjoint": adjoint_leg(args)}), flush=True)
         return
     if args.contact_only:
-        opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=int(not args.no_predict))
+        opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=args.predictor)
         print(json.dumps({"metric": METRIC, "contact": contact_leg(args, opts)}), flush=True)
         return
     if args.serve_only:
-        opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=int(not args.no_predict))
+        opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=args.predictor)
         print(json.dumps({"metric": METRIC, "serving": serve_leg(args, opts)}), flush=True)
         return
     sc = make_scene(args)
     log(f"workload {sc.name}: {sc.n_elems} elements, {sc.n_nodes} nodes; warmup {args.warmup}, steps {args.steps}")
-    opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=int(not args.no_predict),
+    opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=args.predictor,
                 use_graph=int(not args.no_graph), pcg_check_every=args.check_every)
     r, q_mixed, elapsed, agg = run_gpu(sc, native.MIXED, args.steps, args.warmup, profile=True, **opts)
     nnz = int(len(r.pattern(sc.dt)[1]))
@@ -426,14 +458,16 @@ def main():
         cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sc, agg, args.steps, cores)
         log(f"cpu baseline: {out['cpu_baseline']}")
+    if args.hex_steps > 0:
+        out["hex"] = hex_leg(args, dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=args.predictor))
     if args.contact_steps > 0:
         out["contact"] = contact_leg(args, dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol,
-                                                predict=int(not args.no_predict)))
+                                                predict=args.predictor))
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_leg(args)
     if args.serve > 0:
         out["serving"] = serve_leg(args, dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol,
-                                              predict=int(not args.no_predict)))
+                                              predict=args.predictor))
     print(json.dumps(out), flush=True)
 
 

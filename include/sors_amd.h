@@ -83,9 +83,12 @@ typedef struct {
     int pcg_check_every;  /* PCG iterations between host convergence checks */
     int use_graph;        /* replay PCG iteration chunks from a captured hipGraph */
     int profile_kernels;  /* time SpMV and assembly launches with HIP events */
-    int predict;          /* start Newton from qPrev + dt*vPrev (Dirichlet DOFs:
-                             qPrev + dt*value) instead of the caller's guess; the
-                             converged state is the same minimiser */
+    int predict;          /* Newton's start instead of the caller's guess: 0 the
+                             guess, 1 qPrev + dt*vPrev, 2 (default) qPrev + dt*(vPrev
+                             + (vPrev - v before the last step)) once a step has been
+                             taken since the state was set; Dirichlet DOFs start at
+                             qPrev + dt*value.  The converged state is the same
+                             minimiser */
 } sors_options;
 
 typedef struct {

@@ -41,7 +41,10 @@ class LegRollout:
         self.T = sc.steps
         self.K = checkpoint_every or max(1, int(round(math.sqrt(self.T))))
         self.adjoint_rtol = adjoint_rtol
-        opts = dict(g_tol=0.0, dx_tol=1e-12, pcg_rtol=1e-12, max_newton=100, project_spd=0)
+        # predict=1: Newton's start depends on the state only (not on the step
+        # before it), so recomputing from any checkpoint reproduces the forward
+        # states bit for bit
+        opts = dict(g_tol=0.0, dx_tol=1e-12, pcg_rtol=1e-12, max_newton=100, project_spd=0, predict=1)
         opts.update(solver)
         self.r = native.Rollout.from_scene(sc, method=method, precision=precision, **opts)
         self.r.set_contact(planes=[((0.0, 0.0, 0.0), (0.0, 0.0, 1.0))], surface=leg.surface,

@@ -240,7 +240,7 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
     sors_default_options(&o);
     o.precision = settings.precision == "fp64" ? SORS_FP64 : SORS_MIXED;
     o.pcg_rtol = settings.pcgTolerance;
-    o.predict = settings.predictor ? 1 : 0;
+    o.predict = settings.predictor ? 2 : 0;
     check_status(sors_set_options(ctx_, &o), ctx_);
 }
 

@@ -76,7 +76,7 @@ struct SimulationSettings {
     // device-path additions (not in the reference)
     std::string precision = "mixed";  // "mixed" | "fp64"
     double pcgTolerance = 1e-6;
-    bool predictor = false;           // start Newton from qPrev + dt vPrev instead of the given solution
+    bool predictor = false;           // start Newton from the extrapolated state instead of the given solution
     void verbose_print() const;
 };
 

@@ -1762,10 +1762,16 @@ __global__ void k_to_f64(long long n, const HT* a, double* out) {
 }
 
 // Newton initial guess q = qPrev + h vPrev (Dirichlet DOFs: qPrev + h value)
-__global__ void k_predict(int n, double h, const double* qprev, const double* vprev, const int* mask,
-                          const double* bcval, double* q) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        q[i] = qprev[i] + h * ((mask && mask[i]) ? bcval[i] : vprev[i]);
+// Newton's starting point: q = qprev + h v with v the last velocity (vold
+// null) or extrapolated with the last acceleration, v + (h/h_old)(v - vold);
+// Dirichlet DOFs move with their prescribed velocity
+__global__ void k_predict(int n, double h, const double* qprev, const double* vprev, const double* vold,
+                          double ratio, const int* mask, const double* bcval, double* q) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double v = vprev[i];
+        if (vold) v += ratio * (vprev[i] - vold[i]);
+        q[i] = qprev[i] + h * ((mask && mask[i]) ? bcval[i] : v);
+    }
 }
 
 __global__ void k_axpy(int n, double alpha, const double* dx, double* q) {
@@ -1774,9 +1780,10 @@ __global__ void k_axpy(int n, double alpha, const double* dx, double* q) {
 
 // TimeIntegrator::update_state (integrator.h:79-97)
 __global__ void k_update_state(int n, int scheme, double dt, const double* q, double* qprev, double* vprev,
-                               double* gprev, const double* gtmp) {
+                               double* gprev, const double* gtmp, double* vold) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double v = (scheme == SORS_BACKWARD_EULER) ? (q[i] - qprev[i]) / dt : 2 * (q[i] - qprev[i]) / dt - vprev[i];
+        if (vold) vold[i] = vprev[i];
         vprev[i] = v;
         qprev[i] = q[i];
         gprev[i] = gtmp[i];
@@ -1871,6 +1878,9 @@ struct sors_ctx {
     DevMem pcgst_bench, copybuf;
     DevMem sbase, bcol, selfslot;   // sliced node-block layout (node-granular masks)
     DevMem incptr, inc;             // node -> (element, local node) incidence (node-centric assembly)
+    DevMem vold;                    // velocity before the last step (second-order predictor)
+    bool vold_valid = false;
+    double vold_dt = 0.0;
     // active fibre stress (sors_set_active) and parameter adjoints
     bool has_active = false;
     int ngroups = 0;
@@ -2454,6 +2464,31 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
     return SORS_OK;
 }
 
+// Newton's starting point for a step of size h (options.predict: 0 keep q,
+// 1 constant velocity, 2 constant acceleration once a previous step exists)
+void launch_predict(sors_ctx* c, double h) {
+    if (!c->opt.predict) return;
+    const bool acc = c->opt.predict >= 2 && c->vold_valid && c->vold_dt > 0.0;
+    k_predict<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
+        c->n, h, c->qprev.as<double>(), c->vprev.as<double>(), acc ? c->vold.as<double>() : nullptr,
+        acc ? h / c->vold_dt : 0.0, c->has_mask ? c->mask.as<int>() : nullptr, c->bcval.as<double>(),
+        c->q.as<double>());
+}
+
+// TimeIntegrator::update_state after a converged step of size h
+void launch_update_state(sors_ctx* c, double h) {
+    double* vold = nullptr;
+    if (c->opt.predict >= 2) {
+        ensure(c->vold, sizeof(double) * c->n);
+        vold = c->vold.as<double>();
+    }
+    k_update_state<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
+        c->n, c->scheme, h, c->q.as<double>(), c->qprev.as<double>(), c->vprev.as<double>(), c->gprev.as<double>(),
+        c->gtmp.as<double>(), vold);
+    c->vold_valid = vold != nullptr;
+    c->vold_dt = h;
+}
+
 int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
     if (!(dt > 0.0)) return fail(c, SORS_ERR_ARG, "dt must be > 0 (resolve settings.dt before the call)");
     if (substeps < 1) substeps = 1;
@@ -2471,16 +2506,11 @@ int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
     const double h = dt / substeps;
     ensure_pattern(c, h);
     for (int j = 0; j < substeps; ++j) {
-        if (c->opt.predict)
-            k_predict<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
-                c->n, h, c->qprev.as<double>(), c->vprev.as<double>(), c->has_mask ? c->mask.as<int>() : nullptr,
-                c->bcval.as<double>(), c->q.as<double>());
+        launch_predict(c, h);
         contact_lag(c);
         int rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, h, st) : minimize<double>(c, h, st);
         if (rc != SORS_OK) return rc;
-        k_update_state<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
-            c->n, c->scheme, h, c->q.as<double>(), c->qprev.as<double>(), c->vprev.as<double>(),
-            c->gprev.as<double>(), c->gtmp.as<double>());
+        launch_update_state(c, h);
         HIPX(hipGetLastError());
         std::memcpy(c->cargs.dc_last, c->cargs.dc, sizeof(c->cargs.dc));
     }
@@ -2547,7 +2577,7 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->pw,     &c->px,    &c->partials, &c->scal, &c->pcgst,   &c->pcgctl,  &c->pcgpart,
                            &c->flag,   &c->sbase, &c->bcol,    &c->selfslot, &c->csurf, &c->lagN,
                            &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
-                           &c->incptr, &c->inc,
+                           &c->incptr, &c->inc,  &c->vold,
                            &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA};
     double b = 0;
     for (const DevMem* d : all) b += (double)d->bytes;
@@ -2740,10 +2770,7 @@ void srv_tick(sors_server* s) {
         switch (m.phase) {
             case Member::ASSEMBLE:
                 if (m.new_step) {
-                    if (o.predict)
-                        k_predict<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
-                            c->n, dt, c->qprev.as<double>(), c->vprev.as<double>(),
-                            c->has_mask ? c->mask.as<int>() : nullptr, c->bcval.as<double>(), q);
+                    launch_predict(c, dt);
                     m.new_step = false;
                 }
                 clear_flag(c);
@@ -2850,9 +2877,7 @@ void srv_tick(sors_server* s) {
         int status = 1;  // 1: continue, 0: finished, < 0: error
         std::string msg;
         auto end_step = [&] {
-            k_update_state<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(
-                c->n, c->scheme, dt, q, c->qprev.as<double>(), c->vprev.as<double>(), c->gprev.as<double>(),
-                c->gtmp.as<double>());
+            launch_update_state(c, dt);
             m.step++;
             if (m.step >= m.r->steps) {
                 status = 0;
@@ -3032,7 +3057,7 @@ void sors_default_options(sors_options* o) {
     o->pcg_check_every = 32;
     o->use_graph = 1;
     o->profile_kernels = 0;
-    o->predict = 1;
+    o->predict = 2;
 }
 
 int sors_create(const sors_system_desc* d, sors_ctx** out) {
@@ -3214,6 +3239,7 @@ int sors_get_options(const sors_ctx* c, sors_options* o) {
 
 int sors_set_initial_velocity(sors_ctx* c, const double* v) {
     SORS_GUARD(c)
+    c->vold_valid = false;
     if (!v) return fail(c, SORS_ERR_ARG, "null velocity");
     upload(c->vprev, v, (size_t)c->n, c->stream);
     sync(c);
@@ -3223,6 +3249,7 @@ int sors_set_initial_velocity(sors_ctx* c, const double* v) {
 
 int sors_set_initial_deformation(sors_ctx* c, const double* q) {
     SORS_GUARD(c)
+    c->vold_valid = false;
     if (!q) return fail(c, SORS_ERR_ARG, "null positions");
     upload(c->qprev, q, (size_t)c->n, c->stream);
     upload(c->q, q, (size_t)c->n, c->stream);
@@ -3686,6 +3713,7 @@ int sors_set_youngs(sors_ctx* c, const double* youngs) {
 
 int sors_set_state(sors_ctx* c, const double* q, const double* v) {
     SORS_GUARD(c)
+    c->vold_valid = false;
     if (q) {
         upload(c->qprev, q, (size_t)c->n, c->stream);
         upload(c->q, q, (size_t)c->n, c->stream);
