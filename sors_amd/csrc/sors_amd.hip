@@ -40,12 +40,15 @@
 namespace sors {
 
 // ===========================================================================
-// Reductions: deterministic two-level sums (block tree, then the last block
-// to finish sums the per-block partials in a fixed order).
+// Reductions: deterministic two-level sums.  A block reduces its values with
+// a fixed tree (wavefront shuffles, then the wave sums in order) and writes
+// one partial per block; the consumer (the next kernel's prologue, or a
+// one-block finalize) sums the partials in a fixed order.
 // ===========================================================================
+constexpr int kMaxWaves = 1024 / 64;  // largest workgroup on gfx950
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double sh[NV][kBlock / 64];
+    __shared__ double sh[NV][kMaxWaves];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -1521,7 +1524,10 @@ __device__ void pcg_spmv_body(const PcgView<HT>& v, PcgState* st, PartIO io, int
 // partials (r.u, r.r).  One thread per DOF (unit-stride vector traffic), the
 // three residual components of a node meet in LDS for u = M r; kUnroll DOF
 // groups per trip keep their loads in flight together.
-constexpr int kUpdBlock = 192;  // 64 nodes
+#ifndef SORS_UPD_THREADS
+#define SORS_UPD_THREADS 384
+#endif
+constexpr int kUpdBlock = SORS_UPD_THREADS;  // multiple of 3 (whole nodes)
 template <typename HT>
 __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgState* st_next, PartIO io, int first,
                                 int lb, int nb) {
@@ -1553,9 +1559,9 @@ __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgStat
             }
         }
     };
+    if (v.ctl->done) return;  // before the prefetch: launches after convergence cost no traffic
     int base = lb * kUpdBlock;
     if (base < n) load(base);
-    if (v.ctl->done) return;
     double g[2], d[1];
     sum_partials_at<2>(io.inU, io.stride, io.nbU, g);  // gamma_{k-1}, rr_{k-1}
     sum_partials_at<1>(io.inA, io.stride, io.nbA, d);  // delta_{k-1}
