@@ -344,6 +344,8 @@ def main():
     ap.add_argument("--predictor", type=int, default=2, choices=[0, 1, 2],
                     help="Newton start: 0 previous positions, 1 constant velocity, 2 constant acceleration")
     ap.add_argument("--check-every", type=int, default=32, help="PCG iterations between host convergence checks")
+    ap.add_argument("--coarse-nodes", type=int, default=0,
+                    help="nodes per aggregate of the two-level PCG preconditioner (0: block-Jacobi only)")
     ap.add_argument("--serve", type=int, default=48, help="serving requests to run (0: skip the serving leg)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
@@ -391,7 +393,7 @@ def main():
     sc = make_scene(args)
     log(f"workload {sc.name}: {sc.n_elems} elements, {sc.n_nodes} nodes; warmup {args.warmup}, steps {args.steps}")
     opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=args.predictor,
-                use_graph=int(not args.no_graph), pcg_check_every=args.check_every)
+                use_graph=int(not args.no_graph), pcg_check_every=args.check_every, coarse_nodes=args.coarse_nodes)
     r, q_mixed, elapsed, agg = run_gpu(sc, native.MIXED, args.steps, args.warmup, profile=True, **opts)
     nnz = int(len(r.pattern(sc.dt)[1]))
     ms_per_step = 1e3 * elapsed / args.steps
@@ -401,7 +403,7 @@ def main():
     # compulsory bytes of one SpMV launch for the layout in use (values once,
     # one column index per nonzero 3x3 node block, x and y once)
     iso_ms, spmv_bytes = r.bench_kernel(native.KERNEL_SPMV, reps=20)
-    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
     r.close()
     out = {
         "metric": METRIC,

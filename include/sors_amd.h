@@ -89,6 +89,10 @@ typedef struct {
                              taken since the state was set; Dirichlet DOFs start at
                              qPrev + dt*value.  The converged state is the same
                              minimiser */
+    int coarse_nodes;     /* > 0: two-level PCG preconditioner, block-Jacobi plus a
+                             coarse correction on the rigid-body modes of spatial
+                             node aggregates of about this many nodes (sliced
+                             node-block layout only); 0: block-Jacobi alone */
 } sors_options;
 
 typedef struct {

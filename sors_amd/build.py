@@ -47,13 +47,18 @@ def _run(cmd):
 
 def build_lib() -> str:
     os.makedirs(BUILD, exist_ok=True)
-    pat_o = os.path.join(BUILD, "pattern.o")
-    _run([CXX, "-std=c++17", "-O2", "-fPIC", "-pthread", "-I", CSRC, "-c", os.path.join(CSRC, "pattern.cpp"), "-o",
-          pat_o])
+    host_o = []
+    for name in ("pattern", "coarse"):
+        o = os.path.join(BUILD, name + ".o")
+        _run([CXX, "-std=c++17", "-O2", "-fPIC", "-pthread", "-I", CSRC, "-c", os.path.join(CSRC, name + ".cpp"),
+              "-o", o])
+        host_o.append(o)
     hip_o = os.path.join(BUILD, "sors_amd.o")
     _run([HIPCC, *HIP_FLAGS, "-I", INC, "-I", CSRC, "-c", os.path.join(CSRC, "sors_amd.hip"), "-o", hip_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_o, pat_o, "-pthread"])
+    # rocSOLVER (Cholesky of the coarse operator) and rocBLAS (its handle)
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_o, *host_o, "-pthread",
+          "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -15,6 +15,8 @@
 // with SPD-projected element tangents.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -32,6 +34,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "coarse.h"
 #include "kernels.h"
 #include "nh_math.h"
 #include "pattern.h"
@@ -46,6 +49,7 @@ namespace sors {
 // one-block finalize) sums the partials in a fixed order.
 // ===========================================================================
 constexpr int kMaxWaves = 1024 / 64;  // largest workgroup on gfx950
+// fixed-order sum over the workgroup; the totals are left in thread 0 only
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     __shared__ double sh[NV][kMaxWaves];
@@ -1358,6 +1362,7 @@ __global__ __launch_bounds__(kBlock) void k_group_sum(int E, const int* group, c
     if (threadIdx.x == 0) out[gsel] = s[0];
 }
 
+
 struct PcgPart {
     double* u[2];   // 2 slots each: gamma, rr
     double* a;      // 1 slot: delta
@@ -1386,6 +1391,8 @@ struct PartIO {
     int nbU;
     const double* inA;  // consumed delta partials
     int nbA;
+    const double* inC = nullptr;  // coarse part of gamma (two-level preconditioner) or null
+    int nbC = 0;
 };
 
 template <int NV>
@@ -1445,10 +1452,16 @@ __device__ void pcg_init_body(const PcgView<HT>& v, const double* b, double sign
 // SpMV prologue: returns true when the solve is over (converged, breakdown,
 // or a zero right-hand side).  Block 0 of the rollout records the outcome.
 __device__ __forceinline__ bool spmv_prologue(PcgCtl* ctl, PcgState* st, const double* partU, int stride, int nbU,
-                                              int first, double rtol, bool lead) {
+                                              int first, double rtol, bool lead, const double* partC = nullptr,
+                                              int nbC = 0) {
     if (ctl->done) return true;
     double t[2];
     sum_partials_at<2>(partU, stride, nbU, t);  // gamma_k, rr_k
+    if (partC) {
+        double gc[1];
+        sum_partials_at<1>(partC, stride, nbC, gc);
+        t[0] += gc[0];
+    }
     if (first) {
         const bool zero = !(t[1] > 0.0) || !(t[0] > 0.0);
         if (lead) {
@@ -1484,7 +1497,7 @@ __device__ __forceinline__ bool spmv_prologue(PcgCtl* ctl, PcgState* st, const d
 template <typename HT>
 __device__ void pcg_spmv_body(const PcgView<HT>& v, PcgState* st, PartIO io, int first, int lb, int nb) {
     const bool lead = lb == 0 && threadIdx.x == 0;
-    if (spmv_prologue(v.ctl, st, io.inU, io.stride, io.nbU, first, v.rtol, lead)) return;
+    if (spmv_prologue(v.ctl, st, io.inU, io.stride, io.nbU, first, v.rtol, lead, io.inC, io.nbC)) return;
     double acc[1] = {0.0};
     const int lane = threadIdx.x & (kSlice - 1), wv = threadIdx.x / kSlice;
     constexpr int kPer = kBlock / kSlice;  // slices per block
@@ -1565,6 +1578,11 @@ __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgStat
     double g[2], d[1];
     sum_partials_at<2>(io.inU, io.stride, io.nbU, g);  // gamma_{k-1}, rr_{k-1}
     sum_partials_at<1>(io.inA, io.stride, io.nbA, d);  // delta_{k-1}
+    if (io.inC) {
+        double gc[1];
+        sum_partials_at<1>(io.inC, io.stride, io.nbC, gc);  // coarse part of gamma_{k-1}
+        g[0] += gc[0];
+    }
     double alpha, beta;
     bool bad;
     if (first) {
@@ -1632,6 +1650,179 @@ __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgStat
     block_partial_at<2>(acc, io.out, io.stride, lb);
 }
 
+// ===========================================================================
+// Two-level preconditioner: M^-1 = D^-1 + Z E^-1 Z^T with Z the rigid-body
+// modes of spatial node aggregates (coarse.h) and E = Z^T A Z dense.
+// Z_i [t; w] = t + w x d_i,  Z_i^T r_i = [r_i; d_i x r_i],  d_i = (x_i - c_I)/s_I
+// ===========================================================================
+struct CoarseArgs {
+    int nagg, nc;
+    const int* agg;       // [N] aggregate or -1
+    const int* aptr;      // aggregate -> nodes
+    const int* anodes;
+    const double* cent;   // [nagg][3]
+    const double* scale;  // [nagg]
+    const int* rot;       // [nagg]
+    const double* rest;   // [3N]
+};
+
+__device__ __forceinline__ void coarse_d(const CoarseArgs& c, int i, int I, double d[3]) {
+    const double s = 1.0 / c.scale[I];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] = (c.rest[3 * i + k] - c.cent[3 * I + k]) * s;
+}
+
+// E(I, J) block = sum over the pair's fine blocks of Z_i^T A_ij Z_j; one
+// wavefront per aggregate pair, lanes over its contributions, fixed-order
+// wavefront reduction (deterministic)
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_coarse_galerkin(CoarseArgs c, int npairs, const int* pairI,
+                                                            const int* pairJ, const int* pptr, const int* ci,
+                                                            const int* cj, const int* cbase, const HT* vals,
+                                                            double* E) {
+    const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (p >= npairs) return;  // whole wavefronts
+    const int I = pairI[p], J = pairJ[p];
+    const bool rI = c.rot[I], rJ = c.rot[J];
+    double acc[36];
+#pragma unroll
+    for (int k = 0; k < 36; ++k) acc[k] = 0.0;
+    for (int t = pptr[p] + lane; t < pptr[p + 1]; t += 64) {
+        const int i = ci[t], j = cj[t];
+        const long long base = cbase[t];
+        double A[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) A[a][b] = (double)vals[base + (long long)kSlice * (3 * a + b)];
+        double di[3], dj[3];
+        coarse_d(c, i, I, di);
+        coarse_d(c, j, J, dj);
+        // T = A Z_j (3 x 6): columns 0-2 = A, columns 3+k = A (e_k x d_j)
+        double T[3][6];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T[a][0] = A[a][0];
+            T[a][1] = A[a][1];
+            T[a][2] = A[a][2];
+            // e_x x d = (0, -d_z, d_y), e_y x d = (d_z, 0, -d_x), e_z x d = (-d_y, d_x, 0)
+            T[a][3] = rJ ? (-A[a][1] * dj[2] + A[a][2] * dj[1]) : 0.0;
+            T[a][4] = rJ ? (A[a][0] * dj[2] - A[a][2] * dj[0]) : 0.0;
+            T[a][5] = rJ ? (-A[a][0] * dj[1] + A[a][1] * dj[0]) : 0.0;
+        }
+        // Z_i^T T: rows 0-2 = T, row 3+k = (e_k x d_i) . T[:, col]
+#pragma unroll
+        for (int col = 0; col < 6; ++col) {
+            acc[0 * 6 + col] += T[0][col];
+            acc[1 * 6 + col] += T[1][col];
+            acc[2 * 6 + col] += T[2][col];
+            if (rI) {
+                acc[3 * 6 + col] += -di[2] * T[1][col] + di[1] * T[2][col];
+                acc[4 * 6 + col] += di[2] * T[0][col] - di[0] * T[2][col];
+                acc[5 * 6 + col] += -di[1] * T[0][col] + di[0] * T[1][col];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 36; ++k) {
+        double x = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        acc[k] = x;
+    }
+    if (lane < 36) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < 36; ++k)
+            if (k == lane) v = acc[k];
+        const int r = lane / 6, col = lane % 6;
+        E[(size_t)(6 * I + r) * c.nc + 6 * J + col] = v;
+    }
+}
+
+// unknowns of aggregates without rotations: unit diagonal (decoupled)
+__global__ void k_coarse_fixnull(CoarseArgs c, double* E) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= c.nagg || c.rot[I]) return;
+    for (int r = 3; r < 6; ++r) E[(size_t)(6 * I + r) * c.nc + 6 * I + r] = 1.0;
+}
+
+// inverse from the column-major lower triangle (potri) into a full row-major
+// matrix.  The coarse solve stays in FP64 at every precision: E is as badly
+// conditioned as the coarse problem, and an FP32 copy of its inverse need not
+// be positive definite.
+__global__ void k_coarse_symcopy(int nc, const double* L, double* Einv) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nc * nc) return;
+    const int a = (int)(t / nc), b = (int)(t % nc);
+    const double v = a >= b ? L[(size_t)b * nc + a] : L[(size_t)a * nc + b];
+    Einv[t] = v;
+}
+
+// rc = Z^T r: one block per aggregate (fixed-order block sums)
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_coarse_restrict(CoarseArgs c, const PcgCtl* ctl, const HT* r,
+                                                            double* rc) {
+    if (ctl && ctl->done) return;
+    const int I = blockIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int t = c.aptr[I] + threadIdx.x; t < c.aptr[I + 1]; t += blockDim.x) {
+        const int i = c.anodes[t];
+        double d[3];
+        coarse_d(c, i, I, d);
+        const double r0 = (double)r[3 * i], r1 = (double)r[3 * i + 1], r2 = (double)r[3 * i + 2];
+        acc[0] += r0;
+        acc[1] += r1;
+        acc[2] += r2;
+        // (e_k x d) . r = e_k . (d x r)
+        acc[3] += d[1] * r2 - d[2] * r1;
+        acc[4] += d[2] * r0 - d[0] * r2;
+        acc[5] += d[0] * r1 - d[1] * r0;
+    }
+    block_sum<6>(acc);  // block totals in thread 0
+    if (threadIdx.x == 0) {
+        const bool rt = c.rot[I] != 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) rc[6 * I + k] = (k < 3 || rt) ? acc[k] : 0.0;
+    }
+}
+
+// xc = E^-1 rc (one wavefront per row) and partials of gamma_c = rc . xc
+__global__ __launch_bounds__(kBlock) void k_coarse_solve(int nc, const PcgCtl* ctl, const double* Einv,
+                                                         const double* rc, double* xc, double* part) {
+    if (ctl && ctl->done) return;
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    double acc = 0.0;
+    if (row < nc) {
+        const double* Er = Einv + (size_t)row * nc;
+        for (int col = lane; col < nc; col += 64) acc += Er[col] * rc[col];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) xc[row] = acc;
+    }
+    double g[1] = {(row < nc && lane == 0) ? acc * rc[row] : 0.0};
+    block_partial_at<1>(g, part, kMaxRedBlocks, blockIdx.x);
+}
+
+// u += Z xc
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_coarse_prolong(CoarseArgs c, int N, const PcgCtl* ctl, const double* xc,
+                                                           HT* u) {
+    if (ctl && ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int I = c.agg[i];
+    if (I < 0) return;
+    double d[3];
+    coarse_d(c, i, I, d);
+    const double t0 = xc[6 * I], t1 = xc[6 * I + 1], t2 = xc[6 * I + 2];
+    const double w0 = xc[6 * I + 3], w1 = xc[6 * I + 4], w2 = xc[6 * I + 5];
+    u[3 * i] += (HT)(t0 + w1 * d[2] - w2 * d[1]);
+    u[3 * i + 1] += (HT)(t1 + w2 * d[0] - w0 * d[2]);
+    u[3 * i + 2] += (HT)(t2 + w0 * d[1] - w1 * d[0]);
+}
+
 // ---- single-rollout kernels ----------------------------------------------
 template <typename HT>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgView<HT> v, const double* b, double sign, PartIO io) {
@@ -1654,7 +1845,9 @@ template <typename HT, int LPR>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* rowptr, const int* cols, const HT* vals,
                                                      const HT* u, HT* w, PcgCtl* ctl, PcgState* st, PartIO io,
                                                      int first, double rtol) {
-    if (spmv_prologue(ctl, st, io.inU, io.stride, io.nbU, first, rtol, blockIdx.x == 0 && threadIdx.x == 0)) return;
+    if (spmv_prologue(ctl, st, io.inU, io.stride, io.nbU, first, rtol, blockIdx.x == 0 && threadIdx.x == 0, io.inC,
+                      io.nbC))
+        return;
     double v[1] = {0.0};
     const int sub = threadIdx.x % LPR;
     const int rows_per_block = blockDim.x / LPR;
@@ -1865,6 +2058,7 @@ struct sors_ctx {
     // host copies needed for patterns / exports
     std::vector<int> h_conn;   // [E][ed]
     std::vector<int> h_mask;   // [3N]
+    std::vector<double> h_rest;  // [3N] rest positions (coarse-space aggregation)
     // device buffers
     DevMem conn, rest, mu, lam, mass, mask, bcval;
     DevMem q, qprev, vprev, gprev, gtmp, gint, g, fext, dxd, qtmp;
@@ -1885,6 +2079,15 @@ struct sors_ctx {
     DevMem sbase, bcol, selfslot;   // sliced node-block layout (node-granular masks)
     DevMem incptr, inc;             // node -> (element, local node) incidence (node-centric assembly)
     DevMem vold;                    // velocity before the last step (second-order predictor)
+    // two-level preconditioner (options.coarse_nodes > 0)
+    int coarse_built_for = 0;       // coarse_nodes of the built space (0: none)
+    int coarse_gen = 0;             // bumped on every rebuild (captured graphs hold its buffers)
+    const void* coarse_pat = nullptr;
+    bool coarse_ready = false;      // E^-1 valid for the current matrix
+    int nagg = 0, nc = 0, npairs = 0;
+    DevMem cagg, captr, canodes, ccent, cscale, crot, cpairI, cpairJ, cpptr, cci, ccj, ccbase;
+    DevMem cE, cEinv, crc, cxc, cpart, cinfo;
+    rocblas_handle blas = nullptr;
     bool vold_valid = false;
     double vold_dt = 0.0;
     // active fibre stress (sors_set_active) and parameter adjoints
@@ -1902,6 +2105,7 @@ struct sors_ctx {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     const void* gkey[2] = {nullptr, nullptr};
     int gchunk[2] = {0, 0};
+    int gcoarse[2] = {0, 0};
     bool block() const { return pat.node_granular; }
     // per-launch kernel timing (options.profile_kernels)
     sors_stats* prof = nullptr;
@@ -2133,6 +2337,105 @@ void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project
     HIPX(hipGetLastError());
 }
 
+// ---- two-level preconditioner: set-up ---------------------------------------
+CoarseArgs coarse_args(sors_ctx* c) {
+    CoarseArgs a;
+    a.nagg = c->nagg;
+    a.nc = c->nc;
+    a.agg = c->cagg.as<int>();
+    a.aptr = c->captr.as<int>();
+    a.anodes = c->canodes.as<int>();
+    a.cent = c->ccent.as<double>();
+    a.scale = c->cscale.as<double>();
+    a.rot = c->crot.as<int>();
+    a.rest = c->rest.as<double>();
+    return a;
+}
+
+// largest coarse system (6 unknowns per aggregate) handled with a dense inverse
+// (nc^2 FP64 values are read once per PCG iteration)
+constexpr int kMaxCoarse = 6 * 512;
+
+// (re)build the coarse space for the current pattern and options
+void ensure_coarse(sors_ctx* c) {
+    const int want = (c->opt.coarse_nodes > 0 && c->block()) ? c->opt.coarse_nodes : 0;
+    if (want == c->coarse_built_for && (want == 0 || c->coarse_pat == c->vals.p)) return;
+    c->coarse_built_for = 0;
+    c->coarse_ready = false;
+    if (want == 0) return;
+    CoarseSpace C;
+    std::string msg;
+    int target = want;
+    for (;;) {
+        if (!build_coarse(c->N, c->h_rest.data(), c->has_mask ? c->h_mask.data() : nullptr, c->pat, target, C, &msg))
+            throw HipError{"coarse space: " + msg};
+        if (6 * C.nagg <= kMaxCoarse) break;
+        target *= 2;  // keep the dense coarse inverse bounded
+    }
+    c->nagg = C.nagg;
+    c->nc = 6 * C.nagg;
+    c->npairs = (int)C.pairI.size();
+    hipStream_t st = c->stream;
+    upload(c->cagg, C.agg.data(), C.agg.size(), st);
+    upload(c->captr, C.aptr.data(), C.aptr.size(), st);
+    upload(c->canodes, C.anodes.data(), std::max<size_t>(C.anodes.size(), 1), st);
+    upload(c->ccent, C.cent.data(), std::max<size_t>(C.cent.size(), 1), st);
+    upload(c->cscale, C.scale.data(), std::max<size_t>(C.scale.size(), 1), st);
+    upload(c->crot, C.rot.data(), std::max<size_t>(C.rot.size(), 1), st);
+    upload(c->cpairI, C.pairI.data(), std::max<size_t>(C.pairI.size(), 1), st);
+    upload(c->cpairJ, C.pairJ.data(), std::max<size_t>(C.pairJ.size(), 1), st);
+    upload(c->cpptr, C.pptr.data(), C.pptr.size(), st);
+    upload(c->cci, C.ci.data(), std::max<size_t>(C.ci.size(), 1), st);
+    upload(c->ccj, C.cj.data(), std::max<size_t>(C.cj.size(), 1), st);
+    upload(c->ccbase, C.cbase.data(), std::max<size_t>(C.cbase.size(), 1), st);
+    const size_t nc2 = (size_t)std::max(c->nc, 1) * std::max(c->nc, 1);
+    ensure(c->cE, sizeof(double) * nc2);
+    ensure(c->cEinv, sizeof(double) * nc2);
+    ensure(c->crc, sizeof(double) * std::max(c->nc, 1));
+    ensure(c->cxc, sizeof(double) * std::max(c->nc, 1));
+    ensure(c->cpart, sizeof(double) * 2 * kMaxRedBlocks);
+    ensure(c->cinfo, sizeof(int));
+    if (!c->blas) {
+        if (rocblas_create_handle(&c->blas) != rocblas_status_success) throw HipError{"rocblas_create_handle failed"};
+    }
+    if (rocblas_set_stream(c->blas, st) != rocblas_status_success) throw HipError{"rocblas_set_stream failed"};
+    sync(c);
+    c->coarse_built_for = want;
+    c->coarse_pat = c->vals.p;
+    ++c->coarse_gen;
+}
+
+// E = Z^T A Z, its inverse (rocSOLVER Cholesky), as HT; coarse_ready on success
+template <typename HT>
+void coarse_setup(sors_ctx* c) {
+    ensure_coarse(c);
+    c->coarse_ready = false;
+    if (c->coarse_built_for == 0 || c->nc == 0) return;
+    const int nc = c->nc;
+    const size_t nc2 = (size_t)nc * nc;
+    fill<double>(c->cE.as<double>(), (long long)nc2, 0.0, c->stream);
+    const CoarseArgs ca = coarse_args(c);
+    if (c->npairs > 0)
+        k_coarse_galerkin<HT><<<(c->npairs * 64 + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
+            ca, c->npairs, c->cpairI.as<int>(), c->cpairJ.as<int>(), c->cpptr.as<int>(), c->cci.as<int>(),
+            c->ccj.as<int>(), c->ccbase.as<int>(), (const HT*)c->vals.p, c->cE.as<double>());
+    k_coarse_fixnull<<<grid_for(c->nagg), kBlock, 0, c->stream>>>(ca, c->cE.as<double>());
+    HIPX(hipGetLastError());
+    int* info = c->cinfo.as<int>();
+    if (rocsolver_dpotrf(c->blas, rocblas_fill_lower, nc, c->cE.as<double>(), nc, info) != rocblas_status_success)
+        throw HipError{"rocsolver_dpotrf failed"};
+    int h_info = 0;
+    HIPX(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    if (h_info != 0) return;  // not positive definite: block-Jacobi alone for this solve
+    if (rocsolver_dpotri(c->blas, rocblas_fill_lower, nc, c->cE.as<double>(), nc, info) != rocblas_status_success)
+        throw HipError{"rocsolver_dpotri failed"};
+    k_coarse_symcopy<<<(unsigned)((nc2 + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(nc, c->cE.as<double>(),
+                                                                                      c->cEinv.as<double>());
+    HIPX(hipGetLastError());
+    c->coarse_ready = true;
+}
+
 template <typename HT>
 void build_preconditioner(sors_ctx* c) {
     if (c->block())
@@ -2143,6 +2446,7 @@ void build_preconditioner(sors_ctx* c) {
                                                                       c->diag.as<int>(), (const HT*)c->vals.p,
                                                                       (HT*)c->Minv.p);
     HIPX(hipGetLastError());
+    coarse_setup<HT>(c);
 }
 
 // launch geometry of the PCG kernels
@@ -2199,9 +2503,9 @@ PcgView<HT> pcg_view(sors_ctx* c, double rtol) {
 // events around this dispatch itself)
 template <typename HT>
 void launch_spmv(sors_ctx* c, const PcgGrid& g, const PcgView<HT>& v, int par, int nbU, int first,
-                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const double* inC = nullptr, int nbC = 0) {
     const PcgPart part = c->pcg_part();
-    const PartIO io{part.a, kMaxRedBlocks, part.u[par], nbU, nullptr, 0};
+    const PartIO io{part.a, kMaxRedBlocks, part.u[par], nbU, nullptr, 0, inC, nbC};
     PcgState* st = v.st + par;
     if (c->block() && e0)
         hipExtLaunchKernelGGL(k_pcg_spmv_blk<HT>, dim3(g.gs), dim3(kBlock), 0, c->stream, e0, e1, 0, v, st, io,
@@ -2221,10 +2525,22 @@ void launch_spmv(sors_ctx* c, const PcgGrid& g, const PcgView<HT>& v, int par, i
 // U_k reading partU[par ^ 1] (produced with nbU blocks) and partA; writes
 // partU[par] and st[par]
 template <typename HT>
-void launch_update(sors_ctx* c, const PcgGrid& g, const PcgView<HT>& v, int par, int nbU, int first) {
+void launch_update(sors_ctx* c, const PcgGrid& g, const PcgView<HT>& v, int par, int nbU, int first,
+                   const double* inC = nullptr, int nbC = 0) {
     const PcgPart part = c->pcg_part();
-    const PartIO io{part.u[par], kMaxRedBlocks, part.u[par ^ 1], nbU, part.a, g.gs};
+    const PartIO io{part.u[par], kMaxRedBlocks, part.u[par ^ 1], nbU, part.a, g.gs, inC, nbC};
     k_pcg_update<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(v, v.st + (par ^ 1), v.st + par, io, first);
+}
+
+// u += Z E^-1 Z^T r after an update (or the init); gamma's coarse partials into slot par
+template <typename HT>
+void launch_coarse(sors_ctx* c, PcgCtl* ctl, int par) {
+    const CoarseArgs ca = coarse_args(c);
+    k_coarse_restrict<HT><<<c->nagg, kBlock, 0, c->stream>>>(ca, ctl, (const HT*)c->pr.p, c->crc.as<double>());
+    k_coarse_solve<<<(c->nc * 64 + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
+        c->nc, ctl, c->cEinv.as<double>(), c->crc.as<double>(), c->cxc.as<double>(),
+        c->cpart.as<double>() + par * kMaxRedBlocks);
+    k_coarse_prolong<HT><<<grid_for(c->N), kBlock, 0, c->stream>>>(ca, c->N, ctl, c->cxc.as<double>(), (HT*)c->pu.p);
 }
 
 // PCG on the assembled matrix: solves A x = sign*b; x in c->px (HT).
@@ -2249,14 +2565,19 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
             c->prof->spmv_launches += 1;
         }
     };
+    // two-level preconditioner: a coarse correction after every update
+    const bool cs = c->coarse_ready;
+    const int nbC = cs ? (c->nc * 64 + kBlock - 1) / kBlock : 0;
+    auto inC = [&](int par) -> const double* { return cs ? c->cpart.as<double>() + par * kMaxRedBlocks : nullptr; };
     // iteration k >= 2 reads from st[(k-1)%2] and writes st[k%2]
     auto iteration = [&](int k, int j, bool timed) {
         const int par = k & 1;
-        launch_update<HT>(c, g, v, par, g.gu, 0);
+        launch_update<HT>(c, g, v, par, g.gu, 0, inC(par ^ 1), nbC);
+        if (cs) launch_coarse<HT>(c, ctl, par);
         if (timed)
-            launch_spmv<HT>(c, g, v, par, g.gu, 0, ev[2 * j], ev[2 * j + 1]);
+            launch_spmv<HT>(c, g, v, par, g.gu, 0, ev[2 * j], ev[2 * j + 1], inC(par), nbC);
         else
-            launch_spmv<HT>(c, g, v, par, g.gu, 0);
+            launch_spmv<HT>(c, g, v, par, g.gu, 0, nullptr, nullptr, inC(par), nbC);
     };
     PcgCtl* hctl = (PcgCtl*)c->h_pin;
     fill<int>((int*)ctl, sizeof(PcgCtl) / sizeof(int), 0, c->stream);
@@ -2265,9 +2586,11 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
         const PartIO io{part.u[0], kMaxRedBlocks, nullptr, 0, nullptr, 0};
         k_pcg_init<HT><<<g.gn, kBlock, 0, c->stream>>>(v, b, sign, io);
     }
-    launch_spmv<HT>(c, g, v, 0, g.gn, 1);      // A_0
-    launch_update<HT>(c, g, v, 1, g.gn, 1);    // U_1
-    launch_spmv<HT>(c, g, v, 1, g.gu, 0);      // A_1
+    if (cs) launch_coarse<HT>(c, ctl, 0);
+    launch_spmv<HT>(c, g, v, 0, g.gn, 1, nullptr, nullptr, inC(0), nbC);   // A_0
+    launch_update<HT>(c, g, v, 1, g.gn, 1, inC(0), nbC);                    // U_1
+    if (cs) launch_coarse<HT>(c, ctl, 1);
+    launch_spmv<HT>(c, g, v, 1, g.gu, 0, nullptr, nullptr, inC(1), nbC);   // A_1
     HIPX(hipGetLastError());
     int launched = 1;
     bool first_chunk = true, done = false;
@@ -2275,7 +2598,8 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
         const int k = std::min(chunk, max_iter - launched);
         const bool timed = ev && first_chunk;
         if (!timed && c->opt.use_graph && k == chunk) {
-            if (!c->gexec[gi] || c->gkey[gi] != c->vals.p || c->gchunk[gi] != chunk) {
+            if (!c->gexec[gi] || c->gkey[gi] != c->vals.p || c->gchunk[gi] != chunk ||
+                c->gcoarse[gi] != (cs ? c->coarse_gen : 0)) {
                 if (c->gexec[gi]) HIPX(hipGraphExecDestroy(c->gexec[gi]));
                 c->gexec[gi] = nullptr;
                 hipGraph_t graph = nullptr;
@@ -2286,6 +2610,7 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
                 HIPX(hipGraphDestroy(graph));
                 c->gkey[gi] = c->vals.p;
                 c->gchunk[gi] = chunk;
+                c->gcoarse[gi] = cs ? c->coarse_gen : 0;
             }
             HIPX(hipGraphLaunch(c->gexec[gi], c->stream));
         } else {
@@ -2583,7 +2908,9 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->pw,     &c->px,    &c->partials, &c->scal, &c->pcgst,   &c->pcgctl,  &c->pcgpart,
                            &c->flag,   &c->sbase, &c->bcol,    &c->selfslot, &c->csurf, &c->lagN,
                            &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
-                           &c->incptr, &c->inc,  &c->vold,
+                           &c->incptr, &c->inc,  &c->vold,     &c->cagg,   &c->captr,   &c->canodes, &c->ccent,
+                           &c->cscale, &c->crot, &c->cpairI,   &c->cpairJ, &c->cpptr,   &c->cci,     &c->ccj,
+                           &c->ccbase, &c->cE,   &c->cEinv,    &c->crc,    &c->cxc,     &c->cpart,   &c->cinfo,
                            &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA};
     double b = 0;
     for (const DevMem* d : all) b += (double)d->bytes;
@@ -2660,6 +2987,7 @@ void srv_try_admit(sors_server* s) {  // mu held
             std::string msg = rc ? sors_last_error(nullptr) : "";
             if (rc == SORS_OK) {
                 c->opt = s->opt.solver;
+                c->opt.coarse_nodes = 0;  // the batched PCG uses block-Jacobi
                 try {
                     if (!r->v0.empty()) upload(c->vprev, r->v0.data(), (size_t)c->n, c->stream);
                     ensure_pattern(c, r->dt);
@@ -3064,6 +3392,7 @@ void sors_default_options(sors_options* o) {
     o->use_graph = 1;
     o->profile_kernels = 0;
     o->predict = 2;
+    o->coarse_nodes = 0;
 }
 
 int sors_create(const sors_system_desc* d, sors_ctx** out) {
@@ -3113,6 +3442,7 @@ int sors_create(const sors_system_desc* d, sors_ctx** out) {
         HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         const int N = c->N, E = c->E, ed = c->ed, n = c->n;
         c->h_conn.assign(d->conn, d->conn + (size_t)E * ed);
+        c->h_rest.assign(d->rest, d->rest + (size_t)n);
         for (size_t i = 0; i < c->h_conn.size(); ++i)
             if (c->h_conn[i] < 0 || c->h_conn[i] >= N) {
                 g_create_error = "element index out of range";
@@ -3214,6 +3544,7 @@ void sors_destroy(sors_ctx* c) {
     for (hipEvent_t e : c->pev) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; ++k)
         if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+    if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
     delete c;

@@ -46,6 +46,7 @@ class Options(C.Structure):
         ("precision", C.c_int), ("project_spd", C.c_int), ("max_newton", C.c_int), ("dx_tol", C.c_double),
         ("g_tol", C.c_double), ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int), ("max_backtrack", C.c_int),
         ("pcg_check_every", C.c_int), ("use_graph", C.c_int), ("profile_kernels", C.c_int), ("predict", C.c_int),
+        ("coarse_nodes", C.c_int),
     ]
 
 
