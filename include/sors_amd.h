@@ -93,6 +93,9 @@ typedef struct {
                              coarse correction on the rigid-body modes of spatial
                              node aggregates of about this many nodes (sliced
                              node-block layout only); 0: block-Jacobi alone */
+    int coarse_every;     /* coarse operator refreshed on every k-th tangent
+                             (<= 1: every tangent); in between, PCG uses the
+                             last one, an older tangent's Galerkin product */
 } sors_options;
 
 typedef struct {

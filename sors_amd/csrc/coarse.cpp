@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 namespace sors {
@@ -28,7 +29,12 @@ bool build_coarse(int N, const double* rest, const int* mask, const HostPattern&
     C.agg.assign(N, -1);
     if (nfree == 0) {
         C.pptr.assign(1, 0);
+        C.pchunk.assign(1, 0);
+        C.gptr.assign(1, 0);
         C.aptr.assign(1, 0);
+        C.dn.assign(4 * (size_t)N, 0.0f);
+        const int none = -1;
+        for (int v = 0; v < N; ++v) std::memcpy(&C.dn[4 * (size_t)v + 3], &none, sizeof(float));
         return true;
     }
     // bins of equal edge h with about `target` free nodes each
@@ -127,6 +133,8 @@ bool build_coarse(int N, const double* rest, const int* mask, const HostPattern&
         std::stable_sort(rows[a].begin(), rows[a].end(), [](const Item& x, const Item& y) { return x.J < y.J; });
     }
     C.pptr.assign(1, 0);
+    C.pchunk.assign(1, 0);
+    C.gptr.assign(1, 0);
     for (int a = 0; a < C.nagg; ++a) {
         const auto& R = rows[a];
         size_t t = 0;
@@ -139,8 +147,20 @@ bool build_coarse(int N, const double* rest, const int* mask, const HostPattern&
                 C.cj.push_back(R[t].j);
                 C.cbase.push_back(R[t].base);
             }
-            C.pptr.push_back((int)C.ci.size());
+            const int lo = C.pptr.back(), hi = (int)C.ci.size();
+            for (int c = lo; c < hi; c += kGalerkinChunk) C.gptr.push_back(std::min(hi, c + kGalerkinChunk));
+            C.pptr.push_back(hi);
+            C.pchunk.push_back((int)C.gptr.size() - 1);
         }
+    }
+    // per-node rigid-body data
+    C.dn.assign(4 * (size_t)N, 0.0f);
+    for (int v = 0; v < N; ++v) {
+        const int a = C.agg[v];
+        std::memcpy(&C.dn[4 * (size_t)v + 3], &a, sizeof(float));
+        if (a < 0 || !C.rot[a]) continue;
+        for (int k = 0; k < 3; ++k)
+            C.dn[4 * (size_t)v + k] = (float)((rest[3 * v + k] - C.cent[3 * (size_t)a + k]) / C.scale[a]);
     }
     return true;
 }

@@ -27,12 +27,22 @@ struct CoarseSpace {
     std::vector<double> cent;      // [nagg][3] centroid of the rest positions
     std::vector<double> scale;     // [nagg] radius used to scale the rotations
     std::vector<int> rot;          // [nagg] 1: rotations kept, 0: translations only
-    // Galerkin product, grouped by aggregate pair
+    // [N][4] per node: d = (x - c)/s as FP32 (zero where the aggregate keeps
+    // no rotations), then the aggregate index bit-cast to float (-1: Dirichlet).
+    // The device applies Z from these values only, so restriction, prolongation
+    // and the Galerkin product use one and the same Z.
+    std::vector<float> dn;
+    // Galerkin product, grouped by aggregate pair, each pair's contributions
+    // cut into chunks of at most kGalerkinChunk
     std::vector<int> pairI, pairJ; // [npairs]
-    std::vector<int> pptr;         // npairs + 1
+    std::vector<int> pptr;         // npairs + 1, into the contributions
+    std::vector<int> pchunk;       // npairs + 1, into the chunks
+    std::vector<int> gptr;         // ngchunks + 1, into the contributions
     std::vector<int> ci, cj;       // [ncontrib] fine row node, fine column node
     std::vector<int> cbase;        // [ncontrib] value index of the block's (0,0) entry
 };
+
+constexpr int kGalerkinChunk = 256;   // fine blocks summed by one wavefront
 
 // target: nodes per aggregate (the bins are sized for it).  Needs the sliced
 // node-block layout (P.node_granular).

@@ -1657,59 +1657,48 @@ __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgStat
 // ===========================================================================
 struct CoarseArgs {
     int nagg, nc;
-    const int* agg;       // [N] aggregate or -1
-    const int* aptr;      // aggregate -> nodes
-    const int* anodes;
-    const double* cent;   // [nagg][3]
-    const double* scale;  // [nagg]
-    const int* rot;       // [nagg]
-    const double* rest;   // [3N]
+    const float4* dn;     // [N] (d_i, aggregate bits); d_i = 0 where rotations are dropped
+    const int* aptr;      // aggregate -> anodes
+    const int* anodes;    // nodes grouped by aggregate
+    double* rc;           // [nc] Z^T r
 };
 
-__device__ __forceinline__ void coarse_d(const CoarseArgs& c, int i, int I, double d[3]) {
-    const double s = 1.0 / c.scale[I];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) d[k] = (c.rest[3 * i + k] - c.cent[3 * I + k]) * s;
-}
+__device__ __forceinline__ int coarse_agg(float4 d) { return __float_as_int(d.w); }
 
-// E(I, J) block = sum over the pair's fine blocks of Z_i^T A_ij Z_j; one
-// wavefront per aggregate pair, lanes over its contributions, fixed-order
-// wavefront reduction (deterministic)
+// Galerkin product by chunks: chunk g of pair (I, J) sums Z_i^T A_ij Z_j over
+// at most 256 of the pair's fine blocks, one wavefront per chunk with a
+// fixed-order wavefront reduction; k_coarse_gather adds a pair's chunks in
+// order (deterministic).
 template <typename HT>
-__global__ __launch_bounds__(kBlock) void k_coarse_galerkin(CoarseArgs c, int npairs, const int* pairI,
-                                                            const int* pairJ, const int* pptr, const int* ci,
-                                                            const int* cj, const int* cbase, const HT* vals,
-                                                            double* E) {
-    const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+__global__ __launch_bounds__(kBlock) void k_coarse_galerkin(int nchunks, const int* gptr, const int* ci,
+                                                            const int* cj, const int* cbase, const float4* dn,
+                                                            const HT* vals, double* gpart) {
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (p >= npairs) return;  // whole wavefronts
-    const int I = pairI[p], J = pairJ[p];
-    const bool rI = c.rot[I], rJ = c.rot[J];
+    if (g >= nchunks) return;  // whole wavefronts
     double acc[36];
 #pragma unroll
     for (int k = 0; k < 36; ++k) acc[k] = 0.0;
-    for (int t = pptr[p] + lane; t < pptr[p + 1]; t += 64) {
-        const int i = ci[t], j = cj[t];
+    for (int t = gptr[g] + lane; t < gptr[g + 1]; t += 64) {
         const long long base = cbase[t];
         double A[3][3];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = 0; b < 3; ++b) A[a][b] = (double)vals[base + (long long)kSlice * (3 * a + b)];
-        double di[3], dj[3];
-        coarse_d(c, i, I, di);
-        coarse_d(c, j, J, dj);
-        // T = A Z_j (3 x 6): columns 0-2 = A, columns 3+k = A (e_k x d_j)
+        const float4 fi = dn[ci[t]], fj = dn[cj[t]];
+        const double di[3] = {fi.x, fi.y, fi.z}, dj[3] = {fj.x, fj.y, fj.z};
+        // T = A Z_j (3 x 6): columns 0-2 = A, column 3+k = A (e_k x d_j)
         double T[3][6];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            T[a][0] = A[a][0];
-            T[a][1] = A[a][1];
-            T[a][2] = A[a][2];
+        for (int r = 0; r < 3; ++r) {
+            T[r][0] = A[r][0];
+            T[r][1] = A[r][1];
+            T[r][2] = A[r][2];
             // e_x x d = (0, -d_z, d_y), e_y x d = (d_z, 0, -d_x), e_z x d = (-d_y, d_x, 0)
-            T[a][3] = rJ ? (-A[a][1] * dj[2] + A[a][2] * dj[1]) : 0.0;
-            T[a][4] = rJ ? (A[a][0] * dj[2] - A[a][2] * dj[0]) : 0.0;
-            T[a][5] = rJ ? (-A[a][0] * dj[1] + A[a][1] * dj[0]) : 0.0;
+            T[r][3] = -A[r][1] * dj[2] + A[r][2] * dj[1];
+            T[r][4] = A[r][0] * dj[2] - A[r][2] * dj[0];
+            T[r][5] = -A[r][0] * dj[1] + A[r][1] * dj[0];
         }
         // Z_i^T T: rows 0-2 = T, row 3+k = (e_k x d_i) . T[:, col]
 #pragma unroll
@@ -1717,11 +1706,9 @@ __global__ __launch_bounds__(kBlock) void k_coarse_galerkin(CoarseArgs c, int np
             acc[0 * 6 + col] += T[0][col];
             acc[1 * 6 + col] += T[1][col];
             acc[2 * 6 + col] += T[2][col];
-            if (rI) {
-                acc[3 * 6 + col] += -di[2] * T[1][col] + di[1] * T[2][col];
-                acc[4 * 6 + col] += di[2] * T[0][col] - di[0] * T[2][col];
-                acc[5 * 6 + col] += -di[1] * T[0][col] + di[0] * T[1][col];
-            }
+            acc[3 * 6 + col] += -di[2] * T[1][col] + di[1] * T[2][col];
+            acc[4 * 6 + col] += di[2] * T[0][col] - di[0] * T[2][col];
+            acc[5 * 6 + col] += -di[1] * T[0][col] + di[0] * T[1][col];
         }
     }
 #pragma unroll
@@ -1736,16 +1723,23 @@ __global__ __launch_bounds__(kBlock) void k_coarse_galerkin(CoarseArgs c, int np
 #pragma unroll
         for (int k = 0; k < 36; ++k)
             if (k == lane) v = acc[k];
-        const int r = lane / 6, col = lane % 6;
-        E[(size_t)(6 * I + r) * c.nc + 6 * J + col] = v;
+        gpart[36 * (size_t)g + lane] = v;
     }
 }
 
-// unknowns of aggregates without rotations: unit diagonal (decoupled)
-__global__ void k_coarse_fixnull(CoarseArgs c, double* E) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= c.nagg || c.rot[I]) return;
-    for (int r = 3; r < 6; ++r) E[(size_t)(6 * I + r) * c.nc + 6 * I + r] = 1.0;
+// E(I, J) block entries: one thread per (pair, entry), chunks in order.
+// Unknowns of aggregates without rotations get a unit diagonal (decoupled:
+// their Z columns are zero).
+__global__ void k_coarse_gather(int npairs, int nc, const int* pairI, const int* pairJ, const int* pchunk,
+                                const int* rot, const double* gpart, double* E) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 36LL * npairs) return;
+    const int p = (int)(t / 36), k = (int)(t % 36);
+    const int I = pairI[p], J = pairJ[p], r = k / 6, col = k % 6;
+    double v = 0.0;
+    for (int g = pchunk[p]; g < pchunk[p + 1]; ++g) v += gpart[36 * (size_t)g + k];
+    if (I == J && r == col && r >= 3 && !rot[I]) v = 1.0;
+    E[(size_t)(6 * I + r) * nc + 6 * J + col] = v;
 }
 
 // inverse from the column-major lower triangle (potri) into a full row-major
@@ -1760,49 +1754,82 @@ __global__ void k_coarse_symcopy(int nc, const double* L, double* Einv) {
     Einv[t] = v;
 }
 
-// rc = Z^T r: one block per aggregate (fixed-order block sums)
+// rc = Z^T r: one 1024-thread workgroup per aggregate, its nodes in a fixed
+// order, a fixed-order workgroup sum (deterministic)
+constexpr int kRestrictThreads = 1024;
 template <typename HT>
-__global__ __launch_bounds__(kBlock) void k_coarse_restrict(CoarseArgs c, const PcgCtl* ctl, const HT* r,
-                                                            double* rc) {
+__global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict(CoarseArgs c, const PcgCtl* ctl,
+                                                                      const HT* r) {
     if (ctl && ctl->done) return;
     const int I = blockIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int t = c.aptr[I] + threadIdx.x; t < c.aptr[I + 1]; t += blockDim.x) {
+    const int lo = c.aptr[I], hi = c.aptr[I + 1];
+    for (int t = lo + threadIdx.x; t < hi; t += kRestrictThreads) {
         const int i = c.anodes[t];
-        double d[3];
-        coarse_d(c, i, I, d);
+        const float4 f = c.dn[i];
+        const double d0 = f.x, d1 = f.y, d2 = f.z;
         const double r0 = (double)r[3 * i], r1 = (double)r[3 * i + 1], r2 = (double)r[3 * i + 2];
         acc[0] += r0;
         acc[1] += r1;
         acc[2] += r2;
         // (e_k x d) . r = e_k . (d x r)
-        acc[3] += d[1] * r2 - d[2] * r1;
-        acc[4] += d[2] * r0 - d[0] * r2;
-        acc[5] += d[0] * r1 - d[1] * r0;
+        acc[3] += d1 * r2 - d2 * r1;
+        acc[4] += d2 * r0 - d0 * r2;
+        acc[5] += d0 * r1 - d1 * r0;
     }
     block_sum<6>(acc);  // block totals in thread 0
-    if (threadIdx.x == 0) {
-        const bool rt = c.rot[I] != 0;
+    if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) rc[6 * I + k] = (k < 3 || rt) ? acc[k] : 0.0;
-    }
+        for (int k = 0; k < 6; ++k) c.rc[6 * (size_t)I + k] = acc[k];
 }
 
-// xc = E^-1 rc (one wavefront per row) and partials of gamma_c = rc . xc
-__global__ __launch_bounds__(kBlock) void k_coarse_solve(int nc, const PcgCtl* ctl, const double* Einv,
-                                                         const double* rc, double* xc, double* part) {
+// xc = E^-1 rc: two rows per 256-thread workgroup (128 lanes per row, eight
+// independent loads in flight per lane, fixed-order sums), and one partial of
+// gamma_c = rc . xc per workgroup
+constexpr int kMaxCoarse = 6 * 512;  // coarse unknowns: nc^2 FP64 values are read per PCG iteration
+constexpr int kSolveRowThreads = 128;
+static_assert(kMaxCoarse / 2 <= kMaxRedBlocks, "one gamma partial per coarse-solve workgroup");
+__global__ __launch_bounds__(kBlock) void k_coarse_solve(CoarseArgs c, const PcgCtl* ctl,
+                                                         const double* __restrict__ Einv, double* xc, double* part) {
     if (ctl && ctl->done) return;
-    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-    double acc = 0.0;
+    __shared__ double sh[kBlock / 64];
+    const int nc = c.nc;
+    const double* __restrict__ rc = c.rc;
+    const int half = threadIdx.x / kSolveRowThreads, t = threadIdx.x % kSolveRowThreads;
+    const int row = 2 * blockIdx.x + half;
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (row < nc) {
-        const double* Er = Einv + (size_t)row * nc;
-        for (int col = lane; col < nc; col += 64) acc += Er[col] * rc[col];
+        const double* __restrict__ Er = Einv + (size_t)row * nc;
+        int col = t;
+        for (; col + 7 * kSolveRowThreads < nc; col += 8 * kSolveRowThreads) {
+            double e[8], x[8];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if (lane == 0) xc[row] = acc;
+            for (int j = 0; j < 8; ++j) {
+                e[j] = Er[col + j * kSolveRowThreads];
+                x[j] = rc[col + j * kSolveRowThreads];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[j] += e[j] * x[j];
+        }
+        for (; col < nc; col += kSolveRowThreads) a[0] += Er[col] * rc[col];  // fewer than 8 left
     }
-    double g[1] = {(row < nc && lane == 0) ? acc * rc[row] : 0.0};
-    block_partial_at<1>(g, part, kMaxRedBlocks, blockIdx.x);
+    double v = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double g = 0.0;
+        for (int h = 0; h < 2; ++h) {
+            const int r = 2 * blockIdx.x + h;
+            if (r < nc) {
+                const double tot = sh[2 * h] + sh[2 * h + 1];
+                xc[r] = tot;
+                g += tot * rc[r];
+            }
+        }
+        part[blockIdx.x] = g;
+    }
 }
 
 // u += Z xc
@@ -1812,15 +1839,15 @@ __global__ __launch_bounds__(kBlock) void k_coarse_prolong(CoarseArgs c, int N, 
     if (ctl && ctl->done) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const int I = c.agg[i];
+    const float4 f = c.dn[i];
+    const int I = coarse_agg(f);
     if (I < 0) return;
-    double d[3];
-    coarse_d(c, i, I, d);
+    const double d0 = f.x, d1 = f.y, d2 = f.z;
     const double t0 = xc[6 * I], t1 = xc[6 * I + 1], t2 = xc[6 * I + 2];
     const double w0 = xc[6 * I + 3], w1 = xc[6 * I + 4], w2 = xc[6 * I + 5];
-    u[3 * i] += (HT)(t0 + w1 * d[2] - w2 * d[1]);
-    u[3 * i + 1] += (HT)(t1 + w2 * d[0] - w0 * d[2]);
-    u[3 * i + 2] += (HT)(t2 + w0 * d[1] - w1 * d[0]);
+    u[3 * i] += (HT)(t0 + w1 * d2 - w2 * d1);
+    u[3 * i + 1] += (HT)(t1 + w2 * d0 - w0 * d2);
+    u[3 * i + 2] += (HT)(t2 + w0 * d1 - w1 * d0);
 }
 
 // ---- single-rollout kernels ----------------------------------------------
@@ -2082,11 +2109,12 @@ struct sors_ctx {
     // two-level preconditioner (options.coarse_nodes > 0)
     int coarse_built_for = 0;       // coarse_nodes of the built space (0: none)
     int coarse_gen = 0;             // bumped on every rebuild (captured graphs hold its buffers)
+    long long coarse_tangents = 0;  // tangents seen since the coarse space was built
     const void* coarse_pat = nullptr;
     bool coarse_ready = false;      // E^-1 valid for the current matrix
-    int nagg = 0, nc = 0, npairs = 0;
-    DevMem cagg, captr, canodes, ccent, cscale, crot, cpairI, cpairJ, cpptr, cci, ccj, ccbase;
-    DevMem cE, cEinv, crc, cxc, cpart, cinfo;
+    int nagg = 0, nc = 0, npairs = 0, ngchunks = 0;
+    DevMem cdn, captr, canodes, crot, cpairI, cpairJ, cpchunk, cgptr, cci, ccj, ccbase;
+    DevMem cE, cEinv, crc, cxc, cgpart, cpart, cinfo;
     rocblas_handle blas = nullptr;
     bool vold_valid = false;
     double vold_dt = 0.0;
@@ -2342,19 +2370,12 @@ CoarseArgs coarse_args(sors_ctx* c) {
     CoarseArgs a;
     a.nagg = c->nagg;
     a.nc = c->nc;
-    a.agg = c->cagg.as<int>();
+    a.dn = (const float4*)c->cdn.p;
     a.aptr = c->captr.as<int>();
     a.anodes = c->canodes.as<int>();
-    a.cent = c->ccent.as<double>();
-    a.scale = c->cscale.as<double>();
-    a.rot = c->crot.as<int>();
-    a.rest = c->rest.as<double>();
+    a.rc = c->crc.as<double>();
     return a;
 }
-
-// largest coarse system (6 unknowns per aggregate) handled with a dense inverse
-// (nc^2 FP64 values are read once per PCG iteration)
-constexpr int kMaxCoarse = 6 * 512;
 
 // (re)build the coarse space for the current pattern and options
 void ensure_coarse(sors_ctx* c) {
@@ -2375,24 +2396,26 @@ void ensure_coarse(sors_ctx* c) {
     c->nagg = C.nagg;
     c->nc = 6 * C.nagg;
     c->npairs = (int)C.pairI.size();
+    c->ngchunks = (int)C.gptr.size() - 1;
     hipStream_t st = c->stream;
-    upload(c->cagg, C.agg.data(), C.agg.size(), st);
-    upload(c->captr, C.aptr.data(), C.aptr.size(), st);
-    upload(c->canodes, C.anodes.data(), std::max<size_t>(C.anodes.size(), 1), st);
-    upload(c->ccent, C.cent.data(), std::max<size_t>(C.cent.size(), 1), st);
-    upload(c->cscale, C.scale.data(), std::max<size_t>(C.scale.size(), 1), st);
-    upload(c->crot, C.rot.data(), std::max<size_t>(C.rot.size(), 1), st);
-    upload(c->cpairI, C.pairI.data(), std::max<size_t>(C.pairI.size(), 1), st);
-    upload(c->cpairJ, C.pairJ.data(), std::max<size_t>(C.pairJ.size(), 1), st);
-    upload(c->cpptr, C.pptr.data(), C.pptr.size(), st);
-    upload(c->cci, C.ci.data(), std::max<size_t>(C.ci.size(), 1), st);
-    upload(c->ccj, C.cj.data(), std::max<size_t>(C.cj.size(), 1), st);
-    upload(c->ccbase, C.cbase.data(), std::max<size_t>(C.cbase.size(), 1), st);
+    auto up = [&](DevMem& m, const auto& v) { upload(m, v.data(), std::max<size_t>(v.size(), 1), st); };
+    up(c->cdn, C.dn);
+    up(c->canodes, C.anodes);
+    up(c->captr, C.aptr);
+    up(c->crot, C.rot);
+    up(c->cpairI, C.pairI);
+    up(c->cpairJ, C.pairJ);
+    up(c->cpchunk, C.pchunk);
+    up(c->cgptr, C.gptr);
+    up(c->cci, C.ci);
+    up(c->ccj, C.cj);
+    up(c->ccbase, C.cbase);
     const size_t nc2 = (size_t)std::max(c->nc, 1) * std::max(c->nc, 1);
     ensure(c->cE, sizeof(double) * nc2);
     ensure(c->cEinv, sizeof(double) * nc2);
-    ensure(c->crc, sizeof(double) * std::max(c->nc, 1));
     ensure(c->cxc, sizeof(double) * std::max(c->nc, 1));
+    ensure(c->crc, sizeof(double) * std::max(c->nc, 1));
+    ensure(c->cgpart, sizeof(double) * 36 * std::max(c->ngchunks, 1));
     ensure(c->cpart, sizeof(double) * 2 * kMaxRedBlocks);
     ensure(c->cinfo, sizeof(int));
     if (!c->blas) {
@@ -2402,24 +2425,32 @@ void ensure_coarse(sors_ctx* c) {
     sync(c);
     c->coarse_built_for = want;
     c->coarse_pat = c->vals.p;
+    c->coarse_tangents = 0;
     ++c->coarse_gen;
 }
 
-// E = Z^T A Z, its inverse (rocSOLVER Cholesky), as HT; coarse_ready on success
+// E = Z^T A Z, its inverse (rocSOLVER Cholesky); coarse_ready on success
 template <typename HT>
 void coarse_setup(sors_ctx* c) {
     ensure_coarse(c);
+    if (c->coarse_built_for == 0 || c->nc == 0) {
+        c->coarse_ready = false;
+        return;
+    }
+    const long long k = c->coarse_tangents++;
+    if (c->coarse_ready && c->opt.coarse_every > 1 && k % c->opt.coarse_every != 0) return;  // lagged operator
     c->coarse_ready = false;
-    if (c->coarse_built_for == 0 || c->nc == 0) return;
     const int nc = c->nc;
     const size_t nc2 = (size_t)nc * nc;
     fill<double>(c->cE.as<double>(), (long long)nc2, 0.0, c->stream);
-    const CoarseArgs ca = coarse_args(c);
-    if (c->npairs > 0)
-        k_coarse_galerkin<HT><<<(c->npairs * 64 + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
-            ca, c->npairs, c->cpairI.as<int>(), c->cpairJ.as<int>(), c->cpptr.as<int>(), c->cci.as<int>(),
-            c->ccj.as<int>(), c->ccbase.as<int>(), (const HT*)c->vals.p, c->cE.as<double>());
-    k_coarse_fixnull<<<grid_for(c->nagg), kBlock, 0, c->stream>>>(ca, c->cE.as<double>());
+    if (c->ngchunks > 0) {
+        k_coarse_galerkin<HT><<<(c->ngchunks * 64 + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
+            c->ngchunks, c->cgptr.as<int>(), c->cci.as<int>(), c->ccj.as<int>(), c->ccbase.as<int>(),
+            (const float4*)c->cdn.p, (const HT*)c->vals.p, c->cgpart.as<double>());
+        k_coarse_gather<<<grid_for(36LL * c->npairs), kBlock, 0, c->stream>>>(
+            c->npairs, nc, c->cpairI.as<int>(), c->cpairJ.as<int>(), c->cpchunk.as<int>(), c->crot.as<int>(),
+            c->cgpart.as<double>(), c->cE.as<double>());
+    }
     HIPX(hipGetLastError());
     int* info = c->cinfo.as<int>();
     if (rocsolver_dpotrf(c->blas, rocblas_fill_lower, nc, c->cE.as<double>(), nc, info) != rocblas_status_success)
@@ -2532,14 +2563,15 @@ void launch_update(sors_ctx* c, const PcgGrid& g, const PcgView<HT>& v, int par,
     k_pcg_update<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(v, v.st + (par ^ 1), v.st + par, io, first);
 }
 
+inline int coarse_solve_blocks(int nc) { return (nc + 1) / 2; }  // <= kMaxCoarse / 2 <= kMaxRedBlocks
+
 // u += Z E^-1 Z^T r after an update (or the init); gamma's coarse partials into slot par
 template <typename HT>
 void launch_coarse(sors_ctx* c, PcgCtl* ctl, int par) {
     const CoarseArgs ca = coarse_args(c);
-    k_coarse_restrict<HT><<<c->nagg, kBlock, 0, c->stream>>>(ca, ctl, (const HT*)c->pr.p, c->crc.as<double>());
-    k_coarse_solve<<<(c->nc * 64 + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
-        c->nc, ctl, c->cEinv.as<double>(), c->crc.as<double>(), c->cxc.as<double>(),
-        c->cpart.as<double>() + par * kMaxRedBlocks);
+    k_coarse_restrict<HT><<<c->nagg, kRestrictThreads, 0, c->stream>>>(ca, ctl, (const HT*)c->pr.p);
+    k_coarse_solve<<<coarse_solve_blocks(c->nc), kBlock, 0, c->stream>>>(
+        ca, ctl, c->cEinv.as<double>(), c->cxc.as<double>(), c->cpart.as<double>() + par * kMaxRedBlocks);
     k_coarse_prolong<HT><<<grid_for(c->N), kBlock, 0, c->stream>>>(ca, c->N, ctl, c->cxc.as<double>(), (HT*)c->pu.p);
 }
 
@@ -2567,7 +2599,7 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
     };
     // two-level preconditioner: a coarse correction after every update
     const bool cs = c->coarse_ready;
-    const int nbC = cs ? (c->nc * 64 + kBlock - 1) / kBlock : 0;
+    const int nbC = cs ? coarse_solve_blocks(c->nc) : 0;
     auto inC = [&](int par) -> const double* { return cs ? c->cpart.as<double>() + par * kMaxRedBlocks : nullptr; };
     // iteration k >= 2 reads from st[(k-1)%2] and writes st[k%2]
     auto iteration = [&](int k, int j, bool timed) {
@@ -2908,9 +2940,9 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->pw,     &c->px,    &c->partials, &c->scal, &c->pcgst,   &c->pcgctl,  &c->pcgpart,
                            &c->flag,   &c->sbase, &c->bcol,    &c->selfslot, &c->csurf, &c->lagN,
                            &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
-                           &c->incptr, &c->inc,  &c->vold,     &c->cagg,   &c->captr,   &c->canodes, &c->ccent,
-                           &c->cscale, &c->crot, &c->cpairI,   &c->cpairJ, &c->cpptr,   &c->cci,     &c->ccj,
-                           &c->ccbase, &c->cE,   &c->cEinv,    &c->crc,    &c->cxc,     &c->cpart,   &c->cinfo,
+                           &c->incptr, &c->inc,  &c->vold,     &c->cdn,    &c->captr,   &c->canodes, &c->crot,
+                           &c->cpairI, &c->cpairJ, &c->cpchunk, &c->cgptr, &c->cci,     &c->ccj,     &c->ccbase,
+                           &c->cE,     &c->cEinv, &c->crc,     &c->cxc,    &c->cgpart,  &c->cpart,   &c->cinfo,
                            &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA};
     double b = 0;
     for (const DevMem* d : all) b += (double)d->bytes;
@@ -3393,6 +3425,7 @@ void sors_default_options(sors_options* o) {
     o->profile_kernels = 0;
     o->predict = 2;
     o->coarse_nodes = 0;
+    o->coarse_every = 1;
 }
 
 int sors_create(const sors_system_desc* d, sors_ctx** out) {
@@ -3555,6 +3588,7 @@ int sors_set_options(sors_ctx* c, const sors_options* o) {
     if (o->precision != SORS_FP64 && o->precision != SORS_MIXED) return fail(c, SORS_ERR_ARG, "bad precision");
     if (o->max_newton < 1 || o->pcg_max_iter < 1 || o->max_backtrack < 0)
         return fail(c, SORS_ERR_ARG, "iteration limits must be positive");
+    if (o->coarse_nodes < 0) return fail(c, SORS_ERR_ARG, "coarse_nodes must be >= 0");
     c->opt = *o;
     return SORS_OK;
 }

@@ -47,6 +47,7 @@ class Options(C.Structure):
         ("g_tol", C.c_double), ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int), ("max_backtrack", C.c_int),
         ("pcg_check_every", C.c_int), ("use_graph", C.c_int), ("profile_kernels", C.c_int), ("predict", C.c_int),
         ("coarse_nodes", C.c_int),
+        ("coarse_every", C.c_int),
     ]
 
 
