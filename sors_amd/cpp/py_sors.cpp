@@ -273,6 +273,8 @@ PYBIND11_MODULE(py_sors, m) {
         .def_readwrite("precision", &SimulationSettings::precision)
         .def_readwrite("pcgTolerance", &SimulationSettings::pcgTolerance)
         .def_readwrite("predictor", &SimulationSettings::predictor)
+        .def_readwrite("coarseNodes", &SimulationSettings::coarseNodes)
+        .def_readwrite("coarseEvery", &SimulationSettings::coarseEvery)
         .def("verbose_print", &SimulationSettings::verbose_print);
 
     bind_energy<4>(m, "EnergyTet");

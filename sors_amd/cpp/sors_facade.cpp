@@ -89,6 +89,8 @@ void SimulationSettings::verbose_print() const {
               << std::setw(25) << "Time Scheme:" << na(timeSteppingScheme) << "\n"
               << std::setw(25) << "Solver Method:" << na(solverMethod) << "\n"
               << std::setw(25) << "Precision:" << precision << "\n"
+              << std::setw(25) << "PCG Coarse Space:"
+              << (coarseNodes > 0 ? std::to_string(coarseNodes) + " nodes per aggregate" : std::string("off")) << "\n"
               << "================================================================================\n";
 }
 
@@ -241,6 +243,8 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
     o.precision = settings.precision == "fp64" ? SORS_FP64 : SORS_MIXED;
     o.pcg_rtol = settings.pcgTolerance;
     o.predict = settings.predictor ? 2 : 0;
+    o.coarse_nodes = settings.coarseNodes;
+    o.coarse_every = settings.coarseEvery;
     check_status(sors_set_options(ctx_, &o), ctx_);
 }
 

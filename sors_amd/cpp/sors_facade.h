@@ -77,6 +77,8 @@ struct SimulationSettings {
     std::string precision = "mixed";  // "mixed" | "fp64"
     double pcgTolerance = 1e-6;
     bool predictor = false;           // start Newton from the extrapolated state instead of the given solution
+    int coarseNodes = 0;              // > 0: two-level PCG (rigid-body coarse correction on aggregates of this size)
+    int coarseEvery = 4;              // coarse operator refreshed on every k-th tangent
     void verbose_print() const;
 };
 

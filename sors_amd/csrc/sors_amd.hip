@@ -1403,25 +1403,6 @@ __device__ __forceinline__ void block_partial_at(double (&v)[NV], double* base, 
         for (int k = 0; k < NV; ++k) base[k * stride + lb] = v[k];
 }
 
-template <int NV>
-__device__ __forceinline__ void sum_partials_at(const double* base, int stride, int nb, double (&out)[NV]) {
-    __shared__ double bc[NV];
-    double s[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) s[k] = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) s[k] += base[k * stride + i];
-    block_sum<NV>(s);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) bc[k] = s[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] = bc[k];
-    __syncthreads();
-}
-
 // ---- bodies (lb / nb: this block's index and the block count of its rollout)
 // init: x = 0, r = sign*b, p = s = 0, u = M r; partials (r.u, r.r)
 template <typename HT>
@@ -1449,6 +1430,32 @@ __device__ void pcg_init_body(const PcgView<HT>& v, const double* b, double sign
     block_partial_at<2>(acc, io.out, io.stride, lb);
 }
 
+// Sums of the partial sets a kernel consumes, in one pass: t[0], t[1] the two
+// slots of inU; t[2] inA (when given); t[3] inC (when given), each over its
+// own producer's block count in ascending order, then one workgroup sum.
+__device__ __forceinline__ void sum_partials_multi(const PartIO& io, double (&t)[4]) {
+    __shared__ double bc[4];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    const int nbA = io.inA ? io.nbA : 0, nbC = io.inC ? io.nbC : 0;
+    const int nmax = max(io.nbU, max(nbA, nbC));
+    for (int i = threadIdx.x; i < nmax; i += blockDim.x) {
+        if (i < io.nbU) {
+            s[0] += io.inU[i];
+            s[1] += io.inU[io.stride + i];
+        }
+        if (i < nbA) s[2] += io.inA[i];
+        if (i < nbC) s[3] += io.inC[i];
+    }
+    block_sum<4>(s);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) bc[k] = s[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k] = bc[k];
+    __syncthreads();
+}
+
 // SpMV prologue: returns true when the solve is over (converged, breakdown,
 // or a zero right-hand side).  Block 0 of the rollout records the outcome.
 __device__ __forceinline__ bool spmv_prologue(PcgCtl* ctl, PcgState* st, const double* partU, int stride, int nbU,
@@ -1456,11 +1463,12 @@ __device__ __forceinline__ bool spmv_prologue(PcgCtl* ctl, PcgState* st, const d
                                               int nbC = 0) {
     if (ctl->done) return true;
     double t[2];
-    sum_partials_at<2>(partU, stride, nbU, t);  // gamma_k, rr_k
-    if (partC) {
-        double gc[1];
-        sum_partials_at<1>(partC, stride, nbC, gc);
-        t[0] += gc[0];
+    {
+        const PartIO io{nullptr, stride, partU, nbU, nullptr, 0, partC, nbC};
+        double u[4];
+        sum_partials_multi(io, u);  // gamma_k, rr_k, -, coarse part of gamma_k
+        t[0] = u[0] + u[3];
+        t[1] = u[1];
     }
     if (first) {
         const bool zero = !(t[1] > 0.0) || !(t[0] > 0.0);
@@ -1576,12 +1584,15 @@ __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgStat
     int base = lb * kUpdBlock;
     if (base < n) load(base);
     double g[2], d[1];
-    sum_partials_at<2>(io.inU, io.stride, io.nbU, g);  // gamma_{k-1}, rr_{k-1}
-    sum_partials_at<1>(io.inA, io.stride, io.nbA, d);  // delta_{k-1}
-    if (io.inC) {
-        double gc[1];
-        sum_partials_at<1>(io.inC, io.stride, io.nbC, gc);  // coarse part of gamma_{k-1}
-        g[0] += gc[0];
+    {
+        // gamma_{k-1} and rr_{k-1} (inU), delta_{k-1} (inA), coarse part of
+        // gamma_{k-1} (inC): one pass over the three partial sets and one
+        // workgroup sum; each slot keeps its own fixed summation order
+        double t[4];
+        sum_partials_multi(io, t);
+        g[0] = t[0] + t[3];
+        g[1] = t[1];
+        d[0] = t[2];
     }
     double alpha, beta;
     bool bad;
@@ -1742,21 +1753,31 @@ __global__ void k_coarse_gather(int npairs, int nc, const int* pairI, const int*
     E[(size_t)(6 * I + r) * nc + 6 * J + col] = v;
 }
 
-// inverse from the column-major lower triangle (potri) into a full row-major
-// matrix.  The coarse solve stays in FP64 at every precision: E is as badly
+// X = identity (the right-hand sides of the coarse inverse)
+__global__ void k_coarse_eye(int nc, double* X) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nc * nc) return;
+    X[t] = (t / nc == t % nc) ? 1.0 : 0.0;
+}
+
+// E^-1 = (X + X^T)/2 from X = E^-1 (potrs, symmetric up to rounding): an
+// exactly symmetric inverse, so Z E^-1 Z^T is a symmetric preconditioner.
+// The coarse solve stays in FP64 at every precision: E is as badly
 // conditioned as the coarse problem, and an FP32 copy of its inverse need not
 // be positive definite.
-__global__ void k_coarse_symcopy(int nc, const double* L, double* Einv) {
+__global__ void k_coarse_symmetrize(int nc, const double* X, double* Einv) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nc * nc) return;
     const int a = (int)(t / nc), b = (int)(t % nc);
-    const double v = a >= b ? L[(size_t)b * nc + a] : L[(size_t)a * nc + b];
-    Einv[t] = v;
+    Einv[t] = 0.5 * (X[(size_t)a * nc + b] + X[(size_t)b * nc + a]);
 }
 
-// rc = Z^T r: one 1024-thread workgroup per aggregate, its nodes in a fixed
-// order, a fixed-order workgroup sum (deterministic)
-constexpr int kRestrictThreads = 1024;
+// rc = Z^T r: one workgroup per aggregate, its nodes in a fixed order, a
+// fixed-order workgroup sum (deterministic)
+#ifndef SORS_RESTRICT_THREADS
+#define SORS_RESTRICT_THREADS 512
+#endif
+constexpr int kRestrictThreads = SORS_RESTRICT_THREADS;
 template <typename HT>
 __global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict(CoarseArgs c, const PcgCtl* ctl,
                                                                       const HT* r) {
@@ -1783,71 +1804,54 @@ __global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict(CoarseArgs
         for (int k = 0; k < 6; ++k) c.rc[6 * (size_t)I + k] = acc[k];
 }
 
-// xc = E^-1 rc: two rows per 256-thread workgroup (128 lanes per row, eight
-// independent loads in flight per lane, fixed-order sums), and one partial of
-// gamma_c = rc . xc per workgroup
+// xc_I = rows 6I..6I+5 of E^-1 times rc, then u += Z_I xc_I over the nodes of
+// aggregate I: one 1024-thread workgroup per aggregate, so E^-1 is still read
+// once per application and the prolongation needs no second launch.  One
+// partial of gamma_c = rc . xc per workgroup; fixed-order sums throughout.
 constexpr int kMaxCoarse = 6 * 512;  // coarse unknowns: nc^2 FP64 values are read per PCG iteration
-constexpr int kSolveRowThreads = 128;
-static_assert(kMaxCoarse / 2 <= kMaxRedBlocks, "one gamma partial per coarse-solve workgroup");
-__global__ __launch_bounds__(kBlock) void k_coarse_solve(CoarseArgs c, const PcgCtl* ctl,
-                                                         const double* __restrict__ Einv, double* xc, double* part) {
+#ifndef SORS_APPLY_THREADS
+#define SORS_APPLY_THREADS 512
+#endif
+constexpr int kApplyThreads = SORS_APPLY_THREADS;
+static_assert(kMaxCoarse / 6 <= kMaxRedBlocks, "one gamma partial per aggregate");
+template <typename HT>
+__global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(CoarseArgs c, const PcgCtl* ctl,
+                                                                const double* __restrict__ Einv, HT* u,
+                                                                double* part) {
     if (ctl && ctl->done) return;
-    __shared__ double sh[kBlock / 64];
-    const int nc = c.nc;
+    __shared__ double xs[6];
+    const int I = blockIdx.x, nc = c.nc;
     const double* __restrict__ rc = c.rc;
-    const int half = threadIdx.x / kSolveRowThreads, t = threadIdx.x % kSolveRowThreads;
-    const int row = 2 * blockIdx.x + half;
-    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (row < nc) {
-        const double* __restrict__ Er = Einv + (size_t)row * nc;
-        int col = t;
-        for (; col + 7 * kSolveRowThreads < nc; col += 8 * kSolveRowThreads) {
-            double e[8], x[8];
+    const double* __restrict__ E0 = Einv + (size_t)6 * I * nc;
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int col = threadIdx.x; col < nc; col += kApplyThreads) {
+        double e[6];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                e[j] = Er[col + j * kSolveRowThreads];
-                x[j] = rc[col + j * kSolveRowThreads];
-            }
+        for (int k = 0; k < 6; ++k) e[k] = E0[(size_t)k * nc + col];
+        const double x = rc[col];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) a[j] += e[j] * x[j];
-        }
-        for (; col < nc; col += kSolveRowThreads) a[0] += Er[col] * rc[col];  // fewer than 8 left
+        for (int k = 0; k < 6; ++k) a[k] += e[k] * x;
     }
-    double v = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
+    block_sum<6>(a);  // block totals in thread 0
     if (threadIdx.x == 0) {
         double g = 0.0;
-        for (int h = 0; h < 2; ++h) {
-            const int r = 2 * blockIdx.x + h;
-            if (r < nc) {
-                const double tot = sh[2 * h] + sh[2 * h + 1];
-                xc[r] = tot;
-                g += tot * rc[r];
-            }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            xs[k] = a[k];
+            g += a[k] * rc[6 * I + k];
         }
-        part[blockIdx.x] = g;
+        part[I] = g;
     }
-}
-
-// u += Z xc
-template <typename HT>
-__global__ __launch_bounds__(kBlock) void k_coarse_prolong(CoarseArgs c, int N, const PcgCtl* ctl, const double* xc,
-                                                           HT* u) {
-    if (ctl && ctl->done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const float4 f = c.dn[i];
-    const int I = coarse_agg(f);
-    if (I < 0) return;
-    const double d0 = f.x, d1 = f.y, d2 = f.z;
-    const double t0 = xc[6 * I], t1 = xc[6 * I + 1], t2 = xc[6 * I + 2];
-    const double w0 = xc[6 * I + 3], w1 = xc[6 * I + 4], w2 = xc[6 * I + 5];
-    u[3 * i] += (HT)(t0 + w1 * d2 - w2 * d1);
-    u[3 * i + 1] += (HT)(t1 + w2 * d0 - w0 * d2);
-    u[3 * i + 2] += (HT)(t2 + w0 * d1 - w1 * d0);
+    __syncthreads();
+    const double t0 = xs[0], t1 = xs[1], t2 = xs[2], w0 = xs[3], w1 = xs[4], w2 = xs[5];
+    for (int t = c.aptr[I] + threadIdx.x; t < c.aptr[I + 1]; t += kApplyThreads) {
+        const int i = c.anodes[t];
+        const float4 f = c.dn[i];
+        const double d0 = f.x, d1 = f.y, d2 = f.z;
+        u[3 * i] += (HT)(t0 + w1 * d2 - w2 * d1);
+        u[3 * i + 1] += (HT)(t1 + w2 * d0 - w0 * d2);
+        u[3 * i + 2] += (HT)(t2 + w0 * d1 - w1 * d0);
+    }
 }
 
 // ---- single-rollout kernels ----------------------------------------------
@@ -2114,7 +2118,7 @@ struct sors_ctx {
     bool coarse_ready = false;      // E^-1 valid for the current matrix
     int nagg = 0, nc = 0, npairs = 0, ngchunks = 0;
     DevMem cdn, captr, canodes, crot, cpairI, cpairJ, cpchunk, cgptr, cci, ccj, ccbase;
-    DevMem cE, cEinv, crc, cxc, cgpart, cpart, cinfo;
+    DevMem cE, cEinv, cX, crc, cgpart, cpart, cinfo;
     rocblas_handle blas = nullptr;
     bool vold_valid = false;
     double vold_dt = 0.0;
@@ -2141,7 +2145,9 @@ struct sors_ctx {
     hipEvent_t* events(int count) {
         while ((int)pev.size() < count) {
             hipEvent_t e;
-            HIPX(hipEventCreate(&e));
+            // timing only (read after a stream synchronisation): no system-scope
+            // fence, whose cache writeback would be timed with the kernel
+            HIPX(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
             pev.push_back(e);
         }
         return pev.data();
@@ -2413,7 +2419,7 @@ void ensure_coarse(sors_ctx* c) {
     const size_t nc2 = (size_t)std::max(c->nc, 1) * std::max(c->nc, 1);
     ensure(c->cE, sizeof(double) * nc2);
     ensure(c->cEinv, sizeof(double) * nc2);
-    ensure(c->cxc, sizeof(double) * std::max(c->nc, 1));
+    ensure(c->cX, sizeof(double) * nc2);
     ensure(c->crc, sizeof(double) * std::max(c->nc, 1));
     ensure(c->cgpart, sizeof(double) * 36 * std::max(c->ngchunks, 1));
     ensure(c->cpart, sizeof(double) * 2 * kMaxRedBlocks);
@@ -2429,7 +2435,8 @@ void ensure_coarse(sors_ctx* c) {
     ++c->coarse_gen;
 }
 
-// E = Z^T A Z, its inverse (rocSOLVER Cholesky); coarse_ready on success
+// E = Z^T A Z and its inverse (rocSOLVER Cholesky factorisation and solves);
+// coarse_ready on success
 template <typename HT>
 void coarse_setup(sors_ctx* c) {
     ensure_coarse(c);
@@ -2459,10 +2466,14 @@ void coarse_setup(sors_ctx* c) {
     HIPX(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     sync(c);
     if (h_info != 0) return;  // not positive definite: block-Jacobi alone for this solve
-    if (rocsolver_dpotri(c->blas, rocblas_fill_lower, nc, c->cE.as<double>(), nc, info) != rocblas_status_success)
-        throw HipError{"rocsolver_dpotri failed"};
-    k_coarse_symcopy<<<(unsigned)((nc2 + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(nc, c->cE.as<double>(),
-                                                                                      c->cEinv.as<double>());
+    // E X = I with the Cholesky factor (two GEMM-rich triangular solves with nc
+    // right-hand sides), then the exactly symmetric average
+    const unsigned gsq = (unsigned)((nc2 + kBlock - 1) / kBlock);
+    k_coarse_eye<<<gsq, kBlock, 0, c->stream>>>(nc, c->cX.as<double>());
+    if (rocsolver_dpotrs(c->blas, rocblas_fill_lower, nc, nc, c->cE.as<double>(), nc, c->cX.as<double>(), nc) !=
+        rocblas_status_success)
+        throw HipError{"rocsolver_dpotrs failed"};
+    k_coarse_symmetrize<<<gsq, kBlock, 0, c->stream>>>(nc, c->cX.as<double>(), c->cEinv.as<double>());
     HIPX(hipGetLastError());
     c->coarse_ready = true;
 }
@@ -2563,16 +2574,13 @@ void launch_update(sors_ctx* c, const PcgGrid& g, const PcgView<HT>& v, int par,
     k_pcg_update<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(v, v.st + (par ^ 1), v.st + par, io, first);
 }
 
-inline int coarse_solve_blocks(int nc) { return (nc + 1) / 2; }  // <= kMaxCoarse / 2 <= kMaxRedBlocks
-
 // u += Z E^-1 Z^T r after an update (or the init); gamma's coarse partials into slot par
 template <typename HT>
 void launch_coarse(sors_ctx* c, PcgCtl* ctl, int par) {
     const CoarseArgs ca = coarse_args(c);
     k_coarse_restrict<HT><<<c->nagg, kRestrictThreads, 0, c->stream>>>(ca, ctl, (const HT*)c->pr.p);
-    k_coarse_solve<<<coarse_solve_blocks(c->nc), kBlock, 0, c->stream>>>(
-        ca, ctl, c->cEinv.as<double>(), c->cxc.as<double>(), c->cpart.as<double>() + par * kMaxRedBlocks);
-    k_coarse_prolong<HT><<<grid_for(c->N), kBlock, 0, c->stream>>>(ca, c->N, ctl, c->cxc.as<double>(), (HT*)c->pu.p);
+    k_coarse_apply<HT><<<c->nagg, kApplyThreads, 0, c->stream>>>(ca, ctl, c->cEinv.as<double>(), (HT*)c->pu.p,
+                                                                 c->cpart.as<double>() + par * kMaxRedBlocks);
 }
 
 // PCG on the assembled matrix: solves A x = sign*b; x in c->px (HT).
@@ -2599,7 +2607,7 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
     };
     // two-level preconditioner: a coarse correction after every update
     const bool cs = c->coarse_ready;
-    const int nbC = cs ? coarse_solve_blocks(c->nc) : 0;
+    const int nbC = cs ? c->nagg : 0;
     auto inC = [&](int par) -> const double* { return cs ? c->cpart.as<double>() + par * kMaxRedBlocks : nullptr; };
     // iteration k >= 2 reads from st[(k-1)%2] and writes st[k%2]
     auto iteration = [&](int k, int j, bool timed) {
@@ -2942,7 +2950,7 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
                            &c->incptr, &c->inc,  &c->vold,     &c->cdn,    &c->captr,   &c->canodes, &c->crot,
                            &c->cpairI, &c->cpairJ, &c->cpchunk, &c->cgptr, &c->cci,     &c->ccj,     &c->ccbase,
-                           &c->cE,     &c->cEinv, &c->crc,     &c->cxc,    &c->cgpart,  &c->cpart,   &c->cinfo,
+                           &c->cE,     &c->cEinv, &c->crc,     &c->cX,      &c->cgpart,  &c->cpart,   &c->cinfo,
                            &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA};
     double b = 0;
     for (const DevMem* d : all) b += (double)d->bytes;
