@@ -225,6 +225,35 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
         // node -> (element, local index) incidence for the node-centric assembly
         P.incptr = std::move(iptr);
         P.inc = std::move(inc);
+        if (ed == 4) {
+            // slot -> element blocks (counting sort by slot position; elements ascending)
+            const long long S = P.sbase[ns];
+            P.qslice.assign((size_t)(S / kSlice), 0);
+            for (int sl = 0; sl < ns; ++sl)
+                for (int q = P.sbase[sl] / kSlice; q < P.sbase[sl + 1] / kSlice; ++q) P.qslice[q] = sl;
+            P.sptr.assign((size_t)S + 1, 0);
+            auto slot_of = [&](int e, int la, int lb) -> long long {
+                const int rk = P.rank[(size_t)(la * 4 + lb) * E + e];
+                if (rk == kRankSkip) return -1;
+                const int v = conn[(size_t)e * 4 + la];
+                return (long long)P.sbase[v >> 6] + (long long)kSlice * rk + (v & (kSlice - 1));
+            };
+            for (int e = 0; e < E; ++e)
+                for (int la = 0; la < 4; ++la)
+                    for (int lb = 0; lb < 4; ++lb) {
+                        const long long p = slot_of(e, la, lb);
+                        if (p >= 0) P.sptr[p + 1]++;
+                    }
+            for (long long p = 0; p < S; ++p) P.sptr[p + 1] += P.sptr[p];
+            P.sidx.assign((size_t)P.sptr[S], 0);
+            std::vector<int> fill(P.sptr.begin(), P.sptr.end() - 1);
+            for (int e = 0; e < E; ++e)
+                for (int la = 0; la < 4; ++la)
+                    for (int lb = 0; lb < 4; ++lb) {
+                        const long long p = slot_of(e, la, lb);
+                        if (p >= 0) P.sidx[fill[p]++] = 16 * e + 4 * la + lb;
+                    }
+        }
     } else {
         int nd = 3 * ed;
         P.fullmap.resize((size_t)E * nd * nd);

@@ -417,124 +417,126 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
 }
 
 
-// ---- tet4, node-centric assembly on the sliced node-block layout ---------
-// One lane per node (lanes of a wavefront = the 64 nodes of a slice).  The
-// lane walks its incident elements in a fixed order, recomputes each
-// element's row of 3x3 blocks (its own local node la against lb = 0..3) and
-// its gradient row, and accumulates them into its own row of the layout:
-// no atomics, every value written by exactly one lane, deterministic sums.
-// The lane first zeroes its row (padding included), so no fill is needed.
+// ---- tet4, two-phase assembly on the sliced node-block layout ------------
+// Phase 1 (k_tet_elem): one lane per element evaluates the element once: its
+// four FP64 gradient rows V0 P g_a (each padded with the packed ranks of its
+// row) and, in MODE 0, its sixteen 3x3 blocks (la, lb), stored element-major
+// with each block padded to a 16-byte multiple.
+// Phase 2, MODE 0 (k_tet_slots): one lane per slot of the sliced layout sums
+// the element blocks listed for that slot (pattern.h: sptr / sidx, ascending
+// element order) in registers and stores the block once; the lane of a
+// node's own slot also sums the node's gradient rows.  MODE 1 (k_tet_gather):
+// one lane per node sums its gradient rows.  No atomics, no read-modify-write
+// of the layout, every value written by exactly one lane, fixed-order sums.
 // MODE 0: gradient + Hessian; MODE 1: gradient only.
+constexpr int kTetRowBlocks = 16;   // stored 3x3 blocks per element
+constexpr int kTetRowDoubles = 16;  // per element: 4 x (gradient row, packed ranks)
+// stored values per block: 9 padded to a multiple of 16 bytes
+template <typename HT>
+constexpr int tet_blk() { return sizeof(HT) == 4 ? 12 : 10; }
+
 template <typename HT, int PROJ, int MODE>
-__global__ __launch_bounds__(kBlock) void k_tet_node(ElemArgs a, int nslices, const int* incptr, const int* inc) {
-    const int t = xcd_block() * blockDim.x + threadIdx.x;  // lane slot in the sliced layout
-    const int sl = t / kSlice, lane = t % kSlice;
-    if (sl >= nslices) return;
-    const int v = t;  // node id (>= N: padding lane)
-    HT* vals = (HT*)a.vals;
-    const int width = (a.sbase[sl + 1] - a.sbase[sl]) / kSlice;
-    const long long rowbase = 9LL * a.sbase[sl] + lane;
-    if (MODE == 0)
-        for (int k = 0; k < 9 * width; ++k) vals[rowbase + (long long)kSlice * k] = (HT)0;
-    if (v >= a.N) return;
-    double gacc[3] = {0.0, 0.0, 0.0};
-    bool inv = false;
-    const int j0 = incptr[v], j1 = incptr[v + 1];
-    for (int j = j0; j < j1; ++j) {
-        const int code = inc[j];
-        const int e = code >> 2, la = code & 3;
-        int nd[4];
-        double X[4][3], x[4][3];
+__global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT* eb) {
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    double X[4][3], x[4][3];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            nd[k] = a.conn[(size_t)k * a.E + e];
+    for (int k = 0; k < 4; ++k) {
+        const int nd = a.conn[(size_t)k * a.E + e];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                X[k][c] = a.rest[3 * nd[k] + c];
-                x[k][c] = a.q[3 * nd[k] + c];
-            }
+        for (int c = 0; c < 3; ++c) {
+            X[k][c] = a.rest[3 * nd + c];
+            x[k][c] = a.q[3 * nd + c];
         }
-        M3<double> Dm;
+    }
+    M3<double> Dm;
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+    for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) Dm(r, c) = X[c][r] - X[3][r];
-        const M3<double> Cm = cofactor3(Dm);
-        const double detm = Dm.m[0] * Cm.m[0] + Dm.m[1] * Cm.m[1] + Dm.m[2] * Cm.m[2];
-        const double V0 = fabs(detm) / 6.0;
-        double g[4][3];
+        for (int c = 0; c < 3; ++c) Dm(r, c) = X[c][r] - X[3][r];
+    const M3<double> Cm = cofactor3(Dm);
+    const double detm = Dm.m[0] * Cm.m[0] + Dm.m[1] * Cm.m[1] + Dm.m[2] * Cm.m[2];
+    const double V0 = fabs(detm) / 6.0;
+    double g[4][3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < 3; ++k)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) g[k][c] = Cm(c, k) / detm;
+        for (int c = 0; c < 3; ++c) g[k][c] = Cm(c, k) / detm;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) g[3][c] = -(g[0][c] + g[1][c] + g[2][c]);
-        M3<double> D;
+    for (int c = 0; c < 3; ++c) g[3][c] = -(g[0][c] + g[1][c] + g[2][c]);
+    M3<double> D;
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+    for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                double acc = 0;
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) acc += (x[k][r] - X[k][r]) * g[k][c];
-                D(r, c) = acc;
-            }
-        const double mu = a.mu[e], lam = a.lam[e];
-        const NHPoint<double> p = nh_point(D);
-        if (!(p.J > 0)) inv = true;
-        double fa[3] = {0.0, 0.0, 0.0}, Ff[3] = {0.0, 0.0, 0.0}, fg[4] = {0.0, 0.0, 0.0, 0.0};
-        const double sa = active_level(a, e, fa);
-        if (sa != 0.0) {
-#pragma unroll
-            for (int r = 0; r < 3; ++r) Ff[r] = fa[r] + D(r, 0) * fa[0] + D(r, 1) * fa[1] + D(r, 2) * fa[2];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) fg[k] = fa[0] * g[k][0] + fa[1] * g[k][1] + fa[2] * g[k][2];
+            for (int k = 0; k < 4; ++k) acc += (x[k][r] - X[k][r]) * g[k][c];
+            D(r, c) = acc;
         }
-        // gradient row of node la: V0 P g_la (the element scatter's sum order is per element, so the
-        // same addends in another order)
-        {
-            M3<double> P = nh_stress(p, mu, lam);
-            if (sa != 0.0)
+    const double mu = a.mu[e], lam = a.lam[e];
+    const NHPoint<double> p = nh_point(D);
+    if (!(p.J > 0)) atomicOr(a.flag, 1);
+    double fa[3] = {0.0, 0.0, 0.0}, Ff[3] = {0.0, 0.0, 0.0}, fg[4] = {0.0, 0.0, 0.0, 0.0};
+    const double sa = active_level(a, e, fa);
+    if (sa != 0.0) {
 #pragma unroll
-                for (int r = 0; r < 3; ++r)
+        for (int r = 0; r < 3; ++r) Ff[r] = fa[r] + D(r, 0) * fa[0] + D(r, 1) * fa[1] + D(r, 2) * fa[2];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) P(r, c) += sa * Ff[r] * fa[c];
-            double ga[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) ga[c] = g[la][c];
+        for (int k = 0; k < 4; ++k) fg[k] = fa[0] * g[k][0] + fa[1] * g[k][1] + fa[2] * g[k][2];
+    }
+    {
+        M3<double> P = nh_stress(p, mu, lam);
+        if (sa != 0.0)
 #pragma unroll
             for (int r = 0; r < 3; ++r)
-                if (dof_kept(a, 3 * v + r)) gacc[r] += V0 * (P(r, 0) * ga[0] + P(r, 1) * ga[1] + P(r, 2) * ga[2]);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) P(r, c) += sa * Ff[r] * fa[c];
+        double* ge = eg + (size_t)kTetRowDoubles * e;
+#pragma unroll
+        for (int la = 0; la < 4; ++la) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                ge[4 * la + r] = V0 * (P(r, 0) * g[la][0] + P(r, 1) * g[la][1] + P(r, 2) * g[la][2]);
+            // the four ranks of row la (slots of blocks (la, lb) in node la's row), packed
+            long long rk = 0;
+            if (MODE == 0)
+#pragma unroll
+                for (int lb = 0; lb < 4; ++lb)
+                    rk |= (long long)a.rank[(size_t)(la * 4 + lb) * a.E + e] << (8 * lb);
+            ge[4 * la + 3] = __longlong_as_double(rk);
         }
-        if (MODE == 1) continue;
-        const HT w = (HT)(V0 * a.hscale);
-        HT gh[4][3];
+    }
+    if (MODE == 1) return;
+    const HT w = (HT)(V0 * a.hscale);
+    HT gh[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gh[k][c] = (HT)g[k][c];
+    const HT muh = (HT)mu, lamh = (HT)lam;
+    Sym9<HT> S;
+    HT cc = 0;
+    HT h[4][3];
+    if (PROJ) {
+        M3<HT> Dh;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Dh.m[i] = (HT)D.m[i];
+        const NHPoint<HT> ph = nh_point(Dh);
+        nh_projected_tangent(Dh, ph, muh, lamh, S);
+    } else {
+        cc = (HT)(lam * p.lnJ - mu);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) gh[k][c] = (HT)g[k][c];
-        const HT muh = (HT)mu, lamh = (HT)lam;
-        Sym9<HT> S;
-        HT cc = 0;
-        HT h[4][3];
-        if (PROJ) {
-            M3<HT> Dh;
+            for (int r = 0; r < 3; ++r)
+                h[k][r] = (HT)(p.FiT(r, 0) * g[k][0] + p.FiT(r, 1) * g[k][1] + p.FiT(r, 2) * g[k][2]);
+    }
+    HT* be = eb + (size_t)tet_blk<HT>() * kTetRowBlocks * e;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) Dh.m[i] = (HT)D.m[i];
-            const NHPoint<HT> ph = nh_point(Dh);
-            nh_projected_tangent(Dh, ph, muh, lamh, S);
-        } else {
-            cc = (HT)(lam * p.lnJ - mu);
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-                    h[k][r] = (HT)(p.FiT(r, 0) * g[k][0] + p.FiT(r, 1) * g[k][1] + p.FiT(r, 2) * g[k][2]);
-        }
+    for (int la = 0; la < 4; ++la)
 #pragma unroll
         for (int lb = 0; lb < 4; ++lb) {
-            const int rk = a.rank[(size_t)(la * 4 + lb) * a.E + e];
-            if (rk == kRankSkip) continue;
             HT blk[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) blk[i] = 0;
@@ -548,14 +550,109 @@ __global__ __launch_bounds__(kBlock) void k_tet_node(ElemArgs a, int nslices, co
                 blk[4] += s_ab;
                 blk[8] += s_ab;
             }
-            const long long base = rowbase + (long long)kSlice * 9 * rk;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) vals[base + (long long)kSlice * k] += blk[k];
+            for (int i = 0; i < 9; ++i) be[tet_blk<HT>() * (4 * la + lb) + i] = blk[i];
+        }
+}
+
+// sum of node v's gradient rows over its incident elements (element order)
+__device__ __forceinline__ void tet_node_gradient(const ElemArgs& a, int v, const int* incptr, const int* inc,
+                                                  const double* eg) {
+    bool kept[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) kept[r] = dof_kept(a, 3 * v + r);
+    double gacc[3] = {0.0, 0.0, 0.0};
+    const int j0 = incptr[v], j1 = incptr[v + 1];
+    int j = j0;
+    for (; j + 4 <= j1; j += 4) {  // four rows in flight, summed in list order
+        int code[4];
+        double2 g01[4], g2r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) code[u] = inc[j + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double2* ge = (const double2*)(eg + (size_t)kTetRowDoubles * (code[u] >> 2) + 4 * (code[u] & 3));
+            g01[u] = ge[0];
+            g2r[u] = ge[1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (kept[0]) gacc[0] += g01[u].x;
+            if (kept[1]) gacc[1] += g01[u].y;
+            if (kept[2]) gacc[2] += g2r[u].x;
         }
     }
-    if (inv) atomicOr(a.flag, 1);
+    for (; j < j1; ++j) {
+        const int code = inc[j];
+        const double2* ge = (const double2*)(eg + (size_t)kTetRowDoubles * (code >> 2) + 4 * (code & 3));
+        const double2 g01 = ge[0], g2r = ge[1];
+        if (kept[0]) gacc[0] += g01.x;
+        if (kept[1]) gacc[1] += g01.y;
+        if (kept[2]) gacc[2] += g2r.x;
+    }
 #pragma unroll
     for (int r = 0; r < 3; ++r) a.gint[3 * v + r] = gacc[r];
+}
+
+__global__ __launch_bounds__(kBlock) void k_tet_gather(ElemArgs a, const int* incptr, const int* inc, const double* eg) {
+    const int v = xcd_block() * blockDim.x + threadIdx.x;
+    if (v >= a.N) return;
+    tet_node_gradient(a, v, incptr, inc, eg);
+}
+
+// one lane per slot position p = sbase[s] + 64 k + lane (groups of 64 = one
+// wavefront = slot k of slice s)
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_tet_slots(ElemArgs a, int nq, const int* qslice, const int* sptr,
+                                                      const int* sidx, const HT* eb, const int* selfslot,
+                                                      const int* incptr, const int* inc, const double* eg) {
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
+    const int q = p / kSlice, lane = p % kSlice;
+    if (q >= nq) return;
+    const int s = qslice[q];
+    const int k = q - a.sbase[s] / kSlice;
+    constexpr int kPieces = tet_blk<HT>() * (int)sizeof(HT) / 16;
+    HT acc[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) acc[i] = (HT)0;
+    union Blk {
+        uint4 q4[kPieces];
+        HT v[tet_blk<HT>()];
+    };
+    const int c0 = sptr[p], c1 = sptr[p + 1];
+    int c = c0;
+    // groups of four blocks: their indices, then their values, are loaded
+    // together; the sums keep the list order
+    for (; c + 4 <= c1; c += 4) {
+        int id[4];
+        Blk blk[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) id[u] = sidx[c + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint4* src = (const uint4*)(eb + (size_t)tet_blk<HT>() * id[u]);
+#pragma unroll
+            for (int i = 0; i < kPieces; ++i) blk[u].q4[i] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int i = 0; i < 9; ++i) acc[i] += blk[u].v[i];
+    }
+    for (; c < c1; ++c) {
+        Blk blk;
+        const uint4* src = (const uint4*)(eb + (size_t)tet_blk<HT>() * sidx[c]);
+#pragma unroll
+        for (int i = 0; i < kPieces; ++i) blk.q4[i] = src[i];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[i] += blk.v[i];
+    }
+    HT* vals = (HT*)a.vals;
+    const long long vb = 9LL * a.sbase[s] + (long long)kSlice * 9 * k + lane;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) vals[vb + (long long)kSlice * i] = acc[i];
+    const int v = s * kSlice + lane;
+    if (v < a.N && k == selfslot[v]) tet_node_gradient(a, v, incptr, inc, eg);
 }
 
 // ---- hex8: one element per lane --------------------------------------------
@@ -2108,7 +2205,9 @@ struct sors_ctx {
     bool pat_on_device = false;
     DevMem pcgst_bench, copybuf;
     DevMem sbase, bcol, selfslot;   // sliced node-block layout (node-granular masks)
-    DevMem incptr, inc;             // node -> (element, local node) incidence (node-centric assembly)
+    DevMem incptr, inc;             // node -> (element, local node) incidence (tet gradient sums)
+    DevMem teg, teb;                // tet element rows between the two assembly phases
+    DevMem tsptr, tsidx, tqslice;   // tet slot -> element blocks (pattern.h)
     DevMem vold;                    // velocity before the last step (second-order predictor)
     // two-level preconditioner (options.coarse_nodes > 0)
     int coarse_built_for = 0;       // coarse_nodes of the built space (0: none)
@@ -2205,6 +2304,9 @@ void ensure_pattern(sors_ctx* c, double dt) {
         if (c->ed == 4) {
             upload(c->incptr, c->pat.incptr.data(), c->pat.incptr.size(), c->stream);
             upload(c->inc, c->pat.inc.data(), c->pat.inc.size(), c->stream);
+            upload(c->tsptr, c->pat.sptr.data(), c->pat.sptr.size(), c->stream);
+            upload(c->tsidx, c->pat.sidx.data(), std::max<size_t>(c->pat.sidx.size(), 1), c->stream);
+            upload(c->tqslice, c->pat.qslice.data(), std::max<size_t>(c->pat.qslice.size(), 1), c->stream);
         }
     } else {
         upload(c->fullmap, c->pat.fullmap.data(), c->pat.fullmap.size(), c->stream);
@@ -2288,23 +2390,45 @@ void contact_lag(sors_ctx* c) {
     HIPX(hipGetLastError());
 }
 
-// element kernel launch; with e0/e1 the dispatch records the two events
-// the node-centric tet kernel writes every gradient entry and every value of
-// the sliced layout itself (no fills, no atomics)
+// tet4 on the sliced layout: the two-phase assembly writes every gradient
+// entry and every value of the layout itself (no fills, no atomics)
 bool node_assembly(const sors_ctx* c) { return c->block() && c->ed == 4 && c->material == SORS_NEOHOOKEAN; }
 
+// buffers between the two tet assembly phases (hbytes: bytes per Hessian value, 0: gradient only)
+void ensure_tet_rows(sors_ctx* c, size_t hbytes) {
+    if (!node_assembly(c)) return;
+    ensure(c->teg, sizeof(double) * kTetRowDoubles * (size_t)c->E);
+    if (hbytes) ensure(c->teb, hbytes * (hbytes == 4 ? tet_blk<float>() : tet_blk<double>()) * kTetRowBlocks * (size_t)c->E);
+}
+
+// element kernel launch; with e0/e1 the dispatches record the two events
 template <typename HT, int MODE>
 void launch_elem(sors_ctx* c, const ElemArgs& a, int project, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     if (MODE != 2 && node_assembly(c)) {
-        const int lanes = c->pat.nslices * kSlice;
-        const dim3 grid(grid_for(lanes, kBlock));
-        void (*k)(ElemArgs, int, const int*, const int*) =
-            project ? k_tet_node<HT, 1, MODE> : k_tet_node<HT, 0, MODE>;
-        if (e0)
-            hipExtLaunchKernelGGL(k, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a, c->pat.nslices,
-                                  (const int*)c->incptr.as<int>(), (const int*)c->inc.as<int>());
-        else
-            k<<<grid, kBlock, 0, c->stream>>>(a, c->pat.nslices, c->incptr.as<int>(), c->inc.as<int>());
+        if (c->E == 0) {
+            fill<double>(c->gint.as<double>(), c->n, 0.0, c->stream);
+            if (MODE == 0) fill<HT>((HT*)c->vals.p, c->nvals, (HT)0, c->stream);
+            return;
+        }
+        // element rows (phase 1), then the per-slot / per-node sums into the layout (phase 2)
+        ensure_tet_rows(c, MODE == 0 ? sizeof(HT) : 0);
+        double* eg = c->teg.as<double>();
+        HT* eb = MODE == 0 ? (HT*)c->teb.p : nullptr;
+        const dim3 ge(grid_for(c->E, kBlock));
+        void (*k1)(ElemArgs, double*, HT*) = project ? k_tet_elem<HT, 1, MODE> : k_tet_elem<HT, 0, MODE>;
+        const int* incptr = c->incptr.as<int>();
+        const int* inc = c->inc.as<int>();
+        hipExtLaunchKernelGGL(k1, ge, dim3(kBlock), 0, c->stream, e0, nullptr, 0, a, eg, eb);
+        if (MODE == 0) {
+            const int nq = c->pat.sbase[c->pat.nslices] / kSlice;
+            hipExtLaunchKernelGGL(k_tet_slots<HT>, dim3(grid_for((long long)nq * kSlice, kBlock)), dim3(kBlock), 0,
+                                  c->stream, nullptr, e1, 0, a, nq, (const int*)c->tqslice.as<int>(),
+                                  (const int*)c->tsptr.as<int>(), (const int*)c->tsidx.as<int>(), (const HT*)eb,
+                                  (const int*)c->selfslot.as<int>(), incptr, inc, (const double*)eg);
+        } else {
+            hipExtLaunchKernelGGL(k_tet_gather, dim3(grid_for(c->N, kBlock)), dim3(kBlock), 0, c->stream, nullptr,
+                                  e1, 0, a, incptr, inc, (const double*)eg);
+        }
         HIPX(hipGetLastError());
         return;
     }
@@ -2948,7 +3072,7 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->pw,     &c->px,    &c->partials, &c->scal, &c->pcgst,   &c->pcgctl,  &c->pcgpart,
                            &c->flag,   &c->sbase, &c->bcol,    &c->selfslot, &c->csurf, &c->lagN,
                            &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
-                           &c->incptr, &c->inc,  &c->vold,     &c->cdn,    &c->captr,   &c->canodes, &c->crot,
+                           &c->incptr, &c->inc,  &c->teg,      &c->teb,    &c->tsptr,   &c->tsidx,   &c->tqslice, &c->vold,     &c->cdn,    &c->captr,   &c->canodes, &c->crot,
                            &c->cpairI, &c->cpairJ, &c->cpchunk, &c->cgptr, &c->cci,     &c->ccj,     &c->ccbase,
                            &c->cE,     &c->cEinv, &c->crc,     &c->cX,      &c->cgpart,  &c->cpart,   &c->cinfo,
                            &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA};
@@ -2961,7 +3085,10 @@ double ctx_bytes(const sors_ctx* c) {
 // node-block slices padded to 24 (tet) / 32 (hex) blocks per node)
 double estimate_bytes(int N, int E, int ed) {
     const double n = 3.0 * N, nblk = (double)N * (ed == 4 ? 24 : 32);
-    return n * (19 * 8 + 4) + 72.0 * N + nblk * (9 * 8 + 4 + 9 * 4) + (double)E * (ed * 4 + 16 + ed * ed) +
+    // two-phase tet assembly: element rows (16 FP64 + 16 padded FP64 blocks at
+    // worst) and the slot lists (16 entries per element, one pointer per slot)
+    const double rows = ed == 4 ? (double)E * (16 * 8 + 10 * 16 * 8 + 16 * 4) + 4.0 * nblk : 0.0;
+    return n * (19 * 8 + 4) + 72.0 * N + nblk * (9 * 8 + 4 + 9 * 4) + (double)E * (ed * 4 + 16 + ed * ed) + rows +
            8.0 * (5 * kMaxRedBlocks + 4 * kMaxRedBlocks) + (1 << 20);
 }
 
@@ -3031,6 +3158,8 @@ void srv_try_admit(sors_server* s) {  // mu held
                 try {
                     if (!r->v0.empty()) upload(c->vprev, r->v0.data(), (size_t)c->n, c->stream);
                     ensure_pattern(c, r->dt);
+                    // allocated here so that the measured size below includes them
+                    ensure_tet_rows(c, c->opt.precision == SORS_MIXED ? sizeof(float) : sizeof(double));
                     sync(c);
                 } catch (const HipError& e) {
                     rc = SORS_ERR_HIP;
