@@ -94,8 +94,11 @@ typedef struct {
                              node aggregates of about this many nodes (sliced
                              node-block layout only); 0: block-Jacobi alone */
     int coarse_every;     /* coarse operator refreshed on every k-th tangent
-                             (<= 1: every tangent); in between, PCG uses the
-                             last one, an older tangent's Galerkin product */
+                             (<= 1: every tangent, factorised before the
+                             solve); with k > 1 a refresh is factorised on a
+                             second stream while the solve runs with the
+                             previous inverse, and the next tangent installs
+                             it.  PCG always runs with an SPD preconditioner */
 } sors_options;
 
 typedef struct {

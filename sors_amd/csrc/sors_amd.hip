@@ -2213,6 +2213,9 @@ struct sors_ctx {
     int coarse_built_for = 0;       // coarse_nodes of the built space (0: none)
     int coarse_gen = 0;             // bumped on every rebuild (captured graphs hold its buffers)
     long long coarse_tangents = 0;  // tangents seen since the coarse space was built
+    hipStream_t side = nullptr;     // rocSOLVER work (default priority, like the main stream)
+    hipEvent_t evG = nullptr, evR = nullptr;  // E assembled / factorisation done
+    bool refresh_pending = false;   // a factorisation on the side stream not yet collected
     const void* coarse_pat = nullptr;
     bool coarse_ready = false;      // E^-1 valid for the current matrix
     int nagg = 0, nc = 0, npairs = 0, ngchunks = 0;
@@ -2511,6 +2514,8 @@ CoarseArgs coarse_args(sors_ctx* c) {
 void ensure_coarse(sors_ctx* c) {
     const int want = (c->opt.coarse_nodes > 0 && c->block()) ? c->opt.coarse_nodes : 0;
     if (want == c->coarse_built_for && (want == 0 || c->coarse_pat == c->vals.p)) return;
+    if (c->side) HIPX(hipStreamSynchronize(c->side));  // no factorisation in flight on the old buffers
+    c->refresh_pending = false;
     c->coarse_built_for = 0;
     c->coarse_ready = false;
     if (want == 0) return;
@@ -2548,10 +2553,15 @@ void ensure_coarse(sors_ctx* c) {
     ensure(c->cgpart, sizeof(double) * 36 * std::max(c->ngchunks, 1));
     ensure(c->cpart, sizeof(double) * 2 * kMaxRedBlocks);
     ensure(c->cinfo, sizeof(int));
+    if (!c->side) {
+        HIPX(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIPX(hipEventCreateWithFlags(&c->evG, hipEventDisableTiming));
+        HIPX(hipEventCreateWithFlags(&c->evR, hipEventDisableTiming));
+    }
     if (!c->blas) {
         if (rocblas_create_handle(&c->blas) != rocblas_status_success) throw HipError{"rocblas_create_handle failed"};
     }
-    if (rocblas_set_stream(c->blas, st) != rocblas_status_success) throw HipError{"rocblas_set_stream failed"};
+    if (rocblas_set_stream(c->blas, c->side) != rocblas_status_success) throw HipError{"rocblas_set_stream failed"};
     sync(c);
     c->coarse_built_for = want;
     c->coarse_pat = c->vals.p;
@@ -2559,8 +2569,49 @@ void ensure_coarse(sors_ctx* c) {
     ++c->coarse_gen;
 }
 
-// E = Z^T A Z and its inverse (rocSOLVER Cholesky factorisation and solves);
-// coarse_ready on success
+// rocSOLVER work runs on the context's second stream (default priority):
+// factorisation and triangular solves, X = E^-1 in cX when it ends (evR)
+void coarse_factor_async(sors_ctx* c) {
+    const int nc = c->nc;
+    const size_t nc2 = (size_t)nc * nc;
+    HIPX(hipEventRecord(c->evG, c->stream));  // E assembled
+    HIPX(hipStreamWaitEvent(c->side, c->evG, 0));
+    if (rocsolver_dpotrf(c->blas, rocblas_fill_lower, nc, c->cE.as<double>(), nc, c->cinfo.as<int>()) !=
+        rocblas_status_success)
+        throw HipError{"rocsolver_dpotrf failed"};
+    // E X = I with the Cholesky factor: two GEMM-rich triangular solves with nc right-hand sides
+    const unsigned gsq = (unsigned)((nc2 + kBlock - 1) / kBlock);
+    k_coarse_eye<<<gsq, kBlock, 0, c->side>>>(nc, c->cX.as<double>());
+    if (rocsolver_dpotrs(c->blas, rocblas_fill_lower, nc, nc, c->cE.as<double>(), nc, c->cX.as<double>(), nc) !=
+        rocblas_status_success)
+        throw HipError{"rocsolver_dpotrs failed"};
+    HIPX(hipGetLastError());
+    HIPX(hipEventRecord(c->evR, c->side));
+    c->refresh_pending = true;
+}
+
+// wait for the factorisation in flight; on success the main stream installs
+// E^-1 = (X + X^T)/2 and the coarse correction is on
+void coarse_collect(sors_ctx* c) {
+    if (!c->refresh_pending) return;
+    c->refresh_pending = false;
+    int h_info = 1;
+    HIPX(hipMemcpyAsync(&h_info, c->cinfo.p, sizeof(int), hipMemcpyDeviceToHost, c->side));
+    HIPX(hipStreamSynchronize(c->side));
+    if (h_info != 0) return;  // E not positive definite: the previous inverse (if any) stays
+    HIPX(hipStreamWaitEvent(c->stream, c->evR, 0));
+    const size_t nc2 = (size_t)c->nc * c->nc;
+    k_coarse_symmetrize<<<(unsigned)((nc2 + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(
+        c->nc, c->cX.as<double>(), c->cEinv.as<double>());
+    HIPX(hipGetLastError());
+    c->coarse_ready = true;
+}
+
+// E = Z^T A Z and its inverse (rocSOLVER Cholesky factorisation and solves).
+// The first refresh, and every refresh when coarse_every <= 1, completes
+// before the solve.  Later refreshes (coarse_every > 1) factorise on the
+// second stream while the solve runs with the previous inverse; the next
+// tangent installs the new one.
 template <typename HT>
 void coarse_setup(sors_ctx* c) {
     ensure_coarse(c);
@@ -2568,9 +2619,10 @@ void coarse_setup(sors_ctx* c) {
         c->coarse_ready = false;
         return;
     }
+    coarse_collect(c);
     const long long k = c->coarse_tangents++;
-    if (c->coarse_ready && c->opt.coarse_every > 1 && k % c->opt.coarse_every != 0) return;  // lagged operator
-    c->coarse_ready = false;
+    const bool lagged = c->opt.coarse_every > 1;
+    if (c->coarse_ready && lagged && k % c->opt.coarse_every != 0) return;  // lagged operator
     const int nc = c->nc;
     const size_t nc2 = (size_t)nc * nc;
     fill<double>(c->cE.as<double>(), (long long)nc2, 0.0, c->stream);
@@ -2583,23 +2635,10 @@ void coarse_setup(sors_ctx* c) {
             c->cgpart.as<double>(), c->cE.as<double>());
     }
     HIPX(hipGetLastError());
-    int* info = c->cinfo.as<int>();
-    if (rocsolver_dpotrf(c->blas, rocblas_fill_lower, nc, c->cE.as<double>(), nc, info) != rocblas_status_success)
-        throw HipError{"rocsolver_dpotrf failed"};
-    int h_info = 0;
-    HIPX(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    sync(c);
-    if (h_info != 0) return;  // not positive definite: block-Jacobi alone for this solve
-    // E X = I with the Cholesky factor (two GEMM-rich triangular solves with nc
-    // right-hand sides), then the exactly symmetric average
-    const unsigned gsq = (unsigned)((nc2 + kBlock - 1) / kBlock);
-    k_coarse_eye<<<gsq, kBlock, 0, c->stream>>>(nc, c->cX.as<double>());
-    if (rocsolver_dpotrs(c->blas, rocblas_fill_lower, nc, nc, c->cE.as<double>(), nc, c->cX.as<double>(), nc) !=
-        rocblas_status_success)
-        throw HipError{"rocsolver_dpotrs failed"};
-    k_coarse_symmetrize<<<gsq, kBlock, 0, c->stream>>>(nc, c->cX.as<double>(), c->cEinv.as<double>());
-    HIPX(hipGetLastError());
-    c->coarse_ready = true;
+    coarse_factor_async(c);
+    if (c->coarse_ready && lagged) return;  // overlapped with this solve
+    c->coarse_ready = false;
+    coarse_collect(c);  // the solve waits for this one
 }
 
 template <typename HT>
@@ -3714,7 +3753,11 @@ void sors_destroy(sors_ctx* c) {
     for (hipEvent_t e : c->pev) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; ++k)
         if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->blas) (void)rocblas_destroy_handle(c->blas);
+    if (c->evG) (void)hipEventDestroy(c->evG);
+    if (c->evR) (void)hipEventDestroy(c->evR);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
     delete c;

@@ -65,14 +65,16 @@ def test_coarse_aggregates_wider_than_a_workgroup(cn):
     assert its[cn] < its[0]
 
 
-@pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
-def test_coarse_cantilever_trajectory(orc, precision):
+@pytest.mark.parametrize("precision,every", [(native.FP64, 1), (native.MIXED, 1), (native.MIXED, 3)])
+def test_coarse_cantilever_trajectory(orc, precision, every):
     """20x4x4 Kuhn beam, 200 steps: every state within 1e-6 of the
     bounding-box diagonal of the FP64 oracle; fewer PCG iterations in total
-    than block-Jacobi alone."""
+    than block-Jacobi alone.  every = 3: the coarse operator is refreshed on
+    every third tangent, factorised on the second stream while the solve
+    runs with the previous inverse."""
     sc = mesh.cantilever_small()
     o = orc.System.from_scene(sc, method=orc.NEWTON_LS)
-    r = native.Rollout.from_scene(sc, precision=precision, pcg_rtol=1e-6, coarse_nodes=24)
+    r = native.Rollout.from_scene(sc, precision=precision, pcg_rtol=1e-6, coarse_nodes=24, coarse_every=every)
     b = native.Rollout.from_scene(sc, precision=precision, pcg_rtol=1e-6)
     q = qo = qb = sc.X.reshape(-1).copy()
     worst = 0.0
@@ -89,18 +91,20 @@ def test_coarse_cantilever_trajectory(orc, precision):
     assert it_c < it_b
 
 
-@pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
-def test_coarse_graph_replay_matches_direct_launches(precision):
+@pytest.mark.parametrize("precision,every", [(native.FP64, 1), (native.MIXED, 1), (native.MIXED, 2)])
+def test_coarse_graph_replay_matches_direct_launches(precision, every):
     """The coarse kernels inside captured PCG chunks: graph replays and
-    direct launches agree bit for bit."""
+    direct launches agree bit for bit (every = 2: with the factorisation
+    overlapped on the second stream as well)."""
     sc = mesh.beam(nx=12, ny=3, nz=3, e_range=(5e5, 5e6), seed=2)
-    a = native.Rollout.from_scene(sc, use_graph=0, pcg_check_every=16, precision=precision, coarse_nodes=16)
-    b = native.Rollout.from_scene(sc, use_graph=1, pcg_check_every=16, precision=precision, coarse_nodes=16)
+    kw = dict(pcg_check_every=16, precision=precision, coarse_nodes=16, coarse_every=every)
+    a = native.Rollout.from_scene(sc, use_graph=0, **kw)
+    b = native.Rollout.from_scene(sc, use_graph=1, **kw)
     qa = qb = sc.X.reshape(-1).copy()
-    for _ in range(4):
+    for _ in range(6):
         qa, sa = a.step(qa, sc.dt)
         qb, sb = b.step(qb, sc.dt)
-        say("coarse graph replay", precision, sa.pcg_iters, sb.pcg_iters)
+        say("coarse graph replay", precision, every, sa.pcg_iters, sb.pcg_iters)
         assert sa.pcg_iters == sb.pcg_iters
         assert np.array_equal(qa, qb)
 
