@@ -225,7 +225,7 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
         // node -> (element, local index) incidence for the node-centric assembly
         P.incptr = std::move(iptr);
         P.inc = std::move(inc);
-        if (ed == 4) {
+        if ((ed == 4 || ed == 8) && (long long)ed * ed * E <= 0x7fffffffLL) {
             // slot -> element blocks (counting sort by slot position; elements ascending)
             const long long S = P.sbase[ns];
             P.qslice.assign((size_t)(S / kSlice), 0);
@@ -233,14 +233,14 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
                 for (int q = P.sbase[sl] / kSlice; q < P.sbase[sl + 1] / kSlice; ++q) P.qslice[q] = sl;
             P.sptr.assign((size_t)S + 1, 0);
             auto slot_of = [&](int e, int la, int lb) -> long long {
-                const int rk = P.rank[(size_t)(la * 4 + lb) * E + e];
+                const int rk = P.rank[(size_t)(la * ed + lb) * E + e];
                 if (rk == kRankSkip) return -1;
-                const int v = conn[(size_t)e * 4 + la];
+                const int v = conn[(size_t)e * ed + la];
                 return (long long)P.sbase[v >> 6] + (long long)kSlice * rk + (v & (kSlice - 1));
             };
             for (int e = 0; e < E; ++e)
-                for (int la = 0; la < 4; ++la)
-                    for (int lb = 0; lb < 4; ++lb) {
+                for (int la = 0; la < ed; ++la)
+                    for (int lb = 0; lb < ed; ++lb) {
                         const long long p = slot_of(e, la, lb);
                         if (p >= 0) P.sptr[p + 1]++;
                     }
@@ -248,10 +248,10 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
             P.sidx.assign((size_t)P.sptr[S], 0);
             std::vector<int> fill(P.sptr.begin(), P.sptr.end() - 1);
             for (int e = 0; e < E; ++e)
-                for (int la = 0; la < 4; ++la)
-                    for (int lb = 0; lb < 4; ++lb) {
+                for (int la = 0; la < ed; ++la)
+                    for (int lb = 0; lb < ed; ++lb) {
                         const long long p = slot_of(e, la, lb);
-                        if (p >= 0) P.sidx[fill[p]++] = 16 * e + 4 * la + lb;
+                        if (p >= 0) P.sidx[fill[p]++] = ed * ed * e + ed * la + lb;
                     }
         }
     } else {

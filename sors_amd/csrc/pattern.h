@@ -80,10 +80,11 @@ struct HostPattern {
     long long nblocks = 0;          // 3x3 node blocks holding at least one CSR entry
     std::vector<int> incptr;        // N + 1: node -> incident (element, local node) list
     std::vector<int> inc;           // e * ed + local index, element order per node
-    // tet4 only: element blocks summed into each slot of the sliced layout.
-    // Slot position p = sbase[s] + 64 k + lane (the index of bcol) receives
-    // blocks sidx[sptr[p] .. sptr[p+1]), each 16 e + 4 la + lb (element e's
-    // block of local nodes (la, lb)), in ascending element order.
+    // tet4 / hex8: element blocks summed into each slot of the sliced layout
+    // (empty when ed^2 E exceeds int32).  Slot position p = sbase[s] + 64 k +
+    // lane (the index of bcol) receives blocks sidx[sptr[p] .. sptr[p+1]),
+    // each ed^2 e + ed la + lb (element e's block of local nodes (la, lb)),
+    // in ascending element order.
     std::vector<int> sptr;          // sbase[nslices] + 1
     std::vector<int> sidx;
     std::vector<int> qslice;        // sbase[nslices] / 64: slice of each group of 64 slot positions

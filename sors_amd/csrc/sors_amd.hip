@@ -417,20 +417,22 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
 }
 
 
-// ---- tet4, two-phase assembly on the sliced node-block layout ------------
-// Phase 1 (k_tet_elem): one lane per element evaluates the element once: its
-// four FP64 gradient rows V0 P g_a (each padded with the packed ranks of its
-// row) and, in MODE 0, its sixteen 3x3 blocks (la, lb), stored element-major
-// with each block padded to a 16-byte multiple.
-// Phase 2, MODE 0 (k_tet_slots): one lane per slot of the sliced layout sums
+// ---- tet4 / hex8, two-phase assembly on the sliced node-block layout ------
+// Phase 1 evaluates every element once and stores, element-major:
+//   eg[4 (ed e + a) + r]   the FP64 gradient row of local node a (r < 3; one
+//                          double of padding so a row is two 16-byte loads),
+//   eb[blk (ed^2 e + ed a + b) + i]  its 3x3 block (a, b), i < 9, padded to
+//                          blk values (a 16-byte multiple).
+// tet4: k_tet_elem, one lane per element.  hex8: k_hex_grad (one lane per
+// element, FP64 gradient rows) and k_hex_rows (one lane per element row a,
+// its eight blocks summed over the eight Gauss points).
+// Phase 2, MODE 0 (k_slot_sums): one lane per slot of the sliced layout sums
 // the element blocks listed for that slot (pattern.h: sptr / sidx, ascending
 // element order) in registers and stores the block once; the lane of a
-// node's own slot also sums the node's gradient rows.  MODE 1 (k_tet_gather):
-// one lane per node sums its gradient rows.  No atomics, no read-modify-write
-// of the layout, every value written by exactly one lane, fixed-order sums.
-// MODE 0: gradient + Hessian; MODE 1: gradient only.
-constexpr int kTetRowBlocks = 16;   // stored 3x3 blocks per element
-constexpr int kTetRowDoubles = 16;  // per element: 4 x (gradient row, packed ranks)
+// node's own slot also sums the node's gradient rows.  MODE 1
+// (k_grad_gather): one lane per node sums its gradient rows.  No atomics, no
+// read-modify-write of the layout, every value written by exactly one lane,
+// fixed-order sums.  MODE 0: gradient + Hessian; MODE 1: gradient only.
 // stored values per block: 9 padded to a multiple of 16 bytes
 template <typename HT>
 constexpr int tet_blk() { return sizeof(HT) == 4 ? 12 : 10; }
@@ -492,20 +494,12 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
             for (int r = 0; r < 3; ++r)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) P(r, c) += sa * Ff[r] * fa[c];
-        double* ge = eg + (size_t)kTetRowDoubles * e;
+        double* ge = eg + 4 * (4 * (size_t)e);
 #pragma unroll
-        for (int la = 0; la < 4; ++la) {
+        for (int la = 0; la < 4; ++la)
 #pragma unroll
             for (int r = 0; r < 3; ++r)
                 ge[4 * la + r] = V0 * (P(r, 0) * g[la][0] + P(r, 1) * g[la][1] + P(r, 2) * g[la][2]);
-            // the four ranks of row la (slots of blocks (la, lb) in node la's row), packed
-            long long rk = 0;
-            if (MODE == 0)
-#pragma unroll
-                for (int lb = 0; lb < 4; ++lb)
-                    rk |= (long long)a.rank[(size_t)(la * 4 + lb) * a.E + e] << (8 * lb);
-            ge[4 * la + 3] = __longlong_as_double(rk);
-        }
     }
     if (MODE == 1) return;
     const HT w = (HT)(V0 * a.hscale);
@@ -532,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
             for (int r = 0; r < 3; ++r)
                 h[k][r] = (HT)(p.FiT(r, 0) * g[k][0] + p.FiT(r, 1) * g[k][1] + p.FiT(r, 2) * g[k][2]);
     }
-    HT* be = eb + (size_t)tet_blk<HT>() * kTetRowBlocks * e;
+    HT* be = eb + (size_t)tet_blk<HT>() * 16 * e;
 #pragma unroll
     for (int la = 0; la < 4; ++la)
 #pragma unroll
@@ -555,9 +549,10 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
         }
 }
 
-// sum of node v's gradient rows over its incident elements (element order)
-__device__ __forceinline__ void tet_node_gradient(const ElemArgs& a, int v, const int* incptr, const int* inc,
-                                                  const double* eg) {
+// sum of node v's gradient rows over its incident (element, local node)
+// codes ed e + a, in element order
+__device__ __forceinline__ void node_gradient(const ElemArgs& a, int v, const int* incptr, const int* inc,
+                                              const double* eg) {
     bool kept[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) kept[r] = dof_kept(a, 3 * v + r);
@@ -566,44 +561,44 @@ __device__ __forceinline__ void tet_node_gradient(const ElemArgs& a, int v, cons
     int j = j0;
     for (; j + 4 <= j1; j += 4) {  // four rows in flight, summed in list order
         int code[4];
-        double2 g01[4], g2r[4];
+        double2 g01[4], g2[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) code[u] = inc[j + u];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const double2* ge = (const double2*)(eg + (size_t)kTetRowDoubles * (code[u] >> 2) + 4 * (code[u] & 3));
+            const double2* ge = (const double2*)(eg + 4LL * code[u]);
             g01[u] = ge[0];
-            g2r[u] = ge[1];
+            g2[u] = ge[1];
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (kept[0]) gacc[0] += g01[u].x;
             if (kept[1]) gacc[1] += g01[u].y;
-            if (kept[2]) gacc[2] += g2r[u].x;
+            if (kept[2]) gacc[2] += g2[u].x;
         }
     }
     for (; j < j1; ++j) {
-        const int code = inc[j];
-        const double2* ge = (const double2*)(eg + (size_t)kTetRowDoubles * (code >> 2) + 4 * (code & 3));
-        const double2 g01 = ge[0], g2r = ge[1];
+        const double2* ge = (const double2*)(eg + 4LL * inc[j]);
+        const double2 g01 = ge[0], g2 = ge[1];
         if (kept[0]) gacc[0] += g01.x;
         if (kept[1]) gacc[1] += g01.y;
-        if (kept[2]) gacc[2] += g2r.x;
+        if (kept[2]) gacc[2] += g2.x;
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) a.gint[3 * v + r] = gacc[r];
 }
 
-__global__ __launch_bounds__(kBlock) void k_tet_gather(ElemArgs a, const int* incptr, const int* inc, const double* eg) {
+__global__ __launch_bounds__(kBlock) void k_grad_gather(ElemArgs a, const int* incptr, const int* inc,
+                                                        const double* eg) {
     const int v = xcd_block() * blockDim.x + threadIdx.x;
     if (v >= a.N) return;
-    tet_node_gradient(a, v, incptr, inc, eg);
+    node_gradient(a, v, incptr, inc, eg);
 }
 
 // one lane per slot position p = sbase[s] + 64 k + lane (groups of 64 = one
 // wavefront = slot k of slice s)
 template <typename HT>
-__global__ __launch_bounds__(kBlock) void k_tet_slots(ElemArgs a, int nq, const int* qslice, const int* sptr,
+__global__ __launch_bounds__(kBlock) void k_slot_sums(ElemArgs a, int nq, const int* qslice, const int* sptr,
                                                       const int* sidx, const HT* eb, const int* selfslot,
                                                       const int* incptr, const int* inc, const double* eg) {
     const int p = xcd_block() * blockDim.x + threadIdx.x;
@@ -652,45 +647,49 @@ __global__ __launch_bounds__(kBlock) void k_tet_slots(ElemArgs a, int nq, const 
 #pragma unroll
     for (int i = 0; i < 9; ++i) vals[vb + (long long)kSlice * i] = acc[i];
     const int v = s * kSlice + lane;
-    if (v < a.N && k == selfslot[v]) tet_node_gradient(a, v, incptr, inc, eg);
+    if (v < a.N && k == selfslot[v]) node_gradient(a, v, incptr, inc, eg);
 }
 
 // ---- hex8: one element per lane --------------------------------------------
-// Reference cube corners (hexahedron.cpp:171-179) and 5-tet volume split (:106-130)
+// Reference cube corners (hexahedron.cpp:171-179)
 __constant__ signed char kHexXi[8][3] = {{1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, -1},
                                          {1, -1, 1},  {1, 1, 1},  {-1, 1, 1},  {-1, -1, 1}};
-__constant__ unsigned char kHexTets[5][4] = {{0, 1, 3, 4}, {1, 2, 3, 6}, {1, 4, 5, 6}, {3, 4, 6, 7}, {1, 3, 4, 6}};
 
 // G_s = hatG_s hatF_s^-1 with hatF_s = X^T hatG_s (hexahedron.cpp:190-205)
 template <typename T>
 __device__ __forceinline__ void hex_shape_grads(const double X[8][3], int s, T G[8][3]) {
     const double gq = 0.57735026918962576451;  // 1/sqrt(3)
     const double qs0 = gq * kHexXi[s][0], qs1 = gq * kHexXi[s][1], qs2 = gq * kHexXi[s][2];
-    double hg[8][3];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    // reference-cube gradient of shape function i at Gauss point s (recomputed
+    // where used: a few flops, and no 8x3 FP64 array kept live)
+    auto hg = [&](int i, double h[3]) {
         const double n0 = kHexXi[i][0], n1 = kHexXi[i][1], n2 = kHexXi[i][2];
-        hg[i][0] = 0.125 * (n0 * (1 + n1 * qs1) * (1 + n2 * qs2));
-        hg[i][1] = 0.125 * (n1 * (1 + n0 * qs0) * (1 + n2 * qs2));
-        hg[i][2] = 0.125 * (n2 * (1 + n0 * qs0) * (1 + n1 * qs1));
-    }
+        h[0] = 0.125 * (n0 * (1 + n1 * qs1) * (1 + n2 * qs2));
+        h[1] = 0.125 * (n1 * (1 + n0 * qs0) * (1 + n2 * qs2));
+        h[2] = 0.125 * (n2 * (1 + n0 * qs0) * (1 + n1 * qs1));
+    };
     M3<double> hF;
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 9; ++k) hF.m[k] = 0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            double acc = 0;
+    for (int i = 0; i < 8; ++i) {
+        double h[3];
+        hg(i, h);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) acc += X[i][r] * hg[i][c];
-            hF(r, c) = acc;
-        }
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) hF(r, c) += X[i][r] * h[c];
+    }
     const M3<double> C = cofactor3(hF);
     const double det = hF.m[0] * C.m[0] + hF.m[1] * C.m[1] + hF.m[2] * C.m[2];
     // hF^-1 (k, c) = C(c, k) / det
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < 8; ++i) {
+        double h[3];
+        hg(i, h);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) G[i][c] = (T)((hg[i][0] * C(c, 0) + hg[i][1] * C(c, 1) + hg[i][2] * C(c, 2)) / det);
+        for (int c = 0; c < 3; ++c) G[i][c] = (T)((h[0] * C(c, 0) + h[1] * C(c, 1) + h[2] * C(c, 2)) / det);
+    }
 }
 
 __device__ __forceinline__ double tet_vol(const double* a, const double* b, const double* c, const double* d) {
@@ -698,6 +697,18 @@ __device__ __forceinline__ double tet_vol(const double* a, const double* b, cons
     const double ac0 = c[0] - a[0], ac1 = c[1] - a[1], ac2 = c[2] - a[2];
     const double ad0 = d[0] - a[0], ad1 = d[1] - a[1], ad2 = d[2] - a[2];
     return fabs(ab0 * (ac1 * ad2 - ac2 * ad1) + ab1 * (ac2 * ad0 - ac0 * ad2) + ab2 * (ac0 * ad1 - ac1 * ad0)) / 6.0;
+}
+
+// V0 of a hex: the five-tet split {0,1,3,4} {1,2,3,6} {1,4,5,6} {3,4,6,7}
+// {1,3,4,6} (hexahedron.cpp:106-130), summed in that order.  Literal
+// indices keep X in registers (a table lookup would index it at run time).
+__device__ __forceinline__ double hex_volume(const double X[8][3]) {
+    double V0 = tet_vol(X[0], X[1], X[3], X[4]);
+    V0 += tet_vol(X[1], X[2], X[3], X[6]);
+    V0 += tet_vol(X[1], X[4], X[5], X[6]);
+    V0 += tet_vol(X[3], X[4], X[6], X[7]);
+    V0 += tet_vol(X[1], X[3], X[4], X[6]);
+    return V0;
 }
 
 template <typename HT, int PROJ, int MODE>
@@ -714,10 +725,7 @@ __global__ __launch_bounds__(128) void k_hex(ElemArgs a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) u[k][c] = xk[c] - X[k][c];
     }
-    double V0 = 0;
-#pragma unroll
-    for (int t = 0; t < 5; ++t)
-        V0 += tet_vol(X[kHexTets[t][0]], X[kHexTets[t][1]], X[kHexTets[t][2]], X[kHexTets[t][3]]);
+    const double V0 = hex_volume(X);
     const double mu = a.mu[e], lam = a.lam[e];
     const double wq = V0 * 0.125;  // quadrature weight 1/8 times V0 (neoHookean.cpp:26-27)
     // gradient and energy, FP64
@@ -822,6 +830,170 @@ __global__ __launch_bounds__(128) void k_hex(ElemArgs a) {
     }
 }
 
+// ---- hex8 phase 1 of the two-phase assembly ------------------------------
+template <typename UT>
+__device__ __forceinline__ double hex_load(const ElemArgs& a, int e, double X[8][3], UT u[8][3]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int nd = a.conn[(size_t)k * a.E + e];
+        double xk[3];
+        gather_node(a, nd, X[k], xk);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[k][c] = (UT)(xk[c] - X[k][c]);
+    }
+    const double V0 = hex_volume(X);
+    return V0 * 0.125;  // quadrature weight 1/8 times V0 (neoHookean.cpp:26-27)
+}
+
+// FP64 gradient rows of one element (same sums as k_hex)
+__global__ __launch_bounds__(128) void k_hex_grad(ElemArgs a, double* eg) {
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    double X[8][3], u[8][3];
+    const double wq = hex_load(a, e, X, u);
+    const double mu = a.mu[e], lam = a.lam[e];
+    double gr[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) gr[k][0] = gr[k][1] = gr[k][2] = 0;
+    for (int s = 0; s < 8; ++s) {
+        double G[8][3];
+        hex_shape_grads<double>(X, s, G);
+        M3<double> D;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) acc += u[k][r] * G[k][c];
+                D(r, c) = acc;
+            }
+        const NHPoint<double> p = nh_point(D);
+        if (!(p.J > 0)) atomicOr(a.flag, 1);
+        const M3<double> P = nh_stress(p, mu, lam);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) gr[k][r] += wq * (P(r, 0) * G[k][0] + P(r, 1) * G[k][1] + P(r, 2) * G[k][2]);
+    }
+    double* ge = eg + 4 * (8 * (size_t)e);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) ge[4 * k + r] = gr[k][r];
+}
+
+// Hessian rows of hex8 elements: eight lanes per element.  First lane j
+// evaluates Gauss point j (shape gradients, F, the projected 9x9 tangent, or
+// F^-T and lambda ln J - mu for the raw tangent) into LDS; then lane j builds
+// row j, its eight blocks summed over the Gauss points in order.  Each
+// Gauss-point tangent is evaluated once per element, and t_i = g_a^T S(3i+.,
+// .) is formed once per Gauss point and reused for the eight blocks.
+constexpr int kHexRowThreads = 128;  // 16 elements per workgroup
+template <typename HT, int PROJ>
+__global__ __launch_bounds__(kHexRowThreads) void k_hex_rows(ElemArgs a, HT* eb) {
+    constexpr int kEl = kHexRowThreads / 8;
+    __shared__ HT sS[kEl][8][45];  // per Gauss point: Sym9 (PROJ) or F^-T and c (raw)
+    __shared__ HT sG[kEl][8][24];  // per Gauss point: shape gradients G[k][c]
+    const int le = threadIdx.x >> 3, j = threadIdx.x & 7;
+    const int e = (xcd_block() * kHexRowThreads + threadIdx.x) >> 3;
+    const bool live = e < a.E;
+    HT w = 0;
+    if (live) {
+        double X[8][3];
+        HT u[8][3];
+        const double wq = hex_load(a, e, X, u);
+        const double mu = a.mu[e], lam = a.lam[e];
+        w = (HT)(wq * a.hscale);
+        HT G[8][3];
+        hex_shape_grads<HT>(X, j, G);
+        M3<HT> D;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                HT tt = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) tt += u[k][r] * G[k][c];
+                D(r, c) = tt;
+            }
+        const NHPoint<HT> p = nh_point(D);
+        if (PROJ) {
+            Sym9<HT> S;
+            nh_projected_tangent(D, p, (HT)mu, (HT)lam, S);
+#pragma unroll
+            for (int i = 0; i < 45; ++i) sS[le][j][i] = S.a[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) sS[le][j][i] = p.FiT.m[i];
+            sS[le][j][9] = (HT)lam * p.lnJ - (HT)mu;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sG[le][j][3 * k + c] = G[k][c];
+    }
+    __syncthreads();
+    if (!live) return;
+    const int la = j;
+    HT acc[8][9];
+#pragma unroll
+    for (int lb = 0; lb < 8; ++lb)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[lb][i] = 0;
+    for (int s = 0; s < 8; ++s) {
+        const HT* Gs = sG[le][s];
+        const HT ga[3] = {Gs[3 * la], Gs[3 * la + 1], Gs[3 * la + 2]};
+        if (PROJ) {
+            Sym9<HT> S;
+#pragma unroll
+            for (int i = 0; i < 45; ++i) S.a[i] = sS[le][s][i];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                HT t[9];  // t[(k,l)] = sum_j ga_j S(3i+j, 3k+l)
+#pragma unroll
+                for (int kl = 0; kl < 9; ++kl) {
+                    HT tacc = 0;
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj) tacc += ga[jj] * S.get(3 * i + jj, kl);
+                    t[kl] = tacc;
+                }
+#pragma unroll
+                for (int lb = 0; lb < 8; ++lb)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        HT bacc = 0;
+#pragma unroll
+                        for (int l = 0; l < 3; ++l) bacc += t[3 * k + l] * Gs[3 * lb + l];
+                        acc[lb][3 * i + k] += w * bacc;
+                    }
+            }
+        } else {
+            M3<HT> FiT;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) FiT.m[i] = sS[le][s][i];
+            const HT c = sS[le][s][9];
+            HT ha[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) ha[r] = FiT(r, 0) * ga[0] + FiT(r, 1) * ga[1] + FiT(r, 2) * ga[2];
+            const HT muh = (HT)a.mu[e], lamh = (HT)a.lam[e];
+#pragma unroll
+            for (int lb = 0; lb < 8; ++lb) {
+                const HT gb[3] = {Gs[3 * lb], Gs[3 * lb + 1], Gs[3 * lb + 2]};
+                HT hb[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) hb[r] = FiT(r, 0) * gb[0] + FiT(r, 1) * gb[1] + FiT(r, 2) * gb[2];
+                block_raw(muh, lamh, c, ga, gb, ha, hb, w, acc[lb]);
+            }
+        }
+    }
+    HT* be = eb + (size_t)tet_blk<HT>() * (64 * (size_t)e + 8 * la);
+#pragma unroll
+    for (int lb = 0; lb < 8; ++lb)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) be[tet_blk<HT>() * lb + i] = acc[lb][i];
+}
+
 // ---- element energy: absolute (diff = 0) or E(q + alpha dq) - E(q) -------
 __device__ double tet_energy_at(const ElemArgs& a, int e, const int nd[4], double alpha, bool* inv) {
     double X[4][3], x[4][3];
@@ -892,10 +1064,7 @@ __device__ double hex_energy_at(const ElemArgs& a, int e, const int nd[8], doubl
             if (alpha != 0.0) v += alpha * a.dq[3 * nd[k] + c];
             u[k][c] = v - X[k][c];
         }
-    double V0 = 0;
-#pragma unroll
-    for (int t = 0; t < 5; ++t)
-        V0 += tet_vol(X[kHexTets[t][0]], X[kHexTets[t][1]], X[kHexTets[t][2]], X[kHexTets[t][3]]);
+    const double V0 = hex_volume(X);
     double en = 0;
     for (int s = 0; s < 8; ++s) {
         double G[8][3];
@@ -2205,9 +2374,9 @@ struct sors_ctx {
     bool pat_on_device = false;
     DevMem pcgst_bench, copybuf;
     DevMem sbase, bcol, selfslot;   // sliced node-block layout (node-granular masks)
-    DevMem incptr, inc;             // node -> (element, local node) incidence (tet gradient sums)
-    DevMem teg, teb;                // tet element rows between the two assembly phases
-    DevMem tsptr, tsidx, tqslice;   // tet slot -> element blocks (pattern.h)
+    DevMem incptr, inc;             // node -> (element, local node) incidence (gradient sums)
+    DevMem teg, teb;                // element rows between the two assembly phases
+    DevMem tsptr, tsidx, tqslice;   // layout slot -> element blocks (pattern.h)
     DevMem vold;                    // velocity before the last step (second-order predictor)
     // two-level preconditioner (options.coarse_nodes > 0)
     int coarse_built_for = 0;       // coarse_nodes of the built space (0: none)
@@ -2304,7 +2473,7 @@ void ensure_pattern(sors_ctx* c, double dt) {
         upload(c->selfslot, c->pat.self_slot.data(), c->pat.self_slot.size(), c->stream);
         upload(c->diag, c->pat.bdiag.data(), c->pat.bdiag.size(), c->stream);
         c->nvals = 9LL * c->pat.sbase[c->pat.nslices];
-        if (c->ed == 4) {
+        if (!c->pat.sptr.empty()) {
             upload(c->incptr, c->pat.incptr.data(), c->pat.incptr.size(), c->stream);
             upload(c->inc, c->pat.inc.data(), c->pat.inc.size(), c->stream);
             upload(c->tsptr, c->pat.sptr.data(), c->pat.sptr.size(), c->stream);
@@ -2393,15 +2562,18 @@ void contact_lag(sors_ctx* c) {
     HIPX(hipGetLastError());
 }
 
-// tet4 on the sliced layout: the two-phase assembly writes every gradient
-// entry and every value of the layout itself (no fills, no atomics)
-bool node_assembly(const sors_ctx* c) { return c->block() && c->ed == 4 && c->material == SORS_NEOHOOKEAN; }
+// tet4 / hex8 Neo-Hookean on the sliced layout: the two-phase assembly writes
+// every gradient entry and every value of the layout itself (no fills, no atomics)
+bool node_assembly(const sors_ctx* c) {
+    return c->block() && (c->ed == 4 || c->ed == 8) && c->material == SORS_NEOHOOKEAN && !c->pat.sptr.empty();
+}
 
-// buffers between the two tet assembly phases (hbytes: bytes per Hessian value, 0: gradient only)
-void ensure_tet_rows(sors_ctx* c, size_t hbytes) {
+// element rows between the assembly phases (hbytes: bytes per Hessian value, 0: gradient only)
+void ensure_elem_rows(sors_ctx* c, size_t hbytes) {
     if (!node_assembly(c)) return;
-    ensure(c->teg, sizeof(double) * kTetRowDoubles * (size_t)c->E);
-    if (hbytes) ensure(c->teb, hbytes * (hbytes == 4 ? tet_blk<float>() : tet_blk<double>()) * kTetRowBlocks * (size_t)c->E);
+    ensure(c->teg, sizeof(double) * 4 * c->ed * (size_t)c->E);
+    if (hbytes)
+        ensure(c->teb, hbytes * (hbytes == 4 ? tet_blk<float>() : tet_blk<double>()) * c->ed * c->ed * (size_t)c->E);
 }
 
 // element kernel launch; with e0/e1 the dispatches record the two events
@@ -2414,22 +2586,31 @@ void launch_elem(sors_ctx* c, const ElemArgs& a, int project, hipEvent_t e0 = nu
             return;
         }
         // element rows (phase 1), then the per-slot / per-node sums into the layout (phase 2)
-        ensure_tet_rows(c, MODE == 0 ? sizeof(HT) : 0);
+        ensure_elem_rows(c, MODE == 0 ? sizeof(HT) : 0);
         double* eg = c->teg.as<double>();
         HT* eb = MODE == 0 ? (HT*)c->teb.p : nullptr;
-        const dim3 ge(grid_for(c->E, kBlock));
-        void (*k1)(ElemArgs, double*, HT*) = project ? k_tet_elem<HT, 1, MODE> : k_tet_elem<HT, 0, MODE>;
+        if (c->ed == 4) {
+            void (*k1)(ElemArgs, double*, HT*) = project ? k_tet_elem<HT, 1, MODE> : k_tet_elem<HT, 0, MODE>;
+            hipExtLaunchKernelGGL(k1, dim3(grid_for(c->E, kBlock)), dim3(kBlock), 0, c->stream, e0, nullptr, 0, a,
+                                  eg, eb);
+        } else {
+            hipExtLaunchKernelGGL(k_hex_grad, dim3(grid_for(c->E, 128)), dim3(128), 0, c->stream, e0, nullptr, 0, a,
+                                  eg);
+            if (MODE == 0) {
+                void (*kr)(ElemArgs, HT*) = project ? k_hex_rows<HT, 1> : k_hex_rows<HT, 0>;
+                kr<<<grid_for(8LL * c->E, kHexRowThreads), kHexRowThreads, 0, c->stream>>>(a, eb);
+            }
+        }
         const int* incptr = c->incptr.as<int>();
         const int* inc = c->inc.as<int>();
-        hipExtLaunchKernelGGL(k1, ge, dim3(kBlock), 0, c->stream, e0, nullptr, 0, a, eg, eb);
         if (MODE == 0) {
             const int nq = c->pat.sbase[c->pat.nslices] / kSlice;
-            hipExtLaunchKernelGGL(k_tet_slots<HT>, dim3(grid_for((long long)nq * kSlice, kBlock)), dim3(kBlock), 0,
+            hipExtLaunchKernelGGL(k_slot_sums<HT>, dim3(grid_for((long long)nq * kSlice, kBlock)), dim3(kBlock), 0,
                                   c->stream, nullptr, e1, 0, a, nq, (const int*)c->tqslice.as<int>(),
                                   (const int*)c->tsptr.as<int>(), (const int*)c->tsidx.as<int>(), (const HT*)eb,
                                   (const int*)c->selfslot.as<int>(), incptr, inc, (const double*)eg);
         } else {
-            hipExtLaunchKernelGGL(k_tet_gather, dim3(grid_for(c->N, kBlock)), dim3(kBlock), 0, c->stream, nullptr,
+            hipExtLaunchKernelGGL(k_grad_gather, dim3(grid_for(c->N, kBlock)), dim3(kBlock), 0, c->stream, nullptr,
                                   e1, 0, a, incptr, inc, (const double*)eg);
         }
         HIPX(hipGetLastError());
@@ -3124,9 +3305,9 @@ double ctx_bytes(const sors_ctx* c) {
 // node-block slices padded to 24 (tet) / 32 (hex) blocks per node)
 double estimate_bytes(int N, int E, int ed) {
     const double n = 3.0 * N, nblk = (double)N * (ed == 4 ? 24 : 32);
-    // two-phase tet assembly: element rows (16 FP64 + 16 padded FP64 blocks at
-    // worst) and the slot lists (16 entries per element, one pointer per slot)
-    const double rows = ed == 4 ? (double)E * (16 * 8 + 10 * 16 * 8 + 16 * 4) + 4.0 * nblk : 0.0;
+    // two-phase assembly: element rows (4 ed FP64 + ed^2 padded FP64 blocks at
+    // worst) and the slot lists (ed^2 entries per element, one pointer per slot)
+    const double rows = (double)E * (4 * ed * 8 + 10 * ed * ed * 8 + ed * ed * 4) + 4.0 * nblk;
     return n * (19 * 8 + 4) + 72.0 * N + nblk * (9 * 8 + 4 + 9 * 4) + (double)E * (ed * 4 + 16 + ed * ed) + rows +
            8.0 * (5 * kMaxRedBlocks + 4 * kMaxRedBlocks) + (1 << 20);
 }
@@ -3198,7 +3379,7 @@ void srv_try_admit(sors_server* s) {  // mu held
                     if (!r->v0.empty()) upload(c->vprev, r->v0.data(), (size_t)c->n, c->stream);
                     ensure_pattern(c, r->dt);
                     // allocated here so that the measured size below includes them
-                    ensure_tet_rows(c, c->opt.precision == SORS_MIXED ? sizeof(float) : sizeof(double));
+                    ensure_elem_rows(c, c->opt.precision == SORS_MIXED ? sizeof(float) : sizeof(double));
                     sync(c);
                 } catch (const HipError& e) {
                     rc = SORS_ERR_HIP;

@@ -103,7 +103,7 @@ def native_estimate(sc):
     """Device bytes the server reserves for a request before its pattern exists."""
     n, N, E, ed = 3 * sc.n_nodes, sc.n_nodes, sc.n_elems, sc.ed
     nblk = N * (24 if ed == 4 else 32)
-    # two-phase tet assembly: element rows and slot lists
-    rows = E * (16 * 8 + 10 * 16 * 8 + 16 * 4) + 4 * nblk if ed == 4 else 0
+    # two-phase assembly: element rows and slot lists
+    rows = E * (4 * ed * 8 + 10 * ed * ed * 8 + ed * ed * 4) + 4 * nblk
     return (n * (19 * 8 + 4) + 72 * N + nblk * (9 * 8 + 4 + 9 * 4) + E * (ed * 4 + 16 + ed * ed) + rows
             + 8 * 9 * 2048 + (1 << 20))
