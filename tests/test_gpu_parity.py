@@ -230,10 +230,12 @@ def test_graph_replay_matches_direct_launches(precision):
         assert np.array_equal(qa, qb)
 
 
-def test_node_assembly_is_deterministic():
+@pytest.mark.parametrize("element", ["tet", "hex"])
+def test_node_assembly_is_deterministic(element):
     """Two assemblies of the same state give identical gradients and
-    Hessian values (no float atomics on the tet node-block path)."""
-    sc = mesh.beam(nx=8, ny=3, nz=3, e_range=(5e5, 5e6), seed=4)
+    Hessian values (no float atomics on the node-block path: element rows,
+    then fixed-order sums per layout slot)."""
+    sc = mesh.beam(nx=8, ny=3, nz=3, element=element, e_range=(5e5, 5e6), seed=4)
     r = rollout(sc)
     q = perturbed(sc, 3e-4, seed=5)
     for prec in (native.FP64, native.MIXED):
