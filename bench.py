@@ -392,8 +392,7 @@ def main():
         print(json.dumps({"metric": METRIC, "contact": contact_leg(args, solver_opts(args))}), flush=True)
         return
     if args.serve_only:
-        opts = dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol, predict=args.predictor)
-        print(json.dumps({"metric": METRIC, "serving": serve_leg(args, opts)}), flush=True)
+        print(json.dumps({"metric": METRIC, "serving": serve_leg(args, solver_opts(args))}), flush=True)
         return
     sc = make_scene(args)
     log(f"workload {sc.name}: {sc.n_elems} elements, {sc.n_nodes} nodes; warmup {args.warmup}, steps {args.steps}")
@@ -482,8 +481,7 @@ def main():
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_leg(args)
     if args.serve > 0:
-        out["serving"] = serve_leg(args, dict(pcg_rtol=args.pcg_rtol, g_tol=args.g_tol,
-                                              predict=args.predictor))
+        out["serving"] = serve_leg(args, solver_opts(args))
     print(json.dumps(out), flush=True)
 
 

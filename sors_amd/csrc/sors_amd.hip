@@ -2045,10 +2045,8 @@ __global__ void k_coarse_symmetrize(int nc, const double* X, double* Einv) {
 #endif
 constexpr int kRestrictThreads = SORS_RESTRICT_THREADS;
 template <typename HT>
-__global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict(CoarseArgs c, const PcgCtl* ctl,
-                                                                      const HT* r) {
+__device__ void coarse_restrict_body(const CoarseArgs& c, const PcgCtl* ctl, const HT* r, int I) {
     if (ctl && ctl->done) return;
-    const int I = blockIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const int lo = c.aptr[I], hi = c.aptr[I + 1];
     for (int t = lo + threadIdx.x; t < hi; t += kRestrictThreads) {
@@ -2070,6 +2068,12 @@ __global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict(CoarseArgs
         for (int k = 0; k < 6; ++k) c.rc[6 * (size_t)I + k] = acc[k];
 }
 
+template <typename HT>
+__global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict(CoarseArgs c, const PcgCtl* ctl,
+                                                                      const HT* r) {
+    coarse_restrict_body(c, ctl, r, blockIdx.x);
+}
+
 // xc_I = rows 6I..6I+5 of E^-1 times rc, then u += Z_I xc_I over the nodes of
 // aggregate I: one 1024-thread workgroup per aggregate, so E^-1 is still read
 // once per application and the prolongation needs no second launch.  One
@@ -2081,12 +2085,11 @@ constexpr int kMaxCoarse = 6 * 512;  // coarse unknowns: nc^2 FP64 values are re
 constexpr int kApplyThreads = SORS_APPLY_THREADS;
 static_assert(kMaxCoarse / 6 <= kMaxRedBlocks, "one gamma partial per aggregate");
 template <typename HT>
-__global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(CoarseArgs c, const PcgCtl* ctl,
-                                                                const double* __restrict__ Einv, HT* u,
-                                                                double* part) {
+__device__ void coarse_apply_body(const CoarseArgs& c, const PcgCtl* ctl, const double* __restrict__ Einv, HT* u,
+                                  double* part, int I) {
     if (ctl && ctl->done) return;
     __shared__ double xs[6];
-    const int I = blockIdx.x, nc = c.nc;
+    const int nc = c.nc;
     const double* __restrict__ rc = c.rc;
     const double* __restrict__ E0 = Einv + (size_t)6 * I * nc;
     double a[6] = {0, 0, 0, 0, 0, 0};
@@ -2118,6 +2121,13 @@ __global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(CoarseArgs c, co
         u[3 * i + 1] += (HT)(t1 + w2 * d0 - w0 * d2);
         u[3 * i + 2] += (HT)(t2 + w0 * d1 - w1 * d0);
     }
+}
+
+template <typename HT>
+__global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(CoarseArgs c, const PcgCtl* ctl,
+                                                                const double* __restrict__ Einv, HT* u,
+                                                                double* part) {
+    coarse_apply_body(c, ctl, Einv, u, part, blockIdx.x);
 }
 
 // ---- single-rollout kernels ----------------------------------------------
@@ -2179,7 +2189,10 @@ struct BMember {
     PcgView<HT> v;
     const double* rhs;  // b of A x = -b
     double* part;
-    int nbI, nbA, nbU, pad;
+    int nbI, nbA, nbU, nbC;  // nbC: aggregates of the coarse correction, 0 without one
+    CoarseArgs ca;
+    const double* Einv;
+    double* cpart;           // gamma_c partials, two parity slots
 };
 
 // j-th rollout of this launch owns launch blocks [tab[j], tab[j+1])
@@ -2220,8 +2233,25 @@ __global__ __launch_bounds__(kBlock) void k_b_pcg_spmv(const BMember<HT>* mem, c
     const BMember<HT>& m = mem[idx[j]];
     const int lb = blockIdx.x - tab[j], nb = tab[j + 1] - tab[j];
     const PartIO io{m.part + 4 * kMaxRedBlocks, kMaxRedBlocks, m.part + 2 * par * kMaxRedBlocks,
-                    first ? m.nbI : m.nbU, nullptr, 0};
+                    first ? m.nbI : m.nbU, nullptr, 0, m.nbC ? m.cpart + par * kMaxRedBlocks : nullptr, m.nbC};
     pcg_spmv_body(m.v, m.v.st + par, io, first, lb, nb);
+}
+
+// coarse correction of the rollouts that have one (their aggregates are the launch blocks)
+template <typename HT>
+__global__ __launch_bounds__(kRestrictThreads) void k_b_coarse_restrict(const BMember<HT>* mem, const int* idx,
+                                                                        const int* tab, int cnt) {
+    const int j = find_member(tab, cnt);
+    const BMember<HT>& m = mem[idx[j]];
+    coarse_restrict_body(m.ca, m.v.ctl, (const HT*)m.v.r, blockIdx.x - tab[j]);
+}
+
+template <typename HT>
+__global__ __launch_bounds__(kApplyThreads) void k_b_coarse_apply(const BMember<HT>* mem, const int* idx,
+                                                                  const int* tab, int cnt, int par) {
+    const int j = find_member(tab, cnt);
+    const BMember<HT>& m = mem[idx[j]];
+    coarse_apply_body(m.ca, m.v.ctl, m.Einv, m.v.u, m.cpart + par * kMaxRedBlocks, blockIdx.x - tab[j]);
 }
 
 template <typename HT>
@@ -2231,7 +2261,8 @@ __global__ __launch_bounds__(kUpdBlock) void k_b_pcg_update(const BMember<HT>* m
     const BMember<HT>& m = mem[idx[j]];
     const int lb = blockIdx.x - tab[j], nb = tab[j + 1] - tab[j];
     const PartIO io{m.part + 2 * par * kMaxRedBlocks, kMaxRedBlocks, m.part + 2 * (par ^ 1) * kMaxRedBlocks,
-                    first ? m.nbI : m.nbU, m.part + 4 * kMaxRedBlocks, m.nbA};
+                    first ? m.nbI : m.nbU, m.part + 4 * kMaxRedBlocks, m.nbA,
+                    m.nbC ? m.cpart + (par ^ 1) * kMaxRedBlocks : nullptr, m.nbC};
     pcg_update_body(m.v, m.v.st + (par ^ 1), m.v.st + par, io, first, lb, nb);
 }
 
@@ -3303,13 +3334,21 @@ double ctx_bytes(const sors_ctx* c) {
 
 // device bytes of a rollout before its pattern exists (an upper estimate:
 // node-block slices padded to 24 (tet) / 32 (hex) blocks per node)
-double estimate_bytes(int N, int E, int ed) {
+double estimate_bytes(int N, int E, int ed, int coarse_nodes) {
     const double n = 3.0 * N, nblk = (double)N * (ed == 4 ? 24 : 32);
+    // coarse correction: E, its inverse and the potrs right-hand sides (nc^2
+    // FP64 each, nc = 6 per aggregate, aggregates at most 2N / coarse_nodes
+    // and 512), per-node data, and the rocBLAS handle's workspace
+    double coarse = 0.0;
+    if (coarse_nodes > 0) {
+        const double nc = 6.0 * std::min(512.0, std::ceil(2.0 * N / coarse_nodes));
+        coarse = 3.0 * 8.0 * nc * nc + 24.0 * N + 64.0 * (1 << 20);
+    }
     // two-phase assembly: element rows (4 ed FP64 + ed^2 padded FP64 blocks at
     // worst) and the slot lists (ed^2 entries per element, one pointer per slot)
     const double rows = (double)E * (4 * ed * 8 + 10 * ed * ed * 8 + ed * ed * 4) + 4.0 * nblk;
     return n * (19 * 8 + 4) + 72.0 * N + nblk * (9 * 8 + 4 + 9 * 4) + (double)E * (ed * 4 + 16 + ed * ed) + rows +
-           8.0 * (5 * kMaxRedBlocks + 4 * kMaxRedBlocks) + (1 << 20);
+           coarse + 8.0 * (5 * kMaxRedBlocks + 4 * kMaxRedBlocks) + (1 << 20);
 }
 
 }  // namespace
@@ -3374,12 +3413,12 @@ void srv_try_admit(sors_server* s) {  // mu held
             std::string msg = rc ? sors_last_error(nullptr) : "";
             if (rc == SORS_OK) {
                 c->opt = s->opt.solver;
-                c->opt.coarse_nodes = 0;  // the batched PCG uses block-Jacobi
                 try {
                     if (!r->v0.empty()) upload(c->vprev, r->v0.data(), (size_t)c->n, c->stream);
                     ensure_pattern(c, r->dt);
                     // allocated here so that the measured size below includes them
                     ensure_elem_rows(c, c->opt.precision == SORS_MIXED ? sizeof(float) : sizeof(double));
+                    ensure_coarse(c);
                     sync(c);
                 } catch (const HipError& e) {
                     rc = SORS_ERR_HIP;
@@ -3478,6 +3517,10 @@ void srv_prep_loop(sors_server* s) {
     }
 }
 
+// launch tables staged per tick: member indices and block prefixes of the
+// starters (init, SpMV, update, coarse) and the runners (SpMV, update, coarse)
+constexpr int kStageTables = 11;
+
 // one tick over the resident rollouts
 template <typename HT>
 void srv_tick(sors_server* s) {
@@ -3525,8 +3568,10 @@ void srv_tick(sors_server* s) {
         const int M = s->opt.max_active;
         BMember<HT>* hm = (BMember<HT>*)s->h_stage;
         int* ht = (int*)(hm + M);
-        int *idxS = ht, *tIS = ht + (M + 1), *tAS = ht + 2 * (M + 1), *tUS = ht + 3 * (M + 1);
-        int *idxR = ht + 4 * (M + 1), *tAR = ht + 5 * (M + 1), *tUR = ht + 6 * (M + 1);
+        int* tb[kStageTables];
+        for (int k = 0; k < kStageTables; ++k) tb[k] = ht + k * (M + 1);
+        int *idxS = tb[0], *tIS = tb[1], *tAS = tb[2], *tUS = tb[3], *idxCS = tb[4], *tCS = tb[5];
+        int *idxR = tb[6], *tAR = tb[7], *tUR = tb[8], *idxCR = tb[9], *tCR = tb[10];
         for (int i : starters) {
             Member& m = s->active[i];
             BMember<HT>& b = hm[m.slot];
@@ -3537,33 +3582,62 @@ void srv_tick(sors_server* s) {
             b.nbI = m.g.gn;
             b.nbA = m.g.gs;
             b.nbU = m.g.gu;
+            // the rollout's coarse correction as its own pcg() would apply it
+            b.nbC = m.c->coarse_ready ? m.c->nagg : 0;
+            if (b.nbC) {
+                b.ca = coarse_args(m.c);
+                b.Einv = m.c->cEinv.as<double>();
+                b.cpart = m.c->cpart.as<double>();
+            }
         }
-        tIS[0] = tAS[0] = tUS[0] = 0;
+        int nsc = 0, nrc = 0;
+        tIS[0] = tAS[0] = tUS[0] = tCS[0] = 0;
         for (size_t j = 0; j < starters.size(); ++j) {
             const Member& m = s->active[starters[j]];
             idxS[j] = m.slot;
             tIS[j + 1] = tIS[j] + m.g.gn;
             tAS[j + 1] = tAS[j] + m.g.gs;
             tUS[j + 1] = tUS[j] + m.g.gu;
+            if (const int na = hm[m.slot].nbC) {
+                idxCS[nsc] = m.slot;
+                tCS[nsc + 1] = tCS[nsc] + na;
+                ++nsc;
+            }
         }
-        tAR[0] = tUR[0] = 0;
+        tAR[0] = tUR[0] = tCR[0] = 0;
         for (size_t j = 0; j < runners.size(); ++j) {
             const Member& m = s->active[runners[j]];
             idxR[j] = m.slot;
             tAR[j + 1] = tAR[j] + m.g.gs;
             tUR[j + 1] = tUR[j] + m.g.gu;
+            if (const int na = hm[m.slot].nbC) {
+                idxCR[nrc] = m.slot;
+                tCR[nrc + 1] = tCR[nrc] + na;
+                ++nrc;
+            }
         }
-        const size_t stage = sizeof(BMember<HT>) * M + sizeof(int) * 7 * (M + 1);
+        const size_t stage = sizeof(BMember<HT>) * M + sizeof(int) * kStageTables * (M + 1);
         HIPX(hipMemcpyAsync(s->bmem.p, s->h_stage, stage, hipMemcpyHostToDevice, s->stream));
         const BMember<HT>* dm = (const BMember<HT>*)s->bmem.p;
         const int* dt_ = (const int*)((const char*)s->bmem.p + sizeof(BMember<HT>) * M);
-        const int *dIdxS = dt_, *dIS = dt_ + (M + 1), *dAS = dt_ + 2 * (M + 1), *dUS = dt_ + 3 * (M + 1);
-        const int *dIdxR = dt_ + 4 * (M + 1), *dAR = dt_ + 5 * (M + 1), *dUR = dt_ + 6 * (M + 1);
+        const int* db[kStageTables];
+        for (int k = 0; k < kStageTables; ++k) db[k] = dt_ + k * (M + 1);
+        const int *dIdxS = db[0], *dIS = db[1], *dAS = db[2], *dUS = db[3], *dIdxCS = db[4], *dCS = db[5];
+        const int *dIdxR = db[6], *dAR = db[7], *dUR = db[8], *dIdxCR = db[9], *dCR = db[10];
         const int ns = (int)starters.size(), nr = (int)runners.size();
+        // u += Z E^-1 Z^T r after an update (or the init) of the rollouts with a coarse correction
+        auto coarse = [&](const int* didx, const int* dtab, int cnt, int total, int par) {
+            if (!cnt) return;
+            k_b_coarse_restrict<HT><<<total, kRestrictThreads, 0, s->stream>>>(dm, didx, dtab, cnt);
+            k_b_coarse_apply<HT><<<total, kApplyThreads, 0, s->stream>>>(dm, didx, dtab, cnt, par);
+            s->launches += 2;
+        };
         if (ns) {
             k_b_pcg_init<HT><<<tIS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dIS, ns);
+            coarse(dIdxCS, dCS, nsc, tCS[nsc], 0);
             k_b_pcg_spmv<HT><<<tAS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dAS, ns, 0, 1);     // A_0
             k_b_pcg_update<HT><<<tUS[ns], kUpdBlock, 0, s->stream>>>(dm, dIdxS, dUS, ns, 1, 1);  // U_1
+            coarse(dIdxCS, dCS, nsc, tCS[nsc], 1);
             k_b_pcg_spmv<HT><<<tAS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dAS, ns, 1, 0);     // A_1
             s->launches += 4;
         }
@@ -3572,6 +3646,7 @@ void srv_tick(sors_server* s) {
         for (int j = 0; j < chunk; ++j) {
             const int par = j & 1;
             k_b_pcg_update<HT><<<tUR[nr], kUpdBlock, 0, s->stream>>>(dm, dIdxR, dUR, nr, par, 0);
+            coarse(dIdxCR, dCR, nrc, tCR[nrc], par);
             k_b_pcg_spmv<HT><<<tAR[nr], kBlock, 0, s->stream>>>(dm, dIdxR, dAR, nr, par, 0);
         }
         s->launches += 2 * chunk;
@@ -4560,7 +4635,7 @@ int sors_server_create(const sors_server_options* o, sors_server** out) {
         const int M = o->max_active;
         ensure(s->slots, sizeof(SlotRec) * M);
         HIPX(hipMemset(s->slots.p, 0, sizeof(SlotRec) * M));
-        const size_t stage = sizeof(BMember<double>) * M + sizeof(int) * 7 * (M + 1);
+        const size_t stage = sizeof(BMember<double>) * M + sizeof(int) * kStageTables * (M + 1);
         static_assert(sizeof(BMember<double>) >= sizeof(BMember<float>), "stage size");
         ensure(s->bmem, stage);
         HIPX(hipHostMalloc(&s->h_stage, stage, hipHostMallocDefault));
@@ -4617,7 +4692,7 @@ int sors_server_submit(sors_server* s, const sors_system_desc* d, const double* 
     if (v0) r->v0.assign(v0, v0 + n);
     r->dt = dt;
     r->steps = steps;
-    r->est_bytes = estimate_bytes(r->N, r->E, r->ed);
+    r->est_bytes = estimate_bytes(r->N, r->E, r->ed, s->opt.solver.coarse_nodes);
     if (r->est_bytes > s->opt.hbm_budget_bytes) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->err = "request needs more device memory than the server's budget";

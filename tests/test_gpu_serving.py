@@ -23,8 +23,8 @@ def small_request(i, seed=11):
     return mesh.serving_request(i, seed=seed, n_lo=300, n_hi=3000, steps_lo=3, steps_hi=6)
 
 
-def alone(sc, precision=native.MIXED):
-    r = native.Rollout.from_scene(sc, precision=precision, **SOLVER)
+def alone(sc, precision=native.MIXED, **extra):
+    r = native.Rollout.from_scene(sc, precision=precision, **dict(SOLVER, **extra))
     for _ in range(sc.steps):
         r.step_resident(sc.dt)
     q = r.positions().reshape(-1, 3)
@@ -51,6 +51,33 @@ def test_serving_mix_matches_alone():
         assert rs["steps"] == sc.steps and rs["status"] == native.OK
         assert np.array_equal(q, ref)
         assert np.abs(q - sc.X).max() > 1e-6  # the rollout moved
+
+
+@pytest.mark.parametrize("every", [1, 3])
+def test_serving_two_level_pcg_matches_alone(every):
+    """Served rollouts with the coarse correction (each rollout its own
+    aggregates; batched restriction and apply launches) equal stand-alone
+    rollouts with the same options bit for bit, and take fewer PCG
+    iterations than block-Jacobi alone."""
+    coarse = dict(coarse_nodes=64, coarse_every=every)
+    reqs = [small_request(i, seed=7) for i in range(6)]
+    srv = native.Server(max_active=4, prep_threads=2, pcg_chunk=8, **SOLVER, **coarse)
+    ids = [srv.submit(sc, sc.steps, sc.dt) for sc in reqs]
+    out = [srv.wait(i, timeout=120.0) for i in ids]
+    srv.close()
+    bj = native.Server(max_active=4, prep_threads=2, pcg_chunk=8, **SOLVER)
+    ids = [bj.submit(sc, sc.steps, sc.dt) for sc in reqs]
+    out_bj = [bj.wait(i, timeout=120.0) for i in ids]
+    bj.close()
+    it_c = it_b = 0
+    for sc, (q, rs), (_, rb) in zip(reqs, out, out_bj):
+        ref = alone(sc, **coarse)
+        say(sc.name, sc.n_nodes, "coarse served vs alone", np.abs(q - ref).max(), rs["pcg_iters"], rb["pcg_iters"])
+        assert rs["status"] == native.OK
+        assert np.array_equal(q, ref)
+        it_c += rs["pcg_iters"]
+        it_b += rb["pcg_iters"]
+    assert it_c < it_b
 
 
 def test_serving_single_request_matches_plain_fp64():
