@@ -239,12 +239,13 @@ def adjoint_leg(args):
     sigma_max 2e5 Pa, 100 backward Euler steps of dt = 2e-3; loss = squared
     distance of the foot centroid to a seeded target.  Gradient w.r.t. the
     117,600 Young's moduli and the 100 x 32 activations by the adjoint with
-    sqrt(T) = 10 checkpoints, FP64; checked against central differences
-    (h = 1e-6) along `--adjoint-dirs` seeded directions."""
+    sqrt(T) = 10 checkpoints, FP64, two-level PCG (`--adjoint-coarse` nodes
+    per aggregate, refreshed from every tangent); checked against central
+    differences (h = 1e-6) along `--adjoint-dirs` seeded directions."""
     from sors_amd import adjoint, mesh
 
     leg = mesh.leg_prism(steps=args.adjoint_steps, seed=0)
-    ro = adjoint.LegRollout(leg)
+    ro = adjoint.LegRollout(leg, coarse_nodes=args.adjoint_coarse)
     E = leg.scene.youngs.copy()
     t0 = time.perf_counter()
     fwd = ro.forward(E, leg.act)
@@ -257,7 +258,8 @@ def adjoint_leg(args):
            "loss": L, "forward_s": t_fwd, "gradient_s": t_bwd,
            "gradient_over_forward": t_bwd / t_fwd, "params": int(gE.size + gA.size),
            "forward_newton": ro.stats["forward_newton"], "forward_pcg": ro.stats["forward_pcg"],
-           "adjoint_pcg": ro.stats["adjoint_pcg"], "checkpoint_every": ro.K}
+           "adjoint_pcg": ro.stats["adjoint_pcg"], "checkpoint_every": ro.K,
+           "coarse_nodes": args.adjoint_coarse}
     log(f"adjoint: {out}")
     if args.adjoint_dirs > 0:
         rng = np.random.default_rng(7)
@@ -368,6 +370,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=100, help="leg rollout steps (0: skip the adjoint leg)")
     ap.add_argument("--adjoint-dirs", type=int, default=3, help="finite-difference directions checked")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg")
+    ap.add_argument("--adjoint-coarse", type=int, default=256,
+                    help="nodes per aggregate of the adjoint leg's two-level PCG (0: block-Jacobi)")
     ap.add_argument("--hex-steps", type=int, default=2, help="hex8 beam steps (0: skip the hex leg)")
     ap.add_argument("--hex-only", action="store_true", help="run only the hex8 leg")
     args = ap.parse_args()

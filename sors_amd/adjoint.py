@@ -43,8 +43,10 @@ class LegRollout:
         self.adjoint_rtol = adjoint_rtol
         # predict=1: Newton's start depends on the state only (not on the step
         # before it), so recomputing from any checkpoint reproduces the forward
-        # states bit for bit
-        opts = dict(g_tol=0.0, dx_tol=1e-12, pcg_rtol=1e-12, max_newton=100, project_spd=0, predict=1)
+        # states bit for bit; coarse_every=1 for the same reason (a two-level
+        # preconditioner, when asked for, refreshed from every tangent)
+        opts = dict(g_tol=0.0, dx_tol=1e-12, pcg_rtol=1e-12, max_newton=100, project_spd=0, predict=1,
+                    coarse_every=1)
         opts.update(solver)
         self.r = native.Rollout.from_scene(sc, method=method, precision=precision, **opts)
         self.r.set_contact(planes=[((0.0, 0.0, 0.0), (0.0, 0.0, 1.0))], surface=leg.surface,

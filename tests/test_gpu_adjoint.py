@@ -78,9 +78,10 @@ def test_single_step_adjoint_matches_fd():
         assert abs(fd - ad) <= 1e-4 * max(abs(fd), abs(ad)) + 1e-14
 
 
-def test_rollout_adjoint_with_checkpoints_matches_fd():
+@pytest.mark.parametrize("coarse", [0, 64])
+def test_rollout_adjoint_with_checkpoints_matches_fd(coarse):
     leg = small_leg(steps=9)
-    ro = adjoint.LegRollout(leg, checkpoint_every=3)
+    ro = adjoint.LegRollout(leg, checkpoint_every=3, coarse_nodes=coarse)
     E = leg.scene.youngs.copy()
     L, out = adjoint.fd_check(ro, E, leg.act, directions=3, h=1e-6, seed=2)
     say("9-step loss", L, out, ro.stats)
@@ -88,9 +89,12 @@ def test_rollout_adjoint_with_checkpoints_matches_fd():
         assert abs(fd - ad) <= 1e-4 * max(abs(fd), abs(ad)) + 1e-14
 
 
-def test_checkpoint_recompute_reproduces_forward():
+@pytest.mark.parametrize("coarse", [0, 64])
+def test_checkpoint_recompute_reproduces_forward(coarse):
+    """coarse > 0: two-level PCG refreshed from every tangent, so the
+    preconditioner depends on the state only and recomputation stays exact."""
     leg = small_leg(steps=6)
-    ro = adjoint.LegRollout(leg, checkpoint_every=2)
+    ro = adjoint.LegRollout(leg, checkpoint_every=2, coarse_nodes=coarse)
     E = leg.scene.youngs.copy()
     L, qT, ck = ro.forward(E, leg.act)
     keep = {}
