@@ -2078,7 +2078,10 @@ __global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict(CoarseArgs
 // aggregate I: one 1024-thread workgroup per aggregate, so E^-1 is still read
 // once per application and the prolongation needs no second launch.  One
 // partial of gamma_c = rc . xc per workgroup; fixed-order sums throughout.
-constexpr int kMaxCoarse = 6 * 512;  // coarse unknowns: nc^2 FP64 values are read per PCG iteration
+#ifndef SORS_MAX_AGGREGATES
+#define SORS_MAX_AGGREGATES 512
+#endif
+constexpr int kMaxCoarse = 6 * SORS_MAX_AGGREGATES;  // coarse unknowns: nc^2 FP64 values are read per PCG iteration
 #ifndef SORS_APPLY_THREADS
 #define SORS_APPLY_THREADS 512
 #endif

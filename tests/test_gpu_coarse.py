@@ -43,9 +43,10 @@ def test_coarse_linear_solve_matches_direct(orc, precision):
 @pytest.mark.parametrize("cn", [300, 100000])
 def test_coarse_aggregates_wider_than_a_workgroup(cn):
     """Aggregates of several hundred nodes (the restriction sums them over
-    several wavefronts) and a single aggregate holding every free node: the
-    FP64 residual |H x - b| of the solve meets the requested tolerance, in
-    fewer iterations than block-Jacobi."""
+    several wavefronts) and the coarsest space (target above the node count:
+    three cubic bins along the beam): the FP64 residual |H x - b| of the
+    solve meets the requested tolerance, in fewer iterations than
+    block-Jacobi."""
     sp = pytest.importorskip("scipy.sparse")
     sc = mesh.beam(nx=40, ny=6, nz=6, e_range=(5e5, 5e6), seed=5)
     rng = np.random.default_rng(6)
