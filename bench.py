@@ -5,13 +5,18 @@ Workload: the scaled beam -- 200x40x40 cells split into 1,920,000 Kuhn tet4
 (337,881 nodes, 1,013,643 DOFs), compressible Neo-Hookean with E drawn
 log-uniform in [5e5, 5e6] Pa per cell (SplitMix64 seed 0), nu = 0.45,
 rho = 1100, gravity -z, x = 0 clamped, initial velocity 0.5 m/s * x/L in +z,
-backward Euler dt = 1e-3, Newton + Armijo line search, block-Jacobi PCG.
-A step is one call of Solver::step on device-resident state.
+backward Euler dt = 1e-3, Newton + Armijo line search, PCG preconditioned by
+block-Jacobi plus a rigid-body coarse correction on 2048-node aggregates
+(`--coarse-nodes`, refreshed every `--coarse-every` tangents).  A step is one
+call of Solver::step on device-resident state.
 
 The timed run uses mixed precision (FP32 element tangents / CSR / SpMV / PCG
-vectors, FP64 state, residual, energies and dot products).  The same steps
-are then repeated in all-FP64 and the final states compared; the CPU
-baseline is the FP64 oracle timed on a bounded sample on the host cores.
+vectors, FP64 state, residual, energies, dot products and coarse solve).  The
+same steps are then repeated in all-FP64 and the final states compared, and in
+mixed precision with block-Jacobi alone; the CPU baseline is the FP64 oracle
+(block-Jacobi PCG) timed on a bounded sample on the host cores and scaled by
+the block-Jacobi run's per-step counts.  Side legs: hex8 beam, contact poke,
+adjoint of a muscle-driven leg, serving of a seeded request stream.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --profile-kernel spmv|assemble|copy|update [--reps R]
