@@ -2877,6 +2877,10 @@ void build_preconditioner(sors_ctx* c) {
 #define SORS_UPD_BLOCKS 512
 #endif
 constexpr int kSpmvBlocks = SORS_SPMV_BLOCKS;   // multiple of 8 (XCD-contiguous row ranges)
+#ifndef SORS_UPD_MIN_PER_THREAD
+#define SORS_UPD_MIN_PER_THREAD 4
+#endif
+constexpr int kUpdMinPerThread = SORS_UPD_MIN_PER_THREAD;
 constexpr int kUpdBlocks = SORS_UPD_BLOCKS;
 
 struct PcgGrid {
@@ -2888,7 +2892,9 @@ struct PcgGrid {
 PcgGrid pcg_grid(sors_ctx* c) {
     PcgGrid g;
     g.gn = std::min(red_grid(c->N, kBlock), kUpdBlocks);
-    g.gu = std::min(red_grid(c->n, kUpdBlock), kUpdBlocks);
+    // at least kUpdMinPerThread DOFs per thread: every workgroup also pays the
+    // prologue that sums the previous kernels' partials
+    g.gu = std::min(red_grid(c->n, kUpdBlock * kUpdMinPerThread), kUpdBlocks);
     long long work = c->block() ? (c->pat.nslices + kBlock / kSlice - 1) / (kBlock / kSlice)
                                 : (c->n + kBlock / 8 - 1) / (kBlock / 8);
     g.gs = (int)std::min<long long>(std::max<long long>(work, 1), kSpmvBlocks);
