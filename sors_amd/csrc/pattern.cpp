@@ -222,7 +222,7 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
         P.bdiag.assign(n, -1);
         for (int v = 0; v < N; ++v)
             for (int c = 0; c < 3; ++c) P.bdiag[3 * v + c] = (int)block_index(P.sbase.data(), v, P.self_slot[v], c, c);
-        // node -> (element, local index) incidence for the node-centric assembly
+        // node -> (element, local index) incidence for the per-node gradient sums
         P.incptr = std::move(iptr);
         P.inc = std::move(inc);
         if ((ed == 4 || ed == 8) && (long long)ed * ed * E <= 0x7fffffffLL) {
