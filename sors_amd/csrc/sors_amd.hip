@@ -2418,6 +2418,7 @@ struct sors_ctx {
     long long coarse_tangents = 0;  // tangents seen since the coarse space was built
     hipStream_t side = nullptr;     // rocSOLVER work (default priority, like the main stream)
     hipEvent_t evG = nullptr, evR = nullptr;  // E assembled / factorisation done
+    hipEvent_t evPre = nullptr;     // served rollouts: preconditioner built (the batched PCG waits)
     bool refresh_pending = false;   // a factorisation on the side stream not yet collected
     const void* coarse_pat = nullptr;
     bool coarse_ready = false;      // E^-1 valid for the current matrix
@@ -3152,16 +3153,17 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
         auto t0 = std::chrono::steady_clock::now();
         clear_flag(c);
         assemble<HT>(c, q, dt, true, o.project_spd);
-        build_preconditioner<HT>(c);
         const double gg = read_scalar(c, S_GG);
         if (read_flag(c) & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
         stt->gnorm = std::sqrt(gg);
-        stt->ms_assemble += ms_since(t0);
         if (!(gg == gg)) return fail(c, SORS_ERR_NAN, "Newton's method diverged (NaN residual)");
         if (o.g_tol > 0 && stt->gnorm < o.g_tol) {
+            stt->ms_assemble += ms_since(t0);
             converged = true;
             break;
         }
+        build_preconditioner<HT>(c);  // only for a tangent that is solved with
+        stt->ms_assemble += ms_since(t0);
         auto t1 = std::chrono::steady_clock::now();
         bool brk = false;
         // forcing term: pcg_rtol, loosened when the residual criterion is in
@@ -3428,6 +3430,7 @@ void srv_try_admit(sors_server* s) {  // mu held
                     // allocated here so that the measured size below includes them
                     ensure_elem_rows(c, c->opt.precision == SORS_MIXED ? sizeof(float) : sizeof(double));
                     ensure_coarse(c);
+                    HIPX(hipEventCreateWithFlags(&c->evPre, hipEventDisableTiming));
                     sync(c);
                 } catch (const HipError& e) {
                     rc = SORS_ERR_HIP;
@@ -3550,7 +3553,6 @@ void srv_tick(sors_server* s) {
                 }
                 clear_flag(c);
                 assemble<HT>(c, q, dt, true, o.project_spd);
-                build_preconditioner<HT>(c);
                 m.busy = true;
                 break;
             case Member::POST:
@@ -3642,6 +3644,7 @@ void srv_tick(sors_server* s) {
             s->launches += 2;
         };
         if (ns) {
+            for (int i : starters) HIPX(hipStreamWaitEvent(s->stream, s->active[i].c->evPre, 0));
             k_b_pcg_init<HT><<<tIS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dIS, ns);
             coarse(dIdxCS, dCS, nsc, tCS[nsc], 0);
             k_b_pcg_spmv<HT><<<tAS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dAS, ns, 0, 1);     // A_0
@@ -3749,6 +3752,10 @@ void srv_tick(sors_server* s) {
                 double eta = o.pcg_rtol;
                 if (o.g_tol > 0 && m.st.gnorm > 0) eta = std::min(0.1, std::max(eta, 0.5 * o.g_tol / m.st.gnorm));
                 m.eta = eta;
+                // preconditioner only for a tangent that is solved (as minimize()
+                // does); the next tick's batched launches wait for it
+                build_preconditioner<HT>(c);
+                HIPX(hipEventRecord(c->evPre, c->stream));
                 m.phase = Member::PCG_START;
                 break;
             }
@@ -4021,6 +4028,7 @@ void sors_destroy(sors_ctx* c) {
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->evG) (void)hipEventDestroy(c->evG);
+    if (c->evPre) (void)hipEventDestroy(c->evPre);
     if (c->evR) (void)hipEventDestroy(c->evR);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
