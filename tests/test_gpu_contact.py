@@ -63,13 +63,16 @@ def test_contact_gradient_energy_hessian_fp64(orc):
         assert rel_err(hd, ho) < 1e-10
 
 
-@pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
-def test_contact_poke_trajectory(orc, precision):
+@pytest.mark.parametrize("precision,coarse", [(native.FP64, 0), (native.MIXED, 0), (native.MIXED, 32)])
+def test_contact_poke_trajectory(orc, precision, coarse):
     """6^3-cell cube on the ground, disk pressing down with jitter, 30 steps:
-    every state within 1e-6 of the bounding-box diagonal of the oracle."""
+    every state within 1e-6 of the bounding-box diagonal of the oracle
+    (coarse > 0: with the two-level PCG, contact blocks included in the
+    Galerkin product, refreshed every 4 tangents)."""
     p, sc, md, mo = poke_pair(orc, n=6, steps=30, gap=-1e-3, speed=0.05)
-    say("poke", sc.n_nodes, sc.n_elems, "fp64" if precision == native.FP64 else "mixed")
-    r = native.Rollout.from_scene(sc, method=md, precision=precision, pcg_rtol=1e-7)
+    say("poke", sc.n_nodes, sc.n_elems, "fp64" if precision == native.FP64 else "mixed", coarse)
+    r = native.Rollout.from_scene(sc, method=md, precision=precision, pcg_rtol=1e-7, coarse_nodes=coarse,
+                                  coarse_every=4)
     o = orc.System.from_scene(sc, method=mo)
     r.set_contact(**contact_kw(p))
     o.set_contact(**contact_kw(p))
