@@ -17,9 +17,10 @@ def say(*a):
     print(*a, flush=True)
 
 
-@pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
-def test_coarse_linear_solve_matches_direct(orc, precision):
-    sc = mesh.beam(nx=16, ny=3, nz=3, e_range=(5e5, 5e6), seed=1)
+@pytest.mark.parametrize("precision,element", [(native.FP64, "tet"), (native.MIXED, "tet"), (native.FP64, "hex"),
+                                               (native.MIXED, "hex")])
+def test_coarse_linear_solve_matches_direct(orc, precision, element):
+    sc = mesh.beam(nx=16, ny=3, nz=3, element=element, e_range=(5e5, 5e6), seed=1)
     o = orc.System.from_scene(sc, method=orc.NEWTON)
     rng = np.random.default_rng(4)
     q = sc.X.reshape(-1) + rng.normal(scale=1e-4, size=3 * sc.n_nodes) * (1 - sc.bcmask)

@@ -22,8 +22,10 @@ def load():
         sys.path.remove(PKG)
 
 
-@pytest.mark.parametrize("mesh_type", ["Tetrahedron", "Hexahedron"])
-def test_cantilever_driver(orc, mesh_type):
+@pytest.mark.parametrize("mesh_type,coarse", [("Tetrahedron", 0), ("Hexahedron", 0), ("Tetrahedron", 16),
+                                              ("Hexahedron", 16)])
+def test_cantilever_driver(orc, mesh_type, coarse):
+    """coarse > 0: settings.coarseNodes turns on the two-level PCG."""
     ps = load()
     element = "tet" if mesh_type == "Tetrahedron" else "hex"
     sc = mesh.beam(nx=10, ny=2, nz=2, element=element, steps=20)
@@ -32,6 +34,7 @@ def test_cantilever_driver(orc, mesh_type):
     settings.timeSteppingScheme = "backward_euler"
     settings.solverMethod = "minimize_newton_linesearch"
     settings.dt = sc.dt
+    settings.coarseNodes = coarse
     energies = [set(["neohookean"]) for _ in range(sc.n_elems)]
     params = [ps.Params(["density", "youngsModulus", "poissonsRatio"], [1100.0, 1e6, 0.45])
               for _ in range(sc.n_elems)]
@@ -43,9 +46,12 @@ def test_cantilever_driver(orc, mesh_type):
     o = orc.System.from_scene(sc, method=orc.NEWTON_LS)
     q = qo = sc.X.reshape(-1).copy()
     act = ps.Params(["vertexForce"], [np.zeros((sc.n_nodes, 3))])
+    pcg = 0
     for _ in range(sc.steps):
         q = solver.step(q, energy, act, 0.0)   # dt = 0 -> settings.dt (solver.cpp:8-9)
         qo, _ = o.step(qo, sc.dt)
+        pcg += solver.last_stats["pcg_iters"]
+    print(mesh_type, "coarse", coarse, "pcg iterations", pcg, flush=True)
     assert np.abs(q - qo).max() <= 1e-6 * sc.bbox_diag
     assert solver.last_stats["newton_iters"] >= 1
 
