@@ -89,16 +89,18 @@ typedef struct {
                              taken since the state was set; Dirichlet DOFs start at
                              qPrev + dt*value.  The converged state is the same
                              minimiser */
-    int coarse_nodes;     /* > 0: two-level PCG preconditioner, block-Jacobi plus a
-                             coarse correction on the rigid-body modes of spatial
-                             node aggregates of about this many nodes (sliced
-                             node-block layout only); 0: block-Jacobi alone */
+    int coarse_nodes;     /* > 0 (default 2048): two-level PCG preconditioner,
+                             block-Jacobi plus a coarse correction on the
+                             rigid-body modes of spatial node aggregates of
+                             about this many nodes (sliced node-block layout
+                             only); 0: block-Jacobi alone */
     int coarse_every;     /* coarse operator refreshed on every k-th tangent
-                             (<= 1: every tangent, factorised before the
-                             solve); with k > 1 a refresh is factorised on a
-                             second stream while the solve runs with the
-                             previous inverse, and the next tangent installs
-                             it.  PCG always runs with an SPD preconditioner */
+                             (default 8; <= 1: every tangent, factorised
+                             before the solve); with k > 1 a refresh is
+                             factorised on a second stream while the solve
+                             runs with the previous inverse, and the next
+                             tangent installs it.  PCG always runs with an
+                             SPD preconditioner */
 } sors_options;
 
 typedef struct {

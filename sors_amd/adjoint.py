@@ -28,7 +28,7 @@ class LegRollout:
     actuation, contact and loss."""
 
     def __init__(self, leg, precision=native.FP64, checkpoint_every=None, adjoint_rtol=1e-11,
-                 method=native.MINIMIZE_NEWTON, **solver):
+                 method=native.MINIMIZE_NEWTON, coarse_nodes=0, **solver):
         """Solves use minimize_newton (solver.cpp:56-105) on the unprojected
         Hessian by default: its steps converge quadratically to rounding
         (about 5 per step here), whereas an Armijo test stops resolving energy
@@ -46,7 +46,7 @@ class LegRollout:
         # states bit for bit; coarse_every=1 for the same reason (a two-level
         # preconditioner, when asked for, refreshed from every tangent)
         opts = dict(g_tol=0.0, dx_tol=1e-12, pcg_rtol=1e-12, max_newton=100, project_spd=0, predict=1,
-                    coarse_every=1)
+                    coarse_nodes=coarse_nodes, coarse_every=1)
         opts.update(solver)
         self.r = native.Rollout.from_scene(sc, method=method, precision=precision, **opts)
         self.r.set_contact(planes=[((0.0, 0.0, 0.0), (0.0, 0.0, 1.0))], surface=leg.surface,

@@ -77,8 +77,9 @@ struct SimulationSettings {
     std::string precision = "mixed";  // "mixed" | "fp64"
     double pcgTolerance = 1e-6;
     bool predictor = false;           // start Newton from the extrapolated state instead of the given solution
-    int coarseNodes = 0;              // > 0: two-level PCG (rigid-body coarse correction on aggregates of this size)
-    int coarseEvery = 4;              // coarse operator refreshed on every k-th tangent
+    int coarseNodes = 2048;           // > 0: two-level PCG (rigid-body coarse correction on aggregates of this
+                                      // size); 0: block-Jacobi alone
+    int coarseEvery = 8;              // coarse operator refreshed on every k-th tangent
     void verbose_print() const;
 };
 

@@ -3872,8 +3872,8 @@ void sors_default_options(sors_options* o) {
     o->use_graph = 1;
     o->profile_kernels = 0;
     o->predict = 2;
-    o->coarse_nodes = 0;
-    o->coarse_every = 1;
+    o->coarse_nodes = 2048;
+    o->coarse_every = 8;
 }
 
 int sors_create(const sors_system_desc* d, sors_ctx** out) {

@@ -65,7 +65,7 @@ def test_serving_two_level_pcg_matches_alone(every):
     ids = [srv.submit(sc, sc.steps, sc.dt) for sc in reqs]
     out = [srv.wait(i, timeout=120.0) for i in ids]
     srv.close()
-    bj = native.Server(max_active=4, prep_threads=2, pcg_chunk=8, **SOLVER)
+    bj = native.Server(max_active=4, prep_threads=2, pcg_chunk=8, coarse_nodes=0, **SOLVER)
     ids = [bj.submit(sc, sc.steps, sc.dt) for sc in reqs]
     out_bj = [bj.wait(i, timeout=120.0) for i in ids]
     bj.close()
@@ -97,7 +97,7 @@ def test_serving_budget_limits_residency():
     est = [native_estimate(sc) for sc in reqs]
     per = max(est)
     assert sum(est) > 2.5 * per
-    srv = native.Server(hbm_budget_bytes=2.5 * per, max_active=8, prep_threads=2, **SOLVER)
+    srv = native.Server(hbm_budget_bytes=2.5 * per, max_active=8, prep_threads=2, coarse_nodes=0, **SOLVER)
     ids = [srv.submit(sc, sc.steps, sc.dt) for sc in reqs]
     for i in ids:
         srv.wait(i, timeout=120.0)
@@ -127,7 +127,8 @@ def test_serving_reports_failures():
 
 
 def native_estimate(sc):
-    """Device bytes the server reserves for a request before its pattern exists."""
+    """Device bytes the server reserves for a request before its pattern exists
+    (block-Jacobi PCG: coarse_nodes = 0)."""
     n, N, E, ed = 3 * sc.n_nodes, sc.n_nodes, sc.n_elems, sc.ed
     nblk = N * (24 if ed == 4 else 32)
     # two-phase assembly: element rows and slot lists
