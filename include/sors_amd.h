@@ -115,8 +115,9 @@ typedef struct {
     double ms_total;
     double spmv_ms;            /* summed SpMV kernel time (profile_kernels) */
     long long spmv_launches;
-    double assemble_kernel_ms; /* summed element-assembly kernel time (profile_kernels) */
-    long long assemble_launches;
+    double assemble_kernel_ms; /* summed element-assembly kernel time of the tangent (gradient +
+                                  Hessian) assemblies (profile_kernels) */
+    long long assemble_launches; /* tangent assemblies timed; gradient-only residual checks are not */
 } sors_stats;
 
 typedef struct sors_ctx sors_ctx;

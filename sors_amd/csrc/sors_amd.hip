@@ -2687,7 +2687,7 @@ void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project
     a.gint = c->gint.as<double>();
     a.vals = hessian ? c->vals.p : nullptr;
     a.hscale = (c->scheme == SORS_CRANK_NICOLSON && dt != 0.0) ? 0.5 : 1.0;
-    hipEvent_t* ev = c->prof ? c->events(2) : nullptr;
+    hipEvent_t* ev = c->prof && hessian ? c->events(2) : nullptr;
     if (hessian)
         launch_elem<HT, 0>(c, a, project, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     else

@@ -231,6 +231,43 @@ def test_graph_replay_matches_direct_launches(precision):
 
 
 @pytest.mark.parametrize("element", ["tet", "hex"])
+def test_residual_check_after_aimed_solve_is_gradient_only(element):
+    """After a solve whose forcing term came from g_tol, Newton checks the
+    next residual with a gradient-only assembly and assembles the tangent
+    only if the check fails.  With profiling on, the timed (tangent)
+    assemblies then number one per Newton solve, not one more per step, and
+    the residual that ended each step is the true one: ||g|| of an FP64
+    gradient evaluation at the returned state, on a second rollout that
+    still holds the step's history (and then takes the same step, bit for
+    bit).  Served rollouts always assemble the tangent for the check; the
+    serving tests require them to equal stand-alone rollouts bit for bit."""
+    sc = mesh.beam(nx=12, ny=3, nz=3, element=element, e_range=(5e5, 5e6), seed=2)
+    # g_tol such that steps end on the residual criterion (||g|| ~ 1e-7 N
+    # when the step-size criterion would stop them)
+    g_tol = 1e-6
+    opts = dict(g_tol=g_tol, pcg_rtol=3e-3)
+    r = rollout(sc, profile_kernels=1, **opts)
+    lag = rollout(sc, **opts)
+    q = sc.X.reshape(-1).copy()
+    solves = tangents = ended_on_g = 0
+    for _ in range(4):
+        q_new, st = r.step(q, sc.dt)
+        solves += st.newton_iters
+        tangents += st.assemble_launches
+        say("step", element, st.newton_iters, st.assemble_launches, st.gnorm)
+        assert st.newton_iters <= st.assemble_launches <= st.newton_iters + 1
+        if st.gnorm < g_tol:
+            ended_on_g += 1
+            g = lag.gradient(q_new, sc.dt)
+            assert abs(np.linalg.norm(g) - st.gnorm) <= 1e-12 * st.gnorm
+        q_lag, _ = lag.step(q, sc.dt)
+        assert np.array_equal(q_lag, q_new)
+        q = q_new
+    assert ended_on_g > 0
+    assert tangents < solves + 4  # at least one final check skipped the tangent
+
+
+@pytest.mark.parametrize("element", ["tet", "hex"])
 def test_node_assembly_is_deterministic(element):
     """Two assemblies of the same state give identical gradients and
     Hessian values (no float atomics on the node-block path: element rows,
