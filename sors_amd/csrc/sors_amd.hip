@@ -1704,13 +1704,34 @@ __device__ __forceinline__ void sum_partials_multi(const PartIO& io, double (&t)
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     const int nbA = io.inA ? io.nbA : 0, nbC = io.inC ? io.nbC : 0;
     const int nmax = max(io.nbU, max(nbA, nbC));
-    for (int i = threadIdx.x; i < nmax; i += blockDim.x) {
-        if (i < io.nbU) {
-            s[0] += io.inU[i];
-            s[1] += io.inU[io.stride + i];
+    const int step = blockDim.x;
+    // kPartTrips slots per thread and pass: a pass issues all its loads
+    // before its first add (one cache round trip per pass instead of one per
+    // slot); each thread adds its slots in ascending order
+    constexpr int kPartTrips = 4;
+    for (int i0 = threadIdx.x; i0 < nmax; i0 += kPartTrips * step) {
+        double u0[kPartTrips], u1[kPartTrips], a[kPartTrips], cc[kPartTrips];
+#pragma unroll
+        for (int k = 0; k < kPartTrips; ++k) {
+            const int i = i0 + k * step;
+            u0[k] = u1[k] = a[k] = cc[k] = 0.0;
+            if (i < io.nbU) {
+                u0[k] = io.inU[i];
+                u1[k] = io.inU[io.stride + i];
+            }
+            if (i < nbA) a[k] = io.inA[i];
+            if (i < nbC) cc[k] = io.inC[i];
         }
-        if (i < nbA) s[2] += io.inA[i];
-        if (i < nbC) s[3] += io.inC[i];
+#pragma unroll
+        for (int k = 0; k < kPartTrips; ++k) {
+            const int i = i0 + k * step;
+            if (i < io.nbU) {
+                s[0] += u0[k];
+                s[1] += u1[k];
+            }
+            if (i < nbA) s[2] += a[k];
+            if (i < nbC) s[3] += cc[k];
+        }
     }
     block_sum<4>(s);
     if (threadIdx.x == 0)
