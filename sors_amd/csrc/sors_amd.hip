@@ -3357,6 +3357,9 @@ struct Member {
     int slot = -1;
     enum Phase { ASSEMBLE, PCG_START, PCG_RUN, POST, ENERGY } phase = ASSEMBLE;
     bool new_step = true, busy = false, brk = false;
+    // aimed: the last solve's forcing term came from g_tol, so the next
+    // residual check assembles the gradient alone (grad_only), as in minimize()
+    bool aimed = false, grad_only = false;
     int step = 0, it = 0, bt = 0;
     long long launched = 0;
     double eta = 0, dxn = 0, gdx = 0, alpha = 1;
@@ -3590,7 +3593,8 @@ void srv_tick(sors_server* s) {
                     m.new_step = false;
                 }
                 clear_flag(c);
-                assemble<HT>(c, q, dt, true, o.project_spd);
+                m.grad_only = m.aimed;
+                assemble<HT>(c, q, dt, !m.grad_only, o.project_spd);
                 m.busy = true;
                 break;
             case Member::POST:
@@ -3732,6 +3736,7 @@ void srv_tick(sors_server* s) {
                 return;
             }
             m.new_step = true;
+            m.aimed = false;
             m.it = 0;
             m.phase = Member::ASSEMBLE;
         };
@@ -3787,9 +3792,14 @@ void srv_tick(sors_server* s) {
                     end_step();
                     break;
                 }
+                if (m.grad_only) {  // the gradient-only check failed: the tangent on the next tick
+                    m.aimed = false;
+                    break;
+                }
                 double eta = o.pcg_rtol;
                 if (o.g_tol > 0 && m.st.gnorm > 0) eta = std::min(0.1, std::max(eta, 0.5 * o.g_tol / m.st.gnorm));
                 m.eta = eta;
+                m.aimed = o.g_tol > 0 && eta > o.pcg_rtol;
                 // preconditioner only for a tangent that is solved (as minimize()
                 // does); the next tick's batched launches wait for it
                 build_preconditioner<HT>(c);

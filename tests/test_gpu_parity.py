@@ -239,8 +239,7 @@ def test_residual_check_after_aimed_solve_is_gradient_only(element):
     the residual that ended each step is the true one: ||g|| of an FP64
     gradient evaluation at the returned state, on a second rollout that
     still holds the step's history (and then takes the same step, bit for
-    bit).  Served rollouts always assemble the tangent for the check; the
-    serving tests require them to equal stand-alone rollouts bit for bit."""
+    bit)."""
     sc = mesh.beam(nx=12, ny=3, nz=3, element=element, e_range=(5e5, 5e6), seed=2)
     # g_tol such that steps end on the residual criterion (||g|| ~ 1e-7 N
     # when the step-size criterion would stop them)
