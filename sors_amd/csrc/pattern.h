@@ -81,10 +81,15 @@ struct HostPattern {
     std::vector<int> incptr;        // N + 1: node -> incident (element, local node) list
     std::vector<int> inc;           // e * ed + local index, element order per node
     // tet4 / hex8: element blocks summed into each slot of the sliced layout
-    // (empty when ed^2 E exceeds int32).  Slot position p = sbase[s] + 64 k +
-    // lane (the index of bcol) receives blocks sidx[sptr[p] .. sptr[p+1]),
-    // each ed^2 e + ed la + lb (element e's block of local nodes (la, lb)),
-    // in ascending element order.
+    // (empty when the largest entry exceeds int32).  Slot position p =
+    // sbase[s] + 64 k + lane (the index of bcol) receives blocks
+    // sidx[sptr[p] .. sptr[p+1]), in ascending element order, each naming
+    // element e's block of local nodes (la, lb):
+    //   hex8: ed^2 e + ed la + lb;
+    //   tet4: 2 (10 e + t) + f, where t = 4 a - a (a - 1) / 2 + b - a indexes
+    //   (a, b) = (min, max)(la, lb) in the row-major upper triangle (the ten
+    //   stored blocks of a symmetric element Hessian) and f = 1 when la > lb
+    //   (the stored block is read transposed).
     std::vector<int> sptr;          // sbase[nslices] + 1
     std::vector<int> sidx;
     std::vector<int> qslice;        // sbase[nslices] / 64: slice of each group of 64 slot positions

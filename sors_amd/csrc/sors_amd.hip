@@ -422,20 +422,27 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
 //   eg[4 (ed e + a) + r]   the FP64 gradient row of local node a (r < 3; one
 //                          double of padding so a row is two 16-byte loads),
 //   eb[blk (ed^2 e + ed a + b) + i]  its 3x3 block (a, b), i < 9, padded to
-//                          blk values (a 16-byte multiple).
+//                          blk values (a 16-byte multiple) -- hex8;
+//   eb[blk (10 e + t(a, b)) + i]  tet4: the ten blocks a <= b only,
+//                          t(a, b) = 4 a - a (a - 1) / 2 + b - a (the element
+//                          Hessian is symmetric: block (b, a) is the
+//                          transpose of block (a, b)).
 // tet4: k_tet_elem, one lane per element.  hex8: k_hex_grad (one lane per
 // element, FP64 gradient rows) and k_hex_rows (one lane per element row a,
 // its eight blocks summed over the eight Gauss points).
 // Phase 2, MODE 0 (k_slot_sums): one lane per slot of the sliced layout sums
 // the element blocks listed for that slot (pattern.h: sptr / sidx, ascending
-// element order) in registers and stores the block once; the lane of a
-// node's own slot also sums the node's gradient rows.  MODE 1
+// element order; a tet4 entry may name its stored block transposed) in
+// registers and stores the block once; the lane of a node's own slot also
+// sums the node's gradient rows.  MODE 1
 // (k_grad_gather): one lane per node sums its gradient rows.  No atomics, no
 // read-modify-write of the layout, every value written by exactly one lane,
 // fixed-order sums.  MODE 0: gradient + Hessian; MODE 1: gradient only.
 // stored values per block: 9 padded to a multiple of 16 bytes
 template <typename HT>
 constexpr int tet_blk() { return sizeof(HT) == 4 ? 12 : 10; }
+// stored blocks per element: tet4 keeps the upper triangle
+constexpr int elem_blocks(int ed) { return ed == 4 ? 10 : ed * ed; }
 
 template <typename HT, int PROJ, int MODE>
 __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT* eb) {
@@ -526,11 +533,12 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
             for (int r = 0; r < 3; ++r)
                 h[k][r] = (HT)(p.FiT(r, 0) * g[k][0] + p.FiT(r, 1) * g[k][1] + p.FiT(r, 2) * g[k][2]);
     }
-    HT* be = eb + (size_t)tet_blk<HT>() * 16 * e;
+    HT* be = eb + (size_t)tet_blk<HT>() * 10 * e;
+    int t = 0;  // t(la, lb): row-major upper triangle
 #pragma unroll
     for (int la = 0; la < 4; ++la)
 #pragma unroll
-        for (int lb = 0; lb < 4; ++lb) {
+        for (int lb = la; lb < 4; ++lb) {
             HT blk[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) blk[i] = 0;
@@ -545,7 +553,8 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
                 blk[8] += s_ab;
             }
 #pragma unroll
-            for (int i = 0; i < 9; ++i) be[tet_blk<HT>() * (4 * la + lb) + i] = blk[i];
+            for (int i = 0; i < 9; ++i) be[tet_blk<HT>() * t + i] = blk[i];
+            ++t;
         }
 }
 
@@ -596,8 +605,9 @@ __global__ __launch_bounds__(kBlock) void k_grad_gather(ElemArgs a, const int* i
 }
 
 // one lane per slot position p = sbase[s] + 64 k + lane (groups of 64 = one
-// wavefront = slot k of slice s)
-template <typename HT>
+// wavefront = slot k of slice s).  SYM (tet4): a list entry is 2 j + f, the
+// stored block j, transposed when f = 1; otherwise the entry is the block.
+template <typename HT, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_slot_sums(ElemArgs a, int nq, const int* qslice, const int* sptr,
                                                       const int* sidx, const HT* eb, const int* selfslot,
                                                       const int* incptr, const int* inc, const double* eg) {
@@ -614,6 +624,12 @@ __global__ __launch_bounds__(kBlock) void k_slot_sums(ElemArgs a, int nq, const 
         uint4 q4[kPieces];
         HT v[tet_blk<HT>()];
     };
+    auto add = [&](const Blk& b, int code) {
+        const bool tr = SYM && (code & 1);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[i] += tr ? b.v[3 * (i % 3) + i / 3] : b.v[i];
+    };
+    auto block_of = [](int code) { return SYM ? code >> 1 : code; };
     const int c0 = sptr[p], c1 = sptr[p + 1];
     int c = c0;
     // groups of four blocks: their indices, then their values, are loaded
@@ -625,22 +641,20 @@ __global__ __launch_bounds__(kBlock) void k_slot_sums(ElemArgs a, int nq, const 
         for (int u = 0; u < 4; ++u) id[u] = sidx[c + u];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const uint4* src = (const uint4*)(eb + (size_t)tet_blk<HT>() * id[u]);
+            const uint4* src = (const uint4*)(eb + (size_t)tet_blk<HT>() * block_of(id[u]));
 #pragma unroll
             for (int i = 0; i < kPieces; ++i) blk[u].q4[i] = src[i];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int i = 0; i < 9; ++i) acc[i] += blk[u].v[i];
+        for (int u = 0; u < 4; ++u) add(blk[u], id[u]);
     }
     for (; c < c1; ++c) {
         Blk blk;
-        const uint4* src = (const uint4*)(eb + (size_t)tet_blk<HT>() * sidx[c]);
+        const int code = sidx[c];
+        const uint4* src = (const uint4*)(eb + (size_t)tet_blk<HT>() * block_of(code));
 #pragma unroll
         for (int i = 0; i < kPieces; ++i) blk.q4[i] = src[i];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) acc[i] += blk.v[i];
+        add(blk, code);
     }
     HT* vals = (HT*)a.vals;
     const long long vb = 9LL * a.sbase[s] + (long long)kSlice * 9 * k + lane;
@@ -2629,7 +2643,8 @@ void ensure_elem_rows(sors_ctx* c, size_t hbytes) {
     if (!node_assembly(c)) return;
     ensure(c->teg, sizeof(double) * 4 * c->ed * (size_t)c->E);
     if (hbytes)
-        ensure(c->teb, hbytes * (hbytes == 4 ? tet_blk<float>() : tet_blk<double>()) * c->ed * c->ed * (size_t)c->E);
+        ensure(c->teb, hbytes * (hbytes == 4 ? tet_blk<float>() : tet_blk<double>()) * elem_blocks(c->ed) *
+                           (size_t)c->E);
 }
 
 // element kernel launch; with e0/e1 the dispatches record the two events
@@ -2661,7 +2676,8 @@ void launch_elem(sors_ctx* c, const ElemArgs& a, int project, hipEvent_t e0 = nu
         const int* inc = c->inc.as<int>();
         if (MODE == 0) {
             const int nq = c->pat.sbase[c->pat.nslices] / kSlice;
-            hipExtLaunchKernelGGL(k_slot_sums<HT>, dim3(grid_for((long long)nq * kSlice, kBlock)), dim3(kBlock), 0,
+            auto k2 = c->ed == 4 ? k_slot_sums<HT, true> : k_slot_sums<HT, false>;
+            hipExtLaunchKernelGGL(k2, dim3(grid_for((long long)nq * kSlice, kBlock)), dim3(kBlock), 0,
                                   c->stream, nullptr, e1, 0, a, nq, (const int*)c->tqslice.as<int>(),
                                   (const int*)c->tsptr.as<int>(), (const int*)c->tsidx.as<int>(), (const HT*)eb,
                                   (const int*)c->selfslot.as<int>(), incptr, inc, (const double*)eg);

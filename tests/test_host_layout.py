@@ -7,7 +7,9 @@ definition:
 
 - slot -> element-block lists (two-phase assembly): every kept element block
   (e, la, lb) appears exactly once, in the list of the slot of node conn[e,la]
-  whose column is conn[e,lb], lists in ascending element order;
+  whose column is conn[e,lb], lists in ascending element order (tet4 entries
+  name one of the element's ten stored upper-triangle blocks and whether it is
+  read transposed);
 - node -> incident (element, local node) lists;
 - aggregates: every free node in exactly one, Dirichlet nodes in none;
   centroids, RMS radii, the rotation flag and the FP32 rigid-body offsets;
@@ -90,7 +92,17 @@ def test_slot_lists_and_incidence(dump_tool, tmp_path, case):
     # a pair is kept exactly when both nodes are free
     assert np.array_equal(kept, nodefree[v] & nodefree[w])
     p = sbase[v >> 6].astype(np.int64) + 64 * r + (v & 63)
-    p, bid = p[kept], (ed * ed * e_idx + ed * la + lb)[kept]
+    if ed == 4:
+        # tet4: the ten stored blocks a <= b of the symmetric element Hessian,
+        # row-major; block (la, lb) with la > lb is stored block (lb, la), read transposed
+        a_, b_ = np.minimum(la, lb), np.maximum(la, lb)
+        t = 4 * a_ - a_ * (a_ - 1) // 2 + b_ - a_
+        upper = [(x, y) for x in range(4) for y in range(x, 4)]
+        assert [4 * x - x * (x - 1) // 2 + y - x for x, y in upper] == list(range(10))
+        code = 2 * (10 * e_idx + t) + (la > lb)
+    else:
+        code = ed * ed * e_idx + ed * la + lb
+    p, bid = p[kept], code[kept]
     assert np.array_equal(bcol[p], w[kept])  # the slot's column is the block's column node
     order = np.argsort(p, kind="stable")  # block ids ascend within each slot
     counts = np.bincount(p, minlength=S)
