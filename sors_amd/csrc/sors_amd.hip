@@ -631,30 +631,25 @@ __global__ __launch_bounds__(kBlock) void k_slot_sums(ElemArgs a, int nq, const 
     };
     auto block_of = [](int code) { return SYM ? code >> 1 : code; };
     const int c0 = sptr[p], c1 = sptr[p + 1];
-    int c = c0;
-    // groups of four blocks: their indices, then their values, are loaded
-    // together; the sums keep the list order
-    for (; c + 4 <= c1; c += 4) {
-        int id[4];
-        Blk blk[4];
+    // groups of up to kGroup list entries (a typical list is one group): their
+    // indices, then their blocks, are loaded together; the sums keep the list order
+    constexpr int kGroup = sizeof(HT) == 4 ? 8 : 4;
+    for (int c = c0; c < c1; c += kGroup) {
+        const int m = min(kGroup, c1 - c);
+        int id[kGroup];
+        Blk blk[kGroup];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) id[u] = sidx[c + u];
+        for (int u = 0; u < kGroup; ++u) id[u] = u < m ? sidx[c + u] : 0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint4* src = (const uint4*)(eb + (size_t)tet_blk<HT>() * block_of(id[u]));
+        for (int u = 0; u < kGroup; ++u)
+            if (u < m) {
+                const uint4* src = (const uint4*)(eb + (size_t)tet_blk<HT>() * block_of(id[u]));
 #pragma unroll
-            for (int i = 0; i < kPieces; ++i) blk[u].q4[i] = src[i];
-        }
+                for (int i = 0; i < kPieces; ++i) blk[u].q4[i] = src[i];
+            }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) add(blk[u], id[u]);
-    }
-    for (; c < c1; ++c) {
-        Blk blk;
-        const int code = sidx[c];
-        const uint4* src = (const uint4*)(eb + (size_t)tet_blk<HT>() * block_of(code));
-#pragma unroll
-        for (int i = 0; i < kPieces; ++i) blk.q4[i] = src[i];
-        add(blk, code);
+        for (int u = 0; u < kGroup; ++u)
+            if (u < m) add(blk[u], id[u]);
     }
     HT* vals = (HT*)a.vals;
     const long long vb = 9LL * a.sbase[s] + (long long)kSlice * 9 * k + lane;
