@@ -38,6 +38,7 @@ enum Scalar {
 // PCG scalars carried between iterations (two copies, used alternately)
 struct PcgState {
     double alpha, gamma, rr0, tol2;
+    double rr;  // ||r||^2 of iteration iter - 1 (iteration 0 at iter = 0): the host sizes graph chunks from it
     int iter, pad;
 };
 

@@ -1770,6 +1770,7 @@ __device__ __forceinline__ bool spmv_prologue(PcgCtl* ctl, PcgState* st, const d
         const bool zero = !(t[1] > 0.0) || !(t[0] > 0.0);
         if (lead) {
             st->rr0 = t[1];
+            st->rr = t[1];
             st->tol2 = rtol * rtol * t[1];
             st->gamma = t[0];
             st->alpha = 0.0;
@@ -1917,6 +1918,7 @@ __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgStat
         st_next->alpha = alpha;
         st_next->rr0 = st_prev->rr0;
         st_next->tol2 = st_prev->tol2;
+        st_next->rr = g[1];
         st_next->iter = st_prev->iter + 1;
     }
     const HT a = (HT)alpha, bt = (HT)beta;
@@ -2470,10 +2472,11 @@ struct sors_ctx {
     DevMem csurf, lagN, lagn, lagD, gcon;
     long long nvals = 0;            // value slots of the active layout
     // captured PCG iteration chunk (one per precision)
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    const void* gkey[2] = {nullptr, nullptr};
-    int gchunk[2] = {0, 0};
-    int gcoarse[2] = {0, 0};
+    // captured PCG chunks per precision and chunk size (pcg_check_every / 4, / 2, / 1)
+    hipGraphExec_t gexec[2][3] = {};
+    const void* gkey[2][3] = {};
+    int gchunk[2][3] = {};
+    int gcoarse[2][3] = {};
     bool block() const { return pat.node_granular; }
     // per-launch kernel timing (options.profile_kernels)
     sors_stats* prof = nullptr;
@@ -2551,7 +2554,8 @@ void ensure_pattern(sors_ctx* c, double dt) {
         c->nvals = c->pat.nnz;
     }
     ensure(c->vals, sizeof(double) * std::max<long long>(c->nvals, 1));
-    for (int k = 0; k < 2; ++k) c->gkey[k] = nullptr;  // captured graphs hold the old buffers
+    for (auto& row : c->gkey)
+        for (auto& k : row) k = nullptr;  // captured graphs hold the old buffers
     sync(c);  // host vectors of the pattern stay alive in c->pat
     c->pat_dtnz = dtnz;
     c->pat_on_device = true;
@@ -3054,33 +3058,54 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
     if (cs) launch_coarse<HT>(c, ctl, 1);
     launch_spmv<HT>(c, g, v, 1, g.gu, 0, nullptr, nullptr, inC(1), nbC);   // A_1
     HIPX(hipGetLastError());
-    int launched = 1;
+    // chunk sizes (even: the parity pattern repeats); after the first chunk
+    // the next one is the smallest size that should finish the solve at the
+    // convergence rate seen so far, so a converged solve leaves few launches
+    // that return at once
+    const int sizes[3] = {std::max(2, chunk / 8 * 2), std::max(2, chunk / 4 * 2), chunk};
+    PcgState* hst = (PcgState*)(c->h_pin + 8);  // after the PcgCtl record
+    auto next_size = [&]() {
+        const PcgState& st = hst[0].iter >= hst[1].iter ? hst[0] : hst[1];
+        if (!(st.iter > 1 && st.rr0 > 0.0 && st.rr > st.tol2 && st.rr < st.rr0 && st.tol2 > 0.0)) return chunk;
+        // st.rr belongs to iteration st.iter - 1; two iterations of margin
+        const double rate = std::log(st.rr / st.rr0) / (st.iter - 1);
+        const double left = std::log(st.tol2 / st.rr) / rate - 1.0 + 2.0;
+        for (int sz : sizes)
+            if (sz >= left) return sz;
+        return chunk;
+    };
+    int launched = 1, next = chunk;
     bool first_chunk = true, done = false;
     while (launched < max_iter) {
-        const int k = std::min(chunk, max_iter - launched);
+        const int k = std::min(next, max_iter - launched);
         const bool timed = ev && first_chunk;
-        if (!timed && c->opt.use_graph && k == chunk) {
-            if (!c->gexec[gi] || c->gkey[gi] != c->vals.p || c->gchunk[gi] != chunk ||
-                c->gcoarse[gi] != (cs ? c->coarse_gen : 0)) {
-                if (c->gexec[gi]) HIPX(hipGraphExecDestroy(c->gexec[gi]));
-                c->gexec[gi] = nullptr;
+        int si = -1;
+        for (int q = 0; q < 3; ++q)
+            if (k == sizes[q]) si = q;
+        if (!timed && c->opt.use_graph && si >= 0) {
+            hipGraphExec_t& ge = c->gexec[gi][si];
+            if (!ge || c->gkey[gi][si] != c->vals.p || c->gchunk[gi][si] != k ||
+                c->gcoarse[gi][si] != (cs ? c->coarse_gen : 0)) {
+                if (ge) HIPX(hipGraphExecDestroy(ge));
+                ge = nullptr;
                 hipGraph_t graph = nullptr;
                 HIPX(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 for (int j = 0; j < k; ++j) iteration(launched + 1 + j, j, false);  // parity pattern repeats
                 HIPX(hipStreamEndCapture(c->stream, &graph));
-                HIPX(hipGraphInstantiate(&c->gexec[gi], graph, nullptr, nullptr, 0));
+                HIPX(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
                 HIPX(hipGraphDestroy(graph));
-                c->gkey[gi] = c->vals.p;
-                c->gchunk[gi] = chunk;
-                c->gcoarse[gi] = cs ? c->coarse_gen : 0;
+                c->gkey[gi][si] = c->vals.p;
+                c->gchunk[gi][si] = k;
+                c->gcoarse[gi][si] = cs ? c->coarse_gen : 0;
             }
-            HIPX(hipGraphLaunch(c->gexec[gi], c->stream));
+            HIPX(hipGraphLaunch(ge, c->stream));
         } else {
             for (int j = 0; j < k; ++j) iteration(launched + 1 + j, j, timed);
         }
         HIPX(hipGetLastError());
         launched += k;
         HIPX(hipMemcpyAsync(hctl, ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(hst, v.st, 2 * sizeof(PcgState), hipMemcpyDeviceToHost, c->stream));
         sync(c);
         if (timed) {
             // SpMV launches after the one that ended the solve return at once: not counted
@@ -3093,6 +3118,7 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
             done = true;
             break;
         }
+        next = next_size();
     }
     if (!done) {
         HIPX(hipMemcpyAsync(hctl, ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, c->stream));
@@ -4082,8 +4108,9 @@ void sors_destroy(sors_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     for (hipEvent_t e : c->pev) (void)hipEventDestroy(e);
-    for (int k = 0; k < 2; ++k)
-        if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+    for (auto& row : c->gexec)
+        for (hipGraphExec_t g : row)
+            if (g) (void)hipGraphExecDestroy(g);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->evG) (void)hipEventDestroy(c->evG);
