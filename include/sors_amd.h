@@ -80,7 +80,9 @@ typedef struct {
     double pcg_rtol;      /* relative residual of each linear solve */
     int pcg_max_iter;
     int max_backtrack;
-    int pcg_check_every;  /* PCG iterations between host convergence checks */
+    int pcg_check_every;  /* PCG iterations before the first host convergence check; later
+                             checks come after 1/4, 1/2 or all of that many, the fewest
+                             expected to end the solve at its convergence rate so far */
     int use_graph;        /* replay PCG iteration chunks from a captured hipGraph */
     int profile_kernels;  /* time SpMV and assembly launches with HIP events */
     int predict;          /* Newton's start instead of the caller's guess: 0 the
