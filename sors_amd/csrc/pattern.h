@@ -86,14 +86,21 @@ struct HostPattern {
     // sidx[sptr[p] .. sptr[p+1]), in ascending element order, each naming
     // element e's block of local nodes (la, lb):
     //   hex8: ed^2 e + ed la + lb;
-    //   tet4: 2 (10 e + t) + f, where t = 4 a - a (a - 1) / 2 + b - a indexes
-    //   (a, b) = (min, max)(la, lb) in the row-major upper triangle (the ten
-    //   stored blocks of a symmetric element Hessian) and f = 1 when la > lb
-    //   (the stored block is read transposed).
+    //   tet4: 2 elem_row(e, t, 10) + f, where t = 4 a - a (a - 1) / 2 + b - a
+    //   indexes (a, b) = (min, max)(la, lb) in the row-major upper triangle
+    //   (the ten stored blocks of a symmetric element Hessian) and f = 1 when
+    //   la > lb (the stored block is read transposed).
     std::vector<int> sptr;          // sbase[nslices] + 1
     std::vector<int> sidx;
     std::vector<int> qslice;        // sbase[nslices] / 64: slice of each group of 64 slot positions
 };
+
+// Element rows between the two assembly phases are tiled by 64 elements (one
+// wavefront of the element kernels): row j of the R rows of element e is
+// stored at ((e / 64) R + j) 64 + e % 64, so a wavefront's stores of row j
+// cover one contiguous range.  (constexpr: also callable from device code.)
+constexpr long long elem_row(long long e, int j, int R) { return ((e >> 6) * R + j) * 64 + (e & 63); }
+constexpr long long elem_rows_padded(long long E) { return (E + 63) / 64 * 64; }
 
 // mask: 3N flags (may be null).  sqp: SQP assembly semantics (no masking).
 // Returns false (with msg) on invalid connectivity.

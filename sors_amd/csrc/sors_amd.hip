@@ -418,12 +418,13 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
 
 
 // ---- tet4 / hex8, two-phase assembly on the sliced node-block layout ------
-// Phase 1 evaluates every element once and stores, element-major:
-//   eg[4 (ed e + a) + r]   the FP64 gradient row of local node a (r < 3; one
-//                          double of padding so a row is two 16-byte loads),
+// Phase 1 evaluates every element once and stores:
+//   eg[4 elem_row(e, a, ed) + r]  the FP64 gradient row of local node a (r < 3;
+//                          one double of padding so a row is two 16-byte
+//                          loads), tiled by 64 elements (pattern.h),
 //   eb[blk (ed^2 e + ed a + b) + i]  its 3x3 block (a, b), i < 9, padded to
 //                          blk values (a 16-byte multiple) -- hex8;
-//   eb[blk (10 e + t(a, b)) + i]  tet4: the ten blocks a <= b only,
+//   eb[blk elem_row(e, t(a, b), 10) + i]  tet4: the ten blocks a <= b only,
 //                          t(a, b) = 4 a - a (a - 1) / 2 + b - a (the element
 //                          Hessian is symmetric: block (b, a) is the
 //                          transpose of block (a, b)).
@@ -501,12 +502,13 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
             for (int r = 0; r < 3; ++r)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) P(r, c) += sa * Ff[r] * fa[c];
-        double* ge = eg + 4 * (4 * (size_t)e);
 #pragma unroll
-        for (int la = 0; la < 4; ++la)
+        for (int la = 0; la < 4; ++la) {
+            double* ge = eg + 4 * elem_row(e, la, 4);
 #pragma unroll
             for (int r = 0; r < 3; ++r)
-                ge[4 * la + r] = V0 * (P(r, 0) * g[la][0] + P(r, 1) * g[la][1] + P(r, 2) * g[la][2]);
+                ge[r] = V0 * (P(r, 0) * g[la][0] + P(r, 1) * g[la][1] + P(r, 2) * g[la][2]);
+        }
     }
     if (MODE == 1) return;
     const HT w = (HT)(V0 * a.hscale);
@@ -533,7 +535,6 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
             for (int r = 0; r < 3; ++r)
                 h[k][r] = (HT)(p.FiT(r, 0) * g[k][0] + p.FiT(r, 1) * g[k][1] + p.FiT(r, 2) * g[k][2]);
     }
-    HT* be = eb + (size_t)tet_blk<HT>() * 10 * e;
     int t = 0;  // t(la, lb): row-major upper triangle
 #pragma unroll
     for (int la = 0; la < 4; ++la)
@@ -553,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
                 blk[8] += s_ab;
             }
 #pragma unroll
-            for (int i = 0; i < 9; ++i) be[tet_blk<HT>() * t + i] = blk[i];
+            for (int i = 0; i < 9; ++i) eb[tet_blk<HT>() * elem_row(e, t, 10) + i] = blk[i];
             ++t;
         }
 }
@@ -566,6 +567,9 @@ __device__ __forceinline__ void node_gradient(const ElemArgs& a, int v, const in
 #pragma unroll
     for (int r = 0; r < 3; ++r) kept[r] = dof_kept(a, 3 * v + r);
     double gacc[3] = {0.0, 0.0, 0.0};
+    // incidence code ed e + a -> stored gradient row (ed is 4 or 8)
+    const int lg = a.ed == 8 ? 3 : 2;
+    auto row_of = [&](int code) { return elem_row(code >> lg, code & (a.ed - 1), a.ed); };
     const int j0 = incptr[v], j1 = incptr[v + 1];
     int j = j0;
     for (; j + 4 <= j1; j += 4) {  // four rows in flight, summed in list order
@@ -575,7 +579,7 @@ __device__ __forceinline__ void node_gradient(const ElemArgs& a, int v, const in
         for (int u = 0; u < 4; ++u) code[u] = inc[j + u];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const double2* ge = (const double2*)(eg + 4LL * code[u]);
+            const double2* ge = (const double2*)(eg + 4 * row_of(code[u]));
             g01[u] = ge[0];
             g2[u] = ge[1];
         }
@@ -587,7 +591,7 @@ __device__ __forceinline__ void node_gradient(const ElemArgs& a, int v, const in
         }
     }
     for (; j < j1; ++j) {
-        const double2* ge = (const double2*)(eg + 4LL * inc[j]);
+        const double2* ge = (const double2*)(eg + 4 * row_of(inc[j]));
         const double2 g01 = ge[0], g2 = ge[1];
         if (kept[0]) gacc[0] += g01.x;
         if (kept[1]) gacc[1] += g01.y;
@@ -885,11 +889,10 @@ __global__ __launch_bounds__(128) void k_hex_grad(ElemArgs a, double* eg) {
 #pragma unroll
             for (int r = 0; r < 3; ++r) gr[k][r] += wq * (P(r, 0) * G[k][0] + P(r, 1) * G[k][1] + P(r, 2) * G[k][2]);
     }
-    double* ge = eg + 4 * (8 * (size_t)e);
 #pragma unroll
     for (int k = 0; k < 8; ++k)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) ge[4 * k + r] = gr[k][r];
+        for (int r = 0; r < 3; ++r) eg[4 * elem_row(e, k, 8) + r] = gr[k][r];
 }
 
 // Hessian rows of hex8 elements: eight lanes per element.  First lane j
@@ -2484,9 +2487,12 @@ struct sors_ctx {
     hipEvent_t* events(int count) {
         while ((int)pev.size() < count) {
             hipEvent_t e;
-            // timing only (read after a stream synchronisation): no system-scope
-            // fence, whose cache writeback would be timed with the kernel
-            HIPX(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            // timing events of kernels whose results the next kernel reads: a
+            // device-scope release (the per-XCD L2s are written back, as after
+            // any launch), no system-scope one.  Without any release (the
+            // disable-system-fence flag) a following kernel on another XCD
+            // could read stale lines.
+            HIPX(hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
             pev.push_back(e);
         }
         return pev.data();
@@ -2640,10 +2646,10 @@ bool node_assembly(const sors_ctx* c) {
 // element rows between the assembly phases (hbytes: bytes per Hessian value, 0: gradient only)
 void ensure_elem_rows(sors_ctx* c, size_t hbytes) {
     if (!node_assembly(c)) return;
-    ensure(c->teg, sizeof(double) * 4 * c->ed * (size_t)c->E);
+    const size_t Ep = (size_t)elem_rows_padded(c->E);  // rows are tiled by 64 elements
+    ensure(c->teg, sizeof(double) * 4 * c->ed * Ep);
     if (hbytes)
-        ensure(c->teb, hbytes * (hbytes == 4 ? tet_blk<float>() : tet_blk<double>()) * elem_blocks(c->ed) *
-                           (size_t)c->E);
+        ensure(c->teb, hbytes * (hbytes == 4 ? tet_blk<float>() : tet_blk<double>()) * elem_blocks(c->ed) * Ep);
 }
 
 // element kernel launch; with e0/e1 the dispatches record the two events

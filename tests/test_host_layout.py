@@ -99,7 +99,8 @@ def test_slot_lists_and_incidence(dump_tool, tmp_path, case):
         t = 4 * a_ - a_ * (a_ - 1) // 2 + b_ - a_
         upper = [(x, y) for x in range(4) for y in range(x, 4)]
         assert [4 * x - x * (x - 1) // 2 + y - x for x, y in upper] == list(range(10))
-        code = 2 * (10 * e_idx + t) + (la > lb)
+        # stored blocks are tiled by 64 elements: block t of element e at ((e // 64) 10 + t) 64 + e % 64
+        code = 2 * (((e_idx >> 6) * 10 + t) * 64 + (e_idx & 63)) + (la > lb)
     else:
         code = ed * ed * e_idx + ed * la + lb
     p, bid = p[kept], code[kept]
