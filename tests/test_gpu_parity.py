@@ -266,6 +266,28 @@ def test_residual_check_after_aimed_solve_is_gradient_only(element):
     assert tangents < solves + 4  # at least one final check skipped the tangent
 
 
+@pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
+def test_kernel_timing_does_not_change_results(precision):
+    """profile_kernels launches the first PCG chunk of every solve and the
+    assembly kernels with timing events; the states must equal an untimed
+    run bit for bit.  (The events carry a device-scope release: events that
+    drop it let a following kernel on another XCD read stale lines.)"""
+    sc = mesh.beam(nx=30, ny=6, nz=6, e_range=(5e5, 5e6), seed=6)
+    runs = []
+    for prof in (0, 1):
+        r = rollout(sc, precision=precision, profile_kernels=prof, g_tol=1e-8, pcg_rtol=3e-3)
+        q = sc.X.reshape(-1).copy()
+        iters = []
+        for _ in range(3):
+            q, st = r.step(q, sc.dt)
+            iters.append(st.pcg_iters)
+        runs.append((q, iters))
+        r.close()
+    say("timed vs untimed", precision, runs[0][1], runs[1][1])
+    assert runs[0][1] == runs[1][1]
+    assert np.array_equal(runs[0][0], runs[1][0])
+
+
 @pytest.mark.parametrize("element", ["tet", "hex"])
 def test_node_assembly_is_deterministic(element):
     """Two assemblies of the same state give identical gradients and
