@@ -149,6 +149,13 @@ int sors_step(sors_ctx* ctx, double* q, double dt, int substeps, sors_stats* sta
  * host transfer); read positions with sors_get_positions. */
 int sors_step_resident(sors_ctx* ctx, double dt, int substeps, sors_stats* stats);
 int sors_get_positions(sors_ctx* ctx, double* q);
+/* Solver::calculate_CFL_timestep (cpp/src/solver.cpp:300-331): dt_cfl =
+ * cfl * (shortest distance between two vertices of one element at positions
+ * q, every vertex pair as in cpp/headers/geometries/element.h:131-142) /
+ * max |v_prev| component, or dt when that is not smaller.  q == NULL uses the
+ * resident positions.  Solver::step (solver.cpp:16-20) then takes
+ * ceil(dt / dt_cfl) substeps. */
+int sors_cfl_timestep(sors_ctx* ctx, const double* q, double dt, double cfl, double* dt_cfl);
 
 /* ---- inspection entry points used by the parity tests ---------------- */
 /* Energy::compute_energy (cpp/src/energy.cpp:208-248) */

@@ -202,6 +202,22 @@ def elem_eval(Xe, xe, E, nu, project=False, ed=None):
 
 
 # ---- system level --------------------------------------------------------
+def cfl_timestep(q, conn, v_prev, dt, cfl=1.0):
+    """Solver::calculate_CFL_timestep (solver.cpp:300-331) with
+    Element::compute_minimal_edge_length (element.h:131-142): the shortest
+    distance between two vertices of one element at positions q, every vertex
+    pair, over all elements; the largest |v_prev| component; dt_cfl = cfl *
+    length / speed when that is smaller than dt, else dt."""
+    x = f64(q).reshape(-1, 3)[i32(conn)]  # E x ed x 3
+    ed = x.shape[1]
+    emin = min(float(np.sqrt(((x[:, i] - x[:, j]) ** 2).sum(axis=1)).min())
+               for i in range(ed) for j in range(i + 1, ed))
+    vmax = float(np.abs(f64(v_prev)).max())
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dtc = cfl * np.float64(emin) / np.float64(vmax)
+    return float(dtc) if dtc < dt else float(dt)
+
+
 class System:
     """Energy + TimeIntegrator state of one rollout (energy.cpp:4-204)."""
 
@@ -264,6 +280,10 @@ class System:
         q, v = np.zeros(self.n), np.zeros(self.n)
         lib().orc_get_state(self.h, _d(q), _d(v))
         return q, v
+
+    def cfl_timestep(self, q, dt, cfl=1.0):
+        """calculate_CFL_timestep at positions q with this system's v_prev."""
+        return cfl_timestep(q, self.conn, self.state()[1], dt, cfl)
 
     def energy(self, q, dt):
         q = f64(q).reshape(-1)

@@ -1,6 +1,8 @@
 // sors_facade.cpp -- host interfaces of the reference over the device C ABI.
 #include "sors_facade.h"
 
+#include <cmath>
+
 #include <fstream>
 #include <iomanip>
 #include <sstream>
@@ -318,14 +320,18 @@ std::vector<double> Solver<E>::step(const std::vector<double>& solution, Energy<
                                     int substeps) const {
     if (dt == 0.0) dt = en.settings_.dt;
     if (substeps == 0) substeps = en.settings_.substeps;
-    if (en.settings_.CFLtimeSteppingFlag)
-        throw std::runtime_error("CFL substepping is not implemented on the device path");
     if (!(dt > 0.0)) throw std::invalid_argument("dt must be positive (settings.dt is not set)");
     const bool sqpContact = en.settings_.solverMethod == "minimize_SQP" && en.has_contact();
     if (en.settings_.solverMethod != "minimize_newton" && en.settings_.solverMethod != "minimize_newton_linesearch" &&
         !sqpContact)
         throw std::runtime_error("Solver method not recognized");
     if ((int)solution.size() != 3 * en.num_vertices()) throw std::invalid_argument("solution must have 3N entries");
+    if (en.settings_.CFLtimeSteppingFlag) {
+        // solver.cpp:16-20: Courant number 1, from the positions the step starts at
+        double dtc = dt;
+        check_status(sors_cfl_timestep(en.ctx(), solution.data(), dt, 1.0, &dtc), en.ctx());
+        substeps = (int)std::ceil(dt / dtc);
+    }
     en.apply_actuation(act);
     std::vector<double> q = solution;
     sors_stats st{};

@@ -1119,6 +1119,59 @@ __global__ __launch_bounds__(kBlock) void k_elem_energy(ElemArgs a, int diff, do
 }
 
 // ===========================================================================
+// CFL time step (Solver::calculate_CFL_timestep, solver.cpp:300-331): the
+// shortest distance between two vertices of one element at positions q
+// (Element::compute_minimal_edge_length, element.h:131-142: every vertex
+// pair, so a hex's face and body diagonals too) and the largest |v_prev|
+// component.  One (min, max) pair per block in part[b], part[kMaxRedBlocks + b].
+// ===========================================================================
+template <bool MIN>
+__device__ __forceinline__ double block_extremum(double v) {
+    __shared__ double sh[kBlock / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = MIN ? fmin(v, o) : fmax(v, o);
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = MIN ? fmin(v, sh[w]) : fmax(v, sh[w]);
+    __syncthreads();
+    return v;  // valid in thread 0
+}
+
+template <int ED>
+__global__ __launch_bounds__(kBlock) void k_cfl_partials(int E, const int* conn, const double* q, int n,
+                                                         const double* v, double* part) {
+    double emin = INFINITY, vmax = 0.0;
+    const int stride = gridDim.x * blockDim.x;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
+        double x[ED][3];
+#pragma unroll
+        for (int k = 0; k < ED; ++k) {
+            const int nd = conn[(size_t)k * E + e];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[k][c] = q[3 * nd + c];
+        }
+#pragma unroll
+        for (int i = 0; i < ED; ++i)
+#pragma unroll
+            for (int j = i + 1; j < ED; ++j) {
+                const double d0 = x[i][0] - x[j][0], d1 = x[i][1] - x[j][1], d2 = x[i][2] - x[j][2];
+                emin = fmin(emin, sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+            }
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) vmax = fmax(vmax, fabs(v[i]));
+    emin = block_extremum<true>(emin);
+    vmax = block_extremum<false>(vmax);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = emin;
+        part[kMaxRedBlocks + blockIdx.x] = vmax;
+    }
+}
+
+// ===========================================================================
 // Node kernels: inertia, gravity, damping, external force, Dirichlet rows
 // (integrator.cpp:45-124, energy.cpp:282-306, 381-388)
 // ===========================================================================
@@ -4242,6 +4295,37 @@ int sors_gradient(sors_ctx* c, const double* q, double dt, double* g) {
     assemble<double>(c, c->qtmp.as<double>(), dt, false, 0);
     HIPX(hipMemcpyAsync(g, c->g.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
     sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_cfl_timestep(sors_ctx* c, const double* q, double dt, double cfl, double* dt_cfl) {
+    SORS_GUARD(c)
+    if (!dt_cfl) return fail(c, SORS_ERR_ARG, "null argument");
+    if (c->ed != 4 && c->ed != 8) return fail(c, SORS_ERR_UNSUPPORTED, "CFL time step: tet4 and hex8 only");
+    const double* qd = c->q.as<double>();
+    if (q) {
+        upload(c->qtmp, q, (size_t)c->n, c->stream);
+        qd = c->qtmp.as<double>();
+    }
+    const int nb = red_grid(std::max<long long>(c->E, c->n), kBlock);
+    double* part = c->partials.as<double>();
+    if (c->ed == 4)
+        k_cfl_partials<4><<<nb, kBlock, 0, c->stream>>>(c->E, c->conn.as<int>(), qd, c->n, c->vprev.as<double>(), part);
+    else
+        k_cfl_partials<8><<<nb, kBlock, 0, c->stream>>>(c->E, c->conn.as<int>(), qd, c->n, c->vprev.as<double>(), part);
+    HIPX(hipGetLastError());
+    std::vector<double> h((size_t)kMaxRedBlocks + nb);
+    HIPX(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    double emin = INFINITY, vmax = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        emin = std::fmin(emin, h[b]);
+        vmax = std::fmax(vmax, h[(size_t)kMaxRedBlocks + b]);
+    }
+    // solver.cpp:325-330 (no motion or no elements: dt)
+    const double dtc = cfl * emin / vmax;
+    *dt_cfl = dtc < dt ? dtc : dt;
     return SORS_OK;
     SORS_GUARD_END(c)
 }

@@ -138,6 +138,7 @@ def lib():
         L.sors_get_state.argtypes = [vp, _dp, _dp]
         L.sors_step.argtypes = [vp, _dp, C.c_double, C.c_int, C.POINTER(Stats)]
         L.sors_step_resident.argtypes = [vp, C.c_double, C.c_int, C.POINTER(Stats)]
+        L.sors_cfl_timestep.argtypes = [vp, _dp, C.c_double, C.c_double, _dp]
         L.sors_energy.argtypes = [vp, _dp, C.c_double, _dp]
         L.sors_gradient.argtypes = [vp, _dp, C.c_double, _dp]
         L.sors_pattern_nnz.argtypes = [vp, C.c_double]
@@ -356,6 +357,14 @@ class Rollout:
         q = np.zeros(self.n)
         self._check(lib().sors_get_positions(self.h, _d(q)))
         return q
+
+    def cfl_timestep(self, q, dt, cfl=1.0):
+        """Solver::calculate_CFL_timestep at positions q (None: the resident
+        ones) with this rollout's v_prev."""
+        out = np.zeros(1)
+        qa = None if q is None else f64(q).reshape(-1)
+        self._check(lib().sors_cfl_timestep(self.h, _d(qa), float(dt), float(cfl), _d(out)))
+        return float(out[0])
 
     # ---- stepping --------------------------------------------------------
     def step(self, q, dt, substeps=1):
