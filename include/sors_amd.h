@@ -35,8 +35,18 @@ extern "C" {
 enum { SORS_BACKWARD_EULER = 0, SORS_CRANK_NICOLSON = 1 };
 /* SimulationSettings::solverMethod (cpp/src/solver.cpp:28-40).  The line
  * search variant keeps minimize_newton's Dirichlet handling; SQP selects the
- * unmasked assembly semantics (energy.cpp:275,337) for inspection calls. */
-enum { SORS_MINIMIZE_NEWTON = 0, SORS_MINIMIZE_NEWTON_LINESEARCH = 1, SORS_MINIMIZE_SQP = 2 };
+ * unmasked assembly semantics (energy.cpp:275,337) for inspection calls.
+ * FROZENHESSIAN (minimize_frozenhessian, solver.cpp:268-294): the tangent of
+ * the step's initial guess, and its preconditioner, serve every iteration;
+ * each iteration assembles the gradient alone; no line search; stopping after
+ * max_newton iterations without ||dx|| < dx_tol is not an error (the
+ * reference prints a message and returns the iterate). */
+enum {
+    SORS_MINIMIZE_NEWTON = 0,
+    SORS_MINIMIZE_NEWTON_LINESEARCH = 1,
+    SORS_MINIMIZE_SQP = 2,
+    SORS_MINIMIZE_FROZENHESSIAN = 3
+};
 /* ElementEnergy plugins (cpp/headers/energies/elementEnergy.h): compressible
  * Neo-Hookean, and the polynomial test energy of the reference's unit tests
  * (cpp/src/_test.cpp:8-66). */
@@ -63,7 +73,7 @@ typedef struct {
     double grav[3];
     double damping_alpha;
     int scheme;             /* SORS_BACKWARD_EULER | SORS_CRANK_NICOLSON */
-    int method;             /* SORS_MINIMIZE_NEWTON[_LINESEARCH] | SORS_MINIMIZE_SQP */
+    int method;             /* SORS_MINIMIZE_NEWTON[_LINESEARCH] | _SQP | _FROZENHESSIAN */
 } sors_system_desc;
 
 /* Solver controls.  Defaults (sors_default_options): mixed precision, SPD

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 BE, CN = 0, 1
-NEWTON, NEWTON_LS, SQP = 0, 1, 2
+NEWTON, NEWTON_LS, SQP, FROZEN = 0, 1, 2, 3
 LDLT, PCG = 0, 1
 TEST_POLY = -4
 ERRORS = {

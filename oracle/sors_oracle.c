@@ -1288,10 +1288,57 @@ void orc_update_state(orc_sys* s, const double* q, double dt) {
     }
 }
 
+/* minimize_frozenhessian (solver.cpp:268-294): the Hessian of the initial
+ * guess serves every iteration; x += H0^-1 (-g(x)) until ||dx|| < dx_tol (or
+ * the optional residual criterion); no line search; running out of
+ * iterations returns the iterate without an error, as the reference does. */
+static int orc_minimize_frozen(orc_sys* s, double* x, double dt, const orc_opts* o, orc_stats* st) {
+    int n = s->n;
+    orc_pattern(s, dt);
+    double* g = (double*)malloc(sizeof(double) * n);
+    double* dx = (double*)malloc(sizeof(double) * n);
+    double* rhs = (double*)malloc(sizeof(double) * n);
+    double* vals = (double*)malloc(sizeof(double) * (size_t)(s->nnz > 0 ? s->nnz : 1));
+    int status = orc_hessian(s, x, dt, o->project, vals);
+    for (int it = 0; status == ORC_OK && it < o->max_iter; ++it) {
+        orc_gradient(s, x, dt, g);
+        st->gnorm = norm2(n, g);
+        if (o->g_tol > 0 && st->gnorm < o->g_tol) break;
+        for (int i = 0; i < n; ++i) rhs[i] = -g[i];
+        if (o->linear_solver == ORC_LIN_LDLT) {
+            if (ldlt_solve(n, s->rowptr, s->cols, vals, rhs, dx) != ORC_OK) {
+                for (int i = 0; i < n; ++i) dx[i] = NAN;
+            }
+        } else {
+            long long k = pcg_solve(n, s->rowptr, s->cols, vals, rhs, dx, o->pcg_rtol, o->pcg_max_iter, s->nthreads);
+            if (k < 0) {
+                status = ORC_ERR_SOLVE;
+                break;
+            }
+            st->pcg_iters += k;
+        }
+        st->newton_iters++;
+        double dxn = norm2(n, dx);
+        if (dxn != dxn) {
+            status = ORC_ERR_NAN;
+            break;
+        }
+        for (int i = 0; i < n; ++i) x[i] += dx[i];
+        st->dxnorm = dxn;
+        if (dxn < o->dx_tol) break;
+    }
+    free(g);
+    free(dx);
+    free(rhs);
+    free(vals);
+    return status;
+}
+
 /* Newton minimisation: minimize_newton (solver.cpp:56-105) and
  * minimize_newton_linesearch (:201-264, Armijo alpha0 = 1, beta = 0.5,
  * c = 0.1) with the minimize_newton Dirichlet handling. */
 int orc_minimize(orc_sys* s, double* x, double dt, const orc_opts* o, orc_stats* st) {
+    if (s->method == ORC_FROZEN) return orc_minimize_frozen(s, x, dt, o, st);
     int n = s->n;
     orc_pattern(s, dt);
     double* g = (double*)malloc(sizeof(double) * n);

@@ -35,9 +35,10 @@ extern "C" {
 
 enum { ORC_BE = 0, ORC_CN = 1 };
 /* solverMethod: minimize_newton, minimize_newton_linesearch (with the
- * minimize_newton Dirichlet handling), and the SQP assembly semantics (no
- * masking of element terms, no Dirichlet override). */
-enum { ORC_NEWTON = 0, ORC_NEWTON_LS = 1, ORC_SQP = 2 };
+ * minimize_newton Dirichlet handling), the SQP assembly semantics (no
+ * masking of element terms, no Dirichlet override), and
+ * minimize_frozenhessian (the minimize_newton assembly semantics). */
+enum { ORC_NEWTON = 0, ORC_NEWTON_LS = 1, ORC_SQP = 2, ORC_FROZEN = 3 };
 enum { ORC_LIN_LDLT = 0, ORC_LIN_PCG = 1 };
 enum {
     ORC_OK = 0,

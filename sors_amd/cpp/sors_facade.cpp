@@ -214,6 +214,8 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
         d.method = SORS_MINIMIZE_NEWTON;
     else if (settings.solverMethod == "minimize_newton_linesearch")
         d.method = SORS_MINIMIZE_NEWTON_LINESEARCH;
+    else if (settings.solverMethod == "minimize_frozenhessian")
+        d.method = SORS_MINIMIZE_FROZENHESSIAN;
     else if (settings.solverMethod == "minimize_SQP")
         // the reference's contact solver; with contact the device runs its
         // penalty formulation under Newton + line search
@@ -322,8 +324,9 @@ std::vector<double> Solver<E>::step(const std::vector<double>& solution, Energy<
     if (substeps == 0) substeps = en.settings_.substeps;
     if (!(dt > 0.0)) throw std::invalid_argument("dt must be positive (settings.dt is not set)");
     const bool sqpContact = en.settings_.solverMethod == "minimize_SQP" && en.has_contact();
+    const bool frozen = en.settings_.solverMethod == "minimize_frozenhessian";
     if (en.settings_.solverMethod != "minimize_newton" && en.settings_.solverMethod != "minimize_newton_linesearch" &&
-        !sqpContact)
+        !frozen && !sqpContact)
         throw std::runtime_error("Solver method not recognized");
     if ((int)solution.size() != 3 * en.num_vertices()) throw std::invalid_argument("solution must have 3N entries");
     if (en.settings_.CFLtimeSteppingFlag) {
@@ -337,6 +340,13 @@ std::vector<double> Solver<E>::step(const std::vector<double>& solution, Energy<
     sors_stats st{};
     check_status(sors_step(en.ctx(), q.data(), dt, substeps, &st), en.ctx());
     last_stats = st;
+    if (frozen) {
+        // solver.cpp:290-292: the frozen iteration reports, not raises, a miss
+        sors_options o{};
+        check_status(sors_get_options(en.ctx(), &o), en.ctx());
+        if (!(st.dxnorm < o.dx_tol) && !(o.g_tol > 0 && st.gnorm < o.g_tol))
+            std::cout << "Frozen Newton's method did not converge" << std::endl;
+    }
     return q;
 }
 

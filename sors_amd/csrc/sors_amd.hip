@@ -3263,9 +3263,55 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Solver::minimize_frozenhessian (solver.cpp:268-294): the tangent at the
+// initial guess is assembled, and its preconditioner built, once; every
+// iteration assembles the gradient alone and solves with that tangent; no
+// line search.  max_newton iterations without ||dx|| < dx_tol end the step
+// without an error, as the reference returns its iterate.
+template <typename HT>
+int minimize_frozen(sors_ctx* c, double dt, sors_stats* stt) {
+    const sors_options& o = c->opt;
+    double* q = c->q.as<double>();
+    double* g = c->g.as<double>();
+    double* dx = c->dxd.as<double>();
+    for (int it = 0; it < o.max_newton; ++it) {
+        auto t0 = std::chrono::steady_clock::now();
+        clear_flag(c);
+        assemble<HT>(c, q, dt, it == 0, o.project_spd);
+        const double gg = read_scalar(c, S_GG);
+        if (read_flag(c) & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
+        stt->gnorm = std::sqrt(gg);
+        if (!(gg == gg)) return fail(c, SORS_ERR_NAN, "Frozen Newton's method diverged (NaN residual)");
+        if (o.g_tol > 0 && stt->gnorm < o.g_tol) {
+            stt->ms_assemble += ms_since(t0);
+            break;
+        }
+        if (it == 0) build_preconditioner<HT>(c);
+        stt->ms_assemble += ms_since(t0);
+        auto t1 = std::chrono::steady_clock::now();
+        bool brk = false;
+        stt->pcg_iters += pcg<HT>(c, g, -1.0, o.pcg_rtol, o.pcg_max_iter, &brk);
+        enqueue_step_dots<HT>(c, g, dx);
+        double dots[2];
+        read_scalars(c, dots, S_DD, 2);
+        stt->ms_solve += ms_since(t1);
+        stt->newton_iters++;
+        const double dxn = std::sqrt(dots[0]);
+        if (!(dxn == dxn)) return fail(c, SORS_ERR_NAN, "Frozen Newton's method diverged");
+        if (brk && dxn == 0.0) return fail(c, SORS_ERR_NAN, "PCG breakdown (indefinite tangent)");
+        k_axpy<<<red_grid(c->n, kBlock), kBlock, 0, c->stream>>>(c->n, 1.0, dx, q);
+        HIPX(hipGetLastError());
+        stt->dxnorm = dxn;
+        if (dxn < o.dx_tol) break;
+    }
+    return SORS_OK;
+}
+
 // minimize_newton / minimize_newton_linesearch on the device state c->q
+// (minimize_frozenhessian above)
 template <typename HT>
 int minimize(sors_ctx* c, double dt, sors_stats* stt) {
+    if (c->method == SORS_MINIMIZE_FROZENHESSIAN) return minimize_frozen<HT>(c, dt, stt);
     const sors_options& o = c->opt;
     double* q = c->q.as<double>();
     double* g = c->g.as<double>();
@@ -4043,7 +4089,7 @@ int sors_create(const sors_system_desc* d, sors_ctx** out) {
         g_create_error = "Unknown time stepping scheme";
         return SORS_ERR_ARG;
     }
-    if (d->method < SORS_MINIMIZE_NEWTON || d->method > SORS_MINIMIZE_SQP) {
+    if (d->method < SORS_MINIMIZE_NEWTON || d->method > SORS_MINIMIZE_FROZENHESSIAN) {
         g_create_error = "Solver method not recognized";
         return SORS_ERR_ARG;
     }
@@ -4862,6 +4908,7 @@ int sors_server_submit(sors_server* s, const sors_system_desc* d, const double* 
     if (d->n_nodes <= 0 || d->n_elems <= 0 || !d->rest || !d->conn) return bad("empty or missing mesh arrays");
     if (!(dt > 0.0) || steps < 1) return bad("dt must be > 0 and steps >= 1");
     if (d->method == SORS_MINIMIZE_SQP) return bad("minimize_SQP (OSQP contact) is not implemented on the device");
+    if (d->method == SORS_MINIMIZE_FROZENHESSIAN) return bad("the server batches Newton solves: minimize_frozenhessian is not served");
     std::unique_ptr<Request> r(new Request());
     r->N = d->n_nodes;
     r->E = d->n_elems;

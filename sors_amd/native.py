@@ -19,14 +19,14 @@ OK = 0
 ERR_INVERTED, ERR_NOCONV, ERR_NAN, ERR_LINESEARCH, ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6, -7
 ERR_TIMEOUT, ERR_BUDGET = -8, -9
 BACKWARD_EULER, CRANK_NICOLSON = 0, 1
-MINIMIZE_NEWTON, MINIMIZE_NEWTON_LINESEARCH, MINIMIZE_SQP = 0, 1, 2
+MINIMIZE_NEWTON, MINIMIZE_NEWTON_LINESEARCH, MINIMIZE_SQP, MINIMIZE_FROZENHESSIAN = 0, 1, 2, 3
 NEOHOOKEAN, TEST_POLY = 0, 1
 FP64, MIXED = 0, 1
 KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_COPY, KERNEL_PCG_UPDATE = 0, 1, 2, 3
 
 SCHEMES = {"backward_euler": BACKWARD_EULER, "crank_nicolson": CRANK_NICOLSON}
 METHODS = {"minimize_newton": MINIMIZE_NEWTON, "minimize_newton_linesearch": MINIMIZE_NEWTON_LINESEARCH,
-           "minimize_SQP": MINIMIZE_SQP}
+           "minimize_SQP": MINIMIZE_SQP, "minimize_frozenhessian": MINIMIZE_FROZENHESSIAN}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

@@ -267,6 +267,32 @@ def test_residual_check_after_aimed_solve_is_gradient_only(element):
 
 
 @pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
+@pytest.mark.parametrize("element", ["tet", "hex"])
+def test_frozen_hessian_trajectory(orc, element, precision):
+    """minimize_frozenhessian (solver.cpp:268-294): one tangent per step,
+    gradient-only iterations; states within 1e-6 of the bounding-box
+    diagonal of the oracle's frozen iteration, and more iterations than
+    Newton takes on the same steps."""
+    sc = mesh.beam(nx=8, ny=2, nz=2, element=element, e_range=(5e5, 5e6), seed=3, steps=10)
+    o = orc.System.from_scene(sc, method=orc.FROZEN)
+    r = rollout(sc, method=native.MINIMIZE_FROZENHESSIAN, precision=precision, pcg_rtol=1e-8)
+    rn = rollout(sc, method=native.MINIMIZE_NEWTON, precision=precision, pcg_rtol=1e-8)
+    q = qn = qo = sc.X.reshape(-1).copy()
+    worst, it_f, it_n = 0.0, 0, 0
+    for _ in range(sc.steps):
+        qo, _ = o.step(qo, sc.dt)
+        q, st = r.step(q, sc.dt)
+        qn, stn = rn.step(qn, sc.dt)
+        it_f += st.newton_iters
+        it_n += stn.newton_iters
+        worst = max(worst, np.abs(q - qo).max())
+    say("frozen", element, precision, worst / sc.bbox_diag, it_f, it_n)
+    assert worst <= 1e-6 * sc.bbox_diag
+    assert it_f > it_n
+    assert np.abs(q - sc.X.reshape(-1)).max() > 1e-6
+
+
+@pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
 def test_kernel_timing_does_not_change_results(precision):
     """profile_kernels launches the first PCG chunk of every solve and the
     assembly kernels with timing events; the states must equal an untimed

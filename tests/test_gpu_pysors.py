@@ -56,6 +56,33 @@ def test_cantilever_driver(orc, mesh_type, coarse):
     assert solver.last_stats["newton_iters"] >= 1
 
 
+def test_frozen_hessian_driver(orc):
+    """settings.solverMethod = "minimize_frozenhessian" through the drop-in,
+    against the oracle's frozen iteration."""
+    ps = load()
+    sc = mesh.beam(nx=10, ny=2, nz=2, element="tet", steps=10)
+    settings = ps.SimulationSettings()
+    settings.meshType = "Tetrahedron"
+    settings.timeSteppingScheme = "backward_euler"
+    settings.solverMethod = "minimize_frozenhessian"
+    settings.dt = sc.dt
+    energies = [set(["neohookean"]) for _ in range(sc.n_elems)]
+    params = [ps.Params(["density", "youngsModulus", "poissonsRatio"], [1100.0, 1e6, 0.45])
+              for _ in range(sc.n_elems)]
+    cparams = ps.Params(["neumannBCmask", "neumannBCvalue"], [sc.bcmask.astype(float), sc.bcvalue])
+    energy = ps.Energy(sc.X, sc.conn, settings, sc.grav, 0.0, energies, params, ["neumannBC"], cparams, [],
+                       ps.Params())
+    energy.set_initial_velocity(sc.v0.reshape(-1, 3))
+    solver = ps.Solver(settings)
+    o = orc.System.from_scene(sc, method=orc.FROZEN)
+    q = qo = sc.X.reshape(-1).copy()
+    for _ in range(sc.steps):
+        q = solver.step(q, energy, 0.0)
+        qo, _ = o.step(qo, sc.dt)
+    assert np.abs(q - qo).max() <= 1e-6 * sc.bbox_diag
+    assert solver.last_stats["newton_iters"] >= 2
+
+
 def test_inverted_element_message(orc):
     ps = load()
     X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.01
