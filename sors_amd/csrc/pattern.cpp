@@ -225,8 +225,8 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
         // node -> (element, local index) incidence for the per-node gradient sums
         P.incptr = std::move(iptr);
         P.inc = std::move(inc);
-        // largest list entry: 2 (10 E') + 1 for tet4 (E' = E padded to 64), ed^2 E for hex8
-        const long long top = ed == 4 ? 20LL * elem_rows_padded(E) : (long long)ed * ed * E;
+        // largest list entry: 2 (10 E') + 1 for tet4, 64 E' for hex8 (E' = E padded to 64)
+        const long long top = (ed == 4 ? 20LL : 64LL) * elem_rows_padded(E);
         if ((ed == 4 || ed == 8) && top <= 0x7fffffffLL) {
             // slot -> element blocks (counting sort by slot position; elements ascending)
             const long long S = P.sbase[ns];
@@ -260,7 +260,7 @@ bool build_pattern(int N, int E, int ed, const int* conn, const int* mask, bool 
                             const int t = 4 * a - a * (a - 1) / 2 + b - a;
                             P.sidx[fill[p]++] = (int)(2 * elem_row(e, t, 10) + (la > lb ? 1 : 0));
                         } else {
-                            P.sidx[fill[p]++] = ed * ed * e + ed * la + lb;
+                            P.sidx[fill[p]++] = (int)elem_row(e, ed * la + lb, ed * ed);
                         }
                     }
         }

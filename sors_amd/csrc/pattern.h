@@ -85,7 +85,7 @@ struct HostPattern {
     // sbase[s] + 64 k + lane (the index of bcol) receives blocks
     // sidx[sptr[p] .. sptr[p+1]), in ascending element order, each naming
     // element e's block of local nodes (la, lb):
-    //   hex8: ed^2 e + ed la + lb;
+    //   hex8: elem_row(e, ed la + lb, ed^2);
     //   tet4: 2 elem_row(e, t, 10) + f, where t = 4 a - a (a - 1) / 2 + b - a
     //   indexes (a, b) = (min, max)(la, lb) in the row-major upper triangle
     //   (the ten stored blocks of a symmetric element Hessian) and f = 1 when

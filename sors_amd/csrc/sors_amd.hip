@@ -422,8 +422,8 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
 //   eg[4 elem_row(e, a, ed) + r]  the FP64 gradient row of local node a (r < 3;
 //                          one double of padding so a row is two 16-byte
 //                          loads), tiled by 64 elements (pattern.h),
-//   eb[blk (ed^2 e + ed a + b) + i]  its 3x3 block (a, b), i < 9, padded to
-//                          blk values (a 16-byte multiple) -- hex8;
+//   eb[blk elem_row(e, 8 a + b, 64) + i]  hex8: its 3x3 block (a, b), i < 9,
+//                          padded to blk values (a 16-byte multiple);
 //   eb[blk elem_row(e, t(a, b), 10) + i]  tet4: the ten blocks a <= b only,
 //                          t(a, b) = 4 a - a (a - 1) / 2 + b - a (the element
 //                          Hessian is symmetric: block (b, a) is the
@@ -999,11 +999,10 @@ __global__ __launch_bounds__(kHexRowThreads) void k_hex_rows(ElemArgs a, HT* eb)
             }
         }
     }
-    HT* be = eb + (size_t)tet_blk<HT>() * (64 * (size_t)e + 8 * la);
 #pragma unroll
     for (int lb = 0; lb < 8; ++lb)
 #pragma unroll
-        for (int i = 0; i < 9; ++i) be[tet_blk<HT>() * lb + i] = acc[lb][i];
+        for (int i = 0; i < 9; ++i) eb[tet_blk<HT>() * elem_row(e, 8 * la + lb, 64) + i] = acc[lb][i];
 }
 
 // ---- element energy: absolute (diff = 0) or E(q + alpha dq) - E(q) -------

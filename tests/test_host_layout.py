@@ -102,7 +102,8 @@ def test_slot_lists_and_incidence(dump_tool, tmp_path, case):
         # stored blocks are tiled by 64 elements: block t of element e at ((e // 64) 10 + t) 64 + e % 64
         code = 2 * (((e_idx >> 6) * 10 + t) * 64 + (e_idx & 63)) + (la > lb)
     else:
-        code = ed * ed * e_idx + ed * la + lb
+        # hex8: all 64 blocks, tiled by 64 elements: block 8 la + lb of element e
+        code = ((e_idx >> 6) * ed * ed + ed * la + lb) * 64 + (e_idx & 63)
     p, bid = p[kept], code[kept]
     assert np.array_equal(bcol[p], w[kept])  # the slot's column is the block's column node
     order = np.argsort(p, kind="stable")  # block ids ascend within each slot
