@@ -59,7 +59,7 @@ def run_gpu(sc, precision, steps, warmup, profile, **options):
         r.step_resident(sc.dt)
     agg = {"newton_iters": 0, "pcg_iters": 0, "backtracks": 0, "spmv_ms": 0.0, "spmv_launches": 0,
            "assemble_kernel_ms": 0.0, "assemble_launches": 0, "ms_assemble": 0.0, "ms_solve": 0.0,
-           "ms_linesearch": 0.0}
+           "ms_linesearch": 0.0, "fp64_fallbacks": 0}
     t0 = time.perf_counter()
     for _ in range(steps):
         st = r.step_resident(sc.dt)
@@ -205,7 +205,7 @@ def contact_leg(args, opts):
     def run(precision, steps):
         r = native.Rollout.from_scene(sc, precision=precision, **opts)
         r.set_contact(**cfg)
-        agg = {"newton_iters": 0, "pcg_iters": 0, "backtracks": 0}
+        agg = {"newton_iters": 0, "pcg_iters": 0, "backtracks": 0, "fp64_fallbacks": 0}
         t0 = time.perf_counter()
         for k in range(steps):
             r.set_disk(0, p.centers[k + 1], (0.0, 0.0, -1.0), p.radius)
@@ -225,6 +225,7 @@ def contact_leg(args, opts):
            "workload": f"{sc.name}: {sc.n_elems} tet4, {sc.n_nodes} nodes, {len(p.surface)} surface nodes; "
                        f"ground penalty 1e5 N/m, mu 0.5, disk r 0.01 m at 0.02 m/s, dt 5e-4, mixed",
            "newton_per_step": agg["newton_iters"] / sc.steps, "pcg_per_step": agg["pcg_iters"] / sc.steps,
+           "fp64_fallbacks": agg["fp64_fallbacks"],
            "max_ground_penetration_m": float(max(0.0, -q[:, 2].min())),
            "indentation_m": float(sc.X[:, 2].max() - q[under, 2].min()) if under.any() else 0.0}
     log(f"contact: {out}")

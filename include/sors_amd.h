@@ -130,6 +130,8 @@ typedef struct {
     double assemble_kernel_ms; /* summed element-assembly kernel time of the tangent (gradient +
                                   Hessian) assemblies (profile_kernels) */
     long long assemble_launches; /* tangent assemblies timed; gradient-only residual checks are not */
+    int fp64_fallbacks;   /* mixed-precision Newton solves that did not converge within max_newton
+                             and were continued from their iterate in FP64 (see sors_step) */
 } sors_stats;
 
 typedef struct sors_ctx sors_ctx;
@@ -153,7 +155,10 @@ int sors_get_state(sors_ctx* ctx, double* q_prev, double* v_prev);
 
 /* Solver::step (cpp/src/solver.cpp:5-44): q is the initial guess on entry
  * and the new positions on return; dt == 0 is rejected (the caller resolves
- * settings.dt); substeps >= 1.  stats may be NULL. */
+ * settings.dt); substeps >= 1.  stats may be NULL.  In mixed precision, a
+ * Newton solve that does not converge within max_newton iterations goes on
+ * from its iterate with FP64 tangents and solves (another max_newton at
+ * most) before "Newton's method did not converge" is returned. */
 int sors_step(sors_ctx* ctx, double* q, double dt, int substeps, sors_stats* stats);
 /* Same, starting from the device-resident state of the previous step (no
  * host transfer); read positions with sors_get_positions. */

@@ -142,6 +142,7 @@ void bind_solver(py::module_& m, const char* name) {
             d["gnorm"] = t.gnorm;
             d["dxnorm"] = t.dxnorm;
             d["ms_total"] = t.ms_total;
+            d["fp64_fallbacks"] = t.fp64_fallbacks;
             return d;
         });
 }

@@ -3140,7 +3140,11 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
         int si = -1;
         for (int q = 0; q < 3; ++q)
             if (k == sizes[q]) si = q;
-        if (!timed && c->opt.use_graph && si >= 0) {
+        // while a coarse factorisation is in flight on the second stream the
+        // chunk is launched directly: replayed graphs overlapping it gave
+        // runs that differed in the last bits (500-step contact rollouts,
+        // FP64 and mixed; direct launches and synchronous refreshes did not)
+        if (!timed && c->opt.use_graph && si >= 0 && !c->refresh_pending) {
             hipGraphExec_t& ge = c->gexec[gi][si];
             if (!ge || c->gkey[gi][si] != c->vals.p || c->gchunk[gi][si] != k ||
                 c->gcoarse[gi][si] != (cs ? c->coarse_gen : 0)) {
@@ -3445,6 +3449,12 @@ int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
         launch_predict(c, h);
         contact_lag(c);
         int rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, h, st) : minimize<double>(c, h, st);
+        if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
+            // the FP32 tangents did not bring this solve to the tolerances:
+            // it goes on from its iterate with FP64 tangents and solves
+            st->fp64_fallbacks++;
+            rc = minimize<double>(c, h, st);
+        }
         if (rc != SORS_OK) return rc;
         launch_update_state(c, h);
         HIPX(hipGetLastError());

@@ -292,6 +292,23 @@ def test_frozen_hessian_trajectory(orc, element, precision):
     assert np.abs(q - sc.X.reshape(-1)).max() > 1e-6
 
 
+def test_mixed_solve_that_does_not_converge_goes_on_in_fp64():
+    """A mixed-precision Newton solve that reaches max_newton without
+    meeting the tolerances continues from its iterate with FP64 tangents and
+    solves, and ends at the FP64 solution."""
+    sc = mesh.beam(nx=8, ny=2, nz=2, element="tet", e_range=(5e5, 5e6), seed=3)
+    tol = dict(g_tol=0.0, dx_tol=1e-10, pcg_rtol=1e-8)
+    r = rollout(sc, precision=native.MIXED, max_newton=3, **tol)
+    r64 = rollout(sc, precision=native.FP64, **tol)
+    q0 = sc.X.reshape(-1).copy()
+    q, st = r.step(q0, sc.dt)
+    q64, st64 = r64.step(q0, sc.dt)
+    say("fallback", st.fp64_fallbacks, st.newton_iters, st64.newton_iters, np.abs(q - q64).max() / sc.bbox_diag)
+    assert st.fp64_fallbacks == 1 and st64.fp64_fallbacks == 0
+    assert st.newton_iters > 3
+    assert np.abs(q - q64).max() <= 1e-9 * sc.bbox_diag
+
+
 @pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
 def test_kernel_timing_does_not_change_results(precision):
     """profile_kernels launches the first PCG chunk of every solve and the
