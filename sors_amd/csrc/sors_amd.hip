@@ -2541,9 +2541,8 @@ struct sors_ctx {
             hipEvent_t e;
             // timing events of kernels whose results the next kernel reads: a
             // device-scope release (the per-XCD L2s are written back, as after
-            // any launch), no system-scope one.  Without any release (the
-            // disable-system-fence flag) a following kernel on another XCD
-            // could read stale lines.
+            // any launch) and no system-scope one; the disable-system-fence
+            // flag skips the cache writeback altogether
             HIPX(hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
             pev.push_back(e);
         }
