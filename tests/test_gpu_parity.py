@@ -312,9 +312,8 @@ def test_mixed_solve_that_does_not_converge_goes_on_in_fp64():
 @pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
 def test_kernel_timing_does_not_change_results(precision):
     """profile_kernels launches the first PCG chunk of every solve and the
-    assembly kernels with timing events; the states must equal an untimed
-    run bit for bit.  (The events carry a device-scope release: events that
-    drop it let a following kernel on another XCD read stale lines.)"""
+    assembly kernels with timing events (created with a device-scope
+    release); the states must equal an untimed run bit for bit."""
     sc = mesh.beam(nx=30, ny=6, nz=6, e_range=(5e5, 5e6), seed=6)
     runs = []
     for prof in (0, 1):
