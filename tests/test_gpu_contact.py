@@ -120,3 +120,28 @@ def test_contact_argument_errors():
         r.set_disk(0, (0, 0, 0), (0, 0, 1), 0.1)
     r.set_contact(planes=[((0, 0, 0), (0, 0, 1))])
     r.set_contact()  # removes contact
+
+
+def test_poke_rollout_repeats_bit_for_bit():
+    """Two identical contact rollouts (lagged coarse refreshes every 3
+    tangents factorised on the second stream, PCG chunks replayed from
+    graphs) give the same states and iteration counts bit for bit."""
+    p = mesh.contact_poke(n=24, steps=300, seed=1)
+    sc = p.scene
+    cfg = dict(planes=[((0.0, 0.0, 0.0), (0.0, 0.0, 1.0))], disks=[(p.centers[0], (0.0, 0.0, -1.0), p.radius)],
+               surface=p.surface, stiffness=p.stiffness, friction=p.friction, eps_v=p.eps_v)
+    runs = []
+    for _ in range(2):
+        r = native.Rollout.from_scene(sc, precision=native.MIXED, g_tol=1e-8, pcg_rtol=3e-3, coarse_nodes=512,
+                                      coarse_every=3)
+        r.set_contact(**cfg)
+        its = []
+        for k in range(sc.steps):
+            r.set_disk(0, p.centers[k + 1], (0.0, 0.0, -1.0), p.radius)
+            st = r.step_resident(sc.dt)
+            its.append((st.newton_iters, st.pcg_iters))
+        runs.append((r.positions(), its))
+        r.close()
+    print("poke repeat newton", sum(i[0] for i in runs[0][1]), sum(i[0] for i in runs[1][1]), flush=True)
+    assert runs[0][1] == runs[1][1]
+    assert np.array_equal(runs[0][0], runs[1][0])
