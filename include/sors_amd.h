@@ -109,10 +109,11 @@ typedef struct {
     int coarse_every;     /* coarse operator refreshed on every k-th tangent
                              (default 8; <= 1: every tangent, factorised
                              before the solve); with k > 1 a refresh is
-                             factorised on a second stream while the solve
-                             runs with the previous inverse, and the next
-                             tangent installs it.  PCG always runs with an
-                             SPD preconditioner */
+                             factorised on a second stream ahead of the
+                             solve's kernels (the host does not wait), the
+                             solve runs with the previous inverse, and the
+                             next tangent installs it.  PCG always runs
+                             with an SPD preconditioner */
 } sors_options;
 
 typedef struct {

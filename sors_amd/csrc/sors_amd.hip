@@ -2924,8 +2924,8 @@ void coarse_collect(sors_ctx* c) {
 // E = Z^T A Z and its inverse (rocSOLVER Cholesky factorisation and solves).
 // The first refresh, and every refresh when coarse_every <= 1, completes
 // before the solve.  Later refreshes (coarse_every > 1) factorise on the
-// second stream while the solve runs with the previous inverse; the next
-// tangent installs the new one.
+// second stream before the solve, which runs with the previous inverse and
+// leaves the host free; the next tangent installs the new one.
 template <typename HT>
 void coarse_setup(sors_ctx* c) {
     ensure_coarse(c);
@@ -2950,7 +2950,14 @@ void coarse_setup(sors_ctx* c) {
     }
     HIPX(hipGetLastError());
     coarse_factor_async(c);
-    if (c->coarse_ready && lagged) return;  // overlapped with this solve
+    if (c->coarse_ready && lagged) {
+        // the solve keeps the previous inverse, but its kernels start only
+        // after the factorisation: repeated rollouts whose factorisation ran
+        // beside the solve differed in the last bits (500-step contact
+        // rollouts), serialised ones did not
+        HIPX(hipStreamWaitEvent(c->stream, c->evR, 0));
+        return;
+    }
     c->coarse_ready = false;
     coarse_collect(c);  // the solve waits for this one
 }
@@ -3139,11 +3146,7 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
         int si = -1;
         for (int q = 0; q < 3; ++q)
             if (k == sizes[q]) si = q;
-        // while a coarse factorisation is in flight on the second stream the
-        // chunk is launched directly: replayed graphs overlapping it gave
-        // runs that differed in the last bits (500-step contact rollouts,
-        // FP64 and mixed; direct launches and synchronous refreshes did not)
-        if (!timed && c->opt.use_graph && si >= 0 && !c->refresh_pending) {
+        if (!timed && c->opt.use_graph && si >= 0) {
             hipGraphExec_t& ge = c->gexec[gi][si];
             if (!ge || c->gkey[gi][si] != c->vals.p || c->gchunk[gi][si] != k ||
                 c->gcoarse[gi][si] != (cs ? c->coarse_gen : 0)) {

@@ -126,8 +126,8 @@ def test_poke_rollout_repeats_bit_for_bit():
     """Two identical contact rollouts (the bench's 40^3 poke, 500 steps:
     lagged coarse refreshes factorised on the second stream, PCG chunks
     replayed from graphs) give the same states and iteration counts bit for
-    bit.  Replayed graphs that overlapped an in-flight factorisation made
-    such runs part after 150-250 steps."""
+    bit.  While the factorisation ran beside the solve's kernels, such runs
+    parted after 79-250 steps; the solve now waits on it."""
     p = mesh.contact_poke(n=40, steps=500, seed=0)
     sc = p.scene
     cfg = dict(planes=[((0.0, 0.0, 0.0), (0.0, 0.0, 1.0))], disks=[(p.centers[0], (0.0, 0.0, -1.0), p.radius)],
