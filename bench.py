@@ -361,7 +361,7 @@ def main():
     ap.add_argument("--check-every", type=int, default=32, help="PCG iterations between host convergence checks")
     ap.add_argument("--coarse-nodes", type=int, default=2048,
                     help="nodes per aggregate of the two-level PCG preconditioner (0: block-Jacobi only)")
-    ap.add_argument("--coarse-every", type=int, default=8, help="coarse operator refreshed every k-th tangent")
+    ap.add_argument("--coarse-every", type=int, default=16, help="coarse operator refreshed every k-th tangent")
     ap.add_argument("--serve", type=int, default=48, help="serving requests to run (0: skip the serving leg)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")

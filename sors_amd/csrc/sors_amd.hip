@@ -4074,7 +4074,7 @@ void sors_default_options(sors_options* o) {
     o->profile_kernels = 0;
     o->predict = 2;
     o->coarse_nodes = 2048;
-    o->coarse_every = 8;
+    o->coarse_every = 16;
 }
 
 int sors_create(const sors_system_desc* d, sors_ctx** out) {
