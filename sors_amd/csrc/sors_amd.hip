@@ -2162,9 +2162,11 @@ __global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict(CoarseArgs
 }
 
 // xc_I = rows 6I..6I+5 of E^-1 times rc, then u += Z_I xc_I over the nodes of
-// aggregate I: one 1024-thread workgroup per aggregate, so E^-1 is still read
-// once per application and the prolongation needs no second launch.  One
-// partial of gamma_c = rc . xc per workgroup; fixed-order sums throughout.
+// aggregate I: one workgroup per aggregate, so E^-1 is still read once per
+// application and the prolongation needs no second launch.  One partial of
+// gamma_c = rc . xc per workgroup; fixed-order sums throughout.  512 threads
+// (restriction and apply alike): 256 and 1024 were both slower on the
+// 1.92 M-tet beam (256 aggregates).
 #ifndef SORS_MAX_AGGREGATES
 #define SORS_MAX_AGGREGATES 512
 #endif
