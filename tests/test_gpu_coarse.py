@@ -72,8 +72,8 @@ def test_coarse_cantilever_trajectory(orc, precision, every):
     """20x4x4 Kuhn beam, 200 steps: every state within 1e-6 of the
     bounding-box diagonal of the FP64 oracle; fewer PCG iterations in total
     than block-Jacobi alone.  every = 3: the coarse operator is refreshed on
-    every third tangent, factorised on the second stream while the solve
-    runs with the previous inverse."""
+    every third tangent, factorised on the second stream ahead of a solve
+    that runs with the previous inverse."""
     sc = mesh.cantilever_small()
     o = orc.System.from_scene(sc, method=orc.NEWTON_LS)
     r = native.Rollout.from_scene(sc, precision=precision, pcg_rtol=1e-6, coarse_nodes=24, coarse_every=every)
@@ -96,8 +96,8 @@ def test_coarse_cantilever_trajectory(orc, precision, every):
 @pytest.mark.parametrize("precision,every", [(native.FP64, 1), (native.MIXED, 1), (native.MIXED, 2)])
 def test_coarse_graph_replay_matches_direct_launches(precision, every):
     """The coarse kernels inside captured PCG chunks: graph replays and
-    direct launches agree bit for bit (every = 2: with the factorisation
-    overlapped on the second stream as well)."""
+    direct launches agree bit for bit (every = 2: with lagged refreshes
+    factorised on the second stream as well)."""
     sc = mesh.beam(nx=12, ny=3, nz=3, e_range=(5e5, 5e6), seed=2)
     kw = dict(pcg_check_every=16, precision=precision, coarse_nodes=16, coarse_every=every)
     a = native.Rollout.from_scene(sc, use_graph=0, **kw)
