@@ -176,6 +176,20 @@ int sors_cfl_timestep(sors_ctx* ctx, const double* q, double dt, double cfl, dou
 /* ---- inspection entry points used by the parity tests ---------------- */
 /* Energy::compute_energy (cpp/src/energy.cpp:208-248) */
 int sors_energy(sors_ctx* ctx, const double* q, double dt, double* energy);
+/* Allocation-free variant of sors_energy.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * q_dev [3 n_nodes] and energy_dev [1] are device pointers; the result is
+ * written to energy_dev by kernels enqueued on `stream` (a hipStream_t;
+ * NULL: the context's stream), and the call returns without waiting: read
+ * energy_dev after synchronising that stream.  Same kernels and summation
+ * order as sors_energy, so the value is identical.  The reduction partials
+ * live in one static device array per GPU (16,448 bytes, held until the
+ * process ends), not in the context: the call allocates nothing, and every
+ * call on one GPU shares that array, so calls that overlap on one GPU (two
+ * streams, two threads, two contexts) race and must be serialised by the
+ * caller.  The context's own buffers are read, not written. */
+int sors_energy_noalloc(sors_ctx* ctx, const double* q_dev, double dt, double* energy_dev, void* stream);
 /* Energy::compute_gradient (cpp/src/energy.cpp:252-308), FP64 */
 int sors_gradient(sors_ctx* ctx, const double* q, double dt, double* g);
 /* CSR pattern of Energy::compute_hessian for this dt (pattern rule in the

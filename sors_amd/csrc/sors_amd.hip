@@ -2409,6 +2409,27 @@ __global__ void k_update_state(int n, int scheme, double dt, const double* q, do
     }
 }
 
+// ===========================================================================
+// Static scratch of the allocation-free energy entry point
+// (sors_energy_noalloc): the reduction partials of its three energy
+// kernels, their three sums and the inverted-element flag.  Each device
+// holds one copy from library load to process exit.  Host code never
+// passes `scratch` itself to a kernel (on the host the name is a stand-in,
+// not a device address): it uses the address hipGetSymbolAddress returns
+// for the current device.
+// ===========================================================================
+constexpr int kScratchPartials = kMaxRedBlocks;       // one partial per reduction block
+constexpr int kScratchSums = kScratchPartials;        // [3] element, node, contact sums
+constexpr int kScratchFlag = kScratchPartials + 4;    // int flag (bit 1: inverted element)
+constexpr int kScratchDoubles = kScratchPartials + 8; // 16,448 bytes per device
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// E = element + node (+ contact) sums, in the order the host sums them for
+// sors_energy
+__global__ void k_energy_total(const double* s3, int with_aux, double* out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = s3[0] + s3[1] + (with_aux ? s3[2] : 0.0);
+}
+
 // plain copy (calibration of the HBM byte counters)
 __global__ void k_copy(long long n4, const int4* __restrict__ src, int4* __restrict__ dst) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
@@ -2425,6 +2446,31 @@ using namespace sors;
 namespace {
 
 thread_local std::string g_create_error;
+
+// Device address of the static `scratch` array, looked up once per device
+// (with that device current) and kept for the life of the process.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[kMaxDevices] = {};
+
+// address of this device's scratch copy, or null with *err set
+double* scratch_address(int dev, std::string* err) {
+    if (dev < 0 || dev >= kMaxDevices) {
+        *err = "device ordinal " + std::to_string(dev) + " beyond the scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(sors::scratch));
+        if (e != hipSuccess || !p) {
+            *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_addr[dev] = (double*)p;
+    }
+    return g_scratch_addr[dev];
+}
 
 struct DevMem {
     void* p = nullptr;
@@ -2557,6 +2603,9 @@ struct sors_ctx {
         return PcgPart{{b, b + 2 * kMaxRedBlocks}, b + 4 * kMaxRedBlocks};
     }
     bool sqp() const { return method == SORS_MINIMIZE_SQP; }
+    // this device's static scratch (sors_energy_noalloc), set up by sors_create
+    double* scratch_dev = nullptr;
+    std::string scratch_err;
 };
 
 namespace {
@@ -3207,30 +3256,52 @@ void read_scalars(sors_ctx* c, double* out, int first, int count) {
     for (int i = 0; i < count; ++i) out[i] = c->h_pin[i];
 }
 
-// energy kernels into S[S_EELEM], S[S_ENODE]: with dx, the line-search
-// difference E(q + alpha dx) - E(q); without, E(q) (FP64, device)
-void enqueue_energy(sors_ctx* c, const double* q, const double* dx, double alpha, double dt) {
+// Where the energy kernels put their work: the stream, the reduction
+// partials, the three result slots (element, node, contact) and the
+// inverted-element flag.  The context's own buffers by default; the
+// allocation-free entry point passes the device's static scratch.
+struct EnergyTarget {
+    hipStream_t st;
+    double* partials;
+    double* out3;
+    int* flag;
+};
+
+EnergyTarget ctx_energy_target(sors_ctx* c) {
+    return EnergyTarget{c->stream, c->partials.as<double>(), c->S() + S_EELEM, c->F()};
+}
+
+// energy kernels into out3[0] (elements), out3[1] (node terms) and out3[2]
+// (contact): with dx, the line-search difference E(q + alpha dx) - E(q);
+// without, E(q) (FP64, device)
+void enqueue_energy(sors_ctx* c, const double* q, const double* dx, double alpha, double dt, const EnergyTarget& t) {
     const int diff = dx != nullptr;
+    const Reduce red{t.partials};
     ElemArgs a = elem_args(c, q);
     a.dq = dx;
     a.alpha = alpha;
+    a.flag = t.flag;
     const int ge = red_grid(c->E, kBlock);
     if (c->ed == 4)
-        k_elem_energy<4><<<ge, kBlock, 0, c->stream>>>(a, diff, c->S() + S_EELEM, c->red());
+        k_elem_energy<4><<<ge, kBlock, 0, t.st>>>(a, diff, t.out3, red);
     else
-        k_elem_energy<8><<<ge, kBlock, 0, c->stream>>>(a, diff, c->S() + S_EELEM, c->red());
-    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), ge, c->S() + S_EELEM);
+        k_elem_energy<8><<<ge, kBlock, 0, t.st>>>(a, diff, t.out3, red);
+    k_finalize<1><<<1, kBlock, 0, t.st>>>(t.partials, ge, t.out3);
     NodeArgs na = node_args(c, q, dt);
     const int gnn = red_grid(c->n, kBlock);
-    k_node_energy<<<gnn, kBlock, 0, c->stream>>>(na, dx, alpha, diff, c->S() + S_ENODE, c->red());
-    k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gnn, c->S() + S_ENODE);
+    k_node_energy<<<gnn, kBlock, 0, t.st>>>(na, dx, alpha, diff, t.out3 + 1, red);
+    k_finalize<1><<<1, kBlock, 0, t.st>>>(t.partials, gnn, t.out3 + 1);
     if (c->has_contact && c->cargs.nsurf > 0) {
         const int gcn = red_grid(c->cargs.nsurf, kBlock);
-        k_contact_energy<<<gcn, kBlock, 0, c->stream>>>(contact_args(c), q, dx, alpha, diff, c->qprev.as<double>(),
-                                                        dt, c->red());
-        k_finalize<1><<<1, kBlock, 0, c->stream>>>(c->partials.as<double>(), gcn, c->S() + S_AUX);
+        k_contact_energy<<<gcn, kBlock, 0, t.st>>>(contact_args(c), q, dx, alpha, diff, c->qprev.as<double>(), dt,
+                                                   red);
+        k_finalize<1><<<1, kBlock, 0, t.st>>>(t.partials, gcn, t.out3 + 2);
     }
     HIPX(hipGetLastError());
+}
+
+void enqueue_energy(sors_ctx* c, const double* q, const double* dx, double alpha, double dt) {
+    enqueue_energy(c, q, dx, alpha, dt, ctx_energy_target(c));
 }
 
 // the energy scalars summed on the host: element + node (+ contact)
@@ -4124,6 +4195,14 @@ int sors_create(const sors_system_desc* d, sors_ctx** out) {
         c->damping = d->damping_alpha;
         sors_default_options(&c->opt);
         HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        {
+            // the allocation-free entry point's scratch: its address on the
+            // current device, looked up here (once per device) with that
+            // device current; a failure is reported by that entry point
+            int dev = -1;
+            HIPX(hipGetDevice(&dev));
+            c->scratch_dev = scratch_address(dev, &c->scratch_err);
+        }
         const int N = c->N, E = c->E, ed = c->ed, n = c->n;
         c->h_conn.assign(d->conn, d->conn + (size_t)E * ed);
         c->h_rest.assign(d->rest, d->rest + (size_t)n);
@@ -4341,6 +4420,30 @@ int sors_energy(sors_ctx* c, const double* q, double dt, double* out) {
     upload(c->qtmp, q, (size_t)c->n, c->stream);
     clear_flag(c);
     *out = energy_abs(c, c->qtmp.as<double>(), dt);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_energy_noalloc(sors_ctx* c, const double* q_dev, double dt, double* energy_dev, void* stream) {
+    SORS_GUARD(c)
+    if (!q_dev || !energy_dev) return fail(c, SORS_ERR_ARG, "null argument");
+    if (!c->scratch_dev) return fail(c, SORS_ERR_HIP, "static scratch unavailable: " + c->scratch_err);
+    // the three reductions run with red_grid() blocks each, capped at
+    // kMaxRedBlocks; the scratch holds kScratchPartials partials
+    const long long work = std::max<long long>(std::max<long long>(c->E, c->n), c->has_contact ? c->cargs.nsurf : 0);
+    if (red_grid(work, kBlock) > kScratchPartials)
+        return fail(c, SORS_ERR_ARG, "energy reduction needs " + std::to_string(red_grid(work, kBlock)) +
+                                         " partial sums; the static scratch holds " +
+                                         std::to_string(kScratchPartials));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    double* sc = c->scratch_dev;
+    int* flag = (int*)(sc + kScratchFlag);
+    // the flag is only ever or-ed into: start it at zero (a fill kernel, not
+    // a memset); partials and sums are overwritten by every call
+    k_fill<int><<<1, 64, 0, st>>>(flag, 1, 0);
+    enqueue_energy(c, q_dev, nullptr, 0.0, dt, EnergyTarget{st, sc, sc + kScratchSums, flag});
+    k_energy_total<<<1, 64, 0, st>>>(sc + kScratchSums, (c->has_contact && c->cargs.nsurf > 0) ? 1 : 0, energy_dev);
+    HIPX(hipGetLastError());
     return SORS_OK;
     SORS_GUARD_END(c)
 }

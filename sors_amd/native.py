@@ -142,6 +142,7 @@ def lib():
         L.sors_cfl_timestep.argtypes = [vp, _dp, C.c_double, C.c_double, _dp]
         L.sors_energy.argtypes = [vp, _dp, C.c_double, _dp]
         L.sors_gradient.argtypes = [vp, _dp, C.c_double, _dp]
+        L.sors_energy_noalloc.argtypes = [vp, vp, C.c_double, vp, vp]
         L.sors_pattern_nnz.argtypes = [vp, C.c_double]
         L.sors_pattern_nnz.restype = C.c_longlong
         L.sors_get_pattern.argtypes = [vp, C.c_double, _ip, _ip]
