@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 BE, CN = 0, 1
 NEWTON, NEWTON_LS, SQP, FROZEN = 0, 1, 2, 3
 LDLT, PCG = 0, 1
+MAT_NH, MAT_SNH = 0, 1
 TEST_POLY = -4
 ERRORS = {
     -1: "Inverted Elements",
@@ -117,6 +118,18 @@ def lib():
         L.orc_set_activation.restype = None
         L.orc_set_youngs.argtypes = [C.c_void_p, _dp]
         L.orc_set_youngs.restype = None
+        L.orc_snh_psi.restype = C.c_double
+        L.orc_snh_psi.argtypes = [_dp, C.c_double, C.c_double]
+        L.orc_snh_dpsi.argtypes = [_dp, C.c_double, C.c_double, _dp]
+        L.orc_snh_dpsi.restype = None
+        L.orc_snh_d2psi.argtypes = [_dp, C.c_double, C.c_double, _dp]
+        L.orc_snh_d2psi.restype = None
+        L.orc_elem_eval_ex.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp, C.c_double,
+                                       C.c_double, C.c_int, _dp, _dp, _dp]
+        L.orc_set_material.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_muscle.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp]
+        L.orc_set_muscle_activation.argtypes = [C.c_void_p, _dp]
+        L.orc_set_muscle_activation.restype = None
         _lib = L
     return _lib
 
@@ -196,6 +209,39 @@ def elem_eval(Xe, xe, E, nu, project=False, ed=None):
     nd = 3 * (4 if ed == TEST_POLY else ed)
     e, g, h = np.zeros(1), np.zeros(nd), np.zeros(nd * nd)
     st = lib().orc_elem_eval(ed, _d(Xe), _d(xe), E, nu, int(project), _d(e), _d(g), _d(h))
+    if st:
+        raise RuntimeError(ERRORS.get(st, st))
+    return float(e[0]), g, h.reshape(nd, nd)
+
+
+def snh_psi(F, mu, lam):
+    return lib().orc_snh_psi(_d(f64(F).reshape(9)), mu, lam)
+
+
+def snh_dpsi(F, mu, lam):
+    P = np.zeros(9)
+    lib().orc_snh_dpsi(_d(f64(F).reshape(9)), mu, lam, _d(P))
+    return P.reshape(3, 3)
+
+
+def snh_d2psi(F, mu, lam):
+    K = np.zeros(81)
+    lib().orc_snh_d2psi(_d(f64(F).reshape(9)), mu, lam, _d(K))
+    return K.reshape(9, 9)
+
+
+def elem_eval_ex(Xe, xe, E, nu, material=MAT_NH, muscle=None, project=False):
+    """Element energy / gradient / Hessian with an elastic material and, with
+    muscle = (direction[3], stiffness, activation), the SoftconMuscle energy."""
+    Xe = f64(Xe).reshape(-1)
+    xe = f64(xe).reshape(-1)
+    nd = xe.size
+    ed = nd // 3
+    m, k, a = (None, 0.0, 0.0) if muscle is None else (f64(muscle[0]).reshape(3), float(muscle[1]),
+                                                        float(muscle[2]))
+    e, g, h = np.zeros(1), np.zeros(nd), np.zeros(nd * nd)
+    st = lib().orc_elem_eval_ex(ed, _d(Xe), _d(xe), E, nu, int(material), _d(m), k, a, int(project), _d(e), _d(g),
+                                _d(h))
     if st:
         raise RuntimeError(ERRORS.get(st, st))
     return float(e[0]), g, h.reshape(nd, nd)
@@ -389,6 +435,24 @@ class System:
 
     def set_youngs(self, E):
         lib().orc_set_youngs(self.h, _d(f64(E)))
+
+    def set_material(self, material):
+        if lib().orc_set_material(self.h, int(material)) != 0:
+            raise ValueError("orc_set_material")
+
+    def set_muscle(self, group, stiffness, direction):
+        """SoftconMuscle on elements with group[e] >= 0."""
+        g = i32(group)
+        self._muscle_keep = (g, f64(stiffness), f64(direction).reshape(-1))
+        self.mgroups = int(g.max()) + 1
+        if lib().orc_set_muscle(self.h, self.mgroups, _i(g), _d(self._muscle_keep[1]),
+                                _d(self._muscle_keep[2])) != 0:
+            raise ValueError("orc_set_muscle")
+
+    def set_muscle_activation(self, act):
+        a = f64(act)
+        assert a.size == self.mgroups
+        lib().orc_set_muscle_activation(self.h, _d(a))
 
     def update_state(self, q, dt):
         q = f64(q).reshape(-1)

@@ -213,6 +213,57 @@ int orc_nh_d2psi(const double F[9], double mu, double lam, double K[81]) {
     return ORC_OK;
 }
 
+/* StableNeoHookeanEnergy constructor (stableNeoHookean.cpp:9-14) from the
+ * Lame constants of (E, nu): mu_ = 4/3 mu, lambda_ = lam + 5/8 mu_,
+ * alpha_ = 1 + 3/4 mu_ / lambda_. */
+void orc_snh_params(double mu, double lam, double* mus, double* lams, double* alpha) {
+    *mus = (4.0 / 3.0) * mu;
+    *lams = lam + (5.0 / 8.0) * (*mus);
+    *alpha = 1 + (3.0 / 4.0) * ((*mus) / (*lams));
+}
+
+/* StableNeoHookeanEnergy::compute_energy per quadrature point
+ * (stableNeoHookean.cpp:31-40) */
+double orc_snh_psi(const double F[9], double mu, double lam) {
+    double m, l, al;
+    orc_snh_params(mu, lam, &m, &l, &al);
+    double J = det3(F);
+    double IC = 0;
+    for (int i = 0; i < 9; ++i) IC += F[i] * F[i];
+    return (m / 2.0) * (IC - 3.0) + (l / 2.0) * (J - al) * (J - al) - (m / 2.0) * log(IC + 1.0);
+}
+
+/* compute_gradient per quadrature point (stableNeoHookean.cpp:64-77) */
+void orc_snh_dpsi(const double F[9], double mu, double lam, double P[9]) {
+    double m, l, al, Fi[9], FiT[9];
+    orc_snh_params(mu, lam, &m, &l, &al);
+    double J = inv3(F, Fi);
+    transpose3(Fi, FiT);
+    double IC = 0;
+    for (int i = 0; i < 9; ++i) IC += F[i] * F[i];
+    for (int i = 0; i < 9; ++i) P[i] = m * F[i] + l * (J - al) * J * FiT[i] - (m * F[i]) / (IC + 1.0);
+}
+
+/* compute_hessian per quadrature point (stableNeoHookean.cpp:100-131):
+ * A = mu_ I + (2J - alpha) lambda_ J f f^T + lambda_ (J - alpha) J FinvTGrad
+ *     - mu_ I / (IC + 1) + 2 mu_ F F^T / (IC + 1)^2 */
+void orc_snh_d2psi(const double F[9], double mu, double lam, double K[81]) {
+    double m, l, al, Fi[9], FiT[9];
+    orc_snh_params(mu, lam, &m, &l, &al);
+    double J = inv3(F, Fi);
+    transpose3(Fi, FiT);
+    double IC = 0;
+    for (int i = 0; i < 9; ++i) IC += F[i] * F[i];
+    for (int r = 0; r < 9; ++r)
+        for (int col = 0; col < 9; ++col) {
+            int a = r / 3, b = r % 3, p = col / 3, q = col % 3;
+            double gradFinvT = -FiT[3 * a + q] * FiT[3 * p + b];
+            double id = (r == col) ? 1.0 : 0.0;
+            K[9 * r + col] = m * id + (2 * J - al) * l * J * FiT[r] * FiT[col] + l * (J - al) * J * gradFinvT +
+                             ((-m * id) / (IC + 1.0)) + ((2 * m * F[r] * F[col]) / ((IC + 1.0) * (IC + 1.0)));
+        }
+}
+
 /* Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix, negative
  * eigenvalues clamped to zero, matrix rebuilt in place (textbook method,
  * Golub & Van Loan 8.5).  Used to SPD-project the 9x9 d2Psi/dF2. */
@@ -288,9 +339,70 @@ static void poly_eval(const double* xe, double* energy, double* grad, double* he
 /* sa, f: active fibre stress sigma_max * a and unit fibre f (f == NULL or
  * sa == 0: passive).  Psi_act = 1/2 sa (|F f|^2 - 1), P_act = sa (F f) f^T,
  * dP_act/dF = sa I (x) f f^T (PSD, added after any projection). */
+/* SoftconMuscleEnergy at one quadrature point (softconMuscle.cpp:12-134):
+ * F, B = nablaF (9 x nd), direction m, stiffness k, activation e.  Adds
+ * w * (energy, gradient, Hessian).  project != 0: the Hessian is
+ * B^T K B with K = k [(r/l^2) vec(u m^T) vec(u m^T)^T + (1-r) vec(1 m^T)
+ * vec(1 m^T)^T] (the reference's Hessian in 9x9 form) clamped to PSD. */
+static void muscle_point(int nd, const double* F, const double* B, const double* m, double k, double e, double w,
+                         int project, double* energy, double* grad, double* hess) {
+    double Fm[3];
+    for (int a = 0; a < 3; ++a) Fm[a] = F[3 * a] * m[0] + F[3 * a + 1] * m[1] + F[3 * a + 2] * m[2];
+    double l = sqrt(Fm[0] * Fm[0] + Fm[1] * Fm[1] + Fm[2] * Fm[2]);
+    double r = (1 - e) / l;
+    if (energy) {
+        /* k_ / 2 * ((1 - r) * F * m_).norm()  (:30-31) */
+        double v0 = (1 - r) * Fm[0], v1 = (1 - r) * Fm[1], v2 = (1 - r) * Fm[2];
+        *energy += (k / 2 * sqrt(v0 * v0 + v1 * v1 + v2 * v2)) * w;
+    }
+    if (!grad && !hess) return;
+    double t1[24], t2[24];
+    for (int i = 0; i < nd; ++i) {
+        /* m_c F_ab m_b nablaF_(ac),i  (:64-72) and nablaF_(ac),i m_c (:108-115) */
+        double a1 = 0, a2 = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) {
+                a1 += m[c] * Fm[a] * B[(3 * a + c) * nd + i];
+                a2 += B[(3 * a + c) * nd + i] * m[c];
+            }
+        t1[i] = a1;
+        t2[i] = a2;
+    }
+    if (grad)
+        for (int i = 0; i < nd; ++i) grad[i] += t1[i] * k * (1 - r) * w;
+    if (!hess) return;
+    if (!project) {
+        for (int i = 0; i < nd; ++i)
+            for (int j = 0; j < nd; ++j)
+                hess[i * nd + j] += k * ((r / (l * l)) * t1[i] * t1[j] + (1 - r) * t2[i] * t2[j]) * w;
+        return;
+    }
+    double K[81], v1[9], v2[9];
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) {
+            v1[3 * a + c] = Fm[a] * m[c];
+            v2[3 * a + c] = m[c];
+        }
+    for (int p = 0; p < 9; ++p)
+        for (int q = 0; q < 9; ++q) K[9 * p + q] = k * ((r / (l * l)) * v1[p] * v1[q] + (1 - r) * v2[p] * v2[q]);
+    orc_project_psd(9, K);
+    for (int i = 0; i < nd; ++i)
+        for (int j = 0; j < nd; ++j) {
+            double acc = 0;
+            for (int p = 0; p < 9; ++p)
+                for (int q = 0; q < 9; ++q) acc += B[p * nd + i] * K[9 * p + q] * B[q * nd + j];
+            hess[i * nd + j] += acc * w;
+        }
+}
+
+/* material: ORC_MAT_NH (neoHookean.cpp) or ORC_MAT_SNH (stableNeoHookean.cpp);
+ * mdir != NULL adds the SoftconMuscle energy with stiffness mk and
+ * activation mact.  Each energy's Hessian is projected on its own. */
 static int elem_eval_pre(int ed, const double* pre, double V0, const double* xe, double mu, double lam,
-                         int project, double* energy, double* grad, double* hess, double sa, const double* f) {
+                         int project, double* energy, double* grad, double* hess, double sa, const double* f,
+                         int material, const double* mdir, double mk, double mact) {
     const int active = f != NULL && sa != 0.0;
+    const int snh = material == ORC_MAT_SNH;
     int nq = (ed == 4) ? 1 : 8;
     int nd = 3 * ed;
     double w = 1.0 / nq;
@@ -333,12 +445,16 @@ static int elem_eval_pre(int ed, const double* pre, double V0, const double* xe,
         if (active)
             for (int r = 0; r < 3; ++r) Ff[r] = F[3 * r] * f[0] + F[3 * r + 1] * f[1] + F[3 * r + 2] * f[2];
         if (energy) {
-            e += orc_nh_psi(F, mu, lam) * w;
+            e += (snh ? orc_snh_psi(F, mu, lam) : orc_nh_psi(F, mu, lam)) * w;
             if (active) e += 0.5 * sa * (Ff[0] * Ff[0] + Ff[1] * Ff[1] + Ff[2] * Ff[2] - 1.0) * w;
         }
+        if (mdir) muscle_point(nd, F, Bbuf, mdir, mk, mact, w, project, energy ? &e : NULL, grad, hess);
         if (grad) {
             double P[9];
-            orc_nh_dpsi(F, mu, lam, P);
+            if (snh)
+                orc_snh_dpsi(F, mu, lam, P);
+            else
+                orc_nh_dpsi(F, mu, lam, P);
             if (active)
                 for (int r = 0; r < 3; ++r)
                     for (int c = 0; c < 3; ++c) P[3 * r + c] += sa * Ff[r] * f[c];
@@ -350,8 +466,12 @@ static int elem_eval_pre(int ed, const double* pre, double V0, const double* xe,
         }
         if (hess) {
             double K[81], KB[9 * 24];
-            int st = orc_nh_d2psi(F, mu, lam, K);
-            if (st != ORC_OK) status = st;
+            if (snh) {
+                orc_snh_d2psi(F, mu, lam, K);
+            } else {
+                int st = orc_nh_d2psi(F, mu, lam, K);
+                if (st != ORC_OK) status = st;
+            }
             if (project) orc_project_psd(9, K);
             if (active)
                 for (int i = 0; i < 3; ++i)
@@ -391,13 +511,37 @@ int orc_elem_eval(int ed, const double* Xe, const double* xe, double E, double n
         double Dminv[9];
         int st = orc_tet_precompute(Xe, Dminv, &V0, NULL);
         if (st) return st;
-        return elem_eval_pre(4, Dminv, V0, xe, mu, lam, project, energy, grad, hess, 0.0, NULL);
+        return elem_eval_pre(4, Dminv, V0, xe, mu, lam, project, energy, grad, hess, 0.0, NULL, ORC_MAT_NH, NULL,
+                             0.0, 0.0);
     }
     if (ed == 8) {
         double G[192];
         int st = orc_hex_precompute(Xe, G, NULL, &V0);
         if (st) return st;
-        return elem_eval_pre(8, G, V0, xe, mu, lam, project, energy, grad, hess, 0.0, NULL);
+        return elem_eval_pre(8, G, V0, xe, mu, lam, project, energy, grad, hess, 0.0, NULL, ORC_MAT_NH, NULL, 0.0,
+                             0.0);
+    }
+    return ORC_ERR_ARG;
+}
+
+int orc_elem_eval_ex(int ed, const double* Xe, const double* xe, double E, double nu, int material,
+                     const double* mdir, double mk, double mact, int project, double* energy, double* grad,
+                     double* hess) {
+    double mu, lam, V0;
+    orc_lame(E, nu, &mu, &lam);
+    if (ed == 4) {
+        double Dminv[9];
+        int st = orc_tet_precompute(Xe, Dminv, &V0, NULL);
+        if (st) return st;
+        return elem_eval_pre(4, Dminv, V0, xe, mu, lam, project, energy, grad, hess, 0.0, NULL, material, mdir, mk,
+                             mact);
+    }
+    if (ed == 8) {
+        double G[192];
+        int st = orc_hex_precompute(Xe, G, NULL, &V0);
+        if (st) return st;
+        return elem_eval_pre(8, G, V0, xe, mu, lam, project, energy, grad, hess, 0.0, NULL, material, mdir, mk,
+                             mact);
     }
     return ORC_ERR_ARG;
 }
@@ -440,6 +584,13 @@ struct orc_sys {
     double* act;    /* [ngroups] */
     double sigma;
     double* nu;     /* Poisson ratio per element (orc_set_youngs) */
+    int material;   /* ORC_MAT_NH | ORC_MAT_SNH */
+    /* SoftconMuscle (orc_set_muscle) */
+    int mngroups;
+    int* mgroup;    /* [nel] group or -1 */
+    double* mk;     /* [nel] softconStiffness */
+    double* mdir;   /* [nel][3] softconDirection */
+    double* mact;   /* [mngroups] activation */
     double* lagN;   /* [nsurf] lagged normal force magnitude */
     double* lagn;   /* [nsurf][3] its normal */
     double* lagD;   /* [nsurf][3] displacement of its primitive over the step */
@@ -538,6 +689,10 @@ void orc_free(orc_sys* s) {
     free(s->agroup);
     free(s->afib);
     free(s->act);
+    free(s->mgroup);
+    free(s->mk);
+    free(s->mdir);
+    free(s->mact);
     free(s->surf);
     free(s->lagN);
     free(s->lagn);
@@ -587,8 +742,15 @@ static int elem(const orc_sys* s, int e, const double* xe, int project, double* 
         sa = s->sigma * s->act[s->agroup[e]];
         f = s->afib + 3 * (size_t)e;
     }
+    const double* md = NULL;
+    double mk = 0.0, ma = 0.0;
+    if (s->mgroup && s->mgroup[e] >= 0) {
+        md = s->mdir + 3 * (size_t)e;
+        mk = s->mk[e];
+        ma = s->mact[s->mgroup[e]];
+    }
     return elem_eval_pre(s->ed, s->pre + (size_t)e * psz, s->V0[e], xe, s->mu[e], s->lam[e], project, energy, grad,
-                         hess, sa, f);
+                         hess, sa, f, s->material, md, mk, ma);
 }
 
 /* TimeIntegrator::integrate_energy (integrator.cpp:3-43) */
@@ -1517,4 +1679,38 @@ void orc_set_activation(orc_sys* s, const double* act) {
 /* neoHookean.cpp:10-11 with the stored Poisson ratios */
 void orc_set_youngs(orc_sys* s, const double* youngs) {
     for (int e = 0; e < s->nel; ++e) orc_lame(youngs[e], s->nu[e], &s->mu[e], &s->lam[e]);
+}
+
+/* ---- element energy plugins ------------------------------------------------ */
+int orc_set_material(orc_sys* s, int material) {
+    if (s->poly || (material != ORC_MAT_NH && material != ORC_MAT_SNH)) return ORC_ERR_ARG;
+    s->material = material;
+    return ORC_OK;
+}
+
+/* SoftconMuscleEnergy on the elements with group >= 0 (tetrahedron.cpp:82-88,
+ * hexahedron.cpp:51-57: "muscleGroup", "softconStiffness", "softconDirection");
+ * activations per group as the "softconMuscle" actuation (element.h:52) */
+int orc_set_muscle(orc_sys* s, int ngroups, const int* group, const double* stiffness, const double* dir) {
+    if (s->poly || ngroups < 1) return ORC_ERR_ARG;
+    free(s->mgroup);
+    free(s->mk);
+    free(s->mdir);
+    free(s->mact);
+    size_t ne = (size_t)(s->nel > 0 ? s->nel : 1);
+    s->mgroup = (int*)malloc(sizeof(int) * ne);
+    s->mk = (double*)malloc(sizeof(double) * ne);
+    s->mdir = (double*)malloc(sizeof(double) * 3 * ne);
+    s->mact = (double*)calloc((size_t)ngroups, sizeof(double));
+    for (int e = 0; e < s->nel; ++e) {
+        s->mgroup[e] = group[e];
+        s->mk[e] = stiffness[e];
+        for (int k = 0; k < 3; ++k) s->mdir[3 * (size_t)e + k] = dir[3 * (size_t)e + k];
+    }
+    s->mngroups = ngroups;
+    return ORC_OK;
+}
+
+void orc_set_muscle_activation(orc_sys* s, const double* act) {
+    if (s->mact) memcpy(s->mact, act, sizeof(double) * (size_t)s->mngroups);
 }

@@ -9,6 +9,7 @@
  * repository root, srl-ethz/sors):
  *   cpp/src/geometries/tetrahedron.cpp, cpp/src/geometries/hexahedron.cpp,
  *   cpp/headers/geometries/element.h, cpp/src/energies/neoHookean.cpp,
+ *   cpp/src/energies/stableNeoHookean.cpp, cpp/src/energies/softconMuscle.cpp,
  *   cpp/src/energy.cpp, cpp/src/integrator.cpp, cpp/headers/integrator.h,
  *   cpp/src/solver.cpp, cpp/headers/common.h.
  * Citations are given per function in sors_oracle.c.
@@ -40,6 +41,9 @@ enum { ORC_BE = 0, ORC_CN = 1 };
  * minimize_frozenhessian (the minimize_newton assembly semantics). */
 enum { ORC_NEWTON = 0, ORC_NEWTON_LS = 1, ORC_SQP = 2, ORC_FROZEN = 3 };
 enum { ORC_LIN_LDLT = 0, ORC_LIN_PCG = 1 };
+/* elastic ElementEnergy: "neohookean" (neoHookean.cpp) or "stableneohookean"
+ * (stableNeoHookean.cpp) */
+enum { ORC_MAT_NH = 0, ORC_MAT_SNH = 1 };
 enum {
     ORC_OK = 0,
     ORC_ERR_INVERTED = -1,   /* "Inverted Elements" (neoHookean.cpp:97-101) */
@@ -63,6 +67,18 @@ void   orc_project_psd(int n, double* K);
  * d2Psi/dF2 to PSD before contraction. */
 int    orc_elem_eval(int ed, const double* Xe, const double* xe, double E, double nu,
                      int project, double* energy, double* grad, double* hess);
+/* StableNeoHookeanEnergy (stableNeoHookean.cpp:3-150); mu, lam: Lame
+ * constants of (E, nu), reparameterised as the reference does */
+void   orc_snh_params(double mu, double lam, double* mus, double* lams, double* alpha);
+double orc_snh_psi(const double F[9], double mu, double lam);
+void   orc_snh_dpsi(const double F[9], double mu, double lam, double P[9]);
+void   orc_snh_d2psi(const double F[9], double mu, double lam, double K[81]);
+/* orc_elem_eval with an elastic material (ORC_MAT_*) and, when mdir != NULL,
+ * the SoftconMuscleEnergy (softconMuscle.cpp:12-134) with direction mdir,
+ * stiffness mk and activation mact */
+int    orc_elem_eval_ex(int ed, const double* Xe, const double* xe, double E, double nu, int material,
+                        const double* mdir, double mk, double mact, int project, double* energy, double* grad,
+                        double* hess);
 
 /* ---- system level ----------------------------------------------------- */
 typedef struct orc_sys orc_sys;
@@ -145,6 +161,15 @@ void orc_contact_lag(orc_sys* s, double dt);
 int  orc_set_active(orc_sys* s, int ngroups, const int* group, const double* fibre, double sigma);
 void orc_set_activation(orc_sys* s, const double* act);
 void orc_set_youngs(orc_sys* s, const double* youngs);
+
+/* ---- element energy plugins ------------------------------------------------
+ * orc_set_material: the elastic energy of every element (ORC_MAT_*).
+ * orc_set_muscle: SoftconMuscleEnergy on the elements with group[e] >= 0,
+ * stiffness[e] ("softconStiffness"), dir[3e..3e+2] ("softconDirection"),
+ * activations per group (the "softconMuscle" actuation, element.h:52). */
+int  orc_set_material(orc_sys* s, int material);
+int  orc_set_muscle(orc_sys* s, int ngroups, const int* group, const double* stiffness, const double* dir);
+void orc_set_muscle_activation(orc_sys* s, const double* act);
 
 #ifdef __cplusplus
 }
