@@ -48,9 +48,14 @@ enum {
     SORS_MINIMIZE_FROZENHESSIAN = 3
 };
 /* ElementEnergy plugins (cpp/headers/energies/elementEnergy.h): compressible
- * Neo-Hookean, and the polynomial test energy of the reference's unit tests
- * (cpp/src/_test.cpp:8-66). */
-enum { SORS_NEOHOOKEAN = 0, SORS_TEST_POLY = 1 };
+ * Neo-Hookean ("neohookean", cpp/src/energies/neoHookean.cpp), the polynomial
+ * test energy of the reference's unit tests (cpp/src/_test.cpp:8-66), and
+ * Stable Neo-Hookean ("stableneohookean", cpp/src/energies/stableNeoHookean.cpp,
+ * with its parameters mu' = 4/3 mu, lambda' = lambda + 5/8 mu',
+ * alpha = 1 + 3/4 mu'/lambda' from each element's youngsModulus and
+ * poissonsRatio).  The SoftCon muscle energy is added per element with
+ * sors_set_muscle. */
+enum { SORS_NEOHOOKEAN = 0, SORS_TEST_POLY = 1, SORS_STABLE_NEOHOOKEAN = 2 };
 enum { SORS_FP64 = 0, SORS_MIXED = 1 };
 
 /* Arguments of Energy<3,E>::Energy (cpp/headers/energy.h:73-84) as arrays:
@@ -67,7 +72,7 @@ typedef struct {
     const double* youngs;   /* [n_elems] */
     const double* poisson;  /* [n_elems] */
     const double* density;  /* [n_elems] */
-    int material;           /* SORS_NEOHOOKEAN | SORS_TEST_POLY */
+    int material;           /* SORS_NEOHOOKEAN | SORS_TEST_POLY | SORS_STABLE_NEOHOOKEAN */
     const int* bc_mask;     /* [3 n_nodes] or NULL */
     const double* bc_value; /* [3 n_nodes] prescribed velocity or NULL */
     double grav[3];
@@ -281,6 +286,27 @@ typedef struct {
 int sors_set_active(sors_ctx* ctx, const sors_active_desc* desc);   /* NULL removes it */
 /* activations [n_groups] used by the following steps (Solver::step's actuation) */
 int sors_set_activation(sors_ctx* ctx, const double* act);
+
+/* SoftconMuscleEnergy (cpp/src/energies/softconMuscle.cpp:12-134) on the
+ * elements with elem_group[e] >= 0, with their "muscleGroup",
+ * "softconStiffness" and "softconDirection" Params
+ * (cpp/src/geometries/tetrahedron.cpp:82-88, hexahedron.cpp:51-57); tet4 and
+ * hex8.  Per Gauss point, u = F m, l = |u|, L = 1 - activation, r = L / l:
+ * energy k/2 |l - L| (as the reference computes it, :27-31), gradient
+ * k (1 - r) u m^T (:55-72, the gradient of k/2 (l - L)^2), Hessian
+ * k [(r/l^2) u u^T + (1 - r) 1 1^T] (x) m m^T (:98-130); with project_spd
+ * its 3x3 factor is clamped to PSD.  NULL removes it. */
+typedef struct {
+    int n_groups;
+    const int* elem_group;     /* [n_elems] group in [0, n_groups) or -1 */
+    const double* direction;   /* [n_elems][3] m, as given (not normalised) */
+    const double* stiffness;   /* [n_elems] k */
+} sors_muscle_desc;
+
+int sors_set_muscle(sors_ctx* ctx, const sors_muscle_desc* desc);
+/* the "softconMuscle" actuation of Solver::step (element.h:52): activations
+ * [n_groups] used by the following steps (0 until set) */
+int sors_set_muscle_activation(sors_ctx* ctx, const double* act);
 /* per-element Young's modulus (Lame constants recomputed with each element's
  * Poisson ratio, neoHookean.cpp:10-11) */
 int sors_set_youngs(sors_ctx* ctx, const double* youngs);

@@ -63,7 +63,7 @@ struct ElemArgs {
     const double* lam;
     const int* mask;      // 3N or null
     int sqp;
-    int material;
+    int material;         // SORS_NEOHOOKEAN | SORS_TEST_POLY | SORS_STABLE_NEOHOOKEAN (mu, lam: mu', lam')
     double hscale;        // element tangent multiplier (BE 1, CN 1/2)
     double* gint;         // fp64 atomics target or null
     void* vals;           // HT atomics target or null
@@ -80,6 +80,11 @@ struct ElemArgs {
     const double* afib;   // [3][E] unit fibre direction in the rest state (SoA)
     const double* act;    // [n_groups] activation
     double sigma;         // sigma_max (Pa)
+    // SoftCon muscle (softconMuscle.cpp:12-134)
+    const int* mgroup;    // [E] muscle group or -1; null: no muscle
+    const double* mdir;   // [3][E] direction m (SoA, as given)
+    const double* mk;     // [E] stiffness k
+    const double* mact;   // [n_muscle_groups] activation
 };
 
 struct NodeArgs {

@@ -227,6 +227,69 @@ __device__ __forceinline__ double active_level(const ElemArgs& a, int e, double 
     return a.sigma * a.act[gi];
 }
 
+// ---- element energy plugins: Stable Neo-Hookean and SoftCon muscle -------
+// SoftCon muscle of element e: returns its stiffness k (0: none), its
+// direction in m and the target length L = 1 - activation
+// (softconMuscle.cpp:27-30)
+__device__ __forceinline__ double muscle_of(const ElemArgs& a, int e, double m[3], double* L) {
+    if (!a.mgroup) return 0.0;
+    const int gi = a.mgroup[e];
+    if (gi < 0) return 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m[c] = a.mdir[(size_t)c * a.E + e];
+    *L = 1.0 - a.mact[gi];
+    return a.mk[e];
+}
+
+__device__ __forceinline__ bool snh_material(const ElemArgs& a) { return a.material == SORS_STABLE_NEOHOOKEAN; }
+
+// elastic first Piola stress (FP64) of the context's material, plus the muscle
+__device__ __forceinline__ M3<double> elem_stress(const ElemArgs& a, const NHPoint<double>& p, const M3<double>& D,
+                                                  double mu, double lam, double mk, const double m[3], double mL) {
+    M3<double> P = snh_material(a) ? snh_stress(p, D, mu, lam) : nh_stress(p, mu, lam);
+    if (mk != 0.0) muscle_add_stress(D, m, mk, mL, P);
+    return P;
+}
+
+// energy density (FP64) of the material plus the muscle; Stable Neo-Hookean's
+// rest value Psi(I) only when absolute (energy differences leave it out, so
+// that it does not cancel in rounding)
+__device__ __forceinline__ double elem_psi(const ElemArgs& a, const M3<double>& D, double mu, double lam, bool* inv,
+                                           bool absolute, double mk, const double m[3], double mL) {
+    double en;
+    if (snh_material(a)) {
+        en = snh_psi_shift(D, mu, lam);
+        if (absolute) en += snh_psi_rest(mu, lam);
+    } else {
+        en = nh_psi_disp(D, mu, lam, inv);
+    }
+    if (mk != 0.0) en += muscle_psi(D, m, mk, mL);
+    return en;
+}
+
+// 9x9 tangent (HT) of the material, projected (PROJ) or raw, plus the muscle's
+// (projected on its own when PROJ), as a packed Sym9
+template <typename HT, int PROJ>
+__device__ __forceinline__ void elem_tangent(const ElemArgs& a, const M3<HT>& D, HT mu, HT lam, double mk,
+                                             const double m[3], double mL, Sym9<HT>& S) {
+    const NHPoint<HT> p = nh_point(D);
+    if (snh_material(a)) {
+        if (PROJ)
+            snh_projected_tangent(D, p, mu, lam, S);
+        else
+            snh_raw_tangent(p, D, mu, lam, S);
+    } else {
+        if (PROJ)
+            nh_projected_tangent(D, p, mu, lam, S);
+        else
+            raw_tangent_sym9(mu, HT(0), lam, lam * p.lnJ - mu, p.F, p.FiT, S);
+    }
+    if (mk != 0.0) {
+        const HT mh[3] = {(HT)m[0], (HT)m[1], (HT)m[2]};
+        muscle_add_tangent(D, mh, (HT)mk, (HT)mL, PROJ != 0, S);
+    }
+}
+
 // ---- tet4: one element per lane ------------------------------------------
 // MODE 0: assemble gradient + Hessian; MODE 1: gradient only; MODE 2: export
 // per-element energy / gradient / Hessian (FP64 gradient & energy; Hessian in HT)
@@ -305,7 +368,11 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
         }
     const double mu = a.mu[e], lam = a.lam[e];
     const NHPoint<double> p = nh_point(D);
-    if (!(p.J > 0)) atomicOr(a.flag, 1);
+    if (!(p.J > 0) && !snh_material(a)) atomicOr(a.flag, 1);
+    // SoftCon muscle; gen: material or muscle beyond plain Neo-Hookean
+    double mdir[3] = {0.0, 0.0, 0.0}, mL = 0.0;
+    const double mk = muscle_of(a, e, mdir, &mL);
+    const bool gen = snh_material(a) || mk != 0.0;
     // active fibre stress: P += sigma a (F f) f^T, H_ab += V0 sigma a (f.g_a)(f.g_b) I
     double fa[3] = {0.0, 0.0, 0.0}, Ff[3] = {0.0, 0.0, 0.0}, fg[4] = {0.0, 0.0, 0.0, 0.0};
     const double sa = active_level(a, e, fa);
@@ -317,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
     }
     // gradient (FP64): grad_a = V0 P g_a
     {
-        M3<double> P = nh_stress(p, mu, lam);
+        M3<double> P = elem_stress(a, p, D, mu, lam, mk, mdir, mL);
         if (sa != 0.0)
 #pragma unroll
             for (int r = 0; r < 3; ++r)
@@ -336,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
             }
         if (MODE == 2 && a.ex_energy) {
             bool inv = false;
-            a.ex_energy[e] = V0 * (nh_psi_disp(D, mu, lam, &inv) +
+            a.ex_energy[e] = V0 * (elem_psi(a, D, mu, lam, &inv, true, mk, mdir, mL) +
                                    0.5 * sa * (Ff[0] * Ff[0] + Ff[1] * Ff[1] + Ff[2] * Ff[2] - 1.0));
         }
     }
@@ -349,7 +416,34 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) gh[k][c] = (HT)g[k][c];
     const HT muh = (HT)mu, lamh = (HT)lam;
-    if (PROJ) {
+    if (gen) {
+        // Stable Neo-Hookean and / or muscle: the 9x9 tangent, then its blocks
+        M3<HT> Dh;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Dh.m[i] = (HT)D.m[i];
+        Sym9<HT> S;
+        elem_tangent<HT, PROJ>(a, Dh, muh, lamh, mk, mdir, mL, S);
+        for (int la = 0; la < 4; ++la)
+            for (int lb = 0; lb < 4; ++lb) {
+                HT blk[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) blk[i] = 0;
+                block_from_tangent(S, gh[la], gh[lb], w, blk);
+                if (sa != 0.0) {
+                    const HT s_ab = w * (HT)(sa * fg[la] * fg[lb]);
+                    blk[0] += s_ab;
+                    blk[4] += s_ab;
+                    blk[8] += s_ab;
+                }
+                if (MODE == 2) {
+                    for (int c = 0; c < 3; ++c)
+                        for (int d = 0; d < 3; ++d)
+                            a.ex_hess[(size_t)e * 144 + (3 * la + c) * 12 + 3 * lb + d] = (double)blk[3 * c + d];
+                } else {
+                    scatter_block<HT>(a, e, la, lb, nd[la], blk);
+                }
+            }
+    } else if (PROJ) {
         M3<HT> Dh;
 #pragma unroll
         for (int i = 0; i < 9; ++i) Dh.m[i] = (HT)D.m[i];
@@ -445,7 +539,9 @@ constexpr int tet_blk() { return sizeof(HT) == 4 ? 12 : 10; }
 // stored blocks per element: tet4 keeps the upper triangle
 constexpr int elem_blocks(int ed) { return ed == 4 ? 10 : ed * ed; }
 
-template <typename HT, int PROJ, int MODE>
+// GEN = 1: Stable Neo-Hookean and / or SoftCon muscle (the tangent is formed
+// as a 9x9 and contracted); GEN = 0: plain Neo-Hookean (structured blocks)
+template <typename HT, int PROJ, int MODE, int GEN>
 __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT* eb) {
     const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
@@ -486,7 +582,9 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
         }
     const double mu = a.mu[e], lam = a.lam[e];
     const NHPoint<double> p = nh_point(D);
-    if (!(p.J > 0)) atomicOr(a.flag, 1);
+    if (!(p.J > 0) && !(GEN && snh_material(a))) atomicOr(a.flag, 1);
+    double mdir[3] = {0.0, 0.0, 0.0}, mL = 0.0;
+    const double mk = GEN ? muscle_of(a, e, mdir, &mL) : 0.0;
     double fa[3] = {0.0, 0.0, 0.0}, Ff[3] = {0.0, 0.0, 0.0}, fg[4] = {0.0, 0.0, 0.0, 0.0};
     const double sa = active_level(a, e, fa);
     if (sa != 0.0) {
@@ -496,7 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
         for (int k = 0; k < 4; ++k) fg[k] = fa[0] * g[k][0] + fa[1] * g[k][1] + fa[2] * g[k][2];
     }
     {
-        M3<double> P = nh_stress(p, mu, lam);
+        M3<double> P = GEN ? elem_stress(a, p, D, mu, lam, mk, mdir, mL) : nh_stress(p, mu, lam);
         if (sa != 0.0)
 #pragma unroll
             for (int r = 0; r < 3; ++r)
@@ -521,7 +619,12 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
     Sym9<HT> S;
     HT cc = 0;
     HT h[4][3];
-    if (PROJ) {
+    if (GEN) {
+        M3<HT> Dh;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Dh.m[i] = (HT)D.m[i];
+        elem_tangent<HT, PROJ>(a, Dh, muh, lamh, mk, mdir, mL, S);
+    } else if (PROJ) {
         M3<HT> Dh;
 #pragma unroll
         for (int i = 0; i < 9; ++i) Dh.m[i] = (HT)D.m[i];
@@ -543,7 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
             HT blk[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) blk[i] = 0;
-            if (PROJ)
+            if (PROJ || GEN)
                 block_from_tangent(S, gh[la], gh[lb], w, blk);
             else
                 block_raw(muh, lamh, cc, gh[la], gh[lb], h[la], h[lb], w, blk);
@@ -740,6 +843,9 @@ __global__ __launch_bounds__(128) void k_hex(ElemArgs a) {
     }
     const double V0 = hex_volume(X);
     const double mu = a.mu[e], lam = a.lam[e];
+    double mdir[3] = {0.0, 0.0, 0.0}, mL = 0.0;
+    const double mk = muscle_of(a, e, mdir, &mL);
+    const bool gen = snh_material(a) || mk != 0.0;
     const double wq = V0 * 0.125;  // quadrature weight 1/8 times V0 (neoHookean.cpp:26-27)
     // gradient and energy, FP64
     {
@@ -761,15 +867,15 @@ __global__ __launch_bounds__(128) void k_hex(ElemArgs a) {
                     D(r, c) = acc;
                 }
             const NHPoint<double> p = nh_point(D);
-            if (!(p.J > 0)) atomicOr(a.flag, 1);
-            const M3<double> P = nh_stress(p, mu, lam);
+            if (!(p.J > 0) && !snh_material(a)) atomicOr(a.flag, 1);
+            const M3<double> P = elem_stress(a, p, D, mu, lam, mk, mdir, mL);
 #pragma unroll
             for (int k = 0; k < 8; ++k)
 #pragma unroll
                 for (int r = 0; r < 3; ++r) gr[k][r] += wq * (P(r, 0) * G[k][0] + P(r, 1) * G[k][1] + P(r, 2) * G[k][2]);
             if (MODE == 2) {
                 bool inv = false;
-                en += wq * nh_psi_disp(D, mu, lam, &inv);
+                en += wq * elem_psi(a, D, mu, lam, &inv, true, mk, mdir, mL);
             }
         }
 #pragma unroll
@@ -808,7 +914,11 @@ __global__ __launch_bounds__(128) void k_hex(ElemArgs a) {
                     D(r, c) = t;
                 }
             const NHPoint<HT> p = nh_point(D);
-            if (PROJ) {
+            if (gen) {
+                Sym9<HT> S;
+                elem_tangent<HT, PROJ>(a, D, muh, lamh, mk, mdir, mL, S);
+                for (int lb = 0; lb < 8; ++lb) block_from_tangent(S, G[la], G[lb], w, acc[lb]);
+            } else if (PROJ) {
                 Sym9<HT> S;
                 nh_projected_tangent(D, p, muh, lamh, S);
 #pragma unroll
@@ -858,13 +968,16 @@ __device__ __forceinline__ double hex_load(const ElemArgs& a, int e, double X[8]
     return V0 * 0.125;  // quadrature weight 1/8 times V0 (neoHookean.cpp:26-27)
 }
 
-// FP64 gradient rows of one element (same sums as k_hex)
+// FP64 gradient rows of one element (same sums as k_hex); GEN as in k_tet_elem
+template <int GEN>
 __global__ __launch_bounds__(128) void k_hex_grad(ElemArgs a, double* eg) {
     const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
     double X[8][3], u[8][3];
     const double wq = hex_load(a, e, X, u);
     const double mu = a.mu[e], lam = a.lam[e];
+    double mdir[3] = {0.0, 0.0, 0.0}, mL = 0.0;
+    const double mk = GEN ? muscle_of(a, e, mdir, &mL) : 0.0;
     double gr[8][3];
 #pragma unroll
     for (int k = 0; k < 8; ++k) gr[k][0] = gr[k][1] = gr[k][2] = 0;
@@ -882,8 +995,8 @@ __global__ __launch_bounds__(128) void k_hex_grad(ElemArgs a, double* eg) {
                 D(r, c) = acc;
             }
         const NHPoint<double> p = nh_point(D);
-        if (!(p.J > 0)) atomicOr(a.flag, 1);
-        const M3<double> P = nh_stress(p, mu, lam);
+        if (!(p.J > 0) && !(GEN && snh_material(a))) atomicOr(a.flag, 1);
+        const M3<double> P = GEN ? elem_stress(a, p, D, mu, lam, mk, mdir, mL) : nh_stress(p, mu, lam);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
 #pragma unroll
@@ -902,7 +1015,7 @@ __global__ __launch_bounds__(128) void k_hex_grad(ElemArgs a, double* eg) {
 // Gauss-point tangent is evaluated once per element, and t_i = g_a^T S(3i+.,
 // .) is formed once per Gauss point and reused for the eight blocks.
 constexpr int kHexRowThreads = 128;  // 16 elements per workgroup
-template <typename HT, int PROJ>
+template <typename HT, int PROJ, int GEN>
 __global__ __launch_bounds__(kHexRowThreads) void k_hex_rows(ElemArgs a, HT* eb) {
     constexpr int kEl = kHexRowThreads / 8;
     __shared__ HT sS[kEl][8][45];  // per Gauss point: Sym9 (PROJ) or F^-T and c (raw)
@@ -930,7 +1043,14 @@ __global__ __launch_bounds__(kHexRowThreads) void k_hex_rows(ElemArgs a, HT* eb)
                 D(r, c) = tt;
             }
         const NHPoint<HT> p = nh_point(D);
-        if (PROJ) {
+        if (GEN) {
+            double mdir[3] = {0.0, 0.0, 0.0}, mL = 0.0;
+            const double mk = muscle_of(a, e, mdir, &mL);
+            Sym9<HT> S;
+            elem_tangent<HT, PROJ>(a, D, (HT)mu, (HT)lam, mk, mdir, mL, S);
+#pragma unroll
+            for (int i = 0; i < 45; ++i) sS[le][j][i] = S.a[i];
+        } else if (PROJ) {
             Sym9<HT> S;
             nh_projected_tangent(D, p, (HT)mu, (HT)lam, S);
 #pragma unroll
@@ -956,7 +1076,7 @@ __global__ __launch_bounds__(kHexRowThreads) void k_hex_rows(ElemArgs a, HT* eb)
     for (int s = 0; s < 8; ++s) {
         const HT* Gs = sG[le][s];
         const HT ga[3] = {Gs[3 * la], Gs[3 * la + 1], Gs[3 * la + 2]};
-        if (PROJ) {
+        if (PROJ || GEN) {
             Sym9<HT> S;
 #pragma unroll
             for (int i = 0; i < 45; ++i) S.a[i] = sS[le][s][i];
@@ -1006,7 +1126,8 @@ __global__ __launch_bounds__(kHexRowThreads) void k_hex_rows(ElemArgs a, HT* eb)
 }
 
 // ---- element energy: absolute (diff = 0) or E(q + alpha dq) - E(q) -------
-__device__ double tet_energy_at(const ElemArgs& a, int e, const int nd[4], double alpha, bool* inv) {
+// absolute: include Stable Neo-Hookean's rest energy (left out of differences)
+__device__ double tet_energy_at(const ElemArgs& a, int e, const int nd[4], double alpha, bool* inv, bool absolute) {
     double X[4][3], x[4][3];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -1049,7 +1170,9 @@ __device__ double tet_energy_at(const ElemArgs& a, int e, const int nd[4], doubl
             for (int k = 0; k < 4; ++k) acc += (x[k][r] - X[k][r]) * g[k][c];
             D(r, c) = acc;
         }
-    double en = nh_psi_disp(D, a.mu[e], a.lam[e], inv);
+    double mdir[3] = {0.0, 0.0, 0.0}, mL = 0.0;
+    const double mk = muscle_of(a, e, mdir, &mL);
+    double en = elem_psi(a, D, a.mu[e], a.lam[e], inv, absolute, mk, mdir, mL);
     double fa[3];
     const double sa = active_level(a, e, fa);
     if (sa != 0.0) {
@@ -1064,7 +1187,7 @@ __device__ double tet_energy_at(const ElemArgs& a, int e, const int nd[4], doubl
     return fabs(detm) / 6.0 * en;
 }
 
-__device__ double hex_energy_at(const ElemArgs& a, int e, const int nd[8], double alpha, bool* inv) {
+__device__ double hex_energy_at(const ElemArgs& a, int e, const int nd[8], double alpha, bool* inv, bool absolute) {
     double X[8][3], u[8][3];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
@@ -1076,6 +1199,8 @@ __device__ double hex_energy_at(const ElemArgs& a, int e, const int nd[8], doubl
             u[k][c] = v - X[k][c];
         }
     const double V0 = hex_volume(X);
+    double mdir[3] = {0.0, 0.0, 0.0}, mL = 0.0;
+    const double mk = muscle_of(a, e, mdir, &mL);
     double en = 0;
     for (int s = 0; s < 8; ++s) {
         double G[8][3];
@@ -1090,7 +1215,7 @@ __device__ double hex_energy_at(const ElemArgs& a, int e, const int nd[8], doubl
                 for (int k = 0; k < 8; ++k) acc += u[k][r] * G[k][c];
                 D(r, c) = acc;
             }
-        en += nh_psi_disp(D, a.mu[e], a.lam[e], inv);
+        en += elem_psi(a, D, a.mu[e], a.lam[e], inv, absolute, mk, mdir, mL);
     }
     return V0 * 0.125 * en;
 }
@@ -1105,11 +1230,11 @@ __global__ __launch_bounds__(kBlock) void k_elem_energy(ElemArgs a, int diff, do
         for (int k = 0; k < ED; ++k) nd[k] = a.conn[(size_t)k * a.E + e];
         double en;
         if (ED == 4) {
-            en = tet_energy_at(a, e, nd, a.alpha, &inv);
-            if (diff) en -= tet_energy_at(a, e, nd, 0.0, &inv);
+            en = tet_energy_at(a, e, nd, a.alpha, &inv, !diff);
+            if (diff) en -= tet_energy_at(a, e, nd, 0.0, &inv, false);
         } else {
-            en = hex_energy_at(a, e, nd, a.alpha, &inv);
-            if (diff) en -= hex_energy_at(a, e, nd, 0.0, &inv);
+            en = hex_energy_at(a, e, nd, a.alpha, &inv, !diff);
+            if (diff) en -= hex_energy_at(a, e, nd, 0.0, &inv, false);
         }
         v[0] += en;
     }
@@ -2569,6 +2694,10 @@ struct sors_ctx {
     double sigma = 0.0;
     std::vector<double> h_nu;       // Poisson ratio per element (Lame updates, dE adjoint)
     DevMem agroup, afib, act, nu, adjb, adjq, adjv, adjE, adjA;
+    // SoftCon muscle (sors_set_muscle)
+    bool has_muscle = false;
+    int mngroups = 0;
+    DevMem mgroup, mdir, mk, mact;
     // contact (sors_set_contact): primitives by value, candidates and lag on the device
     bool has_contact = false;
     ContactArgs cargs{};
@@ -2609,6 +2738,18 @@ struct sors_ctx {
 };
 
 namespace {
+
+// per-element material constants held on the device: the Lame constants of
+// (E, nu) (neoHookean.cpp:10-11), reparameterised for Stable Neo-Hookean as
+// mu' = 4/3 mu, lam' = lam + 5/8 mu' (stableNeoHookean.cpp:9-14)
+void material_constants(int material, double Ey, double nu, double* mu, double* lam) {
+    *lam = (Ey * nu) / ((1 + nu) * (1 - 2 * nu));
+    *mu = Ey / (2 * (1 + nu));
+    if (material == SORS_STABLE_NEOHOOKEAN) {
+        *mu = (4.0 / 3.0) * (*mu);
+        *lam = *lam + (5.0 / 8.0) * (*mu);
+    }
+}
 
 int fail(sors_ctx* c, int code, const std::string& m) {
     if (c) c->err = m;
@@ -2696,6 +2837,12 @@ ElemArgs elem_args(sors_ctx* c, const double* q) {
         a.act = c->act.as<double>();
         a.sigma = c->sigma;
     }
+    if (c->has_muscle) {
+        a.mgroup = c->mgroup.as<int>();
+        a.mdir = c->mdir.as<double>();
+        a.mk = c->mk.as<double>();
+        a.mact = c->mact.as<double>();
+    }
     return a;
 }
 
@@ -2767,15 +2914,20 @@ void launch_elem(sors_ctx* c, const ElemArgs& a, int project, hipEvent_t e0 = nu
         ensure_elem_rows(c, MODE == 0 ? sizeof(HT) : 0);
         double* eg = c->teg.as<double>();
         HT* eb = MODE == 0 ? (HT*)c->teb.p : nullptr;
+        // Stable Neo-Hookean or a muscle: the general element kernels
+        const bool gen = c->material == SORS_STABLE_NEOHOOKEAN || c->has_muscle;
         if (c->ed == 4) {
-            void (*k1)(ElemArgs, double*, HT*) = project ? k_tet_elem<HT, 1, MODE> : k_tet_elem<HT, 0, MODE>;
+            void (*k1)(ElemArgs, double*, HT*) =
+                gen ? (project ? k_tet_elem<HT, 1, MODE, 1> : k_tet_elem<HT, 0, MODE, 1>)
+                    : (project ? k_tet_elem<HT, 1, MODE, 0> : k_tet_elem<HT, 0, MODE, 0>);
             hipExtLaunchKernelGGL(k1, dim3(grid_for(c->E, kBlock)), dim3(kBlock), 0, c->stream, e0, nullptr, 0, a,
                                   eg, eb);
         } else {
-            hipExtLaunchKernelGGL(k_hex_grad, dim3(grid_for(c->E, 128)), dim3(128), 0, c->stream, e0, nullptr, 0, a,
-                                  eg);
+            hipExtLaunchKernelGGL(gen ? k_hex_grad<1> : k_hex_grad<0>, dim3(grid_for(c->E, 128)), dim3(128), 0,
+                                  c->stream, e0, nullptr, 0, a, eg);
             if (MODE == 0) {
-                void (*kr)(ElemArgs, HT*) = project ? k_hex_rows<HT, 1> : k_hex_rows<HT, 0>;
+                void (*kr)(ElemArgs, HT*) = gen ? (project ? k_hex_rows<HT, 1, 1> : k_hex_rows<HT, 0, 1>)
+                                                : (project ? k_hex_rows<HT, 1, 0> : k_hex_rows<HT, 0, 0>);
                 kr<<<grid_for(8LL * c->E, kHexRowThreads), kHexRowThreads, 0, c->stream>>>(a, eb);
             }
         }
@@ -3604,7 +3756,8 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->incptr, &c->inc,  &c->teg,      &c->teb,    &c->tsptr,   &c->tsidx,   &c->tqslice, &c->vold,     &c->cdn,    &c->captr,   &c->canodes, &c->crot,
                            &c->cpairI, &c->cpairJ, &c->cpchunk, &c->cgptr, &c->cci,     &c->ccj,     &c->ccbase,
                            &c->cE,     &c->cEinv, &c->crc,     &c->cX,      &c->cgpart,  &c->cpart,   &c->cinfo,
-                           &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA};
+                           &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA,   &c->mgroup, &c->mdir,
+                           &c->mk,     &c->mact};
     double b = 0;
     for (const DevMem* d : all) b += (double)d->bytes;
     return b;
@@ -4161,6 +4314,10 @@ int sors_create(const sors_system_desc* d, sors_ctx** out) {
         g_create_error = "nodes_per_elem must be 4 (Tetrahedron) or 8 (Hexahedron)";
         return SORS_ERR_ARG;
     }
+    if (d->material != SORS_NEOHOOKEAN && d->material != SORS_TEST_POLY && d->material != SORS_STABLE_NEOHOOKEAN) {
+        g_create_error = "Unknown energy type (material " + std::to_string(d->material) + ")";
+        return SORS_ERR_ARG;
+    }
     if (d->material == SORS_TEST_POLY && d->nodes_per_elem != 4) {
         g_create_error = "the polynomial test energy is defined for tetrahedra";
         return SORS_ERR_ARG;
@@ -4227,8 +4384,7 @@ int sors_create(const sors_system_desc* d, sors_ctx** out) {
         for (int e = 0; e < E; ++e) {
             for (int k = 0; k < ed; ++k) connT[(size_t)k * E + e] = c->h_conn[(size_t)e * ed + k];
             const double Ey = d->youngs ? d->youngs[e] : 0.0, nu = d->poisson ? d->poisson[e] : 0.0;
-            lam[e] = (Ey * nu) / ((1 + nu) * (1 - 2 * nu));
-            mu[e] = Ey / (2 * (1 + nu));
+            material_constants(c->material, Ey, nu, &mu[e], &lam[e]);
             const int* cn = &c->h_conn[(size_t)e * ed];
             auto P = [&](int k) { return d->rest + 3 * (size_t)cn[k]; };
             auto tv = [&](int a, int b, int cc, int dd) {
@@ -4251,7 +4407,7 @@ int sors_create(const sors_system_desc* d, sors_ctx** out) {
                 V0 = 0;
                 for (int t = 0; t < 5; ++t) V0 += tv(T5[t][0], T5[t][1], T5[t][2], T5[t][3]);
             }
-            if (!(V0 > 0.0) && c->material == SORS_NEOHOOKEAN) {
+            if (!(V0 > 0.0) && c->material != SORS_TEST_POLY) {
                 g_create_error = "Volume of element must be positive (element " + std::to_string(e) + ")";
                 return SORS_ERR_ARG;
             }
@@ -4857,15 +5013,50 @@ int sors_set_activation(sors_ctx* c, const double* act) {
     SORS_GUARD_END(c)
 }
 
+int sors_set_muscle(sors_ctx* c, const sors_muscle_desc* d) {
+    SORS_GUARD(c)
+    if (!d) {
+        c->has_muscle = false;
+        return SORS_OK;
+    }
+    if (c->material == SORS_TEST_POLY)
+        return fail(c, SORS_ERR_UNSUPPORTED, "the muscle energy needs a Tetrahedron or Hexahedron element");
+    if (d->n_groups < 1 || !d->elem_group || !d->direction || !d->stiffness)
+        return fail(c, SORS_ERR_ARG, "muscle needs n_groups >= 1, groups, directions and stiffnesses");
+    const int E = c->E;
+    std::vector<double> dir((size_t)3 * E);
+    for (int e = 0; e < E; ++e) {
+        const int gi = d->elem_group[e];
+        if (gi < -1 || gi >= d->n_groups) return fail(c, SORS_ERR_ARG, "element muscle group out of range");
+        for (int k = 0; k < 3; ++k) dir[(size_t)k * E + e] = d->direction[3 * (size_t)e + k];
+    }
+    upload(c->mgroup, d->elem_group, (size_t)E, c->stream);
+    upload(c->mdir, dir.data(), dir.size(), c->stream);
+    upload(c->mk, d->stiffness, (size_t)E, c->stream);
+    ensure(c->mact, sizeof(double) * d->n_groups);
+    fill<double>(c->mact.as<double>(), d->n_groups, 0.0, c->stream);
+    c->mngroups = d->n_groups;
+    c->has_muscle = true;
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_set_muscle_activation(sors_ctx* c, const double* act) {
+    SORS_GUARD(c)
+    if (!c->has_muscle) return fail(c, SORS_ERR_ARG, "no muscle energy set");
+    if (!act) return fail(c, SORS_ERR_ARG, "null activations");
+    upload(c->mact, act, (size_t)c->mngroups, c->stream);
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
 int sors_set_youngs(sors_ctx* c, const double* youngs) {
     SORS_GUARD(c)
     if (!youngs) return fail(c, SORS_ERR_ARG, "null Young's moduli");
     std::vector<double> mu(c->E), lam(c->E);
-    for (int e = 0; e < c->E; ++e) {
-        const double Ey = youngs[e], nu = c->h_nu[e];
-        lam[e] = (Ey * nu) / ((1 + nu) * (1 - 2 * nu));
-        mu[e] = Ey / (2 * (1 + nu));
-    }
+    for (int e = 0; e < c->E; ++e) material_constants(c->material, youngs[e], c->h_nu[e], &mu[e], &lam[e]);
     upload(c->mu, mu.data(), mu.size(), c->stream);
     upload(c->lam, lam.data(), lam.size(), c->stream);
     sync(c);
@@ -4899,6 +5090,7 @@ int sors_adjoint_step(sors_ctx* c, const double* q_prev, const double* v_prev, c
                     "the adjoint is implemented for backward Euler, Newton methods, Neo-Hookean tetrahedra");
     if (c->has_contact && (c->cargs.mu > 0.0 || c->cargs.ndisks > 0))
         return fail(c, SORS_ERR_UNSUPPORTED, "the adjoint takes frictionless plane contact only");
+    if (c->has_muscle) return fail(c, SORS_ERR_UNSUPPORTED, "the adjoint does not take the muscle energy");
     const int n = c->n, E = c->E;
     ensure_pattern(c, dt);
     upload(c->qprev, q_prev, (size_t)n, c->stream);

@@ -20,7 +20,7 @@ ERR_INVERTED, ERR_NOCONV, ERR_NAN, ERR_LINESEARCH, ERR_ARG, ERR_HIP, ERR_UNSUPPO
 ERR_TIMEOUT, ERR_BUDGET = -8, -9
 BACKWARD_EULER, CRANK_NICOLSON = 0, 1
 MINIMIZE_NEWTON, MINIMIZE_NEWTON_LINESEARCH, MINIMIZE_SQP, MINIMIZE_FROZENHESSIAN = 0, 1, 2, 3
-NEOHOOKEAN, TEST_POLY = 0, 1
+NEOHOOKEAN, TEST_POLY, STABLE_NEOHOOKEAN = 0, 1, 2
 FP64, MIXED = 0, 1
 KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_COPY, KERNEL_PCG_UPDATE = 0, 1, 2, 3
 
@@ -75,6 +75,10 @@ class ContactDesc(C.Structure):
 
 class ActiveDesc(C.Structure):
     _fields_ = [("n_groups", C.c_int), ("elem_group", _ip), ("fibre", _dp), ("sigma_max", C.c_double)]
+
+
+class MuscleDesc(C.Structure):
+    _fields_ = [("n_groups", C.c_int), ("elem_group", _ip), ("direction", _dp), ("stiffness", _dp)]
 
 
 class ServerOptions(C.Structure):
@@ -156,6 +160,8 @@ def lib():
         L.sors_set_active.argtypes = [vp, C.POINTER(ActiveDesc)]
         L.sors_set_activation.argtypes = [vp, _dp]
         L.sors_set_youngs.argtypes = [vp, _dp]
+        L.sors_set_muscle.argtypes = [vp, C.POINTER(MuscleDesc)]
+        L.sors_set_muscle_activation.argtypes = [vp, _dp]
         L.sors_set_state.argtypes = [vp, _dp, _dp]
         L.sors_adjoint_step.argtypes = [vp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
                                         C.c_int, C.POINTER(C.c_longlong)]
@@ -297,6 +303,25 @@ class Rollout:
         a = f64(act)
         assert a.size == self.n_groups
         self._check(lib().sors_set_activation(self.h, _d(a)))
+
+    def set_muscle(self, group, stiffness, direction):
+        """SoftCon muscle on elements with group[e] >= 0 (stiffness [E],
+        direction [E, 3]); group None removes it."""
+        if group is None:
+            self._check(lib().sors_set_muscle(self.h, None))
+            return
+        g = i32(group)
+        k = f64(stiffness)
+        m = f64(direction).reshape(-1)
+        self._muscle_keep = (g, k, m)
+        self.muscle_groups = int(g.max()) + 1
+        d = MuscleDesc(self.muscle_groups, _i(g), _d(m), _d(k))
+        self._check(lib().sors_set_muscle(self.h, C.byref(d)))
+
+    def set_muscle_activation(self, act):
+        a = f64(act)
+        assert a.size == self.muscle_groups
+        self._check(lib().sors_set_muscle_activation(self.h, _d(a)))
 
     def set_youngs(self, E):
         e = f64(E)
