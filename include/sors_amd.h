@@ -339,8 +339,9 @@ int sors_adjoint_step(sors_ctx* ctx, const double* q_prev, const double* v_prev,
  * and line-search kernels run on the rollout's own stream while the PCG
  * iterations of all rollouts in their linear solve run as one batched
  * launch per kernel.  Each rollout follows exactly the Newton logic of
- * sors_step with sors_step's predictor (options.predict), so its result is
- * that of sors_step alone up to the rounding order of float atomics. */
+ * sors_step with sors_step's predictor (options.predict) and its FP64
+ * continuation of a mixed solve that reaches max_newton, so on the sliced
+ * node-block layout its result is that of sors_step alone bit for bit. */
 typedef struct sors_server sors_server;
 
 typedef struct {
@@ -359,6 +360,7 @@ typedef struct {
     double queue_ms;           /* submit -> admitted */
     double prep_ms;            /* admitted -> resident (pattern, upload) */
     double device_bytes;       /* device memory held while resident */
+    int fp64_fallbacks;        /* mixed solves continued in FP64 (as sors_step) */
 } sors_request_stats;
 
 typedef struct {

@@ -1,6 +1,8 @@
 // sors_facade.cpp -- host interfaces of the reference over the device C ABI.
 #include "sors_facade.h"
 
+#include <algorithm>
+
 #include <cmath>
 
 #include <fstream>
@@ -127,19 +129,50 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
         throw std::invalid_argument("one energy set and one Params per element are required");
     std::vector<double> rest = X.row_major();
     std::vector<double> Y(nel), P(nel), D(nel);
-    int material = SORS_NEOHOOKEAN;
+    // element energy sets (tetrahedron.cpp:55-95, hexahedron.cpp:24-64): one
+    // elastic energy, the same on every element ("neohookean" or
+    // "stableneohookean"), plus "softconMuscle" on any subset of elements
+    int material = -1;
+    std::vector<int> mgroup(nel, -1);
+    std::vector<double> mk(nel, 0.0), mdir(3 * (size_t)nel, 0.0);
+    int mgroups = 0;
     for (int e = 0; e < nel; ++e) {
         const auto& set = elementEnergiesList[e];
-        for (const auto& name : set)
-            if (name != "neohookean")
-                throw std::runtime_error("Unknown energy type: " + name +
-                                         " (the device path implements \"neohookean\")");
         if (set.empty()) throw std::runtime_error("element " + std::to_string(e) + " has no energy");
         const Params& p = elementParameterList[e];
+        int elastic = -1;
+        for (const auto& name : set) {
+            if (name == "neohookean" || name == "stableneohookean") {
+                const int m = name == "neohookean" ? SORS_NEOHOOKEAN : SORS_STABLE_NEOHOOKEAN;
+                if (elastic >= 0)
+                    throw std::runtime_error("element " + std::to_string(e) +
+                                             ": the device path takes one elastic energy per element");
+                elastic = m;
+            } else if (name == "softconMuscle") {
+                mgroup[e] = (int)p.get_value("muscleGroup")(0, 0);
+                mk[e] = p.get_value("softconStiffness")(0, 0);
+                auto dir = p.get_value("softconDirection").row_major();  // reshaped<RowMajor>()
+                if (dir.size() != 3) throw std::invalid_argument("softconDirection must have 3 entries");
+                for (int k = 0; k < 3; ++k) mdir[3 * (size_t)e + k] = dir[k];
+                if (mgroup[e] < 0) throw std::invalid_argument("muscleGroup must be >= 0");
+                mgroups = std::max(mgroups, mgroup[e] + 1);
+            } else {
+                throw std::runtime_error("Unknown energy type: " + name +
+                                         " (the device path implements \"neohookean\", \"stableneohookean\" "
+                                         "and \"softconMuscle\")");
+            }
+        }
+        if (elastic < 0)
+            throw std::runtime_error("element " + std::to_string(e) +
+                                     ": the device path needs an elastic energy (neohookean or stableneohookean)");
+        if (material >= 0 && elastic != material)
+            throw std::runtime_error("the device path takes one elastic energy type for all elements");
+        material = elastic;
         D[e] = p.get_value("density")(0, 0);
         Y[e] = p.get_value("youngsModulus")(0, 0);
         P[e] = p.get_value("poissonsRatio")(0, 0);
     }
+    if (material < 0) material = SORS_NEOHOOKEAN;
     std::vector<int> mask;
     std::vector<double> bcv;
     // planeContact / diskContact (energy.cpp:69-105): the device resolves them
@@ -218,11 +251,24 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
         d.method = SORS_MINIMIZE_FROZENHESSIAN;
     else if (settings.solverMethod == "minimize_SQP")
         // the reference's contact solver; with contact the device runs its
-        // penalty formulation under Newton + line search
-        d.method = hasContact_ ? SORS_MINIMIZE_NEWTON_LINESEARCH : SORS_MINIMIZE_SQP;
+        // penalty formulation under Newton + line search, or plain Newton
+        // when a muscle is present (its energy is not the potential of its
+        // gradient, softconMuscle.cpp:31 vs :72, so an Armijo test on it is
+        // meaningless; SQP itself never evaluates the energy)
+        d.method = hasContact_ ? (mgroups > 0 ? SORS_MINIMIZE_NEWTON : SORS_MINIMIZE_NEWTON_LINESEARCH)
+                               : SORS_MINIMIZE_SQP;
     else
         throw std::runtime_error("Solver method not recognized");
     check_status(sors_create(&d, &ctx_), nullptr);
+    if (mgroups > 0) {
+        sors_muscle_desc md{};
+        md.n_groups = mgroups;
+        md.elem_group = mgroup.data();
+        md.direction = mdir.data();
+        md.stiffness = mk.data();
+        check_status(sors_set_muscle(ctx_, &md), ctx_);
+        muscleGroups_ = mgroups;
+    }
     if (hasContact_) {
         sors_contact_desc cd{};
         cd.n_planes = (int)(pp.size() / 3);
@@ -287,6 +333,14 @@ std::vector<double> Energy<E>::get_vPrev() const {
 // (energy.cpp:282-293, vertexForce.cpp:4-8)
 template <int E>
 void Energy<E>::apply_actuation(const Params& act) {
+    // "softconMuscle": one activation per muscle group (element.h:52,
+    // softconMuscle.cpp:29 actuation.coeff(muscleGroup_))
+    if (muscleGroups_ > 0 && act.has("softconMuscle")) {
+        auto a = act.get_value("softconMuscle").row_major();
+        if ((int)a.size() < muscleGroups_)
+            throw std::invalid_argument("softconMuscle actuation needs one value per muscle group");
+        check_status(sors_set_muscle_activation(ctx_, a.data()), ctx_);
+    }
     bool vf = false;
     for (const auto& f : forceTypesList_) vf |= (f == "vertexForce");
     if (!vf) return;

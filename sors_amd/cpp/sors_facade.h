@@ -77,9 +77,12 @@ struct SimulationSettings {
     std::string precision = "mixed";  // "mixed" | "fp64"
     double pcgTolerance = 1e-6;
     bool predictor = false;           // start Newton from the extrapolated state instead of the given solution
+                                      // (the same minimiser).  Off: Solver::step starts Newton at the caller's
+                                      // solution as the reference does (solver.cpp:22-40); the C ABI turns
+                                      // it on by default (sors_default_options: predict = 2)
     int coarseNodes = 2048;           // > 0: two-level PCG (rigid-body coarse correction on aggregates of this
                                       // size); 0: block-Jacobi alone
-    int coarseEvery = 8;              // coarse operator refreshed on every k-th tangent
+    int coarseEvery = 16;             // coarse operator refreshed on every k-th tangent (as sors_default_options)
     void verbose_print() const;
 };
 
@@ -118,6 +121,7 @@ private:
     sors_ctx* ctx_ = nullptr;
     int numVertices_ = 0;
     bool hasContact_ = false;
+    int muscleGroups_ = 0;
 };
 
 // Non-owning handle on the diskContact entry of an Energy's constraint list

@@ -3738,6 +3738,9 @@ struct Member {
     // aimed: the last solve's forcing term came from g_tol, so the next
     // residual check assembles the gradient alone (grad_only), as in minimize()
     bool aimed = false, grad_only = false;
+    // the current solve runs in FP64: an all-FP64 server, or a mixed solve that
+    // reached max_newton and goes on from its iterate in FP64 (as sors_step)
+    bool fp64 = false;
     int step = 0, it = 0, bt = 0;
     long long launched = 0;
     double eta = 0, dxn = 0, gdx = 0, alpha = 1;
@@ -3896,6 +3899,7 @@ void srv_retire(sors_server* s, Member& m, int status, const std::string& msg) {
     r->stats.newton_iters = m.st.newton_iters;
     r->stats.pcg_iters = m.st.pcg_iters;
     r->stats.backtracks = m.st.backtracks;
+    r->stats.fp64_fallbacks = m.st.fp64_fallbacks;
     std::lock_guard<std::mutex> lk(s->mu);
     s->tasks.push_front([s, r, c, status, msg] {
         int rc = status;
@@ -3953,52 +3957,17 @@ void srv_prep_loop(sors_server* s) {
 // starters (init, SpMV, update, coarse) and the runners (SpMV, update, coarse)
 constexpr int kStageTables = 11;
 
-// one tick over the resident rollouts
+// the batched PCG chunk of the rollouts in their linear solve at precision HT
+// (starters begin their solve this tick)
 template <typename HT>
-void srv_tick(sors_server* s) {
-    const sors_options& o = s->opt.solver;
-    std::vector<int> starters, runners;
-    // 1. per-rollout phase work on the rollouts' streams
-    for (int i = 0; i < (int)s->active.size(); ++i) {
-        Member& m = s->active[i];
-        sors_ctx* c = m.c;
-        const double dt = m.r->dt;
-        double* q = c->q.as<double>();
-        m.busy = false;
-        switch (m.phase) {
-            case Member::ASSEMBLE:
-                if (m.new_step) {
-                    launch_predict(c, dt);
-                    m.new_step = false;
-                }
-                clear_flag(c);
-                m.grad_only = m.aimed;
-                assemble<HT>(c, q, dt, !m.grad_only, o.project_spd);
-                m.busy = true;
-                break;
-            case Member::POST:
-                enqueue_step_dots<HT>(c, c->g.as<double>(), c->dxd.as<double>());
-                if (c->method == SORS_MINIMIZE_NEWTON_LINESEARCH) enqueue_energy(c, q, c->dxd.as<double>(), 1.0, dt);
-                m.busy = true;
-                break;
-            case Member::ENERGY:
-                enqueue_energy(c, q, c->dxd.as<double>(), m.alpha, dt);
-                m.busy = true;
-                break;
-            case Member::PCG_START:
-                starters.push_back(i);
-                runners.push_back(i);
-                break;
-            case Member::PCG_RUN:
-                runners.push_back(i);
-                break;
-        }
-    }
-    // 2. batched PCG on the server stream
+void srv_batched_pcg(sors_server* s, const std::vector<int>& starters, const std::vector<int>& runners) {
     const int chunk = std::max(2, (s->opt.pcg_chunk + 1) / 2 * 2);
-    if (!runners.empty()) {
+    {
         const int M = s->opt.max_active;
-        BMember<HT>* hm = (BMember<HT>*)s->h_stage;
+        // one staging area per precision: both may be in flight in one tick
+        const size_t area = sizeof(BMember<double>) * M + sizeof(int) * kStageTables * (M + 1);
+        const int which = sizeof(HT) == 8 ? 1 : 0;
+        BMember<HT>* hm = (BMember<HT>*)((char*)s->h_stage + which * area);
         int* ht = (int*)(hm + M);
         int* tb[kStageTables];
         for (int k = 0; k < kStageTables; ++k) tb[k] = ht + k * (M + 1);
@@ -4049,9 +4018,10 @@ void srv_tick(sors_server* s) {
             }
         }
         const size_t stage = sizeof(BMember<HT>) * M + sizeof(int) * kStageTables * (M + 1);
-        HIPX(hipMemcpyAsync(s->bmem.p, s->h_stage, stage, hipMemcpyHostToDevice, s->stream));
-        const BMember<HT>* dm = (const BMember<HT>*)s->bmem.p;
-        const int* dt_ = (const int*)((const char*)s->bmem.p + sizeof(BMember<HT>) * M);
+        char* dbase = (char*)s->bmem.p + which * area;
+        HIPX(hipMemcpyAsync(dbase, hm, stage, hipMemcpyHostToDevice, s->stream));
+        const BMember<HT>* dm = (const BMember<HT>*)dbase;
+        const int* dt_ = (const int*)(dbase + sizeof(BMember<HT>) * M);
         const int* db[kStageTables];
         for (int k = 0; k < kStageTables; ++k) db[k] = dt_ + k * (M + 1);
         const int *dIdxS = db[0], *dIS = db[1], *dAS = db[2], *dUS = db[3], *dIdxCS = db[4], *dCS = db[5];
@@ -4090,6 +4060,60 @@ void srv_tick(sors_server* s) {
         }
         for (int i : runners) s->active[i].launched += chunk;
     }
+}
+
+// one tick over the resident rollouts; each rollout's kernels run in its own
+// precision (member fp64), and the batched PCG once per precision in use
+void srv_tick(sors_server* s) {
+    const sors_options& o = s->opt.solver;
+    const bool mixed = o.precision == SORS_MIXED;
+    auto in_fp64 = [&](const Member& m) { return !mixed || m.fp64; };
+    std::vector<int> starters[2], runners[2];  // [0] FP32 solves, [1] FP64 solves
+    // 1. per-rollout phase work on the rollouts' streams
+    for (int i = 0; i < (int)s->active.size(); ++i) {
+        Member& m = s->active[i];
+        sors_ctx* c = m.c;
+        const double dt = m.r->dt;
+        double* q = c->q.as<double>();
+        m.busy = false;
+        switch (m.phase) {
+            case Member::ASSEMBLE:
+                if (m.new_step) {
+                    launch_predict(c, dt);
+                    m.new_step = false;
+                }
+                clear_flag(c);
+                m.grad_only = m.aimed;
+                if (in_fp64(m))
+                    assemble<double>(c, q, dt, !m.grad_only, o.project_spd);
+                else
+                    assemble<float>(c, q, dt, !m.grad_only, o.project_spd);
+                m.busy = true;
+                break;
+            case Member::POST:
+                if (in_fp64(m))
+                    enqueue_step_dots<double>(c, c->g.as<double>(), c->dxd.as<double>());
+                else
+                    enqueue_step_dots<float>(c, c->g.as<double>(), c->dxd.as<double>());
+                if (c->method == SORS_MINIMIZE_NEWTON_LINESEARCH) enqueue_energy(c, q, c->dxd.as<double>(), 1.0, dt);
+                m.busy = true;
+                break;
+            case Member::ENERGY:
+                enqueue_energy(c, q, c->dxd.as<double>(), m.alpha, dt);
+                m.busy = true;
+                break;
+            case Member::PCG_START:
+                starters[in_fp64(m)].push_back(i);
+                runners[in_fp64(m)].push_back(i);
+                break;
+            case Member::PCG_RUN:
+                runners[in_fp64(m)].push_back(i);
+                break;
+        }
+    }
+    // 2. batched PCG on the server stream, FP32 solves then FP64 solves
+    if (!runners[0].empty()) srv_batched_pcg<float>(s, starters[0], runners[0]);
+    if (!runners[1].empty()) srv_batched_pcg<double>(s, starters[1], runners[1]);
     // 3. one readback of every slot
     for (Member& m : s->active)
         if (m.busy) HIPX(hipStreamSynchronize(m.c->stream));
@@ -4116,11 +4140,22 @@ void srv_tick(sors_server* s) {
             }
             m.new_step = true;
             m.aimed = false;
+            m.fp64 = false;
             m.it = 0;
             m.phase = Member::ASSEMBLE;
         };
         auto next_newton = [&] {
             if (++m.it >= o.max_newton) {
+                if (mixed && !m.fp64) {
+                    // as sors_step: the FP32 tangents did not bring this solve
+                    // to the tolerances; it goes on from its iterate in FP64
+                    m.fp64 = true;
+                    m.st.fp64_fallbacks++;
+                    m.it = 0;
+                    m.aimed = false;
+                    m.phase = Member::ASSEMBLE;
+                    return;
+                }
                 status = SORS_ERR_NOCONV;
                 msg = "Newton's method did not converge";
                 return;
@@ -4181,7 +4216,10 @@ void srv_tick(sors_server* s) {
                 m.aimed = o.g_tol > 0 && eta > o.pcg_rtol;
                 // preconditioner only for a tangent that is solved (as minimize()
                 // does); the next tick's batched launches wait for it
-                build_preconditioner<HT>(c);
+                if (in_fp64(m))
+                    build_preconditioner<double>(c);
+                else
+                    build_preconditioner<float>(c);
                 HIPX(hipEventRecord(c->evPre, c->stream));
                 m.phase = Member::PCG_START;
                 break;
@@ -4237,7 +4275,6 @@ void srv_tick(sors_server* s) {
 
 void srv_worker(sors_server* s) {
     (void)hipSetDevice(s->device);
-    const bool mixed = s->opt.solver.precision == SORS_MIXED;
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
@@ -4265,10 +4302,7 @@ void srv_worker(sors_server* s) {
         }
         if (s->active.empty()) continue;
         try {
-            if (mixed)
-                srv_tick<float>(s);
-            else
-                srv_tick<double>(s);
+            srv_tick(s);
         } catch (const HipError& e) {
             std::vector<Member> all;
             all.swap(s->active);
@@ -5182,7 +5216,8 @@ int sors_server_create(const sors_server_options* o, sors_server** out) {
         const int M = o->max_active;
         ensure(s->slots, sizeof(SlotRec) * M);
         HIPX(hipMemset(s->slots.p, 0, sizeof(SlotRec) * M));
-        const size_t stage = sizeof(BMember<double>) * M + sizeof(int) * kStageTables * (M + 1);
+        // two staging areas (FP32 and FP64 solves may both run in one tick)
+        const size_t stage = 2 * (sizeof(BMember<double>) * M + sizeof(int) * kStageTables * (M + 1));
         static_assert(sizeof(BMember<double>) >= sizeof(BMember<float>), "stage size");
         ensure(s->bmem, stage);
         HIPX(hipHostMalloc(&s->h_stage, stage, hipHostMallocDefault));

@@ -92,7 +92,7 @@ class RequestStats(C.Structure):
     _fields_ = [
         ("status", C.c_int), ("steps", C.c_int), ("newton_iters", C.c_longlong), ("pcg_iters", C.c_longlong),
         ("backtracks", C.c_longlong), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
-        ("prep_ms", C.c_double), ("device_bytes", C.c_double),
+        ("prep_ms", C.c_double), ("device_bytes", C.c_double), ("fp64_fallbacks", C.c_int),
     ]
 
     def as_dict(self):

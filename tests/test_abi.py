@@ -48,7 +48,7 @@ def test_server_declarations_and_option_checks():
 
 def test_struct_sizes_match_header():
     # layouts the ctypes mirror relies on (x86-64: 8-byte aligned doubles)
-    assert ctypes.sizeof(native.RequestStats) == 4 + 4 + 3 * 8 + 4 * 8
+    assert ctypes.sizeof(native.RequestStats) == 4 + 4 + 3 * 8 + 4 * 8 + 8  # + fp64_fallbacks and padding
     assert ctypes.sizeof(native.ServerStats) == 3 * 8 + 4 + 4 + 2 * 8 + 2 * 8
     assert ctypes.sizeof(native.ServerOptions) == 8 + 3 * 4 + 4 + ctypes.sizeof(native.Options)
 

@@ -144,3 +144,88 @@ def test_pokeflex_style_disk_contact_driver(orc):
     assert err <= 1e-6
     top = q.reshape(-1, 3)[:, 2].max()
     assert top < sc.X[:, 2].max()  # indented
+
+
+def test_stable_neohookean_cantilever_driver(orc):
+    """The structure of python/d02_cantilever.py:32-38, 60-108: a clamped
+    tet beam with "stableneohookean" per element, Crank-Nicolson,
+    minimize_newton, two substeps, through the py_sors drop-in."""
+    ps = load()
+    sc = mesh.beam(nx=10, ny=2, nz=2, E=2.6e5, nu=0.49, rho=1070.0, steps=15, dt=2e-3)
+    settings = ps.SimulationSettings()
+    settings.meshType = "Tetrahedron"
+    settings.timeSteppingScheme = "crank_nicolson"
+    settings.solverMethod = "minimize_newton"
+    settings.dt = sc.dt
+    settings.substeps = 2
+    energies = [set(["stableneohookean"]) for _ in range(sc.n_elems)]
+    params = [ps.Params(["density", "youngsModulus", "poissonsRatio"], [1070.0, 2.6e5, 0.49])
+              for _ in range(sc.n_elems)]
+    cparams = ps.Params(["neumannBCmask", "neumannBCvalue"], [sc.bcmask.astype(float), sc.bcvalue])
+    energy = ps.Energy(sc.X, sc.conn, settings, sc.grav, 0.0, energies, params, ["neumannBC"], cparams, [],
+                       ps.Params())
+    energy.set_initial_velocity(sc.v0.reshape(-1, 3))
+    solver = ps.Solver(settings)
+    o = orc.System.from_scene(sc, scheme=orc.CN, method=orc.NEWTON)
+    o.set_material(orc.MAT_SNH)
+    q = qo = sc.X.reshape(-1).copy()
+    for _ in range(sc.steps):
+        q = solver.step(q, energy)
+        qo, _ = o.step(qo, sc.dt, substeps=2)
+    err = np.abs(q - qo).max() / sc.bbox_diag
+    print("stableneohookean cantilever driver vs oracle", err, flush=True)
+    assert err <= 1e-6
+
+
+def test_hopping_leg_style_muscle_driver(orc):
+    """The structure of python/d05_hopping_leg.py:120-219, 331-360: hex
+    elements with {"neohookean", "softconMuscle"} in a band, two muscle groups
+    along +z ("muscleGroup", "softconStiffness", "softconDirection"), a ground
+    plane ("planeContact" over the surface vertices), minimize_SQP, and a
+    two-channel "softconMuscle" actuation ramp.  The device runs its penalty
+    contact with plain Newton (the muscle energy is not the potential of its
+    gradient); compared with the oracle's restatement of the same model."""
+    ps = load()
+    sc = mesh.beam(nx=3, ny=3, nz=10, lx=0.03, ly=0.03, lz=0.1, element="hex", E=5e5, nu=0.4, rho=1000.0,
+                   tip_speed=0.0, steps=20)
+    X = sc.X + np.array([0.0, 0.0, 1e-4])            # resting just above the ground plane z = 0
+    settings = ps.SimulationSettings()
+    settings.meshType = "Hexahedron"
+    settings.timeSteppingScheme = "backward_euler"
+    settings.solverMethod = "minimize_SQP"
+    settings.dt = 2e-3
+    cen = X[sc.conn].mean(axis=1)
+    energies, params, group = [], [], np.full(sc.n_elems, -1, np.int32)
+    for e in range(sc.n_elems):
+        p = ps.Params(["density", "youngsModulus", "poissonsRatio"], [1000.0, 5e5, 0.4])
+        es = set(["neohookean"])
+        if 0.02 < cen[e, 2] < 0.08:
+            es.add("softconMuscle")
+            group[e] = int(cen[e, 0] >= 0.015)
+            p.set_param("softconStiffness", 5e5)
+            p.set_param("softconDirection", [0.0, 0.0, 1.0])
+            p.set_param("muscleGroup", float(group[e]))
+        energies.append(es)
+        params.append(p)
+    surf = mesh.box_surface_nodes(X)
+    cparams = ps.Params(["positionVectorListPlanes", "normalVectorListPlanes", "surfaceVerticesIdx",
+                         "contactStiffness"], [[0.0, 0.0, 0.0], [0.0, 0.0, 1.0], surf.astype(float), [1e5]])
+    energy = ps.Energy(X, sc.conn, settings, sc.grav, 0.0, energies, params, ["planeContact"], cparams,
+                       ["vertexForce"], ps.Params())
+    solver = ps.Solver(settings)
+    o = orc.System(X, sc.conn, sc.youngs, sc.poisson, sc.density, grav=sc.grav, method=orc.NEWTON)
+    o.set_contact(planes=[((0.0, 0.0, 0.0), (0.0, 0.0, 1.0))], surface=surf, stiffness=1e5)
+    o.set_muscle(group, np.full(sc.n_elems, 5e5), np.tile([0.0, 0.0, 1.0], (sc.n_elems, 1)))
+    q = qo = X.reshape(-1).copy()
+    act = ps.Params(["softconMuscle", "vertexForce"], [np.zeros((2, 1)), np.zeros((sc.n_nodes, 3))])
+    for k in range(sc.steps):
+        a = np.array([[0.4 * min(1.0, k / 8.0)], [0.15 * min(1.0, k / 8.0)]])
+        act.set_param("softconMuscle", a)
+        o.set_muscle_activation(a.reshape(-1))
+        q = solver.step(q, energy, act, 0.0)
+        qo, _ = o.step(qo, settings.dt)
+    err = np.abs(q - qo).max() / sc.bbox_diag
+    moved = np.abs(qo - X.reshape(-1)).max()
+    print("hopping-leg-style muscle driver vs oracle", err, "max displacement", moved, flush=True)
+    assert moved > 1e-4
+    assert err <= 1e-6

@@ -135,3 +135,88 @@ def native_estimate(sc):
     rows = E * (4 * ed * 8 + 10 * ed * ed * 8 + ed * ed * 4) + 4 * nblk
     return (n * (19 * 8 + 4) + 72 * N + nblk * (9 * 8 + 4 + 9 * 4) + E * (ed * 4 + 16 + ed * ed) + rows
             + 8 * 9 * 2048 + (1 << 20))
+
+
+def test_served_stream_requests_match_cpu_oracle(orc):
+    """Requests 0, 2 and 5 of the seeded serving stream (seed 0; the stream's
+    generator with node counts drawn in [2k, 20k]: a 3,321-node beam, an
+    8,000-node cube and a 3,375-node cube, each with its own E and nu),
+    served together in mixed precision and checked against the FP64 CPU
+    oracle after 4 steps each: within 1e-6 of the bounding-box diagonal."""
+    reqs = []
+    for i in (0, 2, 5):
+        sc = mesh.serving_request(i, seed=0, n_lo=2000, n_hi=20000)
+        sc.steps = 4
+        reqs.append(sc)
+    say("served vs oracle", [(sc.name, sc.n_nodes) for sc in reqs])
+    srv = native.Server(max_active=4, prep_threads=3, pcg_chunk=16, **SOLVER)
+    ids = [srv.submit(sc, sc.steps, sc.dt) for sc in reqs]
+    out = [srv.wait(i, timeout=300.0) for i in ids]
+    srv.close()
+    for sc, (q, rs) in zip(reqs, out):
+        o = orc.System.from_scene(sc, method=orc.NEWTON_LS, nthreads=16)
+        qo = sc.X.reshape(-1).copy()
+        for _ in range(sc.steps):
+            qo, _ = o.step(qo, sc.dt, linear_solver=orc.PCG, project=1, pcg_rtol=1e-9, g_tol=1e-9)
+        err = np.abs(q.reshape(-1) - qo).max() / sc.bbox_diag
+        say(sc.name, sc.n_nodes, "served vs oracle", err, "moved", np.abs(qo - sc.X.reshape(-1)).max())
+        assert rs["status"] == native.OK and rs["steps"] == sc.steps
+        assert err <= 1e-6
+
+
+def test_serving_fp64_continuation_matches_alone():
+    """A mixed solve that reaches max_newton goes on from its iterate in FP64,
+    served as alone (sors_step), so the served result equals the stand-alone
+    one bit for bit and the request reports the fallback."""
+    # loose inner solves converge linearly (about 20x per Newton iteration):
+    # four mixed iterations leave the residual above g_tol, four more in FP64
+    # from that iterate bring it below
+    opts = dict(g_tol=1e-9, pcg_rtol=5e-2, max_newton=4)
+    reqs = [small_request(i, seed=13) for i in range(4)]
+    srv = native.Server(max_active=4, prep_threads=2, pcg_chunk=8, **opts)
+    ids = [srv.submit(sc, sc.steps, sc.dt) for sc in reqs]
+    res = []
+    for i in ids:
+        try:
+            res.append(srv.wait(i, timeout=120.0))
+        except native.SorsError as e:
+            res.append(e.code)
+    srv.close()
+    fallbacks = 0
+    for sc, got in zip(reqs, res):
+        r = native.Rollout.from_scene(sc, precision=native.MIXED, **opts)
+        code, nfb = native.OK, 0
+        try:
+            for _ in range(sc.steps):
+                nfb += r.step_resident(sc.dt).fp64_fallbacks
+        except native.SorsError as e:
+            code = e.code
+        ref = r.positions().reshape(-1, 3)
+        r.close()
+        if code != native.OK:
+            say(sc.name, "failed alone with", code, "served:", got if not isinstance(got, tuple) else "ok")
+            assert got == code
+            continue
+        q, rs = got
+        say(sc.name, "fallbacks served", rs["fp64_fallbacks"], "alone", nfb)
+        assert rs["fp64_fallbacks"] == nfb
+        assert np.array_equal(q, ref)
+        fallbacks += nfb
+    assert fallbacks > 0
+
+
+def test_serving_long_rollouts_with_lagged_refresh_match_alone():
+    """300-step served rollouts with lagged coarse refreshes (coarse_every = 4,
+    factorised on each rollout's side stream while other rollouts' kernels
+    run) equal their stand-alone runs bit for bit."""
+    coarse = dict(coarse_nodes=64, coarse_every=4)
+    reqs = [mesh.serving_request(i, seed=17, n_lo=800, n_hi=2500, steps_lo=300, steps_hi=300) for i in range(3)]
+    srv = native.Server(max_active=3, prep_threads=2, pcg_chunk=8, **SOLVER, **coarse)
+    ids = [srv.submit(sc, sc.steps, sc.dt) for sc in reqs]
+    out = [srv.wait(i, timeout=300.0) for i in ids]
+    srv.close()
+    for sc, (q, rs) in zip(reqs, out):
+        ref = alone(sc, **coarse)
+        say(sc.name, sc.n_nodes, sc.steps, "long served vs alone", np.abs(q - ref).max(), rs["pcg_iters"])
+        assert rs["status"] == native.OK and rs["steps"] == 300
+        assert np.array_equal(q, ref)
