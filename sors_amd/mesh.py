@@ -19,6 +19,7 @@ C++ and Python sides can reproduce them bit for bit.
 from __future__ import annotations
 
 import dataclasses
+import functools
 import math
 
 import numpy as np
@@ -226,19 +227,36 @@ def serving_request(index: int, seed: int = 0, n_lo: int = 2_000, n_hi: int = 20
     if kind == "beam":
         # (5k + 1)(k + 1)^2 nodes
         k = max(1, int(round((nodes / 5.0) ** (1.0 / 3.0))))
-        sc = beam(nx=5 * k, ny=k, nz=k, E=E, nu=nu, dt=dt, steps=steps, name=f"req{index}_beam")
-        return sc
-    k = max(1, int(round(nodes ** (1.0 / 3.0))) - 1)
-    L = 0.05
-    X, hexes = box_grid(k, k, k, L, L, L)
-    conn = kuhn_split(hexes).astype(np.int32)
-    N, nel = X.shape[0], conn.shape[0]
-    clamp = X[:, 2] < 1e-4 * L
-    v0 = np.zeros((N, 3))
-    v0[:, 0] = 0.5 * X[:, 2] / L
-    return Scene(name=f"req{index}_cube", X=X, conn=conn, youngs=np.full(nel, E), poisson=np.full(nel, nu),
-                 density=np.full(nel, 1100.0), bcmask=np.repeat(clamp.astype(np.int32), 3), bcvalue=np.zeros(3 * N),
-                 v0=v0.reshape(-1), grav=np.array([0.0, 0.0, -9.81]), dt=dt, steps=steps)
+        g = _request_geometry("beam", k)
+    else:
+        k = max(1, int(round(nodes ** (1.0 / 3.0))) - 1)
+        g = _request_geometry("cube", k)
+    X, conn, bcmask, v0 = g
+    nel = conn.shape[0]
+    return Scene(name=f"req{index}_{kind}", X=X, conn=conn, youngs=np.full(nel, E), poisson=np.full(nel, nu),
+                 density=np.full(nel, 1100.0), bcmask=bcmask, bcvalue=np.zeros(X.size), v0=v0,
+                 grav=np.array([0.0, 0.0, -9.81]), dt=dt, steps=steps)
+
+
+@functools.lru_cache(maxsize=None)
+def _request_geometry(kind: str, k: int):
+    """Mesh arrays of a serving request, shared (read-only) by every request
+    of the same kind and size: the beam of beam() with 5k x k x k cells, or
+    the 0.05 m cube with k^3 cells clamped at z = 0, v_x = 0.5 z / L."""
+    if kind == "beam":
+        sc = beam(nx=5 * k, ny=k, nz=k)
+        X, conn, bcmask, v0 = sc.X, sc.conn, sc.bcmask, sc.v0
+    else:
+        L = 0.05
+        X, hexes = box_grid(k, k, k, L, L, L)
+        conn = kuhn_split(hexes).astype(np.int32)
+        clamp = X[:, 2] < 1e-4 * L
+        v = np.zeros_like(X)
+        v[:, 0] = 0.5 * X[:, 2] / L
+        bcmask, v0 = np.repeat(clamp.astype(np.int32), 3), v.reshape(-1)
+    for a in (X, conn, bcmask, v0):
+        a.flags.writeable = False
+    return X, conn, bcmask, v0
 
 
 def serving_arrivals(count: int, rate: float = 200.0, seed: int = 0) -> np.ndarray:
