@@ -1,7 +1,8 @@
 // py_sors.cpp -- pybind11 module with the reference's Python surface
 // (cpp/py_sors.cpp:25-261) over the device path: Energy(...), EnergyTet/Hex,
-// Solver(settings), SolverTet/Hex.step, Params, SimulationSettings and the
-// structured-mesh IO statics, Plane3D and the DiskContact handle.  numpy arrays are copied in and out like the
+// Solver(settings), SolverTet/Hex.step, Params, SimulationSettings, the IO
+// statics (VTU / PVD, Gmsh readers, surfaces, grids), VisualType /
+// VisualOption, Plane3D and the DiskContact handle.  numpy arrays are copied in and out like the
 // reference's Eigen conversions; step() releases the GIL (an addition).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -11,6 +12,7 @@
 #include <cmath>
 
 #include "sors_facade.h"
+#include "sors_io.h"
 
 namespace py = pybind11;
 using namespace sors_host;
@@ -147,75 +149,27 @@ void bind_solver(py::module_& m, const char* name) {
         });
 }
 
-// IO::make_voxel_hex_grid (cpp/src/io.cpp:931-987): vid = i + NX*(j + NY*k)
-py::tuple make_voxel_hex_grid(int nx, int ny, int nz, double dx, double dy, double dz) {
-    if (nx < 1 || ny < 1 || nz < 1) throw std::runtime_error("nx, ny, nz must be >= 1");
-    if (dx <= 0.0 || dy <= 0.0 || dz <= 0.0) throw std::runtime_error("dx, dy, dz must be > 0");
-    const int NX = nx + 1, NY = ny + 1, NZ = nz + 1;
-    py::array_t<double> V({(py::ssize_t)NX * NY * NZ, (py::ssize_t)3});
-    auto v = V.mutable_unchecked<2>();
-    for (int k = 0; k < NZ; ++k)
-        for (int j = 0; j < NY; ++j)
-            for (int i = 0; i < NX; ++i) {
-                const int id = i + NX * (j + NY * k);
-                v(id, 0) = i * dx;
-                v(id, 1) = j * dy;
-                v(id, 2) = k * dz;
-            }
-    py::array_t<int> H({(py::ssize_t)nx * ny * nz, (py::ssize_t)8});
-    auto h = H.mutable_unchecked<2>();
-    int e = 0;
-    for (int k = 0; k < nz; ++k)
-        for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i, ++e) {
-                auto id = [&](int a, int b, int c) { return a + NX * (b + NY * c); };
-                const int c8[8] = {id(i, j, k),         id(i + 1, j, k),         id(i + 1, j + 1, k),
-                                   id(i, j + 1, k),     id(i, j, k + 1),         id(i + 1, j, k + 1),
-                                   id(i + 1, j + 1, k + 1), id(i, j + 1, k + 1)};
-                for (int t = 0; t < 8; ++t) h(e, t) = c8[t];
-            }
-    return py::make_tuple(V, H);
+std::vector<int> int_rows(const py::handle& obj, int cols, const char* what) {
+    auto a = py::array_t<int, py::array::c_style | py::array::forcecast>::ensure(obj);
+    if (!a || a.ndim() != 2 || a.shape(1) != cols)
+        throw std::invalid_argument(std::string(what) + " must be a 2-D integer array with " + std::to_string(cols) +
+                                    " columns");
+    return std::vector<int>(a.data(), a.data() + a.size());
 }
 
-// IO::hex_to_5_tets (cpp/src/io.cpp:899-928)
-py::array_t<int> hex_to_5_tets(const py::handle& hexes) {
-    auto a = py::array_t<int, py::array::c_style | py::array::forcecast>::ensure(hexes);
-    if (!a || a.ndim() != 2 || a.shape(1) != 8) throw std::invalid_argument("hexahedra must have 8 vertices");
-    static const int pat[5][4] = {{0, 1, 3, 4}, {1, 2, 3, 6}, {1, 4, 5, 6}, {3, 4, 6, 7}, {1, 3, 4, 6}};
-    const py::ssize_t nh = a.shape(0);
-    py::array_t<int> T({nh * 5, (py::ssize_t)4});
-    auto t = T.mutable_unchecked<2>();
-    auto r = a.unchecked<2>();
-    for (py::ssize_t j = 0; j < nh; ++j)
-        for (int s = 0; s < 5; ++s)
-            for (int c = 0; c < 4; ++c) t(5 * j + s, c) = r(j, pat[s][c]);
-    return T;
+py::array_t<int> int_array(const std::vector<int>& v, int cols) {
+    py::array_t<int> out({(py::ssize_t)(v.size() / cols), (py::ssize_t)cols});
+    std::copy(v.begin(), v.end(), out.mutable_data());
+    return out;
 }
 
-struct IO {};
+std::array<double, 3> vec3(const py::handle& v) {
+    auto a = to_vec(v);
+    if (a.size() != 3) throw std::invalid_argument("expected a 3-vector");
+    return {a[0], a[1], a[2]};
+}
 
-// Plane3D (cpp/headers/geometries/plane3D.h:15-89): a point and a unit normal
-struct Plane3D {
-    double p[3] = {0.0, 0.0, 0.0};
-    double n[3] = {0.0, 0.0, 1.0};
-    static std::array<double, 3> vec3(const py::handle& v) {
-        auto a = to_vec(v);
-        if (a.size() != 3) throw std::invalid_argument("expected a 3-vector");
-        return {a[0], a[1], a[2]};
-    }
-    void set_point(const py::handle& v) {
-        auto a = vec3(v);
-        std::copy(a.begin(), a.end(), p);
-    }
-    void set_normal(const py::handle& v) {  // plane3D.h:73-76 stores the normal as given
-        auto a = vec3(v);
-        std::copy(a.begin(), a.end(), n);
-    }
-    double distance(const py::handle& x) const {
-        auto a = vec3(x);
-        return n[0] * (a[0] - p[0]) + n[1] * (a[1] - p[1]) + n[2] * (a[2] - p[2]);
-    }
-};
+py::array_t<double> arr3(const std::array<double, 3>& a) { return from_vec({a[0], a[1], a[2]}); }
 
 template <int E>
 DiskContactHandle<E> disk_handle(Energy<E>& en, int idx) {
@@ -230,7 +184,7 @@ void bind_disk(py::module_& m, const char* name) {
         .def(
             "set_disk",
             [](DiskContactHandle<E>& h, const py::handle& c, const py::handle& nrm, double r, int idx) {
-                auto cc = Plane3D::vec3(c), nn = Plane3D::vec3(nrm);
+                auto cc = vec3(c), nn = vec3(nrm);
                 h.set_disk(cc.data(), nn.data(), r, idx);
             },
             py::arg("center"), py::arg("normal"), py::arg("radius"), py::arg("diskIdx") = 0);
@@ -307,39 +261,106 @@ PYBIND11_MODULE(py_sors, m) {
         throw std::runtime_error("Unknown mesh type: " + settings.meshType);
     });
 
-    // py_sors.cpp:216-227
+    // py_sors.cpp:201-227
+    py::enum_<VisualType>(m, "VisualType")
+        .value("VectorVertex", VisualType::VectorVertex)
+        .value("VectorElement", VisualType::VectorElement)
+        .value("ScalarVertex", VisualType::ScalarVertex)
+        .value("ScalarElement", VisualType::ScalarElement);
+    py::class_<VisualOption>(m, "VisualOption")
+        .def(py::init<>())
+        .def_readwrite("name", &VisualOption::name)
+        .def_readwrite("type", &VisualOption::type)
+        .def_property(
+            "data", [](const VisualOption& v) { return from_mat(v.data); },
+            [](VisualOption& v, const py::handle& d) { v.data = to_mat(d); })
+        .def_readwrite("threshold", &VisualOption::threshold);
     py::class_<Plane3D>(m, "Plane3D")
         .def(py::init<>())
         .def(py::init([](const py::handle& point, const py::handle& normal) {
-                 Plane3D pl;
-                 pl.set_point(point);
-                 auto a = Plane3D::vec3(normal);
-                 const double l = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-                 if (!(l > 0.0)) throw std::invalid_argument("plane normal must be nonzero");
-                 for (int k = 0; k < 3; ++k) pl.n[k] = a[k] / l;  // normal.normalized() (plane3D.h:21-22)
-                 return pl;
+                 return Plane3D(vec3(point), vec3(normal));
              }),
              py::arg("point"), py::arg("normal"))
-        .def("distance_to_point", [](const Plane3D& pl, const py::handle& x) { return pl.distance(x); })
-        .def("is_point_on_plane",
-             [](const Plane3D& pl, const py::handle& x) { return std::fabs(pl.distance(x)) < 1e-12; })
+        .def("is_point_on_plane", [](const Plane3D& pl, const py::handle& x) { return pl.is_point_on_plane(vec3(x)); })
+        .def("distance_to_point", [](const Plane3D& pl, const py::handle& x) { return pl.distance_to_point(vec3(x)); })
         .def("project_point_on_plane",
-             [](const Plane3D& pl, const py::handle& x) {
-                 auto a = Plane3D::vec3(x);
-                 const double d = pl.distance(x);
-                 return from_vec({a[0] - d * pl.n[0], a[1] - d * pl.n[1], a[2] - d * pl.n[2]});
-             })
-        .def("get_point", [](const Plane3D& pl) { return from_vec({pl.p[0], pl.p[1], pl.p[2]}); })
-        .def("set_point", &Plane3D::set_point)
-        .def("get_normal", [](const Plane3D& pl) { return from_vec({pl.n[0], pl.n[1], pl.n[2]}); })
-        .def("set_normal", &Plane3D::set_normal);
+             [](const Plane3D& pl, const py::handle& x) { return arr3(pl.project_point_on_plane(vec3(x))); })
+        .def("create_square_points", &Plane3D::create_square_points, py::arg("squareSide"))
+        .def("get_point", [](const Plane3D& pl) { return arr3(pl.get_point()); })
+        .def("set_point", [](Plane3D& pl, const py::handle& p) { pl.set_point(vec3(p)); })
+        .def("get_normal", [](const Plane3D& pl) { return arr3(pl.get_normal()); })
+        .def("set_normal", [](Plane3D& pl, const py::handle& n) { pl.set_normal(vec3(n)); })
+        .def("get_square_points", &Plane3D::get_square_points);
 
     bind_disk<4>(m, "DiskContact");
     bind_disk<8>(m, "DiskContactHex");
 
+    // py_sors.cpp:180-198 (read_*_MSH and make_voxel_hex_grid are additions:
+    // the reference exposes them to C++ only)
     py::class_<IO>(m, "IO")
-        .def_static("make_voxel_hex_grid", &make_voxel_hex_grid, py::arg("nx"), py::arg("ny"), py::arg("nz"),
-                    py::arg("dx"), py::arg("dy"), py::arg("dz"))
-        .def_static("hex_to_5_tets", &hex_to_5_tets, "Converts hexahedral elements to 5 tetrahedra.",
-                    py::arg("hexElements"));
+        .def_static(
+            "save_tet_VTU",
+            [](const std::string& f, const py::handle& V, const py::handle& T, const std::vector<VisualOption>& vis,
+               const std::vector<Plane3D>& planes) {
+                return IO::save_tet_VTU(f, to_mat(V), int_rows(T, 4, "elements"), vis, planes);
+            },
+            "Save tetrahedral mesh (vertices and elements) to VTU file.", py::arg("filename"), py::arg("vertices"),
+            py::arg("elements"), py::arg("visuals") = std::vector<VisualOption>(),
+            py::arg("planes") = std::vector<Plane3D>())
+        .def_static(
+            "save_hex_VTU",
+            [](const std::string& f, const py::handle& V, const py::handle& H, const std::vector<VisualOption>& vis,
+               const std::vector<Plane3D>& planes) {
+                return IO::save_hex_VTU(f, to_mat(V), int_rows(H, 8, "elements"), vis, planes);
+            },
+            "Save hexahedral mesh (vertices and elements) to VTU file.", py::arg("filename"), py::arg("vertices"),
+            py::arg("elements"), py::arg("visuals") = std::vector<VisualOption>(),
+            py::arg("planes") = std::vector<Plane3D>())
+        .def_static("save_pvd", &IO::save_pvd, "Save PVD file summarizing all VTUs.", py::arg("filename"),
+                    py::arg("outputFolder"), py::arg("timesteps") = 1, py::arg("dt") = 1.0)
+        .def_static(
+            "extract_tet_surfaces",
+            [](const py::handle& V, const py::handle& T, bool verbose) {
+                return IO::extract_tet_surfaces(to_mat(V), int_rows(T, 4, "eleIdx"), verbose);
+            },
+            "Extract surfaces from tetrahedral mesh.", py::arg("vertices"), py::arg("eleIdx"),
+            py::arg("verboseFlag") = false)
+        .def_static(
+            "extract_hex_surfaces",
+            [](const py::handle& V, const py::handle& H, bool verbose) {
+                return IO::extract_hex_surfaces(to_mat(V), int_rows(H, 8, "eleIdx"), verbose);
+            },
+            "Extract surfaces from hexahedral mesh.", py::arg("vertices"), py::arg("eleIdx"),
+            py::arg("verboseFlag") = false)
+        .def_static(
+            "read_tet_MSH",
+            [](const std::string& f, bool verbose) {
+                Mat V;
+                std::vector<int> T;
+                IO::read_tet_MSH(f, V, T, verbose);
+                return py::make_tuple(from_mat(V), int_array(T, 4));
+            },
+            "Read the vertices and tetrahedra of a Gmsh .msh file (2.2 / 4.1, ASCII / binary).", py::arg("filename"),
+            py::arg("verboseFlag") = false)
+        .def_static(
+            "read_hex_MSH",
+            [](const std::string& f, bool verbose) {
+                Mat V;
+                std::vector<int> H;
+                IO::read_hex_MSH(f, V, H, verbose);
+                return py::make_tuple(from_mat(V), int_array(H, 8));
+            },
+            "Read the vertices and hexahedra of a Gmsh .msh file (2.2 / 4.1, ASCII / binary).", py::arg("filename"),
+            py::arg("verboseFlag") = false)
+        .def_static(
+            "make_voxel_hex_grid",
+            [](int nx, int ny, int nz, double dx, double dy, double dz) {
+                auto g = IO::make_voxel_hex_grid(nx, ny, nz, dx, dy, dz);
+                return py::make_tuple(from_mat(g.first), int_array(g.second, 8));
+            },
+            py::arg("nx"), py::arg("ny"), py::arg("nz"), py::arg("dx"), py::arg("dy"), py::arg("dz"))
+        .def_static(
+            "hex_to_5_tets",
+            [](const py::handle& H) { return int_array(IO::hex_to_5_tets(int_rows(H, 8, "hexahedra")), 4); },
+            "Converts hexahedral elements to 5 tetrahedra.", py::arg("hexElements"));
 }
