@@ -1932,12 +1932,25 @@ __device__ __forceinline__ void sum_partials_multi(const PartIO& io, double (&t)
     __syncthreads();
 }
 
+// ctl->done as one value for the whole workgroup: the flag can be set by
+// another block of the same launch while this one runs, and the waves of a
+// workgroup must take the same early exit (a persistent launch goes on to
+// further virtual blocks, barriers included, after an exit)
+__device__ __forceinline__ bool block_done(const PcgCtl* ctl) {
+    __shared__ int flag;
+    if (threadIdx.x == 0) flag = ctl->done;
+    __syncthreads();
+    const bool d = flag != 0;
+    __syncthreads();
+    return d;
+}
+
 // SpMV prologue: returns true when the solve is over (converged, breakdown,
 // or a zero right-hand side).  Block 0 of the rollout records the outcome.
 __device__ __forceinline__ bool spmv_prologue(PcgCtl* ctl, PcgState* st, const double* partU, int stride, int nbU,
                                               int first, double rtol, bool lead, const double* partC = nullptr,
                                               int nbC = 0) {
-    if (ctl->done) return true;
+    if (block_done(ctl)) return true;
     double t[2];
     {
         const PartIO io{nullptr, stride, partU, nbU, nullptr, 0, partC, nbC};
@@ -2057,7 +2070,7 @@ __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgStat
             }
         }
     };
-    if (v.ctl->done) return;  // before the prefetch: launches after convergence cost no traffic
+    if (block_done(v.ctl)) return;  // before the prefetch: launches after convergence cost no traffic
     int base = lb * kUpdBlock;
     if (base < n) load(base);
     double g[2], d[1];
@@ -2258,7 +2271,7 @@ __global__ void k_coarse_symmetrize(int nc, const double* X, double* Einv) {
 constexpr int kRestrictThreads = SORS_RESTRICT_THREADS;
 template <typename HT>
 __device__ void coarse_restrict_body(const CoarseArgs& c, const PcgCtl* ctl, const HT* r, int I) {
-    if (ctl && ctl->done) return;
+    if (ctl && block_done(ctl)) return;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const int lo = c.aptr[I], hi = c.aptr[I + 1];
     for (int t = lo + threadIdx.x; t < hi; t += kRestrictThreads) {
@@ -2304,7 +2317,7 @@ static_assert(kMaxCoarse / 6 <= kMaxRedBlocks, "one gamma partial per aggregate"
 template <typename HT>
 __device__ void coarse_apply_body(const CoarseArgs& c, const PcgCtl* ctl, const double* __restrict__ Einv, HT* u,
                                   double* part, int I) {
-    if (ctl && ctl->done) return;
+    if (ctl && block_done(ctl)) return;
     __shared__ double xs[6];
     const int nc = c.nc;
     const double* __restrict__ rc = c.rc;
@@ -2412,10 +2425,23 @@ struct BMember {
     double* cpart;           // gamma_c partials, two parity slots
 };
 
-// j-th rollout of this launch owns launch blocks [tab[j], tab[j+1])
-__device__ __forceinline__ int find_member(const int* tab, int cnt) {
+// Packed launches: a batched kernel walks the virtual blocks of all rollouts
+// in the launch, vb = blockIdx.x, blockIdx.x + gridDim.x, ...; the j-th
+// rollout owns virtual blocks [tab[j], tab[j+1]), and the number of rollouts
+// is read from device memory (*cntp).  The grid is one workgroup per virtual
+// block up to a persistent cap (below).  A virtual block computes exactly
+// what block lb = vb - tab[j] of the rollout's own launch computes (same
+// partial slot, same summation order).  With the caps multiples of 8, virtual
+// block vb runs on XCD vb % 8 as it would in a grid of its own.
+constexpr int kPersistBlocks = 2048;        // 256-thread kernels: 8 per CU
+constexpr int kPersistUpdBlocks = 1536;     // 384-thread update: 6 per CU
+constexpr int kPersistCoarseBlocks = 1024;  // 512-thread coarse kernels: 4 per CU
+static_assert(kPersistBlocks % kNumXcd == 0 && kPersistUpdBlocks % kNumXcd == 0 &&
+                  kPersistCoarseBlocks % kNumXcd == 0,
+              "XCD-aligned persistent grids");
+
+__device__ __forceinline__ int find_member(const int* tab, int cnt, int b) {
     int lo = 0, hi = cnt - 1;
-    const int b = blockIdx.x;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (tab[mid] <= b)
@@ -2426,61 +2452,77 @@ __device__ __forceinline__ int find_member(const int* tab, int cnt) {
     return lo;
 }
 
+// calls body(j, lb, nb) for every virtual block of this launch owned by this
+// workgroup; the trailing barrier keeps one virtual block's LDS use apart
+// from the next one's
+template <typename F>
+__device__ __forceinline__ void for_virtual_blocks(const int* tab, const int* cntp, F&& body) {
+    const int cnt = *cntp;
+    const int total = cnt > 0 ? tab[cnt] : 0;
+    for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const int j = find_member(tab, cnt, vb);
+        body(j, vb - tab[j], tab[j + 1] - tab[j]);
+        __syncthreads();
+    }
+}
+
 template <typename HT>
 __global__ __launch_bounds__(kBlock) void k_b_pcg_init(const BMember<HT>* mem, const int* idx, const int* tab,
-                                                       int cnt) {
-    const int j = find_member(tab, cnt);
-    const BMember<HT>& m = mem[idx[j]];
-    const int lb = blockIdx.x - tab[j], nb = tab[j + 1] - tab[j];
-    if (lb == 0 && threadIdx.x == 0) {
-        m.v.ctl->done = 0;
-        m.v.ctl->breakdown = 0;
-        m.v.ctl->iter = 0;
-        m.v.ctl->rr = 0.0;
-    }
-    const PartIO io{m.part, kMaxRedBlocks, nullptr, 0, nullptr, 0};
-    pcg_init_body(m.v, m.rhs, -1.0, io, lb, nb);
+                                                       const int* cntp) {
+    for_virtual_blocks(tab, cntp, [&](int j, int lb, int nb) {
+        const BMember<HT>& m = mem[idx[j]];
+        if (lb == 0 && threadIdx.x == 0) {
+            m.v.ctl->done = 0;
+            m.v.ctl->breakdown = 0;
+            m.v.ctl->iter = 0;
+            m.v.ctl->rr = 0.0;
+        }
+        const PartIO io{m.part, kMaxRedBlocks, nullptr, 0, nullptr, 0};
+        pcg_init_body(m.v, m.rhs, -1.0, io, lb, nb);
+    });
 }
 
 // par: parity of the iteration; first: A_0 (consumes the init partials)
 template <typename HT>
-__global__ __launch_bounds__(kBlock) void k_b_pcg_spmv(const BMember<HT>* mem, const int* idx, const int* tab, int cnt,
-                                                       int par, int first) {
-    const int j = find_member(tab, cnt);
-    const BMember<HT>& m = mem[idx[j]];
-    const int lb = blockIdx.x - tab[j], nb = tab[j + 1] - tab[j];
-    const PartIO io{m.part + 4 * kMaxRedBlocks, kMaxRedBlocks, m.part + 2 * par * kMaxRedBlocks,
-                    first ? m.nbI : m.nbU, nullptr, 0, m.nbC ? m.cpart + par * kMaxRedBlocks : nullptr, m.nbC};
-    pcg_spmv_body(m.v, m.v.st + par, io, first, lb, nb);
+__global__ __launch_bounds__(kBlock) void k_b_pcg_spmv(const BMember<HT>* mem, const int* idx, const int* tab,
+                                                       const int* cntp, int par, int first) {
+    for_virtual_blocks(tab, cntp, [&](int j, int lb, int nb) {
+        const BMember<HT>& m = mem[idx[j]];
+        const PartIO io{m.part + 4 * kMaxRedBlocks, kMaxRedBlocks, m.part + 2 * par * kMaxRedBlocks,
+                        first ? m.nbI : m.nbU, nullptr, 0, m.nbC ? m.cpart + par * kMaxRedBlocks : nullptr, m.nbC};
+        pcg_spmv_body(m.v, m.v.st + par, io, first, lb, nb);
+    });
 }
 
-// coarse correction of the rollouts that have one (their aggregates are the launch blocks)
+// coarse correction of the rollouts that have one (their aggregates are the virtual blocks)
 template <typename HT>
 __global__ __launch_bounds__(kRestrictThreads) void k_b_coarse_restrict(const BMember<HT>* mem, const int* idx,
-                                                                        const int* tab, int cnt) {
-    const int j = find_member(tab, cnt);
-    const BMember<HT>& m = mem[idx[j]];
-    coarse_restrict_body(m.ca, m.v.ctl, (const HT*)m.v.r, blockIdx.x - tab[j]);
+                                                                        const int* tab, const int* cntp) {
+    for_virtual_blocks(tab, cntp, [&](int j, int lb, int) {
+        const BMember<HT>& m = mem[idx[j]];
+        coarse_restrict_body(m.ca, m.v.ctl, (const HT*)m.v.r, lb);
+    });
 }
 
 template <typename HT>
 __global__ __launch_bounds__(kApplyThreads) void k_b_coarse_apply(const BMember<HT>* mem, const int* idx,
-                                                                  const int* tab, int cnt, int par) {
-    const int j = find_member(tab, cnt);
-    const BMember<HT>& m = mem[idx[j]];
-    coarse_apply_body(m.ca, m.v.ctl, m.Einv, m.v.u, m.cpart + par * kMaxRedBlocks, blockIdx.x - tab[j]);
+                                                                  const int* tab, const int* cntp, int par) {
+    for_virtual_blocks(tab, cntp, [&](int j, int lb, int) {
+        const BMember<HT>& m = mem[idx[j]];
+        coarse_apply_body(m.ca, m.v.ctl, m.Einv, m.v.u, m.cpart + par * kMaxRedBlocks, lb);
+    });
 }
 
 template <typename HT>
 __global__ __launch_bounds__(kUpdBlock) void k_b_pcg_update(const BMember<HT>* mem, const int* idx, const int* tab,
-                                                            int cnt, int par, int first) {
-    const int j = find_member(tab, cnt);
-    const BMember<HT>& m = mem[idx[j]];
-    const int lb = blockIdx.x - tab[j], nb = tab[j + 1] - tab[j];
-    const PartIO io{m.part + 2 * par * kMaxRedBlocks, kMaxRedBlocks, m.part + 2 * (par ^ 1) * kMaxRedBlocks,
-                    first ? m.nbI : m.nbU, m.part + 4 * kMaxRedBlocks, m.nbA,
-                    m.nbC ? m.cpart + (par ^ 1) * kMaxRedBlocks : nullptr, m.nbC};
-    pcg_update_body(m.v, m.v.st + (par ^ 1), m.v.st + par, io, first, lb, nb);
+                                                            const int* cntp, int par, int first) {
+    for_virtual_blocks(tab, cntp, [&](int j, int lb, int nb) {
+        const BMember<HT>& m = mem[idx[j]];
+        const PartIO io{m.part + 2 * par * kMaxRedBlocks, kMaxRedBlocks, m.part + 2 * (par ^ 1) * kMaxRedBlocks,
+                        first ? m.nbI : m.nbU, m.part + 4 * kMaxRedBlocks, m.nbA,
+                        m.nbC ? m.cpart + (par ^ 1) * kMaxRedBlocks : nullptr, m.nbC};
+        pcg_update_body(m.v, m.v.st + (par ^ 1), m.v.st + par, io, first, lb, nb);
+    });
 }
 
 // ===========================================================================
@@ -3954,8 +3996,21 @@ void srv_prep_loop(sors_server* s) {
 }
 
 // launch tables staged per tick: member indices and block prefixes of the
-// starters (init, SpMV, update, coarse) and the runners (SpMV, update, coarse)
+// starters (init, SpMV, update, coarse) and the runners (SpMV, update,
+// coarse), M + 1 ints each, then the rollout counts of the four launch kinds
 constexpr int kStageTables = 11;
+enum { CNT_S = 0, CNT_SC, CNT_R, CNT_RC, kStageCounts };
+
+// bytes staged per tick at precision HT, and the bytes of one precision's
+// staging area (a multiple of 16, so the second area stays aligned)
+template <typename HT>
+size_t stage_bytes(int M) {
+    return sizeof(BMember<HT>) * M + sizeof(int) * ((size_t)kStageTables * (M + 1) + kStageCounts);
+}
+size_t stage_area(int M) {
+    static_assert(sizeof(BMember<double>) >= sizeof(BMember<float>), "FP64 member table is the larger");
+    return (stage_bytes<double>(M) + 15) / 16 * 16;
+}
 
 // the batched PCG chunk of the rollouts in their linear solve at precision HT
 // (starters begin their solve this tick)
@@ -3965,7 +4020,7 @@ void srv_batched_pcg(sors_server* s, const std::vector<int>& starters, const std
     {
         const int M = s->opt.max_active;
         // one staging area per precision: both may be in flight in one tick
-        const size_t area = sizeof(BMember<double>) * M + sizeof(int) * kStageTables * (M + 1);
+        const size_t area = stage_area(M);
         const int which = sizeof(HT) == 8 ? 1 : 0;
         BMember<HT>* hm = (BMember<HT>*)((char*)s->h_stage + which * area);
         int* ht = (int*)(hm + M);
@@ -4017,7 +4072,12 @@ void srv_batched_pcg(sors_server* s, const std::vector<int>& starters, const std
                 ++nrc;
             }
         }
-        const size_t stage = sizeof(BMember<HT>) * M + sizeof(int) * kStageTables * (M + 1);
+        int* cnt = ht + kStageTables * (M + 1);
+        cnt[CNT_S] = (int)starters.size();
+        cnt[CNT_SC] = nsc;
+        cnt[CNT_R] = (int)runners.size();
+        cnt[CNT_RC] = nrc;
+        const size_t stage = stage_bytes<HT>(M);
         char* dbase = (char*)s->bmem.p + which * area;
         HIPX(hipMemcpyAsync(dbase, hm, stage, hipMemcpyHostToDevice, s->stream));
         const BMember<HT>* dm = (const BMember<HT>*)dbase;
@@ -4026,31 +4086,40 @@ void srv_batched_pcg(sors_server* s, const std::vector<int>& starters, const std
         for (int k = 0; k < kStageTables; ++k) db[k] = dt_ + k * (M + 1);
         const int *dIdxS = db[0], *dIS = db[1], *dAS = db[2], *dUS = db[3], *dIdxCS = db[4], *dCS = db[5];
         const int *dIdxR = db[6], *dAR = db[7], *dUR = db[8], *dIdxCR = db[9], *dCR = db[10];
+        const int* dCnt = dt_ + kStageTables * (M + 1);
         const int ns = (int)starters.size(), nr = (int)runners.size();
+        hipStream_t st = s->stream;
+        // one workgroup per virtual block up to the persistent cap; beyond
+        // it the workgroups walk the remaining virtual blocks
+        auto grid = [](int total, int cap) { return std::min(total, cap); };
         // u += Z E^-1 Z^T r after an update (or the init) of the rollouts with a coarse correction
-        auto coarse = [&](const int* didx, const int* dtab, int cnt, int total, int par) {
-            if (!cnt) return;
-            k_b_coarse_restrict<HT><<<total, kRestrictThreads, 0, s->stream>>>(dm, didx, dtab, cnt);
-            k_b_coarse_apply<HT><<<total, kApplyThreads, 0, s->stream>>>(dm, didx, dtab, cnt, par);
+        auto coarse = [&](const int* didx, const int* dtab, const int* dcnt, int total, int par) {
+            if (!total) return;
+            const int gc = grid(total, kPersistCoarseBlocks);
+            k_b_coarse_restrict<HT><<<gc, kRestrictThreads, 0, st>>>(dm, didx, dtab, dcnt);
+            k_b_coarse_apply<HT><<<gc, kApplyThreads, 0, st>>>(dm, didx, dtab, dcnt, par);
             s->launches += 2;
         };
         if (ns) {
-            for (int i : starters) HIPX(hipStreamWaitEvent(s->stream, s->active[i].c->evPre, 0));
-            k_b_pcg_init<HT><<<tIS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dIS, ns);
-            coarse(dIdxCS, dCS, nsc, tCS[nsc], 0);
-            k_b_pcg_spmv<HT><<<tAS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dAS, ns, 0, 1);     // A_0
-            k_b_pcg_update<HT><<<tUS[ns], kUpdBlock, 0, s->stream>>>(dm, dIdxS, dUS, ns, 1, 1);  // U_1
-            coarse(dIdxCS, dCS, nsc, tCS[nsc], 1);
-            k_b_pcg_spmv<HT><<<tAS[ns], kBlock, 0, s->stream>>>(dm, dIdxS, dAS, ns, 1, 0);     // A_1
+            for (int i : starters) HIPX(hipStreamWaitEvent(st, s->active[i].c->evPre, 0));
+            const int gi = grid(tIS[ns], kPersistBlocks), ga = grid(tAS[ns], kPersistBlocks);
+            const int gu = grid(tUS[ns], kPersistUpdBlocks);
+            k_b_pcg_init<HT><<<gi, kBlock, 0, st>>>(dm, dIdxS, dIS, dCnt + CNT_S);
+            coarse(dIdxCS, dCS, dCnt + CNT_SC, tCS[nsc], 0);
+            k_b_pcg_spmv<HT><<<ga, kBlock, 0, st>>>(dm, dIdxS, dAS, dCnt + CNT_S, 0, 1);     // A_0
+            k_b_pcg_update<HT><<<gu, kUpdBlock, 0, st>>>(dm, dIdxS, dUS, dCnt + CNT_S, 1, 1);  // U_1
+            coarse(dIdxCS, dCS, dCnt + CNT_SC, tCS[nsc], 1);
+            k_b_pcg_spmv<HT><<<ga, kBlock, 0, st>>>(dm, dIdxS, dAS, dCnt + CNT_S, 1, 0);     // A_1
             s->launches += 4;
         }
         // iterations k >= 2 of every rollout in its solve: every rollout has
         // completed an odd number of iterations, so parity follows j
+        const int ga = grid(tAR[nr], kPersistBlocks), gu = grid(tUR[nr], kPersistUpdBlocks);
         for (int j = 0; j < chunk; ++j) {
             const int par = j & 1;
-            k_b_pcg_update<HT><<<tUR[nr], kUpdBlock, 0, s->stream>>>(dm, dIdxR, dUR, nr, par, 0);
-            coarse(dIdxCR, dCR, nrc, tCR[nrc], par);
-            k_b_pcg_spmv<HT><<<tAR[nr], kBlock, 0, s->stream>>>(dm, dIdxR, dAR, nr, par, 0);
+            k_b_pcg_update<HT><<<gu, kUpdBlock, 0, st>>>(dm, dIdxR, dUR, dCnt + CNT_R, par, 0);
+            coarse(dIdxCR, dCR, dCnt + CNT_RC, tCR[nrc], par);
+            k_b_pcg_spmv<HT><<<ga, kBlock, 0, st>>>(dm, dIdxR, dAR, dCnt + CNT_R, par, 0);
         }
         s->launches += 2 * chunk;
         HIPX(hipGetLastError());
@@ -5217,7 +5286,7 @@ int sors_server_create(const sors_server_options* o, sors_server** out) {
         ensure(s->slots, sizeof(SlotRec) * M);
         HIPX(hipMemset(s->slots.p, 0, sizeof(SlotRec) * M));
         // two staging areas (FP32 and FP64 solves may both run in one tick)
-        const size_t stage = 2 * (sizeof(BMember<double>) * M + sizeof(int) * kStageTables * (M + 1));
+        const size_t stage = 2 * stage_area(M);
         static_assert(sizeof(BMember<double>) >= sizeof(BMember<float>), "stage size");
         ensure(s->bmem, stage);
         HIPX(hipHostMalloc(&s->h_stage, stage, hipHostMallocDefault));
