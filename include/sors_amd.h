@@ -215,6 +215,14 @@ int sors_element_eval(sors_ctx* ctx, const double* q, int precision, int project
 int sors_linear_solve(sors_ctx* ctx, const double* q, double dt, const double* b, double* x, double rtol,
                       int max_iter, int* iters);
 
+/* The two-level preconditioner's coarse operator (no reference counterpart:
+ * it replaces the reference's direct factorisation, solver.cpp:82-84, by a
+ * PCG whose coarse correction needs E = Z^T A Z and E^-1).  nc receives the
+ * coarse size (0 when no coarse correction is in use); E (nc x nc, the last
+ * refresh's Galerkin matrix, row-major) and Einv (the inverse in use) are
+ * copied when not NULL.  With coarse_every = 1 the two belong together. */
+int sors_get_coarse(sors_ctx* ctx, int* nc, double* E, double* Einv);
+
 /* ---- kernel micro-benchmarks (profiling runs) ------------------------ */
 enum { SORS_KERNEL_SPMV = 0, SORS_KERNEL_ASSEMBLE = 1, SORS_KERNEL_COPY = 2, SORS_KERNEL_PCG_UPDATE = 3 };
 /* Launch one kernel `reps` times back to back on the context's stream, on the

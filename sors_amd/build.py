@@ -56,9 +56,8 @@ def build_lib() -> str:
     hip_o = os.path.join(BUILD, "sors_amd.o")
     _run([HIPCC, *HIP_FLAGS, "-I", INC, "-I", CSRC, "-c", os.path.join(CSRC, "sors_amd.hip"), "-o", hip_o])
     tmp = LIB + ".tmp"
-    # rocSOLVER (Cholesky of the coarse operator) and rocBLAS (its handle)
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_o, *host_o, "-pthread",
-          "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+          "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -15,8 +15,6 @@
 // with SPD-projected element tangents.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -2244,23 +2242,148 @@ __global__ void k_coarse_gather(int npairs, int nc, const int* pairI, const int*
     E[(size_t)(6 * I + r) * nc + 6 * J + col] = v;
 }
 
-// X = identity (the right-hand sides of the coarse inverse)
-__global__ void k_coarse_eye(int nc, double* X) {
+// ---- coarse inverse: blocked Gauss-Jordan on the SPD matrix E -------------
+// E (nc x nc) is inverted in place of a factorisation library: blocked
+// Gauss-Jordan elimination without pivoting (E is SPD, so every pivot block,
+// a Schur complement of E, is SPD too), one kernel per pivot block of
+// kGjPivot rows on the whole padded matrix, ping-ponging between two
+// buffers.  Every output value is formed by one thread in a fixed order, so
+// the inverse is the same bit for bit whatever else runs on the card.
+// Cost: 2 npad^3 FP64 flops and npad / kGjPivot launches.
+constexpr int kGjTile = 64;     // output tile of one workgroup (npad is a multiple)
+constexpr int kGjPivot = 32;    // pivot block
+constexpr int kGjThreads = 256; // 16 x 16 threads, 4 x 4 outputs each
+static_assert(kGjTile % kGjPivot == 0, "pivot blocks inside output tiles");
+
+// X = E padded to npad with the identity, E read from its lower triangle (the
+// matrix a Cholesky factorisation of E would see)
+__global__ void k_coarse_pad(int nc, int npad, const double* __restrict__ E, double* __restrict__ X) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)nc * nc) return;
-    X[t] = (t / nc == t % nc) ? 1.0 : 0.0;
+    if (t >= (long long)npad * npad) return;
+    const int i = (int)(t / npad), j = (int)(t % npad);
+    double v = (i == j) ? 1.0 : 0.0;
+    if (i < nc && j < nc) v = i >= j ? E[(size_t)i * nc + j] : E[(size_t)j * nc + i];
+    X[t] = v;
 }
 
-// E^-1 = (X + X^T)/2 from X = E^-1 (potrs, symmetric up to rounding): an
-// exactly symmetric inverse, so Z E^-1 Z^T is a symmetric preconditioner.
-// The coarse solve stays in FP64 at every precision: E is as badly
-// conditioned as the coarse problem, and an FP32 copy of its inverse need not
-// be positive definite.
-__global__ void k_coarse_symmetrize(int nc, const double* X, double* Einv) {
+// One elimination step with pivot rows / columns K = [k0, k0 + kGjPivot):
+//   out[K, K] = P^-1                      (P = in[K, K])
+//   out[K, j] = P^-1 in[K, j]             (j not in K)
+//   out[i, K] = -in[i, K] P^-1            (i not in K)
+//   out[i, j] = in[i, j] - in[i, K] P^-1 in[K, j]
+// Each workgroup inverts P itself (Gauss-Jordan on the 32 x 32 block in LDS)
+// and forms one 64 x 64 output tile.  A non-positive pivot sets *info.
+__global__ __launch_bounds__(kGjThreads) void k_spd_gj_step(int npad, int k0, const double* __restrict__ in,
+                                                            double* __restrict__ out, int* info) {
+    constexpr int NB = kGjPivot, TB = kGjTile;
+    __shared__ double P[NB][NB + 1];   // pivot block, then its inverse
+    __shared__ double B[NB][TB + 1];   // in[K, J], then P^-1 in[K, J] (P^-1 on the K columns)
+    __shared__ double A[TB][NB + 1];   // in[I, K]
+    const int tiles = npad / TB;
+    const int ti = blockIdx.x / tiles, tj = blockIdx.x % tiles;
+    const int i0 = ti * TB, j0 = tj * TB, tid = threadIdx.x;
+    for (int e = tid; e < NB * NB; e += kGjThreads) P[e / NB][e % NB] = in[(size_t)(k0 + e / NB) * npad + k0 + e % NB];
+    for (int e = tid; e < NB * TB; e += kGjThreads) B[e / TB][e % TB] = in[(size_t)(k0 + e / TB) * npad + j0 + e % TB];
+    for (int e = tid; e < TB * NB; e += kGjThreads) A[e / NB][e % NB] = in[(size_t)(i0 + e / NB) * npad + k0 + e % NB];
+    __syncthreads();
+    // P^-1 in place: new[r][c] from the old values of step t
+    bool bad = false;
+    for (int t = 0; t < NB; ++t) {
+        const double piv = P[t][t];
+        bad = bad || !(piv > 0.0);
+        const double rp = 1.0 / piv;
+        double v[NB * NB / kGjThreads];
+#pragma unroll
+        for (int q = 0; q < NB * NB / kGjThreads; ++q) {
+            const int e = tid + q * kGjThreads, r = e / NB, c = e % NB;
+            const double prt = P[r][t], ptc = P[t][c];
+            if (r == t)
+                v[q] = c == t ? rp : ptc * rp;
+            else
+                v[q] = c == t ? -prt * rp : P[r][c] - (prt * ptc) * rp;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NB * NB / kGjThreads; ++q) {
+            const int e = tid + q * kGjThreads;
+            P[e / NB][e % NB] = v[q];
+        }
+        __syncthreads();
+    }
+    if (bad && blockIdx.x == 0 && tid == 0) *info = 1;
+    // B = P^-1 in[K, J]; on the columns of K, P^-1 itself
+    {
+        constexpr int kPer = NB * TB / kGjThreads;
+        double v[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kGjThreads, r = e / TB, c = e % TB;
+            const int j = j0 + c;
+            if (j >= k0 && j < k0 + NB) {
+                v[q] = P[r][j - k0];
+            } else {
+                double acc = 0.0;
+#pragma unroll 8
+                for (int u = 0; u < NB; ++u) acc += P[r][u] * B[u][c];
+                v[q] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kGjThreads;
+            B[e / TB][e % TB] = v[q];
+        }
+        __syncthreads();
+    }
+    // the tile: rows of K take B; the others in[i, j] (0 on the K columns) - A B
+    const int tx = tid % 16, ty = tid / 16;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+#pragma unroll 4
+    for (int t = 0; t < NB; ++t) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) av[a] = A[ty + 16 * a][t];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bv[b] = B[t][tx + 16 * b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int r = ty + 16 * a, i = i0 + r;
+        const bool rowK = i >= k0 && i < k0 + NB;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int c = tx + 16 * b, j = j0 + c;
+            double v;
+            if (rowK) {
+                v = B[i - k0][c];
+            } else {
+                const double base = (j >= k0 && j < k0 + NB) ? 0.0 : in[(size_t)i * npad + j];
+                v = base - acc[a][b];
+            }
+            out[(size_t)i * npad + j] = v;
+        }
+    }
+}
+
+// E^-1 = (X + X^T)/2 from X = E^-1 (ld: X's row stride), symmetric up to
+// rounding: an exactly symmetric inverse, so Z E^-1 Z^T is a symmetric
+// preconditioner.  The coarse solve stays in FP64 at every precision: E is
+// as badly conditioned as the coarse problem, and an FP32 copy of its
+// inverse need not be positive definite.
+__global__ void k_coarse_symmetrize(int nc, int ld, const double* X, double* Einv) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nc * nc) return;
     const int a = (int)(t / nc), b = (int)(t % nc);
-    Einv[t] = 0.5 * (X[(size_t)a * nc + b] + X[(size_t)b * nc + a]);
+    Einv[t] = 0.5 * (X[(size_t)a * ld + b] + X[(size_t)b * ld + a]);
 }
 
 // rc = Z^T r: one workgroup per aggregate, its nodes in a fixed order, a
@@ -2718,16 +2841,17 @@ struct sors_ctx {
     int coarse_built_for = 0;       // coarse_nodes of the built space (0: none)
     int coarse_gen = 0;             // bumped on every rebuild (captured graphs hold its buffers)
     long long coarse_tangents = 0;  // tangents seen since the coarse space was built
-    hipStream_t side = nullptr;     // rocSOLVER work (default priority, like the main stream)
+    hipStream_t side = nullptr;     // coarse inverse (default priority, like the main stream)
     hipEvent_t evG = nullptr, evR = nullptr;  // E assembled / factorisation done
     hipEvent_t evPre = nullptr;     // served rollouts: preconditioner built (the batched PCG waits)
     bool refresh_pending = false;   // a factorisation on the side stream not yet collected
     const void* coarse_pat = nullptr;
     bool coarse_ready = false;      // E^-1 valid for the current matrix
     int nagg = 0, nc = 0, npairs = 0, ngchunks = 0;
+    int npad = 0;                   // nc rounded up to the Gauss-Jordan tile
     DevMem cdn, captr, canodes, crot, cpairI, cpairJ, cpchunk, cgptr, cci, ccj, ccbase;
-    DevMem cE, cEinv, cX, crc, cgpart, cpart, cinfo;
-    rocblas_handle blas = nullptr;
+    DevMem cE, cEinv, cX, cY, crc, cgpart, cpart, cinfo;   // cX / cY: Gauss-Jordan ping-pong (npad^2)
+    const double* cres = nullptr;   // the buffer holding E^-1 after the last elimination
     bool vold_valid = false;
     double vold_dt = 0.0;
     // active fibre stress (sors_set_active) and parameter adjoints
@@ -2804,6 +2928,15 @@ void upload(DevMem& d, const T* h, size_t count, hipStream_t st) {
     if (count) HIPX(hipMemcpyAsync(d.p, h, sizeof(T) * count, hipMemcpyHostToDevice, st));
 }
 
+// a host vector, which may be empty: at least one element is allocated so
+// that the device pointer is valid, and only the vector's own elements are copied
+template <typename T>
+void upload_vec(DevMem& d, const std::vector<T>& v, hipStream_t st) {
+    if (!d.alloc(sizeof(T) * std::max<size_t>(v.size(), 1)))
+        throw HipError{"hipMalloc failed (" + std::to_string(sizeof(T) * v.size()) + " bytes)"};
+    if (!v.empty()) HIPX(hipMemcpyAsync(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+}
+
 void ensure(DevMem& d, size_t bytes) {
     if (!d.alloc(bytes)) throw HipError{"hipMalloc failed (" + std::to_string(bytes) + " bytes)"};
 }
@@ -2836,8 +2969,8 @@ void ensure_pattern(sors_ctx* c, double dt) {
             upload(c->incptr, c->pat.incptr.data(), c->pat.incptr.size(), c->stream);
             upload(c->inc, c->pat.inc.data(), c->pat.inc.size(), c->stream);
             upload(c->tsptr, c->pat.sptr.data(), c->pat.sptr.size(), c->stream);
-            upload(c->tsidx, c->pat.sidx.data(), std::max<size_t>(c->pat.sidx.size(), 1), c->stream);
-            upload(c->tqslice, c->pat.qslice.data(), std::max<size_t>(c->pat.qslice.size(), 1), c->stream);
+            upload_vec(c->tsidx, c->pat.sidx, c->stream);
+            upload_vec(c->tqslice, c->pat.qslice, c->stream);
         }
     } else {
         upload(c->fullmap, c->pat.fullmap.data(), c->pat.fullmap.size(), c->stream);
@@ -2928,10 +3061,12 @@ void contact_lag(sors_ctx* c) {
     HIPX(hipGetLastError());
 }
 
-// tet4 / hex8 Neo-Hookean on the sliced layout: the two-phase assembly writes
-// every gradient entry and every value of the layout itself (no fills, no atomics)
+// tet4 / hex8 Neo-Hookean or Stable Neo-Hookean on the sliced layout: the
+// two-phase assembly writes every gradient entry and every value of the
+// layout itself (no fills, no atomics)
 bool node_assembly(const sors_ctx* c) {
-    return c->block() && (c->ed == 4 || c->ed == 8) && c->material == SORS_NEOHOOKEAN && !c->pat.sptr.empty();
+    return c->block() && (c->ed == 4 || c->ed == 8) &&
+           (c->material == SORS_NEOHOOKEAN || c->material == SORS_STABLE_NEOHOOKEAN) && !c->pat.sptr.empty();
 }
 
 // element rows between the assembly phases (hbytes: bytes per Hessian value, 0: gradient only)
@@ -3082,12 +3217,19 @@ void ensure_coarse(sors_ctx* c) {
         if (6 * C.nagg <= kMaxCoarse) break;
         target *= 2;  // keep the dense coarse inverse bounded
     }
+    if (C.nagg == 0) {
+        // every node fixed: nothing to correct, block-Jacobi alone
+        c->nagg = c->nc = c->npad = 0;
+        c->coarse_built_for = want;
+        c->coarse_pat = c->vals.p;
+        return;
+    }
     c->nagg = C.nagg;
     c->nc = 6 * C.nagg;
     c->npairs = (int)C.pairI.size();
     c->ngchunks = (int)C.gptr.size() - 1;
     hipStream_t st = c->stream;
-    auto up = [&](DevMem& m, const auto& v) { upload(m, v.data(), std::max<size_t>(v.size(), 1), st); };
+    auto up = [&](DevMem& m, const auto& v) { upload_vec(m, v, st); };
     up(c->cdn, C.dn);
     up(c->canodes, C.anodes);
     up(c->captr, C.aptr);
@@ -3100,9 +3242,12 @@ void ensure_coarse(sors_ctx* c) {
     up(c->ccj, C.cj);
     up(c->ccbase, C.cbase);
     const size_t nc2 = (size_t)std::max(c->nc, 1) * std::max(c->nc, 1);
+    c->npad = (std::max(c->nc, 1) + kGjTile - 1) / kGjTile * kGjTile;
+    const size_t np2 = (size_t)c->npad * c->npad;
     ensure(c->cE, sizeof(double) * nc2);
     ensure(c->cEinv, sizeof(double) * nc2);
-    ensure(c->cX, sizeof(double) * nc2);
+    ensure(c->cX, sizeof(double) * np2);
+    ensure(c->cY, sizeof(double) * np2);
     ensure(c->crc, sizeof(double) * std::max(c->nc, 1));
     ensure(c->cgpart, sizeof(double) * 36 * std::max(c->ngchunks, 1));
     ensure(c->cpart, sizeof(double) * 2 * kMaxRedBlocks);
@@ -3112,15 +3257,6 @@ void ensure_coarse(sors_ctx* c) {
         HIPX(hipEventCreateWithFlags(&c->evG, hipEventDisableTiming));
         HIPX(hipEventCreateWithFlags(&c->evR, hipEventDisableTiming));
     }
-    if (!c->blas) {
-        if (rocblas_create_handle(&c->blas) != rocblas_status_success) throw HipError{"rocblas_create_handle failed"};
-        // no atomic accumulation inside rocBLAS / rocSOLVER: the coarse
-        // inverse, and so every trajectory, is the same bit for bit from run
-        // to run and between a served and a stand-alone rollout
-        if (rocblas_set_atomics_mode(c->blas, rocblas_atomics_not_allowed) != rocblas_status_success)
-            throw HipError{"rocblas_set_atomics_mode failed"};
-    }
-    if (rocblas_set_stream(c->blas, c->side) != rocblas_status_success) throw HipError{"rocblas_set_stream failed"};
     sync(c);
     c->coarse_built_for = want;
     c->coarse_pat = c->vals.p;
@@ -3128,22 +3264,25 @@ void ensure_coarse(sors_ctx* c) {
     ++c->coarse_gen;
 }
 
-// rocSOLVER work runs on the context's second stream (default priority):
-// factorisation and triangular solves, X = E^-1 in cX when it ends (evR)
+// the coarse inverse runs on the context's second stream (default priority):
+// E padded, then npad / kGjPivot Gauss-Jordan steps; E^-1 in c->cres and the
+// pivot check in cinfo when it ends (evR)
 void coarse_factor_async(sors_ctx* c) {
-    const int nc = c->nc;
-    const size_t nc2 = (size_t)nc * nc;
+    const int np = c->npad;
+    const long long np2 = (long long)np * np;
     HIPX(hipEventRecord(c->evG, c->stream));  // E assembled
     HIPX(hipStreamWaitEvent(c->side, c->evG, 0));
-    if (rocsolver_dpotrf(c->blas, rocblas_fill_lower, nc, c->cE.as<double>(), nc, c->cinfo.as<int>()) !=
-        rocblas_status_success)
-        throw HipError{"rocsolver_dpotrf failed"};
-    // E X = I with the Cholesky factor: two GEMM-rich triangular solves with nc right-hand sides
-    const unsigned gsq = (unsigned)((nc2 + kBlock - 1) / kBlock);
-    k_coarse_eye<<<gsq, kBlock, 0, c->side>>>(nc, c->cX.as<double>());
-    if (rocsolver_dpotrs(c->blas, rocblas_fill_lower, nc, nc, c->cE.as<double>(), nc, c->cX.as<double>(), nc) !=
-        rocblas_status_success)
-        throw HipError{"rocsolver_dpotrs failed"};
+    fill<int>(c->cinfo.as<int>(), 1, 0, c->side);
+    k_coarse_pad<<<(unsigned)((np2 + kBlock - 1) / kBlock), kBlock, 0, c->side>>>(c->nc, np, c->cE.as<double>(),
+                                                                                   c->cX.as<double>());
+    double* in = c->cX.as<double>();
+    double* out = c->cY.as<double>();
+    const unsigned tiles = (unsigned)(np / kGjTile);
+    for (int k0 = 0; k0 < np; k0 += kGjPivot) {
+        k_spd_gj_step<<<tiles * tiles, kGjThreads, 0, c->side>>>(np, k0, in, out, c->cinfo.as<int>());
+        std::swap(in, out);
+    }
+    c->cres = in;
     HIPX(hipGetLastError());
     HIPX(hipEventRecord(c->evR, c->side));
     c->refresh_pending = true;
@@ -3161,16 +3300,16 @@ void coarse_collect(sors_ctx* c) {
     HIPX(hipStreamWaitEvent(c->stream, c->evR, 0));
     const size_t nc2 = (size_t)c->nc * c->nc;
     k_coarse_symmetrize<<<(unsigned)((nc2 + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(
-        c->nc, c->cX.as<double>(), c->cEinv.as<double>());
+        c->nc, c->npad, c->cres, c->cEinv.as<double>());
     HIPX(hipGetLastError());
     c->coarse_ready = true;
 }
 
-// E = Z^T A Z and its inverse (rocSOLVER Cholesky factorisation and solves).
-// The first refresh, and every refresh when coarse_every <= 1, completes
-// before the solve.  Later refreshes (coarse_every > 1) factorise on the
-// second stream before the solve, which runs with the previous inverse and
-// leaves the host free; the next tangent installs the new one.
+// E = Z^T A Z and its inverse (blocked Gauss-Jordan).  The first refresh,
+// and every refresh when coarse_every <= 1, completes before the solve.
+// Later refreshes (coarse_every > 1) invert on the second stream while the
+// solve runs with the previous inverse; the next tangent installs the new
+// one.
 template <typename HT>
 void coarse_setup(sors_ctx* c) {
     ensure_coarse(c);
@@ -3195,14 +3334,7 @@ void coarse_setup(sors_ctx* c) {
     }
     HIPX(hipGetLastError());
     coarse_factor_async(c);
-    if (c->coarse_ready && lagged) {
-        // the solve keeps the previous inverse, but its kernels start only
-        // after the factorisation: repeated rollouts whose factorisation ran
-        // beside the solve differed in the last bits (500-step contact
-        // rollouts), serialised ones did not
-        HIPX(hipStreamWaitEvent(c->stream, c->evR, 0));
-        return;
-    }
+    if (c->coarse_ready && lagged) return;  // the solve keeps the previous inverse
     c->coarse_ready = false;
     coarse_collect(c);  // the solve waits for this one
 }
@@ -3800,7 +3932,7 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
                            &c->incptr, &c->inc,  &c->teg,      &c->teb,    &c->tsptr,   &c->tsidx,   &c->tqslice, &c->vold,     &c->cdn,    &c->captr,   &c->canodes, &c->crot,
                            &c->cpairI, &c->cpairJ, &c->cpchunk, &c->cgptr, &c->cci,     &c->ccj,     &c->ccbase,
-                           &c->cE,     &c->cEinv, &c->crc,     &c->cX,      &c->cgpart,  &c->cpart,   &c->cinfo,
+                           &c->cE,     &c->cEinv, &c->crc,     &c->cX,      &c->cY,      &c->cgpart,  &c->cpart,   &c->cinfo,
                            &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA,   &c->mgroup, &c->mdir,
                            &c->mk,     &c->mact};
     double b = 0;
@@ -3812,13 +3944,14 @@ double ctx_bytes(const sors_ctx* c) {
 // node-block slices padded to 24 (tet) / 32 (hex) blocks per node)
 double estimate_bytes(int N, int E, int ed, int coarse_nodes) {
     const double n = 3.0 * N, nblk = (double)N * (ed == 4 ? 24 : 32);
-    // coarse correction: E, its inverse and the potrs right-hand sides (nc^2
-    // FP64 each, nc = 6 per aggregate, aggregates at most 2N / coarse_nodes
-    // and 512), per-node data, and the rocBLAS handle's workspace
+    // coarse correction: E and its inverse (nc^2 FP64 each, nc = 6 per
+    // aggregate, aggregates at most 2N / coarse_nodes and 512), the two
+    // Gauss-Jordan buffers (npad^2 FP64 each) and per-node data
     double coarse = 0.0;
     if (coarse_nodes > 0) {
         const double nc = 6.0 * std::min(512.0, std::ceil(2.0 * N / coarse_nodes));
-        coarse = 3.0 * 8.0 * nc * nc + 24.0 * N + 64.0 * (1 << 20);
+        const double np = std::ceil(nc / kGjTile) * kGjTile;
+        coarse = 2.0 * 8.0 * nc * nc + 2.0 * 8.0 * np * np + 24.0 * N;
     }
     // two-phase assembly: element rows (4 ed FP64 + ed^2 padded FP64 blocks at
     // worst) and the slot lists (ed^2 entries per element, one pointer per slot)
@@ -4568,7 +4701,6 @@ void sors_destroy(sors_ctx* c) {
         for (hipGraphExec_t g : row)
             if (g) (void)hipGraphExecDestroy(g);
     if (c->side) (void)hipStreamSynchronize(c->side);
-    if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->evG) (void)hipEventDestroy(c->evG);
     if (c->evPre) (void)hipEventDestroy(c->evPre);
     if (c->evR) (void)hipEventDestroy(c->evR);
@@ -4641,6 +4773,19 @@ int sors_get_state(sors_ctx* c, double* qprev, double* vprev) {
     SORS_GUARD(c)
     if (qprev) HIPX(hipMemcpyAsync(qprev, c->qprev.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
     if (vprev) HIPX(hipMemcpyAsync(vprev, c->vprev.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_get_coarse(sors_ctx* c, int* nc, double* E, double* Einv) {
+    SORS_GUARD(c)
+    if (!nc) return fail(c, SORS_ERR_ARG, "null nc");
+    *nc = c->coarse_ready ? c->nc : 0;
+    if (!c->coarse_ready) return SORS_OK;
+    const size_t nc2 = (size_t)c->nc * c->nc;
+    if (E) HIPX(hipMemcpyAsync(E, c->cE.p, sizeof(double) * nc2, hipMemcpyDeviceToHost, c->stream));
+    if (Einv) HIPX(hipMemcpyAsync(Einv, c->cEinv.p, sizeof(double) * nc2, hipMemcpyDeviceToHost, c->stream));
     sync(c);
     return SORS_OK;
     SORS_GUARD_END(c)
@@ -5043,7 +5188,7 @@ int sors_set_contact(sors_ctx* c, const sors_contact_desc* d) {
         for (int v = 0; v < c->N; ++v) surf[(size_t)v] = v;
     }
     a.nsurf = (int)surf.size();
-    upload(c->csurf, surf.data(), std::max<size_t>(surf.size(), 1), c->stream);
+    upload_vec(c->csurf, surf, c->stream);
     ensure(c->lagN, sizeof(double) * std::max(a.nsurf, 1));
     ensure(c->lagn, sizeof(double) * 3 * std::max(a.nsurf, 1));
     ensure(c->lagD, sizeof(double) * 3 * std::max(a.nsurf, 1));

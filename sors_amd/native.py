@@ -141,6 +141,7 @@ def lib():
                   "sors_get_positions"):
             getattr(L, f).argtypes = [vp, _dp]
         L.sors_get_state.argtypes = [vp, _dp, _dp]
+        L.sors_get_coarse.argtypes = [vp, C.POINTER(C.c_int), _dp, _dp]
         L.sors_step.argtypes = [vp, _dp, C.c_double, C.c_int, C.POINTER(Stats)]
         L.sors_step_resident.argtypes = [vp, C.c_double, C.c_int, C.POINTER(Stats)]
         L.sors_cfl_timestep.argtypes = [vp, _dp, C.c_double, C.c_double, _dp]
@@ -384,6 +385,19 @@ class Rollout:
         q = np.zeros(self.n)
         self._check(lib().sors_get_positions(self.h, _d(q)))
         return q
+
+    def coarse_operator(self):
+        """(E, E^-1) of the two-level preconditioner: the last refresh's
+        Galerkin matrix and the inverse in use (None when no coarse
+        correction is active)."""
+        nc = C.c_int(0)
+        self._check(lib().sors_get_coarse(self.h, C.byref(nc), None, None))
+        if nc.value == 0:
+            return None
+        E = np.zeros((nc.value, nc.value))
+        Einv = np.zeros_like(E)
+        self._check(lib().sors_get_coarse(self.h, C.byref(nc), _d(E), _d(Einv)))
+        return E, Einv
 
     def cfl_timestep(self, q, dt, cfl=1.0):
         """Solver::calculate_CFL_timestep at positions q (None: the resident
