@@ -318,6 +318,18 @@ int sors_set_muscle_activation(sors_ctx* ctx, const double* act);
 /* per-element Young's modulus (Lame constants recomputed with each element's
  * Poisson ratio, neoHookean.cpp:10-11) */
 int sors_set_youngs(sors_ctx* ctx, const double* youngs);
+/* Energy::update_state (cpp/src/energy.cpp:539-557, integrator.h:79-83): q_prev
+ * = q, v_prev from q by the integrator's rule (BE: (q - q_prev)/dt; CN:
+ * 2 (q - q_prev)/dt - v_prev), g_prev = the internal force of the last
+ * gradient evaluation (gPrevTmp_).  The positions become q. */
+int sors_update_state(sors_ctx* ctx, const double* q, double dt);
+/* Solver::minimize_newton / minimize_newton_linesearch / minimize_frozenhessian
+ * (cpp/src/solver.cpp:56-107, 201-266, 268-298): the minimiser of the step
+ * energy from the initial guess q, returned in q; the integrator state is not
+ * updated (Solver::step = minimize + update_state).  A mixed-precision solve
+ * that reaches max_newton goes on in FP64, as in sors_step.  minimize_SQP
+ * maps to minimize_newton_linesearch in the facade (contact is a penalty). */
+int sors_minimize(sors_ctx* ctx, double* q, double dt, int method, sors_stats* stats);
 /* TimeIntegrator state (integrator.h:101-102): qPrev = q (and the positions),
  * vPrev = v; either may be NULL to keep it */
 int sors_set_state(sors_ctx* ctx, const double* q, const double* v);

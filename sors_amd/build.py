@@ -2,6 +2,7 @@
 
   sors_amd/libsors_amd.so   HIP kernels + C ABI (include/sors_amd.h)
   sors_amd/py_sors*.so      pybind11 module mirroring the reference's py_sors
+  tests/cpp/test_facade     C++ test of the host interfaces (run by a GPU test)
 
 Run ``python -m sors_amd.build`` (or ``__graft_entry__.build()``).  The build
 always recompiles; nothing is cached outside the tree.
@@ -82,9 +83,23 @@ def build_pybind() -> str | None:
     return out
 
 
+def build_cpp_tests() -> str | None:
+    """The C++ test program of the host interfaces, linked against the library."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "test_facade")
+    srcs = [src] + [os.path.join(CPP, f) for f in ("sors_elements.cpp", "sors_facade.cpp", "sors_io.cpp")]
+    _run([CXX, "-std=c++17", "-O2", "-I", INC, "-I", CPP, *srcs, "-L", PKG, "-lsors_amd",
+          "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../sors_amd", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build() -> None:
     build_lib()
     build_pybind()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 
+#include "sors_elements.h"
+
 #include <cmath>
 
 #include <fstream>
@@ -129,48 +131,42 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
         throw std::invalid_argument("one energy set and one Params per element are required");
     std::vector<double> rest = X.row_major();
     std::vector<double> Y(nel), P(nel), D(nel);
-    // element energy sets (tetrahedron.cpp:55-95, hexahedron.cpp:24-64): one
-    // elastic energy, the same on every element ("neohookean" or
-    // "stableneohookean"), plus "softconMuscle" on any subset of elements
+    // every element's energy plugins, built by name through the registry
+    // (tetrahedron.cpp:55-95, hexahedron.cpp:24-64); their descriptions give
+    // the device arrays.  One elastic energy per element ("neohookean" or
+    // "stableneohookean", the same type on every element), plus
+    // "softconMuscle" on any subset of elements
     int material = -1;
     std::vector<int> mgroup(nel, -1);
     std::vector<double> mk(nel, 0.0), mdir(3 * (size_t)nel, 0.0);
     int mgroups = 0;
+    elements_.reserve((size_t)nel);
     for (int e = 0; e < nel; ++e) {
         const auto& set = elementEnergiesList[e];
         if (set.empty()) throw std::runtime_error("element " + std::to_string(e) + " has no energy");
-        const Params& p = elementParameterList[e];
-        int elastic = -1;
-        for (const auto& name : set) {
-            if (name == "neohookean" || name == "stableneohookean") {
-                const int m = name == "neohookean" ? SORS_NEOHOOKEAN : SORS_STABLE_NEOHOOKEAN;
-                if (elastic >= 0)
-                    throw std::runtime_error("element " + std::to_string(e) +
-                                             ": the device path takes one elastic energy per element");
-                elastic = m;
-            } else if (name == "softconMuscle") {
-                mgroup[e] = (int)p.get_value("muscleGroup")(0, 0);
-                mk[e] = p.get_value("softconStiffness")(0, 0);
-                auto dir = p.get_value("softconDirection").row_major();  // reshaped<RowMajor>()
-                if (dir.size() != 3) throw std::invalid_argument("softconDirection must have 3 entries");
-                for (int k = 0; k < 3; ++k) mdir[3 * (size_t)e + k] = dir[k];
-                if (mgroup[e] < 0) throw std::invalid_argument("muscleGroup must be >= 0");
-                mgroups = std::max(mgroups, mgroup[e] + 1);
-            } else {
-                throw std::runtime_error("Unknown energy type: " + name +
-                                         " (the device path implements \"neohookean\", \"stableneohookean\" "
-                                         "and \"softconMuscle\")");
-            }
+        Mat Xe(E, 3);
+        for (int a = 0; a < E; ++a) {
+            const int v = conn[(size_t)E * e + a];
+            if (v < 0 || v >= numVertices_) throw std::invalid_argument("eleIdx refers to a vertex out of range");
+            for (int k = 0; k < 3; ++k) Xe(a, k) = X(v, k);
         }
-        if (elastic < 0)
+        elements_.push_back(std::make_unique<ElementType<E>>(elementParameterList[e], Xe, set));
+        const DeviceElementDesc dd = elements_.back()->device_description();
+        if (dd.material < 0)
             throw std::runtime_error("element " + std::to_string(e) +
                                      ": the device path needs an elastic energy (neohookean or stableneohookean)");
-        if (material >= 0 && elastic != material)
+        if (material >= 0 && dd.material != material)
             throw std::runtime_error("the device path takes one elastic energy type for all elements");
-        material = elastic;
-        D[e] = p.get_value("density")(0, 0);
-        Y[e] = p.get_value("youngsModulus")(0, 0);
-        P[e] = p.get_value("poissonsRatio")(0, 0);
+        material = dd.material;
+        if (dd.muscleGroup >= 0) {
+            mgroup[e] = dd.muscleGroup;
+            mk[e] = dd.muscleStiffness;
+            for (int k = 0; k < 3; ++k) mdir[3 * (size_t)e + k] = dd.muscleDirection[k];
+            mgroups = std::max(mgroups, dd.muscleGroup + 1);
+        }
+        D[e] = elements_.back()->get_density();
+        Y[e] = dd.youngs;
+        P[e] = dd.poisson;
     }
     if (material < 0) material = SORS_NEOHOOKEAN;
     std::vector<int> mask;
@@ -259,6 +255,7 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
                                : SORS_MINIMIZE_SQP;
     else
         throw std::runtime_error("Solver method not recognized");
+    sqpMethod_ = mgroups > 0 ? SORS_MINIMIZE_NEWTON : SORS_MINIMIZE_NEWTON_LINESEARCH;
     check_status(sors_create(&d, &ctx_), nullptr);
     if (mgroups > 0) {
         sors_muscle_desc md{};
@@ -369,7 +366,82 @@ std::vector<double> Energy<E>::compute_gradient(const std::vector<double>& q, do
     return g;
 }
 
+template <int E>
+SparseCSR Energy<E>::compute_hessian(const std::vector<double>& q, double dt, const Params& actuation) {
+    if ((int)q.size() != 3 * numVertices_) throw std::invalid_argument("q must have 3N entries");
+    apply_actuation(actuation);
+    const long long nnz = sors_pattern_nnz(ctx_, dt);
+    if (nnz < 0) check_status((int)nnz, ctx_);
+    SparseCSR H;
+    H.rows = H.cols = 3L * numVertices_;
+    H.outer.resize((size_t)H.rows + 1);
+    H.inner.resize((size_t)nnz);
+    H.values.resize((size_t)nnz);
+    check_status(sors_get_pattern(ctx_, dt, H.outer.data(), H.inner.data()), ctx_);
+    check_status(sors_hessian(ctx_, q.data(), dt, SORS_FP64, 0, H.values.data()), ctx_);
+    return H;
+}
+
+template <int E>
+void Energy<E>::update_state(const std::vector<double>& q, double dt) {
+    if ((int)q.size() != 3 * numVertices_) throw std::invalid_argument("q must have 3N entries");
+    check_status(sors_update_state(ctx_, q.data(), dt), ctx_);
+}
+
 // ---------------------------------------------------------------- Solver
+template <int E>
+std::vector<double> Solver<E>::minimize(int method, const std::vector<double>& guess, Energy<E>& en, double dt,
+                                        const Params& act) const {
+    if (!(dt > 0.0)) throw std::invalid_argument("dt must be positive");
+    if ((int)guess.size() != 3 * en.num_vertices()) throw std::invalid_argument("initialGuess must have 3N entries");
+    en.apply_actuation(act);
+    std::vector<double> q = guess;
+    sors_stats st{};
+    check_status(sors_minimize(en.ctx(), q.data(), dt, method, &st), en.ctx());
+    last_stats = st;
+    if (method == SORS_MINIMIZE_FROZENHESSIAN) {
+        // solver.cpp:290-292: the frozen iteration reports, not raises, a miss
+        sors_options o{};
+        check_status(sors_get_options(en.ctx(), &o), en.ctx());
+        if (!(st.dxnorm < o.dx_tol) && !(o.g_tol > 0 && st.gnorm < o.g_tol))
+            std::cout << "Frozen Newton's method did not converge" << std::endl;
+    }
+    return q;
+}
+
+template <int E>
+std::vector<double> Solver<E>::minimize_newton(const std::vector<double>& guess, Energy<E>& en, double dt,
+                                               const Params& act) const {
+    return minimize(SORS_MINIMIZE_NEWTON, guess, en, dt, act);
+}
+
+template <int E>
+std::vector<double> Solver<E>::minimize_newton_linesearch(const std::vector<double>& guess, Energy<E>& en, double dt,
+                                                          const Params& act) const {
+    return minimize(SORS_MINIMIZE_NEWTON_LINESEARCH, guess, en, dt, act);
+}
+
+template <int E>
+std::vector<double> Solver<E>::minimize_frozenhessian(const std::vector<double>& guess, Energy<E>& en, double dt,
+                                                      const Params& act) const {
+    return minimize(SORS_MINIMIZE_FROZENHESSIAN, guess, en, dt, act);
+}
+
+template <int E>
+std::vector<double> Solver<E>::minimize_SQP(const std::vector<double>& guess, Energy<E>& en, double dt,
+                                            const Params& act) const {
+    return minimize(en.sqp_method(), guess, en, dt, act);
+}
+
+template <int E>
+double Solver<E>::calculate_CFL_timestep(const std::vector<double>& solution, Energy<E>& en, double dt,
+                                         double cflConstant) const {
+    if ((int)solution.size() != 3 * en.num_vertices()) throw std::invalid_argument("solution must have 3N entries");
+    double out = dt;
+    check_status(sors_cfl_timestep(en.ctx(), solution.data(), dt, cflConstant, &out), en.ctx());
+    return out;
+}
+
 // Solver::step (solver.cpp:5-44)
 template <int E>
 std::vector<double> Solver<E>::step(const std::vector<double>& solution, Energy<E>& en, const Params& act, double dt,

@@ -13,6 +13,7 @@
 #include <iostream>
 #include <limits>
 #include <map>
+#include <memory>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -39,6 +40,24 @@ struct Mat {
         return out;
     }
 };
+
+// compressed sparse rows (the role of Eigen::SparseMatrix<double> returned by
+// Energy::compute_hessian); coeff() is 0 for an entry not stored
+struct SparseCSR {
+    long rows = 0, cols = 0;
+    std::vector<int> outer;     // rows + 1
+    std::vector<int> inner;     // column of each stored value
+    std::vector<double> values;
+    long nonZeros() const { return (long)values.size(); }
+    double coeff(long r, long c) const {
+        for (int k = outer[(size_t)r]; k < outer[(size_t)r + 1]; ++k)
+            if (inner[(size_t)k] == c) return values[(size_t)k];
+        return 0.0;
+    }
+};
+
+template <int E>
+class Element;
 
 // Key-value parameter store (cpp/headers/params.h:30-175)
 class Params {
@@ -105,6 +124,13 @@ public:
     std::vector<double> get_vPrev() const;
     double compute_energy(const std::vector<double>& q, double dt, const Params& actuation);
     std::vector<double> compute_gradient(const std::vector<double>& q, double dt, const Params& actuation);
+    // energy.cpp:312-391: element Hessians (raw, not SPD-projected), force
+    // terms, the integrator's mass / damping diagonal and the Neumann rows of
+    // minimize_newton; FP64 values on the reference's CSR pattern
+    SparseCSR compute_hessian(const std::vector<double>& q, double dt, const Params& actuation);
+    // energy.cpp:539-557: the integrator's state (q_prev, v_prev, g_prev) from q;
+    // the element plugins here keep no state of their own
+    void update_state(const std::vector<double>& q, double dt);
     void apply_actuation(const Params& actuation);
 
     // DiskContact::set_disk (cpp/headers/constraints/diskContact.h:72-80)
@@ -113,6 +139,11 @@ public:
 
     sors_ctx* ctx() const { return ctx_; }
     int num_vertices() const { return numVertices_; }
+    // the device method minimize_SQP resolves to (contact is a penalty on the device)
+    int sqp_method() const { return sqpMethod_; }
+    // energy.h: one Tetrahedron / Hexahedron per element, its energy plugins
+    // built through the ElementEnergy registry; they evaluate on the device
+    std::vector<std::unique_ptr<Element<E>>> elements_;
     SimulationSettings settings_;
     std::vector<std::string> forceTypesList_;
     std::vector<std::string> constraintTypesList_;
@@ -122,6 +153,7 @@ private:
     int numVertices_ = 0;
     bool hasContact_ = false;
     int muscleGroups_ = 0;
+    int sqpMethod_ = SORS_MINIMIZE_NEWTON_LINESEARCH;
 };
 
 // Non-owning handle on the diskContact entry of an Energy's constraint list
@@ -145,7 +177,28 @@ public:
                              double dt = 0.0, int substeps = 0) const;
     std::vector<double> step(const std::vector<double>& solution, Energy<E>& systemEnergy, double dt = 0.0,
                              int substeps = 0) const;
+    // solver.cpp:56-107, 201-266, 268-298: one minimisation of the step energy
+    // from initialGuess; the energy's state is not updated (step = minimize +
+    // update_state)
+    std::vector<double> minimize_newton(const std::vector<double>& initialGuess, Energy<E>& systemEnergy, double dt,
+                                        const Params& act) const;
+    std::vector<double> minimize_newton_linesearch(const std::vector<double>& initialGuess, Energy<E>& systemEnergy,
+                                                   double dt, const Params& act) const;
+    std::vector<double> minimize_frozenhessian(const std::vector<double>& initialGuess, Energy<E>& systemEnergy,
+                                               double dt, const Params& act) const;
+    // solver.cpp:109-197: contact as the device's penalty; Newton with line
+    // search (plain Newton with a muscle: its energy is not the potential of
+    // its gradient)
+    std::vector<double> minimize_SQP(const std::vector<double>& initialGuess, Energy<E>& systemEnergy, double dt,
+                                     const Params& act) const;
+    // solver.cpp:300-331
+    double calculate_CFL_timestep(const std::vector<double>& solution, Energy<E>& systemEnergy, double dt,
+                                  double cflConstant) const;
     mutable sors_stats last_stats{};
+
+private:
+    std::vector<double> minimize(int method, const std::vector<double>& initialGuess, Energy<E>& systemEnergy,
+                                 double dt, const Params& act) const;
 };
 
 // throws std::runtime_error carrying the reference's message for a status code

@@ -4778,6 +4778,53 @@ int sors_get_state(sors_ctx* c, double* qprev, double* vprev) {
     SORS_GUARD_END(c)
 }
 
+int sors_update_state(sors_ctx* c, const double* q, double dt) {
+    SORS_GUARD(c)
+    if (!q) return fail(c, SORS_ERR_ARG, "null positions");
+    if (!(dt > 0.0)) return fail(c, SORS_ERR_ARG, "dt must be > 0");
+    upload(c->q, q, (size_t)c->n, c->stream);
+    launch_update_state(c, dt);
+    HIPX(hipGetLastError());
+    std::memcpy(c->cargs.dc_last, c->cargs.dc, sizeof(c->cargs.dc));
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_minimize(sors_ctx* c, double* q, double dt, int method, sors_stats* st) {
+    SORS_GUARD(c)
+    if (!q) return fail(c, SORS_ERR_ARG, "null positions");
+    if (!(dt > 0.0)) return fail(c, SORS_ERR_ARG, "dt must be > 0");
+    if (method != SORS_MINIMIZE_NEWTON && method != SORS_MINIMIZE_NEWTON_LINESEARCH &&
+        method != SORS_MINIMIZE_FROZENHESSIAN)
+        return fail(c, SORS_ERR_ARG, "method must be minimize_newton, minimize_newton_linesearch or "
+                                     "minimize_frozenhessian");
+    sors_stats local{};
+    if (!st) st = &local;
+    *st = sors_stats{};
+    struct MethodReset {
+        sors_ctx* c;
+        int m;
+        ~MethodReset() { c->method = m; }
+    } reset{c, c->method};
+    c->method = method;
+    auto t0 = std::chrono::steady_clock::now();
+    upload(c->q, q, (size_t)c->n, c->stream);
+    ensure_pattern(c, dt);
+    contact_lag(c);
+    int rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, dt, st) : minimize<double>(c, dt, st);
+    if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
+        st->fp64_fallbacks++;
+        rc = minimize<double>(c, dt, st);
+    }
+    if (rc != SORS_OK) return rc;
+    HIPX(hipMemcpyAsync(q, c->q.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    st->ms_total = ms_since(t0);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
 int sors_get_coarse(sors_ctx* c, int* nc, double* E, double* Einv) {
     SORS_GUARD(c)
     if (!nc) return fail(c, SORS_ERR_ARG, "null nc");
