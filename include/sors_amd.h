@@ -315,6 +315,26 @@ int sors_set_muscle(sors_ctx* ctx, const sors_muscle_desc* desc);
 /* the "softconMuscle" actuation of Solver::step (element.h:52): activations
  * [n_groups] used by the following steps (0 until set) */
 int sors_set_muscle_activation(sors_ctx* ctx, const double* act);
+/* PressureForce (cpp/src/forces/pressureForce.cpp:14-185; the "pressure"
+ * force of Energy, energy.cpp:155-162 with forceParameterList
+ * "surfaceVertexIdx" / "surfaceGroups"): faces of 3 vertices (tet4 meshes) or
+ * 4 (hex8), faces[nv f + j], each in an actuation group; the force -p A / nv
+ * on each face vertex (A: the face's area vector at the current positions).
+ * It enters the energy as +f(q).(q - qPrev) and the gradient as -f(q)
+ * (energy.cpp:228-244, 282-293).  Its Jacobian (the reference's p/6 blocks,
+ * minimize_newton only) is part of sors_hessian's result; the solve's SPD
+ * tangent leaves it out, so Newton's iterates differ from the reference's
+ * but not its converged state.  Needs the node-block layout; n_faces = 0 or
+ * NULL removes the force.  Pressures start at 0. */
+typedef struct {
+    int n_faces;
+    const int* faces;         /* [n_faces][3 or 4] vertex indices */
+    const int* groups;        /* [n_faces] actuation group */
+    int n_groups;
+} sors_pressure_desc;
+int sors_set_pressure(sors_ctx* ctx, const sors_pressure_desc* desc);
+/* the "pressure" actuation: pressure per group [n_groups] */
+int sors_set_pressure_activation(sors_ctx* ctx, const double* pressure);
 /* per-element Young's modulus (Lame constants recomputed with each element's
  * Poisson ratio, neoHookean.cpp:10-11) */
 int sors_set_youngs(sors_ctx* ctx, const double* youngs);

@@ -130,6 +130,13 @@ def lib():
         L.orc_set_muscle.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp]
         L.orc_set_muscle_activation.argtypes = [C.c_void_p, _dp]
         L.orc_set_muscle_activation.restype = None
+        L.orc_set_pressure.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, C.c_int]
+        L.orc_set_pressure_activation.argtypes = [C.c_void_p, _dp]
+        L.orc_set_pressure_activation.restype = None
+        L.orc_pressure_force.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_pressure_force.restype = None
+        L.orc_pressure_jacobian.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_pressure_jacobian.restype = None
         _lib = L
     return _lib
 
@@ -453,6 +460,34 @@ class System:
         a = f64(act)
         assert a.size == self.mgroups
         lib().orc_set_muscle_activation(self.h, _d(a))
+
+    def set_pressure(self, faces, groups, ngroups=None):
+        """PressureForce on faces (F x 3 for tet meshes, F x 4 for hex meshes)
+        with an actuation group each; empty faces remove it."""
+        f = i32(faces)
+        g = i32(groups).reshape(-1)
+        nv = f.shape[1] if f.ndim == 2 and f.size else (4 if self.ed == 8 else 3)
+        self.pngroups = int(g.max()) + 1 if ngroups is None and g.size else int(ngroups or 0)
+        self._pressure_keep = (f, g)
+        if lib().orc_set_pressure(self.h, int(f.size // max(nv, 1)), int(nv), _i(f), _i(g), self.pngroups) != 0:
+            raise ValueError("orc_set_pressure")
+
+    def set_pressure_activation(self, p):
+        a = f64(p).reshape(-1)
+        assert a.size == self.pngroups
+        lib().orc_set_pressure_activation(self.h, _d(a))
+
+    def pressure_force(self, q):
+        f = np.zeros(self.n)
+        lib().orc_pressure_force(self.h, _d(f64(q).reshape(-1)), _d(f))
+        return f
+
+    def pressure_jacobian(self, q):
+        """Blocks J[f, j, k] (3 x 3) of the reference's force-gradient triplets."""
+        nf, nv = self._pressure_keep[0].shape
+        J = np.zeros(nf * nv * nv * 9)
+        lib().orc_pressure_jacobian(self.h, _d(f64(q).reshape(-1)), _d(J))
+        return J.reshape(nf, nv, nv, 3, 3)
 
     def update_state(self, q, dt):
         q = f64(q).reshape(-1)

@@ -591,6 +591,11 @@ struct orc_sys {
     double* mk;     /* [nel] softconStiffness */
     double* mdir;   /* [nel][3] softconDirection */
     double* mact;   /* [mngroups] activation */
+    /* PressureForce (orc_set_pressure) */
+    int npf, pnv, pngroups;
+    int* pfaces;    /* [npf][pnv] */
+    int* pgroup;    /* [npf] */
+    double* pact;   /* [pngroups] pressure */
     double* lagN;   /* [nsurf] lagged normal force magnitude */
     double* lagn;   /* [nsurf][3] its normal */
     double* lagD;   /* [nsurf][3] displacement of its primitive over the step */
@@ -693,6 +698,9 @@ void orc_free(orc_sys* s) {
     free(s->mk);
     free(s->mdir);
     free(s->mact);
+    free(s->pfaces);
+    free(s->pgroup);
+    free(s->pact);
     free(s->surf);
     free(s->lagN);
     free(s->lagn);
@@ -711,6 +719,140 @@ void orc_set_external_force(orc_sys* s, const double* f) {
     }
     if (!s->fext) s->fext = (double*)malloc(sizeof(double) * s->n);
     memcpy(s->fext, f, sizeof(double) * s->n);
+}
+
+/* PressureForce::PressureForce (pressureForce.cpp:4-11) */
+int orc_set_pressure(orc_sys* s, int nfaces, int nv, const int* faces, const int* groups, int ngroups) {
+    free(s->pfaces);
+    free(s->pgroup);
+    free(s->pact);
+    s->pfaces = NULL;
+    s->pgroup = NULL;
+    s->pact = NULL;
+    s->npf = 0;
+    if (nfaces <= 0) return ORC_OK;
+    if (nv != (s->ed == 8 ? 4 : 3) || ngroups < 1) return ORC_ERR_ARG;
+    for (int f = 0; f < nfaces; ++f) {
+        if (groups[f] < 0 || groups[f] >= ngroups) return ORC_ERR_ARG;
+        for (int j = 0; j < nv; ++j)
+            if (faces[(size_t)nv * f + j] < 0 || faces[(size_t)nv * f + j] >= s->N) return ORC_ERR_ARG;
+    }
+    s->npf = nfaces;
+    s->pnv = nv;
+    s->pngroups = ngroups;
+    s->pfaces = (int*)malloc(sizeof(int) * (size_t)nv * nfaces);
+    s->pgroup = (int*)malloc(sizeof(int) * (size_t)nfaces);
+    s->pact = (double*)calloc((size_t)ngroups, sizeof(double));
+    memcpy(s->pfaces, faces, sizeof(int) * (size_t)nv * nfaces);
+    memcpy(s->pgroup, groups, sizeof(int) * (size_t)nfaces);
+    return ORC_OK;
+}
+
+void orc_set_pressure_activation(orc_sys* s, const double* p) {
+    if (s->npf > 0) memcpy(s->pact, p, sizeof(double) * s->pngroups);
+}
+
+/* area vector of a face at q: a triangle's (x1 - x0) x (x2 - x0) / 2, a
+ * quad's two triangles (x0, x1, x3) and (x2, x3, x1) (pressureForce.cpp:22-33) */
+static void face_area(const orc_sys* s, int f, const double* q, double A[3]) {
+    const int* fv = s->pfaces + (size_t)s->pnv * f;
+    const double* x0 = q + 3 * (size_t)fv[0];
+    const double* x1 = q + 3 * (size_t)fv[1];
+    if (s->pnv == 3) {
+        const double* x2 = q + 3 * (size_t)fv[2];
+        double a[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = x1[k] - x0[k];
+            b[k] = x2[k] - x0[k];
+        }
+        A[0] = 0.5 * (a[1] * b[2] - a[2] * b[1]);
+        A[1] = 0.5 * (a[2] * b[0] - a[0] * b[2]);
+        A[2] = 0.5 * (a[0] * b[1] - a[1] * b[0]);
+    } else {
+        const double* x2 = q + 3 * (size_t)fv[2];
+        const double* x3 = q + 3 * (size_t)fv[3];
+        double a[3], b[3], c[3], d[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = x1[k] - x0[k];
+            b[k] = x3[k] - x0[k];
+            c[k] = x3[k] - x2[k];
+            d[k] = x1[k] - x2[k];
+        }
+        A[0] = 0.5 * (a[1] * b[2] - a[2] * b[1]);
+        A[1] = 0.5 * (a[2] * b[0] - a[0] * b[2]);
+        A[2] = 0.5 * (a[0] * b[1] - a[1] * b[0]);
+        A[0] += 0.5 * (c[1] * d[2] - c[2] * d[1]);
+        A[1] += 0.5 * (c[2] * d[0] - c[0] * d[2]);
+        A[2] += 0.5 * (c[0] * d[1] - c[1] * d[0]);
+    }
+}
+
+/* PressureForce::compute_force (pressureForce.cpp:14-55): -p A / nv on every
+ * vertex of the face, faces in order */
+void orc_pressure_force(const orc_sys* s, const double* q, double* f) {
+    memset(f, 0, sizeof(double) * s->n);
+    for (int fi = 0; fi < s->npf; ++fi) {
+        double A[3];
+        face_area(s, fi, q, A);
+        const double p = s->pact[s->pgroup[fi]];
+        double fP[3];
+        for (int k = 0; k < 3; ++k) fP[k] = -p * A[k] / s->pnv;
+        for (int j = 0; j < s->pnv; ++j)
+            for (int k = 0; k < 3; ++k) f[3 * (size_t)s->pfaces[(size_t)s->pnv * fi + j] + k] += fP[k];
+    }
+}
+
+/* the 3x3 matrix M(a, b) of the reference's hessX blocks: rows
+ * (0, b2 - a2, a1 - b1), (a2 - b2, 0, b0 - a0), (b1 - a1, a0 - b0, 0) for
+ * the block of vertex k with the other two triangle vertices a (next) and
+ * b (previous) in the reference's order (pressureForce.cpp:66-80) */
+static void cross_block(const double* xa, const double* xb, double* M) {
+    M[0] = 0;
+    M[1] = -xb[2] + xa[2];
+    M[2] = xb[1] - xa[1];
+    M[3] = xb[2] - xa[2];
+    M[4] = 0;
+    M[5] = -xb[0] + xa[0];
+    M[6] = -xb[1] + xa[1];
+    M[7] = xb[0] - xa[0];
+    M[8] = 0;
+}
+
+/* PressureForce::compute_force_gradient (pressureForce.cpp:60-185): block
+ * (j, k) = p / 6 * hessX[k], the same for every row vertex j.  Triangles:
+ * hessX0 = M(x1, x2), hessX1 = M(x2, x0), hessX2 = M(x0, x1).  Quads: the
+ * triangle (x0, x1, x3) gives hessX0, hessX1, hessX3 by the same rule and the
+ * triangle (x2, x3, x1) adds hessX2 (set) and its x3 / x1 blocks (added). */
+void orc_pressure_jacobian(const orc_sys* s, const double* q, double* J) {
+    const int nv = s->pnv;
+    for (int fi = 0; fi < s->npf; ++fi) {
+        const int* fv = s->pfaces + (size_t)nv * fi;
+        double H[4][9];
+        if (nv == 3) {
+            const double *x0 = q + 3 * (size_t)fv[0], *x1 = q + 3 * (size_t)fv[1], *x2 = q + 3 * (size_t)fv[2];
+            cross_block(x1, x2, H[0]);
+            cross_block(x2, x0, H[1]);
+            cross_block(x0, x1, H[2]);
+        } else {
+            const double *q0 = q + 3 * (size_t)fv[0], *q1 = q + 3 * (size_t)fv[1], *q2 = q + 3 * (size_t)fv[2],
+                         *q3 = q + 3 * (size_t)fv[3];
+            double T[9];
+            /* first triangle: x0 = q0, x1 = q1, x2 = q3 */
+            cross_block(q1, q3, H[0]);
+            cross_block(q3, q0, H[1]);
+            cross_block(q0, q1, H[3]);
+            /* second triangle: x0 = q2, x1 = q3, x2 = q1 */
+            cross_block(q3, q1, H[2]);
+            cross_block(q1, q2, T);     /* its x1 (= q3) block */
+            for (int t = 0; t < 9; ++t) H[3][t] += T[t];
+            cross_block(q2, q3, T);     /* its x2 (= q1) block */
+            for (int t = 0; t < 9; ++t) H[1][t] += T[t];
+        }
+        const double p = s->pact[s->pgroup[fi]];
+        for (int j = 0; j < nv; ++j)
+            for (int k = 0; k < nv; ++k)
+                for (int t = 0; t < 9; ++t) J[(((size_t)fi * nv + j) * nv + k) * 9 + t] = p / 6 * H[k][t];
+    }
 }
 
 int orc_ndof(const orc_sys* s) { return s->n; }
@@ -1018,6 +1160,12 @@ double orc_energy(orc_sys* s, const double* q, double dt) {
     (void)bad;
     if (s->fext)
         for (int i = 0; i < s->n; ++i) energy += s->fext[i] * (q[i] - s->qprev[i]);
+    if (s->npf > 0) {
+        double* fp = (double*)malloc(sizeof(double) * s->n);
+        orc_pressure_force(s, q, fp);
+        for (int i = 0; i < s->n; ++i) energy += fp[i] * (q[i] - s->qprev[i]);
+        free(fp);
+    }
     return integrate_energy(s, q, dt, energy) + contact_eval(s, q, dt, NULL, NULL);
 }
 
@@ -1055,6 +1203,12 @@ int orc_gradient(orc_sys* s, const double* q, double dt, double* g) {
     free(acc);
     if (s->fext)
         for (int i = 0; i < n; ++i) g[i] -= s->fext[i];
+    if (s->npf > 0) {
+        double* fp = (double*)malloc(sizeof(double) * n);
+        orc_pressure_force(s, q, fp);
+        for (int i = 0; i < n; ++i) g[i] -= fp[i];
+        free(fp);
+    }
     /* integrate_gradient */
     if (dt != 0) {
         memcpy(s->gprevtmp, g, sizeof(double) * n);
@@ -1241,6 +1395,28 @@ int orc_hessian(orc_sys* s, const double* q, double dt, int project, double* val
                 }
             }
         }
+    }
+    if (s->npf > 0 && s->method == ORC_NEWTON) {
+        /* force Jacobian triplets, unmasked rows and columns (energy.cpp:354-372) */
+        const int nv = s->pnv;
+        double* J = (double*)malloc(sizeof(double) * 9 * (size_t)s->npf * nv * nv);
+        orc_pressure_jacobian(s, q, J);
+        for (int fi = 0; fi < s->npf; ++fi)
+            for (int j = 0; j < nv; ++j)
+                for (int k = 0; k < nv; ++k)
+                    for (int l = 0; l < 3; ++l)
+                        for (int m = 0; m < 3; ++m) {
+                            const int R = 3 * s->pfaces[(size_t)nv * fi + j] + l;
+                            const int C = 3 * s->pfaces[(size_t)nv * fi + k] + m;
+                            if (s->mask[R] || s->mask[C]) continue;
+                            const int idx = csr_find(s, R, C);
+                            if (idx < 0) {
+                                status = ORC_ERR_ARG;  /* a face pair outside the element pattern */
+                                continue;
+                            }
+                            vals[idx] += J[(((size_t)fi * nv + j) * nv + k) * 9 + 3 * l + m];
+                        }
+        free(J);
     }
     if (dt != 0) {
         double fac = (s->scheme == ORC_BE) ? 1.0 : 0.5;

@@ -171,6 +171,21 @@ int  orc_set_material(orc_sys* s, int material);
 int  orc_set_muscle(orc_sys* s, int ngroups, const int* group, const double* stiffness, const double* dir);
 void orc_set_muscle_activation(orc_sys* s, const double* act);
 
+/* ---- PressureForce (cpp/src/forces/pressureForce.cpp:14-185) ---------------
+ * Faces of nv = 3 (tet4 meshes) or 4 (hex8) vertex indices, faces[nv f + j],
+ * each in an actuation group; pressures per group (the "pressure"
+ * actuation).  The force enters the energy as +f(q).(q - qPrev) and the
+ * gradient as -f(q) (energy.cpp:228-244, 282-293); its Jacobian enters the
+ * Hessian only for minimize_newton (energy.cpp:354-372), with the reference's
+ * p/6 factor on both face types.  nfaces = 0 removes the force.
+ * orc_pressure_force / orc_pressure_jacobian expose the two pieces: f [n],
+ * and the dense Jacobian block list J[f][j][k][3][3] of the reference's
+ * triplets (row 3 faces[j] + l, column 3 faces[k] + m). */
+int  orc_set_pressure(orc_sys* s, int nfaces, int nv, const int* faces, const int* groups, int ngroups);
+void orc_set_pressure_activation(orc_sys* s, const double* p);
+void orc_pressure_force(const orc_sys* s, const double* q, double* f);
+void orc_pressure_jacobian(const orc_sys* s, const double* q, double* J);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,7 +121,7 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
                   const std::vector<std::set<std::string>>& elementEnergiesList,
                   const std::vector<Params>& elementParameterList, const std::vector<std::string>& constraintTypesList,
                   const Params& constraintParameterList, const std::vector<std::string>& forceTypesList,
-                  const Params& /*forceParameterList*/)
+                  const Params& forceParameterList)
     : settings_(settings), forceTypesList_(forceTypesList) {
     if (X.cols != 3) throw std::invalid_argument("undeformedVertices must be N x 3");
     numVertices_ = (int)X.rows;
@@ -216,9 +216,26 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
                                      " (the device path implements neumannBC, planeContact, diskContact)");
         }
     }
-    for (const auto& f : forceTypesList)
-        if (f != "vertexForce")
-            throw std::runtime_error("Unknown force type: " + f + " (the device path implements vertexForce)");
+    // external forces (energy.cpp:155-172): "vertexForce" (the actuation
+    // vector) and "pressure" (PressureForce on "surfaceVertexIdx" faces in
+    // "surfaceGroups")
+    std::vector<int> pfaces, pgroups;
+    for (const auto& f : forceTypesList) {
+        if (f == "vertexForce") continue;
+        if (f == "pressure") {
+            const Mat fv = forceParameterList.get_value("surfaceVertexIdx");
+            const std::vector<double> g = forceParameterList.get_value("surfaceGroups").row_major();  // reshaped<RowMajor>()
+            const long nv = E == 8 ? 4 : 3;
+            if (fv.cols != nv) throw std::invalid_argument("surfaceVertexIdx must have " + std::to_string(nv) + " columns");
+            if ((long)g.size() != fv.rows) throw std::invalid_argument("one surfaceGroups entry per face is required");
+            for (double v : fv.row_major()) pfaces.push_back((int)v);
+            for (double v : g) pgroups.push_back((int)v);
+            pressureGroups_ = 0;
+            for (int v : pgroups) pressureGroups_ = std::max(pressureGroups_, v + 1);
+            continue;
+        }
+        throw std::runtime_error("Unknown force type: " + f + " (the device path implements vertexForce, pressure)");
+    }
     sors_system_desc d{};
     d.n_nodes = numVertices_;
     d.n_elems = nel;
@@ -265,6 +282,14 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
         md.stiffness = mk.data();
         check_status(sors_set_muscle(ctx_, &md), ctx_);
         muscleGroups_ = mgroups;
+    }
+    if (!pgroups.empty()) {
+        sors_pressure_desc pd{};
+        pd.n_faces = (int)pgroups.size();
+        pd.faces = pfaces.data();
+        pd.groups = pgroups.data();
+        pd.n_groups = pressureGroups_;
+        check_status(sors_set_pressure(ctx_, &pd), ctx_);
     }
     if (hasContact_) {
         sors_contact_desc cd{};
@@ -337,6 +362,13 @@ void Energy<E>::apply_actuation(const Params& act) {
         if ((int)a.size() < muscleGroups_)
             throw std::invalid_argument("softconMuscle actuation needs one value per muscle group");
         check_status(sors_set_muscle_activation(ctx_, a.data()), ctx_);
+    }
+    // "pressure": one value per surface group (pressureForce.cpp:34, act.row(group))
+    if (pressureGroups_ > 0 && act.has("pressure")) {
+        auto p = act.get_value("pressure").row_major();  // transpose().reshaped()
+        if ((int)p.size() < pressureGroups_)
+            throw std::invalid_argument("pressure actuation needs one value per surface group");
+        check_status(sors_set_pressure_activation(ctx_, p.data()), ctx_);
     }
     bool vf = false;
     for (const auto& f : forceTypesList_) vf |= (f == "vertexForce");

@@ -153,6 +153,7 @@ private:
     int numVertices_ = 0;
     bool hasContact_ = false;
     int muscleGroups_ = 0;
+    int pressureGroups_ = 0;
     int sqpMethod_ = SORS_MINIMIZE_NEWTON_LINESEARCH;
 };
 

@@ -108,6 +108,20 @@ struct NodeArgs {
     void* vals;           // HT or null
     const int* diag;
     const double* gcon;   // contact gradient per DOF (masked DOFs 0) or null
+    const double* fpres;  // pressure force at q per DOF or null
+    const double* fpres1; // pressure force at q + alpha dq (energy differences) or null
+};
+
+// PressureForce faces (pressureForce.cpp): nv = 3 (tet4 meshes) or 4 (hex8)
+// vertices per face, SoA faces[j * nf + f]; per-node incident faces in
+// ascending order (incptr / inc)
+struct PressArgs {
+    int nf, nv, N;
+    const int* faces;
+    const int* group;
+    const double* pact;   // pressure per group
+    const int* incptr;
+    const int* inc;
 };
 
 // Contact primitives and per-candidate friction lag (penalty + lagged

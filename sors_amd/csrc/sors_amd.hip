@@ -1294,6 +1294,144 @@ __global__ __launch_bounds__(kBlock) void k_cfl_partials(int E, const int* conn,
 }
 
 // ===========================================================================
+// PressureForce (pressureForce.cpp:14-185): per face the area vector A (a
+// triangle's (x1 - x0) x (x2 - x0) / 2; a quad's triangles (x0, x1, x3) and
+// (x2, x3, x1)) and the force -p A / nv on each of its vertices; per node the
+// sum over its faces in ascending order (no atomics).  The Jacobian blocks
+// (row vertex j, column vertex k) = p/6 hessX[k], added row by row into the
+// node-block layout (one lane per row node).
+// ===========================================================================
+__device__ __forceinline__ void press_pos(const double* q, const double* dq, double alpha, int v, double x[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double t = q[3 * v + k];
+        if (dq) t += alpha * dq[3 * v + k];
+        x[k] = t;
+    }
+}
+
+__device__ __forceinline__ void half_cross(const double a[3], const double b[3], double out[3]) {
+    out[0] = 0.5 * (a[1] * b[2] - a[2] * b[1]);
+    out[1] = 0.5 * (a[2] * b[0] - a[0] * b[2]);
+    out[2] = 0.5 * (a[0] * b[1] - a[1] * b[0]);
+}
+
+// ff[3 f + k] = -p A_k / nv at x = q + alpha dq (dq may be null)
+__global__ __launch_bounds__(kBlock) void k_pressure_faces(PressArgs a, const double* q, const double* dq,
+                                                           double alpha, double* ff) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nf) return;
+    double x[4][3];
+    for (int j = 0; j < a.nv; ++j) press_pos(q, dq, alpha, a.faces[(size_t)j * a.nf + f], x[j]);
+    double A[3], u[3], v[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        u[k] = x[1][k] - x[0][k];
+        v[k] = x[a.nv == 3 ? 2 : 3][k] - x[0][k];
+    }
+    half_cross(u, v, A);
+    if (a.nv == 4) {
+        double B[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            u[k] = x[3][k] - x[2][k];
+            v[k] = x[1][k] - x[2][k];
+        }
+        half_cross(u, v, B);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) A[k] += B[k];
+    }
+    const double p = a.pact[a.group[f]];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ff[3 * (size_t)f + k] = -p * A[k] / a.nv;
+}
+
+// fp[3 v + k] = sum over the faces of v, ascending
+__global__ __launch_bounds__(kBlock) void k_pressure_nodes(PressArgs a, const double* ff, double* fp) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.N) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int t = a.incptr[v]; t < a.incptr[v + 1]; ++t) {
+        const int f = a.inc[t];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s[k] += ff[3 * (size_t)f + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) fp[3 * (size_t)v + k] = s[k];
+}
+
+// the reference's 3x3 block for a vertex with the other two triangle
+// vertices xa, xb: rows (0, xa2 - xb2, xb1 - xa1), (xb2 - xa2, 0, xa0 - xb0),
+// (xa1 - xb1, xb0 - xa0, 0)
+__device__ __forceinline__ void press_block(const double xa[3], const double xb[3], double M[9], bool add) {
+    const double m[9] = {0.0, -xb[2] + xa[2], xb[1] - xa[1], xb[2] - xa[2], 0.0, -xb[0] + xa[0],
+                         -xb[1] + xa[1], xb[0] - xa[0], 0.0};
+#pragma unroll
+    for (int t = 0; t < 9; ++t) M[t] = add ? M[t] + m[t] : m[t];
+}
+
+// rows of node v: += scale * p/6 hessX[k] for every face of v and vertex k of
+// that face (both nodes free); *flag |= 4 when a face pair is not in the layout
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_pressure_jacobian(PressArgs a, const double* q, const int* sbase,
+                                                              const int* bcol, const int* self_slot, const int* mask,
+                                                              double scale, HT* vals, int* flag) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.N) return;
+    if (mask && mask[3 * v]) return;
+    const int sl = v >> 6, lane = v & (kSlice - 1);
+    const int width = (sbase[sl + 1] - sbase[sl]) / kSlice;
+    for (int t = a.incptr[v]; t < a.incptr[v + 1]; ++t) {
+        const int f = a.inc[t];
+        int fv[4];
+        double x[4][3];
+        for (int j = 0; j < a.nv; ++j) {
+            fv[j] = a.faces[(size_t)j * a.nf + f];
+            press_pos(q, nullptr, 0.0, fv[j], x[j]);
+        }
+        double H[4][9];
+        if (a.nv == 3) {
+            press_block(x[1], x[2], H[0], false);
+            press_block(x[2], x[0], H[1], false);
+            press_block(x[0], x[1], H[2], false);
+        } else {
+            press_block(x[1], x[3], H[0], false);  // triangle (x0, x1, x3)
+            press_block(x[3], x[0], H[1], false);
+            press_block(x[0], x[1], H[3], false);
+            press_block(x[3], x[1], H[2], false);  // triangle (x2, x3, x1)
+            press_block(x[1], x[2], H[3], true);
+            press_block(x[2], x[3], H[1], true);
+        }
+        const double c6 = a.pact[a.group[f]] / 6;
+        for (int k = 0; k < a.nv; ++k) {
+            const int cn = fv[k];
+            if (mask && mask[3 * cn]) continue;
+            int slot = -1;
+            if (cn == v) {
+                slot = self_slot[v];
+            } else {
+                for (int w = 0; w < width; ++w)
+                    if (bcol[sbase[sl] + kSlice * w + lane] == cn) {
+                        slot = w;
+                        break;
+                    }
+            }
+            if (slot < 0) {
+                atomicOr(flag, 4);
+                continue;
+            }
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const long long idx = block_index(sbase, v, slot, l, m);
+                    vals[idx] = (HT)((double)vals[idx] + scale * (c6 * H[k][3 * l + m]));
+                }
+        }
+    }
+}
+
+// ===========================================================================
 // Node kernels: inertia, gravity, damping, external force, Dirichlet rows
 // (integrator.cpp:45-124, energy.cpp:282-306, 381-388)
 // ===========================================================================
@@ -1304,6 +1442,7 @@ __global__ __launch_bounds__(kBlock) void k_node_terms(NodeArgs a, double* gg_ou
         const double m = a.mass[i], qi = a.q[i];
         double gi = a.gint ? a.gint[i] : 0.0;
         if (a.fext) gi -= a.fext[i];
+        if (a.fpres) gi -= a.fpres[i];
         const double gc = -m * a.grav[i % 3];
         double g, hd = 0.0;
         if (a.dt != 0.0) {
@@ -1353,6 +1492,7 @@ __global__ __launch_bounds__(kBlock) void k_node_energy(NodeArgs a, const double
             const double d = alpha * dq[i];
             en = -m * d * gc;
             if (a.fext) en += a.fext[i] * d;
+            if (a.fpres) en += a.fpres1[i] * (qi + d - a.qprev[i]) - a.fpres[i] * (qi - a.qprev[i]);
             if (a.dt != 0.0) {
                 if (a.scheme == SORS_BACKWARD_EULER) {
                     const double vel = (qi - a.qprev[i]) / a.dt, acc = (vel - a.vprev[i]) / a.dt;
@@ -1369,6 +1509,7 @@ __global__ __launch_bounds__(kBlock) void k_node_energy(NodeArgs a, const double
         } else {
             en = -m * qi * gc;
             if (a.fext) en += a.fext[i] * (qi - a.qprev[i]);
+            if (a.fpres) en += a.fpres[i] * (qi - a.qprev[i]);
             if (a.dt != 0.0) {
                 if (a.scheme == SORS_BACKWARD_EULER) {
                     const double vel = (qi - a.qprev[i]) / a.dt, acc = (vel - a.vprev[i]) / a.dt;
@@ -2864,6 +3005,13 @@ struct sors_ctx {
     bool has_muscle = false;
     int mngroups = 0;
     DevMem mgroup, mdir, mk, mact;
+    // PressureForce (sors_set_pressure): faces, groups, pressures, the
+    // node -> face incidence, face forces and the node forces at q (fp0)
+    // and at the line-search point (fp1)
+    bool has_pressure = false;
+    bool export_pressure_jacobian = false;  // set by sors_hessian alone
+    int pnf = 0, pnv = 0, pngroups = 0;
+    DevMem pfaces, pgroup, pact, pincptr, pinc, pff, fp0, fp1;
     // contact (sors_set_contact): primitives by value, candidates and lag on the device
     bool has_contact = false;
     ContactArgs cargs{};
@@ -3021,8 +3169,29 @@ ElemArgs elem_args(sors_ctx* c, const double* q) {
     return a;
 }
 
+PressArgs press_args(sors_ctx* c) {
+    PressArgs a;
+    a.nf = c->pnf;
+    a.nv = c->pnv;
+    a.N = c->N;
+    a.faces = c->pfaces.as<int>();
+    a.group = c->pgroup.as<int>();
+    a.pact = c->pact.as<double>();
+    a.incptr = c->pincptr.as<int>();
+    a.inc = c->pinc.as<int>();
+    return a;
+}
+
+// node pressure forces at q + alpha dq (dq may be null) into fp
+void pressure_force(sors_ctx* c, const double* q, const double* dq, double alpha, double* fp, hipStream_t st) {
+    const PressArgs a = press_args(c);
+    k_pressure_faces<<<grid_for(a.nf, kBlock), kBlock, 0, st>>>(a, q, dq, alpha, c->pff.as<double>());
+    k_pressure_nodes<<<grid_for(a.N, kBlock), kBlock, 0, st>>>(a, c->pff.as<double>(), fp);
+}
+
 NodeArgs node_args(sors_ctx* c, const double* q, double dt) {
     NodeArgs a{};
+    if (c->has_pressure) a.fpres = c->fp0.as<double>();
     a.n = c->n;
     a.dt = dt;
     a.scheme = c->scheme;
@@ -3171,6 +3340,18 @@ void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project
         HIPX(hipEventElapsedTime(&ms, ev[0], ev[1]));
         c->prof->assemble_kernel_ms += ms;
         c->prof->assemble_launches += 1;
+    }
+    if (c->has_pressure) {
+        pressure_force(c, q, nullptr, 0.0, c->fp0.as<double>(), c->stream);
+        // the force Jacobian (minimize_newton, energy.cpp:354-372) is part of
+        // the exported Hessian; the solve's SPD tangent leaves it out (it is
+        // not symmetric), so Newton's iterates differ but its fixed point does not
+        if (hessian && c->export_pressure_jacobian && c->method == SORS_MINIMIZE_NEWTON) {
+            const int* mk = c->has_mask ? c->mask.as<int>() : nullptr;
+            k_pressure_jacobian<HT><<<grid_for(c->N, kBlock), kBlock, 0, c->stream>>>(
+                press_args(c), q, c->sbase.as<int>(), c->bcol.as<int>(), c->selfslot.as<int>(), mk, a.hscale,
+                (HT*)c->vals.p, c->F());
+        }
     }
     if (c->has_contact && c->cargs.nsurf > 0) {
         const bool blk = c->block();
@@ -3614,6 +3795,13 @@ void enqueue_energy(sors_ctx* c, const double* q, const double* dx, double alpha
         k_elem_energy<8><<<ge, kBlock, 0, t.st>>>(a, diff, t.out3, red);
     k_finalize<1><<<1, kBlock, 0, t.st>>>(t.partials, ge, t.out3);
     NodeArgs na = node_args(c, q, dt);
+    if (c->has_pressure) {
+        pressure_force(c, q, nullptr, 0.0, c->fp0.as<double>(), t.st);
+        if (diff) {
+            pressure_force(c, q, dx, alpha, c->fp1.as<double>(), t.st);
+            na.fpres1 = c->fp1.as<double>();
+        }
+    }
     const int gnn = red_grid(c->n, kBlock);
     k_node_energy<<<gnn, kBlock, 0, t.st>>>(na, dx, alpha, diff, t.out3 + 1, red);
     k_finalize<1><<<1, kBlock, 0, t.st>>>(t.partials, gnn, t.out3 + 1);
@@ -4879,6 +5067,8 @@ int sors_energy_noalloc(sors_ctx* c, const double* q_dev, double dt, double* ene
     SORS_GUARD(c)
     if (!q_dev || !energy_dev) return fail(c, SORS_ERR_ARG, "null argument");
     if (!c->scratch_dev) return fail(c, SORS_ERR_HIP, "static scratch unavailable: " + c->scratch_err);
+    if (c->has_pressure)
+        return fail(c, SORS_ERR_UNSUPPORTED, "pressure forces need the context's own buffers (use sors_energy)");
     // the three reductions run with red_grid() blocks each, capped at
     // kMaxRedBlocks; the scratch holds kScratchPartials partials
     const long long work = std::max<long long>(std::max<long long>(c->E, c->n), c->has_contact ? c->cargs.nsurf : 0);
@@ -4994,11 +5184,17 @@ int sors_hessian(sors_ctx* c, const double* q, double dt, int precision, int pro
     ensure_pattern(c, dt);
     clear_flag(c);
     const long long nnz = c->pat.nnz;
+    struct ExportReset {
+        sors_ctx* c;
+        ~ExportReset() { c->export_pressure_jacobian = false; }
+    } export_reset{c};
+    c->export_pressure_jacobian = true;
     if (precision == SORS_MIXED) {
         assemble<float>(c, c->qtmp.as<double>(), dt, true, project);
     } else {
         assemble<double>(c, c->qtmp.as<double>(), dt, true, project);
     }
+    c->export_pressure_jacobian = false;
     sync(c);
     if (c->block()) {
         DevMem out;
@@ -5024,7 +5220,9 @@ int sors_hessian(sors_ctx* c, const double* q, double dt, int precision, int pro
         HIPX(hipMemcpyAsync(vals, c->vals.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, c->stream));
         sync(c);
     }
-    if (read_flag(c) & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
+    const int fl = read_flag(c);
+    if (fl & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
+    if (fl & 4) return fail(c, SORS_ERR_ARG, "a pressure face joins vertices that share no element");
     return SORS_OK;
     SORS_GUARD_END(c)
 }
@@ -5347,6 +5545,66 @@ int sors_set_muscle_activation(sors_ctx* c, const double* act) {
     SORS_GUARD_END(c)
 }
 
+int sors_set_pressure(sors_ctx* c, const sors_pressure_desc* d) {
+    SORS_GUARD(c)
+    if (!d || d->n_faces <= 0) {
+        c->has_pressure = false;
+        c->pnf = 0;
+        return SORS_OK;
+    }
+    const int nv = c->ed == 8 ? 4 : 3;
+    if (c->ed != 4 && c->ed != 8) return fail(c, SORS_ERR_ARG, "pressure needs a tet4 or hex8 mesh");
+    if (!d->faces || !d->groups) return fail(c, SORS_ERR_ARG, "null faces or groups");
+    if (d->n_groups < 1) return fail(c, SORS_ERR_ARG, "n_groups must be >= 1");
+    if (!c->block())
+        return fail(c, SORS_ERR_UNSUPPORTED, "pressure needs the node-block layout (node-granular Dirichlet mask)");
+    const int nf = d->n_faces;
+    std::vector<int> soa((size_t)nv * nf), grp(d->groups, d->groups + nf);
+    std::vector<int> cnt((size_t)c->N + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+        if (grp[f] < 0 || grp[f] >= d->n_groups) return fail(c, SORS_ERR_ARG, "surface group out of range");
+        for (int j = 0; j < nv; ++j) {
+            const int v = d->faces[(size_t)nv * f + j];
+            if (v < 0 || v >= c->N) return fail(c, SORS_ERR_ARG, "surface vertex index out of range");
+            for (int i = 0; i < j; ++i)
+                if (d->faces[(size_t)nv * f + i] == v) return fail(c, SORS_ERR_ARG, "a face lists a vertex twice");
+            soa[(size_t)j * nf + f] = v;
+            cnt[(size_t)v + 1]++;
+        }
+    }
+    // node -> faces, ascending face index per node
+    for (int v = 0; v < c->N; ++v) cnt[(size_t)v + 1] += cnt[v];
+    std::vector<int> inc((size_t)cnt[c->N]), pos(cnt.begin(), cnt.end() - 1);
+    for (int f = 0; f < nf; ++f)
+        for (int j = 0; j < nv; ++j) inc[(size_t)pos[soa[(size_t)j * nf + f]]++] = f;
+    upload_vec(c->pfaces, soa, c->stream);
+    upload_vec(c->pgroup, grp, c->stream);
+    upload_vec(c->pincptr, cnt, c->stream);
+    upload_vec(c->pinc, inc, c->stream);
+    std::vector<double> zero((size_t)d->n_groups, 0.0);
+    upload_vec(c->pact, zero, c->stream);
+    ensure(c->pff, sizeof(double) * 3 * (size_t)nf);
+    ensure(c->fp0, sizeof(double) * c->n);
+    ensure(c->fp1, sizeof(double) * c->n);
+    c->pnf = nf;
+    c->pnv = nv;
+    c->pngroups = d->n_groups;
+    c->has_pressure = true;
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
+int sors_set_pressure_activation(sors_ctx* c, const double* p) {
+    SORS_GUARD(c)
+    if (!c->has_pressure) return fail(c, SORS_ERR_ARG, "no pressure force set");
+    if (!p) return fail(c, SORS_ERR_ARG, "null pressures");
+    upload(c->pact, p, (size_t)c->pngroups, c->stream);
+    sync(c);
+    return SORS_OK;
+    SORS_GUARD_END(c)
+}
+
 int sors_set_youngs(sors_ctx* c, const double* youngs) {
     SORS_GUARD(c)
     if (!youngs) return fail(c, SORS_ERR_ARG, "null Young's moduli");
@@ -5386,6 +5644,7 @@ int sors_adjoint_step(sors_ctx* c, const double* q_prev, const double* v_prev, c
     if (c->has_contact && (c->cargs.mu > 0.0 || c->cargs.ndisks > 0))
         return fail(c, SORS_ERR_UNSUPPORTED, "the adjoint takes frictionless plane contact only");
     if (c->has_muscle) return fail(c, SORS_ERR_UNSUPPORTED, "the adjoint does not take the muscle energy");
+    if (c->has_pressure) return fail(c, SORS_ERR_UNSUPPORTED, "the adjoint does not take pressure forces");
     const int n = c->n, E = c->E;
     ensure_pattern(c, dt);
     upload(c->qprev, q_prev, (size_t)n, c->stream);

@@ -81,6 +81,10 @@ class MuscleDesc(C.Structure):
     _fields_ = [("n_groups", C.c_int), ("elem_group", _ip), ("direction", _dp), ("stiffness", _dp)]
 
 
+class PressureDesc(C.Structure):
+    _fields_ = [("n_faces", C.c_int), ("faces", _ip), ("groups", _ip), ("n_groups", C.c_int)]
+
+
 class ServerOptions(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_double), ("max_active", C.c_int), ("prep_threads", C.c_int),
@@ -163,6 +167,10 @@ def lib():
         L.sors_set_youngs.argtypes = [vp, _dp]
         L.sors_set_muscle.argtypes = [vp, C.POINTER(MuscleDesc)]
         L.sors_set_muscle_activation.argtypes = [vp, _dp]
+        L.sors_set_pressure.argtypes = [vp, C.POINTER(PressureDesc)]
+        L.sors_set_pressure_activation.argtypes = [vp, _dp]
+        L.sors_update_state.argtypes = [vp, _dp, C.c_double]
+        L.sors_minimize.argtypes = [vp, _dp, C.c_double, C.c_int, C.POINTER(Stats)]
         L.sors_set_state.argtypes = [vp, _dp, _dp]
         L.sors_adjoint_step.argtypes = [vp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
                                         C.c_int, C.POINTER(C.c_longlong)]
@@ -323,6 +331,35 @@ class Rollout:
         a = f64(act)
         assert a.size == self.muscle_groups
         self._check(lib().sors_set_muscle_activation(self.h, _d(a)))
+
+    def set_pressure(self, faces, groups, n_groups=None):
+        """PressureForce on faces (F x 3 on tet meshes, F x 4 on hex meshes),
+        each in an actuation group; faces None removes it."""
+        if faces is None:
+            self._check(lib().sors_set_pressure(self.h, None))
+            return
+        f = i32(faces)
+        g = i32(groups).reshape(-1)
+        self.pressure_groups = int(n_groups) if n_groups is not None else int(g.max()) + 1
+        self._pressure_keep = (f, g)
+        d = PressureDesc(int(g.size), _i(f), _i(g), self.pressure_groups)
+        self._check(lib().sors_set_pressure(self.h, C.byref(d)))
+
+    def set_pressure_activation(self, p):
+        a = f64(p).reshape(-1)
+        assert a.size == self.pressure_groups
+        self._check(lib().sors_set_pressure_activation(self.h, _d(a)))
+
+    def minimize(self, q, dt, method=None):
+        """Solver::minimize_* from the guess q (no state update)."""
+        q = f64(q).reshape(-1).copy()
+        st = Stats()
+        m = MINIMIZE_NEWTON_LINESEARCH if method is None else method
+        self._check(lib().sors_minimize(self.h, _d(q), float(dt), int(m), C.byref(st)))
+        return q, st
+
+    def update_state(self, q, dt):
+        self._check(lib().sors_update_state(self.h, _d(f64(q).reshape(-1)), float(dt)))
 
     def set_youngs(self, E):
         e = f64(E)
