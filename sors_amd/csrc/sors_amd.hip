@@ -2407,78 +2407,162 @@ __global__ void k_coarse_pad(int nc, int npad, const double* __restrict__ E, dou
     X[t] = v;
 }
 
-// One elimination step with pivot rows / columns K = [k0, k0 + kGjPivot):
-//   out[K, K] = P^-1                      (P = in[K, K])
-//   out[K, j] = P^-1 in[K, j]             (j not in K)
-//   out[i, K] = -in[i, K] P^-1            (i not in K)
-//   out[i, j] = in[i, j] - in[i, K] P^-1 in[K, j]
-// Each workgroup inverts P itself (Gauss-Jordan on the 32 x 32 block in LDS)
-// and forms one 64 x 64 output tile.  A non-positive pivot sets *info.
-__global__ __launch_bounds__(kGjThreads) void k_spd_gj_step(int npad, int k0, const double* __restrict__ in,
-                                                            double* __restrict__ out, int* info) {
-    constexpr int NB = kGjPivot, TB = kGjTile;
-    __shared__ double P[NB][NB + 1];   // pivot block, then its inverse
-    __shared__ double B[NB][TB + 1];   // in[K, J], then P^-1 in[K, J] (P^-1 on the K columns)
-    __shared__ double A[TB][NB + 1];   // in[I, K]
-    const int tiles = npad / TB;
-    const int ti = blockIdx.x / tiles, tj = blockIdx.x % tiles;
-    const int i0 = ti * TB, j0 = tj * TB, tid = threadIdx.x;
-    for (int e = tid; e < NB * NB; e += kGjThreads) P[e / NB][e % NB] = in[(size_t)(k0 + e / NB) * npad + k0 + e % NB];
-    for (int e = tid; e < NB * TB; e += kGjThreads) B[e / TB][e % TB] = in[(size_t)(k0 + e / TB) * npad + j0 + e % TB];
-    for (int e = tid; e < TB * NB; e += kGjThreads) A[e / NB][e % NB] = in[(size_t)(i0 + e / NB) * npad + k0 + e % NB];
-    __syncthreads();
-    // P^-1 in place: new[r][c] from the old values of step t
+// In-place Gauss-Jordan inverse of the SPD 32 x 32 block held in P[0] (two
+// LDS copies, so each elimination step costs one barrier; no pivoting, P is
+// a Schur complement of E).  The inverse ends in P[kGjPivot & 1]; returns
+// false on a non-positive pivot.
+__device__ bool gj_invert32(double (*P)[kGjPivot][kGjPivot + 1]) {
+    constexpr int NB = kGjPivot, kPer = NB * NB / kGjThreads;
+    const int tid = threadIdx.x;
     bool bad = false;
     for (int t = 0; t < NB; ++t) {
-        const double piv = P[t][t];
+        const int cur = t & 1;
+        const double piv = P[cur][t][t];
         bad = bad || !(piv > 0.0);
         const double rp = 1.0 / piv;
-        double v[NB * NB / kGjThreads];
 #pragma unroll
-        for (int q = 0; q < NB * NB / kGjThreads; ++q) {
+        for (int q = 0; q < kPer; ++q) {
             const int e = tid + q * kGjThreads, r = e / NB, c = e % NB;
-            const double prt = P[r][t], ptc = P[t][c];
+            const double prt = P[cur][r][t], ptc = P[cur][t][c];
+            double v;
             if (r == t)
-                v[q] = c == t ? rp : ptc * rp;
+                v = c == t ? rp : ptc * rp;
             else
-                v[q] = c == t ? -prt * rp : P[r][c] - (prt * ptc) * rp;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < NB * NB / kGjThreads; ++q) {
-            const int e = tid + q * kGjThreads;
-            P[e / NB][e % NB] = v[q];
+                v = c == t ? -prt * rp : P[cur][r][c] - (prt * ptc) * rp;
+            P[cur ^ 1][r][c] = v;
         }
         __syncthreads();
     }
-    if (bad && blockIdx.x == 0 && tid == 0) *info = 1;
-    // B = P^-1 in[K, J]; on the columns of K, P^-1 itself
-    {
-        constexpr int kPer = NB * TB / kGjThreads;
+    return !bad;
+}
+
+// P^-1 of the first pivot block in[0:32, 0:32] (the later ones come from
+// k_spd_gj_step).  A non-positive pivot sets *info.
+__global__ __launch_bounds__(kGjThreads) void k_gj_pivot(int npad, const double* __restrict__ in,
+                                                         double* __restrict__ pinv, int* info) {
+    constexpr int NB = kGjPivot;
+    __shared__ double P[2][NB][NB + 1];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < NB * NB; e += kGjThreads) P[0][e / NB][e % NB] = in[(size_t)(e / NB) * npad + e % NB];
+    __syncthreads();
+    if (!gj_invert32(P) && tid == 0) *info = 1;
+    for (int e = tid; e < NB * NB; e += kGjThreads) pinv[e] = P[NB & 1][e / NB][e % NB];
+}
+
+// One elimination step with pivot rows / columns K = [k0, k0 + kGjPivot),
+// given P^-1 (pinv):
+//   out[K, K] = P^-1
+//   out[K, j] = P^-1 in[K, j]             (j not in K)
+//   out[i, K] = -in[i, K] P^-1            (i not in K)
+//   out[i, j] = in[i, j] - in[i, K] P^-1 in[K, j]
+// Workgroups 1.. form one 64 x 64 output tile each.  Workgroup 0 forms the
+// next pivot block K' = K + 32 of out by the same operations in the same
+// order as its tile's workgroup (so the two agree bit for bit), inverts it
+// and writes pinv_next, beside the tiles instead of after them.  A
+// non-positive pivot sets *info.
+__global__ __launch_bounds__(kGjThreads) void k_spd_gj_step(int npad, int k0, const double* __restrict__ in,
+                                                            const double* __restrict__ pinv,
+                                                            double* __restrict__ pinv_next,
+                                                            double* __restrict__ out, int* info) {
+    constexpr int NB = kGjPivot, TB = kGjTile;
+    __shared__ double P[NB][NB + 1];   // P^-1
+    __shared__ double B[NB][TB + 1];   // in[K, J], then P^-1 in[K, J] (P^-1 on the K columns)
+    __shared__ double A[TB][NB + 1];   // in[I, K]
+    const int tid = threadIdx.x;
+    for (int e = tid; e < NB * NB; e += kGjThreads) P[e / NB][e % NB] = pinv[e];
+    if (blockIdx.x == 0) {
+        const int k1 = k0 + NB;
+        if (k1 >= npad) return;
+        // in[K, K'] and in[K', K]; then B1 = P^-1 in[K, K'] column by column
+        double (*S)[NB][NB + 1] = reinterpret_cast<double (*)[NB][NB + 1]>(&A[0][0]);  // two 32 x 33 copies
+        for (int e = tid; e < NB * NB; e += kGjThreads) {
+            B[e / NB][e % NB] = in[(size_t)(k0 + e / NB) * npad + k1 + e % NB];
+            S[1][e / NB][e % NB] = in[(size_t)(k1 + e / NB) * npad + k0 + e % NB];
+        }
+        __syncthreads();
+        constexpr int kPer = NB * NB / kGjThreads;
+        const int c = tid % NB, r0 = tid / NB;  // rows r0, r0 + 8, ... of column c
         double v[kPer];
+        {
+            double col[NB];
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            const int e = tid + q * kGjThreads, r = e / TB, c = e % TB;
-            const int j = j0 + c;
-            if (j >= k0 && j < k0 + NB) {
-                v[q] = P[r][j - k0];
-            } else {
+            for (int u = 0; u < NB; ++u) col[u] = B[u][c];
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                const int r = r0 + q * (kGjThreads / NB);
                 double acc = 0.0;
-#pragma unroll 8
-                for (int u = 0; u < NB; ++u) acc += P[r][u] * B[u][c];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) acc += P[r][u] * col[u];
                 v[q] = acc;
             }
         }
         __syncthreads();
 #pragma unroll
+        for (int q = 0; q < kPer; ++q) B[r0 + q * (kGjThreads / NB)][c] = v[q];
+        __syncthreads();
+        // S = in[K', K'] - in[K', K] B1, each entry's sum over t ascending
+#pragma unroll
         for (int q = 0; q < kPer; ++q) {
-            const int e = tid + q * kGjThreads;
-            B[e / TB][e % TB] = v[q];
+            const int r = r0 + q * (kGjThreads / NB);
+            double acc = 0.0;
+            for (int t = 0; t < NB; ++t) acc += S[1][r][t] * B[t][c];
+            v[q] = in[(size_t)(k1 + r) * npad + k1 + c] - acc;
         }
         __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) S[0][r0 + q * (kGjThreads / NB)][c] = v[q];
+        __syncthreads();
+        if (!gj_invert32(S) && tid == 0) *info = 1;
+        for (int e = tid; e < NB * NB; e += kGjThreads) pinv_next[e] = S[NB & 1][e / NB][e % NB];
+        return;
     }
-    // the tile: rows of K take B; the others in[i, j] (0 on the K columns) - A B
+    const int tiles = npad / TB;
+    const int tile = blockIdx.x - 1;
+    const int ti = tile / tiles, tj = tile % tiles;
+    const int i0 = ti * TB, j0 = tj * TB;
+    for (int e = tid; e < NB * TB; e += kGjThreads) B[e / TB][e % TB] = in[(size_t)(k0 + e / TB) * npad + j0 + e % TB];
+    for (int e = tid; e < TB * NB; e += kGjThreads) A[e / NB][e % NB] = in[(size_t)(i0 + e / NB) * npad + k0 + e % NB];
+    __syncthreads();
+    // B = P^-1 in[K, J]; on the columns of K, P^-1 itself.  A thread's eight
+    // outputs share one column: that column of in[K, J] is read once
+    {
+        constexpr int kPer = NB * TB / kGjThreads;  // rows r0, r0 + 4, ... of one column c
+        const int c = tid % TB, r0 = tid / TB;
+        const int j = j0 + c;
+        const bool colK = j >= k0 && j < k0 + NB;
+        double col[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) col[u] = B[u][c];
+        double v[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int r = r0 + q * (kGjThreads / TB);
+            if (colK) {
+                v[q] = P[r][j - k0];
+            } else {
+                double acc = 0.0;
+#pragma unroll
+                for (int u = 0; u < NB; ++u) acc += P[r][u] * col[u];
+                v[q] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) B[r0 + q * (kGjThreads / TB)][c] = v[q];
+        __syncthreads();
+    }
+    // the tile: rows of K take B; the others in[i, j] (0 on the K columns) - A B.
+    // The 16 values of in[i, j] are loaded before the product, all in flight at once
     const int tx = tid % 16, ty = tid / 16;
+    double base[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
+            const bool skip = (i >= k0 && i < k0 + NB) || (j >= k0 && j < k0 + NB);
+            base[a][b] = skip ? 0.0 : in[(size_t)i * npad + j];
+        }
     double acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -2504,12 +2588,10 @@ __global__ __launch_bounds__(kGjThreads) void k_spd_gj_step(int npad, int k0, co
         for (int b = 0; b < 4; ++b) {
             const int c = tx + 16 * b, j = j0 + c;
             double v;
-            if (rowK) {
+            if (rowK)
                 v = B[i - k0][c];
-            } else {
-                const double base = (j >= k0 && j < k0 + NB) ? 0.0 : in[(size_t)i * npad + j];
-                v = base - acc[a][b];
-            }
+            else
+                v = base[a][b] - acc[a][b];
             out[(size_t)i * npad + j] = v;
         }
     }
@@ -2992,6 +3074,7 @@ struct sors_ctx {
     int npad = 0;                   // nc rounded up to the Gauss-Jordan tile
     DevMem cdn, captr, canodes, crot, cpairI, cpairJ, cpchunk, cgptr, cci, ccj, ccbase;
     DevMem cE, cEinv, cX, cY, crc, cgpart, cpart, cinfo;   // cX / cY: Gauss-Jordan ping-pong (npad^2)
+    DevMem cpinv;                   // pivot block inverses of the current and the next step (2 x 32 x 32)
     const double* cres = nullptr;   // the buffer holding E^-1 after the last elimination
     bool vold_valid = false;
     double vold_dt = 0.0;
@@ -3429,6 +3512,7 @@ void ensure_coarse(sors_ctx* c) {
     ensure(c->cEinv, sizeof(double) * nc2);
     ensure(c->cX, sizeof(double) * np2);
     ensure(c->cY, sizeof(double) * np2);
+    ensure(c->cpinv, sizeof(double) * 2 * kGjPivot * kGjPivot);
     ensure(c->crc, sizeof(double) * std::max(c->nc, 1));
     ensure(c->cgpart, sizeof(double) * 36 * std::max(c->ngchunks, 1));
     ensure(c->cpart, sizeof(double) * 2 * kMaxRedBlocks);
@@ -3458,9 +3542,13 @@ void coarse_factor_async(sors_ctx* c) {
                                                                                    c->cX.as<double>());
     double* in = c->cX.as<double>();
     double* out = c->cY.as<double>();
+    double* pinv = c->cpinv.as<double>();  // two 32 x 32 inverses, by step parity
     const unsigned tiles = (unsigned)(np / kGjTile);
-    for (int k0 = 0; k0 < np; k0 += kGjPivot) {
-        k_spd_gj_step<<<tiles * tiles, kGjThreads, 0, c->side>>>(np, k0, in, out, c->cinfo.as<int>());
+    k_gj_pivot<<<1, kGjThreads, 0, c->side>>>(np, in, pinv, c->cinfo.as<int>());
+    for (int k0 = 0, s = 0; k0 < np; k0 += kGjPivot, ++s) {
+        double* pc = pinv + (s & 1) * kGjPivot * kGjPivot;
+        double* pn = pinv + ((s + 1) & 1) * kGjPivot * kGjPivot;
+        k_spd_gj_step<<<tiles * tiles + 1, kGjThreads, 0, c->side>>>(np, k0, in, pc, pn, out, c->cinfo.as<int>());
         std::swap(in, out);
     }
     c->cres = in;
@@ -4120,7 +4208,7 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
                            &c->incptr, &c->inc,  &c->teg,      &c->teb,    &c->tsptr,   &c->tsidx,   &c->tqslice, &c->vold,     &c->cdn,    &c->captr,   &c->canodes, &c->crot,
                            &c->cpairI, &c->cpairJ, &c->cpchunk, &c->cgptr, &c->cci,     &c->ccj,     &c->ccbase,
-                           &c->cE,     &c->cEinv, &c->crc,     &c->cX,      &c->cY,      &c->cgpart,  &c->cpart,   &c->cinfo,
+                           &c->cE,     &c->cEinv, &c->crc,     &c->cX,      &c->cY,      &c->cpinv,   &c->cgpart,  &c->cpart,   &c->cinfo,
                            &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA,   &c->mgroup, &c->mdir,
                            &c->mk,     &c->mact};
     double b = 0;
