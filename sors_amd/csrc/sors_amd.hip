@@ -600,10 +600,13 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
                 for (int c = 0; c < 3; ++c) P(r, c) += sa * Ff[r] * fa[c];
 #pragma unroll
         for (int la = 0; la < 4; ++la) {
-            double* ge = eg + 4 * elem_row(e, la, 4);
+            double gr[3];
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
-                ge[r] = V0 * (P(r, 0) * g[la][0] + P(r, 1) * g[la][1] + P(r, 2) * g[la][2]);
+            for (int r = 0; r < 3; ++r) gr[r] = V0 * (P(r, 0) * g[la][0] + P(r, 1) * g[la][1] + P(r, 2) * g[la][2]);
+            // the padded 4-double row as two 16-byte stores
+            double2* ge = reinterpret_cast<double2*>(eg + 4 * elem_row(e, la, 4));
+            ge[0] = make_double2(gr[0], gr[1]);
+            ge[1] = make_double2(gr[2], 0.0);
         }
     }
     if (MODE == 1) return;
@@ -654,8 +657,17 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
                 blk[4] += s_ab;
                 blk[8] += s_ab;
             }
+            HT* dst = eb + (size_t)tet_blk<HT>() * elem_row(e, t, 10);
+            if constexpr (sizeof(HT) == 4) {
+                // the padded 12-float block as three 16-byte stores
+                float4* d4 = reinterpret_cast<float4*>(dst);
+                d4[0] = make_float4(blk[0], blk[1], blk[2], blk[3]);
+                d4[1] = make_float4(blk[4], blk[5], blk[6], blk[7]);
+                d4[2] = make_float4(blk[8], 0.0f, 0.0f, 0.0f);
+            } else {
 #pragma unroll
-            for (int i = 0; i < 9; ++i) eb[tet_blk<HT>() * elem_row(e, t, 10) + i] = blk[i];
+                for (int i = 0; i < 9; ++i) dst[i] = blk[i];
+            }
             ++t;
         }
 }
