@@ -99,8 +99,8 @@ def test_slot_lists_and_incidence(dump_tool, tmp_path, case):
         t = 4 * a_ - a_ * (a_ - 1) // 2 + b_ - a_
         upper = [(x, y) for x in range(4) for y in range(x, 4)]
         assert [4 * x - x * (x - 1) // 2 + y - x for x, y in upper] == list(range(10))
-        # stored blocks are tiled by 64 elements: block t of element e at ((e // 64) 10 + t) 64 + e % 64
-        code = 2 * (((e_idx >> 6) * 10 + t) * 64 + (e_idx & 63)) + (la > lb)
+        # an element's ten stored blocks are contiguous: block t of element e at 10 e + t
+        code = 2 * (10 * e_idx + t) + (la > lb)
     else:
         # hex8: all 64 blocks, tiled by 64 elements: block 8 la + lb of element e
         code = ((e_idx >> 6) * ed * ed + ed * la + lb) * 64 + (e_idx & 63)
