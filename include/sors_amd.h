@@ -34,8 +34,17 @@ extern "C" {
 /* SimulationSettings::timeSteppingScheme (cpp/headers/simulationSettings.h:31) */
 enum { SORS_BACKWARD_EULER = 0, SORS_CRANK_NICOLSON = 1 };
 /* SimulationSettings::solverMethod (cpp/src/solver.cpp:28-40).  The line
- * search variant keeps minimize_newton's Dirichlet handling; SQP selects the
- * unmasked assembly semantics (energy.cpp:275,337) for inspection calls.
+ * search variant keeps minimize_newton's Dirichlet handling.
+ * SQP (minimize_SQP, solver.cpp:109-197): the unmasked assembly semantics
+ * (energy.cpp:275,337) with the Dirichlet mask as NeumannBC equality rows
+ * x - xPrev - dt v = 0 (neumannBC.cpp:4-40) and the planes and disks of
+ * sors_set_contact as the PlaneContact / DiskContact inequality rows
+ * n.(x - p) >= 0 on their active surface vertices (planeContact.cpp,
+ * diskContact.cpp; active sets updated once per step from the initial guess,
+ * the disk footprint frozen between steps as the reference does); each SQP
+ * iteration solves its QP exactly (working-set iterations of projected PCG
+ * solves, where the reference calls OSQP) and takes the full step, until
+ * ||dx|| < dx_tol; no predictor (the caller's guess is the start).
  * FROZENHESSIAN (minimize_frozenhessian, solver.cpp:268-294): the tangent of
  * the step's initial guess, and its preconditioner, serve every iteration;
  * each iteration assembles the gradient alone; no line search; stopping after
@@ -138,6 +147,7 @@ typedef struct {
     long long assemble_launches; /* tangent assemblies timed; gradient-only residual checks are not */
     int fp64_fallbacks;   /* mixed-precision Newton solves that did not converge within max_newton
                              and were continued from their iterate in FP64 (see sors_step) */
+    int qp_iters;         /* minimize_SQP: working-set iterations (one projected PCG solve each) */
 } sors_stats;
 
 typedef struct sors_ctx sors_ctx;
@@ -250,7 +260,10 @@ int sors_bench_kernel(sors_ctx* ctx, int kernel, int reps, double* avg_ms, doubl
  * Parameters keep the reference's names: positionVectorListPlanes /
  * normalVectorListPlanes, diskCenterVectorList / diskNormalVectorList /
  * diskRadiusList, surfaceVerticesIdx.  Normals are normalised.  Applies to
- * minimize_newton(_linesearch); masked DOFs take no contact terms. */
+ * minimize_newton(_linesearch) and the frozen Hessian; masked DOFs take no
+ * contact terms.  Under SORS_MINIMIZE_SQP the same primitives are the
+ * reference's hard constraints instead (see the solverMethod enum) and
+ * stiffness, friction and eps_v are not used. */
 typedef struct {
     int n_planes;                   /* <= 4 */
     const double* plane_point;      /* [n_planes][3] */
@@ -348,7 +361,8 @@ int sors_update_state(sors_ctx* ctx, const double* q, double dt);
  * energy from the initial guess q, returned in q; the integrator state is not
  * updated (Solver::step = minimize + update_state).  A mixed-precision solve
  * that reaches max_newton goes on in FP64, as in sors_step.  minimize_SQP
- * maps to minimize_newton_linesearch in the facade (contact is a penalty). */
+ * (method SORS_MINIMIZE_SQP) updates the contact active sets from q first,
+ * as the reference's minimize_SQP does. */
 int sors_minimize(sors_ctx* ctx, double* q, double dt, int method, sors_stats* stats);
 /* TimeIntegrator state (integrator.h:101-102): qPrev = q (and the positions),
  * vPrev = v; either may be NULL to keep it */

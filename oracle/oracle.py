@@ -137,6 +137,9 @@ def lib():
         L.orc_pressure_force.restype = None
         L.orc_pressure_jacobian.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_pressure_jacobian.restype = None
+        L.orc_qp_solve.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
+                                   C.POINTER(C.c_int)]
+        L.orc_sqp_active.argtypes = [C.c_void_p, _dp, _ip, _ip, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -492,3 +495,27 @@ class System:
     def update_state(self, q, dt):
         q = f64(q).reshape(-1)
         lib().orc_update_state(self.h, _d(q), float(dt))
+
+    def sqp_active(self, q):
+        """PlaneContact / DiskContact update_active_constraints at q: (plane
+        index per surface vertex or -1, captured disk or -1, frozen flag)."""
+        ns = len(self._contact_keep[5]) if self._contact_keep[5] is not None else self.N
+        pm, dm, fr = np.zeros(ns, np.int32), np.zeros(ns, np.int32), C.c_int(0)
+        if lib().orc_sqp_active(self.h, _d(f64(q).reshape(-1)), _i(pm), _i(dm), C.byref(fr)) != 0:
+            raise ValueError("orc_sqp_active: no contact set")
+        return pm, dm, int(fr.value)
+
+
+def qp_solve(rowptr, cols, P, q, vtx, a, l, u):
+    """min 1/2 x^T P x + q^T x s.t. l <= A x <= u, row i of A acting on the
+    DOFs of vertex vtx[i] with coefficients a[i] (OSQP's ADMM with polishing,
+    run to the polished KKT point).  Returns (x, y, admm_iters)."""
+    rowptr, cols, P, q = i32(rowptr), i32(cols), f64(P), f64(q)
+    vtx, a, l, u = i32(vtx), f64(a).reshape(-1), f64(l), f64(u)
+    n, m = len(rowptr) - 1, len(vtx)
+    x, y, it = np.zeros(n), np.zeros(max(m, 1)), C.c_int(0)
+    code = lib().orc_qp_solve(n, _i(rowptr), _i(cols), _d(P), _d(q), m, _i(vtx), _d(a), _d(l), _d(u), _d(x), _d(y),
+                              C.byref(it))
+    if code:
+        raise RuntimeError(ERRORS.get(code, code))
+    return x, y[:m], int(it.value)

@@ -599,6 +599,12 @@ struct orc_sys {
     double* lagN;   /* [nsurf] lagged normal force magnitude */
     double* lagn;   /* [nsurf][3] its normal */
     double* lagD;   /* [nsurf][3] displacement of its primitive over the step */
+    /* minimize_SQP active sets: closest plane / captured disk per surface
+     * vertex or -1 (planeCollisionMask_, diskCollisionMask_) and the disk
+     * footprint's frozen flag (frozenActiveSet_) */
+    int* pmask;
+    int* dmask;
+    int dfrozen;
 };
 
 void orc_default_opts(orc_opts* o) {
@@ -705,6 +711,8 @@ void orc_free(orc_sys* s) {
     free(s->lagN);
     free(s->lagn);
     free(s->lagD);
+    free(s->pmask);
+    free(s->dmask);
     free(s);
 }
 
@@ -984,6 +992,12 @@ int orc_set_contact(orc_sys* s, int nplanes, const double* pp, const double* pn,
     s->lagN = (double*)calloc(s->nsurf > 0 ? s->nsurf : 1, sizeof(double));
     s->lagn = (double*)calloc(3 * (size_t)(s->nsurf > 0 ? s->nsurf : 1), sizeof(double));
     s->lagD = (double*)calloc(3 * (size_t)(s->nsurf > 0 ? s->nsurf : 1), sizeof(double));
+    free(s->pmask);
+    free(s->dmask);
+    s->pmask = (int*)malloc(sizeof(int) * (s->nsurf > 0 ? s->nsurf : 1));
+    s->dmask = (int*)malloc(sizeof(int) * (s->nsurf > 0 ? s->nsurf : 1));
+    for (int i = 0; i < s->nsurf; ++i) s->pmask[i] = s->dmask[i] = -1;
+    s->dfrozen = 0;
     s->kc = stiffness;
     s->muf = friction;
     s->eps_v = eps_v;
@@ -1098,7 +1112,8 @@ static void project_psd3(double* H) {
 /* energy, and optionally gradient (added into g) / node blocks (into Hn,
  * [nsurf][9]) of all contact terms at q */
 static double contact_eval(const orc_sys* s, const double* q, double dt, double* g, double* Hn) {
-    if (!s->has_contact) return 0.0;
+    /* under minimize_SQP the planes and disks are constraints, not energy */
+    if (!s->has_contact || s->method == ORC_SQP) return 0.0;
     double e = 0;
     for (int i = 0; i < s->nsurf; ++i) {
         int node = s->surf[i];
@@ -1429,7 +1444,7 @@ int orc_hessian(orc_sys* s, const double* q, double dt, int project, double* val
                                                : (2 * m / (dt * dt) + 2 * s->alpha * m / dt);
         }
     }
-    if (s->has_contact) {
+    if (s->has_contact && s->method != ORC_SQP) {
         double* Hn = (double*)malloc(sizeof(double) * 9 * (size_t)(s->nsurf > 0 ? s->nsurf : 1));
         contact_eval(s, q, dt, NULL, Hn);
         for (int i = 0; i < s->nsurf; ++i)
@@ -1454,25 +1469,43 @@ int orc_hessian(orc_sys* s, const double* q, double dt, int project, double* val
 /* ------------------------------------------------------------------------ */
 
 /* Envelope (skyline) LDL^T without pivoting -- stands in for Eigen's
- * SimplicialLDLT (solver.cpp:65,82-84).  Returns ORC_ERR_SOLVE on a zero
+ * SimplicialLDLT (solver.cpp:65,82-84).  The factor is kept for repeated
+ * solves (the QP solver below).  ldl_factor returns ORC_ERR_SOLVE on a zero
  * pivot. */
-static int ldlt_solve(int n, const int* rowptr, const int* cols, const double* vals, const double* b, double* x) {
-    int* first = (int*)malloc(sizeof(int) * n);
-    long long* off = (long long*)malloc(sizeof(long long) * (n + 1));
+typedef struct {
+    int n;
+    int* first;      /* first column of row i's envelope */
+    long long* off;  /* row i's envelope in L */
+    double* L;
+    double* D;
+} ldl_fac;
+
+static void ldl_free(ldl_fac* f) {
+    free(f->first);
+    free(f->off);
+    free(f->L);
+    free(f->D);
+    memset(f, 0, sizeof(*f));
+}
+
+static int ldl_factor(int n, const int* rowptr, const int* cols, const double* vals, ldl_fac* f) {
+    f->n = n;
+    f->first = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+    f->off = (long long*)malloc(sizeof(long long) * (n + 1));
     for (int i = 0; i < n; ++i) {
-        int f = i;
+        int fi = i;
         for (int p = rowptr[i]; p < rowptr[i + 1]; ++p)
-            if (cols[p] < f) f = cols[p];
-        first[i] = f;
+            if (cols[p] < fi) fi = cols[p];
+        f->first[i] = fi;
     }
-    off[0] = 0;
-    for (int i = 0; i < n; ++i) off[i + 1] = off[i] + (i - first[i]);
-    double* L = (double*)calloc((size_t)off[n] + 1, sizeof(double));
-    double* D = (double*)malloc(sizeof(double) * n);
-    double* u = (double*)malloc(sizeof(double) * n);
+    f->off[0] = 0;
+    for (int i = 0; i < n; ++i) f->off[i + 1] = f->off[i] + (i - f->first[i]);
+    f->L = (double*)calloc((size_t)f->off[n] + 1, sizeof(double));
+    f->D = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+    double* u = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
     int status = ORC_OK;
     for (int i = 0; i < n && status == ORC_OK; ++i) {
-        int fi = first[i];
+        int fi = f->first[i];
         double aii = 0;
         for (int j = fi; j < i; ++j) u[j] = 0;
         for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
@@ -1480,44 +1513,52 @@ static int ldlt_solve(int n, const int* rowptr, const int* cols, const double* v
             if (c < i) u[c] = vals[p];
             if (c == i) aii = vals[p];
         }
-        double* Li = L + off[i] - fi; /* Li[j] for j in [fi, i) */
+        double* Li = f->L + f->off[i] - fi; /* Li[j] for j in [fi, i) */
         for (int j = fi; j < i; ++j) {
-            int fj = first[j];
+            int fj = f->first[j];
             int k0 = fi > fj ? fi : fj;
-            const double* Lj = L + off[j] - fj;
+            const double* Lj = f->L + f->off[j] - fj;
             double sum = u[j];
             for (int k = k0; k < j; ++k) sum -= u[k] * Lj[k];
             u[j] = sum;
         }
         double di = aii;
         for (int j = fi; j < i; ++j) {
-            Li[j] = u[j] / D[j];
+            Li[j] = u[j] / f->D[j];
             di -= u[j] * Li[j];
         }
         if (di == 0 || di != di) status = ORC_ERR_SOLVE;
-        D[i] = di;
+        f->D[i] = di;
     }
-    if (status == ORC_OK) {
-        for (int i = 0; i < n; ++i) x[i] = b[i];
-        for (int i = 0; i < n; ++i) {
-            const double* Li = L + off[i] - first[i];
-            double sum = x[i];
-            for (int j = first[i]; j < i; ++j) sum -= Li[j] * x[j];
-            x[i] = sum;
-        }
-        for (int i = 0; i < n; ++i) x[i] /= D[i];
-        for (int i = n - 1; i >= 0; --i) {
-            const double* Li = L + off[i] - first[i];
-            double xi = x[i];
-            for (int j = first[i]; j < i; ++j) x[j] -= Li[j] * xi;
-        }
-    }
-    free(first);
-    free(off);
-    free(L);
-    free(D);
     free(u);
+    if (status != ORC_OK) ldl_free(f);
     return status;
+}
+
+static void ldl_apply(const ldl_fac* f, const double* b, double* x) {
+    const int n = f->n;
+    for (int i = 0; i < n; ++i) x[i] = b[i];
+    for (int i = 0; i < n; ++i) {
+        const double* Li = f->L + f->off[i] - f->first[i];
+        double sum = x[i];
+        for (int j = f->first[i]; j < i; ++j) sum -= Li[j] * x[j];
+        x[i] = sum;
+    }
+    for (int i = 0; i < n; ++i) x[i] /= f->D[i];
+    for (int i = n - 1; i >= 0; --i) {
+        const double* Li = f->L + f->off[i] - f->first[i];
+        double xi = x[i];
+        for (int j = f->first[i]; j < i; ++j) x[j] -= Li[j] * xi;
+    }
+}
+
+static int ldlt_solve(int n, const int* rowptr, const int* cols, const double* vals, const double* b, double* x) {
+    ldl_fac f;
+    int status = ldl_factor(n, rowptr, cols, vals, &f);
+    if (status != ORC_OK) return status;
+    ldl_apply(&f, b, x);
+    ldl_free(&f);
+    return ORC_OK;
 }
 
 /* FP64 PCG with a 3x3 block-Jacobi preconditioner (the CPU form of the
@@ -1675,8 +1716,11 @@ static int orc_minimize_frozen(orc_sys* s, double* x, double dt, const orc_opts*
 /* Newton minimisation: minimize_newton (solver.cpp:56-105) and
  * minimize_newton_linesearch (:201-264, Armijo alpha0 = 1, beta = 0.5,
  * c = 0.1) with the minimize_newton Dirichlet handling. */
+static int orc_minimize_sqp(orc_sys* s, double* x, double dt, const orc_opts* o, orc_stats* st);
+
 int orc_minimize(orc_sys* s, double* x, double dt, const orc_opts* o, orc_stats* st) {
     if (s->method == ORC_FROZEN) return orc_minimize_frozen(s, x, dt, o, st);
+    if (s->method == ORC_SQP) return orc_minimize_sqp(s, x, dt, o, st);
     int n = s->n;
     orc_pattern(s, dt);
     double* g = (double*)malloc(sizeof(double) * n);
@@ -1775,6 +1819,543 @@ int orc_minimize(orc_sys* s, double* x, double dt, const orc_opts* o, orc_stats*
     free(rhs);
     free(xn);
     free(vals);
+    return status;
+}
+
+/* ===========================================================================
+ * minimize_SQP (solver.cpp:109-197) and its quadratic programs.
+ *
+ * Each SQP iteration of the reference hands the QP
+ *     min 1/2 dx^T H dx + g^T dx   s.t.  l <= A dx <= u
+ * to OSQP (default settings with scaling = 0 and max_iter = MAX_ITER_QP =
+ * 250, solver.cpp:117-122; common.h:23).  OSQP is a dependency absent from
+ * the reference tree (the empty submodule external/osqp, v1.0 API per the
+ * OSQPSolver / OSQPCscMatrix_new calls of solver.cpp:147-163).  Its published
+ * algorithm is restated here: the ADMM iteration of Stellato, Banjac,
+ * Goulart, Bemporad and Boyd, "OSQP: an operator splitting solver for
+ * quadratic programs", Math. Prog. Comp. 12 (2020), with its defaults
+ * sigma = 1e-6, alpha = 1.6, rho = 0.1 (rho x 1e3 on equality rows), the
+ * adaptive rho rule, and its solution polishing.  It is run until the
+ * polished solution passes the KKT checks rather than stopped at eps = 1e-3
+ * or 250 iterations: a strictly convex QP has one solution, which OSQP
+ * approximates and at which the SQP iteration stops.
+ *
+ * Every constraint row of the reference acts on one vertex (NeumannBC: a
+ * unit row per masked DOF; PlaneContact / DiskContact: the normal on a
+ * surface vertex), so a row is (vertex, 3 coefficients, l, u).
+ * ===========================================================================
+ */
+#define ORC_PLANE_THRESHOLD 0.01        /* PLANE_CONSTRAINT_THRESHOLD, common.h:29 */
+#define ORC_DISK_PLANE_THRESHOLD 0.01   /* DISK_PLANAR_CONSTRAINT_THRESHOLD, common.h:33 */
+#define ORC_DISK_RADIAL_THRESHOLD 0.001 /* DISK_RADIAL_CONSTRAINT_THRESHOLD, common.h:35 */
+
+static double vmax_abs(int n, const double* v) {
+    double m = 0;
+    for (int i = 0; i < n; ++i) {
+        double a = fabs(v[i]);
+        if (a > m) m = a;
+    }
+    return m;
+}
+
+static int csr_at(const int* rowptr, const int* cols, int R, int C) {
+    for (int p = rowptr[R]; p < rowptr[R + 1]; ++p)
+        if (cols[p] == C) return p;
+    return -1;
+}
+
+static void csr_mv(int n, const int* rowptr, const int* cols, const double* vals, const double* x, double* out) {
+    for (int i = 0; i < n; ++i) {
+        double acc = 0;
+        for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) acc += vals[p] * x[cols[p]];
+        out[i] = acc;
+    }
+}
+
+/* out = A x */
+static void rows_mv(int m, const int* vtx, const double* a, const double* x, double* out) {
+    for (int i = 0; i < m; ++i) {
+        const double* xv = x + 3 * (size_t)vtx[i];
+        out[i] = a[3 * i] * xv[0] + a[3 * i + 1] * xv[1] + a[3 * i + 2] * xv[2];
+    }
+}
+
+/* out += A^T y */
+static void rows_mtv(int m, const int* vtx, const double* a, const double* y, double* out) {
+    for (int i = 0; i < m; ++i)
+        for (int k = 0; k < 3; ++k) out[3 * (size_t)vtx[i] + k] += a[3 * i + k] * y[i];
+}
+
+/* P + sigma I + A^T diag(rho) A on P's pattern (a row's a a^T lands in its
+ * vertex's diagonal 3x3 block), factorised */
+static int admm_factor(int n, const int* rowptr, const int* cols, const double* P, double sigma, int m,
+                       const int* vtx, const double* a, const double* rho, double* work, ldl_fac* f) {
+    memcpy(work, P, sizeof(double) * (size_t)rowptr[n]);
+    for (int i = 0; i < n; ++i) {
+        int d = csr_at(rowptr, cols, i, i);
+        if (d < 0) return ORC_ERR_ARG;
+        work[d] += sigma;
+    }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                double v = rho[i] * a[3 * i + j] * a[3 * i + k];
+                if (v == 0.0) continue;
+                int p = csr_at(rowptr, cols, 3 * vtx[i] + j, 3 * vtx[i] + k);
+                if (p < 0) return ORC_ERR_ARG;
+                work[p] += v;
+            }
+    return ldl_factor(n, rowptr, cols, work, f);
+}
+
+/* OSQP's solution polishing: the QP restricted to the rows guessed active
+ * from the ADMM iterate (lower: z_i - l_i < -y_i, upper: u_i - z_i < y_i;
+ * equality rows always) is solved as the quasi-definite system
+ *   [P + d I, A_a^T; A_a, -d I] [x; y_a] = [-q; b_a]    (d = 1e-6)
+ * with iterative refinement against the same system without d.  Accepted
+ * (returns 1) when A x lies within [l, u] and each active row's multiplier
+ * has its bound's sign (y <= 0 on a lower bound, >= 0 on an upper one). */
+static int qp_polish(int n, const int* rowptr, const int* cols, const double* P, const double* q, int m,
+                     const int* vtx, const double* a, const double* l, const double* u, const double* z,
+                     const double* yk, double* xo, double* yo) {
+    const double delta = 1e-6;
+    int* side = (int*)calloc(m > 0 ? m : 1, sizeof(int)); /* -1 lower, +1 upper, 2 equality, 0 inactive */
+    int* aidx = (int*)malloc(sizeof(int) * (m > 0 ? m : 1));
+    int ma = 0;
+    for (int i = 0; i < m; ++i) {
+        if (l[i] == u[i])
+            side[i] = 2;
+        else if (z[i] - l[i] < -yk[i])
+            side[i] = -1;
+        else if (u[i] - z[i] < yk[i])
+            side[i] = 1;
+        if (side[i]) aidx[ma++] = i;
+    }
+    const int N2 = n + ma;
+    /* active rows per vertex, for the coupling columns of rows R < n */
+    const int nv = n / 3;
+    int* vcnt = (int*)calloc(nv + 1, sizeof(int));
+    for (int j = 0; j < ma; ++j) vcnt[vtx[aidx[j]] + 1]++;
+    for (int v = 0; v < nv; ++v) vcnt[v + 1] += vcnt[v];
+    int* vrows = (int*)malloc(sizeof(int) * (ma > 0 ? ma : 1));
+    int* vfill = (int*)malloc(sizeof(int) * (nv > 0 ? nv : 1));
+    for (int v = 0; v < nv; ++v) vfill[v] = vcnt[v];
+    for (int j = 0; j < ma; ++j) vrows[vfill[vtx[aidx[j]]]++] = j;
+    long long nnz = rowptr[n] + 2LL * 3 * ma + ma;
+    int* kp = (int*)malloc(sizeof(int) * (N2 + 1));
+    int* kc = (int*)malloc(sizeof(int) * nnz);
+    double* kv = (double*)malloc(sizeof(double) * nnz);   /* regularised */
+    double* kh = (double*)malloc(sizeof(double) * nnz);   /* exact */
+    long long t = 0;
+    for (int R = 0; R < n; ++R) {
+        kp[R] = (int)t;
+        for (int p = rowptr[R]; p < rowptr[R + 1]; ++p) {
+            kc[t] = cols[p];
+            kh[t] = P[p];
+            kv[t] = P[p] + (cols[p] == R ? delta : 0.0);
+            ++t;
+        }
+        int v = R / 3, k = R % 3;
+        for (int w = vcnt[v]; w < vcnt[v + 1]; ++w) {
+            int j = vrows[w];
+            kc[t] = n + j;
+            kh[t] = kv[t] = a[3 * aidx[j] + k];
+            ++t;
+        }
+    }
+    for (int j = 0; j < ma; ++j) {
+        int i = aidx[j];
+        kp[n + j] = (int)t;
+        for (int k = 0; k < 3; ++k) {
+            kc[t] = 3 * vtx[i] + k;
+            kh[t] = kv[t] = a[3 * i + k];
+            ++t;
+        }
+        kc[t] = n + j;
+        kh[t] = 0.0;
+        kv[t] = -delta;
+        ++t;
+    }
+    kp[N2] = (int)t;
+    double* rhs = (double*)malloc(sizeof(double) * N2);
+    double* sol = (double*)calloc(N2, sizeof(double));
+    double* res = (double*)malloc(sizeof(double) * N2);
+    double* cor = (double*)malloc(sizeof(double) * N2);
+    for (int i = 0; i < n; ++i) rhs[i] = -q[i];
+    for (int j = 0; j < ma; ++j) {
+        int i = aidx[j];
+        rhs[n + j] = side[i] == 1 ? u[i] : l[i];
+    }
+    ldl_fac f;
+    int ok = ldl_factor(N2, kp, kc, kv, &f) == ORC_OK;
+    if (ok) {
+        ldl_apply(&f, rhs, sol);
+        for (int it = 0; it < 25; ++it) {
+            csr_mv(N2, kp, kc, kh, sol, res);
+            for (int i = 0; i < N2; ++i) res[i] = rhs[i] - res[i];
+            if (vmax_abs(N2, res) <= 1e-15 * (1.0 + vmax_abs(N2, rhs))) break;
+            ldl_apply(&f, res, cor);
+            for (int i = 0; i < N2; ++i) sol[i] += cor[i];
+        }
+        ldl_free(&f);
+    }
+    if (ok) {
+        double* Ax = (double*)malloc(sizeof(double) * (m > 0 ? m : 1));
+        double* Px = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+        rows_mv(m, vtx, a, sol, Ax);
+        csr_mv(n, rowptr, cols, P, sol, Px);
+        double bmax = 0;
+        for (int i = 0; i < m; ++i) {
+            if (fabs(Ax[i]) > bmax) bmax = fabs(Ax[i]);
+            if (isfinite(l[i]) && fabs(l[i]) > bmax) bmax = fabs(l[i]);
+        }
+        const double tolp = 1e-13 + 1e-10 * bmax;
+        const double told = 1e-13 + 1e-9 * (vmax_abs(n, q) > vmax_abs(n, Px) ? vmax_abs(n, q) : vmax_abs(n, Px));
+        for (int i = 0; i < m && ok; ++i)
+            if (Ax[i] < l[i] - tolp || Ax[i] > u[i] + tolp) ok = 0;
+        for (int j = 0; j < ma && ok; ++j) {
+            int i = aidx[j];
+            double yj = sol[n + j];
+            if ((side[i] == -1 && yj > told) || (side[i] == 1 && yj < -told)) ok = 0;
+        }
+        if (ok) {
+            memcpy(xo, sol, sizeof(double) * n);
+            for (int i = 0; i < m; ++i) yo[i] = 0.0;
+            for (int j = 0; j < ma; ++j) yo[aidx[j]] = sol[n + j];
+        }
+        free(Ax);
+        free(Px);
+    }
+    free(side);
+    free(aidx);
+    free(vcnt);
+    free(vrows);
+    free(vfill);
+    free(kp);
+    free(kc);
+    free(kv);
+    free(kh);
+    free(rhs);
+    free(sol);
+    free(res);
+    free(cor);
+    return ok;
+}
+
+int orc_qp_solve(int n, const int* rowptr, const int* cols, const double* P, const double* q, int m, const int* vtx,
+                 const double* a, const double* l, const double* u, double* x, double* y, int* iters) {
+    const double sigma = 1e-6, alpha = 1.6;
+    double rho = 0.1;
+    int status = ORC_ERR_NOCONV;
+    if (iters) *iters = 0;
+    for (int i = 0; i < m; ++i)
+        if (vtx[i] < 0 || 3 * vtx[i] + 2 >= n || !(l[i] <= u[i])) return ORC_ERR_ARG;
+    double* work = (double*)malloc(sizeof(double) * (size_t)(rowptr[n] > 0 ? rowptr[n] : 1));
+    if (m == 0) {
+        /* unconstrained: P x = -q */
+        double* b = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+        for (int i = 0; i < n; ++i) b[i] = -q[i];
+        status = ldlt_solve(n, rowptr, cols, P, b, x);
+        free(b);
+        free(work);
+        return status;
+    }
+    double* rv = (double*)malloc(sizeof(double) * m);     /* rho per row */
+    double* z = (double*)calloc(m, sizeof(double));
+    double* zp = (double*)malloc(sizeof(double) * m);
+    double* zt = (double*)malloc(sizeof(double) * m);
+    double* Ax = (double*)malloc(sizeof(double) * m);
+    double* xt = (double*)malloc(sizeof(double) * n);
+    double* rhs = (double*)malloc(sizeof(double) * n);
+    double* Px = (double*)malloc(sizeof(double) * n);
+    double* Aty = (double*)malloc(sizeof(double) * n);
+    double* xs = (double*)malloc(sizeof(double) * n);
+    double* ys = (double*)malloc(sizeof(double) * m);
+    for (int i = 0; i < n; ++i) x[i] = 0.0;
+    for (int i = 0; i < m; ++i) y[i] = 0.0;
+    for (int i = 0; i < m; ++i) rv[i] = (l[i] == u[i]) ? 1e3 * rho : rho;
+    ldl_fac f;
+    int fst = admm_factor(n, rowptr, cols, P, sigma, m, vtx, a, rv, work, &f);
+    if (fst != ORC_OK) status = fst;
+    double eps = 1e-6;
+    const int max_total = 400000;
+    for (int k = 0; fst == ORC_OK && k < max_total; ++k) {
+        /* (P + sigma I + A^T R A) xt = sigma x - q + A^T (R z - y) */
+        for (int i = 0; i < n; ++i) rhs[i] = sigma * x[i] - q[i];
+        for (int i = 0; i < m; ++i) zp[i] = rv[i] * z[i] - y[i];
+        rows_mtv(m, vtx, a, zp, rhs);
+        ldl_apply(&f, rhs, xt);
+        rows_mv(m, vtx, a, xt, zt);
+        for (int i = 0; i < n; ++i) x[i] = alpha * xt[i] + (1 - alpha) * x[i];
+        for (int i = 0; i < m; ++i) {
+            zp[i] = z[i];
+            double w = alpha * zt[i] + (1 - alpha) * zp[i];
+            double zn = w + y[i] / rv[i];
+            z[i] = zn < l[i] ? l[i] : (zn > u[i] ? u[i] : zn);
+            y[i] += rv[i] * (w - z[i]);
+        }
+        if (iters) *iters = k + 1;
+        if ((k + 1) % 25) continue;
+        /* residuals (termination relative to the scales sp, sd at eps) */
+        rows_mv(m, vtx, a, x, Ax);
+        csr_mv(n, rowptr, cols, P, x, Px);
+        for (int i = 0; i < n; ++i) Aty[i] = 0.0;
+        rows_mtv(m, vtx, a, y, Aty);
+        double rp = 0, rd = 0;
+        for (int i = 0; i < m; ++i) rp = fmax(rp, fabs(Ax[i] - z[i]));
+        for (int i = 0; i < n; ++i) rd = fmax(rd, fabs(Px[i] + q[i] + Aty[i]));
+        /* primal scale also counts |x| (unit-normal rows): an active row with
+         * l = 0 has A x = z = 0 at the solution */
+        const double sp = fmax(fmax(vmax_abs(m, Ax), vmax_abs(m, z)), vmax_abs(n, x));
+        const double sd = fmax(fmax(vmax_abs(n, Px), vmax_abs(n, Aty)), vmax_abs(n, q));
+        if (rp <= eps * sp && rd <= eps * sd) {
+            if (qp_polish(n, rowptr, cols, P, q, m, vtx, a, l, u, z, y, xs, ys)) {
+                memcpy(x, xs, sizeof(double) * n);
+                memcpy(y, ys, sizeof(double) * m);
+                status = ORC_OK;
+                break;
+            }
+            if (eps > 1e-13) eps *= 1e-2;
+        }
+        /* adaptive rho: rho sqrt((rp / sp) / (rd / sd)), refactorised when it
+         * moves by more than 5x */
+        if (rp > 0 && rd > 0 && sp > 0 && sd > 0) {
+            double rn = rho * sqrt((rp / sp) / (rd / sd));
+            rn = rn < 1e-6 ? 1e-6 : (rn > 1e6 ? 1e6 : rn);
+            if (rn > 5 * rho || rn < rho / 5) {
+                rho = rn;
+                for (int i = 0; i < m; ++i) rv[i] = (l[i] == u[i]) ? 1e3 * rho : rho;
+                ldl_free(&f);
+                fst = admm_factor(n, rowptr, cols, P, sigma, m, vtx, a, rv, work, &f);
+                if (fst != ORC_OK) status = fst;
+            }
+        }
+    }
+    if (fst == ORC_OK) ldl_free(&f);
+    free(work);
+    free(rv);
+    free(z);
+    free(zp);
+    free(zt);
+    free(Ax);
+    free(xt);
+    free(rhs);
+    free(Px);
+    free(Aty);
+    free(xs);
+    free(ys);
+    return status;
+}
+
+/* PlaneContact::update_active_constraints (planeContact.cpp:87-124): each
+ * surface vertex takes the plane of smallest signed distance n.(x - p)
+ * (plane3D.h:30-33) among those closer than PLANE_CONSTRAINT_THRESHOLD, or
+ * -1. */
+static void sqp_plane_active(orc_sys* s, const double* x) {
+    for (int i = 0; i < s->nsurf; ++i) {
+        const double* xv = x + 3 * (size_t)s->surf[i];
+        double best = INFINITY;
+        int idx = -1;
+        for (int j = 0; j < s->nplanes; ++j) {
+            double d = 0;
+            for (int k = 0; k < 3; ++k) d += (xv[k] - s->plane_p[j][k]) * s->plane_n[j][k];
+            if (d < ORC_PLANE_THRESHOLD && d < best) {
+                best = d;
+                idx = j;
+            }
+        }
+        s->pmask[i] = idx;
+    }
+}
+
+static double disk_plane_dist(const orc_sys* s, int d, const double* xv) {
+    double t = 0;
+    for (int k = 0; k < 3; ++k) t += s->disk_n[d][k] * (xv[k] - s->disk_c[d][k]);
+    return t;
+}
+
+static double disk_radial_dist(const orc_sys* s, int d, const double* xv) {
+    double v[3], t = disk_plane_dist(s, d, xv), r = 0;
+    for (int k = 0; k < 3; ++k) v[k] = xv[k] - s->disk_c[d][k];
+    for (int k = 0; k < 3; ++k) {
+        double w = v[k] - t * s->disk_n[d][k];
+        r += w * w;
+    }
+    return sqrt(r);
+}
+
+/* DiskContact::update_active_constraints (diskContact.cpp:91-202): a frozen
+ * footprint stays until every vertex in it is at least 3 thresholds above
+ * its disk's plane; otherwise, when some surface vertex is within the plane
+ * threshold and the radius (+ radial threshold) of a disk, every surface
+ * vertex within [-5, +1] thresholds of that disk's plane and inside its
+ * radius is captured (the disk of smaller |plane distance| when two
+ * compete) and the footprint freezes. */
+static void sqp_disk_active(orc_sys* s, const double* x) {
+    const double pt = ORC_DISK_PLANE_THRESHOLD, rt = ORC_DISK_RADIAL_THRESHOLD;
+    const double clear = 3.0 * pt, depth = 5.0 * pt;
+    if (s->dfrozen) {
+        int all_clear = 1;
+        for (int i = 0; i < s->nsurf && all_clear; ++i) {
+            int d = s->dmask[i];
+            if (d == -1) continue;
+            if (disk_plane_dist(s, d, x + 3 * (size_t)s->surf[i]) < clear) all_clear = 0;
+        }
+        if (all_clear) {
+            s->dfrozen = 0;
+            for (int i = 0; i < s->nsurf; ++i) s->dmask[i] = -1;
+        }
+        return;
+    }
+    int hit[ORC_MAX_PRIM] = {0}, any = 0;
+    for (int i = 0; i < s->nsurf; ++i) {
+        const double* xv = x + 3 * (size_t)s->surf[i];
+        for (int d = 0; d < s->ndisks; ++d) {
+            if (disk_plane_dist(s, d, xv) >= pt) continue;
+            if (disk_radial_dist(s, d, xv) > s->disk_r[d] + rt) continue;
+            hit[d] = 1;
+        }
+    }
+    for (int d = 0; d < s->ndisks; ++d) any |= hit[d];
+    for (int i = 0; i < s->nsurf; ++i) s->dmask[i] = -1;
+    if (!any) return;
+    int nact = 0;
+    for (int d = 0; d < s->ndisks; ++d) {
+        if (!hit[d]) continue;
+        for (int i = 0; i < s->nsurf; ++i) {
+            const double* xv = x + 3 * (size_t)s->surf[i];
+            double pd = disk_plane_dist(s, d, xv);
+            if (pd > pt || pd < -depth) continue;
+            if (disk_radial_dist(s, d, xv) > s->disk_r[d] + rt) continue;
+            if (s->dmask[i] == -1) {
+                s->dmask[i] = d;
+                ++nact;
+            } else if (fabs(pd) < fabs(disk_plane_dist(s, s->dmask[i], xv))) {
+                s->dmask[i] = d;
+            }
+        }
+    }
+    if (nact > 0) s->dfrozen = 1;
+}
+
+int orc_sqp_active(orc_sys* s, const double* x, int* pmask, int* dmask, int* frozen) {
+    if (!s->has_contact) return ORC_ERR_ARG;
+    sqp_plane_active(s, x);
+    sqp_disk_active(s, x);
+    if (pmask) memcpy(pmask, s->pmask, sizeof(int) * s->nsurf);
+    if (dmask) memcpy(dmask, s->dmask, sizeof(int) * s->nsurf);
+    if (frozen) *frozen = s->dfrozen;
+    return ORC_OK;
+}
+
+/* The stacked constraint rows of Energy::compute_constraint_bounds /
+ * compute_constraint_matrix_qp (energy.cpp:394-458): NeumannBC equality
+ * rows x_i - xPrev_i - dt v_i = 0 on the masked DOFs (neumannBC.cpp:4-40:
+ * row e_i, l = u = -(x_i - xPrev_i - dt v_i)), then PlaneContact rows
+ * n.(x - p) >= 0 (planeContact.cpp:4-84: row n, l = -n.(x - p), u = inf)
+ * and DiskContact rows n.(x - c) >= 0 (diskContact.cpp:4-88) on the active
+ * surface vertices.  Returns the row count; arrays sized by the caller. */
+static int sqp_rows(const orc_sys* s, const double* x, double dt, int* vtx, double* a, double* l, double* u) {
+    int m = 0;
+    for (int i = 0; i < s->n; ++i) {
+        if (!s->mask[i]) continue;
+        vtx[m] = i / 3;
+        for (int k = 0; k < 3; ++k) a[3 * m + k] = (k == i % 3) ? 1.0 : 0.0;
+        l[m] = u[m] = -(x[i] - s->qprev[i] - dt * s->bcval[i]);
+        ++m;
+    }
+    if (s->has_contact) {
+        for (int i = 0; i < s->nsurf; ++i) {
+            int j = s->pmask[i];
+            if (j < 0) continue;
+            const double* xv = x + 3 * (size_t)s->surf[i];
+            double d = 0;
+            for (int k = 0; k < 3; ++k) {
+                d += s->plane_n[j][k] * (xv[k] - s->plane_p[j][k]);
+                a[3 * m + k] = s->plane_n[j][k];
+            }
+            vtx[m] = s->surf[i];
+            l[m] = -d;
+            u[m] = INFINITY;
+            ++m;
+        }
+        for (int i = 0; i < s->nsurf; ++i) {
+            int j = s->dmask[i];
+            if (j < 0) continue;
+            const double* xv = x + 3 * (size_t)s->surf[i];
+            for (int k = 0; k < 3; ++k) a[3 * m + k] = s->disk_n[j][k];
+            vtx[m] = s->surf[i];
+            l[m] = -disk_plane_dist(s, j, xv);
+            u[m] = INFINITY;
+            ++m;
+        }
+    }
+    return m;
+}
+
+/* Solver::minimize_SQP (solver.cpp:109-197): the active sets are updated once
+ * from the initial guess; then up to max_iter times the gradient and Hessian
+ * (SQP assembly: no masking, no Dirichlet override) define the QP above,
+ * x += dx, and ||dx|| < dx_tol ends it.  A NaN step or running out of
+ * iterations is "SQP method did not converge" (ORC_ERR_NOCONV).  The QP's
+ * ADMM iterations are counted in st->pcg_iters. */
+static int orc_minimize_sqp(orc_sys* s, double* x, double dt, const orc_opts* o, orc_stats* st) {
+    const int n = s->n;
+    if (s->has_contact) {
+        sqp_plane_active(s, x);
+        sqp_disk_active(s, x);
+    }
+    orc_pattern(s, dt);
+    const int mmax = n + 2 * (s->has_contact ? s->nsurf : 0);
+    double* g = (double*)malloc(sizeof(double) * n);
+    double* dx = (double*)malloc(sizeof(double) * n);
+    double* vals = (double*)malloc(sizeof(double) * (size_t)(s->nnz > 0 ? s->nnz : 1));
+    int* vtx = (int*)malloc(sizeof(int) * (mmax > 0 ? mmax : 1));
+    double* a = (double*)malloc(sizeof(double) * 3 * (size_t)(mmax > 0 ? mmax : 1));
+    double* l = (double*)malloc(sizeof(double) * (mmax > 0 ? mmax : 1));
+    double* u = (double*)malloc(sizeof(double) * (mmax > 0 ? mmax : 1));
+    double* y = (double*)malloc(sizeof(double) * (mmax > 0 ? mmax : 1));
+    int status = ORC_ERR_NOCONV;
+    for (int it = 0; it < o->max_iter; ++it) {
+        double t0 = now_s();
+        orc_gradient(s, x, dt, g);
+        double t1 = now_s();
+        st->t_grad += t1 - t0;
+        st->gnorm = norm2(n, g);
+        int hst = orc_hessian(s, x, dt, o->project, vals);
+        double t2 = now_s();
+        st->t_hess += t2 - t1;
+        if (hst != ORC_OK) {
+            status = hst;
+            break;
+        }
+        int m = sqp_rows(s, x, dt, vtx, a, l, u);
+        int qit = 0;
+        int qst = orc_qp_solve(n, s->rowptr, s->cols, vals, g, m, vtx, a, l, u, dx, y, &qit);
+        st->pcg_iters += qit;
+        st->t_solve += now_s() - t2;
+        if (qst != ORC_OK) {
+            status = ORC_ERR_SOLVE;
+            break;
+        }
+        st->newton_iters++;
+        double dxn = norm2(n, dx);
+        for (int i = 0; i < n; ++i) x[i] += dx[i];
+        st->dxnorm = dxn;
+        if (dxn != dxn) break;
+        if (dxn < o->dx_tol) {
+            status = ORC_OK;
+            break;
+        }
+    }
+    free(g);
+    free(dx);
+    free(vals);
+    free(vtx);
+    free(a);
+    free(l);
+    free(u);
+    free(y);
     return status;
 }
 

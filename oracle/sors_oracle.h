@@ -129,7 +129,8 @@ int      orc_hessian(orc_sys* s, const double* q, double dt, int project, double
 /* Solver::step for one substep-loop call (solver.cpp:5-44, no CFL). */
 int      orc_step(orc_sys* s, double* q, double dt, int substeps, const orc_opts* o, orc_stats* st);
 void     orc_update_state(orc_sys* s, const double* q, double dt);
-/* one minimisation without the state update (minimize_newton[_linesearch]) */
+/* one minimisation without the state update (minimize_newton[_linesearch],
+ * minimize_frozenhessian, minimize_SQP) */
 int      orc_minimize(orc_sys* s, double* x, double dt, const orc_opts* o, orc_stats* st);
 /* CPU-baseline sample: seconds per FP64 PCG iteration on H(q) (includes the
  * block-Jacobi apply and both dot products; assembly not timed). */
@@ -185,6 +186,28 @@ int  orc_set_pressure(orc_sys* s, int nfaces, int nv, const int* faces, const in
 void orc_set_pressure_activation(orc_sys* s, const double* p);
 void orc_pressure_force(const orc_sys* s, const double* q, double* f);
 void orc_pressure_jacobian(const orc_sys* s, const double* q, double* J);
+
+/* ---- minimize_SQP (solver.cpp:109-197) -------------------------------------
+ * Under ORC_SQP the planes and disks of orc_set_contact are the reference's
+ * planeContact / diskContact inequality constraints and the Dirichlet mask
+ * its neumannBC equality constraints; no penalty or friction terms.
+ *
+ * orc_qp_solve: min 1/2 x^T P x + q^T x  s.t.  l <= A x <= u, P on a CSR
+ * pattern (n x n, full symmetric storage, diagonal present), row i of A
+ * acting on the DOFs 3 vtx[i] + (0, 1, 2) with coefficients a[3i..3i+2]
+ * (u[i] = INFINITY for one-sided rows).  OSQP's ADMM with adaptive rho and
+ * solution polishing (the QP solver the reference calls), run until the
+ * polished solution is primal feasible with multipliers y of the right sign
+ * (OSQP's convention: P x + q + A^T y = 0, y <= 0 on an active lower bound).
+ * iters: ADMM iterations.
+ *
+ * orc_sqp_active: PlaneContact / DiskContact update_active_constraints at x
+ * (planeContact.cpp:87-124, diskContact.cpp:91-202): per surface vertex the
+ * plane index or -1, the captured disk or -1, and the disk footprint's
+ * frozen flag (which persists between calls, as in the reference). */
+int  orc_qp_solve(int n, const int* rowptr, const int* cols, const double* P, const double* q, int m, const int* vtx,
+                  const double* a, const double* l, const double* u, double* x, double* y, int* iters);
+int  orc_sqp_active(orc_sys* s, const double* x, int* pmask, int* dmask, int* frozen);
 
 #ifdef __cplusplus
 }

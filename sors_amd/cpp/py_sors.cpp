@@ -230,6 +230,7 @@ PYBIND11_MODULE(py_sors, m) {
         .def_readwrite("predictor", &SimulationSettings::predictor)
         .def_readwrite("coarseNodes", &SimulationSettings::coarseNodes)
         .def_readwrite("coarseEvery", &SimulationSettings::coarseEvery)
+        .def_readwrite("contactModel", &SimulationSettings::contactModel)
         .def("verbose_print", &SimulationSettings::verbose_print);
 
     bind_energy<4>(m, "EnergyTet");

@@ -95,6 +95,7 @@ void SimulationSettings::verbose_print() const {
               << std::setw(25) << "Time Scheme:" << na(timeSteppingScheme) << "\n"
               << std::setw(25) << "Solver Method:" << na(solverMethod) << "\n"
               << std::setw(25) << "Precision:" << precision << "\n"
+              << std::setw(25) << "Contact Model:" << contactModel << "\n"
               << std::setw(25) << "PCG Coarse Space:"
               << (coarseNodes > 0 ? std::to_string(coarseNodes) + " nodes per aggregate" : std::string("off")) << "\n"
               << "================================================================================\n";
@@ -171,8 +172,9 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
     if (material < 0) material = SORS_NEOHOOKEAN;
     std::vector<int> mask;
     std::vector<double> bcv;
-    // planeContact / diskContact (energy.cpp:69-105): the device resolves them
-    // with a penalty + lagged Coulomb friction (sors_set_contact)
+    // planeContact / diskContact (energy.cpp:69-105): constraints of
+    // minimize_SQP, or (contactModel "penalty") a penalty + lagged Coulomb
+    // friction (sors_set_contact)
     std::vector<double> pp, pn, dc, dn, dr;
     std::vector<int> surf;
     bool haveSurf = false;
@@ -250,6 +252,9 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
     d.bc_value = bcv.empty() ? nullptr : bcv.data();
     for (int k = 0; k < 3; ++k) d.grav[k] = grav[k];
     d.damping_alpha = dampingAlpha;
+    if (settings.contactModel != "constraints" && settings.contactModel != "penalty")
+        throw std::invalid_argument("contactModel must be \"constraints\" or \"penalty\"");
+    const bool penaltyContact = hasContact_ && settings.contactModel == "penalty";
     if (settings.timeSteppingScheme == "backward_euler")
         d.scheme = SORS_BACKWARD_EULER;
     else if (settings.timeSteppingScheme == "crank_nicolson")
@@ -263,16 +268,17 @@ Energy<E>::Energy(const Mat& X, const std::vector<int>& conn, const SimulationSe
     else if (settings.solverMethod == "minimize_frozenhessian")
         d.method = SORS_MINIMIZE_FROZENHESSIAN;
     else if (settings.solverMethod == "minimize_SQP")
-        // the reference's contact solver; with contact the device runs its
-        // penalty formulation under Newton + line search, or plain Newton
-        // when a muscle is present (its energy is not the potential of its
-        // gradient, softconMuscle.cpp:31 vs :72, so an Armijo test on it is
-        // meaningless; SQP itself never evaluates the energy)
-        d.method = hasContact_ ? (mgroups > 0 ? SORS_MINIMIZE_NEWTON : SORS_MINIMIZE_NEWTON_LINESEARCH)
-                               : SORS_MINIMIZE_SQP;
+        // the reference's contact solver.  With contactModel "penalty" and
+        // contact the device runs the penalty formulation under Newton + line
+        // search, or plain Newton when a muscle is present (its energy is not
+        // the potential of its gradient, softconMuscle.cpp:31 vs :72, so an
+        // Armijo test on it is meaningless; SQP never evaluates the energy)
+        d.method = penaltyContact ? (mgroups > 0 ? SORS_MINIMIZE_NEWTON : SORS_MINIMIZE_NEWTON_LINESEARCH)
+                                  : SORS_MINIMIZE_SQP;
     else
         throw std::runtime_error("Solver method not recognized");
-    sqpMethod_ = mgroups > 0 ? SORS_MINIMIZE_NEWTON : SORS_MINIMIZE_NEWTON_LINESEARCH;
+    sqpMethod_ = d.method == SORS_MINIMIZE_SQP ? SORS_MINIMIZE_SQP
+                                               : (mgroups > 0 ? SORS_MINIMIZE_NEWTON : SORS_MINIMIZE_NEWTON_LINESEARCH);
     check_status(sors_create(&d, &ctx_), nullptr);
     if (mgroups > 0) {
         sors_muscle_desc md{};
@@ -481,10 +487,9 @@ std::vector<double> Solver<E>::step(const std::vector<double>& solution, Energy<
     if (dt == 0.0) dt = en.settings_.dt;
     if (substeps == 0) substeps = en.settings_.substeps;
     if (!(dt > 0.0)) throw std::invalid_argument("dt must be positive (settings.dt is not set)");
-    const bool sqpContact = en.settings_.solverMethod == "minimize_SQP" && en.has_contact();
     const bool frozen = en.settings_.solverMethod == "minimize_frozenhessian";
     if (en.settings_.solverMethod != "minimize_newton" && en.settings_.solverMethod != "minimize_newton_linesearch" &&
-        !frozen && !sqpContact)
+        !frozen && en.settings_.solverMethod != "minimize_SQP")
         throw std::runtime_error("Solver method not recognized");
     if ((int)solution.size() != 3 * en.num_vertices()) throw std::invalid_argument("solution must have 3N entries");
     if (en.settings_.CFLtimeSteppingFlag) {

@@ -102,6 +102,10 @@ struct SimulationSettings {
     int coarseNodes = 2048;           // > 0: two-level PCG (rigid-body coarse correction on aggregates of this
                                       // size); 0: block-Jacobi alone
     int coarseEvery = 16;             // coarse operator refreshed on every k-th tangent (as sors_default_options)
+    std::string contactModel = "constraints";  // "constraints": minimize_SQP with planeContact / diskContact as
+                                               // the reference's hard constraints; "penalty": this build's penalty
+                                               // + Coulomb friction under Newton (contactStiffness,
+                                               // frictionCoefficient, frictionVelocity)
     void verbose_print() const;
 };
 
@@ -139,7 +143,8 @@ public:
 
     sors_ctx* ctx() const { return ctx_; }
     int num_vertices() const { return numVertices_; }
-    // the device method minimize_SQP resolves to (contact is a penalty on the device)
+    // the device method minimize_SQP resolves to: SORS_MINIMIZE_SQP, or with
+    // contactModel "penalty" Newton on the penalty formulation
     int sqp_method() const { return sqpMethod_; }
     // energy.h: one Tetrahedron / Hexahedron per element, its energy plugins
     // built through the ElementEnergy registry; they evaluate on the device
@@ -154,7 +159,7 @@ private:
     bool hasContact_ = false;
     int muscleGroups_ = 0;
     int pressureGroups_ = 0;
-    int sqpMethod_ = SORS_MINIMIZE_NEWTON_LINESEARCH;
+    int sqpMethod_ = SORS_MINIMIZE_SQP;
 };
 
 // Non-owning handle on the diskContact entry of an Energy's constraint list
@@ -187,9 +192,10 @@ public:
                                                    double dt, const Params& act) const;
     std::vector<double> minimize_frozenhessian(const std::vector<double>& initialGuess, Energy<E>& systemEnergy,
                                                double dt, const Params& act) const;
-    // solver.cpp:109-197: contact as the device's penalty; Newton with line
-    // search (plain Newton with a muscle: its energy is not the potential of
-    // its gradient)
+    // solver.cpp:109-197: planeContact / diskContact / neumannBC as
+    // constraints, one exactly solved QP per SQP iteration (contactModel
+    // "penalty": Newton with line search on the penalty formulation, plain
+    // Newton with a muscle, whose energy is not the potential of its gradient)
     std::vector<double> minimize_SQP(const std::vector<double>& initialGuess, Energy<E>& systemEnergy, double dt,
                                      const Params& act) const;
     // solver.cpp:300-331

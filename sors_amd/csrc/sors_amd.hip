@@ -2009,16 +2009,40 @@ __device__ __forceinline__ void block_partial_at(double (&v)[NV], double* base, 
         for (int k = 0; k < NV; ++k) base[k * stride + lb] = v[k];
 }
 
-// ---- bodies (lb / nb: this block's index and the block count of its rollout)
-// init: x = 0, r = sign*b, p = s = 0, u = M r; partials (r.u, r.r)
+// Projection of the residual onto the null space of per-node linear
+// constraints (minimize_SQP's working set): a node nd with kmap[nd] = k >= 0
+// has the symmetric 3x3 projector P[9k .. 9k+8], the others none.  With the
+// block-Jacobi rows of those nodes replaced by P D^-1 P, PCG on the projected
+// residual is CG on the constrained subspace (p, x stay in it).
 template <typename HT>
-__device__ void pcg_init_body(const PcgView<HT>& v, const double* b, double sign, PartIO io, int lb, int nb) {
+struct PcgProj {
+    const int* kmap = nullptr;
+    const HT* P = nullptr;
+};
+
+// ---- bodies (lb / nb: this block's index and the block count of its rollout)
+// init: x = 0, r = sign*b (projected with PROJ), p = s = 0, u = M r; partials (r.u, r.r)
+template <typename HT, bool PROJ = false>
+__device__ void pcg_init_body(const PcgView<HT>& v, const double* b, double sign, PartIO io, int lb, int nb,
+                              PcgProj<HT> pj = {}) {
     double acc[2] = {0.0, 0.0};
     for (int nd = lb * blockDim.x + threadIdx.x; nd < v.N; nd += nb * blockDim.x) {
         HT rr[3];
 #pragma unroll
+        for (int c = 0; c < 3; ++c) rr[c] = (HT)(sign * b[3 * nd + c]);
+        if constexpr (PROJ) {
+            const int k = pj.kmap[nd];
+            if (k >= 0) {
+                const HT* P = pj.P + 9 * (size_t)k;
+                HT t[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) t[c] = P[3 * c] * rr[0] + P[3 * c + 1] * rr[1] + P[3 * c + 2] * rr[2];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) rr[c] = t[c];
+            }
+        }
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
-            rr[c] = (HT)(sign * b[3 * nd + c]);
             v.r[3 * nd + c] = rr[c];
             v.x[3 * nd + c] = 0;
             v.p[3 * nd + c] = 0;
@@ -2190,9 +2214,9 @@ __device__ void pcg_spmv_body(const PcgView<HT>& v, PcgState* st, PartIO io, int
 #define SORS_UPD_THREADS 384
 #endif
 constexpr int kUpdBlock = SORS_UPD_THREADS;  // multiple of 3 (whole nodes)
-template <typename HT>
+template <typename HT, bool PROJ = false>
 __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgState* st_next, PartIO io, int first,
-                                int lb, int nb) {
+                                int lb, int nb, PcgProj<HT> pj = {}) {
 #ifndef SORS_UPD_UNROLL
 #define SORS_UPD_UNROLL 8
 #endif
@@ -2280,11 +2304,33 @@ __device__ void pcg_update_body(const PcgView<HT>& v, PcgState* st_prev, PcgStat
                 v.s[i] = sn;
                 v.x[i] = xx[q] + a * pn;
                 rr = rr0[q] - a * sn;
-                v.r[i] = rr;
+                if constexpr (!PROJ) v.r[i] = rr;
             }
             rs[q][threadIdx.x] = rr;
         }
         __syncthreads();
+        if constexpr (PROJ) {
+            // r <- P r on constrained nodes (the node's three residual
+            // components meet in LDS as for u = M r)
+            HT rp[kUnroll];
+#pragma unroll
+            for (int q = 0; q < kUnroll; ++q) {
+                const int i = base + q * stride + threadIdx.x;
+                rp[q] = rs[q][threadIdx.x];
+                if (i < n) {
+                    const int k = pj.kmap[i / 3];
+                    if (k >= 0) {
+                        const HT* P = pj.P + 9 * (size_t)k + 3 * c;
+                        rp[q] = P[0] * rs[q][first3] + P[1] * rs[q][first3 + 1] + P[2] * rs[q][first3 + 2];
+                    }
+                    v.r[i] = rp[q];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kUnroll; ++q) rs[q][threadIdx.x] = rp[q];
+            __syncthreads();
+        }
 #pragma unroll
         for (int q = 0; q < kUnroll; ++q) {
             const int i = base + q * stride + threadIdx.x;
@@ -2735,6 +2781,20 @@ __global__ __launch_bounds__(kUpdBlock) void k_pcg_update(PcgView<HT> v, PcgStat
     pcg_update_body(v, st_prev, st_next, io, first, blockIdx.x, gridDim.x);
 }
 
+// the same two kernels with the residual projected onto the constrained
+// subspace (minimize_SQP)
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_pcg_init_proj(PcgView<HT> v, const double* b, double sign, PartIO io,
+                                                          PcgProj<HT> pj) {
+    pcg_init_body<HT, true>(v, b, sign, io, blockIdx.x, gridDim.x, pj);
+}
+
+template <typename HT>
+__global__ __launch_bounds__(kUpdBlock) void k_pcg_update_proj(PcgView<HT> v, PcgState* st_prev, PcgState* st_next,
+                                                               PartIO io, int first, PcgProj<HT> pj) {
+    pcg_update_body<HT, true>(v, st_prev, st_next, io, first, blockIdx.x, gridDim.x, pj);
+}
+
 // w = A u on scalar CSR rows (general masks), LPR lanes per row
 template <typename HT, int LPR>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* rowptr, const int* cols, const HT* vals,
@@ -3037,6 +3097,369 @@ struct HipError {
 
 }  // namespace
 
+// ===========================================================================
+// minimize_SQP (solver.cpp:109-197): contact as hard constraints.
+//
+// Every constraint row of the reference acts on one vertex: NeumannBC rows
+// x_i - xPrev_i - dt v_i = 0 on masked DOFs (neumannBC.cpp:4-40), and
+// PlaneContact / DiskContact rows n.(x + dx - p) >= 0 on active surface
+// vertices (planeContact.cpp:4-84, diskContact.cpp:4-88).  The nodes that
+// carry rows form the K table (sorted, one entry per node).  Each SQP
+// iteration solves the QP  min 1/2 dx^T H dx + g^T dx  s.t. those rows by a
+// primal-dual active set method over the inequality rows: for a working set
+// W, the rows of each K node (its equality rows and its rows in W, linearly
+// dependent ones dropped) give the node's particular step dxp (minimum norm),
+// its projector P = I - A^T (A A^T)^-1 A and the projected PCG solve of
+// the rest; then the multipliers (A A^T)^-1 A (H dx + g)_node decide which
+// W rows leave (negative multiplier) and which violated rows join.  W starts
+// from the rows violated at the step's start and carries over between SQP
+// iterations; the reference solves the same QPs with OSQP.
+// ===========================================================================
+constexpr double kPlaneThreshold = 0.01;       // PLANE_CONSTRAINT_THRESHOLD (common.h:29)
+constexpr double kDiskPlaneThreshold = 0.01;   // DISK_PLANAR_CONSTRAINT_THRESHOLD (common.h:33)
+constexpr double kDiskRadialThreshold = 0.001; // DISK_RADIAL_CONSTRAINT_THRESHOLD (common.h:35)
+constexpr int kSqpActiveThreads = 1024;
+
+struct SqpDev {
+    int nK, nsurf;
+    const int* knode;    // [nK] node of entry k (ascending)
+    const int* keq;      // [nK] masked DOFs of the node (bit c: DOF 3 node + c)
+    const int* surf2k;   // [nsurf] entry of each surface vertex
+    int* kpid;           // [nK] active plane of the node or -1
+    int* kdid;           // [nK] captured disk of the node or -1
+    int* kw;             // [nK] working set: bit 0 the plane row, bit 1 the disk row
+    int* pmask;          // [nsurf] planeCollisionMask_
+    int* dmask;          // [nsurf] diskCollisionMask_ (kept between steps)
+    int* frozen;         // [1] frozenActiveSet_
+    int* changes;        // [1] working-set changes of the last multiplier pass
+    double* dxfull;      // [n] particular step (nonzero on K nodes only)
+    double vtol;         // violation tolerance of a row outside W (m)
+};
+
+__device__ __forceinline__ double plane_dist_dev(const ContactArgs& c, int j, const double* x) {
+    return (x[0] - c.pp[j][0]) * c.pn[j][0] + (x[1] - c.pp[j][1]) * c.pn[j][1] + (x[2] - c.pp[j][2]) * c.pn[j][2];
+}
+
+__device__ __forceinline__ double disk_plane_dist_dev(const ContactArgs& c, int d, const double* x) {
+    return c.dn[d][0] * (x[0] - c.dc[d][0]) + c.dn[d][1] * (x[1] - c.dc[d][1]) + c.dn[d][2] * (x[2] - c.dc[d][2]);
+}
+
+__device__ __forceinline__ double disk_radial_dist_dev(const ContactArgs& c, int d, const double* x) {
+    const double t = disk_plane_dist_dev(c, d, x);
+    double r = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double w = (x[k] - c.dc[d][k]) - t * c.dn[d][k];
+        r += w * w;
+    }
+    return sqrt(r);
+}
+
+// PlaneContact::update_active_constraints (planeContact.cpp:87-124) and
+// DiskContact::update_active_constraints (diskContact.cpp:91-202) at the
+// step's initial guess x, in one workgroup (the disk rule has two
+// footprint-wide decisions); then the K table's active rows and the initial
+// working set (rows violated at x).
+__global__ __launch_bounds__(kSqpActiveThreads) void k_sqp_active(SqpDev s, ContactArgs c, const double* x) {
+    __shared__ int sh_clear_fail, sh_hit, sh_cnt;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < s.nsurf; i += nt) {
+        const double* xv = x + 3 * (size_t)c.surf[i];
+        double best = 0.0;
+        int idx = -1;
+        for (int j = 0; j < c.nplanes; ++j) {
+            const double d = plane_dist_dev(c, j, xv);
+            if (d < kPlaneThreshold && (idx < 0 || d < best)) {
+                best = d;
+                idx = j;
+            }
+        }
+        s.pmask[i] = idx;
+    }
+    if (c.ndisks > 0) {
+        const double pt = kDiskPlaneThreshold, rt = kDiskRadialThreshold;
+        const double clear = 3.0 * pt, depth = 5.0 * pt;
+        if (tid == 0) sh_clear_fail = sh_hit = sh_cnt = 0;
+        __syncthreads();
+        if (*s.frozen) {
+            // frozen: keep the footprint until all of it is clearly above its disk
+            for (int i = tid; i < s.nsurf; i += nt) {
+                const int d = s.dmask[i];
+                if (d >= 0 && disk_plane_dist_dev(c, d, x + 3 * (size_t)c.surf[i]) < clear)
+                    atomicOr(&sh_clear_fail, 1);
+            }
+            __syncthreads();
+            if (!sh_clear_fail) {
+                for (int i = tid; i < s.nsurf; i += nt) s.dmask[i] = -1;
+                if (tid == 0) *s.frozen = 0;
+            }
+        } else {
+            for (int i = tid; i < s.nsurf; i += nt) {
+                const double* xv = x + 3 * (size_t)c.surf[i];
+                for (int d = 0; d < c.ndisks; ++d)
+                    if (disk_plane_dist_dev(c, d, xv) < pt && !(disk_radial_dist_dev(c, d, xv) > c.dr[d] + rt))
+                        atomicOr(&sh_hit, 1 << d);
+            }
+            __syncthreads();
+            const int hit = sh_hit;
+            for (int i = tid; i < s.nsurf; i += nt) {
+                const double* xv = x + 3 * (size_t)c.surf[i];
+                int slot = -1;
+                double sabs = 0.0;
+                for (int d = 0; d < c.ndisks; ++d) {
+                    if (!((hit >> d) & 1)) continue;
+                    const double pd = disk_plane_dist_dev(c, d, xv);
+                    if (pd > pt || pd < -depth) continue;
+                    if (disk_radial_dist_dev(c, d, xv) > c.dr[d] + rt) continue;
+                    if (slot < 0 || fabs(pd) < sabs) {
+                        slot = d;
+                        sabs = fabs(pd);
+                    }
+                }
+                s.dmask[i] = slot;
+                if (slot >= 0) atomicAdd(&sh_cnt, 1);
+            }
+            __syncthreads();
+            if (tid == 0 && sh_cnt > 0) *s.frozen = 1;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < s.nsurf; i += nt) {
+        const int k = s.surf2k[i];
+        const double* xv = x + 3 * (size_t)c.surf[i];
+        const int pid = s.pmask[i], did = c.ndisks > 0 ? s.dmask[i] : -1;
+        int w = 0;
+        if (pid >= 0 && -plane_dist_dev(c, pid, xv) > 0.0) w |= 1;
+        if (did >= 0 && -disk_plane_dist_dev(c, did, xv) > 0.0) w |= 2;
+        s.kpid[k] = pid;
+        s.kdid[k] = did;
+        s.kw[k] = w;
+    }
+}
+
+// The rows of K entry k for working set w: equality rows first (unit
+// vectors of the masked DOFs), then the plane row and the disk row when in w;
+// a row linearly dependent on the rows before it is dropped (its bit cleared
+// in *w).  Builds the Cholesky factor L of the Gram matrix A A^T of the kept
+// rows.  b: each row's right-hand side (a.dx >= b or = b).
+struct SqpRows {
+    int nr;          // kept rows
+    int kind[3];     // 0 equality, 1 plane, 2 disk
+    double a[3][3];
+    double b[3];
+    double L[3][3];  // lower triangular, G = L L^T
+};
+
+__device__ bool sqp_try_row(SqpRows& R, const double* a, double b, int kind) {
+    if (R.nr == 3) return false;
+    double y[3];
+    const double aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    double d = aa;
+    for (int j = 0; j < R.nr; ++j) {
+        double gj = R.a[j][0] * a[0] + R.a[j][1] * a[1] + R.a[j][2] * a[2];
+        for (int i = 0; i < j; ++i) gj -= R.L[j][i] * y[i];
+        y[j] = gj / R.L[j][j];
+        d -= y[j] * y[j];
+    }
+    if (!(d > 1e-10 * aa)) return false;
+    const int r = R.nr;
+    for (int j = 0; j < r; ++j) R.L[r][j] = y[j];
+    R.L[r][r] = sqrt(d);
+    for (int k = 0; k < 3; ++k) R.a[r][k] = a[k];
+    R.b[r] = b;
+    R.kind[r] = kind;
+    R.nr = r + 1;
+    return true;
+}
+
+__device__ void sqp_rows(const SqpDev& s, const ContactArgs& c, int k, int* w, const double* x, const double* xprev,
+                         const double* bcval, double dt, SqpRows& R) {
+    R.nr = 0;
+    const int node = s.knode[k], eq = s.keq[k];
+    const double* xv = x + 3 * (size_t)node;
+    for (int comp = 0; comp < 3; ++comp) {
+        if (!((eq >> comp) & 1)) continue;
+        const size_t i = 3 * (size_t)node + comp;
+        const double a[3] = {comp == 0 ? 1.0 : 0.0, comp == 1 ? 1.0 : 0.0, comp == 2 ? 1.0 : 0.0};
+        sqp_try_row(R, a, -(x[i] - xprev[i] - dt * bcval[i]), 0);
+    }
+    const int pid = s.kpid[k], did = s.kdid[k];
+    if (pid >= 0 && (*w & 1) && !sqp_try_row(R, c.pn[pid], -plane_dist_dev(c, pid, xv), 1)) *w &= ~1;
+    if (did >= 0 && (*w & 2) && !sqp_try_row(R, c.dn[did], -disk_plane_dist_dev(c, did, xv), 2)) *w &= ~2;
+}
+
+// G^-1 t for the kept rows (two triangular solves with L)
+__device__ void sqp_gram_solve(const SqpRows& R, const double* t, double* out) {
+    double y[3];
+    for (int j = 0; j < R.nr; ++j) {
+        double v = t[j];
+        for (int i = 0; i < j; ++i) v -= R.L[j][i] * y[i];
+        y[j] = v / R.L[j][j];
+    }
+    for (int j = R.nr - 1; j >= 0; --j) {
+        double v = y[j];
+        for (int i = j + 1; i < R.nr; ++i) v -= R.L[i][j] * out[i];
+        out[j] = v / R.L[j][j];
+    }
+}
+
+// block-Jacobi rows of the K nodes, kept before their projection
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_sqp_save_minv(SqpDev s, const HT* Minv, HT* Msave) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.nK) return;
+    const HT* M = Minv + 9 * (size_t)s.knode[k];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Msave[9 * (size_t)k + i] = M[i];
+}
+
+// per K entry for the working set: dxp = A^T G^-1 b (into dxfull), P = I -
+// A^T G^-1 A, and the node's block-Jacobi rows P D^-1 P
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_sqp_setup(SqpDev s, ContactArgs c, const double* x, const double* xprev,
+                                                      const double* bcval, double dt, const HT* Msave, HT* P,
+                                                      HT* Minv) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.nK) return;
+    int w = s.kw[k];
+    SqpRows R;
+    sqp_rows(s, c, k, &w, x, xprev, bcval, dt, R);
+    s.kw[k] = w;
+    double yb[3] = {0.0, 0.0, 0.0};
+    sqp_gram_solve(R, R.b, yb);
+    // A^T G^-1 A column by column: G^-1 (A e_m) for m = 0..2
+    double Pm[9];
+    for (int m = 0; m < 3; ++m) {
+        double t[3], z[3] = {0.0, 0.0, 0.0};
+        for (int j = 0; j < R.nr; ++j) t[j] = R.a[j][m];
+        sqp_gram_solve(R, t, z);
+        for (int l = 0; l < 3; ++l) {
+            double v = 0.0;
+            for (int j = 0; j < R.nr; ++j) v += R.a[j][l] * z[j];
+            Pm[3 * l + m] = (l == m ? 1.0 : 0.0) - v;
+        }
+    }
+    const int node = s.knode[k];
+    for (int l = 0; l < 3; ++l) {
+        double v = 0.0;
+        for (int j = 0; j < R.nr; ++j) v += R.a[j][l] * yb[j];
+        s.dxfull[3 * (size_t)node + l] = v;
+    }
+    const HT* Ms = Msave + 9 * (size_t)k;
+    double MP[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0.0;
+            for (int l = 0; l < 3; ++l) v += (double)Ms[3 * i + l] * Pm[3 * l + j];
+            MP[3 * i + j] = v;
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0.0;
+            for (int l = 0; l < 3; ++l) v += Pm[3 * i + l] * MP[3 * l + j];
+            Minv[9 * (size_t)node + 3 * i + j] = (HT)v;
+            P[9 * (size_t)k + 3 * i + j] = (HT)Pm[3 * i + j];
+        }
+}
+
+// (H x)_node on the sliced node-block layout, FP64 accumulation; x = xa + xb
+template <typename HT, typename XB>
+__device__ __forceinline__ void blk_row_apply(const int* sbase, const int* bcol, const HT* vals, int node,
+                                              const double* xa, const XB* xb, double (&out)[3]) {
+    const int sl = node / kSlice, lane = node % kSlice;
+    const int base = sbase[sl];
+    const int width = (sbase[sl + 1] - base) / kSlice;
+    out[0] = out[1] = out[2] = 0.0;
+    for (int j = 0; j < width; ++j) {
+        const int nbr = bcol[base + kSlice * j + lane];
+        double xx[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) xx[cc] = xa[3 * (size_t)nbr + cc] + (xb ? (double)xb[3 * (size_t)nbr + cc] : 0.0);
+        const HT* bb = vals + 9LL * base + lane + (long long)kSlice * 9 * j;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            out[r] += (double)bb[(3 * r) * kSlice] * xx[0] + (double)bb[(3 * r + 1) * kSlice] * xx[1] +
+                      (double)bb[(3 * r + 2) * kSlice] * xx[2];
+    }
+}
+
+// rhs = -(g + H dxp): the QP's residual at its particular step
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_sqp_rhs(int N, const int* sbase, const int* bcol, const HT* vals,
+                                                    const double* dxp, const double* g, double* rhs) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= N) return;
+    double hx[3];
+    blk_row_apply<HT, double>(sbase, bcol, vals, node, dxp, (const double*)nullptr, hx);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) rhs[3 * (size_t)node + r] = -(g[3 * (size_t)node + r] + hx[r]);
+}
+
+// Multipliers and the next working set.  dx = dxp + y (y: the projected
+// solve, HT); per K node s = (H dx + g)_node and the kept rows' multipliers
+// G^-1 A s.  A working-set row with a negative multiplier (below -1e-6
+// |s|_inf) leaves; an active row outside it that dx violates by more than
+// vtol, and is independent of the node's kept rows, joins.  Counts the
+// changes into *changes.
+template <typename HT>
+__global__ __launch_bounds__(kBlock) void k_sqp_multipliers(SqpDev s, ContactArgs c, const int* sbase,
+                                                            const int* bcol, const HT* vals, const HT* y,
+                                                            const double* g, const double* x, const double* xprev,
+                                                            const double* bcval, double dt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.nK) return;
+    const int node = s.knode[k];
+    int w = s.kw[k];
+    SqpRows R;
+    sqp_rows(s, c, k, &w, x, xprev, bcval, dt, R);
+    double hx[3];
+    blk_row_apply<HT, HT>(sbase, bcol, vals, node, s.dxfull, y, hx);
+    double sv[3], dxv[3];
+    for (int r = 0; r < 3; ++r) {
+        sv[r] = g[3 * (size_t)node + r] + hx[r];
+        dxv[r] = s.dxfull[3 * (size_t)node + r] + (double)y[3 * (size_t)node + r];
+    }
+    const double smax = fmax(fabs(sv[0]), fmax(fabs(sv[1]), fabs(sv[2])));
+    double t[3], lam[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < R.nr; ++j) t[j] = R.a[j][0] * sv[0] + R.a[j][1] * sv[1] + R.a[j][2] * sv[2];
+    sqp_gram_solve(R, t, lam);
+    int nw = w, chg = 0;
+    for (int j = 0; j < R.nr; ++j) {
+        if (R.kind[j] == 0 || !(lam[j] < -1e-6 * smax)) continue;
+        nw &= ~(R.kind[j] == 1 ? 1 : 2);
+        ++chg;
+    }
+    // rows outside the working set: join when violated and independent
+    const double* xv = x + 3 * (size_t)node;
+    const int pid = s.kpid[k], did = s.kdid[k];
+    for (int kind = 1; kind <= 2; ++kind) {
+        const int bit = kind == 1 ? 1 : 2;
+        const int id = kind == 1 ? pid : did;
+        if (id < 0 || (w & bit)) continue;
+        const double* a = kind == 1 ? c.pn[id] : c.dn[id];
+        const double b = kind == 1 ? -plane_dist_dev(c, id, xv) : -disk_plane_dist_dev(c, id, xv);
+        if (!(a[0] * dxv[0] + a[1] * dxv[1] + a[2] * dxv[2] < b - s.vtol)) continue;
+        if (!sqp_try_row(R, a, b, kind)) continue;
+        nw |= bit;
+        ++chg;
+    }
+    s.kw[k] = nw;
+    if (chg) atomicAdd(s.changes, chg);
+}
+
+// dx = dxp + y (FP64)
+template <typename HT>
+__global__ void k_sqp_step(long long n, const double* dxp, const HT* y, double* dx) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dx[i] = dxp[i] + (double)y[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sumsq(int n, const double* v, Reduce red) {
+    double acc[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc[0] += v[i] * v[i];
+    block_partial<1>(acc, red.partials);
+}
+
 struct sors_ctx {
     int N = 0, E = 0, ed = 0, n = 0;
     int material = SORS_NEOHOOKEAN, scheme = SORS_BACKWARD_EULER, method = SORS_MINIMIZE_NEWTON;
@@ -3141,6 +3564,15 @@ struct sors_ctx {
         return PcgPart{{b, b + 2 * kMaxRedBlocks}, b + 4 * kMaxRedBlocks};
     }
     bool sqp() const { return method == SORS_MINIMIZE_SQP; }
+    bool pat_sqp = false;           // the device pattern has the SQP (unmasked) semantics
+    // minimize_SQP: the K table of constrained nodes (Dirichlet DOFs, contact
+    // candidates), their active and working sets, projectors and steps
+    bool sqp_built = false;
+    int nK = 0;
+    double sqp_vtol = 0.0;
+    std::vector<int> h_surf;        // contact candidates (host copy)
+    DevMem sknode, skeq, ssurf2k, skpid, skdid, skw, skmap, spmask, sdmask, sfrozen, schg;
+    DevMem sP, sMsave, sdxfull, srhs;
     // this device's static scratch (sors_energy_noalloc), set up by sors_create
     double* scratch_dev = nullptr;
     std::string scratch_err;
@@ -3194,7 +3626,7 @@ void sync(sors_ctx* c) { HIPX(hipStreamSynchronize(c->stream)); }
 // (re)build the pattern for dt == 0 / dt != 0 and upload it
 void ensure_pattern(sors_ctx* c, double dt) {
     const int dtnz = dt != 0.0;
-    if (c->pat_dtnz == dtnz && c->pat_on_device) return;
+    if (c->pat_dtnz == dtnz && c->pat_sqp == c->sqp() && c->pat_on_device) return;
     std::string msg;
     if (!build_pattern(c->N, c->E, c->ed, c->h_conn.data(), c->has_mask ? c->h_mask.data() : nullptr, dtnz, c->sqp(),
                        c->pat, &msg))
@@ -3225,6 +3657,7 @@ void ensure_pattern(sors_ctx* c, double dt) {
         for (auto& k : row) k = nullptr;  // captured graphs hold the old buffers
     sync(c);  // host vectors of the pattern stay alive in c->pat
     c->pat_dtnz = dtnz;
+    c->pat_sqp = c->sqp();
     c->pat_on_device = true;
 }
 
@@ -3284,6 +3717,10 @@ void pressure_force(sors_ctx* c, const double* q, const double* dq, double alpha
     k_pressure_nodes<<<grid_for(a.N, kBlock), kBlock, 0, st>>>(a, c->pff.as<double>(), fp);
 }
 
+// contact enters the objective as the penalty terms (minimize_newton[_linesearch],
+// frozen Hessian); under minimize_SQP it is a set of constraints
+bool penalty_on(const sors_ctx* c) { return c->has_contact && c->cargs.nsurf > 0 && !c->sqp(); }
+
 NodeArgs node_args(sors_ctx* c, const double* q, double dt) {
     NodeArgs a{};
     if (c->has_pressure) a.fpres = c->fp0.as<double>();
@@ -3305,7 +3742,7 @@ NodeArgs node_args(sors_ctx* c, const double* q, double dt) {
     a.g = c->g.as<double>();
     a.gtmp = c->gtmp.as<double>();
     a.diag = c->diag.as<int>();
-    a.gcon = c->has_contact ? c->gcon.as<double>() : nullptr;
+    a.gcon = penalty_on(c) ? c->gcon.as<double>() : nullptr;
     return a;
 }
 
@@ -3448,7 +3885,7 @@ void assemble(sors_ctx* c, const double* q, double dt, bool hessian, int project
                 (HT*)c->vals.p, c->F());
         }
     }
-    if (c->has_contact && c->cargs.nsurf > 0) {
+    if (penalty_on(c)) {
         const bool blk = c->block();
         k_contact_terms<HT><<<grid_for(c->cargs.nsurf), kBlock, 0, c->stream>>>(
             contact_args(c), q, c->qprev.as<double>(), dt, c->gcon.as<double>(), hessian ? (HT*)c->vals.p : nullptr,
@@ -3477,7 +3914,8 @@ CoarseArgs coarse_args(sors_ctx* c) {
 
 // (re)build the coarse space for the current pattern and options
 void ensure_coarse(sors_ctx* c) {
-    const int want = (c->opt.coarse_nodes > 0 && c->block()) ? c->opt.coarse_nodes : 0;
+    // minimize_SQP's projected solves run on block-Jacobi alone
+    const int want = (c->opt.coarse_nodes > 0 && c->block() && !c->sqp()) ? c->opt.coarse_nodes : 0;
     if (want == c->coarse_built_for && (want == 0 || c->coarse_pat == c->vals.p)) return;
     if (c->side) HIPX(hipStreamSynchronize(c->side));  // no factorisation in flight on the old buffers
     c->refresh_pending = false;
@@ -3716,10 +4154,13 @@ void launch_spmv(sors_ctx* c, const PcgGrid& g, const PcgView<HT>& v, int par, i
 // partU[par] and st[par]
 template <typename HT>
 void launch_update(sors_ctx* c, const PcgGrid& g, const PcgView<HT>& v, int par, int nbU, int first,
-                   const double* inC = nullptr, int nbC = 0) {
+                   const double* inC = nullptr, int nbC = 0, const PcgProj<HT>* proj = nullptr) {
     const PcgPart part = c->pcg_part();
     const PartIO io{part.u[par], kMaxRedBlocks, part.u[par ^ 1], nbU, part.a, g.gs, inC, nbC};
-    k_pcg_update<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(v, v.st + (par ^ 1), v.st + par, io, first);
+    if (proj)
+        k_pcg_update_proj<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(v, v.st + (par ^ 1), v.st + par, io, first, *proj);
+    else
+        k_pcg_update<HT><<<g.gu, kUpdBlock, 0, c->stream>>>(v, v.st + (par ^ 1), v.st + par, io, first);
 }
 
 // u += Z E^-1 Z^T r after an update (or the init); gamma's coarse partials into slot par
@@ -3736,9 +4177,11 @@ void launch_coarse(sors_ctx* c, PcgCtl* ctl, int par) {
 // iterations (U_k, A_k), replayed from a captured hipGraph when use_graph.
 // The host reads the outcome record between chunks.  With profile_kernels
 // the first chunk of every solve is launched directly with a HIP event pair
-// around each SpMV.
+// around each SpMV.  With proj the residual is projected on the constrained
+// nodes (minimize_SQP; launched directly, without the coarse correction).
 template <typename HT>
-long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_iter, bool* breakdown) {
+long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_iter, bool* breakdown,
+              const PcgProj<HT>* proj = nullptr) {
     const PcgView<HT> v = pcg_view<HT>(c, rtol);
     PcgCtl* ctl = v.ctl;
     const PcgGrid g = pcg_grid(c);
@@ -3754,13 +4197,13 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
         }
     };
     // two-level preconditioner: a coarse correction after every update
-    const bool cs = c->coarse_ready;
+    const bool cs = c->coarse_ready && !proj;
     const int nbC = cs ? c->nagg : 0;
     auto inC = [&](int par) -> const double* { return cs ? c->cpart.as<double>() + par * kMaxRedBlocks : nullptr; };
     // iteration k >= 2 reads from st[(k-1)%2] and writes st[k%2]
     auto iteration = [&](int k, int j, bool timed) {
         const int par = k & 1;
-        launch_update<HT>(c, g, v, par, g.gu, 0, inC(par ^ 1), nbC);
+        launch_update<HT>(c, g, v, par, g.gu, 0, inC(par ^ 1), nbC, proj);
         if (cs) launch_coarse<HT>(c, ctl, par);
         if (timed)
             launch_spmv<HT>(c, g, v, par, g.gu, 0, ev[2 * j], ev[2 * j + 1], inC(par), nbC);
@@ -3772,11 +4215,14 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
     {
         const PcgPart part = c->pcg_part();
         const PartIO io{part.u[0], kMaxRedBlocks, nullptr, 0, nullptr, 0};
-        k_pcg_init<HT><<<g.gn, kBlock, 0, c->stream>>>(v, b, sign, io);
+        if (proj)
+            k_pcg_init_proj<HT><<<g.gn, kBlock, 0, c->stream>>>(v, b, sign, io, *proj);
+        else
+            k_pcg_init<HT><<<g.gn, kBlock, 0, c->stream>>>(v, b, sign, io);
     }
     if (cs) launch_coarse<HT>(c, ctl, 0);
     launch_spmv<HT>(c, g, v, 0, g.gn, 1, nullptr, nullptr, inC(0), nbC);   // A_0
-    launch_update<HT>(c, g, v, 1, g.gn, 1, inC(0), nbC);                    // U_1
+    launch_update<HT>(c, g, v, 1, g.gn, 1, inC(0), nbC, proj);              // U_1
     if (cs) launch_coarse<HT>(c, ctl, 1);
     launch_spmv<HT>(c, g, v, 1, g.gu, 0, nullptr, nullptr, inC(1), nbC);   // A_1
     HIPX(hipGetLastError());
@@ -3804,7 +4250,7 @@ long long pcg(sors_ctx* c, const double* b, double sign, double rtol, int max_it
         int si = -1;
         for (int q = 0; q < 3; ++q)
             if (k == sizes[q]) si = q;
-        if (!timed && c->opt.use_graph && si >= 0) {
+        if (!timed && c->opt.use_graph && si >= 0 && !proj) {
             hipGraphExec_t& ge = c->gexec[gi][si];
             if (!ge || c->gkey[gi][si] != c->vals.p || c->gchunk[gi][si] != k ||
                 c->gcoarse[gi][si] != (cs ? c->coarse_gen : 0)) {
@@ -3905,7 +4351,7 @@ void enqueue_energy(sors_ctx* c, const double* q, const double* dx, double alpha
     const int gnn = red_grid(c->n, kBlock);
     k_node_energy<<<gnn, kBlock, 0, t.st>>>(na, dx, alpha, diff, t.out3 + 1, red);
     k_finalize<1><<<1, kBlock, 0, t.st>>>(t.partials, gnn, t.out3 + 1);
-    if (c->has_contact && c->cargs.nsurf > 0) {
+    if (penalty_on(c)) {
         const int gcn = red_grid(c->cargs.nsurf, kBlock);
         k_contact_energy<<<gcn, kBlock, 0, t.st>>>(contact_args(c), q, dx, alpha, diff, c->qprev.as<double>(), dt,
                                                    red);
@@ -3920,7 +4366,7 @@ void enqueue_energy(sors_ctx* c, const double* q, const double* dx, double alpha
 
 // the energy scalars summed on the host: element + node (+ contact)
 double energy_sum(const sors_ctx* c, const double* s3) {
-    return s3[0] + s3[1] + ((c->has_contact && c->cargs.nsurf > 0) ? s3[2] : 0.0);
+    return s3[0] + s3[1] + (penalty_on(c) ? s3[2] : 0.0);
 }
 
 static_assert(S_ENODE == S_EELEM + 1 && S_AUX == S_EELEM + 2, "energy scalars are read as one triple");
@@ -4093,6 +4539,181 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
     return SORS_OK;
 }
 
+// ---- minimize_SQP -------------------------------------------------------
+constexpr int kSqpMaxWorkingSet = 40;  // working-set iterations per QP
+
+// the K table (nodes with Dirichlet DOFs or on the contact candidate list)
+// and the SQP buffers; built once per contact / mask set-up
+void ensure_sqp(sors_ctx* c) {
+    if (c->sqp_built) return;
+    const bool con = c->has_contact && c->cargs.nsurf > 0;
+    std::vector<int> eqbits((size_t)c->N, 0);
+    std::vector<char> isK((size_t)c->N, 0);
+    if (c->has_mask)
+        for (int i = 0; i < c->n; ++i)
+            if (c->h_mask[(size_t)i]) {
+                eqbits[(size_t)i / 3] |= 1 << (i % 3);
+                isK[(size_t)i / 3] = 1;
+            }
+    if (con)
+        for (int v : c->h_surf) isK[(size_t)v] = 1;
+    std::vector<int> knode, keq, kmap((size_t)c->N, -1), surf2k;
+    for (int v = 0; v < c->N; ++v)
+        if (isK[(size_t)v]) {
+            kmap[(size_t)v] = (int)knode.size();
+            knode.push_back(v);
+            keq.push_back(eqbits[(size_t)v]);
+        }
+    if (con)
+        for (int v : c->h_surf) surf2k.push_back(kmap[(size_t)v]);
+    c->nK = (int)knode.size();
+    hipStream_t st = c->stream;
+    upload_vec(c->sknode, knode, st);
+    upload_vec(c->skeq, keq, st);
+    upload_vec(c->skmap, kmap, st);
+    upload_vec(c->ssurf2k, surf2k, st);
+    const int nk1 = std::max(c->nK, 1), ns1 = std::max(con ? c->cargs.nsurf : 0, 1);
+    ensure(c->skpid, sizeof(int) * nk1);
+    ensure(c->skdid, sizeof(int) * nk1);
+    ensure(c->skw, sizeof(int) * nk1);
+    ensure(c->spmask, sizeof(int) * ns1);
+    ensure(c->sdmask, sizeof(int) * ns1);
+    ensure(c->sfrozen, sizeof(int));
+    ensure(c->schg, sizeof(int));
+    ensure(c->sP, sizeof(double) * 9 * nk1);
+    ensure(c->sMsave, sizeof(double) * 9 * nk1);
+    ensure(c->sdxfull, sizeof(double) * c->n);
+    ensure(c->srhs, sizeof(double) * c->n);
+    fill<int>(c->skpid.as<int>(), nk1, -1, st);
+    fill<int>(c->skdid.as<int>(), nk1, -1, st);
+    fill<int>(c->skw.as<int>(), nk1, 0, st);
+    fill<int>(c->spmask.as<int>(), ns1, -1, st);
+    fill<int>(c->sdmask.as<int>(), ns1, -1, st);
+    fill<int>(c->sfrozen.as<int>(), 1, 0, st);
+    // rows outside the working set count as violated beyond 1e-9 of the
+    // rest bounding-box diagonal
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int v = 0; v < c->N; ++v)
+        for (int k = 0; k < 3; ++k) {
+            const double x = c->h_rest[3 * (size_t)v + k];
+            if (v == 0 || x < lo[k]) lo[k] = x;
+            if (v == 0 || x > hi[k]) hi[k] = x;
+        }
+    const double diag = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                  (hi[2] - lo[2]) * (hi[2] - lo[2]));
+    c->sqp_vtol = 1e-9 * (diag > 0.0 ? diag : 1.0);
+    sync(c);
+    c->sqp_built = true;
+}
+
+SqpDev sqp_dev(sors_ctx* c) {
+    SqpDev s{};
+    s.nK = c->nK;
+    s.nsurf = (c->has_contact && c->cargs.nsurf > 0) ? c->cargs.nsurf : 0;
+    s.knode = c->sknode.as<int>();
+    s.keq = c->skeq.as<int>();
+    s.surf2k = c->ssurf2k.as<int>();
+    s.kpid = c->skpid.as<int>();
+    s.kdid = c->skdid.as<int>();
+    s.kw = c->skw.as<int>();
+    s.pmask = c->spmask.as<int>();
+    s.dmask = c->sdmask.as<int>();
+    s.frozen = c->sfrozen.as<int>();
+    s.changes = c->schg.as<int>();
+    s.dxfull = c->sdxfull.as<double>();
+    s.vtol = c->sqp_vtol;
+    return s;
+}
+
+// Solver::minimize_SQP (solver.cpp:109-197) on the device state c->q: the
+// active sets from the initial guess (update_active: once per step, as the
+// reference; an FP64 continuation keeps them), then up to max_newton SQP
+// iterations -- gradient and tangent with the SQP semantics (no masking, no
+// Dirichlet override, no penalty), the QP by working-set iterations of
+// projected PCG solves, x += dx -- until ||dx|| < dx_tol.  No line search, as
+// the reference.
+template <typename HT>
+int minimize_sqp(sors_ctx* c, double dt, sors_stats* stt, bool update_active) {
+    if (!c->block()) return fail(c, SORS_ERR_UNSUPPORTED, "minimize_SQP needs the node-block layout");
+    ensure_sqp(c);
+    const sors_options& o = c->opt;
+    hipStream_t st = c->stream;
+    double* q = c->q.as<double>();
+    double* g = c->g.as<double>();
+    double* dx = c->dxd.as<double>();
+    const SqpDev s = sqp_dev(c);
+    const ContactArgs ca = contact_args(c);
+    if (update_active && s.nsurf > 0) k_sqp_active<<<1, kSqpActiveThreads, 0, st>>>(s, ca, q);
+    const PcgProj<HT> pj{c->skmap.as<int>(), (const HT*)c->sP.p};
+    const int gK = grid_for(std::max(c->nK, 1));
+    const double* qprev = c->qprev.as<double>();
+    const double* bcval = c->bcval.as<double>();
+    bool converged = false;
+    for (int it = 0; it < o.max_newton; ++it) {
+        auto t0 = std::chrono::steady_clock::now();
+        clear_flag(c);
+        assemble<HT>(c, q, dt, true, o.project_spd);
+        const double gg = read_scalar(c, S_GG);
+        if (read_flag(c) & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
+        stt->gnorm = std::sqrt(gg);
+        if (!(gg == gg)) return fail(c, SORS_ERR_NAN, "SQP method diverged (NaN residual)");
+        build_preconditioner<HT>(c);
+        if (c->nK) k_sqp_save_minv<HT><<<gK, kBlock, 0, st>>>(s, (const HT*)c->Minv.p, (HT*)c->sMsave.p);
+        fill<double>(c->sdxfull.as<double>(), c->n, 0.0, st);
+        stt->ms_assemble += ms_since(t0);
+        auto t1 = std::chrono::steady_clock::now();
+        bool settled = false;
+        for (int pass = 0; pass < kSqpMaxWorkingSet; ++pass) {
+            if (c->nK)
+                k_sqp_setup<HT><<<gK, kBlock, 0, st>>>(s, ca, q, qprev, bcval, dt, (const HT*)c->sMsave.p,
+                                                        (HT*)c->sP.p, (HT*)c->Minv.p);
+            k_sqp_rhs<HT><<<grid_for(c->N), kBlock, 0, st>>>(c->N, c->sbase.as<int>(), c->bcol.as<int>(),
+                                                              (const HT*)c->vals.p, c->sdxfull.as<double>(), g,
+                                                              c->srhs.as<double>());
+            HIPX(hipGetLastError());
+            bool brk = false;
+            stt->pcg_iters += pcg<HT>(c, c->srhs.as<double>(), 1.0, o.pcg_rtol, o.pcg_max_iter, &brk, &pj);
+            stt->qp_iters++;
+            if (!c->nK) {
+                settled = true;
+                break;
+            }
+            fill<int>(c->schg.as<int>(), 1, 0, st);
+            k_sqp_multipliers<HT><<<gK, kBlock, 0, st>>>(s, ca, c->sbase.as<int>(), c->bcol.as<int>(),
+                                                          (const HT*)c->vals.p, (const HT*)c->px.p, g, q, qprev,
+                                                          bcval, dt);
+            HIPX(hipGetLastError());
+            int* hchg = (int*)c->h_pin;
+            HIPX(hipMemcpyAsync(hchg, c->schg.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            sync(c);
+            if (*hchg == 0) {
+                settled = true;
+                break;
+            }
+        }
+        if (!settled) return fail(c, SORS_ERR_NOCONV, "SQP working set did not settle");
+        k_sqp_step<HT><<<red_grid(c->n, kBlock), kBlock, 0, st>>>(c->n, c->sdxfull.as<double>(),
+                                                                   (const HT*)c->px.p, dx);
+        const int gd = red_grid(c->n, kBlock);
+        k_sumsq<<<gd, kBlock, 0, st>>>(c->n, dx, c->red());
+        k_finalize<1><<<1, kBlock, 0, st>>>(c->partials.as<double>(), gd, c->S() + S_DD);
+        HIPX(hipGetLastError());
+        const double dxn = std::sqrt(read_scalar(c, S_DD));
+        stt->ms_solve += ms_since(t1);
+        stt->newton_iters++;
+        if (!(dxn == dxn)) return fail(c, SORS_ERR_NAN, "SQP method diverged");
+        k_axpy<<<red_grid(c->n, kBlock), kBlock, 0, st>>>(c->n, 1.0, dx, q);
+        HIPX(hipGetLastError());
+        stt->dxnorm = dxn;
+        if (dxn < o.dx_tol) {
+            converged = true;
+            break;
+        }
+    }
+    if (!converged) return fail(c, SORS_ERR_NOCONV, "SQP method did not converge");
+    return SORS_OK;
+}
+
 // Newton's starting point for a step of size h (options.predict: 0 keep q,
 // 1 constant velocity, 2 constant acceleration once a previous step exists)
 void launch_predict(sors_ctx* c, double h) {
@@ -4121,8 +4742,6 @@ void launch_update_state(sors_ctx* c, double h) {
 int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
     if (!(dt > 0.0)) return fail(c, SORS_ERR_ARG, "dt must be > 0 (resolve settings.dt before the call)");
     if (substeps < 1) substeps = 1;
-    if (c->method == SORS_MINIMIZE_SQP)
-        return fail(c, SORS_ERR_UNSUPPORTED, "minimize_SQP (OSQP contact) is not implemented on the device");
     sors_stats local{};
     if (!st) st = &local;
     *st = sors_stats{};
@@ -4135,14 +4754,25 @@ int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
     const double h = dt / substeps;
     ensure_pattern(c, h);
     for (int j = 0; j < substeps; ++j) {
-        launch_predict(c, h);
-        contact_lag(c);
-        int rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, h, st) : minimize<double>(c, h, st);
-        if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
-            // the FP32 tangents did not bring this solve to the tolerances:
-            // it goes on from its iterate with FP64 tangents and solves
-            st->fp64_fallbacks++;
-            rc = minimize<double>(c, h, st);
+        int rc;
+        if (c->sqp()) {
+            // minimize_SQP starts from the caller's guess (solver.cpp:30-31)
+            rc = (c->opt.precision == SORS_MIXED) ? minimize_sqp<float>(c, h, st, true)
+                                                   : minimize_sqp<double>(c, h, st, true);
+            if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
+                st->fp64_fallbacks++;
+                rc = minimize_sqp<double>(c, h, st, false);
+            }
+        } else {
+            launch_predict(c, h);
+            contact_lag(c);
+            rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, h, st) : minimize<double>(c, h, st);
+            if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
+                // the FP32 tangents did not bring this solve to the tolerances:
+                // it goes on from its iterate with FP64 tangents and solves
+                st->fp64_fallbacks++;
+                rc = minimize<double>(c, h, st);
+            }
         }
         if (rc != SORS_OK) return rc;
         launch_update_state(c, h);
@@ -5084,9 +5714,9 @@ int sors_minimize(sors_ctx* c, double* q, double dt, int method, sors_stats* st)
     if (!q) return fail(c, SORS_ERR_ARG, "null positions");
     if (!(dt > 0.0)) return fail(c, SORS_ERR_ARG, "dt must be > 0");
     if (method != SORS_MINIMIZE_NEWTON && method != SORS_MINIMIZE_NEWTON_LINESEARCH &&
-        method != SORS_MINIMIZE_FROZENHESSIAN)
-        return fail(c, SORS_ERR_ARG, "method must be minimize_newton, minimize_newton_linesearch or "
-                                     "minimize_frozenhessian");
+        method != SORS_MINIMIZE_FROZENHESSIAN && method != SORS_MINIMIZE_SQP)
+        return fail(c, SORS_ERR_ARG, "method must be minimize_newton, minimize_newton_linesearch, "
+                                     "minimize_frozenhessian or minimize_SQP");
     sors_stats local{};
     if (!st) st = &local;
     *st = sors_stats{};
@@ -5099,11 +5729,21 @@ int sors_minimize(sors_ctx* c, double* q, double dt, int method, sors_stats* st)
     auto t0 = std::chrono::steady_clock::now();
     upload(c->q, q, (size_t)c->n, c->stream);
     ensure_pattern(c, dt);
-    contact_lag(c);
-    int rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, dt, st) : minimize<double>(c, dt, st);
-    if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
-        st->fp64_fallbacks++;
-        rc = minimize<double>(c, dt, st);
+    int rc;
+    if (c->sqp()) {
+        rc = (c->opt.precision == SORS_MIXED) ? minimize_sqp<float>(c, dt, st, true)
+                                               : minimize_sqp<double>(c, dt, st, true);
+        if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
+            st->fp64_fallbacks++;
+            rc = minimize_sqp<double>(c, dt, st, false);
+        }
+    } else {
+        contact_lag(c);
+        rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, dt, st) : minimize<double>(c, dt, st);
+        if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
+            st->fp64_fallbacks++;
+            rc = minimize<double>(c, dt, st);
+        }
     }
     if (rc != SORS_OK) return rc;
     HIPX(hipMemcpyAsync(q, c->q.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
@@ -5183,7 +5823,7 @@ int sors_energy_noalloc(sors_ctx* c, const double* q_dev, double dt, double* ene
     // a memset); partials and sums are overwritten by every call
     k_fill<int><<<1, 64, 0, st>>>(flag, 1, 0);
     enqueue_energy(c, q_dev, nullptr, 0.0, dt, EnergyTarget{st, sc, sc + kScratchSums, flag});
-    k_energy_total<<<1, 64, 0, st>>>(sc + kScratchSums, (c->has_contact && c->cargs.nsurf > 0) ? 1 : 0, energy_dev);
+    k_energy_total<<<1, 64, 0, st>>>(sc + kScratchSums, penalty_on(c) ? 1 : 0, energy_dev);
     HIPX(hipGetLastError());
     return SORS_OK;
     SORS_GUARD_END(c)
@@ -5491,6 +6131,7 @@ int sors_set_contact(sors_ctx* c, const sors_contact_desc* d) {
     SORS_GUARD(c)
     if (!d) {
         c->has_contact = false;
+        c->sqp_built = false;
         return SORS_OK;
     }
     if (d->n_planes < 0 || d->n_planes > kMaxPrim || d->n_disks < 0 || d->n_disks > kMaxPrim)
@@ -5500,8 +6141,6 @@ int sors_set_contact(sors_ctx* c, const sors_contact_desc* d) {
         return fail(c, SORS_ERR_ARG, "missing contact primitive arrays");
     if (!(d->stiffness >= 0.0) || !(d->friction >= 0.0) || (d->friction > 0.0 && !(d->eps_v > 0.0)))
         return fail(c, SORS_ERR_ARG, "contact needs stiffness >= 0, friction >= 0 and eps_v > 0 with friction");
-    if (c->method == SORS_MINIMIZE_SQP)
-        return fail(c, SORS_ERR_UNSUPPORTED, "contact terms apply to minimize_newton(_linesearch)");
     ContactArgs a{};
     a.nplanes = d->n_planes;
     a.ndisks = d->n_disks;
@@ -5534,6 +6173,8 @@ int sors_set_contact(sors_ctx* c, const sors_contact_desc* d) {
     }
     a.nsurf = (int)surf.size();
     upload_vec(c->csurf, surf, c->stream);
+    c->h_surf = surf;
+    c->sqp_built = false;  // the K table and the disk footprint start afresh
     ensure(c->lagN, sizeof(double) * std::max(a.nsurf, 1));
     ensure(c->lagn, sizeof(double) * 3 * std::max(a.nsurf, 1));
     ensure(c->lagD, sizeof(double) * 3 * std::max(a.nsurf, 1));

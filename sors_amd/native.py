@@ -57,7 +57,7 @@ class Stats(C.Structure):
         ("dxnorm", C.c_double), ("ms_assemble", C.c_double), ("ms_solve", C.c_double),
         ("ms_linesearch", C.c_double), ("ms_total", C.c_double), ("spmv_ms", C.c_double),
         ("spmv_launches", C.c_longlong), ("assemble_kernel_ms", C.c_double), ("assemble_launches", C.c_longlong),
-        ("fp64_fallbacks", C.c_int),
+        ("fp64_fallbacks", C.c_int), ("qp_iters", C.c_int),
     ]
 
     def as_dict(self):

@@ -103,8 +103,9 @@ def test_inverted_element_message(orc):
 def test_pokeflex_style_disk_contact_driver(orc):
     """The structure of python/d03_pokeflex.py: neumannBC + diskContact, the
     SQP solver setting, the disk moved through get_disk_contact_constraint()
-    before every step.  The device runs its penalty formulation; compared
-    with the oracle's restatement of the same formulation."""
+    before every step.  With contactModel "penalty" the device runs its
+    penalty formulation; compared with the oracle's restatement of the same
+    formulation."""
     ps = load()
     p = mesh.contact_poke(n=5, steps=20, gap=-5e-4, speed=0.05)
     sc = p.scene
@@ -114,6 +115,7 @@ def test_pokeflex_style_disk_contact_driver(orc):
     settings.meshType = "Tetrahedron"
     settings.timeSteppingScheme = "backward_euler"
     settings.solverMethod = "minimize_SQP"
+    settings.contactModel = "penalty"
     settings.dt = sc.dt
     energies = [set(["neohookean"]) for _ in range(sc.n_elems)]
     params = [ps.Params(["density", "youngsModulus", "poissonsRatio"], [1100.0, 2e5, 0.4])
@@ -182,9 +184,10 @@ def test_hopping_leg_style_muscle_driver(orc):
     elements with {"neohookean", "softconMuscle"} in a band, two muscle groups
     along +z ("muscleGroup", "softconStiffness", "softconDirection"), a ground
     plane ("planeContact" over the surface vertices), minimize_SQP, and a
-    two-channel "softconMuscle" actuation ramp.  The device runs its penalty
-    contact with plain Newton (the muscle energy is not the potential of its
-    gradient); compared with the oracle's restatement of the same model."""
+    two-channel "softconMuscle" actuation ramp.  With contactModel "penalty"
+    the device runs its penalty contact with plain Newton (the muscle energy
+    is not the potential of its gradient); compared with the oracle's
+    restatement of the same model."""
     ps = load()
     sc = mesh.beam(nx=3, ny=3, nz=10, lx=0.03, ly=0.03, lz=0.1, element="hex", E=5e5, nu=0.4, rho=1000.0,
                    tip_speed=0.0, steps=20)
@@ -193,6 +196,7 @@ def test_hopping_leg_style_muscle_driver(orc):
     settings.meshType = "Hexahedron"
     settings.timeSteppingScheme = "backward_euler"
     settings.solverMethod = "minimize_SQP"
+    settings.contactModel = "penalty"
     settings.dt = 2e-3
     cen = X[sc.conn].mean(axis=1)
     energies, params, group = [], [], np.full(sc.n_elems, -1, np.int32)
