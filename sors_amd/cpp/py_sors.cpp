@@ -11,6 +11,7 @@
 #include <array>
 #include <cmath>
 
+#include "sors_demos.h"
 #include "sors_facade.h"
 #include "sors_io.h"
 
@@ -197,6 +198,9 @@ void bind_disk(py::module_& m, const char* name) {
 
 PYBIND11_MODULE(py_sors, m) {
     m.doc() = "SORS implicit Newton step on MI355X (HIP device path behind the reference's interface)";
+    // py_sors.cpp:27
+    m.def("run_msd_timestepping", &run_msd_timestepping, "Run msd demo", py::arg("meshFilePath"),
+          py::arg("outputFolder"), py::arg("numSteps"), py::call_guard<py::gil_scoped_release>());
 
     py::class_<Params>(m, "Params")
         .def(py::init<>())
