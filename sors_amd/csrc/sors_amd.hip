@@ -3912,6 +3912,12 @@ CoarseArgs coarse_args(sors_ctx* c) {
     return a;
 }
 
+// nodes per aggregate: options.coarse_nodes, smaller on small meshes so that
+// the coarse space keeps about 32 aggregates or more (64 nodes at least); a
+// 2,000-node mesh with 2,048-node aggregates would get one aggregate, whose
+// six rigid modes barely help
+int coarse_target(int N, int coarse_nodes) { return std::min(coarse_nodes, std::max(64, N / 32)); }
+
 // (re)build the coarse space for the current pattern and options
 void ensure_coarse(sors_ctx* c) {
     // minimize_SQP's projected solves run on block-Jacobi alone
@@ -3924,7 +3930,7 @@ void ensure_coarse(sors_ctx* c) {
     if (want == 0) return;
     CoarseSpace C;
     std::string msg;
-    int target = want;
+    int target = coarse_target(c->N, want);
     for (;;) {
         if (!build_coarse(c->N, c->h_rest.data(), c->has_mask ? c->h_mask.data() : nullptr, c->pat, target, C, &msg))
             throw HipError{"coarse space: " + msg};
@@ -4739,6 +4745,31 @@ void launch_update_state(sors_ctx* c, double h) {
     c->vold_dt = h;
 }
 
+// One minimisation of a step (the Newton family or SQP) with the mixed-
+// precision continuations: a mixed solve that runs out of iterations goes on
+// from its iterate with FP64 tangents and solves; one that inverts an
+// element restarts from its initial guess in FP64 (an FP32 tangent or solve
+// can overshoot where the FP64 ones do not).  SQP keeps the step's active sets.
+int minimize_step(sors_ctx* c, double h, sors_stats* st) {
+    const bool mixed = c->opt.precision == SORS_MIXED;
+    if (mixed) {
+        ensure(c->qtmp, sizeof(double) * c->n);
+        HIPX(hipMemcpyAsync(c->qtmp.p, c->q.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    int rc;
+    if (c->sqp())
+        rc = mixed ? minimize_sqp<float>(c, h, st, true) : minimize_sqp<double>(c, h, st, true);
+    else
+        rc = mixed ? minimize<float>(c, h, st) : minimize<double>(c, h, st);
+    if (mixed && (rc == SORS_ERR_NOCONV || rc == SORS_ERR_INVERTED)) {
+        if (rc == SORS_ERR_INVERTED)
+            HIPX(hipMemcpyAsync(c->q.p, c->qtmp.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+        st->fp64_fallbacks++;
+        rc = c->sqp() ? minimize_sqp<double>(c, h, st, false) : minimize<double>(c, h, st);
+    }
+    return rc;
+}
+
 int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
     if (!(dt > 0.0)) return fail(c, SORS_ERR_ARG, "dt must be > 0 (resolve settings.dt before the call)");
     if (substeps < 1) substeps = 1;
@@ -4754,26 +4785,12 @@ int do_steps(sors_ctx* c, double dt, int substeps, sors_stats* st) {
     const double h = dt / substeps;
     ensure_pattern(c, h);
     for (int j = 0; j < substeps; ++j) {
-        int rc;
-        if (c->sqp()) {
-            // minimize_SQP starts from the caller's guess (solver.cpp:30-31)
-            rc = (c->opt.precision == SORS_MIXED) ? minimize_sqp<float>(c, h, st, true)
-                                                   : minimize_sqp<double>(c, h, st, true);
-            if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
-                st->fp64_fallbacks++;
-                rc = minimize_sqp<double>(c, h, st, false);
-            }
-        } else {
+        // minimize_SQP starts from the caller's guess (solver.cpp:30-31)
+        if (!c->sqp()) {
             launch_predict(c, h);
             contact_lag(c);
-            rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, h, st) : minimize<double>(c, h, st);
-            if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
-                // the FP32 tangents did not bring this solve to the tolerances:
-                // it goes on from its iterate with FP64 tangents and solves
-                st->fp64_fallbacks++;
-                rc = minimize<double>(c, h, st);
-            }
         }
+        const int rc = minimize_step(c, h, st);
         if (rc != SORS_OK) return rc;
         launch_update_state(c, h);
         HIPX(hipGetLastError());
@@ -4863,11 +4880,11 @@ double ctx_bytes(const sors_ctx* c) {
 double estimate_bytes(int N, int E, int ed, int coarse_nodes) {
     const double n = 3.0 * N, nblk = (double)N * (ed == 4 ? 24 : 32);
     // coarse correction: E and its inverse (nc^2 FP64 each, nc = 6 per
-    // aggregate, aggregates at most 2N / coarse_nodes and 512), the two
+    // aggregate, aggregates at most 2N / (nodes per aggregate) and 512), the two
     // Gauss-Jordan buffers (npad^2 FP64 each) and per-node data
     double coarse = 0.0;
     if (coarse_nodes > 0) {
-        const double nc = 6.0 * std::min(512.0, std::ceil(2.0 * N / coarse_nodes));
+        const double nc = 6.0 * std::min(512.0, std::ceil(2.0 * N / coarse_target(N, coarse_nodes)));
         const double np = std::ceil(nc / kGjTile) * kGjTile;
         coarse = 2.0 * 8.0 * nc * nc + 2.0 * 8.0 * np * np + 24.0 * N;
     }
@@ -5200,6 +5217,12 @@ void srv_tick(sors_server* s) {
             case Member::ASSEMBLE:
                 if (m.new_step) {
                     launch_predict(c, dt);
+                    if (mixed) {
+                        // the step's initial guess, for an FP64 restart after an inversion
+                        ensure(c->qtmp, sizeof(double) * c->n);
+                        HIPX(hipMemcpyAsync(c->qtmp.p, c->q.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice,
+                                            c->stream));
+                    }
                     m.new_step = false;
                 }
                 clear_flag(c);
@@ -5312,6 +5335,18 @@ void srv_tick(sors_server* s) {
             case Member::ASSEMBLE: {
                 const double gg = sr.scal[S_GG];
                 if (sr.flag & 1) {
+                    if (mixed && !m.fp64) {
+                        // as sors_step: a mixed solve that inverted an element
+                        // restarts from the step's initial guess in FP64
+                        HIPX(hipMemcpyAsync(c->q.p, c->qtmp.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice,
+                                            c->stream));
+                        m.fp64 = true;
+                        m.st.fp64_fallbacks++;
+                        m.it = 0;
+                        m.aimed = false;
+                        m.phase = Member::ASSEMBLE;
+                        break;
+                    }
                     status = SORS_ERR_INVERTED;
                     msg = "Inverted Elements";
                     break;
@@ -5729,22 +5764,8 @@ int sors_minimize(sors_ctx* c, double* q, double dt, int method, sors_stats* st)
     auto t0 = std::chrono::steady_clock::now();
     upload(c->q, q, (size_t)c->n, c->stream);
     ensure_pattern(c, dt);
-    int rc;
-    if (c->sqp()) {
-        rc = (c->opt.precision == SORS_MIXED) ? minimize_sqp<float>(c, dt, st, true)
-                                               : minimize_sqp<double>(c, dt, st, true);
-        if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
-            st->fp64_fallbacks++;
-            rc = minimize_sqp<double>(c, dt, st, false);
-        }
-    } else {
-        contact_lag(c);
-        rc = (c->opt.precision == SORS_MIXED) ? minimize<float>(c, dt, st) : minimize<double>(c, dt, st);
-        if (rc == SORS_ERR_NOCONV && c->opt.precision == SORS_MIXED) {
-            st->fp64_fallbacks++;
-            rc = minimize<double>(c, dt, st);
-        }
-    }
+    if (!c->sqp()) contact_lag(c);
+    const int rc = minimize_step(c, dt, st);
     if (rc != SORS_OK) return rc;
     HIPX(hipMemcpyAsync(q, c->q.p, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
     sync(c);
