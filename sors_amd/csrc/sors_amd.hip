@@ -531,9 +531,13 @@ __global__ __launch_bounds__(kBlock) void k_tet(ElemArgs a) {
 // (k_grad_gather): one lane per node sums its gradient rows.  No atomics, no
 // read-modify-write of the layout, every value written by exactly one lane,
 // fixed-order sums.  MODE 0: gradient + Hessian; MODE 1: gradient only.
-// stored values per block: 9 padded to a multiple of 16 bytes
+// A stored 3x3 element block: nine values and one pad, so every block starts
+// 8-byte aligned (FP32, 40 B) or 16-byte aligned (FP64, 80 B)
 template <typename HT>
-constexpr int tet_blk() { return sizeof(HT) == 4 ? 12 : 10; }
+constexpr int tet_blk() { return 10; }
+// the unit a stored block is moved in: 8 bytes (FP32), 16 bytes (FP64)
+template <typename HT>
+using BlkPiece = typename std::conditional<sizeof(HT) == 4, uint2, uint4>::type;
 // stored blocks per element: tet4 keeps the upper triangle
 constexpr int elem_blocks(int ed) { return ed == 4 ? 10 : ed * ed; }
 
@@ -659,14 +663,21 @@ __global__ __launch_bounds__(kBlock) void k_tet_elem(ElemArgs a, double* eg, HT*
             }
             HT* dst = eb + (size_t)tet_blk<HT>() * elem_row(e, t, 10);
             if constexpr (sizeof(HT) == 4) {
-                // the padded 12-float block as three 16-byte stores
-                float4* d4 = reinterpret_cast<float4*>(dst);
-                d4[0] = make_float4(blk[0], blk[1], blk[2], blk[3]);
-                d4[1] = make_float4(blk[4], blk[5], blk[6], blk[7]);
-                d4[2] = make_float4(blk[8], 0.0f, 0.0f, 0.0f);
+                // the padded 10-float block as five 8-byte stores
+                float2* d2 = reinterpret_cast<float2*>(dst);
+                d2[0] = make_float2(blk[0], blk[1]);
+                d2[1] = make_float2(blk[2], blk[3]);
+                d2[2] = make_float2(blk[4], blk[5]);
+                d2[3] = make_float2(blk[6], blk[7]);
+                d2[4] = make_float2(blk[8], 0.0f);
             } else {
-#pragma unroll
-                for (int i = 0; i < 9; ++i) dst[i] = blk[i];
+                // the padded 10-double block as five 16-byte stores
+                double2* d2 = reinterpret_cast<double2*>(dst);
+                d2[0] = make_double2(blk[0], blk[1]);
+                d2[1] = make_double2(blk[2], blk[3]);
+                d2[2] = make_double2(blk[4], blk[5]);
+                d2[3] = make_double2(blk[6], blk[7]);
+                d2[4] = make_double2(blk[8], 0.0);
             }
             ++t;
         }
@@ -733,12 +744,12 @@ __global__ __launch_bounds__(kBlock) void k_slot_sums(ElemArgs a, int nq, const 
     if (q >= nq) return;
     const int s = qslice[q];
     const int k = q - a.sbase[s] / kSlice;
-    constexpr int kPieces = tet_blk<HT>() * (int)sizeof(HT) / 16;
+    constexpr int kPieces = tet_blk<HT>() * (int)sizeof(HT) / (int)sizeof(BlkPiece<HT>);
     HT acc[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) acc[i] = (HT)0;
     union Blk {
-        uint4 q4[kPieces];
+        BlkPiece<HT> q4[kPieces];
         HT v[tet_blk<HT>()];
     };
     auto add = [&](const Blk& b, int code) {
@@ -760,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_slot_sums(ElemArgs a, int nq, const 
 #pragma unroll
         for (int u = 0; u < kGroup; ++u)
             if (u < m) {
-                const uint4* src = (const uint4*)(eb + (size_t)tet_blk<HT>() * block_of(id[u]));
+                const BlkPiece<HT>* src = (const BlkPiece<HT>*)(eb + (size_t)tet_blk<HT>() * block_of(id[u]));
 #pragma unroll
                 for (int i = 0; i < kPieces; ++i) blk[u].q4[i] = src[i];
             }
