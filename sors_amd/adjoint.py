@@ -36,6 +36,8 @@ class LegRollout:
         Newton converge only linearly; either would leave forward states (and
         finite differences of the loss) noisy at the solver tolerance."""
         self.leg = leg
+        self._init = dict(precision=precision, checkpoint_every=checkpoint_every, adjoint_rtol=adjoint_rtol,
+                          method=method, coarse_nodes=coarse_nodes, **solver)
         sc = leg.scene
         self.sc = sc
         self.T = sc.steps
@@ -59,6 +61,11 @@ class LegRollout:
 
     def close(self):
         self.r.close()
+
+    def clone(self):
+        """A second rollout of the same leg and options on a context (and
+        stream) of its own: its forward runs are bit-identical to this one's."""
+        return LegRollout(self.leg, **self._init)
 
     # ---- loss -------------------------------------------------------------
     def loss_of(self, q):
@@ -184,22 +191,55 @@ def rollout_loss(E, act, rollout):
     return _FN.apply(E, act, rollout)
 
 
-def fd_check(rollout, E, act, directions=3, h=1e-6, seed=0):
+def fd_check(rollout, E, act, directions=3, h=1e-6, seed=0, workers=1):
     """Central differences of the loss along seeded directions in (E, act)
     space (E perturbed relatively: E * (1 + h d_E)), against the adjoint
-    gradient.  Returns a list of (fd, adjoint) directional derivatives."""
+    gradient.  Returns a list of (fd, adjoint) directional derivatives.
+
+    workers > 1: the 2 x directions perturbed rollouts run on that many
+    clones of `rollout` (each its own device context and stream) from as many
+    host threads, so their kernels overlap on the device; every rollout is
+    deterministic, so the losses are those of a sequential run."""
     rng = np.random.default_rng(seed)
     L, gE, gA = rollout.gradient(E, act)
-    out = []
+    jobs, ads = [], []
     for _ in range(directions):
         dE = rng.normal(size=E.shape)
         dA = rng.normal(size=act.shape)
         nrm = math.sqrt(float(dE @ dE) + float((dA * dA).sum()))
         dE /= nrm
         dA /= nrm
-        Lp = rollout.loss(E * (1 + h * dE), act + h * dA)
-        Lm = rollout.loss(E * (1 - h * dE), act - h * dA)
-        fd = (Lp - Lm) / (2 * h)
-        ad = float((gE * E) @ dE) + float((gA * dA).sum())
-        out.append((fd, ad))
+        jobs += [(E * (1 + h * dE), act + h * dA), (E * (1 - h * dE), act - h * dA)]
+        ads.append(float((gE * E) @ dE) + float((gA * dA).sum()))
+    losses = perturbed_losses(rollout, jobs, workers)
+    out = [((losses[2 * k] - losses[2 * k + 1]) / (2 * h), ads[k]) for k in range(directions)]
     return L, out
+
+
+def perturbed_losses(rollout, jobs, workers=1):
+    """Losses of the rollouts (E, act) in `jobs`, in order; workers > 1 runs
+    them on that many clones of `rollout` from as many host threads."""
+    if workers <= 1 or len(jobs) <= 1:
+        return [rollout.loss(Ej, Aj) for Ej, Aj in jobs]
+    import queue
+    from concurrent.futures import ThreadPoolExecutor
+
+    n = min(workers, len(jobs))
+    pool = queue.Queue()
+    clones = [rollout.clone() for _ in range(n)]
+    for c in clones:
+        pool.put(c)
+
+    def run(job):
+        c = pool.get()
+        try:
+            return c.loss(*job)
+        finally:
+            pool.put(c)
+
+    try:
+        with ThreadPoolExecutor(n) as ex:
+            return list(ex.map(run, jobs))
+    finally:
+        for c in clones:
+            c.close()
