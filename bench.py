@@ -320,7 +320,9 @@ def adjoint_leg(args):
 def hex_leg(args, opts):
     """The same beam physics on the 200 x 40 x 40 hex8 grid (320,000
     elements, 2 x 2 x 2 Gauss points, 27-point node stencil), mixed
-    precision, against plain FP64 on the GPU for the same steps."""
+    precision, against plain FP64 on the GPU for the same steps; unless
+    --no-cpu, both precisions' first step from rest against one full step of
+    the FP64 CPU oracle."""
     from sors_amd import mesh, native
 
     sc = mesh.cantilever_large(element="hex", seed=0)
@@ -336,6 +338,21 @@ def hex_leg(args, opts):
                      "pcg_per_step": agg["pcg_iters"] / args.hex_steps}
         log(f"hex {name}: {out[name]}")
     out["max_pos_err_over_bbox_mixed_vs_fp64"] = float(np.abs(res["mixed"] - res["fp64"]).max() / sc.bbox_diag)
+    if not args.no_cpu:
+        # the first step from rest against one full step of the FP64 CPU oracle
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        cpu, q_cpu = cpu_baseline(sc, opts, cores)
+        first = {}
+        for name, prec in (("mixed", native.MIXED), ("fp64", native.FP64)):
+            r1, q1, _, _ = run_gpu(sc, prec, 1, 0, profile=False, **opts)
+            r1.close()
+            first[name] = float(np.abs(q1 - q_cpu).max() / sc.bbox_diag)
+        out["vs_cpu_oracle_first_step"] = {
+            "max_pos_err_over_bbox_mixed": first["mixed"], "max_pos_err_over_bbox_fp64": first["fp64"],
+            "moved_over_bbox": float(np.abs(q_cpu - sc.X.reshape(-1)).max() / sc.bbox_diag),
+            "oracle_seconds": cpu["seconds_per_step"], "oracle_newton": cpu["newton_iters"],
+            "oracle_pcg": cpu["pcg_iters"], "cores": cores}
+        log(f"hex first step vs CPU oracle: {out['vs_cpu_oracle_first_step']}")
     return out
 
 
