@@ -215,7 +215,9 @@ def contact_leg(args, opts):
     resting on the ground plane (penalty 1e5 N/m, Coulomb mu = 0.5), a rigid
     disk (r = 0.01 m) moving -z at 0.02 m/s with a seeded 1 mm lateral
     jitter; dt = 5e-4, mixed precision.  The first `--contact-check` steps
-    are repeated in plain FP64 for the error."""
+    are repeated in plain FP64 for the error, and unless --no-cpu
+    `--contact-oracle-steps` steps from the FP64 state at step
+    `--contact-oracle-from` are compared with the FP64 CPU oracle."""
     from sors_amd import mesh, native
 
     p = mesh.contact_poke(n=40, steps=args.contact_steps, seed=0)
@@ -223,20 +225,23 @@ def contact_leg(args, opts):
     cfg = dict(planes=[((0.0, 0.0, 0.0), (0.0, 0.0, 1.0))], disks=[(p.centers[0], (0.0, 0.0, -1.0), p.radius)],
                surface=p.surface, stiffness=p.stiffness, friction=p.friction, eps_v=p.eps_v)
 
-    def run(precision, steps):
+    def run(precision, steps, start=None, j0=0, state=False):
         r = native.Rollout.from_scene(sc, precision=precision, **opts)
         r.set_contact(**cfg)
+        if start is not None:
+            r.set_state(*start)
         agg = {"newton_iters": 0, "pcg_iters": 0, "backtracks": 0, "fp64_fallbacks": 0}
         t0 = time.perf_counter()
-        for k in range(steps):
+        for k in range(j0, j0 + steps):
             r.set_disk(0, p.centers[k + 1], (0.0, 0.0, -1.0), p.radius)
             st = r.step_resident(sc.dt)
             for key in agg:
                 agg[key] += getattr(st, key)
         el = time.perf_counter() - t0
         q = r.positions().reshape(-1, 3)
+        qv = r.state() if state else None
         r.close()
-        return q, el, agg
+        return (q, el, agg, qv) if state else (q, el, agg)
 
     q, el, agg = run(native.MIXED, sc.steps)
     c = p.centers[sc.steps]
@@ -257,6 +262,36 @@ def contact_leg(args, opts):
         out["check_steps"] = k
         out["max_pos_err_over_bbox_mixed_vs_fp64"] = float(np.abs(qm - q64).max() / sc.bbox_diag)
         out["check_steps_per_s"] = {"mixed": k / elm, "fp64": k / el64}
+    if not args.no_cpu and args.contact_oracle_steps > 0:
+        # steps against the FP64 CPU oracle (block-Jacobi PCG at the same
+        # tolerances, the same contact, friction and disk path), from the
+        # FP64 state after `--contact-oracle-from` steps (the disk pressing)
+        from oracle import oracle as O
+
+        j0 = min(args.contact_oracle_from, sc.steps)
+        k = min(args.contact_oracle_steps, sc.steps - j0)
+        start = run(native.FP64, j0, state=True)[3] if j0 > 0 else (sc.X.reshape(-1).copy(), np.zeros(3 * sc.n_nodes))
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        o = O.System.from_scene(sc, method=O.NEWTON_LS, nthreads=cores)
+        o.set_contact(**cfg)
+        o.set_initial_deformation(start[0])
+        o.set_initial_velocity(start[1])
+        qo = start[0].copy()
+        t0 = time.perf_counter()
+        for j in range(j0, j0 + k):
+            o.set_disk(0, p.centers[j + 1], (0.0, 0.0, -1.0), p.radius)
+            qo, _ = o.step(qo, sc.dt, linear_solver=O.PCG, project=1, pcg_rtol=opts["pcg_rtol"], g_tol=opts["g_tol"],
+                           nthreads=cores)
+        el_o = time.perf_counter() - t0
+        qm = run(native.MIXED, k, start=start, j0=j0)[0]
+        q64 = run(native.FP64, k, start=start, j0=j0)[0]
+        moved = float(np.abs(qo - start[0]).max() / sc.bbox_diag)
+        out["vs_cpu_oracle"] = {
+            "from_step": j0, "steps": k,
+            "max_pos_err_over_bbox_mixed": float(np.abs(qm.reshape(-1) - qo).max() / sc.bbox_diag),
+            "max_pos_err_over_bbox_fp64": float(np.abs(q64.reshape(-1) - qo).max() / sc.bbox_diag),
+            "moved_over_bbox": moved, "oracle_s_per_step": el_o / k, "cores": cores}
+        log(f"contact vs CPU oracle: {out['vs_cpu_oracle']}")
     return out
 
 
@@ -419,6 +454,10 @@ def main():
     ap.add_argument("--contact-steps", type=int, default=500, help="contact poke steps (0: skip the contact leg)")
     ap.add_argument("--contact-check", type=int, default=500, help="contact steps repeated in FP64 for the error")
     ap.add_argument("--contact-only", action="store_true", help="run only the contact leg")
+    ap.add_argument("--contact-oracle-steps", type=int, default=5,
+                    help="contact steps compared with the FP64 CPU oracle (0: skip)")
+    ap.add_argument("--contact-oracle-from", type=int, default=100,
+                    help="step (of the FP64 GPU rollout) the oracle comparison starts from")
     ap.add_argument("--adjoint-steps", type=int, default=100, help="leg rollout steps (0: skip the adjoint leg)")
     ap.add_argument("--adjoint-dirs", type=int, default=20, help="finite-difference directions checked")
     ap.add_argument("--adjoint-workers", type=int, default=8,
