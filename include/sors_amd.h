@@ -128,6 +128,14 @@ typedef struct {
                              solve runs with the previous inverse, and the
                              next tangent installs it.  PCG always runs
                              with an SPD preconditioner */
+    double pcg_rtol_first; /* > pcg_rtol: relative PCG tolerance of the first
+                             Newton solve of each minimisation (default 0: off,
+                             every solve uses pcg_rtol).  The first Newton step
+                             from the predicted start is limited by the
+                             problem's nonlinearity (its residual falls by about
+                             1e-2 however exactly it is solved), so a looser
+                             first solve saves PCG iterations without changing
+                             the converged state, which g_tol / dx_tol fix */
 } sors_options;
 
 typedef struct {

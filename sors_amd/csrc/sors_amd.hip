@@ -4462,6 +4462,14 @@ int minimize_frozen(sors_ctx* c, double dt, sors_stats* stt) {
     return SORS_OK;
 }
 
+// relative PCG tolerance of Newton iteration `it` before the g_tol
+// loosening: pcg_rtol_first (when set above pcg_rtol) for the first solve of
+// a minimisation, whose reduction the nonlinearity of the first Newton step
+// caps anyway, pcg_rtol for the others
+double forcing_base(const sors_options& o, int it) {
+    return it == 0 && o.pcg_rtol_first > o.pcg_rtol ? o.pcg_rtol_first : o.pcg_rtol;
+}
+
 // minimize_newton / minimize_newton_linesearch on the device state c->q
 // (minimize_frozenhessian above)
 template <typename HT>
@@ -4500,12 +4508,14 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
         stt->ms_assemble += ms_since(t0);
         auto t1 = std::chrono::steady_clock::now();
         bool brk = false;
-        // forcing term: pcg_rtol, loosened when the residual criterion is in
-        // reach (the linear residual of the step only has to drop below
-        // g_tol: eta ||g|| < g_tol / 2), never above 0.1
-        double eta = o.pcg_rtol;
+        // forcing term: pcg_rtol (pcg_rtol_first for the minimisation's first
+        // solve), loosened when the residual criterion is in reach (the linear
+        // residual of the step only has to drop below g_tol: eta ||g|| <
+        // g_tol / 2), never above 0.1
+        const double base = forcing_base(o, it);
+        double eta = base;
         if (o.g_tol > 0 && stt->gnorm > 0) eta = std::min(0.1, std::max(eta, 0.5 * o.g_tol / stt->gnorm));
-        aimed = o.g_tol > 0 && eta > o.pcg_rtol;
+        aimed = o.g_tol > 0 && eta > base;
         stt->pcg_iters += pcg<HT>(c, g, -1.0, eta, o.pcg_max_iter, &brk);
         enqueue_step_dots<HT>(c, g, dx);
         double dots[2];
@@ -5376,10 +5386,11 @@ void srv_tick(sors_server* s) {
                     m.aimed = false;
                     break;
                 }
-                double eta = o.pcg_rtol;
+                const double base = forcing_base(o, m.it);
+                double eta = base;
                 if (o.g_tol > 0 && m.st.gnorm > 0) eta = std::min(0.1, std::max(eta, 0.5 * o.g_tol / m.st.gnorm));
                 m.eta = eta;
-                m.aimed = o.g_tol > 0 && eta > o.pcg_rtol;
+                m.aimed = o.g_tol > 0 && eta > base;
                 // preconditioner only for a tangent that is solved (as minimize()
                 // does); the next tick's batched launches wait for it
                 if (in_fp64(m))
@@ -5501,6 +5512,7 @@ void sors_default_options(sors_options* o) {
     o->predict = 2;
     o->coarse_nodes = 2048;
     o->coarse_every = 16;
+    o->pcg_rtol_first = 0.0;
 }
 
 int sors_create(const sors_system_desc* d, sors_ctx** out) {
@@ -5680,6 +5692,8 @@ int sors_set_options(sors_ctx* c, const sors_options* o) {
     if (o->max_newton < 1 || o->pcg_max_iter < 1 || o->max_backtrack < 0)
         return fail(c, SORS_ERR_ARG, "iteration limits must be positive");
     if (o->coarse_nodes < 0) return fail(c, SORS_ERR_ARG, "coarse_nodes must be >= 0");
+    if (!(o->pcg_rtol_first >= 0.0 && o->pcg_rtol_first < 1.0))
+        return fail(c, SORS_ERR_ARG, "pcg_rtol_first must be in [0, 1)");
     c->opt = *o;
     return SORS_OK;
 }

@@ -48,6 +48,7 @@ class Options(C.Structure):
         ("pcg_check_every", C.c_int), ("use_graph", C.c_int), ("profile_kernels", C.c_int), ("predict", C.c_int),
         ("coarse_nodes", C.c_int),
         ("coarse_every", C.c_int),
+        ("pcg_rtol_first", C.c_double),
     ]
 
 

@@ -212,6 +212,30 @@ def test_cantilever_small_trajectory(orc, precision):
     assert worst <= 1e-6 * sc.bbox_diag
 
 
+def test_cantilever_small_trajectory_bench_solver(orc):
+    """The same 200 steps in mixed precision with the bench's solver settings
+    (PCG rtol 3e-3, 3e-2 for each step's first Newton solve, ||g|| < 1e-8 N,
+    two-level preconditioner): still within 1e-6 of the bounding-box
+    diagonal of the oracle, and the looser first solves take fewer PCG
+    iterations than rtol 3e-3 throughout."""
+    sc = mesh.cantilever_small()
+    o = orc.System.from_scene(sc, method=orc.NEWTON_LS)
+    base = dict(precision=native.MIXED, pcg_rtol=3e-3, g_tol=1e-8, coarse_nodes=64, coarse_every=4)
+    runs = {f: rollout(sc, pcg_rtol_first=f, **base) for f in (3e-2, 0.0)}
+    q = {f: sc.X.reshape(-1).copy() for f in runs}
+    qo = sc.X.reshape(-1).copy()
+    worst, pcg = 0.0, {f: 0 for f in runs}
+    for k in range(sc.steps):
+        qo, _ = o.step(qo, sc.dt)
+        for f, r in runs.items():
+            q[f], st = r.step(q[f], sc.dt)
+            pcg[f] += st.pcg_iters
+        worst = max(worst, np.abs(q[3e-2] - qo).max())
+    say("worst |dq| / bbox diag", worst / sc.bbox_diag, "PCG iterations", pcg)
+    assert worst <= 1e-6 * sc.bbox_diag
+    assert pcg[3e-2] < pcg[0.0]
+
+
 @pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
 def test_graph_replay_matches_direct_launches(precision):
     """PCG chunks replayed from a captured hipGraph give the same states as

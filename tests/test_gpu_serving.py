@@ -80,6 +80,26 @@ def test_serving_two_level_pcg_matches_alone(every):
     assert it_c < it_b
 
 
+def test_serving_first_solve_tolerance_matches_alone():
+    """pcg_rtol_first (a looser first Newton solve per step) is applied by the
+    served Newton phases exactly as by sors_step: served equals alone bit for
+    bit, and the looser first solves take fewer PCG iterations in total."""
+    first = dict(pcg_rtol_first=3e-2, coarse_nodes=64, coarse_every=2)
+    reqs = [small_request(i, seed=5) for i in range(6)]
+    res = {}
+    for name, extra in (("first", first), ("plain", dict(first, pcg_rtol_first=0.0))):
+        srv = native.Server(max_active=4, prep_threads=2, pcg_chunk=8, **SOLVER, **extra)
+        ids = [srv.submit(sc, sc.steps, sc.dt) for sc in reqs]
+        res[name] = [srv.wait(i, timeout=120.0) for i in ids]
+        srv.close()
+    it = {name: sum(rs["pcg_iters"] for _, rs in out) for name, out in res.items()}
+    say("PCG iterations", it)
+    for sc, (q, rs) in zip(reqs, res["first"]):
+        assert rs["status"] == native.OK
+        assert np.array_equal(q, alone(sc, **first))
+    assert it["first"] < it["plain"]
+
+
 def test_serving_single_request_matches_plain_fp64():
     sc = small_request(3)
     srv = native.Server(max_active=2, prep_threads=1, precision=native.FP64, **SOLVER)
