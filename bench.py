@@ -73,8 +73,9 @@ def run_gpu(sc, precision, steps, warmup, profile, **options):
 
 def solver_opts(args):
     """Solver options shared by the beam, hex and contact legs."""
-    return dict(pcg_rtol=args.pcg_rtol, pcg_rtol_first=args.pcg_rtol_first, g_tol=args.g_tol, predict=args.predictor,
-                coarse_nodes=args.coarse_nodes, coarse_every=args.coarse_every)
+    return dict(pcg_rtol=args.pcg_rtol, pcg_rtol_first=args.pcg_rtol_first, reuse_tangent=args.reuse_tangent,
+                g_tol=args.g_tol, predict=args.predictor, coarse_nodes=args.coarse_nodes,
+                coarse_every=args.coarse_every)
 
 
 def cpu_baseline(sc, opts, cores):
@@ -434,6 +435,8 @@ def main():
     ap.add_argument("--pcg-rtol", type=float, default=3e-3)
     ap.add_argument("--pcg-rtol-first", type=float, default=3e-2,
                     help="PCG tolerance of each step's first Newton solve (0: --pcg-rtol)")
+    ap.add_argument("--reuse-tangent", type=int, default=1, choices=[0, 1],
+                    help="a solve aimed at g_tol may reuse the last tangent (options.reuse_tangent)")
     ap.add_argument("--g-tol", type=float, default=1e-8, help="Newton residual criterion ||g||_2 (N)")
     ap.add_argument("--no-predict", action="store_true", help="start Newton from the previous positions")
     ap.add_argument("--predictor", type=int, default=2, choices=[0, 1, 2],
@@ -524,7 +527,7 @@ def main():
             "workload": f"{sc.name}: {sc.n_elems} {'tet4' if sc.ed == 4 else 'hex8'}, {sc.n_nodes} nodes, "
                         f"{n} DOFs, Hessian nnz {nnz}; E log-uniform [5e5,5e6] seed 0, nu 0.45, BE dt 1e-3, "
                         f"Newton + Armijo (||g|| < {args.g_tol:g} N or ||dx|| < 1e-9 m), PCG rtol {args.pcg_rtol:g} "
-                        f"({args.pcg_rtol_first:g} for a step's first solve) "
+                        f"({args.pcg_rtol_first:g} for a step's first solve{', tangent reuse' if args.reuse_tangent else ''}) "
                         f"preconditioned by block-Jacobi + rigid-body coarse correction ({args.coarse_nodes} nodes "
                         f"per aggregate, refreshed every {args.coarse_every} tangents), mixed FP32/FP64",
             "elements": sc.n_elems, "nodes": sc.n_nodes, "nnz": nnz, "precision": "mixed",

@@ -136,6 +136,15 @@ typedef struct {
                              1e-2 however exactly it is solved), so a looser
                              first solve saves PCG iterations without changing
                              the converged state, which g_tol / dx_tol fix */
+    int reuse_tangent;    /* 1: when the last Newton solve's forcing term predicts a
+                             residual within reach of g_tol (the next solve would
+                             be aimed at it), the next iteration evaluates the
+                             gradient alone first and, if the residual is within
+                             reach but not converged, solves with the last
+                             tangent and its preconditioner once more instead of
+                             assembling a new one (default 0: every solve gets a
+                             fresh tangent).  Needs g_tol > 0; the converged state
+                             is fixed by g_tol / dx_tol as before */
 } sors_options;
 
 typedef struct {

@@ -4470,6 +4470,21 @@ double forcing_base(const sors_options& o, int it) {
     return it == 0 && o.pcg_rtol_first > o.pcg_rtol ? o.pcg_rtol_first : o.pcg_rtol;
 }
 
+// a residual of norm g is within reach of g_tol when the solve that follows
+// would be aimed at it (its forcing term would come from g_tol)
+bool in_reach(const sors_options& o, int it, double g) {
+    return o.g_tol > 0 && g > 0 && 0.5 * o.g_tol / g > forcing_base(o, it);
+}
+
+// options.reuse_tangent: Newton iteration `it` evaluates its gradient alone
+// first and, should the residual be within reach of g_tol, solves with the
+// tangent (and preconditioner) of the last solve once more, when that last
+// solve's forcing term eta applied to its residual g predicts a residual
+// within reach and the last solve did not reuse its tangent already
+bool may_reuse_tangent(const sors_options& o, int it, bool reused, double eta, double g) {
+    return o.reuse_tangent && it >= 1 && !reused && in_reach(o, it, eta * g);
+}
+
 // minimize_newton / minimize_newton_linesearch on the device state c->q
 // (minimize_frozenhessian above)
 template <typename HT>
@@ -4483,11 +4498,13 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
     // the previous solve was aimed at the residual criterion (its forcing
     // term came from g_tol): the next residual most likely passes, so its
     // gradient is assembled alone first and the tangent only if it does not
-    bool aimed = false;
+    bool aimed = false, reused = false;
+    double last_eta = 0.0;
     for (int it = 0; it < o.max_newton; ++it) {
         auto t0 = std::chrono::steady_clock::now();
-        bool assembled = false;
-        for (int pass = aimed ? 0 : 1; pass < 2 && !assembled; ++pass) {
+        bool assembled = false, reuse = false;
+        const bool may_reuse = may_reuse_tangent(o, it, reused, last_eta, stt->gnorm);
+        for (int pass = aimed || may_reuse ? 0 : 1; pass < 2 && !assembled; ++pass) {
             clear_flag(c);
             assemble<HT>(c, q, dt, pass == 1, o.project_spd);
             const double gg = read_scalar(c, S_GG);
@@ -4498,13 +4515,18 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
                 converged = true;
                 break;
             }
+            if (pass == 0 && may_reuse && in_reach(o, it, stt->gnorm)) {
+                reuse = true;  // the matrix and preconditioner of the last solve
+                break;
+            }
             assembled = pass == 1;
         }
         if (converged) {
             stt->ms_assemble += ms_since(t0);
             break;
         }
-        build_preconditioner<HT>(c);  // only for a tangent that is solved with
+        if (!reuse) build_preconditioner<HT>(c);  // only for a tangent that is solved with
+        reused = reuse;
         stt->ms_assemble += ms_since(t0);
         auto t1 = std::chrono::steady_clock::now();
         bool brk = false;
@@ -4516,6 +4538,7 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
         double eta = base;
         if (o.g_tol > 0 && stt->gnorm > 0) eta = std::min(0.1, std::max(eta, 0.5 * o.g_tol / stt->gnorm));
         aimed = o.g_tol > 0 && eta > base;
+        last_eta = eta;
         stt->pcg_iters += pcg<HT>(c, g, -1.0, eta, o.pcg_max_iter, &brk);
         enqueue_step_dots<HT>(c, g, dx);
         double dots[2];
@@ -4868,6 +4891,11 @@ struct Member {
     // aimed: the last solve's forcing term came from g_tol, so the next
     // residual check assembles the gradient alone (grad_only), as in minimize()
     bool aimed = false, grad_only = false;
+    // options.reuse_tangent, as in minimize(): may_reuse for this Newton
+    // iteration, reused when the last solve reused its tangent, check_failed
+    // after a gradient-only check that neither converged nor reuses (the
+    // tangent is assembled on the next tick)
+    bool may_reuse = false, reused = false, check_failed = false;
     // the current solve runs in FP64: an all-FP64 server, or a mixed solve that
     // reached max_newton and goes on from its iterate in FP64 (as sors_step)
     bool fp64 = false;
@@ -5247,7 +5275,9 @@ void srv_tick(sors_server* s) {
                     m.new_step = false;
                 }
                 clear_flag(c);
-                m.grad_only = m.aimed;
+                m.may_reuse = !m.check_failed && may_reuse_tangent(o, m.it, m.reused, m.eta, m.st.gnorm);
+                m.grad_only = !m.check_failed && (m.aimed || m.may_reuse);
+                m.check_failed = false;
                 if (in_fp64(m))
                     assemble<double>(c, q, dt, !m.grad_only, o.project_spd);
                 else
@@ -5306,6 +5336,7 @@ void srv_tick(sors_server* s) {
             m.aimed = false;
             m.fp64 = false;
             m.it = 0;
+            m.reused = m.check_failed = false;
             m.phase = Member::ASSEMBLE;
         };
         auto next_newton = [&] {
@@ -5316,6 +5347,7 @@ void srv_tick(sors_server* s) {
                     m.fp64 = true;
                     m.st.fp64_fallbacks++;
                     m.it = 0;
+                    m.reused = m.check_failed = false;
                     m.aimed = false;
                     m.phase = Member::ASSEMBLE;
                     return;
@@ -5364,6 +5396,7 @@ void srv_tick(sors_server* s) {
                         m.fp64 = true;
                         m.st.fp64_fallbacks++;
                         m.it = 0;
+                        m.reused = m.check_failed = false;
                         m.aimed = false;
                         m.phase = Member::ASSEMBLE;
                         break;
@@ -5382,8 +5415,10 @@ void srv_tick(sors_server* s) {
                     end_step();
                     break;
                 }
-                if (m.grad_only) {  // the gradient-only check failed: the tangent on the next tick
+                const bool reuse = m.grad_only && m.may_reuse && in_reach(o, m.it, m.st.gnorm);
+                if (m.grad_only && !reuse) {  // the gradient-only check failed: the tangent on the next tick
                     m.aimed = false;
+                    m.check_failed = true;
                     break;
                 }
                 const double base = forcing_base(o, m.it);
@@ -5391,12 +5426,16 @@ void srv_tick(sors_server* s) {
                 if (o.g_tol > 0 && m.st.gnorm > 0) eta = std::min(0.1, std::max(eta, 0.5 * o.g_tol / m.st.gnorm));
                 m.eta = eta;
                 m.aimed = o.g_tol > 0 && eta > base;
+                m.reused = reuse;
                 // preconditioner only for a tangent that is solved (as minimize()
-                // does); the next tick's batched launches wait for it
-                if (in_fp64(m))
-                    build_preconditioner<double>(c);
-                else
-                    build_preconditioner<float>(c);
+                // does; a reused tangent keeps its own); the next tick's batched
+                // launches wait for it
+                if (!reuse) {
+                    if (in_fp64(m))
+                        build_preconditioner<double>(c);
+                    else
+                        build_preconditioner<float>(c);
+                }
                 HIPX(hipEventRecord(c->evPre, c->stream));
                 m.phase = Member::PCG_START;
                 break;
@@ -5513,6 +5552,7 @@ void sors_default_options(sors_options* o) {
     o->coarse_nodes = 2048;
     o->coarse_every = 16;
     o->pcg_rtol_first = 0.0;
+    o->reuse_tangent = 0;
 }
 
 int sors_create(const sors_system_desc* d, sors_ctx** out) {
@@ -5694,6 +5734,7 @@ int sors_set_options(sors_ctx* c, const sors_options* o) {
     if (o->coarse_nodes < 0) return fail(c, SORS_ERR_ARG, "coarse_nodes must be >= 0");
     if (!(o->pcg_rtol_first >= 0.0 && o->pcg_rtol_first < 1.0))
         return fail(c, SORS_ERR_ARG, "pcg_rtol_first must be in [0, 1)");
+    if (o->reuse_tangent != 0 && o->reuse_tangent != 1) return fail(c, SORS_ERR_ARG, "reuse_tangent must be 0 or 1");
     c->opt = *o;
     return SORS_OK;
 }

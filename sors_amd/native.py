@@ -49,6 +49,7 @@ class Options(C.Structure):
         ("coarse_nodes", C.c_int),
         ("coarse_every", C.c_int),
         ("pcg_rtol_first", C.c_double),
+        ("reuse_tangent", C.c_int),
     ]
 
 

@@ -214,26 +214,34 @@ def test_cantilever_small_trajectory(orc, precision):
 
 def test_cantilever_small_trajectory_bench_solver(orc):
     """The same 200 steps in mixed precision with the bench's solver settings
-    (PCG rtol 3e-3, 3e-2 for each step's first Newton solve, ||g|| < 1e-8 N,
-    two-level preconditioner): still within 1e-6 of the bounding-box
-    diagonal of the oracle, and the looser first solves take fewer PCG
-    iterations than rtol 3e-3 throughout."""
+    (PCG rtol 3e-3, 3e-2 for each step's first Newton solve, a solve aimed at
+    ||g|| < 1e-8 N reusing the last tangent, two-level preconditioner): still
+    within 1e-6 of the bounding-box diagonal of the oracle; the looser first
+    solves take fewer PCG iterations than rtol 3e-3 throughout, and the reuse
+    assembles fewer tangents."""
     sc = mesh.cantilever_small()
     o = orc.System.from_scene(sc, method=orc.NEWTON_LS)
-    base = dict(precision=native.MIXED, pcg_rtol=3e-3, g_tol=1e-8, coarse_nodes=64, coarse_every=4)
-    runs = {f: rollout(sc, pcg_rtol_first=f, **base) for f in (3e-2, 0.0)}
-    q = {f: sc.X.reshape(-1).copy() for f in runs}
+    base = dict(precision=native.MIXED, pcg_rtol=3e-3, g_tol=1e-8, coarse_nodes=64, coarse_every=4,
+                profile_kernels=1)
+    cases = {"bench": dict(pcg_rtol_first=3e-2, reuse_tangent=1), "no_first": dict(reuse_tangent=1),
+             "no_reuse": dict(pcg_rtol_first=3e-2)}
+    runs = {k: rollout(sc, **base, **v) for k, v in cases.items()}
+    q = {k: sc.X.reshape(-1).copy() for k in runs}
     qo = sc.X.reshape(-1).copy()
-    worst, pcg = 0.0, {f: 0 for f in runs}
-    for k in range(sc.steps):
+    worst = {k: 0.0 for k in runs}
+    pcg = {k: 0 for k in runs}
+    tangents = {k: 0 for k in runs}
+    for _ in range(sc.steps):
         qo, _ = o.step(qo, sc.dt)
-        for f, r in runs.items():
-            q[f], st = r.step(q[f], sc.dt)
-            pcg[f] += st.pcg_iters
-        worst = max(worst, np.abs(q[3e-2] - qo).max())
-    say("worst |dq| / bbox diag", worst / sc.bbox_diag, "PCG iterations", pcg)
-    assert worst <= 1e-6 * sc.bbox_diag
-    assert pcg[3e-2] < pcg[0.0]
+        for k, r in runs.items():
+            q[k], st = r.step(q[k], sc.dt)
+            pcg[k] += st.pcg_iters
+            tangents[k] += st.assemble_launches
+            worst[k] = max(worst[k], np.abs(q[k] - qo).max())
+    say("worst |dq| / bbox diag", {k: w / sc.bbox_diag for k, w in worst.items()}, "PCG", pcg, "tangents", tangents)
+    assert max(worst.values()) <= 1e-6 * sc.bbox_diag
+    assert pcg["bench"] < pcg["no_first"]
+    assert tangents["bench"] < tangents["no_reuse"]
 
 
 @pytest.mark.parametrize("precision", [native.FP64, native.MIXED])

@@ -80,11 +80,12 @@ def test_serving_two_level_pcg_matches_alone(every):
     assert it_c < it_b
 
 
-def test_serving_first_solve_tolerance_matches_alone():
-    """pcg_rtol_first (a looser first Newton solve per step) is applied by the
+def test_serving_bench_solver_options_match_alone():
+    """pcg_rtol_first (a looser first Newton solve per step) and reuse_tangent
+    (a solve aimed at g_tol may reuse the last tangent) are applied by the
     served Newton phases exactly as by sors_step: served equals alone bit for
     bit, and the looser first solves take fewer PCG iterations in total."""
-    first = dict(pcg_rtol_first=3e-2, coarse_nodes=64, coarse_every=2)
+    first = dict(pcg_rtol_first=3e-2, reuse_tangent=1, coarse_nodes=64, coarse_every=2)
     reqs = [small_request(i, seed=5) for i in range(6)]
     res = {}
     for name, extra in (("first", first), ("plain", dict(first, pcg_rtol_first=0.0))):
