@@ -4101,6 +4101,8 @@ constexpr int kSpmvBlocks = SORS_SPMV_BLOCKS;   // multiple of 8 (XCD-contiguous
 #endif
 constexpr int kUpdMinPerThread = SORS_UPD_MIN_PER_THREAD;
 constexpr int kUpdBlocks = SORS_UPD_BLOCKS;
+static_assert(kSpmvBlocks <= kMaxRedBlocks && kUpdBlocks <= kMaxRedBlocks,
+              "a PCG kernel's blocks each write a partial-sum slot (kMaxRedBlocks per reduction)");
 
 struct PcgGrid {
     int gn;     // init (per node)
