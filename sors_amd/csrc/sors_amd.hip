@@ -4321,6 +4321,16 @@ double read_scalar(sors_ctx* c, int slot) {
     return c->h_pin[0];
 }
 
+// ||g||^2 (scal[S_GG]) and the inverted-element flag of an assembly, read
+// back with one wait
+double read_gg_flag(sors_ctx* c, int* flag) {
+    HIPX(hipMemcpyAsync(c->h_pin, c->S() + S_GG, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipMemcpyAsync(c->h_pin + 1, c->F(), sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    *flag = *(const int*)(c->h_pin + 1);
+    return c->h_pin[0];
+}
+
 void read_scalars(sors_ctx* c, double* out, int first, int count) {
     HIPX(hipMemcpyAsync(c->h_pin, c->S() + first, sizeof(double) * count, hipMemcpyDeviceToHost,
                         c->stream));
@@ -4435,8 +4445,9 @@ int minimize_frozen(sors_ctx* c, double dt, sors_stats* stt) {
         auto t0 = std::chrono::steady_clock::now();
         clear_flag(c);
         assemble<HT>(c, q, dt, it == 0, o.project_spd);
-        const double gg = read_scalar(c, S_GG);
-        if (read_flag(c) & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
+        int flag = 0;
+        const double gg = read_gg_flag(c, &flag);
+        if (flag & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
         stt->gnorm = std::sqrt(gg);
         if (!(gg == gg)) return fail(c, SORS_ERR_NAN, "Frozen Newton's method diverged (NaN residual)");
         if (o.g_tol > 0 && stt->gnorm < o.g_tol) {
@@ -4509,8 +4520,9 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
         for (int pass = aimed || may_reuse ? 0 : 1; pass < 2 && !assembled; ++pass) {
             clear_flag(c);
             assemble<HT>(c, q, dt, pass == 1, o.project_spd);
-            const double gg = read_scalar(c, S_GG);
-            if (read_flag(c) & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
+            int flag = 0;
+            const double gg = read_gg_flag(c, &flag);
+            if (flag & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
             stt->gnorm = std::sqrt(gg);
             if (!(gg == gg)) return fail(c, SORS_ERR_NAN, "Newton's method diverged (NaN residual)");
             if (o.g_tol > 0 && stt->gnorm < o.g_tol) {
@@ -4543,8 +4555,14 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
         last_eta = eta;
         stt->pcg_iters += pcg<HT>(c, g, -1.0, eta, o.pcg_max_iter, &brk);
         enqueue_step_dots<HT>(c, g, dx);
-        double dots[2];
-        read_scalars(c, dots, S_DD, 2);
+        // Newton + Armijo with a residual criterion: the line search's first
+        // trial (alpha = 1) is evaluated with the step dots and read back with
+        // them (one wait fewer per iteration)
+        static_assert(S_GD == S_DD + 1 && S_EELEM == S_DD + 2, "dots and energy scalars are read as one range");
+        const bool first_trial = c->method == SORS_MINIMIZE_NEWTON_LINESEARCH && o.g_tol > 0;
+        if (first_trial) enqueue_energy(c, q, dx, 1.0, dt);
+        double dots[5];
+        read_scalars(c, dots, S_DD, first_trial ? 5 : 2);
         stt->ms_solve += ms_since(t1);
         stt->newton_iters++;
         const double dxn = std::sqrt(dots[0]), gdx = dots[1];
@@ -4564,7 +4582,7 @@ int minimize(sors_ctx* c, double dt, sors_stats* stt) {
         double alpha = 1.0;
         bool accepted = false;
         for (int bt = 0; bt <= o.max_backtrack; ++bt) {
-            const double dE = energy_delta(c, q, dx, alpha, dt);
+            const double dE = bt == 0 && first_trial ? energy_sum(c, dots + 2) : energy_delta(c, q, dx, alpha, dt);
             if (dE <= 0.1 * alpha * gdx) {
                 accepted = true;
                 break;
@@ -4705,8 +4723,9 @@ int minimize_sqp(sors_ctx* c, double dt, sors_stats* stt, bool update_active) {
         auto t0 = std::chrono::steady_clock::now();
         clear_flag(c);
         assemble<HT>(c, q, dt, true, o.project_spd);
-        const double gg = read_scalar(c, S_GG);
-        if (read_flag(c) & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
+        int flag = 0;
+        const double gg = read_gg_flag(c, &flag);
+        if (flag & 1) return fail(c, SORS_ERR_INVERTED, "Inverted Elements");
         stt->gnorm = std::sqrt(gg);
         if (!(gg == gg)) return fail(c, SORS_ERR_NAN, "SQP method diverged (NaN residual)");
         build_preconditioner<HT>(c);
