@@ -121,9 +121,9 @@ typedef struct {
                              about this many nodes (sliced node-block layout
                              only); 0: block-Jacobi alone */
     int coarse_every;     /* coarse operator refreshed on every k-th tangent
-                             (default 16; <= 1: every tangent, factorised
+                             (default 16; <= 1: every tangent, inverted
                              before the solve); with k > 1 a refresh is
-                             factorised on a second stream ahead of the
+                             inverted on a second stream ahead of the
                              solve's kernels (the host does not wait), the
                              solve runs with the previous inverse, and the
                              next tangent installs it.  PCG always runs

@@ -3511,9 +3511,9 @@ struct sors_ctx {
     int coarse_gen = 0;             // bumped on every rebuild (captured graphs hold its buffers)
     long long coarse_tangents = 0;  // tangents seen since the coarse space was built
     hipStream_t side = nullptr;     // coarse inverse (default priority, like the main stream)
-    hipEvent_t evG = nullptr, evR = nullptr;  // E assembled / factorisation done
+    hipEvent_t evG = nullptr, evR = nullptr;  // E assembled / inversion done
     hipEvent_t evPre = nullptr;     // served rollouts: preconditioner built (the batched PCG waits)
-    bool refresh_pending = false;   // a factorisation on the side stream not yet collected
+    bool refresh_pending = false;   // an inversion on the side stream not yet collected
     const void* coarse_pat = nullptr;
     bool coarse_ready = false;      // E^-1 valid for the current matrix
     int nagg = 0, nc = 0, npairs = 0, ngchunks = 0;
@@ -3934,7 +3934,7 @@ void ensure_coarse(sors_ctx* c) {
     // minimize_SQP's projected solves run on block-Jacobi alone
     const int want = (c->opt.coarse_nodes > 0 && c->block() && !c->sqp()) ? c->opt.coarse_nodes : 0;
     if (want == c->coarse_built_for && (want == 0 || c->coarse_pat == c->vals.p)) return;
-    if (c->side) HIPX(hipStreamSynchronize(c->side));  // no factorisation in flight on the old buffers
+    if (c->side) HIPX(hipStreamSynchronize(c->side));  // no inversion in flight on the old buffers
     c->refresh_pending = false;
     c->coarse_built_for = 0;
     c->coarse_ready = false;
@@ -4024,7 +4024,7 @@ void coarse_factor_async(sors_ctx* c) {
     c->refresh_pending = true;
 }
 
-// wait for the factorisation in flight; on success the main stream installs
+// wait for the inversion in flight; on success the main stream installs
 // E^-1 = (X + X^T)/2 and the coarse correction is on
 void coarse_collect(sors_ctx* c) {
     if (!c->refresh_pending) return;
