@@ -2671,11 +2671,13 @@ __global__ __launch_bounds__(kGjThreads) void k_spd_gj_step(int npad, int k0, co
 // preconditioner.  The coarse solve stays in FP64 at every precision: E is
 // as badly conditioned as the coarse problem, and an FP32 copy of its
 // inverse need not be positive definite.
-__global__ void k_coarse_symmetrize(int nc, int ld, const double* X, double* Einv) {
+__global__ void k_coarse_symmetrize(int nc, int ld, const double* X, double* Einv, float* Einv32) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nc * nc) return;
     const int a = (int)(t / nc), b = (int)(t % nc);
-    Einv[t] = 0.5 * (X[(size_t)a * ld + b] + X[(size_t)b * ld + a]);
+    const double e = 0.5 * (X[(size_t)a * ld + b] + X[(size_t)b * ld + a]);
+    Einv[t] = e;
+    Einv32[t] = (float)e;  // the copy the mixed-precision apply reads
 }
 
 // rc = Z^T r: one workgroup per aggregate, its nodes in a fixed order, a
@@ -2730,18 +2732,18 @@ constexpr int kMaxCoarse = 6 * SORS_MAX_AGGREGATES;  // coarse unknowns: nc^2 FP
 constexpr int kApplyThreads = SORS_APPLY_THREADS;
 static_assert(kMaxCoarse / 6 <= kMaxRedBlocks, "one gamma partial per aggregate");
 template <typename HT>
-__device__ void coarse_apply_body(const CoarseArgs& c, const PcgCtl* ctl, const double* __restrict__ Einv, HT* u,
+__device__ void coarse_apply_body(const CoarseArgs& c, const PcgCtl* ctl, const HT* __restrict__ Einv, HT* u,
                                   double* part, int I) {
     if (ctl && block_done(ctl)) return;
     __shared__ double xs[6];
     const int nc = c.nc;
     const double* __restrict__ rc = c.rc;
-    const double* __restrict__ E0 = Einv + (size_t)6 * I * nc;
+    const HT* __restrict__ E0 = Einv + (size_t)6 * I * nc;
     double a[6] = {0, 0, 0, 0, 0, 0};
     for (int col = threadIdx.x; col < nc; col += kApplyThreads) {
         double e[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) e[k] = E0[(size_t)k * nc + col];
+        for (int k = 0; k < 6; ++k) e[k] = (double)E0[(size_t)k * nc + col];
         const double x = rc[col];
 #pragma unroll
         for (int k = 0; k < 6; ++k) a[k] += e[k] * x;
@@ -2770,7 +2772,7 @@ __device__ void coarse_apply_body(const CoarseArgs& c, const PcgCtl* ctl, const 
 
 template <typename HT>
 __global__ __launch_bounds__(kApplyThreads) void k_coarse_apply(CoarseArgs c, const PcgCtl* ctl,
-                                                                const double* __restrict__ Einv, HT* u,
+                                                                const HT* __restrict__ Einv, HT* u,
                                                                 double* part) {
     coarse_apply_body(c, ctl, Einv, u, part, blockIdx.x);
 }
@@ -2850,7 +2852,7 @@ struct BMember {
     double* part;
     int nbI, nbA, nbU, nbC;  // nbC: aggregates of the coarse correction, 0 without one
     CoarseArgs ca;
-    const double* Einv;
+    const HT* Einv;  // E^-1 in the rollout's precision (FP32 copy for mixed)
     double* cpart;           // gamma_c partials, two parity slots
 };
 
@@ -3520,6 +3522,7 @@ struct sors_ctx {
     int npad = 0;                   // nc rounded up to the Gauss-Jordan tile
     DevMem cdn, captr, canodes, crot, cpairI, cpairJ, cpchunk, cgptr, cci, ccj, ccbase;
     DevMem cE, cEinv, cX, cY, crc, cgpart, cpart, cinfo;   // cX / cY: Gauss-Jordan ping-pong (npad^2)
+    DevMem cEinv32;                 // FP32 copy of E^-1 (the mixed-precision apply)
     DevMem cpinv;                   // pivot block inverses of the current and the next step (2 x 32 x 32)
     const double* cres = nullptr;   // the buffer holding E^-1 after the last elimination
     bool vold_valid = false;
@@ -3980,6 +3983,7 @@ void ensure_coarse(sors_ctx* c) {
     const size_t np2 = (size_t)c->npad * c->npad;
     ensure(c->cE, sizeof(double) * nc2);
     ensure(c->cEinv, sizeof(double) * nc2);
+    ensure(c->cEinv32, sizeof(float) * nc2);
     ensure(c->cX, sizeof(double) * np2);
     ensure(c->cY, sizeof(double) * np2);
     ensure(c->cpinv, sizeof(double) * 2 * kGjPivot * kGjPivot);
@@ -4027,6 +4031,18 @@ void coarse_factor_async(sors_ctx* c) {
     c->refresh_pending = true;
 }
 
+// E^-1 in a solve's precision: the FP32 copy for FP32 solves.  The additive
+// two-level preconditioner D^-1 + Z E^-1 Z^T stays SPD with E^-1 rounded to
+// FP32 (the rounding is far below the block-Jacobi term), and the apply
+// still accumulates in FP64
+template <typename HT>
+const HT* coarse_inverse(const sors_ctx* c) {
+    if constexpr (sizeof(HT) == 4)
+        return c->cEinv32.as<float>();
+    else
+        return c->cEinv.as<double>();
+}
+
 // wait for the inversion in flight; on success the main stream installs
 // E^-1 = (X + X^T)/2 and the coarse correction is on
 void coarse_collect(sors_ctx* c) {
@@ -4039,7 +4055,7 @@ void coarse_collect(sors_ctx* c) {
     HIPX(hipStreamWaitEvent(c->stream, c->evR, 0));
     const size_t nc2 = (size_t)c->nc * c->nc;
     k_coarse_symmetrize<<<(unsigned)((nc2 + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(
-        c->nc, c->npad, c->cres, c->cEinv.as<double>());
+        c->nc, c->npad, c->cres, c->cEinv.as<double>(), c->cEinv32.as<float>());
     HIPX(hipGetLastError());
     c->coarse_ready = true;
 }
@@ -4190,7 +4206,7 @@ template <typename HT>
 void launch_coarse(sors_ctx* c, PcgCtl* ctl, int par) {
     const CoarseArgs ca = coarse_args(c);
     k_coarse_restrict<HT><<<c->nagg, kRestrictThreads, 0, c->stream>>>(ca, ctl, (const HT*)c->pr.p);
-    k_coarse_apply<HT><<<c->nagg, kApplyThreads, 0, c->stream>>>(ca, ctl, c->cEinv.as<double>(), (HT*)c->pu.p,
+    k_coarse_apply<HT><<<c->nagg, kApplyThreads, 0, c->stream>>>(ca, ctl, coarse_inverse<HT>(c), (HT*)c->pu.p,
                                                                  c->cpart.as<double>() + par * kMaxRedBlocks);
 }
 
@@ -4940,7 +4956,7 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
                            &c->incptr, &c->inc,  &c->teg,      &c->teb,    &c->tsptr,   &c->tsidx,   &c->tqslice, &c->vold,     &c->cdn,    &c->captr,   &c->canodes, &c->crot,
                            &c->cpairI, &c->cpairJ, &c->cpchunk, &c->cgptr, &c->cci,     &c->ccj,     &c->ccbase,
-                           &c->cE,     &c->cEinv, &c->crc,     &c->cX,      &c->cY,      &c->cpinv,   &c->cgpart,  &c->cpart,   &c->cinfo,
+                           &c->cE,     &c->cEinv, &c->cEinv32, &c->crc,     &c->cX,      &c->cY,      &c->cpinv,   &c->cgpart,  &c->cpart,   &c->cinfo,
                            &c->adjb,   &c->adjq,  &c->adjv,    &c->adjE,   &c->adjA,   &c->mgroup, &c->mdir,
                            &c->mk,     &c->mact};
     double b = 0;
@@ -4952,14 +4968,15 @@ double ctx_bytes(const sors_ctx* c) {
 // node-block slices padded to 24 (tet) / 32 (hex) blocks per node)
 double estimate_bytes(int N, int E, int ed, int coarse_nodes) {
     const double n = 3.0 * N, nblk = (double)N * (ed == 4 ? 24 : 32);
-    // coarse correction: E and its inverse (nc^2 FP64 each, nc = 6 per
-    // aggregate, aggregates at most 2N / (nodes per aggregate) and 512), the two
-    // Gauss-Jordan buffers (npad^2 FP64 each) and per-node data
+    // coarse correction: E and its inverse (nc^2 FP64 each) with the inverse's
+    // FP32 copy (nc = 6 per aggregate, aggregates at most 2N / (nodes per
+    // aggregate) and 512), the two Gauss-Jordan buffers (npad^2 FP64 each)
+    // and per-node data
     double coarse = 0.0;
     if (coarse_nodes > 0) {
         const double nc = 6.0 * std::min(512.0, std::ceil(2.0 * N / coarse_target(N, coarse_nodes)));
         const double np = std::ceil(nc / kGjTile) * kGjTile;
-        coarse = 2.0 * 8.0 * nc * nc + 2.0 * 8.0 * np * np + 24.0 * N;
+        coarse = (2.0 * 8.0 + 4.0) * nc * nc + 2.0 * 8.0 * np * np + 24.0 * N;
     }
     // two-phase assembly: element rows (4 ed FP64 + ed^2 padded FP64 blocks at
     // worst) and the slot lists (ed^2 entries per element, one pointer per slot)
@@ -5183,7 +5200,7 @@ void srv_batched_pcg(sors_server* s, const std::vector<int>& starters, const std
             b.nbC = m.c->coarse_ready ? m.c->nagg : 0;
             if (b.nbC) {
                 b.ca = coarse_args(m.c);
-                b.Einv = m.c->cEinv.as<double>();
+                b.Einv = coarse_inverse<HT>(m.c);
                 b.cpart = m.c->cpart.as<double>();
             }
         }
