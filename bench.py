@@ -454,7 +454,7 @@ def main():
     ap.add_argument("--serve-budget", type=float, default=200e9, help="HBM budget of resident rollouts (bytes)")
     ap.add_argument("--serve-max-active", type=int, default=64)
     ap.add_argument("--serve-prep-threads", type=int, default=8)
-    ap.add_argument("--serve-chunk", type=int, default=16)
+    ap.add_argument("--serve-chunk", type=int, default=8)
     ap.add_argument("--no-serve-seq", action="store_true", help="skip the sequential FP64-GPU serving baseline")
     ap.add_argument("--serve-seq-every", type=int, default=32,
                     help="the sequential FP64 comparator measures every n-th request")
