@@ -77,7 +77,7 @@ def solver_opts(args):
     """Solver options shared by the beam, hex and contact legs."""
     return dict(pcg_rtol=args.pcg_rtol, pcg_rtol_first=args.pcg_rtol_first, reuse_tangent=args.reuse_tangent,
                 g_tol=args.g_tol, predict=args.predictor, coarse_nodes=args.coarse_nodes,
-                coarse_every=args.coarse_every)
+                coarse_every=args.coarse_every, pcg_check_every=args.check_every)
 
 
 def cpu_baseline(sc, opts, cores):
@@ -443,7 +443,8 @@ def main():
     ap.add_argument("--no-predict", action="store_true", help="start Newton from the previous positions")
     ap.add_argument("--predictor", type=int, default=2, choices=[0, 1, 2],
                     help="Newton start: 0 previous positions, 1 constant velocity, 2 constant acceleration")
-    ap.add_argument("--check-every", type=int, default=32, help="PCG iterations between host convergence checks")
+    ap.add_argument("--check-every", type=int, default=16,
+                    help="PCG iterations before a solve's first host convergence check")
     ap.add_argument("--coarse-nodes", type=int, default=2048,
                     help="nodes per aggregate of the two-level PCG preconditioner (0: block-Jacobi only)")
     ap.add_argument("--coarse-every", type=int, default=16, help="coarse operator refreshed every k-th tangent")
