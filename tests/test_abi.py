@@ -27,6 +27,17 @@ def test_default_options_match_reference_constants():
     assert o.project_spd == 1 and o.precision == native.MIXED
 
 
+def test_default_options_fill_every_field():
+    """sors_default_options writes the struct's last fields where the ctypes
+    mirror has them (sentinels overwritten): the two layouts agree, and the
+    forcing-term extensions are off by default."""
+    o = native.Options()
+    o.coarse_nodes, o.coarse_every, o.pcg_rtol_first, o.reuse_tangent = -7, -7, 7.0, 7
+    native.lib().sors_default_options(ctypes.byref(o))
+    assert o.coarse_nodes == 2048 and o.coarse_every == 16
+    assert o.pcg_rtol_first == 0.0 and o.reuse_tangent == 0
+
+
 def test_server_declarations_and_option_checks():
     syms = native.declared_symbols()
     for s in ("sors_server_create", "sors_server_submit", "sors_server_wait", "sors_server_get_stats",

@@ -408,3 +408,15 @@ def test_block_layout_export_matches_csr_assembly(orc):
     _, _, hd = r.hessian(q, sc.dt, native.FP64, project=True)
     _, _, ho = o.hessian(q, sc.dt, project=True)
     assert rel_err(hd, ho) < 1e-9
+
+
+def test_forcing_options_are_validated():
+    sc = mesh.beam(nx=2, ny=1, nz=1)
+    r = rollout(sc)
+    for bad in (dict(pcg_rtol_first=1.5), dict(pcg_rtol_first=-0.1), dict(reuse_tangent=2)):
+        with pytest.raises(native.SorsError):
+            r.set_options(**bad)
+    r.set_options(pcg_rtol_first=3e-2, reuse_tangent=1)
+    o = r.options()
+    assert o.pcg_rtol_first == 3e-2 and o.reuse_tangent == 1
+    r.close()
