@@ -1991,6 +1991,7 @@ struct PcgView {
     int N, n, nslices;
     const int* sbase;
     const int* bcol;
+    const short* bcol16 = nullptr;  // non-null: the SpMV reads 16-bit column offsets
     const HT* vals;
     const HT* Minv;
     HT *x, *r, *u, *p, *s, *w;
@@ -2194,18 +2195,23 @@ __device__ void pcg_spmv_body(const PcgView<HT>& v, PcgState* st, PartIO io, int
         if (local >= per_xcd || g >= ngroups || sl >= v.nslices) continue;  // uniform per wavefront
         const int base = v.sbase[sl];
         const int width = (v.sbase[sl + 1] - base) / kSlice;
-        const int* cp = v.bcol + base + lane;
         const HT* vp = v.vals + 9LL * base + lane;
+        const int node = sl * kSlice + lane;
         HT a0 = 0, a1 = 0, a2 = 0;
-        for (int j = 0; j < width; ++j) {
-            const int nbr = cp[kSlice * j];
+        auto slot = [&](int j, int nbr) {
             const HT x0 = v.u[3 * nbr], x1 = v.u[3 * nbr + 1], x2 = v.u[3 * nbr + 2];
             const HT* bb = vp + (long long)kSlice * 9 * j;
             a0 += bb[0] * x0 + bb[kSlice] * x1 + bb[2 * kSlice] * x2;
             a1 += bb[3 * kSlice] * x0 + bb[4 * kSlice] * x1 + bb[5 * kSlice] * x2;
             a2 += bb[6 * kSlice] * x0 + bb[7 * kSlice] * x1 + bb[8 * kSlice] * x2;
+        };
+        if (v.bcol16) {  // 2-byte column offsets: half the index traffic, the same neighbours
+            const short* cp = v.bcol16 + base + lane;
+            for (int j = 0; j < width; ++j) slot(j, node + cp[kSlice * j]);
+        } else {
+            const int* cp = v.bcol + base + lane;
+            for (int j = 0; j < width; ++j) slot(j, cp[kSlice * j]);
         }
-        const int node = sl * kSlice + lane;
         if (node < v.N) {
             v.w[3 * node] = a0;
             v.w[3 * node + 1] = a1;
@@ -3504,6 +3510,8 @@ struct sors_ctx {
     bool pat_on_device = false;
     DevMem pcgst_bench, copybuf;
     DevMem sbase, bcol, selfslot;   // sliced node-block layout (node-granular masks)
+    DevMem bcol16;                  // bcol as 16-bit offsets from the row's node (when they all fit)
+    bool col16 = false;
     DevMem incptr, inc;             // node -> (element, local node) incidence (gradient sums)
     DevMem teg, teb;                // element rows between the two assembly phases
     DevMem tsptr, tsidx, tqslice;   // layout slot -> element blocks (pattern.h)
@@ -3637,6 +3645,32 @@ int grid_for(long long work, int block = kBlock) {
 
 void sync(sors_ctx* c) { HIPX(hipStreamSynchronize(c->stream)); }
 
+// The SpMV's column table as 16-bit offsets from each row's node, when every
+// offset fits (node orderings with a bandwidth under 32,768 nodes, such as
+// the structured beams' 1,723); padding lanes past the last node point at
+// the last node (their blocks are zero and their rows are not stored)
+void build_col16(sors_ctx* c) {
+    c->col16 = false;
+    const int ns = c->pat.nslices;
+    const long long slots = c->pat.sbase[ns];
+    if (c->N == 0 || slots == 0) return;
+    std::vector<short> off((size_t)slots);
+    for (int sl = 0; sl < ns; ++sl) {
+        const int base = c->pat.sbase[sl], w = (c->pat.sbase[sl + 1] - base) / kSlice;
+        for (int lane = 0; lane < kSlice; ++lane) {
+            const int v = sl * kSlice + lane;
+            for (int k = 0; k < w; ++k) {
+                const long long p = base + (long long)kSlice * k + lane;
+                const int d = v < c->N ? c->pat.bcol[(size_t)p] - v : (c->N - 1) - v;
+                if (d < -32768 || d > 32767) return;
+                off[(size_t)p] = (short)d;
+            }
+        }
+    }
+    upload_vec(c->bcol16, off, c->stream);
+    c->col16 = true;
+}
+
 // (re)build the pattern for dt == 0 / dt != 0 and upload it
 void ensure_pattern(sors_ctx* c, double dt) {
     const int dtnz = dt != 0.0;
@@ -3651,6 +3685,7 @@ void ensure_pattern(sors_ctx* c, double dt) {
         upload(c->rank, c->pat.rank.data(), c->pat.rank.size(), c->stream);
         upload(c->sbase, c->pat.sbase.data(), c->pat.sbase.size(), c->stream);
         upload(c->bcol, c->pat.bcol.data(), c->pat.bcol.size(), c->stream);
+        build_col16(c);
         upload(c->selfslot, c->pat.self_slot.data(), c->pat.self_slot.size(), c->stream);
         upload(c->diag, c->pat.bdiag.data(), c->pat.bdiag.size(), c->stream);
         c->nvals = 9LL * c->pat.sbase[c->pat.nslices];
@@ -4150,6 +4185,7 @@ PcgView<HT> pcg_view(sors_ctx* c, double rtol) {
     v.nslices = c->pat.nslices;
     v.sbase = c->sbase.as<int>();
     v.bcol = c->bcol.as<int>();
+    if (c->col16) v.bcol16 = c->bcol16.as<short>();
     v.vals = (const HT*)c->vals.p;
     v.Minv = (const HT*)c->Minv.p;
     v.x = (HT*)c->px.p;
@@ -4958,7 +4994,7 @@ double ctx_bytes(const sors_ctx* c) {
                            &c->fext,   &c->dxd,   &c->qtmp,    &c->rowptr, &c->cols,   &c->diag,    &c->rank,
                            &c->fullmap, &c->vals, &c->Minv,    &c->pr,    &c->pu,      &c->pp,      &c->ps,
                            &c->pw,     &c->px,    &c->partials, &c->scal, &c->pcgst,   &c->pcgctl,  &c->pcgpart,
-                           &c->flag,   &c->sbase, &c->bcol,    &c->selfslot, &c->csurf, &c->lagN,
+                           &c->flag,   &c->sbase, &c->bcol,    &c->bcol16,   &c->selfslot, &c->csurf, &c->lagN,
                            &c->lagn,   &c->lagD,  &c->gcon,    &c->agroup, &c->afib,    &c->act,     &c->nu,
                            &c->incptr, &c->inc,  &c->teg,      &c->teb,    &c->tsptr,   &c->tsidx,   &c->tqslice, &c->vold,     &c->cdn,    &c->captr,   &c->canodes, &c->crot,
                            &c->cpairI, &c->cpairJ, &c->cpchunk, &c->cgptr, &c->cci,     &c->ccj,     &c->ccbase,
@@ -4987,7 +5023,7 @@ double estimate_bytes(int N, int E, int ed, int coarse_nodes) {
     // two-phase assembly: element rows (4 ed FP64 + ed^2 padded FP64 blocks at
     // worst) and the slot lists (ed^2 entries per element, one pointer per slot)
     const double rows = (double)E * (4 * ed * 8 + 10 * ed * ed * 8 + ed * ed * 4) + 4.0 * nblk;
-    return n * (19 * 8 + 4) + 72.0 * N + nblk * (9 * 8 + 4 + 9 * 4) + (double)E * (ed * 4 + 16 + ed * ed) + rows +
+    return n * (19 * 8 + 4) + 72.0 * N + nblk * (9 * 8 + 4 + 2 + 9 * 4) + (double)E * (ed * 4 + 16 + ed * ed) + rows +
            coarse + 8.0 * (5 * kMaxRedBlocks + 4 * kMaxRedBlocks) + (1 << 20);
 }
 
@@ -6199,7 +6235,7 @@ int sors_bench_kernel(sors_ctx* c, int kernel, int reps, double* avg_ms, double*
     if (kernel == SORS_KERNEL_SPMV) {
         // compulsory bytes: every stored nonzero once, one column index per
         // row (CSR) or per nonzero 3x3 node block (sliced layout), x and y once
-        bytes = c->block() ? hb * nnz + 4.0 * c->pat.nblocks + 2.0 * hb * n
+        bytes = c->block() ? hb * nnz + (c->col16 ? 2.0 : 4.0) * c->pat.nblocks + 2.0 * hb * n
                            : nnz * (hb + 4.0) + 4.0 * (n + 1) + 2.0 * hb * n;
         if (mixed) {
             fill<float>(c->pu.as<float>(), n, 1.0f, c->stream);
