@@ -509,7 +509,7 @@ def main():
     spmv_ms = agg["spmv_ms"] / max(1, agg["spmv_launches"])
     n = 3 * sc.n_nodes
     # compulsory bytes of one SpMV launch for the layout in use (values once,
-    # one column index per nonzero 3x3 node block, x and y once)
+    # one column index (or 16-bit offset) per nonzero 3x3 node block, x and y once)
     iso_ms, spmv_bytes = r.bench_kernel(native.KERNEL_SPMV, reps=20)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
     r.close()
