@@ -420,3 +420,34 @@ def test_forcing_options_are_validated():
     o = r.options()
     assert o.pcg_rtol_first == 3e-2 and o.reuse_tangent == 1
     r.close()
+
+
+def test_spmv_column_offsets_and_their_fallback():
+    """The SpMV reads 16-bit column offsets when the node ordering allows
+    (a structured beam) and the 32-bit table when it does not (the same beam
+    with its nodes shuffled: offsets beyond 32,767); the isolated SpMV's
+    algorithmic bytes show which, and both orderings step to the same state."""
+    sc = mesh.beam(nx=100, ny=18, nz=18, e_range=(5e5, 5e6), seed=2, steps=1)
+    N = sc.n_nodes
+    assert N > 32768
+    perm = np.random.default_rng(0).permutation(N)      # new index of each node
+    inv = np.argsort(perm)
+    X = sc.X.reshape(-1, 3)
+    mask = np.asarray(sc.bcmask).reshape(-1, 3)
+    v0 = np.asarray(sc.v0).reshape(-1, 3)
+    runs = {}
+    for name, order in (("structured", np.arange(N)), ("shuffled", inv)):
+        where = np.argsort(order)                      # position of node k in the new numbering
+        r = native.Rollout(X[order], where[sc.conn], sc.youngs, sc.poisson, sc.density,
+                           mask[order].reshape(-1), np.zeros(3 * N), sc.grav, precision=native.FP64,
+                           pcg_rtol=1e-10, g_tol=1e-10)
+        r.set_initial_velocity(v0[order].reshape(-1))
+        q, _ = r.step(X[order].reshape(-1), sc.dt)
+        by = r.bench_kernel(native.KERNEL_SPMV, reps=2)[1]
+        runs[name] = (by, q.reshape(-1, 3)[where])
+        r.close()
+    (by_s, q_s), (by_p, q_p) = runs["structured"], runs["shuffled"]
+    nblocks = (by_p - by_s) / 2.0          # 4 B against 2 B per stored block
+    say("SpMV bytes", by_s, by_p, "blocks", nblocks, "max |dq|/bbox", np.abs(q_s - q_p).max() / sc.bbox_diag)
+    assert nblocks > N and nblocks == int(nblocks)
+    assert np.abs(q_s - q_p).max() <= 1e-8 * sc.bbox_diag
