@@ -3649,8 +3649,12 @@ void sync(sors_ctx* c) { HIPX(hipStreamSynchronize(c->stream)); }
 // offset fits (node orderings with a bandwidth under 32,768 nodes, such as
 // the structured beams' 1,723); padding lanes past the last node point at
 // the last node (their blocks are zero and their rows are not stored)
+#ifndef SORS_COL16
+#define SORS_COL16 1
+#endif
 void build_col16(sors_ctx* c) {
     c->col16 = false;
+    if (!SORS_COL16) return;
     const int ns = c->pat.nslices;
     const long long slots = c->pat.sbase[ns];
     if (c->N == 0 || slots == 0) return;
